@@ -1,0 +1,135 @@
+/*
+ * mahyco_types.h -- plain-C description of one MaHyCo hydro case on a structured
+ * Cartesian mesh: mesh, environments (EOS + pseudo-viscosity coefficients),
+ * module options, remap options, boundary conditions and the initial-condition
+ * model.  The layout is POD so it crosses a C ABI (ctypes, cgo, JNI) unchanged.
+ *
+ * Every field names the reference option it carries:
+ *   Mahyco.axl            src/Mahyco.axl:263-557        (module options)
+ *   RemapADI.axl          src/Remap/RemapADI.axl:165-209 (remap options)
+ *   PerfectGasEOS.axl     src/eos/perfectgas/PerfectGasEOS.axl:157-183
+ *   StiffenedGasEOS.axl   src/eos/stiffenedgas/StiffenedGasEOS.axl:158-184
+ *   <cartesian> block of the dataset (Arcane Cartesian mesh generator)
+ */
+#ifndef MAHYCO_TYPES_H
+#define MAHYCO_TYPES_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MH_MAX_ENV 4 /* environments per case (bitmask per cell fits in one byte) */
+#define MH_MAX_BC 8
+
+/* Status codes.  The reference calls exit(1) at these sites; the C ABI returns them. */
+enum MhStatus {
+  MH_OK = 0,
+  MH_STOPPED = 1,           /* final time reached (MahycoModule.cc:1841-1857) */
+  MH_ERR_NEG_VOLUME = -1,   /* MahycoModule.cc:1164-1191 */
+  MH_ERR_DT_MIN = -2,       /* MahycoModule.cc:1816-1833 */
+  MH_ERR_NAN_REMAP = -3,    /* RemapADIFinal.cc:274-284 */
+  MH_ERR_INVALID = -10,     /* bad case description or argument */
+  MH_ERR_UNSUPPORTED = -11, /* option outside the implemented subset */
+  MH_ERR_DEVICE = -20,      /* HIP runtime error */
+  MH_ERR_NOMEM = -21        /* allocation failed / admission refused */
+};
+
+/* Equation-of-state services (IEquationOfState implementations). */
+enum MhEos { MH_EOS_PERFECT_GAS = 0, MH_EOS_STIFFENED_GAS = 1 };
+
+/* Face groups built by PrepareFaceGroup (MahycoAnnexe.cc:149-235). */
+enum MhSurface { MH_XMIN = 0, MH_XMAX = 1, MH_YMIN = 2, MH_YMAX = 3, MH_ZMIN = 4, MH_ZMAX = 5 };
+
+/* TypesMahyco::eBoundaryCondition (TypesMahyco.h:16-31), velocity subset. */
+enum MhBcType { MH_BC_VX = 0, MH_BC_VY = 1, MH_BC_VZ = 2 };
+
+/* TypesMahyco limiteur (TypesMahyco.h:96-105), classical subset.  Any other id is
+ * first order (fluxLimiter returns 0, UtilesRemap.cc:108-120). */
+enum MhLimiter { MH_MINMOD = 0, MH_SUPERBEE = 1, MH_VANLEER = 2 };
+
+/* Case models (IInitialisations services). */
+enum MhCaseModel {
+  MH_CASE_SOD = 0,         /* casTest/SOD/SODService.cc; cas_test 3,4,5 (mono) 13,14,15 (bi) */
+  MH_CASE_SEDOV = 1,       /* casTest/SEDOV/SEDOVService.cc; cas_test 1 (mono) */
+  MH_CASE_RIEMANN = 2,     /* two-state shock tube with geometric-overlap volume fractions */
+  MH_CASE_POINTTRIPLE = 3  /* casTest/POINTTRIPLE/POINTTRIPLEService.cc; cas_test 12 (3 envs) */
+};
+
+/* Node coordinate convention of the Cartesian generator. */
+enum MhCoordRule { MH_COORD_MUL = 0 /* x_i = o + i*(l/n) */, MH_COORD_DIV = 1 /* x_i = o + (l*i)/n */ };
+
+typedef struct MhMesh {
+  int dim;           /* 2 or 3 */
+  int nx, ny, nz;    /* cells per direction (nz ignored in 2D) */
+  double origin[3];  /* <origine> */
+  double length[3];  /* <lx>, <ly>, <lz> */
+  int coord_rule;    /* MhCoordRule */
+} MhMesh;
+
+typedef struct MhEnvironment {
+  int eos;                       /* MhEos */
+  double adiabatic_cst;          /* gamma (adiabatic-cst, default 1.4) */
+  double limit_tension;          /* stiffened-gas pi (limit-tension) */
+  double specific_heat;          /* cv (specific-heat) */
+  double linear_pseudo_coeff;    /* a1 (Mahyco.axl:322-327, default 0.5) */
+  double quadratic_pseudo_coeff; /* a2 (Mahyco.axl:328-333, default 1.2) */
+  double densite_initiale;       /* densite-initiale */
+  double pression_initiale;      /* pression-initiale */
+  double vitesse_initiale[3];    /* vitesse-initiale */
+} MhEnvironment;
+
+typedef struct MhBoundaryCondition {
+  int surface; /* MhSurface */
+  int type;    /* MhBcType */
+  double value;
+} MhBoundaryCondition;
+
+typedef struct MhOptions {
+  /* module (Mahyco.axl) */
+  double cfl;               /* cfl, default 0.3 */
+  double threshold;         /* threshold, default 1e-10 (face groups) */
+  double deltat_init;
+  double deltat_min;
+  double deltat_max;
+  double final_time;
+  int with_projection;      /* 1: MahycoLoop (Lagrange + ADI remap); 0: MahycoLagrangeLoop */
+  int calcul_vitson_max;    /* default 1 */
+  int nb_bc;
+  MhBoundaryCondition bc[MH_MAX_BC];
+  /* remap (RemapADI.axl) */
+  int ordre_projection;       /* 1 or 2 */
+  int projection_limiteur_id; /* MhLimiter */
+  double remap_threshold;     /* threshold, default 1e-10 */
+  /* case model */
+  int cas_model;            /* MhCaseModel */
+  int cas_test;             /* reference cas-test id (TypesMahyco.h enum Test) */
+  double interface_position;/* MH_CASE_RIEMANN: interface coordinate */
+  int interface_axis;       /* MH_CASE_RIEMANN: 0,1,2 */
+  double sedov_deposit;     /* SEDOV total energy deposit (reference constant 0.244816) */
+} MhOptions;
+
+/* Field identifiers for get/set.  Cell fields take an environment index
+ * (-1 = cell value, k = value of environment k, 0 for non-member cells). */
+enum MhField {
+  MH_F_DENSITY = 0,
+  MH_F_PRESSURE = 1,
+  MH_F_INTERNAL_ENERGY = 2,
+  MH_F_SOUND_SPEED = 3,
+  MH_F_PSEUDO_VISCOSITY = 4,
+  MH_F_CELL_MASS = 5,
+  MH_F_CELL_VOLUME = 6,
+  MH_F_FRACVOL = 7,
+  MH_F_MASS_FRACTION = 8,
+  MH_F_CARACTERISTIC_LENGTH = 9,
+  MH_F_ENV_MASK = 10,       /* membership bitmask, as double */
+  MH_F_MATERIAU = 11,
+  /* node fields; component selects x/y/z */
+  MH_N_VELOCITY = 100,
+  MH_N_NODE_MASS = 101,
+  MH_N_COORD = 102
+};
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAHYCO_TYPES_H */

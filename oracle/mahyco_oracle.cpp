@@ -1,0 +1,1498 @@
+// mahyco_oracle.cpp -- CPU restatement of MaHyCo's Lagrange + ADI-remap cycle.
+//
+// TEST INFRASTRUCTURE ONLY: this file is the parity checker for the HIP path and
+// the CPU-baseline timing ("port").  The product library never links it.
+//
+// It follows the reference operation by operation, in the reference's evaluation
+// order (no FMA contraction: build with -ffp-contract=off), so that its results
+// can be compared with the reference's non-regression fixtures and, to the last
+// bit, with the HIP kernels.  Each function cites the reference file:line it
+// restates (paths relative to the MaHyCo tree).
+//
+// Arcane semantics restated here:
+//  * material variables (MaterialVariableCellReal) hold one global value per cell
+//    plus one partial value per (environment, cell) for cells with several
+//    environments; for a cell with exactly one environment the environment value
+//    IS the global value (same storage).  MatVar/ev() below model that aliasing.
+//  * ENUMERATE_ENV / ENUMERATE_ENVCELL visit the member cells of one environment in
+//    cell order; ENUMERATE_CELL visits cells in local-id order, which for the
+//    Cartesian generator is x fastest, then y, then z.
+//  * Scatter loops of the reference (node force, node mass, dual mass fluxes) are
+//    written here as per-node gathers that visit the contributing cells/faces in
+//    the same order as the scatter does, so every node sees the identical sequence
+//    of additions while the loop can run on several threads.
+//  * Face node order follows Arcane's reordering (smallest unique id first, then
+//    its smaller neighbour), which makes every Cartesian face normal +e_d.
+//
+// Computations whose results no parity field ever reads are omitted and listed
+// where they would sit: temperature (cv), phase fractions, stress deviators and
+// plastic variables (all remap slots other than V_k, M_k, E_k and V*q: each slot
+// is swept independently, so dropping a slot changes no other slot), the
+// elasto-plastic force term of a NoModel run (adds +-0), gravity (0), and
+// Endommagement (inert at its default thresholds -1e12 / 0).
+
+#include "mahyco_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+struct R3 {
+  double x, y, z;
+};
+inline R3 operator+(R3 a, R3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline R3 operator-(R3 a, R3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline R3 operator*(double s, R3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline R3& operator+=(R3& a, R3 b) { a.x += b.x; a.y += b.y; a.z += b.z; return a; }
+inline double dot(R3 a, R3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }  // Arcane math::dot
+inline R3 vecMul(R3 u, R3 v) {                                                 // Arcane math::vecMul
+  return {u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x};
+}
+inline double normL2(R3 a) { return std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+inline double squareNormL2(R3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+inline double comp(R3 a, int d) { return d == 0 ? a.x : (d == 1 ? a.y : a.z); }
+inline double& compr(R3& a, int d) { return d == 0 ? a.x : (d == 1 ? a.y : a.z); }
+inline double amax(double a, double b) { return (a < b) ? b : a; }  // Arcane math::max
+inline double amin(double a, double b) { return (a < b) ? a : b; }  // Arcane math::min
+
+// Arcane MaterialVariableCellReal: global [nc] + partial [nc][MH_MAX_ENV].
+struct MatVar {
+  std::vector<double> g, p;
+  void init(long nc) { g.assign(nc, 0.0); p.assign(nc * MH_MAX_ENV, 0.0); }
+};
+
+}  // namespace
+
+struct OracleCase {
+  // ---------------------------------------------------------------- mesh
+  int dim = 3, nx = 1, ny = 1, nz = 1;
+  long nc = 0, nn = 0, nf = 0;
+  int nnpc = 8;  // nodes per cell
+  int nfpc = 6;  // faces per cell
+  std::vector<int> cell_nodes;   // [nc][nnpc]
+  std::vector<int> cell_faces;   // [nc][nfpc], Arcane local face order
+  std::vector<int> face_nodes;   // [nf][4]
+  std::vector<int> face_nbnode;  // [nf]
+  std::vector<int> face_dir;     // structural normal direction
+  std::vector<int> face_prev, face_next;  // DirFace previousCell / nextCell (-1 = none)
+  long face_off[3] = {0, 0, 0}, face_cnt[3] = {0, 0, 0};
+  std::vector<int> node_prev[3], node_next[3];  // DirNode previous / next (-1 = none)
+  std::vector<int> nc_off, nc_cell, nc_local;  // node -> cells in cell order, local index
+  std::vector<int> nf_off[3], nf_face;         // node -> faces of dir d in face order
+  std::vector<int> nf_list[3];
+  int cells_in_dir[3] = {1, 1, 1};
+
+  // ---------------------------------------------------------------- options
+  int m = 1;
+  MhEnvironment env[MH_MAX_ENV];
+  MhOptions opt;
+
+  // ---------------------------------------------------------------- state
+  std::vector<R3> node_coord, node_coord_0, velocity, velocity_n, force;
+  std::vector<double> node_mass;
+  std::vector<R3> cell_coord, face_coord, face_normal;
+  std::vector<R3> cqs, cqs_n;            // [nc][nnpc]
+  std::vector<R3> outer_face_normal;     // [nc][nfpc]
+  std::vector<double> caracteristic_length, div_u, euler_volume, materiau;
+  std::vector<uint8_t> mask;             // environment membership bits
+  std::vector<int8_t> nbenv;
+  MatVar fracvol, mass_fraction, density, density_n, internal_energy, internal_energy_n,
+      pressure, pressure_n, sound_speed, pseudo, pseudo_n, pseudo_nm1, cell_volume,
+      cell_volume_n, cell_mass, tau_density;
+  std::vector<MatVar*> matvars;
+
+  // remap service state (RemapADIService.h); slots: k V_k, m+k M_k, 2m+k E_k, 3m V*q
+  int nv = 4;
+  std::vector<double> u_lagrange, phi_lagrange, grad_phi, grad_phi_face, phi_face, dual_phi_flux;
+  std::vector<double> u_dual, phi_dual, dual_grad_phi;  // [nn][5]
+  std::vector<double> deltax_lagrange, face_normal_velocity;
+  std::vector<R3> face_length_lagrange;
+  std::vector<double> back_flux_mass, front_flux_mass;
+  int sens_projection = 0;
+
+  // boundary face groups -> node lists (XMIN..ZMAX)
+  std::vector<int> group_nodes[6];
+  std::vector<double> riemann_f1;  // MH_CASE_RIEMANN: volume fraction of env 1
+
+  // time loop
+  double global_time = 0.0, global_deltat = 0.0, global_old_deltat = 0.0, old_deltat = 0.0;
+  int global_iteration = 0;
+  bool stopped = false;
+  std::vector<double> history;  // rows of 1 + 5m values
+
+  // ---------------------------------------------------------------- helpers
+  int node_id(int i, int j, int k) const { return i + (nx + 1) * (j + (ny + 1) * k); }
+  int cell_id(int i, int j, int k) const { return i + nx * (j + ny * k); }
+  double& ev(MatVar& v, int k, long c) { return nbenv[c] == 1 ? v.g[c] : v.p[c * MH_MAX_ENV + k]; }
+  bool member(int k, long c) const { return (mask[c] >> k) & 1; }
+  int nnodes_face() const { return dim == 3 ? 4 : 2; }
+
+  void build_mesh(const MhMesh& mesh);
+  void hydroStartInit();
+  void hydroStartInitEnvAndMat();
+  void initMatAndVar();
+  void computeCellMass();
+  void computeNodeMass();
+  void InitGeometricValues();
+  void PrepareFaceGroup();
+
+  void saveValuesAtN();
+  void updateVelocity();
+  void applyBoundaryCondition();
+  void updatePosition();
+  int computeGeometricValues();
+  void computeCQs(const R3* coord, const R3* face_coord, R3* out);
+  void updateDensity();
+  void computeArtificialViscosity();
+  void updateEnergyAndPressure();
+  void applyEOS(int k);
+  void computePressionMoyenne();
+  int remap();
+  void computeVariablesForRemap();
+  void computeFaceQuantitesForRemap();
+  void computeGradPhiFace(int idir);
+  void computeGradPhiCell(int idir);
+  void computeUpwindFaceQuantitiesForProjection(int idir);
+  void computeUremap(int idir);
+  void computeDualUremap(int idir);
+  int remapVariables();
+  void setMembership(const std::vector<uint8_t>& newmask);
+  int computeDeltaT();
+  void timeHistoryWrite();
+  int cycle();
+};
+
+// ============================================================================ mesh
+// Arcane Cartesian generator restated: node unique id i + (nx+1)(j + (ny+1)k),
+// hexahedron nodes (i,j,k),(i+1,j,k),(i+1,j+1,k),(i,j+1,k), then the same at k+1
+// (NONREGRESSION/CAS_BiSodCaseX/output/depouillement/ensight.geo000000 connectivity).
+void OracleCase::build_mesh(const MhMesh& mesh) {
+  dim = mesh.dim;
+  nx = mesh.nx;
+  ny = mesh.ny;
+  nz = dim == 3 ? mesh.nz : 1;
+  nnpc = dim == 3 ? 8 : 4;
+  nfpc = dim == 3 ? 6 : 4;
+  nc = (long)nx * ny * nz;
+  const int nnz = dim == 3 ? nz + 1 : 1;
+  nn = (long)(nx + 1) * (ny + 1) * nnz;
+  cells_in_dir[0] = nx;
+  cells_in_dir[1] = ny;
+  cells_in_dir[2] = dim == 3 ? nz : 1;
+
+  auto coord1 = [&](int d, int i, int n) {
+    double o = mesh.origin[d], l = mesh.length[d];
+    if (mesh.coord_rule == MH_COORD_DIV) return o + (l * (double)i) / (double)n;
+    return o + (double)i * (l / (double)n);
+  };
+  node_coord.resize(nn);
+  for (int k = 0; k < nnz; ++k)
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i <= nx; ++i) {
+        R3 x{coord1(0, i, nx), coord1(1, j, ny), dim == 3 ? coord1(2, k, nz) : 0.0};
+        node_coord[node_id(i, j, k)] = x;
+      }
+
+  cell_nodes.resize(nc * nnpc);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        int* cn = &cell_nodes[(long)cell_id(i, j, k) * nnpc];
+        cn[0] = node_id(i, j, k);
+        cn[1] = node_id(i + 1, j, k);
+        cn[2] = node_id(i + 1, j + 1, k);
+        cn[3] = node_id(i, j + 1, k);
+        if (dim == 3) {
+          cn[4] = node_id(i, j, k + 1);
+          cn[5] = node_id(i + 1, j, k + 1);
+          cn[6] = node_id(i + 1, j + 1, k + 1);
+          cn[7] = node_id(i, j + 1, k + 1);
+        }
+      }
+
+  // Faces, one family per normal direction, each ordered k, j, i (i fastest).
+  int fdims[3][3];
+  for (int d = 0; d < dim; ++d) {
+    for (int e = 0; e < 3; ++e) fdims[d][e] = (e == 0 ? nx : e == 1 ? ny : nz);
+    fdims[d][d] += 1;
+    if (dim == 2) fdims[d][2] = 1;
+  }
+  nf = 0;
+  for (int d = 0; d < dim; ++d) {
+    face_off[d] = nf;
+    face_cnt[d] = (long)fdims[d][0] * fdims[d][1] * fdims[d][2];
+    nf += face_cnt[d];
+  }
+  face_nodes.assign(nf * 4, -1);
+  face_nbnode.assign(nf, dim == 3 ? 4 : 2);
+  face_dir.assign(nf, 0);
+  face_prev.assign(nf, -1);
+  face_next.assign(nf, -1);
+  auto fid = [&](int d, int i, int j, int k) -> long {
+    return face_off[d] + i + (long)fdims[d][0] * (j + (long)fdims[d][1] * k);
+  };
+  for (int d = 0; d < dim; ++d)
+    for (int k = 0; k < fdims[d][2]; ++k)
+      for (int j = 0; j < fdims[d][1]; ++j)
+        for (int i = 0; i < fdims[d][0]; ++i) {
+          long f = fid(d, i, j, k);
+          face_dir[f] = d;
+          int* fn = &face_nodes[f * 4];
+          if (dim == 3) {
+            if (d == 0) {
+              fn[0] = node_id(i, j, k); fn[1] = node_id(i, j + 1, k);
+              fn[2] = node_id(i, j + 1, k + 1); fn[3] = node_id(i, j, k + 1);
+            } else if (d == 1) {
+              fn[0] = node_id(i, j, k); fn[1] = node_id(i + 1, j, k);
+              fn[2] = node_id(i + 1, j, k + 1); fn[3] = node_id(i, j, k + 1);
+            } else {
+              fn[0] = node_id(i, j, k); fn[1] = node_id(i + 1, j, k);
+              fn[2] = node_id(i + 1, j + 1, k); fn[3] = node_id(i, j + 1, k);
+            }
+          } else {
+            if (d == 0) { fn[0] = node_id(i, j, 0); fn[1] = node_id(i, j + 1, 0); }
+            else { fn[0] = node_id(i, j, 0); fn[1] = node_id(i + 1, j, 0); }
+          }
+          int idx[3] = {i, j, k};
+          int lim = d == 0 ? nx : d == 1 ? ny : nz;
+          int c = idx[d];
+          if (c - 1 >= 0) { int t[3] = {i, j, k}; t[d] = c - 1; face_prev[f] = cell_id(t[0], t[1], t[2]); }
+          if (c < lim) face_next[f] = cell_id(i, j, k);
+        }
+  cell_faces.resize(nc * nfpc);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        int* cf = &cell_faces[(long)cell_id(i, j, k) * nfpc];
+        if (dim == 3) {  // Hexaedron8 local faces: (0,3,2,1) (0,4,7,3) (0,1,5,4) (4,5,6,7) (1,2,6,5) (2,3,7,6)
+          cf[0] = (int)fid(2, i, j, k);
+          cf[1] = (int)fid(0, i, j, k);
+          cf[2] = (int)fid(1, i, j, k);
+          cf[3] = (int)fid(2, i, j, k + 1);
+          cf[4] = (int)fid(0, i + 1, j, k);
+          cf[5] = (int)fid(1, i, j + 1, k);
+        } else {  // Quad4 local faces: (0,1) (1,2) (2,3) (3,0)
+          cf[0] = (int)fid(1, i, j, 0);
+          cf[1] = (int)fid(0, i + 1, j, 0);
+          cf[2] = (int)fid(1, i, j + 1, 0);
+          cf[3] = (int)fid(0, i, j, 0);
+        }
+      }
+
+  // Node direction neighbours (NodeDirectionMng).
+  for (int d = 0; d < dim; ++d) {
+    node_prev[d].assign(nn, -1);
+    node_next[d].assign(nn, -1);
+  }
+  for (int k = 0; k < nnz; ++k)
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i <= nx; ++i) {
+        int n = node_id(i, j, k);
+        if (i > 0) node_prev[0][n] = node_id(i - 1, j, k);
+        if (i < nx) node_next[0][n] = node_id(i + 1, j, k);
+        if (j > 0) node_prev[1][n] = node_id(i, j - 1, k);
+        if (j < ny) node_next[1][n] = node_id(i, j + 1, k);
+        if (dim == 3) {
+          if (k > 0) node_prev[2][n] = node_id(i, j, k - 1);
+          if (k < nz) node_next[2][n] = node_id(i, j, k + 1);
+        }
+      }
+
+  // node -> cells in cell order (the order in which a cell-order scatter reaches the node)
+  std::vector<int> cnt(nn, 0);
+  for (long c = 0; c < nc; ++c)
+    for (int l = 0; l < nnpc; ++l) cnt[cell_nodes[c * nnpc + l]]++;
+  nc_off.assign(nn + 1, 0);
+  for (long n = 0; n < nn; ++n) nc_off[n + 1] = nc_off[n] + cnt[n];
+  nc_cell.resize(nc_off[nn]);
+  nc_local.resize(nc_off[nn]);
+  std::vector<int> pos(nc_off.begin(), nc_off.end() - 1);
+  for (long c = 0; c < nc; ++c)
+    for (int l = 0; l < nnpc; ++l) {
+      int n = cell_nodes[c * nnpc + l];
+      nc_cell[pos[n]] = (int)c;
+      nc_local[pos[n]++] = l;
+    }
+  // node -> faces of direction d in face iteration order
+  for (int d = 0; d < dim; ++d) {
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (long f = face_off[d]; f < face_off[d] + face_cnt[d]; ++f)
+      for (int l = 0; l < face_nbnode[f]; ++l) cnt[face_nodes[f * 4 + l]]++;
+    nf_off[d].assign(nn + 1, 0);
+    for (long n = 0; n < nn; ++n) nf_off[d][n + 1] = nf_off[d][n] + cnt[n];
+    nf_list[d].resize(nf_off[d][nn]);
+    std::vector<int> p2(nf_off[d].begin(), nf_off[d].end() - 1);
+    for (long f = face_off[d]; f < face_off[d] + face_cnt[d]; ++f)
+      for (int l = 0; l < face_nbnode[f]; ++l) nf_list[d][p2[face_nodes[f * 4 + l]]++] = (int)f;
+  }
+}
+
+// ============================================================================ init
+// MahycoModule::hydroStartInit, MahycoModule.cc:21-210.
+void OracleCase::hydroStartInit() {
+  global_deltat = opt.deltat_init;  // :64-65
+  hydroStartInitEnvAndMat();        // :69
+  computeGeometricValues();         // :71
+  PrepareFaceGroup();               // :74
+  // :88-108 fills: fraction 1, mass fraction 1, pseudo 0, e_n 0 (global and partial)
+  std::fill(fracvol.g.begin(), fracvol.g.end(), 1.0);
+  std::fill(fracvol.p.begin(), fracvol.p.end(), 1.0);
+  std::fill(mass_fraction.g.begin(), mass_fraction.g.end(), 1.0);
+  std::fill(mass_fraction.p.begin(), mass_fraction.p.end(), 1.0);
+  std::fill(pseudo.g.begin(), pseudo.g.end(), 0.0);
+  std::fill(pseudo.p.begin(), pseudo.p.end(), 0.0);
+  initMatAndVar();  // casModel()->initVar :110-140 (membership was set by initMat)
+  // :143-152 initEOS per environment
+  for (int k = 0; k < m; ++k) {
+    const MhEnvironment& e = env[k];
+    const double gamma = e.adiabatic_cst;
+    for (long c = 0; c < nc; ++c) {
+      if (!member(k, c)) continue;
+      double p = ev(pressure, k, c), rho = ev(density, k, c);
+      if (e.eos == MH_EOS_STIFFENED_GAS) {  // StiffenedGasEOSService.cc:14-33
+        const double pi = e.limit_tension;
+        ev(internal_energy, k, c) = (p + (gamma * pi)) / ((gamma - 1.) * rho);
+        ev(sound_speed, k, c) = std::sqrt((gamma / rho) * (p + pi));
+      } else {  // PerfectGasEOSService.cc:12-37
+        ev(internal_energy, k, c) = p / ((gamma - 1.) * rho);
+        ev(sound_speed, k, c) = std::sqrt(gamma * p / rho);
+      }
+    }
+  }
+  // :154-170 mixed-cell averages (density and pressure are not zeroed first)
+  for (long c = 0; c < nc; ++c) {
+    if (nbenv[c] == 1) continue;
+    internal_energy.g[c] = 0.;
+    for (int k = 0; k < m; ++k) {
+      if (!member(k, c)) continue;
+      internal_energy.g[c] += ev(internal_energy, k, c) * ev(mass_fraction, k, c);
+      sound_speed.g[c] = std::max(ev(sound_speed, k, c), sound_speed.g[c]);
+      density.g[c] += ev(density, k, c) * ev(fracvol, k, c);
+      pressure.g[c] += ev(pressure, k, c) * ev(fracvol, k, c);
+    }
+  }
+  euler_volume = cell_volume.g;  // :174-178
+  InitGeometricValues();         // :195
+  // :199-209 volume recomputed from the CQS
+  for (long c = 0; c < nc; ++c) {
+    double volume = 0.0;
+    for (int l = 0; l < nnpc; ++l) volume += dot(node_coord[cell_nodes[c * nnpc + l]], cqs[c * nnpc + l]);
+    volume /= dim;
+    cell_volume.g[c] = volume;
+  }
+}
+
+// MahycoAnnexe.cc:11-142 (environment registration, cell centres, initMat, membership).
+void OracleCase::hydroStartInitEnvAndMat() {
+  const double one_over_nbnode = dim == 2 ? .25 : .125;
+  for (long c = 0; c < nc; ++c) {  // :75-82
+    R3 somme{0., 0., 0.};
+    for (int l = 0; l < nnpc; ++l) somme += node_coord[cell_nodes[c * nnpc + l]];
+    cell_coord[c] = one_over_nbnode * somme;
+  }
+  // casModel()->initMat: sets Materiau (SODService.cc:41-69, SEDOVService.cc:11-38,
+  // POINTTRIPLEService.cc:44-62); RIEMANN: geometric overlap
+  riemann_f1.assign(nc, 0.0);
+  std::vector<double>& f1 = riemann_f1;
+  for (long c = 0; c < nc; ++c) {
+    double mat = 0.;
+    if (opt.cas_model == MH_CASE_SOD) {
+      int t = opt.cas_test;
+      if (t == 13 || t == 14 || t == 15) {
+        double r = t == 13 ? cell_coord[c].x : t == 14 ? cell_coord[c].y : cell_coord[c].z;
+        mat = (r < 0.5) ? 0. : 1.;
+      }
+    } else if (opt.cas_model == MH_CASE_POINTTRIPLE) {
+      if (opt.cas_test != 2) {
+        if (cell_coord[c].x <= 0.01) mat = 0.;
+        else mat = (cell_coord[c].y <= 0.015) ? 1. : 2.;
+      }
+    } else if (opt.cas_model == MH_CASE_RIEMANN) {
+      int a = opt.interface_axis;
+      int n0 = cell_nodes[c * nnpc + 0];
+      int n1 = a == 0 ? cell_nodes[c * nnpc + 1] : a == 1 ? cell_nodes[c * nnpc + 3] : cell_nodes[c * nnpc + 4];
+      double lo = comp(node_coord[n0], a), hi = comp(node_coord[n1], a);
+      double fr = (hi - opt.interface_position) / (hi - lo);
+      fr = fr < 0. ? 0. : (fr > 1. ? 1. : fr);
+      f1[c] = fr;
+      mat = fr;
+    }
+    materiau[c] = mat;
+  }
+  // :97-139 membership from Materiau
+  for (long c = 0; c < nc; ++c) {
+    double v = materiau[c];
+    uint8_t bits;
+    if (v == 0.) bits = 1;
+    else if (v > 0. && v < 1.) bits = 3;
+    else if (v == 1.) bits = 2;
+    else if (v > 1. && v < 2.) bits = 6;
+    else if (v == 2.) bits = 4;
+    else if (v > 2. && v < 3.) bits = 12;
+    else if (v == 3.) bits = 8;
+    else bits = 15;
+    bits &= (uint8_t)((1u << m) - 1u);
+    mask[c] = bits;
+    nbenv[c] = (int8_t)__builtin_popcount(bits);
+  }
+}
+
+// Case initial conditions: initVar of the case services.
+void OracleCase::initMatAndVar() {
+  if (opt.cas_model == MH_CASE_SOD) {
+    int t = opt.cas_test;
+    if (t == 3 || t == 4 || t == 5) {  // SODService::initVarMono :11-40
+      for (long c = 0; c < nc; ++c) {
+        double r = t == 3 ? cell_coord[c].x : t == 4 ? cell_coord[c].y : cell_coord[c].z;
+        double pInit, rhoInit;
+        if (r < 0.5) {
+          pInit = env[0].pression_initiale;
+          rhoInit = env[0].densite_initiale;
+        } else {
+          pInit = env[0].pression_initiale / 10.;
+          rhoInit = env[0].densite_initiale / 8.;
+        }
+        density.g[c] = rhoInit;
+        pressure.g[c] = pInit;
+      }
+    } else {  // SODService::initVar :71-121
+      for (long c = 0; c < nc; ++c) {
+        int s = (materiau[c] == 0) ? 0 : 1;
+        density.g[c] = env[s].densite_initiale;
+        pressure.g[c] = env[s].pression_initiale;
+        if (nbenv[c] != 1)
+          for (int k = 0; k < m; ++k)
+            if (member(k, c)) {
+              ev(density, k, c) = env[s].densite_initiale;
+              ev(pressure, k, c) = env[s].pression_initiale;
+            }
+      }
+    }
+  } else if (opt.cas_model == MH_CASE_SEDOV) {  // SEDOVService::initVarMono :39-86
+    const double rhoInit = env[0].densite_initiale, pInit = env[0].pression_initiale;
+    const double e1 = 0.244816e-5;
+    const double total_energy_deposit = opt.sedov_deposit;
+    const double rmin = 1.e-10;
+    for (long c = 0; c < nc; ++c) {
+      bool isCenterCell = false;
+      internal_energy.g[c] = e1;
+      density.g[c] = rhoInit;
+      pressure.g[c] = pInit;
+      double vol = cell_volume.g[c];
+      for (int l = 0; l < nnpc; ++l) {
+        R3 x = node_coord[cell_nodes[c * nnpc + l]];
+        double rnode;
+        if (dim == 3) rnode = std::sqrt((x.x - 0.) * (x.x - 0.) + (x.y - 0.) * (x.y - 0.) + (x.z - 0.) * (x.z - 0.));
+        else rnode = std::sqrt((x.x - 0.) * (x.x - 0.) + (x.y - 0.) * (x.y - 0.));
+        if (rnode < rmin) isCenterCell = true;
+      }
+      if (isCenterCell) {
+        internal_energy.g[c] += total_energy_deposit / vol;
+        pressure.g[c] = 0.4 * rhoInit * internal_energy.g[c];
+      }
+    }
+  } else if (opt.cas_model == MH_CASE_POINTTRIPLE) {  // POINTTRIPLEService::initVar :64-128
+    for (long c = 0; c < nc; ++c) {
+      int s = cell_coord[c].x <= 0.01 ? 0 : (cell_coord[c].y <= 0.015 ? 1 : 2);
+      density.g[c] = env[s].densite_initiale;
+      pressure.g[c] = env[s].pression_initiale;
+      if (nbenv[c] != 1)
+        for (int k = 0; k < m; ++k)
+          if (member(k, c) && k == s) {
+            ev(density, k, c) = env[s].densite_initiale;
+            ev(pressure, k, c) = env[s].pression_initiale;
+          }
+    }
+  } else if (opt.cas_model == MH_CASE_RIEMANN) {
+    // Two-state tube with geometric-overlap fractions (no reference counterpart):
+    // env k takes (densite_initiale, pression_initiale) of env k; mass fractions are
+    // f_k rho_k / sum f_j rho_j; the cell density, pressure and sound speed of mixed
+    // cells start at 0 so that the averaging of hydroStartInit yields sums.
+    for (long c = 0; c < nc; ++c) {
+      if (nbenv[c] != 1) {
+        fracvol.p[c * MH_MAX_ENV + 0] = 1.0 - riemann_f1[c];
+        fracvol.p[c * MH_MAX_ENV + 1] = riemann_f1[c];
+      }
+      if (nbenv[c] == 1) {
+        int k = mask[c] == 1 ? 0 : 1;
+        density.g[c] = env[k].densite_initiale;
+        pressure.g[c] = env[k].pression_initiale;
+      } else {
+        double ft = 0.;
+        for (int k = 0; k < 2; ++k)
+          ft += fracvol.p[c * MH_MAX_ENV + k] * env[k].densite_initiale;
+        for (int k = 0; k < 2; ++k) {
+          ev(density, k, c) = env[k].densite_initiale;
+          ev(pressure, k, c) = env[k].pression_initiale;
+          ev(mass_fraction, k, c) = fracvol.p[c * MH_MAX_ENV + k] * env[k].densite_initiale / ft;
+        }
+        density.g[c] = 0.;
+        pressure.g[c] = 0.;
+        sound_speed.g[c] = 0.;
+      }
+    }
+  }
+  for (long n = 0; n < nn; ++n) {
+    const double* v0 = env[0].vitesse_initiale;
+    velocity[n] = (opt.cas_model == MH_CASE_SEDOV) ? R3{0., 0., 0.} : R3{v0[0], v0[1], v0[2]};
+  }
+}
+
+// MahycoModule::InitGeometricValues, MahycoModule.cc:1041-1093.
+void OracleCase::InitGeometricValues() {
+  node_coord_0 = node_coord;
+  const double one_over_nbnode = dim == 2 ? .5 : .25;
+  for (long f = 0; f < nf; ++f) {
+    const int* fn = &face_nodes[f * 4];
+    R3 nrm;
+    if (dim == 3) {
+      R3 v1 = node_coord[fn[2]] - node_coord[fn[0]];
+      R3 v2 = node_coord[fn[3]] - node_coord[fn[1]];
+      nrm.x = v1.y * v2.z - v1.z * v2.y;
+      nrm.y = -(v2.x * v1.z - v2.z * v1.x);
+      nrm.z = v1.x * v2.y - v1.y * v2.x;
+    } else {
+      nrm.x = node_coord[fn[1]].y - node_coord[fn[0]].y;
+      nrm.y = node_coord[fn[1]].x - node_coord[fn[0]].x;
+      nrm.z = 0.;
+    }
+    double nl = normL2(nrm);
+    face_normal[f] = {nrm.x / nl, nrm.y / nl, nrm.z / nl};
+    R3 fc{0., 0., 0.};
+    for (int l = 0; l < face_nbnode[f]; ++l) fc += one_over_nbnode * node_coord[fn[l]];
+    face_coord[f] = fc;
+  }
+  for (long c = 0; c < nc; ++c)
+    for (int l = 0; l < nfpc; ++l) {
+      R3 d = face_coord[cell_faces[c * nfpc + l]] - cell_coord[c];
+      double nl = normL2(d);
+      outer_face_normal[c * nfpc + l] = {d.x / nl, d.y / nl, d.z / nl};
+    }
+}
+
+// MahycoModule::PrepareFaceGroup, MahycoAnnexe.cc:149-235 -> node lists per group.
+void OracleCase::PrepareFaceGroup() {
+  R3 maxC{0., 0., 0.}, minC{100., 100., 100.};
+  for (long n = 0; n < nn; ++n) {
+    maxC.x = std::max(maxC.x, node_coord[n].x);
+    maxC.y = std::max(maxC.y, node_coord[n].y);
+    maxC.z = std::max(maxC.z, node_coord[n].z);
+    minC.x = std::min(minC.x, node_coord[n].x);
+    minC.y = std::min(minC.y, node_coord[n].y);
+    minC.z = std::min(minC.z, node_coord[n].z);
+  }
+  std::vector<char> in(nn);
+  for (int g = 0; g < 6; ++g) {
+    std::fill(in.begin(), in.end(), 0);
+    int d = g / 2;
+    double ref = (g % 2 == 0) ? comp(minC, d) : comp(maxC, d);
+    for (long f = 0; f < nf; ++f) {
+      bool flag = true;
+      for (int l = 0; l < face_nbnode[f]; ++l)
+        if (std::fabs(comp(node_coord[face_nodes[f * 4 + l]], d) - ref) > opt.threshold) flag = false;
+      if (flag)
+        for (int l = 0; l < face_nbnode[f]; ++l) in[face_nodes[f * 4 + l]] = 1;
+    }
+    group_nodes[g].clear();
+    for (long n = 0; n < nn; ++n)
+      if (in[n]) group_nodes[g].push_back((int)n);
+  }
+}
+
+// MahycoModule::computeCellMass :221-237 and computeNodeMass :247-262.
+void OracleCase::computeCellMass() {
+  for (long c = 0; c < nc; ++c) cell_mass.g[c] = density.g[c] * cell_volume.g[c];
+  for (int k = 0; k < m; ++k)
+    for (long c = 0; c < nc; ++c)
+      if (member(k, c)) ev(cell_mass, k, c) = ev(mass_fraction, k, c) * cell_mass.g[c];
+}
+void OracleCase::computeNodeMass() {
+  const double one_over_nbnode = dim == 2 ? .25 : .125;
+#pragma omp parallel for schedule(static)
+  for (long n = 0; n < nn; ++n) {
+    double s = 0.;
+    for (int a = nc_off[n]; a < nc_off[n + 1]; ++a) s += one_over_nbnode * cell_mass.g[nc_cell[a]];
+    node_mass[n] = s;
+  }
+}
+
+// ============================================================================ Lagrange
+// MahycoModule::saveValuesAtN, MahycoModule.cc:311-382.
+void OracleCase::saveValuesAtN() {
+  global_old_deltat = old_deltat;
+  pseudo_nm1 = pseudo_n;
+  pseudo_n = pseudo;
+  pressure_n = pressure;
+  cell_volume_n = cell_volume;
+  density_n = density;
+  internal_energy_n = internal_energy;
+  cqs_n = cqs;
+  velocity_n = velocity;
+}
+
+// MahycoModule::updateVelocity, MahycoModule.cc:438-562 (non-CSTS; elasto force of
+// NoModel adds +-0; gravity 0; anti-drift off with projection).
+void OracleCase::updateVelocity() {
+  const double dt = 0.5 * (global_old_deltat + global_deltat);
+#pragma omp parallel for schedule(static)
+  for (long n = 0; n < nn; ++n) {
+    R3 F{0., 0., 0.};
+    for (int a = nc_off[n]; a < nc_off[n + 1]; ++a) {
+      int c = nc_cell[a];
+      double pr = pressure_n.g[c] + pseudo_n.g[c];
+      F += pr * cqs_n[(long)c * nnpc + nc_local[a]];
+    }
+    velocity[n] += (dt / node_mass[n]) * F;
+  }
+}
+
+// MahycoModule::applyBoundaryCondition, MahycoModule.cc:756-807 (Vx/Vy/Vz).
+void OracleCase::applyBoundaryCondition() {
+  for (int b = 0; b < opt.nb_bc; ++b) {
+    const MhBoundaryCondition& bc = opt.bc[b];
+    for (int n : group_nodes[bc.surface]) compr(velocity[n], bc.type) = bc.value;
+  }
+}
+
+// MahycoModule::updatePosition, MahycoModule.cc:730-751.
+void OracleCase::updatePosition() {
+  const double deltat = global_deltat;
+#pragma omp parallel for schedule(static)
+  for (long n = 0; n < nn; ++n) node_coord[n] += deltat * velocity[n];
+  const double one_over_nbnode = dim == 2 ? .25 : .125;
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < nc; ++c) {
+    R3 somme{0., 0., 0.};
+    for (int l = 0; l < nnpc; ++l) somme += node_coord[cell_nodes[c * nnpc + l]];
+    cell_coord[c] = one_over_nbnode * somme;
+  }
+}
+
+// MahycoModule::computeCQs, MahycoModule.cc:1869-1932.
+void OracleCase::computeCQs(const R3* x, const R3* fc, R3* q) {
+  const R3 c0 = fc[0], c1 = fc[1], c2 = fc[2], c3 = fc[3], c4 = fc[4], c5 = fc[5];
+  const R3 n1a04 = 0.5 * vecMul(x[0] - c0, x[3] - c0);
+  const R3 n1a03 = 0.5 * vecMul(x[3] - c0, x[2] - c0);
+  const R3 n1a02 = 0.5 * vecMul(x[2] - c0, x[1] - c0);
+  const R3 n1a01 = 0.5 * vecMul(x[1] - c0, x[0] - c0);
+  const R3 n2a05 = 0.5 * vecMul(x[0] - c1, x[4] - c1);
+  const R3 n2a12 = 0.5 * vecMul(x[4] - c1, x[7] - c1);
+  const R3 n2a08 = 0.5 * vecMul(x[7] - c1, x[3] - c1);
+  const R3 n2a04 = 0.5 * vecMul(x[3] - c1, x[0] - c1);
+  const R3 n3a01 = 0.5 * vecMul(x[0] - c2, x[1] - c2);
+  const R3 n3a06 = 0.5 * vecMul(x[1] - c2, x[5] - c2);
+  const R3 n3a09 = 0.5 * vecMul(x[5] - c2, x[4] - c2);
+  const R3 n3a05 = 0.5 * vecMul(x[4] - c2, x[0] - c2);
+  const R3 n4a09 = 0.5 * vecMul(x[4] - c3, x[5] - c3);
+  const R3 n4a10 = 0.5 * vecMul(x[5] - c3, x[6] - c3);
+  const R3 n4a11 = 0.5 * vecMul(x[6] - c3, x[7] - c3);
+  const R3 n4a12 = 0.5 * vecMul(x[7] - c3, x[4] - c3);
+  const R3 n5a02 = 0.5 * vecMul(x[1] - c4, x[2] - c4);
+  const R3 n5a07 = 0.5 * vecMul(x[2] - c4, x[6] - c4);
+  const R3 n5a10 = 0.5 * vecMul(x[6] - c4, x[5] - c4);
+  const R3 n5a06 = 0.5 * vecMul(x[5] - c4, x[1] - c4);
+  const R3 n6a03 = 0.5 * vecMul(x[2] - c5, x[3] - c5);
+  const R3 n6a08 = 0.5 * vecMul(x[3] - c5, x[7] - c5);
+  const R3 n6a11 = 0.5 * vecMul(x[7] - c5, x[6] - c5);
+  const R3 n6a07 = 0.5 * vecMul(x[6] - c5, x[2] - c5);
+  const double t = 1. / 12.;
+  q[0] = t * (5. * (n1a01 + n1a04 + n2a04 + n2a05 + n3a05 + n3a01) + (n1a02 + n1a03 + n2a08 + n2a12 + n3a06 + n3a09));
+  q[1] = t * (5. * (n1a01 + n1a02 + n3a01 + n3a06 + n5a06 + n5a02) + (n1a04 + n1a03 + n3a09 + n3a05 + n5a10 + n5a07));
+  q[2] = t * (5. * (n1a02 + n1a03 + n5a07 + n5a02 + n6a07 + n6a03) + (n1a01 + n1a04 + n5a06 + n5a10 + n6a11 + n6a08));
+  q[3] = t * (5. * (n1a03 + n1a04 + n2a08 + n2a04 + n6a08 + n6a03) + (n1a01 + n1a02 + n2a05 + n2a12 + n6a07 + n6a11));
+  q[4] = t * (5. * (n2a05 + n2a12 + n3a05 + n3a09 + n4a09 + n4a12) + (n2a08 + n2a04 + n3a01 + n3a06 + n4a10 + n4a11));
+  q[5] = t * (5. * (n3a06 + n3a09 + n4a09 + n4a10 + n5a10 + n5a06) + (n3a01 + n3a05 + n4a12 + n4a11 + n5a07 + n5a02));
+  q[6] = t * (5. * (n4a11 + n4a10 + n5a10 + n5a07 + n6a07 + n6a11) + (n4a12 + n4a09 + n5a06 + n5a02 + n6a03 + n6a08));
+  q[7] = t * (5. * (n2a08 + n2a12 + n4a12 + n4a11 + n6a11 + n6a08) + (n2a04 + n2a05 + n4a09 + n4a10 + n6a07 + n6a03));
+}
+
+// MahycoModule::computeGeometricValues, MahycoModule.cc:1097-1253 ("racine-cubique-volume").
+int OracleCase::computeGeometricValues() {
+  const double racine = dim == 2 ? .5 : 1. / 3.;
+  int err = 0;
+#pragma omp parallel for schedule(static) reduction(| : err)
+  for (long c = 0; c < nc; ++c) {
+    R3 coord[8];
+    for (int l = 0; l < nnpc; ++l) coord[l] = node_coord[cell_nodes[c * nnpc + l]];
+    R3* q = &cqs[c * nnpc];
+    if (dim == 3) {
+      R3 fc[6];
+      fc[0] = 0.25 * (coord[0] + coord[3] + coord[2] + coord[1]);
+      fc[1] = 0.25 * (coord[0] + coord[4] + coord[7] + coord[3]);
+      fc[2] = 0.25 * (coord[0] + coord[1] + coord[5] + coord[4]);
+      fc[3] = 0.25 * (coord[4] + coord[5] + coord[6] + coord[7]);
+      fc[4] = 0.25 * (coord[1] + coord[2] + coord[6] + coord[5]);
+      fc[5] = 0.25 * (coord[2] + coord[3] + coord[7] + coord[6]);
+      computeCQs(coord, fc, q);
+    } else {
+      R3 npc[5];
+      coord[4] = coord[0];
+      for (int l = 0; l < 4; ++l) {
+        npc[l + 1].x = 0.5 * (coord[l + 1].y - coord[l].y);
+        npc[l + 1].y = 0.5 * (coord[l].x - coord[l + 1].x);
+        npc[l + 1].z = 0.;
+      }
+      npc[0] = npc[4];
+      for (int l = 0; l < 4; ++l) q[l] = npc[l + 1] + npc[l];
+    }
+    double volume = 0.;
+    for (int l = 0; l < nnpc; ++l) volume += dot(node_coord[cell_nodes[c * nnpc + l]], q[l]);
+    volume /= dim;
+    cell_volume.g[c] = volume;
+    if (volume < 0.) err = 1;
+    caracteristic_length[c] = std::pow(cell_volume.g[c], racine);
+  }
+  // mixed cells: V_k = f_k V
+  for (long c = 0; c < nc; ++c) {
+    if (nbenv[c] == 1) continue;
+    for (int k = 0; k < m; ++k)
+      if (member(k, c)) ev(cell_volume, k, c) = ev(fracvol, k, c) * cell_volume.g[c];
+  }
+  return err ? MH_ERR_NEG_VOLUME : MH_OK;
+}
+
+// MahycoModule::updateDensity, MahycoModule.cc:1267-1316 (density-damage threshold 0: no clamp).
+void OracleCase::updateDensity() {
+  for (int k = 0; k < m; ++k) {
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < nc; ++c) {
+      if (!member(k, c)) continue;
+      double new_density = ev(cell_mass, k, c) / ev(cell_volume, k, c);
+      ev(density, k, c) = new_density;
+      ev(tau_density, k, c) = 0.5 * (1.0 / ev(density_n, k, c) + 1.0 / ev(density, k, c));
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < nc; ++c) {
+    double new_density = cell_mass.g[c] / cell_volume.g[c];
+    density.g[c] = new_density;
+    tau_density.g[c] = 0.5 * (1.0 / density_n.g[c] + 1.0 / density.g[c]);
+    div_u[c] = 1.0 / global_deltat * (1.0 / density.g[c] - 1.0 / density_n.g[c]) / tau_density.g[c];
+  }
+}
+
+// MahycoModule::computeArtificialViscosity, MahycoModule.cc:386-426.
+void OracleCase::computeArtificialViscosity() {
+  for (int k = 0; k < m; ++k) {
+    const double a1 = env[k].linear_pseudo_coeff, a2 = env[k].quadratic_pseudo_coeff;
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < nc; ++c) {
+      if (!member(k, c)) continue;
+      ev(pseudo, k, c) = 0.;
+      if (div_u[c] < 0.0) {
+        const double L = caracteristic_length[c];
+        ev(pseudo, k, c) = 1. / ev(tau_density, k, c) *
+                           (-a1 * L * sound_speed.g[c] * div_u[c] + a2 * L * L * div_u[c] * div_u[c]);
+      }
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < nc; ++c) {
+    if (nbenv[c] == 1) continue;
+    pseudo.g[c] = 0.;
+    for (int k = 0; k < m; ++k)
+      if (member(k, c)) pseudo.g[c] += ev(pseudo, k, c) * ev(fracvol, k, c);
+  }
+}
+
+// MahycoModule::updateEnergyAndPressure :1430-1480 -> updateEnergyAndPressureforGP :1623-1659
+// (no CSTS, pseudo-centree off; the gamma of the env EOS for StiffenedGas too).
+void OracleCase::updateEnergyAndPressure() {
+  for (int k = 0; k < m; ++k) {
+    const double adiabatic_cst = env[k].adiabatic_cst;
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < nc; ++c) {
+      if (!member(k, c)) continue;
+      double pseudo_v = 0.;
+      const double rho = ev(density, k, c), rhon = ev(density_n, k, c);
+      const double q = ev(pseudo, k, c);
+      if (q * (1.0 / rho - 1.0 / rhon) < 0.) pseudo_v = q;
+      const double denom = 1 + 0.5 * (adiabatic_cst - 1.0) * rho * (1.0 / rho - 1.0 / rhon);
+      const double numer = ev(internal_energy_n, k, c) - (0.5 * ev(pressure, k, c) + pseudo_v) * (1.0 / rho - 1.0 / rhon);
+      ev(internal_energy, k, c) = numer / denom;
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < nc; ++c) {
+    if (nbenv[c] == 1) continue;
+    internal_energy.g[c] = 0.;
+    for (int k = 0; k < m; ++k)
+      if (member(k, c)) internal_energy.g[c] += ev(mass_fraction, k, c) * ev(internal_energy, k, c);
+  }
+  if (!opt.with_projection) {  // :1465-1479
+    for (int k = 0; k < m; ++k) applyEOS(k);
+    computePressionMoyenne();
+  }
+}
+
+// PerfectGasEOSService::applyEOS :45-67, StiffenedGasEOSService::applyEOS :43-63.
+void OracleCase::applyEOS(int k) {
+  const double gamma = env[k].adiabatic_cst;
+  const double pi = env[k].limit_tension;
+  const bool sg = env[k].eos == MH_EOS_STIFFENED_GAS;
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < nc; ++c) {
+    if (!member(k, c)) continue;
+    const double e = ev(internal_energy, k, c), rho = ev(density, k, c);
+    if (sg) {
+      const double p = ((gamma - 1.) * rho * e) - (gamma * pi);
+      ev(pressure, k, c) = p;
+      ev(sound_speed, k, c) = std::sqrt((gamma / rho) * (p + pi));
+    } else {
+      const double p = (gamma - 1.) * rho * e;
+      ev(pressure, k, c) = p;
+      ev(sound_speed, k, c) = std::sqrt(gamma * p / rho);
+    }
+  }
+}
+
+// MahycoModule::computePressionMoyenne, MahycoModule.cc:1708-1735.
+void OracleCase::computePressionMoyenne() {
+  const double option = opt.calcul_vitson_max ? 1. : 0.;
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < nc; ++c) {
+    if (nbenv[c] == 1) continue;
+    pressure.g[c] = 0.;
+    sound_speed.g[c] = 0.;
+    for (int k = 0; k < m; ++k) {
+      if (!member(k, c)) continue;
+      pressure.g[c] += ev(fracvol, k, c) * ev(pressure, k, c);
+      sound_speed.g[c] = option * std::max(ev(sound_speed, k, c), sound_speed.g[c]);
+      sound_speed.g[c] += (1. - option) * ev(fracvol, k, c) * ev(sound_speed, k, c);
+    }
+  }
+}
+
+// ============================================================================ remap
+// MahycoModule::remap, PrepareRemap.cc:203-251.
+int OracleCase::remap() {
+  computeVariablesForRemap();
+  computeFaceQuantitesForRemap();
+  // RemapADIService::appliRemap, RemapADIService.cc:15-90
+  for (int i = 0; i < dim; ++i) {
+    int idir = (i + sens_projection) % dim;
+    if (cells_in_dir[idir] == 1) continue;
+    computeGradPhiFace(idir);
+    computeGradPhiCell(idir);
+    computeUpwindFaceQuantitiesForProjection(idir);
+    computeUremap(idir);
+    computeDualUremap(idir);
+  }
+  sens_projection = (sens_projection + 1) % dim;
+  node_coord = node_coord_0;  // isEuler() == "Full"
+  int st = remapVariables();
+  if (st != MH_OK) return st;
+  // PrepareRemap.cc:227-240 Materiau
+  for (long c = 0; c < nc; ++c) {
+    double mt = 0.;
+    for (int k = 0; k < m; ++k)
+      if (member(k, c)) mt += k * ev(fracvol, k, c);
+    materiau[c] = mt;
+  }
+  for (int k = 0; k < m; ++k) applyEOS(k);
+  computePressionMoyenne();
+  return MH_OK;
+}
+
+// PrepareRemap.cc:94-193 (slots V_k, M_k, E_k and V*q; dual slots u, m, KE).
+void OracleCase::computeVariablesForRemap() {
+  std::fill(u_lagrange.begin(), u_lagrange.end(), 0.0);
+  for (int k = 0; k < m; ++k) {
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < nc; ++c) {
+      if (!member(k, c)) continue;
+      double* U = &u_lagrange[c * nv];
+      const double V = ev(cell_volume, k, c), rho = ev(density, k, c);
+      U[k] = V;
+      U[m + k] = V * rho;
+      U[2 * m + k] = V * rho * ev(internal_energy, k, c);
+      U[3 * m] = cell_volume.g[c] * pseudo.g[c];
+      for (int v = 0; v < nv; ++v) phi_lagrange[c * nv + v] = U[v] / cell_volume.g[c];
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (long n = 0; n < nn; ++n) {
+    double* Ud = &u_dual[n * 5];
+    double* Pd = &phi_dual[n * 5];
+    const R3 u = velocity[n];
+    const double mn = node_mass[n];
+    Ud[0] = mn * u.x;
+    Ud[1] = mn * u.y;
+    Ud[2] = mn * u.z;
+    Ud[3] = mn;
+    Ud[4] = 0.5 * mn * squareNormL2(u);
+    Pd[0] = u.x;
+    Pd[1] = u.y;
+    Pd[2] = u.z;
+    Pd[3] = mn;
+    Pd[4] = 0.5 * squareNormL2(u);
+  }
+}
+
+// MahycoModule::computeFaceQuantitesForRemap, PrepareRemap.cc:18-59 (non-CSTS).
+void OracleCase::computeFaceQuantitesForRemap() {
+  const double one_over_nbnode = dim == 2 ? .5 : .25;
+#pragma omp parallel for schedule(static)
+  for (long f = 0; f < nf; ++f) {
+    const int* fn = &face_nodes[f * 4];
+    if (dim == 3) {
+      R3 v1 = node_coord[fn[2]] - node_coord[fn[0]];
+      R3 v2 = node_coord[fn[3]] - node_coord[fn[1]];
+      face_length_lagrange[f].x = 0.5 * std::fabs(v1.y * v2.z - v1.z * v2.y);
+      face_length_lagrange[f].y = 0.5 * std::fabs(v2.x * v1.z - v2.z * v1.x);
+      face_length_lagrange[f].z = 0.5 * std::fabs(v1.x * v2.y - v1.y * v2.x);
+    } else {
+      R3 t = node_coord[fn[1]] - node_coord[fn[0]];
+      face_length_lagrange[f] = {std::fabs(t.y), std::fabs(t.x), 0.};
+    }
+    R3 vm{0., 0., 0.}, fc{0., 0., 0.};
+    for (int l = 0; l < face_nbnode[f]; ++l) {
+      fc += one_over_nbnode * node_coord[fn[l]];
+      vm += velocity[fn[l]];
+    }
+    face_coord[f] = fc;
+    face_normal_velocity[f] = dot(one_over_nbnode * vm, face_normal[f]);
+  }
+}
+
+// RemapADIService::computeGradPhiFace, RemapADIService.cc:119-149 (inner faces only;
+// h_cell is used by the G/pente-borne/order-3 paths only and is not formed).
+void OracleCase::computeGradPhiFace(int idir) {
+#pragma omp parallel for schedule(static)
+  for (long f = face_off[idir]; f < face_off[idir] + face_cnt[idir]; ++f) {
+    int cb = face_prev[f], cf = face_next[f];
+    if (cb < 0 || cf < 0) continue;
+    deltax_lagrange[f] = dot(cell_coord[cf] - cell_coord[cb], face_normal[f]);
+    for (int v = 0; v < nv; ++v)
+      grad_phi_face[f * nv + v] = (phi_lagrange[(long)cf * nv + v] - phi_lagrange[(long)cb * nv + v]) / deltax_lagrange[f];
+  }
+}
+
+// RemapADIService::fluxLimiter, UtilesRemap.cc:108-120.
+static inline double fluxLimiter(int id, double r) {
+  if (id == MH_MINMOD) return std::max(0.0, std::min(1.0, r));
+  if (id == MH_SUPERBEE) return std::max(0.0, std::max(std::min(2.0 * r, 1.0), std::min(r, 2.0)));
+  if (id == MH_VANLEER) {
+    if (r <= 0.0) return 0.0;
+    return 2.0 * r / (1.0 + r);
+  }
+  return 0.0;
+}
+
+// RemapADIService::computeGradPhiCell :161-267 + computeAndLimitGradPhi UtilesRemap.cc:325-357
+// (classical limiters; pente-borne off).
+void OracleCase::computeGradPhiCell(int idir) {
+  std::fill(grad_phi.begin(), grad_phi.end(), 0.0);
+  if (opt.ordre_projection <= 1) return;
+  const int lim = opt.projection_limiteur_id;
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < nc; ++c) {
+    long backFace = -1, frontFace = -1;
+    for (int l = 0; l < nfpc; ++l) {
+      int f = cell_faces[c * nfpc + l];
+      if (face_dir[f] != idir) continue;
+      if (face_prev[f] == c) frontFace = f;
+      else if (face_next[f] == c) backFace = f;
+    }
+    for (int v = 0; v < nv; ++v) {
+      double gb = grad_phi_face[backFace * nv + v], gf = grad_phi_face[frontFace * nv + v];
+      double g = 0.;
+      if (gb != 0.) g += 0.5 * (fluxLimiter(lim, gf / gb) * gb);
+      if (gf != 0.) g += 0.5 * (fluxLimiter(lim, gb / gf) * gf);
+      grad_phi[c * nv + v] = g;
+    }
+  }
+}
+
+// RemapADIService::computeUpwindFaceQuantitiesForProjection :279-383 (order <= 2, no pente-borne).
+void OracleCase::computeUpwindFaceQuantitiesForProjection(int idir) {
+  std::fill(phi_face.begin(), phi_face.end(), 0.0);
+  const int order2 = opt.ordre_projection - 1;
+#pragma omp parallel for schedule(static)
+  for (long f = face_off[idir]; f < face_off[idir] + face_cnt[idir]; ++f) {
+    int cb = face_prev[f], cf = face_next[f];
+    if (cb < 0 || cf < 0) continue;
+    int up = (face_normal_velocity[f] * deltax_lagrange[f] > 0.0) ? cb : cf;
+    const double h = order2 * dot(face_coord[f] - cell_coord[up], face_normal[f]);
+    for (int v = 0; v < nv; ++v)
+      phi_face[f * nv + v] = phi_lagrange[(long)up * nv + v] + h * grad_phi[(long)up * nv + v];
+  }
+}
+
+// RemapADIService::computeUremap :395-575 (pente-borne off).
+void OracleCase::computeUremap(int idir) {
+  const R3 dirproj{0.5 * (1 - idir) * (2 - idir), 1.0 * idir * (2 - idir), -0.5 * idir * (1 - idir)};
+  const double deltat = global_deltat;
+  std::fill(dual_phi_flux.begin(), dual_phi_flux.end(), 0.0);
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < nc; ++c) {
+    double flux_face[4 * MH_MAX_ENV + 1];
+    for (int v = 0; v < nv; ++v) flux_face[v] = 0.;
+    for (int l = 0; l < nfpc; ++l) {
+      int f = cell_faces[c * nfpc + l];
+      if (std::fabs(dot(face_normal[f], dirproj)) >= 1.0E-10) {
+        const double fnv = face_normal_velocity[f];
+        const double len = comp(face_length_lagrange[f], idir);
+        const R3 outer = outer_face_normal[c * nfpc + l];
+        const double od = dot(outer, dirproj);
+        for (int v = 0; v < nv; ++v) {
+          double flux = od * fnv * len * deltat * phi_face[(long)f * nv + v];
+          flux_face[v] += flux;
+          dual_phi_flux[c * nv + v] += 0.5 * flux * comp(outer, idir);
+        }
+      }
+    }
+    double* U = &u_lagrange[c * nv];
+    for (int v = 0; v < nv; ++v) U[v] -= flux_face[v];
+    for (int k = 0; k < m; ++k) {
+      if (U[m + k] < 0.) U[m + k] = 0.;
+      if (U[2 * m + k] < 0.) U[2 * m + k] = 0.;
+    }
+    double somme_volume = 0.;
+    for (int k = 0; k < m; ++k) somme_volume += U[k];
+    for (int v = 0; v < nv; ++v) phi_lagrange[c * nv + v] = U[v] / somme_volume;
+  }
+}
+
+// RemapADIService::computeDualUremap, DualPhaseRemap.cc:38-258, with
+// computeDualGradPhi UtilesRemap.cc:19-72 and computeAndLimitGradPhiDual :369-400.
+void OracleCase::computeDualUremap(int idir) {
+  const double deltat = global_deltat;
+  const int lim = opt.projection_limiteur_id;
+  const std::vector<int>& prv = node_prev[idir];
+  const std::vector<int>& nxt = node_next[idir];
+  std::fill(dual_grad_phi.begin(), dual_grad_phi.end(), 0.0);
+  if (opt.ordre_projection > 1) {
+#pragma omp parallel for schedule(static)
+    for (long n = 0; n < nn; ++n) {
+      int back = prv[n], front = nxt[n];
+      if (back < 0 || front < 0) continue;
+      const double xn = comp(node_coord[n], idir);
+      const double xf = comp(node_coord[front], idir), xb = comp(node_coord[back], idir);
+      for (int v = 0; v < 3; ++v) {
+        double gf = (phi_dual[front * 5 + v] - phi_dual[n * 5 + v]) / (xf - xn);
+        double gb = (phi_dual[n * 5 + v] - phi_dual[back * 5 + v]) / (xn - xb);
+        double g = 0.;
+        if (gb != 0.) g += 0.5 * fluxLimiter(lim, gf / gb) * gb;
+        if (gf != 0.) g += 0.5 * fluxLimiter(lim, gb / gf) * gf;
+        dual_grad_phi[n * 5 + v] = g;
+      }
+    }
+  }
+  // back / front mass fluxes: faces of direction idir in face order, env order
+  const double OneOverNbcell = dim == 2 ? .5 : .25;
+#pragma omp parallel for schedule(static)
+  for (long n = 0; n < nn; ++n) {
+    double bsum = 0., fsum = 0.;
+    for (int a = nf_off[idir][n]; a < nf_off[idir][n + 1]; ++a) {
+      int f = nf_list[idir][a];
+      int cb = face_prev[f], cf = face_next[f];
+      if (cb >= 0)
+        for (int k = 0; k < m; ++k) bsum += OneOverNbcell * dual_phi_flux[(long)cb * nv + m + k];
+      if (cf >= 0)
+        for (int k = 0; k < m; ++k) fsum += OneOverNbcell * dual_phi_flux[(long)cf * nv + m + k];
+    }
+    back_flux_mass[n] = bsum;
+    front_flux_mass[n] = fsum;
+  }
+  const int order2 = opt.ordre_projection - 1;
+  std::vector<double> newU(u_dual);
+#pragma omp parallel for schedule(static)
+  for (long n = 0; n < nn; ++n) {
+    int back = prv[n], front = nxt[n];
+    if (back < 0 || front < 0) continue;
+    double* Ud = &newU[n * 5];
+    const double bf = back_flux_mass[n], ff = front_flux_mass[n];
+    Ud[3] += bf - ff;
+    const double xn = comp(node_coord[n], idir);
+    const double xf = comp(node_coord[front], idir), xb = comp(node_coord[back], idir);
+    double FU[3], BU[3], FE, BE;
+    const double ufront = 0.5 * (phi_dual[n * 5 + idir] + phi_dual[front * 5 + idir]);
+    const int signfront = ufront > 0 ? 1 : -1;
+    const int fsrc = (ff < 0.) ? front : (int)n;
+    for (int v = 0; v < 3; ++v)
+      FU[v] = phi_dual[fsrc * 5 + v] + order2 * (0.5 * dual_grad_phi[fsrc * 5 + v] * (signfront * (xf - xn) - deltat * ufront));
+    FE = phi_dual[fsrc * 5 + 4] + order2 * (0.5 * dual_grad_phi[fsrc * 5 + 4] * (signfront * (xf - xn) - deltat * ufront));
+    const double uback = 0.5 * (phi_dual[back * 5 + idir] + phi_dual[n * 5 + idir]);
+    const int signback = uback > 0 ? 1 : -1;
+    const int bsrc = (bf > 0.) ? back : (int)n;
+    for (int v = 0; v < 3; ++v)
+      BU[v] = phi_dual[bsrc * 5 + v] + order2 * (0.5 * dual_grad_phi[bsrc * 5 + v] * (signback * (xn - xb) - deltat * uback));
+    BE = phi_dual[bsrc * 5 + 4] + order2 * (0.5 * dual_grad_phi[bsrc * 5 + 4] * (signback * (xn - xb) - deltat * uback));
+    for (int v = 0; v < 3; ++v) Ud[v] += bf * BU[v] - ff * FU[v];
+    Ud[4] += bf * BE - ff * FE;
+  }
+  u_dual.swap(newU);
+#pragma omp parallel for schedule(static)
+  for (long n = 0; n < nn; ++n) {
+    if (prv[n] < 0 || nxt[n] < 0) continue;
+    double* Ud = &u_dual[n * 5];
+    double* Pd = &phi_dual[n * 5];
+    const double arithmetic_thresold = 1.e-300;
+    for (int v = 0; v < 3; ++v)
+      if (std::fabs(Ud[v]) < arithmetic_thresold) Ud[v] = 0.;
+    Pd[0] = Ud[0] / Ud[3];
+    Pd[1] = Ud[1] / Ud[3];
+    Pd[2] = Ud[2] / Ud[3];
+    Pd[3] = Ud[3];
+    Pd[4] = Ud[4] / Ud[3];
+  }
+}
+
+// Arcane environment membership change (env_cells.addItems/removeItems + forceRecompute).
+// Values kept across the change: a cell that becomes multi-environment copies its global
+// value into the partial slot of the environment it already had; new partial slots start
+// at 0; a cell that becomes single-environment copies that environment's partial value
+// into the global value.  remapVariables overwrites every value the cycle reads next.
+void OracleCase::setMembership(const std::vector<uint8_t>& newmask) {
+  for (long c = 0; c < nc; ++c) {
+    uint8_t o = mask[c], nw = newmask[c];
+    if (o == nw) continue;
+    int on = __builtin_popcount(o), nnb = __builtin_popcount(nw);
+    for (MatVar* v : matvars) {
+      double* part = &v->p[c * MH_MAX_ENV];
+      if (on == 1 && nnb != 1) {
+        int k0 = __builtin_ctz(o);
+        for (int k = 0; k < MH_MAX_ENV; ++k) part[k] = 0.;
+        part[k0] = v->g[c];
+      } else if (on != 1 && nnb == 1) {
+        int k1 = __builtin_ctz(nw);
+        v->g[c] = part[k1];
+      } else {
+        for (int k = 0; k < MH_MAX_ENV; ++k)
+          if (((nw >> k) & 1) && !((o >> k) & 1)) part[k] = 0.;
+      }
+    }
+    mask[c] = nw;
+    nbenv[c] = (int8_t)nnb;
+  }
+}
+
+// RemapADIService::remapVariables, RemapADIFinal.cc:21-346 (with dual projection).
+int OracleCase::remapVariables() {
+  const double thr = opt.remap_threshold;
+  std::vector<uint8_t> newmask(mask);
+  for (int k = 0; k < m; ++k) {  // :29-78
+    for (long c = 0; c < nc; ++c) {
+      const double* U = &u_lagrange[c * nv];
+      const bool was = member(k, c);
+      if (((U[k] / euler_volume[c]) < thr || (U[m + k] == 0.)) && was) {
+        newmask[c] &= (uint8_t)~(1u << k);
+      } else if ((U[k] / euler_volume[c]) > thr && !was) {
+        double volt = 0.;
+        for (int kk = 0; kk < m; ++kk) volt += U[kk];
+        double vol_ev = U[k] * euler_volume[c] / volt;
+        if ((vol_ev / euler_volume[c]) > thr && U[m + k] != 0.) newmask[c] |= (uint8_t)(1u << k);
+      }
+    }
+  }
+  setMembership(newmask);
+  int err = 0;
+#pragma omp parallel for schedule(static) reduction(| : err)
+  for (long c = 0; c < nc; ++c) {
+    const double* U = &u_lagrange[c * nv];
+    double vol_nplus1[MH_MAX_ENV], density_env[MH_MAX_ENV], energy_env[MH_MAX_ENV];
+    const double vol = euler_volume[c];
+    cell_volume.g[c] = euler_volume[c];
+    double volt = 0., masset = 0.;
+    for (int k = 0; k < m; ++k) {
+      vol_nplus1[k] = U[k];
+      volt += vol_nplus1[k];
+      masset += U[m + k];
+    }
+    const double unsurvolt = 1. / volt;
+    for (int k = 0; k < m; ++k) vol_nplus1[k] *= vol * unsurvolt;
+    double somme_frac = 0.;
+    const double unsurvol = 1. / vol;
+    for (int k = 0; k < m; ++k) {
+      if (!member(k, c)) continue;
+      double& f = ev(fracvol, k, c);
+      f = vol_nplus1[k] * unsurvol;
+      if (f < thr) f = 0.;
+      somme_frac += f;
+    }
+    double unsursomme_frac = 0.;
+    if (somme_frac != 0.) unsursomme_frac = 1. / somme_frac;
+    for (int k = 0; k < m; ++k)
+      if (member(k, c)) ev(fracvol, k, c) *= unsursomme_frac;
+    if (masset > thr * thr) {
+      const double unsurmasset = 1. / masset;
+      double fmasset = 0.;
+      for (int k = 0; k < m; ++k) {
+        if (!member(k, c)) continue;
+        double& mf = ev(mass_fraction, k, c);
+        mf = U[m + k] * unsurmasset;
+        if (ev(fracvol, k, c) < thr) mf = 0.;
+        fmasset += mf;
+      }
+      if (fmasset != 0.) {
+        const double unsurfmasset = 1. / fmasset;
+        for (int k = 0; k < m; ++k)
+          if (member(k, c)) ev(mass_fraction, k, c) *= unsurfmasset;
+      }
+    }
+    double density_nplus1 = 0.;
+    for (int k = 0; k < m; ++k) {
+      if (!member(k, c)) continue;
+      density_env[k] = 0.;
+      if (ev(fracvol, k, c) > thr && vol_nplus1[k] != 0.) density_env[k] = U[m + k] / vol_nplus1[k];
+      density_nplus1 += ev(fracvol, k, c) * density_env[k];
+    }
+    for (int k = 0; k < m; ++k) {
+      if (!member(k, c)) continue;
+      energy_env[k] = 0.;
+      if (ev(fracvol, k, c) > thr && U[m + k] != 0.) energy_env[k] = U[2 * m + k] / U[m + k];
+    }
+    density.g[c] = density_nplus1;
+    cell_mass.g[c] = euler_volume[c] * density_nplus1;
+    double energie_nplus1 = 0., pseudo_nplus1 = 0.;
+    for (int k = 0; k < m; ++k) {
+      if (!member(k, c)) continue;
+      ev(cell_mass, k, c) = ev(mass_fraction, k, c) * cell_mass.g[c];
+      ev(pseudo, k, c) = U[3 * m] * unsurvol;
+      ev(density, k, c) = density_env[k];
+      ev(internal_energy, k, c) = energy_env[k];
+      energie_nplus1 += ev(mass_fraction, k, c) * ev(internal_energy, k, c);
+      pseudo_nplus1 += ev(fracvol, k, c) * ev(pseudo, k, c);
+    }
+    internal_energy.g[c] = energie_nplus1;
+    pseudo.g[c] = pseudo_nplus1;
+    for (int k = 0; k < m; ++k) {
+      if (!member(k, c)) continue;
+      if (ev(density, k, c) < 0. || ev(internal_energy, k, c) < 0.) {
+        ev(internal_energy, k, c) = 0.;
+        ev(density, k, c) = 0.;
+        ev(fracvol, k, c) = 0.;
+        ev(mass_fraction, k, c) = 0.;
+      }
+      if (ev(density, k, c) != ev(density, k, c) || ev(internal_energy, k, c) != ev(internal_energy, k, c)) err = 1;
+    }
+  }
+  if (err) return MH_ERR_NAN_REMAP;
+  // :286-297 node velocity from the dual projection
+#pragma omp parallel for schedule(static)
+  for (long n = 0; n < nn; ++n) {
+    const double mp = u_dual[n * 5 + 3];
+    if (mp != 0.) {
+      velocity[n].x = u_dual[n * 5 + 0] / mp;
+      velocity[n].y = u_dual[n * 5 + 1] / mp;
+      velocity[n].z = u_dual[n * 5 + 2] / mp;
+    }
+  }
+  computeNodeMass();  // :299-306
+  for (long c = 0; c < nc; ++c) {  // :308-316
+    if (nbenv[c] == 1) continue;
+    for (int k = 0; k < m; ++k)
+      if (member(k, c)) ev(cell_volume, k, c) = ev(fracvol, k, c) * cell_volume.g[c];
+  }
+  return MH_OK;
+}
+
+// ============================================================================ time step
+// MahycoModule::computeDeltaT, MahycoModule.cc:1739-1864 (isEuler "Full": vmax included).
+int OracleCase::computeDeltaT() {
+  global_old_deltat = global_deltat;
+  old_deltat = global_old_deltat;
+  double minimum_aux = 1.79769313486231570815e+308;
+  std::vector<double> dxs(nc);
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < nc; ++c) {
+    double vmax = 0.;
+    for (int l = 0; l < nnpc; ++l) vmax = amax(normL2(velocity[cell_nodes[c * nnpc + l]]), vmax);
+    dxs[c] = caracteristic_length[c] / (sound_speed.g[c] + vmax);
+  }
+  for (long c = 0; c < nc; ++c) minimum_aux = amin(minimum_aux, dxs[c]);
+  double new_dt = opt.cfl * minimum_aux;
+  new_dt = amin(new_dt, 1.05 * global_old_deltat);
+  new_dt = amin(new_dt, opt.deltat_max);
+  if (new_dt < opt.deltat_min) return MH_ERR_DT_MIN;
+  global_deltat = new_dt;
+  const double stop_time = opt.final_time;
+  bool not_yet_finish = global_time < stop_time;
+  bool finish = global_time > stop_time;
+  bool too_much = (global_time + new_dt) > stop_time;
+  if (not_yet_finish && too_much) stopped = true;
+  if (finish) stopped = true;
+  return MH_OK;
+}
+
+// EnvSummationService::write, time_history/EnvSummationService.cc:77-169 (period 10).
+void OracleCase::timeHistoryWrite() {
+  if (global_iteration % 10 != 0) return;
+  history.push_back(global_time);
+  for (int k = 0; k < m; ++k) {
+    long cnt = 0;
+    double sm = 0., sp = 0., sr = 0., sv = 0., se = 0., smm = 0.;
+    for (long c = 0; c < nc; ++c) {
+      if (!member(k, c)) continue;
+      cnt++;
+      sm += ev(cell_mass, k, c);
+      sp += ev(pressure, k, c);
+      sr += ev(density, k, c) * ev(cell_volume, k, c);
+      sv += ev(cell_volume, k, c);
+      se += ev(internal_energy, k, c) * ev(cell_mass, k, c);
+      smm += ev(cell_mass, k, c);
+    }
+    history.push_back(cnt ? sm / cnt : 0.);
+    history.push_back(cnt ? sp / cnt : 0.);
+    history.push_back(cnt ? sr / sv : 0.);
+    history.push_back(cnt ? se / smm : 0.);
+    history.push_back((double)cnt);
+  }
+}
+
+// One iteration of MahycoLoop / MahycoLagrangeLoop (Mahyco.config:23-36, 57-69).
+int OracleCase::cycle() {
+  global_iteration += 1;
+  global_time += global_deltat;
+  saveValuesAtN();
+  updateVelocity();
+  applyBoundaryCondition();
+  updatePosition();
+  int st = computeGeometricValues();
+  if (st != MH_OK) return st;
+  updateDensity();
+  computeArtificialViscosity();
+  updateEnergyAndPressure();
+  if (opt.with_projection) {
+    st = remap();
+    if (st != MH_OK) return st;
+  }
+  st = computeDeltaT();
+  if (st != MH_OK) return st;
+  timeHistoryWrite();
+  return MH_OK;
+}
+
+// ============================================================================ C ABI
+extern "C" {
+
+OracleCase* oracle_create(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const MhOptions* opt) {
+  if (!mesh || !envs || !opt || nb_env < 1 || nb_env > MH_MAX_ENV) return nullptr;
+  if (mesh->dim != 2 && mesh->dim != 3) return nullptr;
+  if (mesh->nx < 1 || mesh->ny < 1 || (mesh->dim == 3 && mesh->nz < 1)) return nullptr;
+  if (opt->ordre_projection < 1 || opt->ordre_projection > 2) return nullptr;
+  OracleCase* h = new OracleCase();
+  h->m = nb_env;
+  for (int k = 0; k < nb_env; ++k) h->env[k] = envs[k];
+  h->opt = *opt;
+  h->build_mesh(*mesh);
+  const long nc = h->nc, nn = h->nn, nf = h->nf;
+  h->nv = 3 * h->m + 1;
+  h->velocity.assign(nn, {0., 0., 0.});
+  h->force.assign(nn, {0., 0., 0.});
+  h->node_mass.assign(nn, 0.);
+  h->cell_coord.assign(nc, {0., 0., 0.});
+  h->face_coord.assign(nf, {0., 0., 0.});
+  h->face_normal.assign(nf, {0., 0., 0.});
+  h->cqs.assign(nc * h->nnpc, {0., 0., 0.});
+  h->outer_face_normal.assign(nc * h->nfpc, {0., 0., 0.});
+  h->caracteristic_length.assign(nc, 0.);
+  h->div_u.assign(nc, 0.);
+  h->euler_volume.assign(nc, 0.);
+  h->materiau.assign(nc, 0.);
+  h->mask.assign(nc, 1);
+  h->nbenv.assign(nc, 1);
+  h->matvars = {&h->fracvol, &h->mass_fraction, &h->density, &h->density_n, &h->internal_energy,
+                &h->internal_energy_n, &h->pressure, &h->pressure_n, &h->sound_speed, &h->pseudo,
+                &h->pseudo_n, &h->pseudo_nm1, &h->cell_volume, &h->cell_volume_n, &h->cell_mass,
+                &h->tau_density};
+  for (MatVar* v : h->matvars) v->init(nc);
+  const int nv = h->nv;
+  h->u_lagrange.assign(nc * nv, 0.);
+  h->phi_lagrange.assign(nc * nv, 0.);
+  h->grad_phi.assign(nc * nv, 0.);
+  h->grad_phi_face.assign(nf * nv, 0.);
+  h->phi_face.assign(nf * nv, 0.);
+  h->dual_phi_flux.assign(nc * nv, 0.);
+  h->u_dual.assign(nn * 5, 0.);
+  h->phi_dual.assign(nn * 5, 0.);
+  h->dual_grad_phi.assign(nn * 5, 0.);
+  h->deltax_lagrange.assign(nf, 0.);
+  h->face_normal_velocity.assign(nf, 0.);
+  h->face_length_lagrange.assign(nf, {0., 0., 0.});
+  h->back_flux_mass.assign(nn, 0.);
+  h->front_flux_mass.assign(nn, 0.);
+  h->hydroStartInit();
+  h->computeCellMass();
+  h->computeNodeMass();
+  return h;
+}
+
+void oracle_destroy(OracleCase* h) { delete h; }
+
+int oracle_run(OracleCase* h, int max_cycles) {
+  for (int i = 0; i < max_cycles; ++i) {
+    if (h->stopped) return MH_STOPPED;
+    int st = h->cycle();
+    if (st != MH_OK) return st;
+  }
+  return h->stopped ? MH_STOPPED : MH_OK;
+}
+
+double oracle_time(const OracleCase* h) { return h->global_time; }
+double oracle_deltat(const OracleCase* h) { return h->global_deltat; }
+int oracle_iteration(const OracleCase* h) { return h->global_iteration; }
+long oracle_nb_cells(const OracleCase* h) { return h->nc; }
+long oracle_nb_nodes(const OracleCase* h) { return h->nn; }
+
+int oracle_get_field(const OracleCase* hc, int field, int env, int comp_, double* out) {
+  OracleCase* h = const_cast<OracleCase*>(hc);
+  if (field >= 100) {
+    for (long n = 0; n < h->nn; ++n) {
+      if (field == MH_N_VELOCITY) out[n] = comp(h->velocity[n], comp_);
+      else if (field == MH_N_NODE_MASS) out[n] = h->node_mass[n];
+      else if (field == MH_N_COORD) out[n] = comp(h->node_coord[n], comp_);
+      else return MH_ERR_INVALID;
+    }
+    return MH_OK;
+  }
+  if (env >= h->m) return MH_ERR_INVALID;
+  MatVar* v = nullptr;
+  switch (field) {
+    case MH_F_DENSITY: v = &h->density; break;
+    case MH_F_PRESSURE: v = &h->pressure; break;
+    case MH_F_INTERNAL_ENERGY: v = &h->internal_energy; break;
+    case MH_F_SOUND_SPEED: v = &h->sound_speed; break;
+    case MH_F_PSEUDO_VISCOSITY: v = &h->pseudo; break;
+    case MH_F_CELL_MASS: v = &h->cell_mass; break;
+    case MH_F_CELL_VOLUME: v = &h->cell_volume; break;
+    case MH_F_FRACVOL: v = &h->fracvol; break;
+    case MH_F_MASS_FRACTION: v = &h->mass_fraction; break;
+    default: break;
+  }
+  for (long c = 0; c < h->nc; ++c) {
+    if (field == MH_F_CARACTERISTIC_LENGTH) out[c] = h->caracteristic_length[c];
+    else if (field == MH_F_ENV_MASK) out[c] = h->mask[c];
+    else if (field == MH_F_MATERIAU) out[c] = h->materiau[c];
+    else if (!v) return MH_ERR_INVALID;
+    else if (env < 0) out[c] = v->g[c];
+    else out[c] = h->member(env, c) ? h->ev(*v, env, c) : 0.;
+  }
+  return MH_OK;
+}
+
+int oracle_history(const OracleCase* h, double* out, int max_rows) {
+  const int w = 1 + 5 * h->m;
+  int rows = (int)(h->history.size() / w);
+  int n = rows < max_rows ? rows : max_rows;
+  if (out) std::memcpy(out, h->history.data(), sizeof(double) * (size_t)n * w);
+  return rows;
+}
+
+void oracle_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+}  // extern "C"

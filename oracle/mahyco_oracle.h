@@ -1,0 +1,51 @@
+/*
+ * mahyco_oracle.h -- C ABI of the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY.  The oracle is a CPU restatement of MaHyCo's
+ * Lagrange + ADI-remap cycle used as the parity checker for the HIP path and as
+ * the CPU baseline timing.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product library never links it.
+ */
+#ifndef MAHYCO_ORACLE_H
+#define MAHYCO_ORACLE_H
+
+#include "../include/mahyco_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct OracleCase OracleCase;
+
+/* Builds the mesh and runs the init entry points (HydroStartInit, ComputeCellMass,
+ * ComputeNodeMass).  Returns NULL on an invalid description. */
+OracleCase* oracle_create(const MhMesh* mesh, const MhEnvironment* envs, int nb_env,
+                          const MhOptions* opt);
+void oracle_destroy(OracleCase* h);
+
+/* Runs up to max_cycles iterations of the time loop (MahycoLoop or MahycoLagrangeLoop).
+ * Returns MH_OK, MH_STOPPED (final time reached) or a negative MhStatus. */
+int oracle_run(OracleCase* h, int max_cycles);
+
+double oracle_time(const OracleCase* h);
+double oracle_deltat(const OracleCase* h);
+int oracle_iteration(const OracleCase* h);
+long oracle_nb_cells(const OracleCase* h);
+long oracle_nb_nodes(const OracleCase* h);
+
+/* Copies a field (MhField) to out.  Cell fields: env = -1 (cell value) or env index.
+ * Node fields: comp = 0,1,2 for vectors. */
+int oracle_get_field(const OracleCase* h, int field, int env, int comp, double* out);
+
+/* Time history rows (EnvSummation, every 10 iterations): per row
+ * [time, then for env 0..m-1: CellMass avg, Pressure avg, Density vol-avg,
+ * InternalEnergy mass-avg, nb_envcells]. Returns number of rows; fills up to max_rows. */
+int oracle_history(const OracleCase* h, double* out, int max_rows);
+
+/* Number of OpenMP threads used by the oracle loops (0 = runtime default). */
+void oracle_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,157 @@
+"""Converts the reference's non-regression outputs into compact fixtures.
+
+Reads NONREGRESSION/<case>/output of the MaHyCo tree (the reference's own test
+data: EnSight Gold ASCII final states and 17-digit EnvSummation time histories,
+NONREGRESSION/launch_test.sh:74-96) and writes tests/golden/<case>.npz holding:
+
+  th_time                       time column (pins the whole dt sequence)
+  th_<Var>_<env index>          EnvSummation value column, Var in CellMass,
+                                Pressure, Density, InternalEnergy
+  cell_<Var>                    final per-cell field in global cell order
+                                (x fastest), printed with %12.5e by Arcane
+  node_<Var>[_<c>]              final per-node field in global node order
+  final_time                    ensight.case time value
+  env_names                     environment names of the dataset
+
+Parallel runs write one EnSight part per subdomain; parts are mapped back to
+global cells through the part geometry (cell centroids, node coordinates).
+
+Run inside a container that holds the reference tree:
+    python tests/golden/make_fixtures.py /root/reference/NONREGRESSION
+"""
+import os
+import re
+import sys
+
+import numpy as np
+
+CASES = {
+    # name: (dim, (nx, ny, nz), origin, length)
+    "CAS_BiSodCaseX": (3, (100, 5, 5), (0, 0, 0), (1., .1, .1)),
+    "CAS_BiSodCaseY": (3, (5, 100, 5), (0, 0, 0), (.1, 1., .1)),
+    "CAS_BiSodCaseZ": (3, (5, 5, 100), (0, 0, 0), (.1, .1, 1.)),
+    "CAS_SodCaseY": (3, (5, 100, 5), (0, 0, 0), (.1, 1., .1)),
+    "CAS_SodCaseZ": (3, (5, 5, 100), (0, 0, 0), (.1, .1, 1.)),
+    "CAS_Lag_BiSodCaseX": (3, (100, 5, 5), (0, 0, 0), (1., .1, .1)),
+    "CAS_2D_Lag_SodCaseX": (2, (100, 5, 1), (0, 0, 0), (1., .1, 0.)),
+    "CAS_2D_Sedov": (2, (60, 60, 1), (0, 0, 0), (1.2, 1.2, 0.)),
+    "CAS_bitriplepoint_minmod": (2, (70, 30, 1), (0, 0, 0), (.07, .03, 0.)),
+}
+TH_VARS = ["CellMass", "Pressure", "Density", "InternalEnergy"]
+CELL_VARS = ["Density", "Pressure", "InternalEnergy", "CellMass", "PseudoViscosity", "Materiau"]
+
+
+def _env_names(arc_path):
+    arc = re.sub(r"<!--.*?-->", "", open(arc_path, encoding="latin-1").read(), flags=re.S)
+    return re.findall(r"<environment>\s*<name>(\w+)</name>", arc)
+
+
+def _read_geo(path):
+    """Returns a list of parts: (node_xyz [n,3], connectivity [ne, k] 0-based)."""
+    lines = open(path).read().split("\n")
+    parts, i = [], 0
+    while i < len(lines):
+        if lines[i].strip() == "coordinates":
+            n = int(lines[i + 1])
+            vals = np.array([float(v) for v in lines[i + 2:i + 2 + 3 * n]])
+            xyz = vals.reshape(3, n).T
+            j = i + 2 + 3 * n
+            etype = lines[j].strip()
+            ne = int(lines[j + 1])
+            conn = np.array([[int(t) for t in lines[j + 2 + e].split()] for e in range(ne)]) - 1
+            parts.append((xyz, conn, etype))
+            i = j + 2 + ne
+        else:
+            i += 1
+    return parts
+
+
+def _read_var(path, nper, ncomp=1):
+    """Values per part; nper[p] = number of entries of part p."""
+    lines = open(path).read().split("\n")
+    out, i, p = [], 0, 0
+    while i < len(lines):
+        s = lines[i].strip()
+        if s in ("quad4", "hexa8", "coordinates"):
+            cnt = nper[p] * ncomp
+            out.append(np.array([float(v) for v in lines[i + 1:i + 1 + cnt]]))
+            i += 1 + cnt
+            p += 1
+        else:
+            i += 1
+    return out
+
+
+def convert(root, case, outdir):
+    dim, n, origin, length = CASES[case]
+    nx, ny, nz = n
+    if dim == 2:
+        nz = 1
+    d = os.path.join(root, case, "output")
+    dep = os.path.join(d, "depouillement")
+    res = {}
+    names = _env_names(os.path.join(root, case, "Donnees.arc"))
+    res["env_names"] = np.array(names)
+    lines = open(os.path.join(dep, "ensight.case")).read().split()
+    res["final_time"] = np.array(float(lines[-1]))
+    # time histories
+    times = None
+    for k, en in enumerate(names):
+        for v in TH_VARS:
+            fn = os.path.join(d, "courbes", "gnuplot", f"{v}_{en}")
+            if not os.path.exists(fn):
+                continue
+            a = np.atleast_2d(np.loadtxt(fn))
+            res[f"th_{v}_{k}"] = a[:, 1]
+            times = a[:, 0]
+    res["th_time"] = times
+    # geometry -> global index maps
+    geo_dir = os.path.join(dep, "ensight.geo")
+    parts = _read_geo(os.path.join(geo_dir, sorted(os.listdir(geo_dir))[-1]))
+    dx = [length[a] / n[a] if n[a] else 1. for a in range(3)]
+    cell_maps, node_maps = [], []
+    single = len(parts) == 1  # sequential run: one part in global order (Lagrangian meshes move)
+    for xyz, conn, _ in parts:
+        if single:
+            cell_maps.append(np.arange(len(conn)))
+            node_maps.append(np.arange(len(xyz)))
+            continue
+        cen = xyz[conn].mean(axis=1)
+        ijk = [np.floor((cen[:, a] - origin[a]) / dx[a]).astype(int) for a in range(dim)]
+        cid = ijk[0] + nx * ijk[1] + (nx * ny * ijk[2] if dim == 3 else 0)
+        cell_maps.append(cid)
+        nijk = [np.rint((xyz[:, a] - origin[a]) / dx[a]).astype(int) for a in range(dim)]
+        nid = nijk[0] + (nx + 1) * nijk[1] + ((nx + 1) * (ny + 1) * nijk[2] if dim == 3 else 0)
+        node_maps.append(nid)
+    nc = nx * ny * nz
+    nn = (nx + 1) * (ny + 1) * ((nz + 1) if dim == 3 else 1)
+    for v in CELL_VARS:
+        vd = os.path.join(dep, v)
+        vals = _read_var(os.path.join(vd, sorted(os.listdir(vd))[-1]), [len(c) for c in cell_maps])
+        g = np.full(nc, np.nan)
+        for cm, pv in zip(cell_maps, vals):
+            g[cm] = pv
+        assert not np.isnan(g).any(), (case, v)
+        res[f"cell_{v}"] = g
+    vd = os.path.join(dep, "NodeMass")
+    vals = _read_var(os.path.join(vd, sorted(os.listdir(vd))[-1]), [len(m) for m in node_maps])
+    g = np.full(nn, np.nan)
+    for nm, pv in zip(node_maps, vals):
+        g[nm] = pv
+    res["node_NodeMass"] = g
+    vd = os.path.join(dep, "Velocity")
+    vals = _read_var(os.path.join(vd, sorted(os.listdir(vd))[-1]), [len(m) for m in node_maps], 3)
+    for c in range(3):
+        g = np.full(nn, np.nan)
+        for nm, pv in zip(node_maps, vals):
+            g[nm] = pv.reshape(3, -1)[c]
+        res[f"node_Velocity_{c}"] = g
+    np.savez_compressed(os.path.join(outdir, case + ".npz"), **res)
+
+
+if __name__ == "__main__":
+    root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/NONREGRESSION"
+    outdir = os.path.dirname(os.path.abspath(__file__))
+    for c in CASES:
+        convert(root, c, outdir)
+        print("wrote", c)

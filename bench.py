@@ -1,0 +1,159 @@
+#!/usr/bin/env python3
+"""Throughput of the Lagrange + ADI-remap hydro cycle on one MI355X.
+
+One step = one iteration of the time loop (Mahyco.config:23-36) over the whole mesh of
+the 3D bi-material shock tube: [0,1]^3, 256x128x128 hex cells, env A (x<0.5) rho=1 p=1
+gamma=1.4, env B rho=0.125 p=0.1 gamma=1.67, superbee limiter, order 2, CFL 0.3, FP64.
+The state is resident in HBM before the timed region starts; the time step stays on the
+device, so the timed steps run without host round trips.
+
+Prints one JSON line: metric, value (FP64 cell-cycles/s), roofline of the dominant kernel
+(algorithmic bytes per launch / its average HIP-event duration on the library stream),
+the cycle-level HBM fraction, and the CPU baseline (the oracle on the host cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_words(nc, nn, m, d):
+    """Compulsory FP64 words per cycle of the minimal fused schedule:
+    cells W_c = 2(6m+4) + 2(d+1)(3m+1), nodes W_n = 2(2d+1) + 2(d+1)(d+2) + d(d+2)."""
+    wc = 2 * (6 * m + 4) + 2 * (d + 1) * (3 * m + 1)
+    wn = 2 * (2 * d + 1) + 2 * (d + 1) * (d + 2) + d * (d + 2)
+    return nc * wc + nn * wn
+
+
+def kernel_bytes(kind, nc, nn, m, d):
+    """Algorithmic bytes of one launch of each kernel (every field read or written once;
+    neighbour reuse is free; the Eulerian grid is rebuilt from indices)."""
+    nv = 3 * m + 1
+    words = {
+        # read u, x_prev (d each) and M, p, q per cell; write u, x_new (d each) and m_node
+        "k_node_lag": nn * (4 * d + 1) + nc * 3,
+        # read x_new (d) and M, rho, c, p, e per cell; write U (3m+1), V, L
+        "k_cell_lag": nn * d + nc * (5 + nv + 2),
+        # read U (3m+1) and the node coordinates / velocity along the sweep; write U, mass fluxes (m)
+        "k_sweep_cell": nc * (2 * nv + m) + nn * (d + 1),
+        # read and write the dual state (d+1) and phi (d); read x along the sweep and the mass fluxes
+        "k_sweep_node": nn * (2 * (d + 1) + 2 * d + 1) + nc * m,
+        # read U (3m+1), V_E, L, u (d per node); write rho, e, p, c, q, M, f
+        "k_finalize": nc * (nv + 2 + 7) + nn * d,
+        "k_node_velocity": nn * (d + 1 + d),
+        "k_dt": 0,
+    }
+    return 8.0 * words[kind]
+
+
+def cpu_baseline(case_fn, cycles, threads):
+    """The oracle (C++ restatement of the reference, OpenMP) on the host cores."""
+    from oracle.pyoracle import OracleCase
+    case = case_fn()
+    o = OracleCase(case, pow_mode=0, threads=threads)
+    o.run(1)  # first iteration outside the timed region (page-in)
+    t0 = time.perf_counter()
+    o.run(cycles)
+    dt = time.perf_counter() - t0
+    nc = case.nb_cells
+    o.close()
+    return {"value": nc * cycles / dt, "unit": "cell-cycles/s", "cores": threads, "kind": "port",
+            "sample": f"{case.name}: {cycles} iterations after 1 untimed, {dt:.1f} s, OMP threads={threads}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--ny", type=int, default=128)
+    ap.add_argument("--nz", type=int, default=128)
+    ap.add_argument("--graph", type=int, default=6, help="cycles per hipGraph (0: eager launches)")
+    ap.add_argument("--profile-steps", type=int, default=12)
+    ap.add_argument("--cpu-cycles", type=int, default=4)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the path shards by slabs in design only)", file=sys.stderr)
+        sys.exit(2)
+
+    from mahyco_amd import cases
+    from mahyco_amd import types as T
+    from mahyco_amd.lib import GpuCase
+
+    make = lambda: cases.config2_bisod3d(args.nx, args.ny, args.nz)
+    case = make()
+    nc, nn, m, d = case.nb_cells, case.nb_nodes, len(case.envs), case.dim
+    print(f"[bench] {case.name}: cells={nc} nodes={nn} envs={m}", file=sys.stderr, flush=True)
+    g = GpuCase(case)
+    g.queue(args.warmup, graph=args.graph)
+    st = g.synchronize()
+    assert st == T.MH_OK, st
+    t0 = time.perf_counter()
+    g.queue(args.steps, graph=args.graph)
+    st = g.synchronize()
+    t1 = time.perf_counter()
+    assert st == T.MH_OK, T.STATUS_NAMES.get(st, st)
+    wall = t1 - t0
+    ms_per_step = 1e3 * wall / args.steps
+    value = nc * args.steps / wall
+    print(f"[bench] timed {args.steps} iterations: {ms_per_step:.3f} ms/iteration", file=sys.stderr, flush=True)
+
+    # per-kernel HIP-event durations on the library stream over a further profiled region
+    g.profile(True)
+    g.queue(args.profile_steps)
+    g.synchronize()
+    prof = g.profile_read()
+    g.profile(False)
+    kern = {k: {"ms_avg": v[0] / v[1], "launches": v[1], "ms_total": v[0]} for k, v in prof.items() if v[1]}
+    dominant = max(kern, key=lambda k: kern[k]["ms_total"])
+    b = kernel_bytes(dominant, nc, nn, m, d)
+    achieved = b / (kern[dominant]["ms_avg"] * 1e-3) / 1e9
+    for k, v in kern.items():
+        v["alg_GBs"] = kernel_bytes(k, nc, nn, m, d) / (v["ms_avg"] * 1e-3) / 1e9 if v["ms_avg"] > 0 else None
+    alg_cycle = 8.0 * algorithmic_words(nc, nn, m, d)
+    cyc_gbs = alg_cycle / (ms_per_step * 1e-3) / 1e9
+
+    out = {
+        "metric": "FP64 cell-cycles/s + % HBM roofline; ensemble runs/s, p50/p99 latency",
+        "value": value,
+        "unit": "cell-cycles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded initial condition of the configuration)",
+        "config": {"workload": "bisod3d_lagrange_adi_remap", "mesh": [args.nx, args.ny, args.nz],
+                   "cells": nc, "environments": m, "limiter": "superbee", "order": 2, "cfl": 0.3,
+                   "graph_cycles": args.graph},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant,
+                     "alg_bytes_per_launch": b},
+        "roofline_cycle": {"alg_bytes_per_cycle": alg_cycle, "achieved_GBs": cyc_gbs,
+                           "frac": cyc_gbs / HBM_PEAK_GBS},
+        "kernels": kern,
+    }
+    g.close()
+    if not args.no_cpu:
+        small = lambda: cases.config2_bisod3d(args.nx, args.ny, args.nz)
+        out["cpu_baseline"] = cpu_baseline(small, args.cpu_cycles, args.cpu_threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

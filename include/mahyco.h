@@ -1,0 +1,97 @@
+/*
+ * mahyco.h -- C ABI of the MI355X (gfx950) MaHyCo hydro path.
+ *
+ * One handle holds one structured-mesh case resident in HBM; every call is
+ * ordered on the handle's own HIP stream.  Plain pointers and sizes only, so the
+ * header binds from C, ctypes, cgo or JNI.  Status codes are MhStatus values
+ * (mahyco_types.h): the reference's exit(1) sites become negative returns.
+ *
+ * Each declaration names the reference interface it replaces (paths relative to
+ * the MaHyCo tree).
+ */
+#ifndef MAHYCO_H
+#define MAHYCO_H
+
+#include <stddef.h>
+#include "mahyco_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct MhHandle MhHandle;
+
+/* Init entry points HydroStartInit + ComputeCellMass + ComputeNodeMass
+ * (MahycoModule.cc:21-262, MahycoAnnexe.cc:11-235, casModel()->initMat/initVar,
+ * IEquationOfState::initEOS): builds the mesh, the initial state and uploads it. */
+int mahyco_create(const MhMesh* mesh, const MhEnvironment* envs, int nb_env,
+                  const MhOptions* options, MhHandle** out);
+void mahyco_destroy(MhHandle* h);
+
+/* Runs ncycles iterations of the time loop (Mahyco.config:23-36 MahycoLoop, or
+ * :57-69 MahycoLagrangeLoop when with_projection = 0), asynchronously on the handle's
+ * stream.  Iterations after the final-time stop or an error are no-ops on the device.
+ * Returns MH_OK once queued; errors surface in mahyco_status(). */
+int mahyco_cycle(MhHandle* h, int ncycles);
+
+/* Same as mahyco_cycle but replays a captured hipGraph of `cycles_per_graph`
+ * iterations (captured once per handle and graph size). */
+int mahyco_cycle_graph(MhHandle* h, int ncycles, int cycles_per_graph);
+
+/* Waits for the handle's stream; returns the device status word: MH_OK, MH_STOPPED
+ * (final time reached, MahycoModule.cc:1841-1857) or a negative MhStatus. */
+int mahyco_synchronize(MhHandle* h);
+int mahyco_status(MhHandle* h);
+
+/* Time-loop scalars (Arcane m_global_time, m_global_deltat, m_global_iteration). */
+double mahyco_time(MhHandle* h);
+double mahyco_deltat(MhHandle* h);
+int mahyco_iteration(MhHandle* h);
+long mahyco_nb_cells(MhHandle* h);
+long mahyco_nb_nodes(MhHandle* h);
+
+/* Copies one field (MhField) to host memory `out` (nb_cells or nb_nodes doubles).
+ * Cell fields: env = -1 (cell value) or environment index (0 outside the env).
+ * Node fields: comp = 0,1,2.  Replaces reading the Arcane variables by name. */
+int mahyco_get_field(MhHandle* h, int field, int env, int comp, double* out);
+
+/* ---- entry points of MahycoModule (Mahyco.axl:244-261), one call each ----------
+ * They run the same kernels as mahyco_cycle, grouped as the kernels fuse them:
+ *   mahyco_save_values_at_n        SaveValuesAtN        :311-382  (time t += dt)
+ *   mahyco_update_velocity         UpdateVelocity       :438-562  + ApplyBoundaryCondition
+ *                                  :756-807 + UpdatePosition (nodes) :730-751
+ *   mahyco_compute_geometric_values  ComputeGeometricValues :1097-1253 + UpdateDensity
+ *                                  :1267-1316 + ComputeArtificialViscosity :386-426 +
+ *                                  UpdateEnergyAndPressure :1430-1480 (+ remap vector build,
+ *                                  PrepareRemap.cc:94-193)
+ *   mahyco_remap                   Remap  PrepareRemap.cc:203-251 (IRemap::appliRemap
+ *                                  RemapADIService.cc:15-90, remapVariables
+ *                                  RemapADIFinal.cc:21-346, applyEOS, computePressionMoyenne)
+ *   mahyco_compute_delta_t         ComputeDeltaT :1739-1864
+ * The calls must come in time-loop order; an out-of-order call returns MH_ERR_INVALID.
+ * ApplyBoundaryCondition, UpdatePosition, UpdateDensity, ComputeArtificialViscosity,
+ * UpdateElasticityAndPlasticity (NoModel) and UpdateEnergyAndPressure are folded
+ * into the call that precedes them; their functions below only check the order. */
+int mahyco_save_values_at_n(MhHandle* h);
+int mahyco_update_velocity(MhHandle* h);
+int mahyco_apply_boundary_condition(MhHandle* h);
+int mahyco_update_position(MhHandle* h);
+int mahyco_compute_geometric_values(MhHandle* h);
+int mahyco_update_density(MhHandle* h);
+int mahyco_compute_artificial_viscosity(MhHandle* h);
+int mahyco_update_elasticity_and_plasticity(MhHandle* h);
+int mahyco_update_energy_and_pressure(MhHandle* h);
+int mahyco_remap(MhHandle* h);
+int mahyco_compute_delta_t(MhHandle* h);
+
+/* Kernel timing: when enabled, every launch is bracketed by HIP events on the
+ * handle's stream.  mahyco_profile_read fills up to `max` entries of kernel
+ * names (NUL-separated into `names`, `names_len` bytes), total milliseconds and
+ * launch counts; returns the number of kernel kinds. */
+int mahyco_profile_enable(MhHandle* h, int on);
+int mahyco_profile_read(MhHandle* h, char* names, int names_len, double* total_ms, int* counts, int max);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAHYCO_H */

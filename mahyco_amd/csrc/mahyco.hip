@@ -1,0 +1,917 @@
+// mahyco.hip -- host side of the MI355X hydro path: case initialisation, device memory,
+// kernel sequencing of one time-loop iteration, hipGraph replay and the C ABI of
+// include/mahyco.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mahyco.h"
+#include "mh_kernels.h"
+
+namespace {
+
+const char* kKernelNames[] = {"k_node_lag",   "k_cell_lag", "k_sweep_cell", "k_sweep_node",
+                              "k_node_velocity", "k_finalize", "k_dt"};
+enum { KN_NODE = 0, KN_CELL, KN_SWEEP_CELL, KN_SWEEP_NODE, KN_NODE_VEL, KN_FINAL, KN_DT, KN_COUNT };
+
+struct ProfRec {
+  int kind;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct MhHandle {
+  MhMesh mesh;
+  MhOptions opt;
+  MhEnvironment env[MH_MAX_ENV];
+  int m = 1;
+  MhGrid g;
+  MhDev dev;
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+  double* xl[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  double *Ua = nullptr, *Ub = nullptr, *dflux = nullptr, *Ud = nullptr, *Pa = nullptr, *Pb = nullptr;
+  long launched = 0;  // iterations queued by the host
+  int sens = 0;       // m_sens_projection of the next iteration
+  int nblk_c = 0, nblk_n = 0;
+  int phase = 0;  // entry-point order check
+  bool prof = false;
+  std::vector<ProfRec> prof_pending;
+  double prof_ms[KN_COUNT] = {0};
+  int prof_cnt[KN_COUNT] = {0};
+  std::map<long, hipGraphExec_t> graphs;  // key: (cycles << 8) | (parity << 4) | sens
+  std::vector<hipGraph_t> graph_defs;
+};
+
+namespace {
+
+int hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    fprintf(stderr, "mahyco: %s failed: %s\n", what, hipGetErrorString(e));
+    return MH_ERR_DEVICE;
+  }
+  return MH_OK;
+}
+#define HIPCHK(call)                                     \
+  do {                                                   \
+    int _st = hip_check((call), #call);                  \
+    if (_st != MH_OK) return _st;                        \
+  } while (0)
+
+template <class T>
+int dalloc(MhHandle* h, T** p, size_t n) {
+  void* q = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc(&q, n * sizeof(T));
+  if (e != hipSuccess) {
+    fprintf(stderr, "mahyco: hipMalloc(%zu bytes) failed: %s\n", n * sizeof(T), hipGetErrorString(e));
+    return MH_ERR_NOMEM;
+  }
+  h->allocs.push_back(q);
+  *p = (T*)q;
+  return MH_OK;
+}
+
+template <class T>
+int upload(MhHandle* h, T* d, const std::vector<T>& v) {
+  HIPCHK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, h->stream));
+  return MH_OK;
+}
+
+// Node coordinate along one axis of the Cartesian generator.
+double axis_coord(const MhMesh& mesh, int d, int i, int n) {
+  const double o = mesh.origin[d], l = mesh.length[d];
+  if (mesh.coord_rule == MH_COORD_DIV) return o + (l * (double)i) / (double)n;
+  return o + (double)i * (l / (double)n);
+}
+
+// Host-side case initialisation: restates the init entry points HydroStartInit
+// (MahycoModule.cc:21-210, MahycoAnnexe.cc:11-142), the case services' initMat/initVar
+// (SODService.cc, SEDOVService.cc, POINTTRIPLEService.cc), IEquationOfState::initEOS and
+// ComputeCellMass (:221-237).  The node mass is formed by the first node kernel.
+struct HostInit {
+  std::vector<double> ax, ay, az, VE, rho, p, c, q, M, e, fg, fk, chik, rhok, ek, pk, Mk;
+  std::vector<uint8_t> mask;
+  std::vector<double> u[3];
+};
+
+int host_init(MhHandle* h, HostInit& I) {
+  const MhGrid& g = h->g;
+  const MhOptions& opt = h->opt;
+  const int m = h->m, dim = g.dim;
+  const long nc = g.nc, nn = g.nn;
+  I.ax.resize(g.nnx);
+  I.ay.resize(g.nny);
+  I.az.resize(dim == 3 ? g.nnz : 1, 0.);
+  for (int i = 0; i < g.nnx; ++i) I.ax[i] = axis_coord(h->mesh, 0, i, g.nx);
+  for (int j = 0; j < g.nny; ++j) I.ay[j] = axis_coord(h->mesh, 1, j, g.ny);
+  if (dim == 3)
+    for (int k = 0; k < g.nnz; ++k) I.az[k] = axis_coord(h->mesh, 2, k, g.nz);
+  auto node = [&](int i, int j, int k) { return d3(I.ax[i], I.ay[j], dim == 3 ? I.az[k] : 0.); };
+  const size_t ne = (size_t)m * nc;
+  I.VE.assign(nc, 0.);
+  I.rho.assign(nc, 0.);
+  I.p.assign(nc, 0.);
+  I.c.assign(nc, 0.);
+  I.q.assign(nc, 0.);
+  I.M.assign(nc, 0.);
+  I.e.assign(nc, 0.);
+  I.fg.assign(nc, 1.);
+  I.fk.assign(ne, 1.);
+  I.chik.assign(ne, 1.);
+  I.rhok.assign(ne, 0.);
+  I.ek.assign(ne, 0.);
+  I.pk.assign(ne, 0.);
+  I.Mk.assign(ne, 0.);
+  I.mask.assign(nc, 1);
+  for (int d = 0; d < 3; ++d) I.u[d].assign(nn, 0.);
+  std::vector<double> xcx(nc), xcy(nc), xcz(nc), f1(nc, 0.);
+  const double w = dim == 3 ? .125 : .25;
+  for (int k = 0; k < g.nz; ++k)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) {
+        const long c = i + (long)g.nx * (j + (long)g.ny * k);
+        D3 x[8], qv[8];
+        x[0] = node(i, j, k);
+        x[1] = node(i + 1, j, k);
+        x[2] = node(i + 1, j + 1, k);
+        x[3] = node(i, j + 1, k);
+        if (dim == 3) {
+          x[4] = node(i, j, k + 1);
+          x[5] = node(i + 1, j, k + 1);
+          x[6] = node(i + 1, j + 1, k + 1);
+          x[7] = node(i, j + 1, k + 1);
+        }
+        const int nnode = dim == 3 ? 8 : 4;
+        D3 s = d3(0., 0., 0.);
+        for (int l = 0; l < nnode; ++l) s = s + x[l];
+        const D3 xc = w * s;
+        xcx[c] = xc.x;
+        xcy[c] = xc.y;
+        xcz[c] = xc.z;
+        double V;
+        if (dim == 3) {
+          cqs_hex(x, qv);
+          V = cell_volume<8>(x, qv, 3);
+        } else {
+          cqs_quad(x, qv);
+          V = cell_volume<4>(x, qv, 2);
+        }
+        I.VE[c] = V;
+        // Materiau (initMat) and membership (MahycoAnnexe.cc:97-139)
+        double mat = 0.;
+        if (opt.cas_model == MH_CASE_SOD) {
+          const int t = opt.cas_test;
+          if (t == 13 || t == 14 || t == 15) {
+            const double rr = t == 13 ? xc.x : t == 14 ? xc.y : xc.z;
+            mat = (rr < 0.5) ? 0. : 1.;
+          }
+        } else if (opt.cas_model == MH_CASE_POINTTRIPLE) {
+          if (opt.cas_test != 2) {
+            if (xc.x <= 0.01) mat = 0.;
+            else mat = (xc.y <= 0.015) ? 1. : 2.;
+          }
+        } else if (opt.cas_model == MH_CASE_RIEMANN) {
+          const int a = opt.interface_axis;
+          const D3 lo3 = x[0], hi3 = a == 0 ? x[1] : a == 1 ? x[3] : x[4];
+          const double lo = a == 0 ? lo3.x : a == 1 ? lo3.y : lo3.z;
+          const double hi = a == 0 ? hi3.x : a == 1 ? hi3.y : hi3.z;
+          double fr = (hi - opt.interface_position) / (hi - lo);
+          fr = fr < 0. ? 0. : (fr > 1. ? 1. : fr);
+          f1[c] = fr;
+          mat = fr;
+        }
+        uint8_t bits;
+        if (mat == 0.) bits = 1;
+        else if (mat > 0. && mat < 1.) bits = 3;
+        else if (mat == 1.) bits = 2;
+        else if (mat > 1. && mat < 2.) bits = 6;
+        else if (mat == 2.) bits = 4;
+        else if (mat > 2. && mat < 3.) bits = 12;
+        else if (mat == 3.) bits = 8;
+        else bits = 15;
+        bits &= (uint8_t)((1u << m) - 1u);
+        if (bits == 0) return MH_ERR_INVALID;
+        I.mask[c] = bits;
+        // Sedov deposit uses the cell volume of the initial geometry
+        if (opt.cas_model == MH_CASE_SEDOV) {
+          const double rhoInit = h->env[0].densite_initiale, pInit = h->env[0].pression_initiale;
+          bool center = false;
+          for (int l = 0; l < nnode; ++l) {
+            double rn;
+            if (dim == 3)
+              rn = std::sqrt((x[l].x - 0.) * (x[l].x - 0.) + (x[l].y - 0.) * (x[l].y - 0.) + (x[l].z - 0.) * (x[l].z - 0.));
+            else
+              rn = std::sqrt((x[l].x - 0.) * (x[l].x - 0.) + (x[l].y - 0.) * (x[l].y - 0.));
+            if (rn < 1.e-10) center = true;
+          }
+          I.e[c] = 0.244816e-5;
+          I.rho[c] = rhoInit;
+          I.p[c] = pInit;
+          if (center) {
+            I.e[c] += opt.sedov_deposit / V;
+            I.p[c] = 0.4 * rhoInit * I.e[c];
+          }
+        }
+      }
+  // initVar of the other case models (global values; partial values of multi-env cells)
+  for (long c = 0; c < nc; ++c) {
+    const uint8_t bits = I.mask[c];
+    const bool pure = __builtin_popcount(bits) == 1;
+    if (opt.cas_model == MH_CASE_SOD) {
+      const int t = opt.cas_test;
+      if (t == 3 || t == 4 || t == 5) {  // SODService::initVarMono :11-40
+        const double rr = t == 3 ? xcx[c] : t == 4 ? xcy[c] : xcz[c];
+        if (rr < 0.5) {
+          I.p[c] = h->env[0].pression_initiale;
+          I.rho[c] = h->env[0].densite_initiale;
+        } else {
+          I.p[c] = h->env[0].pression_initiale / 10.;
+          I.rho[c] = h->env[0].densite_initiale / 8.;
+        }
+      } else {  // SODService::initVar :71-121
+        const int s = bits == 1 ? 0 : 1;
+        I.rho[c] = h->env[s].densite_initiale;
+        I.p[c] = h->env[s].pression_initiale;
+        if (!pure)
+          for (int k = 0; k < m; ++k)
+            if ((bits >> k) & 1) {
+              I.rhok[k * nc + c] = h->env[s].densite_initiale;
+              I.pk[k * nc + c] = h->env[s].pression_initiale;
+            }
+      }
+    } else if (opt.cas_model == MH_CASE_POINTTRIPLE) {  // POINTTRIPLEService::initVar :64-128
+      const int s = xcx[c] <= 0.01 ? 0 : (xcy[c] <= 0.015 ? 1 : 2);
+      I.rho[c] = h->env[s].densite_initiale;
+      I.p[c] = h->env[s].pression_initiale;
+      if (!pure && ((bits >> s) & 1)) {
+        I.rhok[s * nc + c] = h->env[s].densite_initiale;
+        I.pk[s * nc + c] = h->env[s].pression_initiale;
+      }
+    } else if (opt.cas_model == MH_CASE_RIEMANN) {
+      if (pure) {
+        const int k = bits == 1 ? 0 : 1;
+        I.rho[c] = h->env[k].densite_initiale;
+        I.p[c] = h->env[k].pression_initiale;
+      } else {
+        I.fk[0 * nc + c] = 1.0 - f1[c];
+        I.fk[1 * nc + c] = f1[c];
+        double ft = 0.;
+        for (int k = 0; k < 2; ++k) ft += I.fk[k * nc + c] * h->env[k].densite_initiale;
+        for (int k = 0; k < 2; ++k) {
+          I.rhok[k * nc + c] = h->env[k].densite_initiale;
+          I.pk[k * nc + c] = h->env[k].pression_initiale;
+          I.chik[k * nc + c] = I.fk[k * nc + c] * h->env[k].densite_initiale / ft;
+        }
+        I.rho[c] = 0.;
+        I.p[c] = 0.;
+        I.c[c] = 0.;
+      }
+    }
+  }
+  const double* v0 = h->env[0].vitesse_initiale;
+  if (opt.cas_model != MH_CASE_SEDOV)
+    for (int d = 0; d < 3; ++d) std::fill(I.u[d].begin(), I.u[d].end(), dim == 2 && d == 2 ? 0. : v0[d]);
+  // initEOS per environment (PerfectGasEOSService.cc:12-37, StiffenedGasEOSService.cc:14-33)
+  for (long c = 0; c < nc; ++c) {
+    const uint8_t bits = I.mask[c];
+    const bool pure = __builtin_popcount(bits) == 1;
+    for (int k = 0; k < m; ++k) {
+      if (!((bits >> k) & 1)) continue;
+      const double gamma = h->env[k].adiabatic_cst;
+      double& pp = pure ? I.p[c] : I.pk[k * nc + c];
+      double& rr = pure ? I.rho[c] : I.rhok[k * nc + c];
+      double& ee = pure ? I.e[c] : I.ek[k * nc + c];
+      double cs;
+      if (h->env[k].eos == MH_EOS_STIFFENED_GAS) {
+        const double pi = h->env[k].limit_tension;
+        ee = (pp + (gamma * pi)) / ((gamma - 1.) * rr);
+        cs = std::sqrt((gamma / rr) * (pp + pi));
+      } else {
+        ee = pp / ((gamma - 1.) * rr);
+        cs = std::sqrt(gamma * pp / rr);
+      }
+      if (pure) I.c[c] = cs;
+      else {
+        // mixed-cell averages, MahycoModule.cc:154-170 (density and pressure accumulate
+        // onto the values left by initVar)
+        if (k == __builtin_ctz(bits)) I.e[c] = 0.;
+        I.e[c] += ee * I.chik[k * nc + c];
+        I.c[c] = (cs < I.c[c]) ? I.c[c] : cs;
+        I.rho[c] += rr * I.fk[k * nc + c];
+        I.p[c] += pp * I.fk[k * nc + c];
+      }
+    }
+  }
+  // ComputeCellMass (:221-237); m_euler_volume = V (:174-178)
+  for (long c = 0; c < nc; ++c) {
+    I.M[c] = I.rho[c] * I.VE[c];
+    const uint8_t bits = I.mask[c];
+    if (__builtin_popcount(bits) != 1)
+      for (int k = 0; k < m; ++k)
+        if ((bits >> k) & 1) I.Mk[k * nc + c] = I.chik[k * nc + c] * I.M[c];
+    I.fg[c] = 1.;
+  }
+  return MH_OK;
+}
+
+void prof_begin(MhHandle* h, ProfRec& r, int kind) {
+  r.kind = kind;
+  (void)hipEventCreate(&r.a);
+  (void)hipEventCreate(&r.b);
+  (void)hipEventRecord(r.a, h->stream);
+}
+void prof_end(MhHandle* h, ProfRec& r) {
+  (void)hipEventRecord(r.b, h->stream);
+  h->prof_pending.push_back(r);
+}
+
+#define LAUNCH(h, kind, kern, grid, block, ...)                          \
+  do {                                                                   \
+    ProfRec _r;                                                          \
+    if ((h)->prof) prof_begin((h), _r, (kind));                          \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, (h)->stream, __VA_ARGS__); \
+    if ((h)->prof) prof_end((h), _r);                                    \
+  } while (0)
+
+template <int DIM, int M>
+void launch_sweeps_t(MhHandle* h) {
+  MhDev& S = h->dev;
+  const int dim = DIM;
+  int dirs[3], nd = 0;
+  for (int i = 0; i < dim; ++i) {
+    const int idir = (i + h->sens) % dim;
+    const int ncell = idir == 0 ? h->g.nx : idir == 1 ? h->g.ny : h->g.nz;
+    if (ncell == 1) continue;  // RemapADIService.cc:27
+    dirs[nd++] = idir;
+  }
+  double* Uin = h->Ua;
+  double* Uout = h->Ub;
+  double* Pin = h->Pb;
+  double* Pout = h->Pa;
+  for (int s = 0; s < nd; ++s) {
+    const int first = s == 0, last = (s == nd - 1) && !first;
+    const int D = dirs[s];
+    if (D == 0) {
+      LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, Uin, Uout, h->dflux, first);
+      LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0>), h->nblk_n, MH_BLOCK, S, h->Ud, Pin, Pout, h->dflux, first, last);
+    } else if (D == 1) {
+      LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, Uin, Uout, h->dflux, first);
+      LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 1>), h->nblk_n, MH_BLOCK, S, h->Ud, Pin, Pout, h->dflux, first, last);
+    } else {
+      LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, (DIM == 3 ? 2 : 0), M>), h->nblk_c, MH_BLOCK, S, Uin, Uout, h->dflux, first);
+      LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, (DIM == 3 ? 2 : 0)>), h->nblk_n, MH_BLOCK, S, h->Ud, Pin, Pout, h->dflux, first, last);
+    }
+    std::swap(Uin, Uout);
+    std::swap(Pin, Pout);
+  }
+  if (nd <= 1) LAUNCH(h, KN_NODE_VEL, (k_node_velocity<DIM>), h->nblk_n, MH_BLOCK, S, (const double*)h->Ud, nd == 0 ? 1 : 0);
+  LAUNCH(h, KN_FINAL, (k_finalize<DIM, M>), h->nblk_c, MH_BLOCK, S, (const double*)Uin);
+  h->sens = (h->sens + 1) % dim;
+}
+
+void set_parity(MhHandle* h) {
+  const int par = (int)(h->launched & 1);
+  for (int d = 0; d < 3; ++d) {
+    h->dev.xlp[d] = h->xl[par][d];
+    h->dev.xlc[d] = h->xl[par ^ 1][d];
+  }
+}
+
+template <int DIM, int M>
+void launch_node_t(MhHandle* h) {
+  LAUNCH(h, KN_NODE, (k_node_lag<DIM>), h->nblk_n, MH_BLOCK, h->dev);
+}
+template <int DIM, int M>
+void launch_cell_t(MhHandle* h) {
+  LAUNCH(h, KN_CELL, (k_cell_lag<DIM, M>), h->nblk_c, MH_BLOCK, h->dev, h->Ua);
+}
+
+#define MH_DISPATCH(fn, h)                                      \
+  do {                                                          \
+    const int _d = (h)->g.dim, _m = (h)->m;                     \
+    if (_d == 3) {                                              \
+      if (_m == 1) fn<3, 1>(h);                                 \
+      else if (_m == 2) fn<3, 2>(h);                            \
+      else if (_m == 3) fn<3, 3>(h);                            \
+      else fn<3, 4>(h);                                         \
+    } else {                                                    \
+      if (_m == 1) fn<2, 1>(h);                                 \
+      else if (_m == 2) fn<2, 2>(h);                            \
+      else if (_m == 3) fn<2, 3>(h);                            \
+      else fn<2, 4>(h);                                         \
+    }                                                           \
+  } while (0)
+
+void launch_dt(MhHandle* h) { LAUNCH(h, KN_DT, k_dt, 1, MH_DT_BLOCK, h->dev, h->nblk_c); }
+
+// Phase functions shared by mahyco_cycle and the per-entry-point calls.
+void phase_node(MhHandle* h) {
+  set_parity(h);
+  MH_DISPATCH(launch_node_t, h);
+}
+void phase_cell(MhHandle* h) { MH_DISPATCH(launch_cell_t, h); }
+void phase_remap(MhHandle* h) {
+  if (h->opt.with_projection) MH_DISPATCH(launch_sweeps_t, h);
+}
+void phase_dt(MhHandle* h) {
+  launch_dt(h);
+  h->launched += 1;
+}
+
+void enqueue_cycle(MhHandle* h) {
+  phase_node(h);
+  phase_cell(h);
+  phase_remap(h);
+  phase_dt(h);
+}
+
+int read_scalars(MhHandle* h, MhScalars* out) {
+  HIPCHK(hipMemcpyAsync(out, h->dev.sc, sizeof(MhScalars), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return MH_OK;
+}
+
+int validate(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const MhOptions* o) {
+  if (!mesh || !envs || !o || nb_env < 1 || nb_env > MH_MAX_ENV) return MH_ERR_INVALID;
+  if (mesh->dim != 2 && mesh->dim != 3) return MH_ERR_INVALID;
+  if (mesh->nx < 1 || mesh->ny < 1 || (mesh->dim == 3 && mesh->nz < 1)) return MH_ERR_INVALID;
+  if (o->ordre_projection < 1 || o->ordre_projection > 2) return MH_ERR_UNSUPPORTED;
+  if (o->nb_bc < 0 || o->nb_bc > MH_MAX_BC) return MH_ERR_INVALID;
+  for (int b = 0; b < o->nb_bc; ++b)
+    if (o->bc[b].surface < 0 || o->bc[b].surface > 5 || o->bc[b].type < 0 || o->bc[b].type > 2) return MH_ERR_INVALID;
+  if (o->cas_model < 0 || o->cas_model > 3) return MH_ERR_UNSUPPORTED;
+  if (o->cas_model == MH_CASE_RIEMANN && nb_env < 2) return MH_ERR_INVALID;
+  if (o->cas_model == MH_CASE_POINTTRIPLE && nb_env < 3) return MH_ERR_INVALID;
+  for (int k = 0; k < nb_env; ++k)
+    if (envs[k].eos != MH_EOS_PERFECT_GAS && envs[k].eos != MH_EOS_STIFFENED_GAS) return MH_ERR_UNSUPPORTED;
+  const long nc = (long)mesh->nx * mesh->ny * (mesh->dim == 3 ? mesh->nz : 1);
+  const long nn = (long)(mesh->nx + 1) * (mesh->ny + 1) * (mesh->dim == 3 ? mesh->nz + 1 : 1);
+  if (nc > (1L << 31) || nn > (1L << 31)) return MH_ERR_INVALID;
+  return MH_OK;
+}
+
+}  // namespace
+
+// ============================================================================ C ABI
+extern "C" {
+
+int mahyco_create(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const MhOptions* options,
+                  MhHandle** out) {
+  if (!out) return MH_ERR_INVALID;
+  *out = nullptr;
+  int st = validate(mesh, envs, nb_env, options);
+  if (st != MH_OK) return st;
+  MhHandle* h = new MhHandle();
+  h->mesh = *mesh;
+  h->opt = *options;
+  h->m = nb_env;
+  for (int k = 0; k < nb_env; ++k) h->env[k] = envs[k];
+  MhGrid& g = h->g;
+  g.dim = mesh->dim;
+  g.nx = mesh->nx;
+  g.ny = mesh->ny;
+  g.nz = g.dim == 3 ? mesh->nz : 1;
+  g.nnx = g.nx + 1;
+  g.nny = g.ny + 1;
+  g.nnz = g.dim == 3 ? g.nz + 1 : 1;
+  g.nc = (long)g.nx * g.ny * g.nz;
+  g.nn = (long)g.nnx * g.nny * g.nnz;
+  h->nblk_c = (int)((g.nc + MH_BLOCK - 1) / MH_BLOCK);
+  h->nblk_n = (int)((g.nn + MH_BLOCK - 1) / MH_BLOCK);
+  st = hip_check(hipStreamCreate(&h->stream), "hipStreamCreate");
+  if (st != MH_OK) {
+    delete h;
+    return st;
+  }
+  HostInit I;
+  st = host_init(h, I);
+  if (st != MH_OK) {
+    mahyco_destroy(h);
+    return st;
+  }
+  MhDev& S = h->dev;
+  memset(&S, 0, sizeof(S));
+  S.g = g;
+  S.m = nb_env;
+  S.nv = 3 * nb_env + 1;
+  for (int k = 0; k < nb_env; ++k) {
+    S.env[k].gamma = envs[k].adiabatic_cst;
+    S.env[k].pinf = envs[k].limit_tension;
+    S.env[k].a1 = envs[k].linear_pseudo_coeff;
+    S.env[k].a2 = envs[k].quadratic_pseudo_coeff;
+    S.env[k].stiffened = envs[k].eos == MH_EOS_STIFFENED_GAS;
+  }
+  S.cfl = options->cfl;
+  S.thr_remap = options->remap_threshold;
+  S.dtmin = options->deltat_min;
+  S.dtmax = options->deltat_max;
+  S.tfinal = options->final_time;
+  S.order2 = options->ordre_projection - 1;
+  S.limiter = options->projection_limiteur_id;
+  S.with_proj = options->with_projection ? 1 : 0;
+  S.vitson_max = options->calcul_vitson_max ? 1 : 0;
+  S.nbc = options->nb_bc;
+  for (int b = 0; b < options->nb_bc; ++b) {
+    S.bc_surf[b] = options->bc[b].surface;
+    S.bc_type[b] = options->bc[b].type;
+    S.bc_val[b] = options->bc[b].value;
+  }
+  const long nc = g.nc, nn = g.nn;
+  const size_t ne = (size_t)nb_env * nc;
+  double *ax, *ay, *az, *VE;
+  int ok = MH_OK;
+  ok |= dalloc(h, &ax, g.nnx);
+  ok |= dalloc(h, &ay, g.nny);
+  ok |= dalloc(h, &az, I.az.size());
+  ok |= dalloc(h, &VE, nc);
+  for (int d = 0; d < 3; ++d) {
+    ok |= dalloc(h, &S.u[d], nn);
+    ok |= dalloc(h, &h->xl[0][d], nn);
+    ok |= dalloc(h, &h->xl[1][d], nn);
+  }
+  ok |= dalloc(h, &S.mn, nn);
+  ok |= dalloc(h, &S.rho, nc);
+  ok |= dalloc(h, &S.p, nc);
+  ok |= dalloc(h, &S.c, nc);
+  ok |= dalloc(h, &S.q, nc);
+  ok |= dalloc(h, &S.M, nc);
+  ok |= dalloc(h, &S.e, nc);
+  ok |= dalloc(h, &S.fg, nc);
+  ok |= dalloc(h, &S.mask, nc);
+  ok |= dalloc(h, &S.fk, ne);
+  ok |= dalloc(h, &S.chik, ne);
+  ok |= dalloc(h, &S.rhok, ne);
+  ok |= dalloc(h, &S.ek, ne);
+  ok |= dalloc(h, &S.pk, ne);
+  ok |= dalloc(h, &S.Mk, ne);
+  ok |= dalloc(h, &S.L, nc);
+  ok |= dalloc(h, &S.Vl, nc);
+  ok |= dalloc(h, &S.partial, (size_t)std::max(h->nblk_c, h->nblk_n));
+  ok |= dalloc(h, &S.err, 4);
+  ok |= dalloc(h, &S.sc, 1);
+  if (options->with_projection) {
+    const size_t nv = 3 * nb_env + 1;
+    ok |= dalloc(h, &h->Ua, nv * nc);
+    ok |= dalloc(h, &h->Ub, nv * nc);
+    ok |= dalloc(h, &h->dflux, (size_t)nb_env * nc);
+    ok |= dalloc(h, &h->Ud, (size_t)4 * nn);
+    ok |= dalloc(h, &h->Pa, (size_t)3 * nn);
+    ok |= dalloc(h, &h->Pb, (size_t)3 * nn);
+  } else {
+    ok |= dalloc(h, &h->Ua, 1);
+  }
+  if (ok != MH_OK) {
+    mahyco_destroy(h);
+    return MH_ERR_NOMEM;
+  }
+  S.ax = ax;
+  S.ay = ay;
+  S.az = az;
+  S.VE = VE;
+  // initial Lagrangian coordinates = Eulerian grid
+  std::vector<double> xs[3];
+  for (int d = 0; d < 3; ++d) xs[d].assign(nn, 0.);
+  for (int k = 0; k < g.nnz; ++k)
+    for (int j = 0; j < g.nny; ++j)
+      for (int i = 0; i < g.nnx; ++i) {
+        const long n = i + (long)g.nnx * (j + (long)g.nny * k);
+        xs[0][n] = I.ax[i];
+        xs[1][n] = I.ay[j];
+        xs[2][n] = g.dim == 3 ? I.az[k] : 0.;
+      }
+  MhScalars sc;
+  memset(&sc, 0, sizeof(sc));
+  sc.t = 0.;
+  sc.dt = options->deltat_init;
+  sc.dt_old = 0.;
+  std::vector<int> err(4, 0);
+  ok = MH_OK;
+  ok |= upload(h, ax, I.ax);
+  ok |= upload(h, ay, I.ay);
+  ok |= upload(h, az, I.az);
+  ok |= upload(h, VE, I.VE);
+  for (int d = 0; d < 3; ++d) {
+    ok |= upload(h, S.u[d], I.u[d]);
+    ok |= upload(h, h->xl[0][d], xs[d]);
+    ok |= upload(h, h->xl[1][d], xs[d]);
+  }
+  ok |= upload(h, S.rho, I.rho);
+  ok |= upload(h, S.p, I.p);
+  ok |= upload(h, S.c, I.c);
+  ok |= upload(h, S.q, I.q);
+  ok |= upload(h, S.M, I.M);
+  ok |= upload(h, S.e, I.e);
+  ok |= upload(h, S.fg, I.fg);
+  ok |= upload(h, S.mask, I.mask);
+  ok |= upload(h, S.fk, I.fk);
+  ok |= upload(h, S.chik, I.chik);
+  ok |= upload(h, S.rhok, I.rhok);
+  ok |= upload(h, S.ek, I.ek);
+  ok |= upload(h, S.pk, I.pk);
+  ok |= upload(h, S.Mk, I.Mk);
+  ok |= upload(h, S.err, err);
+  ok |= hip_check(hipMemcpyAsync(S.sc, &sc, sizeof(sc), hipMemcpyHostToDevice, h->stream), "upload scalars");
+  ok |= hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+  if (ok != MH_OK) {
+    mahyco_destroy(h);
+    return MH_ERR_DEVICE;
+  }
+  *out = h;
+  return MH_OK;
+}
+
+void mahyco_destroy(MhHandle* h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (auto& kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto gdef : h->graph_defs) (void)hipGraphDestroy(gdef);
+  for (auto& r : h->prof_pending) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (void* p : h->allocs) (void)hipFree(p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int mahyco_cycle(MhHandle* h, int ncycles) {
+  if (!h || ncycles < 0) return MH_ERR_INVALID;
+  if (h->phase != 0) return MH_ERR_INVALID;
+  for (int i = 0; i < ncycles; ++i) enqueue_cycle(h);
+  return hip_check(hipGetLastError(), "kernel launch");
+}
+
+int mahyco_cycle_graph(MhHandle* h, int ncycles, int cycles_per_graph) {
+  if (!h || ncycles < 0 || cycles_per_graph < 1) return MH_ERR_INVALID;
+  if (h->phase != 0) return MH_ERR_INVALID;
+  if (h->prof) return mahyco_cycle(h, ncycles);
+  // a replayable graph returns the coordinate parity and the sweep rotation to their
+  // starting values: its length is a multiple of lcm(2, dim)
+  const int period = h->g.dim == 3 ? 6 : 2;
+  const int P = ((cycles_per_graph + period - 1) / period) * period;
+  int done = 0;
+  while (ncycles - done >= P) {
+    const long key = ((long)P << 8) | ((h->launched & 1) << 4) | h->sens;
+    auto it = h->graphs.find(key);
+    if (it == h->graphs.end()) {
+      hipGraph_t gdef;
+      HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      const long l0 = h->launched;
+      const int s0 = h->sens;
+      for (int i = 0; i < P; ++i) enqueue_cycle(h);
+      HIPCHK(hipStreamEndCapture(h->stream, &gdef));
+      h->launched = l0;  // capture enqueues nothing: roll the host counters back
+      h->sens = s0;
+      hipGraphExec_t ex;
+      HIPCHK(hipGraphInstantiate(&ex, gdef, nullptr, nullptr, 0));
+      h->graph_defs.push_back(gdef);
+      it = h->graphs.emplace(key, ex).first;
+    }
+    HIPCHK(hipGraphLaunch(it->second, h->stream));
+    h->launched += P;
+    done += P;
+  }
+  for (; done < ncycles; ++done) enqueue_cycle(h);
+  return hip_check(hipGetLastError(), "graph launch");
+}
+
+int mahyco_synchronize(MhHandle* h) {
+  if (!h) return MH_ERR_INVALID;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return mahyco_status(h);
+}
+
+int mahyco_status(MhHandle* h) {
+  if (!h) return MH_ERR_INVALID;
+  MhScalars sc;
+  int st = read_scalars(h, &sc);
+  if (st != MH_OK) return st;
+  if (sc.status != 0) return sc.status;
+  return sc.stopped ? MH_STOPPED : MH_OK;
+}
+
+double mahyco_time(MhHandle* h) {
+  MhScalars sc;
+  if (!h || read_scalars(h, &sc) != MH_OK) return NAN;
+  return sc.t;
+}
+double mahyco_deltat(MhHandle* h) {
+  MhScalars sc;
+  if (!h || read_scalars(h, &sc) != MH_OK) return NAN;
+  return sc.dt;
+}
+int mahyco_iteration(MhHandle* h) {
+  MhScalars sc;
+  if (!h || read_scalars(h, &sc) != MH_OK) return -1;
+  return sc.iteration;
+}
+long mahyco_nb_cells(MhHandle* h) { return h ? h->g.nc : -1; }
+long mahyco_nb_nodes(MhHandle* h) { return h ? h->g.nn : -1; }
+
+int mahyco_get_field(MhHandle* h, int field, int env, int comp, double* out) {
+  if (!h || !out) return MH_ERR_INVALID;
+  MhScalars sc;
+  int st = read_scalars(h, &sc);
+  if (st != MH_OK) return st;
+  const long nc = h->g.nc, nn = h->g.nn;
+  MhDev& S = h->dev;
+  auto d2h = [&](const double* d, double* hst, long n) -> int {
+    HIPCHK(hipMemcpy(hst, d, n * sizeof(double), hipMemcpyDeviceToHost));
+    return MH_OK;
+  };
+  if (field >= 100) {
+    if (comp < 0 || comp > 2) return MH_ERR_INVALID;
+    if (field == MH_N_VELOCITY) return d2h(S.u[comp], out, nn);
+    if (field == MH_N_NODE_MASS) {
+      // m_node_mass after the iteration: sum over cells of M / 2^dim, in cell order
+      std::vector<double> M(nc);
+      st = d2h(S.M, M.data(), nc);
+      if (st != MH_OK) return st;
+      const MhGrid& g = h->g;
+      const double w = g.dim == 3 ? .125 : .25;
+      for (long n = 0; n < nn; ++n) {
+        const int i = (int)(n % g.nnx);
+        const long r = n / g.nnx;
+        const int j = (int)(r % g.nny), k = (int)(r / g.nny);
+        double s = 0.;
+        for (int kk = (g.dim == 3 ? k - 1 : 0); kk <= (g.dim == 3 ? k : 0); ++kk)
+          for (int jj = j - 1; jj <= j; ++jj)
+            for (int ii = i - 1; ii <= i; ++ii) {
+              if (ii < 0 || ii >= g.nx || jj < 0 || jj >= g.ny || kk < 0 || kk >= g.nz) continue;
+              s += w * M[ii + (long)g.nx * (jj + (long)g.ny * kk)];
+            }
+        out[n] = s;
+      }
+      return MH_OK;
+    }
+    if (field == MH_N_COORD) {
+      if (h->opt.with_projection) {
+        const MhGrid& g = h->g;
+        for (long n = 0; n < nn; ++n) {
+          const int i = (int)(n % g.nnx);
+          const long r = n / g.nnx;
+          const int j = (int)(r % g.nny), k = (int)(r / g.nny);
+          out[n] = comp == 0 ? axis_coord(h->mesh, 0, i, g.nx)
+                             : comp == 1 ? axis_coord(h->mesh, 1, j, g.ny)
+                                         : (g.dim == 3 ? axis_coord(h->mesh, 2, k, g.nz) : 0.);
+        }
+        return MH_OK;
+      }
+      return d2h(h->xl[sc.iteration & 1][comp], out, nn);
+    }
+    return MH_ERR_INVALID;
+  }
+  if (env >= h->m || env < -1) return MH_ERR_INVALID;
+  std::vector<uint8_t> mask(nc);
+  HIPCHK(hipMemcpy(mask.data(), S.mask, nc, hipMemcpyDeviceToHost));
+  const double* gsrc = nullptr;
+  const double* esrc = nullptr;
+  switch (field) {
+    case MH_F_DENSITY: gsrc = S.rho; esrc = S.rhok; break;
+    case MH_F_PRESSURE: gsrc = S.p; esrc = S.pk; break;
+    case MH_F_INTERNAL_ENERGY: gsrc = S.e; esrc = S.ek; break;
+    case MH_F_SOUND_SPEED: gsrc = S.c; break;
+    case MH_F_PSEUDO_VISCOSITY: gsrc = S.q; break;
+    case MH_F_CELL_MASS: gsrc = S.M; esrc = S.Mk; break;
+    case MH_F_FRACVOL: gsrc = S.fg; esrc = S.fk; break;
+    case MH_F_MASS_FRACTION: gsrc = nullptr; esrc = S.chik; break;
+    case MH_F_CARACTERISTIC_LENGTH: gsrc = S.L; break;
+    case MH_F_CELL_VOLUME: gsrc = S.VE; break;
+    default: break;
+  }
+  if (field == MH_F_ENV_MASK) {
+    for (long c = 0; c < nc; ++c) out[c] = mask[c];
+    return MH_OK;
+  }
+  if (field == MH_F_MATERIAU || field == MH_F_CELL_VOLUME || field == MH_F_FRACVOL ||
+      field == MH_F_MASS_FRACTION) {
+    std::vector<double> fg(nc), fk((size_t)h->m * nc), V(nc);
+    if ((st = d2h(S.fg, fg.data(), nc)) != MH_OK) return st;
+    if ((st = d2h(S.fk, fk.data(), (long)h->m * nc)) != MH_OK) return st;
+    const bool lagv = !h->opt.with_projection && sc.iteration > 0;
+    if ((st = d2h(lagv ? S.Vl : S.VE, V.data(), nc)) != MH_OK) return st;
+    std::vector<double> chik;
+    if (field == MH_F_MASS_FRACTION) {
+      chik.resize((size_t)h->m * nc);
+      if ((st = d2h(S.chik, chik.data(), (long)h->m * nc)) != MH_OK) return st;
+    }
+    for (long c = 0; c < nc; ++c) {
+      const uint8_t b = mask[c];
+      const bool pure = __builtin_popcount(b) == 1;
+      const int kp = __builtin_ctz(b ? b : 1);
+      double val = 0.;
+      if (field == MH_F_MATERIAU) {
+        if (pure) val = kp * fg[c];
+        else
+          for (int k = 0; k < h->m; ++k)
+            if ((b >> k) & 1) val += k * fk[k * nc + c];
+      } else if (field == MH_F_CELL_VOLUME) {
+        if (env < 0) val = V[c];
+        else if ((b >> env) & 1) val = pure ? V[c] : fk[env * nc + c] * V[c];
+      } else if (field == MH_F_FRACVOL) {
+        if (env < 0) val = fg[c];
+        else if ((b >> env) & 1) val = pure ? fg[c] : fk[env * nc + c];
+      } else {
+        if (env < 0) val = 0.;
+        else if ((b >> env) & 1) val = pure ? 1. : chik[env * nc + c];
+      }
+      out[c] = val;
+    }
+    return MH_OK;
+  }
+  if (!gsrc) return MH_ERR_INVALID;
+  if ((st = d2h(gsrc, out, nc)) != MH_OK) return st;
+  if (env < 0) return MH_OK;
+  std::vector<double> ev;
+  if (esrc) {
+    ev.resize(nc);
+    if ((st = d2h(esrc + (size_t)env * nc, ev.data(), nc)) != MH_OK) return st;
+  }
+  for (long c = 0; c < nc; ++c) {
+    const uint8_t b = mask[c];
+    if (!((b >> env) & 1)) out[c] = 0.;
+    else if (__builtin_popcount(b) != 1) out[c] = esrc ? ev[c] : out[c];
+  }
+  return MH_OK;
+}
+
+// ---- entry points -------------------------------------------------------------------
+static int step_phase(MhHandle* h, int from, int to) {
+  if (!h) return MH_ERR_INVALID;
+  if (h->phase != from) return MH_ERR_INVALID;
+  h->phase = to;
+  return MH_OK;
+}
+int mahyco_save_values_at_n(MhHandle* h) { return step_phase(h, 0, 1); }
+int mahyco_update_velocity(MhHandle* h) {
+  int st = step_phase(h, 1, 2);
+  if (st == MH_OK) phase_node(h);
+  return st;
+}
+int mahyco_apply_boundary_condition(MhHandle* h) { return step_phase(h, 2, 3); }
+int mahyco_update_position(MhHandle* h) { return step_phase(h, 3, 4); }
+int mahyco_compute_geometric_values(MhHandle* h) {
+  int st = step_phase(h, 4, 5);
+  if (st == MH_OK) phase_cell(h);
+  return st;
+}
+int mahyco_update_density(MhHandle* h) { return step_phase(h, 5, 6); }
+int mahyco_compute_artificial_viscosity(MhHandle* h) { return step_phase(h, 6, 7); }
+int mahyco_update_elasticity_and_plasticity(MhHandle* h) { return step_phase(h, 7, 8); }
+int mahyco_update_energy_and_pressure(MhHandle* h) { return step_phase(h, 8, 9); }
+int mahyco_remap(MhHandle* h) {
+  if (!h || !h->opt.with_projection) return MH_ERR_INVALID;
+  int st = step_phase(h, 9, 10);
+  if (st == MH_OK) phase_remap(h);
+  return st;
+}
+int mahyco_compute_delta_t(MhHandle* h) {
+  if (!h) return MH_ERR_INVALID;
+  const int expect = h->opt.with_projection ? 10 : 9;
+  int st = step_phase(h, expect, 0);
+  if (st == MH_OK) phase_dt(h);
+  return st;
+}
+
+int mahyco_profile_enable(MhHandle* h, int on) {
+  if (!h) return MH_ERR_INVALID;
+  h->prof = on != 0;
+  return MH_OK;
+}
+
+int mahyco_profile_read(MhHandle* h, char* names, int names_len, double* total_ms, int* counts, int max) {
+  if (!h) return MH_ERR_INVALID;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (auto& r : h->prof_pending) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, r.a, r.b);
+    h->prof_ms[r.kind] += ms;
+    h->prof_cnt[r.kind] += 1;
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  h->prof_pending.clear();
+  int pos = 0;
+  for (int k = 0; k < KN_COUNT && k < max; ++k) {
+    if (total_ms) total_ms[k] = h->prof_ms[k];
+    if (counts) counts[k] = h->prof_cnt[k];
+    if (names) {
+      const int len = (int)strlen(kKernelNames[k]) + 1;
+      if (pos + len <= names_len) {
+        memcpy(names + pos, kKernelNames[k], len);
+        pos += len;
+      }
+    }
+  }
+  return KN_COUNT;
+}
+
+}  // extern "C"

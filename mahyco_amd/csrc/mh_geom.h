@@ -1,0 +1,163 @@
+// mh_geom.h -- corner vectors (CQS), volumes and small vector helpers shared by the
+// gfx950 kernels and the host-side case initialisation.
+//
+// Every formula keeps the reference's operation order so the device reproduces the
+// CPU reference bit for bit (no FMA contraction: compiled with -ffp-contract=off):
+//   hexahedron CQS       MahycoModule::computeCQs, MahycoModule.cc:1869-1932
+//   face centres         MahycoModule::computeGeometricValues, MahycoModule.cc:1120-1125
+//   quadrangle CQS       MahycoModule.cc:1127-1146
+//   volume               MahycoModule.cc:1149-1162  (V = sum_n x_n . CQS_n / dim)
+#ifndef MH_GEOM_H
+#define MH_GEOM_H
+
+#include "mh_math.h"
+
+struct D3 {
+  double x, y, z;
+};
+MH_HD D3 d3(double x, double y, double z) { D3 r; r.x = x; r.y = y; r.z = z; return r; }
+MH_HD D3 operator+(D3 a, D3 b) { return d3(a.x + b.x, a.y + b.y, a.z + b.z); }
+MH_HD D3 operator-(D3 a, D3 b) { return d3(a.x - b.x, a.y - b.y, a.z - b.z); }
+MH_HD D3 operator*(double s, D3 a) { return d3(s * a.x, s * a.y, s * a.z); }
+MH_HD double dot3(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// half cross product 0.5 * vecMul(a, b)
+MH_HD D3 hcross(D3 a, D3 b) {
+  return 0.5 * d3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// Face centres of a hexahedron, local face order of Arcane's Hexaedron8.
+MH_HD D3 hex_fc0(const D3* x) { return 0.25 * (x[0] + x[3] + x[2] + x[1]); }
+MH_HD D3 hex_fc1(const D3* x) { return 0.25 * (x[0] + x[4] + x[7] + x[3]); }
+MH_HD D3 hex_fc2(const D3* x) { return 0.25 * (x[0] + x[1] + x[5] + x[4]); }
+MH_HD D3 hex_fc3(const D3* x) { return 0.25 * (x[4] + x[5] + x[6] + x[7]); }
+MH_HD D3 hex_fc4(const D3* x) { return 0.25 * (x[1] + x[2] + x[6] + x[5]); }
+MH_HD D3 hex_fc5(const D3* x) { return 0.25 * (x[2] + x[3] + x[7] + x[6]); }
+
+// All eight corner vectors of a hexahedron.
+MH_HD void cqs_hex(const D3* x, D3* q) {
+  const D3 c0 = hex_fc0(x), c1 = hex_fc1(x), c2 = hex_fc2(x);
+  const D3 c3 = hex_fc3(x), c4 = hex_fc4(x), c5 = hex_fc5(x);
+  const D3 n1a04 = hcross(x[0] - c0, x[3] - c0), n1a03 = hcross(x[3] - c0, x[2] - c0);
+  const D3 n1a02 = hcross(x[2] - c0, x[1] - c0), n1a01 = hcross(x[1] - c0, x[0] - c0);
+  const D3 n2a05 = hcross(x[0] - c1, x[4] - c1), n2a12 = hcross(x[4] - c1, x[7] - c1);
+  const D3 n2a08 = hcross(x[7] - c1, x[3] - c1), n2a04 = hcross(x[3] - c1, x[0] - c1);
+  const D3 n3a01 = hcross(x[0] - c2, x[1] - c2), n3a06 = hcross(x[1] - c2, x[5] - c2);
+  const D3 n3a09 = hcross(x[5] - c2, x[4] - c2), n3a05 = hcross(x[4] - c2, x[0] - c2);
+  const D3 n4a09 = hcross(x[4] - c3, x[5] - c3), n4a10 = hcross(x[5] - c3, x[6] - c3);
+  const D3 n4a11 = hcross(x[6] - c3, x[7] - c3), n4a12 = hcross(x[7] - c3, x[4] - c3);
+  const D3 n5a02 = hcross(x[1] - c4, x[2] - c4), n5a07 = hcross(x[2] - c4, x[6] - c4);
+  const D3 n5a10 = hcross(x[6] - c4, x[5] - c4), n5a06 = hcross(x[5] - c4, x[1] - c4);
+  const D3 n6a03 = hcross(x[2] - c5, x[3] - c5), n6a08 = hcross(x[3] - c5, x[7] - c5);
+  const D3 n6a11 = hcross(x[7] - c5, x[6] - c5), n6a07 = hcross(x[6] - c5, x[2] - c5);
+  const double t = 1. / 12.;
+  q[0] = t * (5. * (n1a01 + n1a04 + n2a04 + n2a05 + n3a05 + n3a01) + (n1a02 + n1a03 + n2a08 + n2a12 + n3a06 + n3a09));
+  q[1] = t * (5. * (n1a01 + n1a02 + n3a01 + n3a06 + n5a06 + n5a02) + (n1a04 + n1a03 + n3a09 + n3a05 + n5a10 + n5a07));
+  q[2] = t * (5. * (n1a02 + n1a03 + n5a07 + n5a02 + n6a07 + n6a03) + (n1a01 + n1a04 + n5a06 + n5a10 + n6a11 + n6a08));
+  q[3] = t * (5. * (n1a03 + n1a04 + n2a08 + n2a04 + n6a08 + n6a03) + (n1a01 + n1a02 + n2a05 + n2a12 + n6a07 + n6a11));
+  q[4] = t * (5. * (n2a05 + n2a12 + n3a05 + n3a09 + n4a09 + n4a12) + (n2a08 + n2a04 + n3a01 + n3a06 + n4a10 + n4a11));
+  q[5] = t * (5. * (n3a06 + n3a09 + n4a09 + n4a10 + n5a10 + n5a06) + (n3a01 + n3a05 + n4a12 + n4a11 + n5a07 + n5a02));
+  q[6] = t * (5. * (n4a11 + n4a10 + n5a10 + n5a07 + n6a07 + n6a11) + (n4a12 + n4a09 + n5a06 + n5a02 + n6a03 + n6a08));
+  q[7] = t * (5. * (n2a08 + n2a12 + n4a12 + n4a11 + n6a11 + n6a08) + (n2a04 + n2a05 + n4a09 + n4a10 + n6a07 + n6a03));
+}
+
+// One corner vector of a hexahedron: only the three faces around corner L are formed;
+// each term is computed exactly as in cqs_hex, so the result has the same bits.
+template <int L>
+MH_HD D3 cqs_hex_corner(const D3* x) {
+  constexpr bool F1 = (L <= 3);
+  constexpr bool F2 = (L == 0 || L == 3 || L == 4 || L == 7);
+  constexpr bool F3 = (L == 0 || L == 1 || L == 4 || L == 5);
+  constexpr bool F4 = (L >= 4);
+  constexpr bool F5 = (L == 1 || L == 2 || L == 5 || L == 6);
+  constexpr bool F6 = (L == 2 || L == 3 || L == 6 || L == 7);
+  D3 n1a04, n1a03, n1a02, n1a01, n2a05, n2a12, n2a08, n2a04, n3a01, n3a06, n3a09, n3a05;
+  D3 n4a09, n4a10, n4a11, n4a12, n5a02, n5a07, n5a10, n5a06, n6a03, n6a08, n6a11, n6a07;
+  if (F1) {
+    const D3 c0 = hex_fc0(x);
+    n1a04 = hcross(x[0] - c0, x[3] - c0); n1a03 = hcross(x[3] - c0, x[2] - c0);
+    n1a02 = hcross(x[2] - c0, x[1] - c0); n1a01 = hcross(x[1] - c0, x[0] - c0);
+  }
+  if (F2) {
+    const D3 c1 = hex_fc1(x);
+    n2a05 = hcross(x[0] - c1, x[4] - c1); n2a12 = hcross(x[4] - c1, x[7] - c1);
+    n2a08 = hcross(x[7] - c1, x[3] - c1); n2a04 = hcross(x[3] - c1, x[0] - c1);
+  }
+  if (F3) {
+    const D3 c2 = hex_fc2(x);
+    n3a01 = hcross(x[0] - c2, x[1] - c2); n3a06 = hcross(x[1] - c2, x[5] - c2);
+    n3a09 = hcross(x[5] - c2, x[4] - c2); n3a05 = hcross(x[4] - c2, x[0] - c2);
+  }
+  if (F4) {
+    const D3 c3 = hex_fc3(x);
+    n4a09 = hcross(x[4] - c3, x[5] - c3); n4a10 = hcross(x[5] - c3, x[6] - c3);
+    n4a11 = hcross(x[6] - c3, x[7] - c3); n4a12 = hcross(x[7] - c3, x[4] - c3);
+  }
+  if (F5) {
+    const D3 c4 = hex_fc4(x);
+    n5a02 = hcross(x[1] - c4, x[2] - c4); n5a07 = hcross(x[2] - c4, x[6] - c4);
+    n5a10 = hcross(x[6] - c4, x[5] - c4); n5a06 = hcross(x[5] - c4, x[1] - c4);
+  }
+  if (F6) {
+    const D3 c5 = hex_fc5(x);
+    n6a03 = hcross(x[2] - c5, x[3] - c5); n6a08 = hcross(x[3] - c5, x[7] - c5);
+    n6a11 = hcross(x[7] - c5, x[6] - c5); n6a07 = hcross(x[6] - c5, x[2] - c5);
+  }
+  const double t = 1. / 12.;
+  if (L == 0) return t * (5. * (n1a01 + n1a04 + n2a04 + n2a05 + n3a05 + n3a01) + (n1a02 + n1a03 + n2a08 + n2a12 + n3a06 + n3a09));
+  if (L == 1) return t * (5. * (n1a01 + n1a02 + n3a01 + n3a06 + n5a06 + n5a02) + (n1a04 + n1a03 + n3a09 + n3a05 + n5a10 + n5a07));
+  if (L == 2) return t * (5. * (n1a02 + n1a03 + n5a07 + n5a02 + n6a07 + n6a03) + (n1a01 + n1a04 + n5a06 + n5a10 + n6a11 + n6a08));
+  if (L == 3) return t * (5. * (n1a03 + n1a04 + n2a08 + n2a04 + n6a08 + n6a03) + (n1a01 + n1a02 + n2a05 + n2a12 + n6a07 + n6a11));
+  if (L == 4) return t * (5. * (n2a05 + n2a12 + n3a05 + n3a09 + n4a09 + n4a12) + (n2a08 + n2a04 + n3a01 + n3a06 + n4a10 + n4a11));
+  if (L == 5) return t * (5. * (n3a06 + n3a09 + n4a09 + n4a10 + n5a10 + n5a06) + (n3a01 + n3a05 + n4a12 + n4a11 + n5a07 + n5a02));
+  if (L == 6) return t * (5. * (n4a11 + n4a10 + n5a10 + n5a07 + n6a07 + n6a11) + (n4a12 + n4a09 + n5a06 + n5a02 + n6a03 + n6a08));
+  return t * (5. * (n2a08 + n2a12 + n4a12 + n4a11 + n6a11 + n6a08) + (n2a04 + n2a05 + n4a09 + n4a10 + n6a07 + n6a03));
+}
+
+// Quadrangle corner vectors: half edge normals of the two edges at each corner
+// (x[4] = x[0], npc[0] = npc[4]; cqs[ii] = npc[ii+1] + npc[ii]).
+MH_HD D3 quad_npc(const D3* x, int k) {  // npc[k+1], k = 0..3
+  const D3 a = x[k], b = x[(k + 1) & 3];
+  return d3(0.5 * (b.y - a.y), 0.5 * (a.x - b.x), 0.);
+}
+template <int L>
+MH_HD D3 cqs_quad_corner(const D3* x) {
+  const D3 p1 = quad_npc(x, L);            // npc[L+1]
+  const D3 p0 = quad_npc(x, (L + 3) & 3);  // npc[L] (npc[0] = npc[4])
+  return p1 + p0;
+}
+MH_HD void cqs_quad(const D3* x, D3* q) {
+  q[0] = cqs_quad_corner<0>(x);
+  q[1] = cqs_quad_corner<1>(x);
+  q[2] = cqs_quad_corner<2>(x);
+  q[3] = cqs_quad_corner<3>(x);
+}
+
+// V = sum_n x_n . CQS_n / dim
+template <int NNODE>
+MH_HD double cell_volume(const D3* x, const D3* q, int dim) {
+  double v = 0.;
+  for (int l = 0; l < NNODE; ++l) v += dot3(x[l], q[l]);
+  return v / dim;
+}
+
+// RemapADIService::fluxLimiter, UtilesRemap.cc:108-120 (classical limiters).
+MH_HD double mh_flux_limiter(int id, double r) {
+  // std::min(a, b) = (b < a) ? b : a ; std::max(a, b) = (a < b) ? b : a
+  if (id == 0) {
+    double t = (r < 1.0) ? r : 1.0;  // std::min(1.0, r)
+    return (0.0 < t) ? t : 0.0;      // std::max(0.0, t)
+  }
+  if (id == 1) {
+    double a = (1.0 < 2.0 * r) ? 1.0 : 2.0 * r;  // std::min(2r, 1)
+    double b = (2.0 < r) ? 2.0 : r;              // std::min(r, 2)
+    double c = (a < b) ? b : a;                  // std::max(a, b)
+    return (0.0 < c) ? c : 0.0;                  // std::max(0, c)
+  }
+  if (id == 2) {
+    if (r <= 0.0) return 0.0;
+    return 2.0 * r / (1.0 + r);
+  }
+  return 0.0;
+}
+
+#endif

@@ -1,0 +1,886 @@
+// mh_kernels.h -- gfx950 kernels of the Lagrange + ADI remap cycle (FP64 on the vector ALUs).
+//
+// Per cycle (Mahyco.config:23-36):
+//   k_node_lag      SaveValuesAtN (nodes) + UpdateVelocity + ApplyBoundaryCondition +
+//                   UpdatePosition (nodes): node-centric force gather in cell order
+//   k_cell_lag      UpdatePosition (cells) + ComputeGeometricValues + UpdateDensity +
+//                   ComputeArtificialViscosity + UpdateEnergyAndPressure, then the remap
+//                   vector U (PrepareRemap.cc:94-193) or, without remap, EOS + mean pressure
+//                   and the CFL candidate
+//   k_sweep_cell    one ADI direction, cell part (RemapADIService.cc:119-575)
+//   k_sweep_node    the same direction, dual part (DualPhaseRemap.cc:38-258)
+//   k_finalize      remapVariables (RemapADIFinal.cc:21-316) + applyEOS + computePressionMoyenne
+//                   + the CFL candidate of computeDeltaT
+//   k_dt            computeDeltaT reduction and time-step rules (MahycoModule.cc:1739-1864)
+//
+// Every value is produced by one thread with the reference's operation order, so the
+// results do not depend on launch shape or scheduling.
+#ifndef MH_KERNELS_H
+#define MH_KERNELS_H
+
+#include <hip/hip_runtime.h>
+
+#include "mh_geom.h"
+#include "mh_state.h"
+
+#define MH_BLOCK 256
+#define MH_DT_BLOCK 1024
+
+__device__ __forceinline__ bool mh_halted(const MhScalars* sc) { return sc->status != 0 || sc->stopped != 0; }
+
+// Arcane math::min / math::max
+__device__ __forceinline__ double amin(double a, double b) { return (a < b) ? a : b; }
+__device__ __forceinline__ double amax(double a, double b) { return (a < b) ? b : a; }
+
+__device__ __forceinline__ D3 load3(double* const* a, long n) { return d3(a[0][n], a[1][n], a[2][n]); }
+
+// Block-wide minimum (Arcane math::min semantics on non-NaN input), written by thread 0.
+__device__ __forceinline__ void block_min_store(double v, double* out) {
+  __shared__ double red[MH_BLOCK / 64];
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_xor(v, off, 64);
+    v = amin(v, o);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = amin(r, red[w]);
+    out[blockIdx.x] = r;
+  }
+}
+
+// ----------------------------------------------------------------------------- nodes
+// Corner L of cell (ii,jj,kk) seen from node (i,j,k); accumulates mass and force in the
+// order of a cell-order scatter (MahycoModule.cc:247-262 and :455-462).
+template <int DIM, int DI, int DJ, int DK>
+__device__ __forceinline__ void node_gather(const MhDev& S, int i, int j, int k, double& mass,
+                                            double& fx, double& fy, double& fz) {
+  const int ii = i - DI, jj = j - DJ, kk = k - DK;
+  if (ii < 0 || ii >= S.g.nx || jj < 0 || jj >= S.g.ny) return;
+  if (DIM == 3 && (kk < 0 || kk >= S.g.nz)) return;
+  const long c = ii + (long)S.g.nx * (jj + (long)S.g.ny * kk);
+  const double w = DIM == 3 ? 0.125 : 0.25;
+  mass += w * S.M[c];
+  const double pr = S.p[c] + S.q[c];
+  const long nnx = S.g.nnx, nnxy = (long)S.g.nnx * S.g.nny;
+  const long b = ii + nnx * (jj + (long)S.g.nny * kk);
+  constexpr int L = 4 * DK + (DJ ? (DI ? 2 : 3) : (DI ? 1 : 0));
+  D3 q;
+  if (DIM == 3) {
+    D3 x[8];
+    x[0] = load3(S.xlp, b);
+    x[1] = load3(S.xlp, b + 1);
+    x[2] = load3(S.xlp, b + 1 + nnx);
+    x[3] = load3(S.xlp, b + nnx);
+    x[4] = load3(S.xlp, b + nnxy);
+    x[5] = load3(S.xlp, b + 1 + nnxy);
+    x[6] = load3(S.xlp, b + 1 + nnx + nnxy);
+    x[7] = load3(S.xlp, b + nnx + nnxy);
+    q = cqs_hex_corner<L>(x);
+  } else {
+    D3 x[4];
+    x[0] = d3(S.xlp[0][b], S.xlp[1][b], 0.);
+    x[1] = d3(S.xlp[0][b + 1], S.xlp[1][b + 1], 0.);
+    x[2] = d3(S.xlp[0][b + 1 + nnx], S.xlp[1][b + 1 + nnx], 0.);
+    x[3] = d3(S.xlp[0][b + nnx], S.xlp[1][b + nnx], 0.);
+    q = cqs_quad_corner<L & 3>(x);
+  }
+  fx += pr * q.x;
+  fy += pr * q.y;
+  if (DIM == 3) fz += pr * q.z;
+}
+
+// SaveValuesAtN + UpdateVelocity (:438-562) + ApplyBoundaryCondition (:756-807) +
+// UpdatePosition (:730-751) for one node.
+template <int DIM>
+__global__ __launch_bounds__(MH_BLOCK) void k_node_lag(MhDev S) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= S.g.nn) return;
+  if (mh_halted(S.sc)) return;
+  const int nnx = S.g.nnx, nny = S.g.nny;
+  const int i = (int)(n % nnx);
+  const long r = n / nnx;
+  const int j = (int)(r % nny);
+  const int k = (int)(r / nny);
+  const double dt = S.sc->dt;
+  const double dth = 0.5 * (S.sc->dt_old + dt);
+  double mass = 0., fx = 0., fy = 0., fz = 0.;
+  if (DIM == 3) {
+    node_gather<3, 1, 1, 1>(S, i, j, k, mass, fx, fy, fz);
+    node_gather<3, 0, 1, 1>(S, i, j, k, mass, fx, fy, fz);
+    node_gather<3, 1, 0, 1>(S, i, j, k, mass, fx, fy, fz);
+    node_gather<3, 0, 0, 1>(S, i, j, k, mass, fx, fy, fz);
+  }
+  node_gather<DIM, 1, 1, 0>(S, i, j, k, mass, fx, fy, fz);
+  node_gather<DIM, 0, 1, 0>(S, i, j, k, mass, fx, fy, fz);
+  node_gather<DIM, 1, 0, 0>(S, i, j, k, mass, fx, fy, fz);
+  node_gather<DIM, 0, 0, 0>(S, i, j, k, mass, fx, fy, fz);
+  const double s = dth / mass;
+  double ux = S.u[0][n] + s * fx;
+  double uy = S.u[1][n] + s * fy;
+  double uz = DIM == 3 ? S.u[2][n] + s * fz : 0.;
+  for (int b = 0; b < S.nbc; ++b) {
+    const int sf = S.bc_surf[b];
+    bool on;
+    switch (sf) {
+      case MH_XMIN: on = (i == 0); break;
+      case MH_XMAX: on = (i == S.g.nx); break;
+      case MH_YMIN: on = (j == 0); break;
+      case MH_YMAX: on = (j == S.g.ny); break;
+      case MH_ZMIN: on = DIM == 3 ? (k == 0) : true; break;
+      default: on = DIM == 3 ? (k == S.g.nz) : true; break;
+    }
+    if (on) {
+      const int t = S.bc_type[b];
+      if (t == MH_BC_VX) ux = S.bc_val[b];
+      else if (t == MH_BC_VY) uy = S.bc_val[b];
+      else if (DIM == 3) uz = S.bc_val[b];
+    }
+  }
+  double x0, y0, z0;
+  if (S.with_proj) {
+    x0 = S.ax[i];
+    y0 = S.ay[j];
+    z0 = DIM == 3 ? S.az[k] : 0.;
+  } else {
+    x0 = S.xlp[0][n];
+    y0 = S.xlp[1][n];
+    z0 = DIM == 3 ? S.xlp[2][n] : 0.;
+  }
+  S.u[0][n] = ux;
+  S.u[1][n] = uy;
+  S.xlc[0][n] = x0 + dt * ux;
+  S.xlc[1][n] = y0 + dt * uy;
+  if (DIM == 3) {
+    S.u[2][n] = uz;
+    S.xlc[2][n] = z0 + dt * uz;
+  }
+  S.mn[n] = mass;
+}
+
+// ----------------------------------------------------------------------------- cells
+template <int DIM>
+__device__ __forceinline__ void cell_nodes(const MhDev& S, long b, double* const* xs, D3* x) {
+  const long nnx = S.g.nnx, nnxy = (long)S.g.nnx * S.g.nny;
+  if (DIM == 3) {
+    x[0] = load3(xs, b);
+    x[1] = load3(xs, b + 1);
+    x[2] = load3(xs, b + 1 + nnx);
+    x[3] = load3(xs, b + nnx);
+    x[4] = load3(xs, b + nnxy);
+    x[5] = load3(xs, b + 1 + nnxy);
+    x[6] = load3(xs, b + 1 + nnx + nnxy);
+    x[7] = load3(xs, b + nnx + nnxy);
+  } else {
+    x[0] = d3(xs[0][b], xs[1][b], 0.);
+    x[1] = d3(xs[0][b + 1], xs[1][b + 1], 0.);
+    x[2] = d3(xs[0][b + 1 + nnx], xs[1][b + 1 + nnx], 0.);
+    x[3] = d3(xs[0][b + nnx], xs[1][b + nnx], 0.);
+  }
+}
+
+__device__ __forceinline__ double node_speed(const MhDev& S, long n, int dim) {
+  const double ux = S.u[0][n], uy = S.u[1][n];
+  const double uz = dim == 3 ? S.u[2][n] : 0.;
+  return sqrt(ux * ux + uy * uy + uz * uz);
+}
+
+// max over the cell's nodes of |u| (MahycoModule.cc:1771-1774)
+template <int DIM>
+__device__ __forceinline__ double cell_vmax(const MhDev& S, long b) {
+  const long nnx = S.g.nnx, nnxy = (long)S.g.nnx * S.g.nny;
+  double v = 0.;
+  v = amax(node_speed(S, b, DIM), v);
+  v = amax(node_speed(S, b + 1, DIM), v);
+  v = amax(node_speed(S, b + 1 + nnx, DIM), v);
+  v = amax(node_speed(S, b + nnx, DIM), v);
+  if (DIM == 3) {
+    v = amax(node_speed(S, b + nnxy, DIM), v);
+    v = amax(node_speed(S, b + 1 + nnxy, DIM), v);
+    v = amax(node_speed(S, b + 1 + nnx + nnxy, DIM), v);
+    v = amax(node_speed(S, b + nnx + nnxy, DIM), v);
+  }
+  return v;
+}
+
+// updateEnergyAndPressureforGP, MahycoModule.cc:1633-1659 (pseudo-centree off)
+__device__ __forceinline__ double gp_energy(double e_n, double p, double rho, double rho_n, double q,
+                                            double gamma) {
+  const double dv = 1.0 / rho - 1.0 / rho_n;
+  double pseudo = 0.;
+  if (q * (1.0 / rho - 1.0 / rho_n) < 0.) pseudo = q;
+  const double denom = 1 + 0.5 * (gamma - 1.0) * rho * dv;
+  const double numer = e_n - (0.5 * p + pseudo) * dv;
+  return numer / denom;
+}
+
+// applyEOS (PerfectGasEOSService.cc:45-67, StiffenedGasEOSService.cc:43-63)
+__device__ __forceinline__ void eos(const MhEnvParam& ep, double rho, double e, double& p, double& c) {
+  const double g = ep.gamma;
+  if (ep.stiffened) {
+    p = ((g - 1.) * rho * e) - (g * ep.pinf);
+    c = sqrt((g / rho) * (p + ep.pinf));
+  } else {
+    p = (g - 1.) * rho * e;
+    c = sqrt(g * p / rho);
+  }
+}
+
+// ComputeGeometricValues (:1097-1253) + UpdateDensity (:1267-1316) +
+// ComputeArtificialViscosity (:386-426) + UpdateEnergyAndPressure (:1430-1480) for one cell.
+// With remap: writes U (V_k, V_k rho_k, V_k rho_k e_k, V q; PrepareRemap.cc:100-136), V, L.
+// Without: applies the EOS and the mean pressure and writes the CFL candidate.
+template <int DIM, int M>
+__global__ __launch_bounds__(MH_BLOCK) void k_cell_lag(MhDev S, double* __restrict__ U) {
+  constexpr int NV = 3 * M + 1;
+  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const bool active = c < S.g.nc && !mh_halted(S.sc);
+  double dxs = 1.79769313486231570815e+308;
+  if (active) {
+    const int nx = S.g.nx, ny = S.g.ny;
+    const int i = (int)(c % nx);
+    const long r = c / nx;
+    const int j = (int)(r % ny);
+    const int k = (int)(r / ny);
+    const long b = i + (long)S.g.nnx * (j + (long)S.g.nny * k);
+    D3 x[8], qv[8];
+    cell_nodes<DIM>(S, b, S.xlc, x);
+    double V;
+    if (DIM == 3) {
+      cqs_hex(x, qv);
+      V = cell_volume<8>(x, qv, 3);
+    } else {
+      cqs_quad(x, qv);
+      V = cell_volume<4>(x, qv, 2);
+    }
+    if (V < 0.) S.err[0] = MH_ERR_NEG_VOLUME;
+    const double L = DIM == 3 ? mh_pow_third(V) : mh_pow_half(V);
+    const double dt = S.sc->dt;
+    const uint8_t mk = S.mask[c];
+    const double cs = S.c[c];
+    const double rho_n = S.rho[c];
+    const double Mc = S.M[c];
+    const double rho = Mc / V;
+    const double tau = 0.5 * (1.0 / rho_n + 1.0 / rho);
+    const double divu = 1.0 / dt * (1.0 / rho - 1.0 / rho_n) / tau;
+    const long nc = S.g.nc;
+    double u_out[NV];
+    for (int v = 0; v < NV; ++v) u_out[v] = 0.;
+    if (__popc(mk) == 1) {
+      const int ke = __ffs(mk) - 1;
+      const MhEnvParam& ep = S.env[ke];
+      double q = 0.;
+      if (divu < 0.0) q = 1. / tau * (-ep.a1 * L * cs * divu + ep.a2 * L * L * divu * divu);
+      const double e = gp_energy(S.e[c], S.p[c], rho, rho_n, q, ep.gamma);
+      if (S.with_proj) {
+        for (int kk = 0; kk < M; ++kk)
+          if (kk == ke) {
+            u_out[kk] = V;
+            u_out[M + kk] = V * rho;
+            u_out[2 * M + kk] = V * rho * e;
+          }
+        u_out[3 * M] = V * q;
+      } else {
+        double p, cc;
+        eos(ep, rho, e, p, cc);
+        S.rho[c] = rho;
+        S.e[c] = e;
+        S.p[c] = p;
+        S.c[c] = cc;
+        S.q[c] = q;
+        dxs = L / (cc + cell_vmax<DIM>(S, b));
+      }
+    } else {
+      double fk[M], rk[M], qk[M], ek[M], Vk[M];
+      double qsum = 0., esum = 0.;
+      for (int kk = 0; kk < M; ++kk) {
+        if (!((mk >> kk) & 1)) continue;
+        const MhEnvParam& ep = S.env[kk];
+        const long o = kk * nc + c;
+        fk[kk] = S.fk[o];
+        Vk[kk] = fk[kk] * V;
+        const double rkn = S.rhok[o];
+        rk[kk] = S.Mk[o] / Vk[kk];
+        const double tk = 0.5 * (1.0 / rkn + 1.0 / rk[kk]);
+        qk[kk] = 0.;
+        if (divu < 0.0) qk[kk] = 1. / tk * (-ep.a1 * L * cs * divu + ep.a2 * L * L * divu * divu);
+        ek[kk] = gp_energy(S.ek[o], S.pk[o], rk[kk], rkn, qk[kk], ep.gamma);
+      }
+      for (int kk = 0; kk < M; ++kk)
+        if ((mk >> kk) & 1) qsum += qk[kk] * fk[kk];
+      for (int kk = 0; kk < M; ++kk)
+        if ((mk >> kk) & 1) esum += S.chik[kk * nc + c] * ek[kk];
+      if (S.with_proj) {
+        for (int kk = 0; kk < M; ++kk) {
+          if (!((mk >> kk) & 1)) continue;
+          u_out[kk] = Vk[kk];
+          u_out[M + kk] = Vk[kk] * rk[kk];
+          u_out[2 * M + kk] = Vk[kk] * rk[kk] * ek[kk];
+        }
+        u_out[3 * M] = V * qsum;
+      } else {
+        double ps = 0., cmix = 0.;
+        const double opt = S.vitson_max ? 1. : 0.;
+        for (int kk = 0; kk < M; ++kk) {
+          if (!((mk >> kk) & 1)) continue;
+          double p, cc;
+          eos(S.env[kk], rk[kk], ek[kk], p, cc);
+          const long o = kk * nc + c;
+          S.rhok[o] = rk[kk];
+          S.ek[o] = ek[kk];
+          S.pk[o] = p;
+          ps += fk[kk] * p;
+          cmix = opt * ((cc < cmix) ? cmix : cc);  // std::max(c_k, c)
+          cmix += (1. - opt) * fk[kk] * cc;
+        }
+        S.rho[c] = rho;
+        S.e[c] = esum;
+        S.p[c] = ps;
+        S.c[c] = cmix;
+        S.q[c] = qsum;
+        dxs = L / (cmix + cell_vmax<DIM>(S, b));
+      }
+    }
+    if (S.with_proj) {
+      for (int v = 0; v < NV; ++v) U[v * nc + c] = u_out[v];
+      S.Vl[c] = V;
+    }
+    S.L[c] = L;
+  }
+  if (!S.with_proj) block_min_store(dxs, S.partial);
+}
+
+// ----------------------------------------------------------------------------- remap sweeps
+// Lagrangian cell centre, direction-D component (UpdatePosition, MahycoModule.cc:743-750).
+template <int DIM, int D>
+__device__ __forceinline__ double cell_centre_d(const MhDev& S, long b) {
+  const long nnx = S.g.nnx, nnxy = (long)S.g.nnx * S.g.nny;
+  const double* xd = S.xlc[D];
+  double s = 0.;
+  s += xd[b];
+  s += xd[b + 1];
+  s += xd[b + 1 + nnx];
+  s += xd[b + nnx];
+  if (DIM == 3) {
+    s += xd[b + nnxy];
+    s += xd[b + 1 + nnxy];
+    s += xd[b + 1 + nnx + nnxy];
+    s += xd[b + nnx + nnxy];
+    return 0.125 * s;
+  }
+  return 0.25 * s;
+}
+
+// Lagrangian quantities of the face of direction D whose first node is f0
+// (computeFaceQuantitesForRemap, PrepareRemap.cc:18-59): projected area on D,
+// centre coordinate along D, normal velocity.  Face nodes in Arcane order.
+template <int DIM, int D>
+__device__ __forceinline__ void face_quant(const MhDev& S, long f0, double& area, double& xf, double& vf) {
+  const long nnx = S.g.nnx, nnxy = (long)S.g.nnx * S.g.nny;
+  if (DIM == 3) {
+    long nd[4];
+    if (D == 0) { nd[0] = f0; nd[1] = f0 + nnx; nd[2] = f0 + nnx + nnxy; nd[3] = f0 + nnxy; }
+    else if (D == 1) { nd[0] = f0; nd[1] = f0 + 1; nd[2] = f0 + 1 + nnxy; nd[3] = f0 + nnxy; }
+    else { nd[0] = f0; nd[1] = f0 + 1; nd[2] = f0 + 1 + nnx; nd[3] = f0 + nnx; }
+    const D3 x0 = load3(S.xlc, nd[0]), x1 = load3(S.xlc, nd[1]);
+    const D3 x2 = load3(S.xlc, nd[2]), x3 = load3(S.xlc, nd[3]);
+    const D3 v1 = x2 - x0, v2 = x3 - x1;
+    if (D == 0) area = 0.5 * fabs(v1.y * v2.z - v1.z * v2.y);
+    else if (D == 1) area = 0.5 * fabs(v2.x * v1.z - v2.z * v1.x);
+    else area = 0.5 * fabs(v1.x * v2.y - v1.y * v2.x);
+    const double xa = D == 0 ? x0.x : D == 1 ? x0.y : x0.z;
+    const double xb = D == 0 ? x1.x : D == 1 ? x1.y : x1.z;
+    const double xc = D == 0 ? x2.x : D == 1 ? x2.y : x2.z;
+    const double xd = D == 0 ? x3.x : D == 1 ? x3.y : x3.z;
+    double s = 0.;
+    s += 0.25 * xa;
+    s += 0.25 * xb;
+    s += 0.25 * xc;
+    s += 0.25 * xd;
+    xf = s;
+    const double* ud = S.u[D];
+    double us = 0.;
+    us += ud[nd[0]];
+    us += ud[nd[1]];
+    us += ud[nd[2]];
+    us += ud[nd[3]];
+    vf = 0.25 * us;
+  } else {
+    const long n1 = D == 0 ? f0 + nnx : f0 + 1;
+    const double dxv = S.xlc[0][n1] - S.xlc[0][f0];
+    const double dyv = S.xlc[1][n1] - S.xlc[1][f0];
+    area = D == 0 ? fabs(dyv) : fabs(dxv);
+    const double* xd = S.xlc[D];
+    double s = 0.;
+    s += 0.5 * xd[f0];
+    s += 0.5 * xd[n1];
+    xf = s;
+    const double* ud = S.u[D];
+    double us = 0.;
+    us += ud[f0];
+    us += ud[n1];
+    vf = 0.5 * us;
+  }
+}
+
+// One ADI direction, cell part: computeGradPhiFace (RemapADIService.cc:119-149),
+// computeGradPhiCell + computeAndLimitGradPhi (:161-267, UtilesRemap.cc:325-357),
+// computeUpwindFaceQuantitiesForProjection (:279-383) and computeUremap (:395-575).
+// Phi of the stencil is rebuilt from U (U / V before the first sweep, U / sum V_k after).
+template <int DIM, int D, int M>
+__global__ __launch_bounds__(MH_BLOCK) void k_sweep_cell(MhDev S, const double* __restrict__ Uin,
+                                                         double* __restrict__ Uout,
+                                                         double* __restrict__ dflux, int first) {
+  constexpr int NV = 3 * M + 1;
+  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (c >= S.g.nc) return;
+  if (mh_halted(S.sc)) return;
+  const int nx = S.g.nx, ny = S.g.ny;
+  const int i = (int)(c % nx);
+  const long r = c / nx;
+  const int j = (int)(r % ny);
+  const int k = (int)(r / ny);
+  const int a = D == 0 ? i : D == 1 ? j : k;
+  const int na = D == 0 ? nx : D == 1 ? ny : S.g.nz;
+  const long sD = D == 0 ? 1 : D == 1 ? (long)nx : (long)nx * ny;
+  const long nD = D == 0 ? 1 : D == 1 ? (long)S.g.nnx : (long)S.g.nnx * S.g.nny;
+  const long b = i + (long)S.g.nnx * (j + (long)S.g.nny * k);
+  const long nc = S.g.nc;
+  const double dt = S.sc->dt;
+  // per stencil cell (offsets -2..2 along D): existence, Phi denominator, centre along D
+  double den[5], xc[5];
+  bool valid[5];
+#pragma unroll
+  for (int o = -2; o <= 2; ++o) {
+    const int bb = a + o;
+    valid[o + 2] = (bb >= 0 && bb < na);
+    den[o + 2] = 1.;
+    xc[o + 2] = 0.;
+    if (valid[o + 2]) {
+      const long cc = c + o * sD;
+      if (first) {
+        den[o + 2] = S.Vl[cc];
+      } else {
+        double sv = 0.;
+        for (int kk = 0; kk < M; ++kk) sv += Uin[kk * nc + cc];
+        den[o + 2] = sv;
+      }
+      xc[o + 2] = cell_centre_d<DIM, D>(S, b + o * nD);
+    }
+  }
+  // face f lies between offsets f-2 and f-1; Delta x along D (boundary faces: none)
+  double dxf[4];
+#pragma unroll
+  for (int f = 0; f < 4; ++f) dxf[f] = (valid[f] && valid[f + 1]) ? xc[f + 1] - xc[f] : 0.;
+  // the two faces of the cell along D: s = 0 back (outer normal -e_D), s = 1 front (+e_D)
+  bool inner[2];
+  double area[2], vf[2], hup[2];
+  int up[2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    inner[s] = s == 0 ? (a > 0) : (a < na - 1);
+    area[s] = 0.;
+    vf[s] = 0.;
+    hup[s] = 0.;
+    up[s] = s;
+    if (!inner[s]) continue;
+    double xf;
+    face_quant<DIM, D>(S, b + s * nD, area[s], xf, vf[s]);
+    up[s] = (vf[s] * dxf[s + 1] > 0.0) ? s - 1 : s;  // offset of the upwind cell
+    hup[s] = (double)S.order2 * (xf - xc[up[s] + 2]);
+  }
+  // cell faces in Arcane local order: 3D min face first; 2D x: max face first
+  const int s0 = (DIM == 2 && D == 0) ? 1 : 0, s1 = 1 - s0;
+  const double od[2] = {-1., 1.};
+  for (int v = 0; v < NV; ++v) {
+    double phi[5];
+#pragma unroll
+    for (int o = 0; o < 5; ++o) phi[o] = valid[o] ? Uin[v * nc + c + (o - 2) * sD] / den[o] : 0.;
+    double gface[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) gface[f] = (valid[f] && valid[f + 1]) ? (phi[f + 1] - phi[f]) / dxf[f] : 0.;
+    double grad[3];
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+      double g = 0.;
+      if (S.order2 && valid[o + 1]) {
+        const double gb = gface[o], gf = gface[o + 1];
+        if (gb != 0.) g += 0.5 * (mh_flux_limiter(S.limiter, gf / gb) * gb);
+        if (gf != 0.) g += 0.5 * (mh_flux_limiter(S.limiter, gb / gf) * gf);
+      }
+      grad[o] = g;
+    }
+    double flux[2] = {0., 0.};
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      if (!inner[s]) continue;
+      const double pf = phi[up[s] + 2] + hup[s] * grad[up[s] + 1];
+      flux[s] = od[s] * vf[s] * area[s] * dt * pf;
+    }
+    double ff = 0.;
+    if (inner[s0]) ff += flux[s0];
+    if (inner[s1]) ff += flux[s1];
+    double un = Uin[v * nc + c] - ff;
+    if (v >= M && v < 2 * M) {
+      double dd = 0.;
+      if (inner[s0]) dd += 0.5 * flux[s0] * od[s0];
+      if (inner[s1]) dd += 0.5 * flux[s1] * od[s1];
+      dflux[(v - M) * nc + c] = dd;
+    }
+    if (v >= M && v < 3 * M && un < 0.) un = 0.;  // computeUremap clamps, :465-485
+    Uout[v * nc + c] = un;
+  }
+}
+
+// One ADI direction, dual part for one node: limited nodal velocity gradients
+// (computeDualGradPhi UtilesRemap.cc:19-72, computeAndLimitGradPhiDual :369-400), back/front
+// mass fluxes gathered from the cells behind/ahead in face order, env order
+// (DualPhaseRemap.cc:75-108), upwind momentum update (:110-239), filter and phi (:241-257).
+// Nodes with no neighbour along D keep their values.  On the last sweep the node velocity
+// u = mu / m is formed for every node (RemapADIFinal.cc:286-297).
+template <int DIM, int D>
+__global__ __launch_bounds__(MH_BLOCK) void k_sweep_node(MhDev S, double* __restrict__ Ud,
+                                                         const double* __restrict__ Pin,
+                                                         double* __restrict__ Pout,
+                                                         const double* __restrict__ dflux, int first,
+                                                         int last) {
+  constexpr int NC = DIM;  // velocity components carried
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= S.g.nn) return;
+  if (mh_halted(S.sc)) return;
+  const int nnx = S.g.nnx, nny = S.g.nny;
+  const int i = (int)(n % nnx);
+  const long r = n / nnx;
+  const int j = (int)(r % nny);
+  const int k = (int)(r / nny);
+  const int a = D == 0 ? i : D == 1 ? j : k;
+  const int na = D == 0 ? S.g.nx : D == 1 ? S.g.ny : S.g.nz;
+  const long nD = D == 0 ? 1 : D == 1 ? (long)nnx : (long)nnx * nny;
+  const long nn = S.g.nn, nc = S.g.nc;
+  double U[NC + 1];
+  if (first) {
+    const double mass = S.mn[n];
+    for (int cc = 0; cc < NC; ++cc) U[cc] = mass * S.u[cc][n];
+    U[NC] = mass;
+  } else {
+    for (int cc = 0; cc <= NC; ++cc) U[cc] = Ud[cc * nn + n];
+  }
+  const bool inner = a >= 1 && a <= na - 1;
+  if (!inner) {
+    for (int cc = 0; cc <= NC; ++cc) Ud[cc * nn + n] = U[cc];
+    for (int cc = 0; cc < NC; ++cc) Pout[cc * nn + n] = first ? S.u[cc][n] : Pin[cc * nn + n];
+    if (last && U[NC] != 0.)
+      for (int cc = 0; cc < NC; ++cc) S.u[cc][n] = U[cc] / U[NC];
+    return;
+  }
+  const double dt = S.sc->dt;
+  double P[5][NC], X[5];
+#pragma unroll
+  for (int o = -2; o <= 2; ++o) {
+    const int bb = a + o;
+    if (bb >= 0 && bb <= na) {
+      const long m2 = n + o * nD;
+      for (int cc = 0; cc < NC; ++cc) P[o + 2][cc] = first ? S.u[cc][m2] : Pin[cc * nn + m2];
+      X[o + 2] = S.xlc[D][m2];
+    } else {
+      for (int cc = 0; cc < NC; ++cc) P[o + 2][cc] = 0.;
+      X[o + 2] = 0.;
+    }
+  }
+  double G[3][NC];
+#pragma unroll
+  for (int o = -1; o <= 1; ++o) {
+    const int bb = a + o;
+    const bool in = S.order2 && bb >= 1 && bb <= na - 1;
+    for (int cc = 0; cc < NC; ++cc) {
+      double g = 0.;
+      if (in) {
+        const double gf = (P[o + 3][cc] - P[o + 2][cc]) / (X[o + 3] - X[o + 2]);
+        const double gb = (P[o + 2][cc] - P[o + 1][cc]) / (X[o + 2] - X[o + 1]);
+        if (gb != 0.) g += 0.5 * mh_flux_limiter(S.limiter, gf / gb) * gb;
+        if (gf != 0.) g += 0.5 * mh_flux_limiter(S.limiter, gb / gf) * gf;
+      }
+      G[o + 1][cc] = g;
+    }
+  }
+  // back / front mass fluxes over the faces of direction D through this node
+  const double w = DIM == 3 ? 0.25 : 0.5;
+  const int M = S.m;
+  double back = 0., front = 0.;
+  const long sD = D == 0 ? 1 : D == 1 ? (long)S.g.nx : (long)S.g.nx * S.g.ny;
+  // transverse axes (t1 fastest, t2 slowest) and their cell counts
+  const int t1 = D == 0 ? 1 : 0;
+  const int t2 = D == 2 ? 1 : 2;
+  const int it1 = t1 == 0 ? i : j;
+  const int it2 = t2 == 1 ? j : k;
+  const int n1 = t1 == 0 ? S.g.nx : S.g.ny;
+  const int n2 = t2 == 1 ? S.g.ny : S.g.nz;
+  const long st1 = t1 == 0 ? 1 : (long)S.g.nx;
+  const long st2 = t2 == 1 ? (long)S.g.nx : (long)S.g.nx * S.g.ny;
+  const int lo2 = DIM == 3 ? it2 - 1 : 0, hi2 = DIM == 3 ? it2 : 0;
+  // cell at position a along D and (p1, p2) along the transverse axes
+  const long cbase = D == 0 ? (long)a : D == 1 ? (long)a * S.g.nx : (long)a * S.g.nx * S.g.ny;
+  for (int p2 = lo2; p2 <= hi2; ++p2) {
+    if (DIM == 3 && (p2 < 0 || p2 >= n2)) continue;
+    for (int p1 = it1 - 1; p1 <= it1; ++p1) {
+      if (p1 < 0 || p1 >= n1) continue;
+      const long cf = cbase + p1 * st1 + (DIM == 3 ? p2 * st2 : 0);
+      const long cb = cf - sD;
+      for (int kk = 0; kk < M; ++kk) back += w * dflux[kk * nc + cb];
+      for (int kk = 0; kk < M; ++kk) front += w * dflux[kk * nc + cf];
+    }
+  }
+  U[NC] += back - front;
+  const double o2 = (double)S.order2;
+  const double ufront = 0.5 * (P[2][D] + P[3][D]);
+  const int sf = ufront > 0 ? 1 : -1;
+  const int fo = (front < 0.) ? 1 : 0;
+  const double uback = 0.5 * (P[1][D] + P[2][D]);
+  const int sb = uback > 0 ? 1 : -1;
+  const int bo = (back > 0.) ? -1 : 0;
+  for (int cc = 0; cc < NC; ++cc) {
+    const double FU = P[fo + 2][cc] + o2 * (0.5 * G[fo + 1][cc] * (sf * (X[3] - X[2]) - dt * ufront));
+    const double BU = P[bo + 2][cc] + o2 * (0.5 * G[bo + 1][cc] * (sb * (X[2] - X[1]) - dt * uback));
+    U[cc] += back * BU - front * FU;
+  }
+  for (int cc = 0; cc < NC; ++cc)
+    if (fabs(U[cc]) < 1.e-300) U[cc] = 0.;
+  for (int cc = 0; cc <= NC; ++cc) Ud[cc * nn + n] = U[cc];
+  for (int cc = 0; cc < NC; ++cc) {
+    const double ph = U[cc] / U[NC];
+    Pout[cc * nn + n] = ph;
+    if (last && U[NC] != 0.) S.u[cc][n] = ph;
+  }
+}
+
+// u = mu / m for every node with a non-zero projected mass (RemapADIFinal.cc:286-297), for
+// cycles whose remap has a single sweep (the sweep kernel still reads u) or none.
+template <int DIM>
+__global__ __launch_bounds__(MH_BLOCK) void k_node_velocity(MhDev S, const double* __restrict__ Ud,
+                                                            int materialize) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= S.g.nn) return;
+  if (mh_halted(S.sc)) return;
+  const long nn = S.g.nn;
+  double U[DIM + 1];
+  if (materialize) {
+    const double mass = S.mn[n];
+    for (int cc = 0; cc < DIM; ++cc) U[cc] = mass * S.u[cc][n];
+    U[DIM] = mass;
+  } else {
+    for (int cc = 0; cc <= DIM; ++cc) U[cc] = Ud[cc * nn + n];
+  }
+  if (U[DIM] != 0.)
+    for (int cc = 0; cc < DIM; ++cc) S.u[cc][n] = U[cc] / U[DIM];
+}
+
+// ----------------------------------------------------------------------------- finalize
+// remapVariables (RemapADIFinal.cc:21-316), the remap tail (PrepareRemap.cc:220-249:
+// applyEOS per environment, computePressionMoyenne) and the per-cell CFL candidate of
+// computeDeltaT (MahycoModule.cc:1763-1779).  Pure cells write through the global arrays
+// only (their environment value is the global value).
+template <int DIM, int M>
+__global__ __launch_bounds__(MH_BLOCK) void k_finalize(MhDev S, const double* __restrict__ Uin) {
+  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const bool active = c < S.g.nc && !mh_halted(S.sc);
+  double dxs = 1.79769313486231570815e+308;
+  if (active) {
+    const long nc = S.g.nc;
+    const double thr = S.thr_remap;
+    double U[3 * M + 1];
+    for (int v = 0; v <= 3 * M; ++v) U[v] = Uin[v * nc + c];
+    const double VE = S.VE[c];
+    // environment membership (:29-78)
+    const uint8_t om = S.mask[c];
+    uint8_t nm = om;
+    for (int kk = 0; kk < M; ++kk) {
+      const bool was = (om >> kk) & 1;
+      if (((U[kk] / VE) < thr || (U[M + kk] == 0.)) && was) {
+        nm &= (uint8_t)~(1u << kk);
+      } else if ((U[kk] / VE) > thr && !was) {
+        double volt = 0.;
+        for (int k2 = 0; k2 < M; ++k2) volt += U[k2];
+        const double vol_ev = U[kk] * VE / volt;
+        if ((vol_ev / VE) > thr && U[M + kk] != 0.) nm |= (uint8_t)(1u << kk);
+      }
+    }
+    const bool pure = __popc(nm) == 1;
+    // (:85-216)
+    const double vol = VE;
+    double volt = 0., masset = 0., vn1[M];
+    for (int kk = 0; kk < M; ++kk) {
+      vn1[kk] = U[kk];
+      volt += vn1[kk];
+      masset += U[M + kk];
+    }
+    const double unsurvolt = 1. / volt;
+    for (int kk = 0; kk < M; ++kk) vn1[kk] *= vol * unsurvolt;
+    double f[M], chi[M], dk[M], ek[M], pk[M], ck[M];
+    double somme_frac = 0.;
+    const double unsurvol = 1. / vol;
+    for (int kk = 0; kk < M; ++kk) {
+      f[kk] = 0.;
+      if (!((nm >> kk) & 1)) continue;
+      f[kk] = vn1[kk] * unsurvol;
+      if (f[kk] < thr) f[kk] = 0.;
+      somme_frac += f[kk];
+    }
+    double unsursomme = 0.;
+    if (somme_frac != 0.) unsursomme = 1. / somme_frac;
+    for (int kk = 0; kk < M; ++kk)
+      if ((nm >> kk) & 1) f[kk] *= unsursomme;
+    if (masset > thr * thr) {
+      const double unsurmasset = 1. / masset;
+      double fm = 0.;
+      for (int kk = 0; kk < M; ++kk) {
+        chi[kk] = 0.;
+        if (!((nm >> kk) & 1)) continue;
+        chi[kk] = U[M + kk] * unsurmasset;
+        if (f[kk] < thr) chi[kk] = 0.;
+        fm += chi[kk];
+      }
+      if (fm != 0.) {
+        const double unsurfm = 1. / fm;
+        for (int kk = 0; kk < M; ++kk)
+          if ((nm >> kk) & 1) chi[kk] *= unsurfm;
+      }
+    } else {
+      // the reference keeps the previous mass fractions of such (vacuum) cells; they are
+      // not stored for pure cells here, so the case is reported
+      S.err[0] = MH_ERR_UNSUPPORTED;
+      for (int kk = 0; kk < M; ++kk) chi[kk] = 0.;
+    }
+    double density = 0.;
+    for (int kk = 0; kk < M; ++kk) {
+      dk[kk] = 0.;
+      if (!((nm >> kk) & 1)) continue;
+      if (f[kk] > thr && vn1[kk] != 0.) dk[kk] = U[M + kk] / vn1[kk];
+      density += f[kk] * dk[kk];
+    }
+    for (int kk = 0; kk < M; ++kk) {
+      ek[kk] = 0.;
+      if (!((nm >> kk) & 1)) continue;
+      if (f[kk] > thr && U[M + kk] != 0.) ek[kk] = U[2 * M + kk] / U[M + kk];
+    }
+    // writes, with the aliasing of single-environment cells (:221-253)
+    double rho_g = density;
+    double M_g = VE * density;
+    double e_g, q_g;
+    double Mk[M], qk[M];
+    double energie = 0., pseudo = 0.;
+    for (int kk = 0; kk < M; ++kk) {
+      if (!((nm >> kk) & 1)) continue;
+      Mk[kk] = chi[kk] * M_g;
+      if (pure) M_g = Mk[kk];
+      qk[kk] = U[3 * M] * unsurvol;
+      if (pure) rho_g = dk[kk];
+      energie += chi[kk] * ek[kk];
+      pseudo += f[kk] * qk[kk];
+    }
+    e_g = energie;
+    q_g = pseudo;
+    for (int kk = 0; kk < M; ++kk) {
+      if (!((nm >> kk) & 1)) continue;
+      if (pure) ek[kk] = e_g;  // the cell value overwrites the aliased environment value
+      if (dk[kk] < 0. || ek[kk] < 0.) {
+        ek[kk] = 0.;
+        dk[kk] = 0.;
+        f[kk] = 0.;
+        chi[kk] = 0.;
+        if (pure) {
+          e_g = 0.;
+          rho_g = 0.;
+        }
+      }
+      if (dk[kk] != dk[kk] || ek[kk] != ek[kk]) S.err[0] = MH_ERR_NAN_REMAP;
+    }
+    // applyEOS per environment, computePressionMoyenne (MahycoModule.cc:1708-1735)
+    double p_g = 0., c_g = 0.;
+    const double opt = S.vitson_max ? 1. : 0.;
+    for (int kk = 0; kk < M; ++kk) {
+      if (!((nm >> kk) & 1)) continue;
+      eos(S.env[kk], pure ? rho_g : dk[kk], ek[kk], pk[kk], ck[kk]);
+      if (pure) {
+        p_g = pk[kk];
+        c_g = ck[kk];
+      }
+    }
+    if (!pure) {
+      p_g = 0.;
+      c_g = 0.;
+      for (int kk = 0; kk < M; ++kk) {
+        if (!((nm >> kk) & 1)) continue;
+        p_g += f[kk] * pk[kk];
+        c_g = opt * ((ck[kk] < c_g) ? c_g : ck[kk]);
+        c_g += (1. - opt) * f[kk] * ck[kk];
+      }
+      for (int kk = 0; kk < M; ++kk) {
+        if (!((nm >> kk) & 1)) continue;
+        const long o = kk * nc + c;
+        S.fk[o] = f[kk];
+        S.chik[o] = chi[kk];
+        S.rhok[o] = dk[kk];
+        S.ek[o] = ek[kk];
+        S.pk[o] = pk[kk];
+        S.Mk[o] = Mk[kk];
+      }
+    }
+    double fg = 0.;
+    for (int kk = 0; kk < M; ++kk)
+      if ((nm >> kk) & 1) fg = f[kk];
+    S.mask[c] = nm;
+    S.rho[c] = rho_g;
+    S.e[c] = e_g;
+    S.p[c] = p_g;
+    S.c[c] = c_g;
+    S.q[c] = q_g;
+    S.M[c] = M_g;
+    S.fg[c] = fg;
+    const int nx = S.g.nx, ny = S.g.ny;
+    const int i = (int)(c % nx);
+    const long r = c / nx;
+    const int j = (int)(r % ny);
+    const int k = (int)(r / ny);
+    const long b = i + (long)S.g.nnx * (j + (long)S.g.nny * k);
+    dxs = S.L[c] / (c_g + cell_vmax<DIM>(S, b));
+  }
+  block_min_store(dxs, S.partial);
+}
+
+// computeDeltaT (MahycoModule.cc:1739-1864) on the block minima, plus the time increment
+// of the iteration and the final-time stop rule.  One block.
+__global__ __launch_bounds__(MH_DT_BLOCK) void k_dt(MhDev S, int nblocks) {
+  __shared__ double red[MH_DT_BLOCK / 64];
+  MhScalars* sc = S.sc;
+  if (mh_halted(sc)) return;
+  double v = 1.79769313486231570815e+308;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) v = amin(v, S.partial[b]);
+  for (int off = 32; off > 0; off >>= 1) v = amin(v, __shfl_xor(v, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double mn = red[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mn = amin(mn, red[w]);
+  const double dt = sc->dt;
+  const double t = sc->t + dt;
+  double new_dt = S.cfl * mn;
+  new_dt = amin(new_dt, 1.05 * dt);
+  new_dt = amin(new_dt, S.dtmax);
+  int status = S.err[0];
+  if (status == 0 && new_dt < S.dtmin) status = MH_ERR_DT_MIN;
+  sc->t = t;
+  sc->iteration = sc->iteration + 1;
+  sc->dxmin = mn;
+  if (status != 0) {
+    sc->status = status;
+    return;
+  }
+  sc->dt_old = dt;
+  sc->dt = new_dt;
+  const double T = S.tfinal;
+  if ((t < T && t + new_dt > T) || t > T) sc->stopped = 1;
+}
+
+#endif

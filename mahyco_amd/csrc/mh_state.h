@@ -1,0 +1,75 @@
+// mh_state.h -- device-resident state of one structured-mesh case and its HBM layout.
+//
+// Cells are numbered i + nx (j + ny k) and nodes i + (nx+1)(j + (ny+1) k), x fastest,
+// like the Arcane Cartesian generator.  Every array is structure-of-arrays FP64 so that a
+// wavefront of 64 consecutive cells (or nodes) reads 512 contiguous bytes per field.
+//
+// Material storage follows Arcane's split between global and partial values: the
+// global arrays hold the value of every cell; a cell with exactly one environment has no
+// other storage (its environment value is the global value).  The per-environment
+// arrays env[k][cell] are meaningful only for cells with two or more environments;
+// kernels read them only for such cells, so a mostly pure mesh streams the global
+// arrays alone.
+#ifndef MH_STATE_H
+#define MH_STATE_H
+
+#include <cstdint>
+
+#include "../../include/mahyco_types.h"
+
+struct MhGrid {
+  int dim, nx, ny, nz;  // nz = 1 in 2D
+  int nnx, nny, nnz;    // node counts (nnz = 1 in 2D)
+  long nc, nn;
+};
+
+struct MhEnvParam {
+  double gamma, pinf, a1, a2;
+  int stiffened;
+  int pad;
+};
+
+// Time-loop scalars kept on the device so that cycles run without host round trips.
+struct MhScalars {
+  double t;       // m_global_time of the last completed iteration
+  double dt;      // m_global_deltat used by the next iteration
+  double dt_old;  // m_global_old_deltat seen by the next iteration (0 before the first)
+  double dxmin;   // last CFL minimum (diagnostics)
+  int iteration;
+  int stopped;  // final time reached: later iterations are no-ops
+  int status;   // 0 or a negative MhStatus: later iterations are no-ops
+  int pad;
+};
+
+struct MhDev {
+  MhGrid g;
+  int m;   // environments
+  int nv;  // remap slots: V_k, M_k, E_k per env and V*q
+  MhEnvParam env[MH_MAX_ENV];
+  double cfl, thr_remap, dtmin, dtmax, tfinal;
+  int order2, limiter, with_proj, vitson_max;
+  int nbc;
+  int bc_surf[MH_MAX_BC];
+  int bc_type[MH_MAX_BC];
+  double bc_val[MH_MAX_BC];
+  // Eulerian grid: node coordinate along each axis (0 bytes per node to rebuild x0)
+  const double* ax;
+  const double* ay;
+  const double* az;
+  const double* VE;  // Euler volume per cell (m_euler_volume)
+  // node state
+  double* u[3];    // velocity
+  double* xlp[3];  // Lagrangian coordinates of the previous cycle (CQS_n, MahycoModule.cc:1097)
+  double* xlc[3];  // Lagrangian coordinates of this cycle
+  double* mn;      // node mass used this cycle
+  // cell state
+  double *rho, *p, *c, *q, *M, *e, *fg;
+  uint8_t* mask;                              // environment membership bits
+  double *fk, *chik, *rhok, *ek, *pk, *Mk;  // [m][nc], mixed cells only
+  double *L, *Vl;                             // characteristic length, Lagrangian volume
+  double* partial;                            // per-block CFL minima
+  int* err;                                   // error word raised inside a cycle
+  MhScalars* sc;
+};
+
+#endif

@@ -1,0 +1,139 @@
+"""ctypes binding of the HIP library (include/mahyco.h).
+
+The library is built in-tree by mahyco_amd/build.py.  Loading fails loudly when it is
+missing: there is no CPU fallback for the hydro path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import types as T
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libmahyco_hip.so")
+_lib = None
+
+
+class MahycoError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        super().__init__(f"{what}: {T.STATUS_NAMES.get(status, status)}")
+
+
+def load():
+    """Loads libmahyco_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP library not built: {LIB_PATH} (run python -m mahyco_amd.build)")
+    lib = C.CDLL(LIB_PATH)
+    H = C.c_void_p
+    lib.mahyco_create.argtypes = [C.POINTER(T.MhMesh), C.c_void_p, C.c_int, C.POINTER(T.MhOptions),
+                                  C.POINTER(C.c_void_p)]
+    lib.mahyco_create.restype = C.c_int
+    lib.mahyco_destroy.argtypes = [H]
+    lib.mahyco_destroy.restype = None
+    for name in ("mahyco_cycle",):
+        getattr(lib, name).argtypes = [H, C.c_int]
+        getattr(lib, name).restype = C.c_int
+    lib.mahyco_cycle_graph.argtypes = [H, C.c_int, C.c_int]
+    lib.mahyco_cycle_graph.restype = C.c_int
+    for name in ("mahyco_synchronize", "mahyco_status", "mahyco_iteration",
+                 "mahyco_save_values_at_n", "mahyco_update_velocity", "mahyco_apply_boundary_condition",
+                 "mahyco_update_position", "mahyco_compute_geometric_values", "mahyco_update_density",
+                 "mahyco_compute_artificial_viscosity", "mahyco_update_elasticity_and_plasticity",
+                 "mahyco_update_energy_and_pressure", "mahyco_remap", "mahyco_compute_delta_t"):
+        getattr(lib, name).argtypes = [H]
+        getattr(lib, name).restype = C.c_int
+    for name in ("mahyco_time", "mahyco_deltat"):
+        getattr(lib, name).argtypes = [H]
+        getattr(lib, name).restype = C.c_double
+    for name in ("mahyco_nb_cells", "mahyco_nb_nodes"):
+        getattr(lib, name).argtypes = [H]
+        getattr(lib, name).restype = C.c_long
+    lib.mahyco_get_field.argtypes = [H, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    lib.mahyco_get_field.restype = C.c_int
+    lib.mahyco_profile_enable.argtypes = [H, C.c_int]
+    lib.mahyco_profile_enable.restype = C.c_int
+    lib.mahyco_profile_read.argtypes = [H, C.c_char_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+    lib.mahyco_profile_read.restype = C.c_int
+    _lib = lib
+    return lib
+
+
+class GpuCase:
+    """One case resident on the GPU (the MahycoModule state of one mesh)."""
+
+    def __init__(self, case):
+        self.lib = load()
+        self.case = case
+        self._mesh, self._envs, self._opt = case.mesh_c(), case.envs_c(), case.options_c()
+        h = C.c_void_p()
+        st = self.lib.mahyco_create(C.byref(self._mesh), self._envs, len(case.envs), C.byref(self._opt),
+                                    C.byref(h))
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_create")
+        self.h = h
+        self.nc = self.lib.mahyco_nb_cells(h)
+        self.nn = self.lib.mahyco_nb_nodes(h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mahyco_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, ncycles, graph=0):
+        """Queues ncycles iterations (graph > 0: replay hipGraphs of that many cycles)."""
+        st = (self.lib.mahyco_cycle_graph(self.h, ncycles, graph) if graph
+              else self.lib.mahyco_cycle(self.h, ncycles))
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_cycle")
+        return self.synchronize()
+
+    def queue(self, ncycles, graph=0):
+        st = (self.lib.mahyco_cycle_graph(self.h, ncycles, graph) if graph
+              else self.lib.mahyco_cycle(self.h, ncycles))
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_cycle")
+
+    def synchronize(self):
+        return self.lib.mahyco_synchronize(self.h)
+
+    @property
+    def time(self):
+        return self.lib.mahyco_time(self.h)
+
+    @property
+    def deltat(self):
+        return self.lib.mahyco_deltat(self.h)
+
+    @property
+    def iteration(self):
+        return self.lib.mahyco_iteration(self.h)
+
+    def get(self, field, env=-1, comp=0):
+        n = self.nn if field >= 100 else self.nc
+        out = np.empty(n, dtype=np.float64)
+        st = self.lib.mahyco_get_field(self.h, field, env, comp, out.ctypes.data)
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_get_field")
+        return out
+
+    def profile(self, on=True):
+        self.lib.mahyco_profile_enable(self.h, int(on))
+
+    def profile_read(self):
+        names = C.create_string_buffer(1024)
+        ms = np.zeros(16)
+        cnt = np.zeros(16, dtype=np.int32)
+        n = self.lib.mahyco_profile_read(self.h, names, 1024, ms.ctypes.data, cnt.ctypes.data, 16)
+        lst = names.raw.split(b"\0")[:n]
+        return {lst[i].decode(): (float(ms[i]), int(cnt[i])) for i in range(n)}

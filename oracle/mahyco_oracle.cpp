@@ -32,6 +32,7 @@
 // Endommagement (inert at its default thresholds -1e12 / 0).
 
 #include "mahyco_oracle.h"
+#include "../mahyco_amd/csrc/mh_math.h"
 
 #include <algorithm>
 #include <cmath>
@@ -43,6 +44,10 @@
 #endif
 
 namespace {
+
+// 0: L = std::pow(V, 1/dim) as the reference (glibc); 1: the correctly rounded power of
+// mh_math.h, which the HIP kernels use (glibc misrounds ~0.08% of arguments by one ulp).
+int g_pow_mode = 0;
 
 struct R3 {
   double x, y, z;
@@ -747,7 +752,8 @@ int OracleCase::computeGeometricValues() {
     volume /= dim;
     cell_volume.g[c] = volume;
     if (volume < 0.) err = 1;
-    caracteristic_length[c] = std::pow(cell_volume.g[c], racine);
+    caracteristic_length[c] = g_pow_mode == 0 ? std::pow(cell_volume.g[c], racine)
+                              : (dim == 3 ? mh_pow_third(cell_volume.g[c]) : mh_pow_half(cell_volume.g[c]));
   }
   // mixed cells: V_k = f_k V
   for (long c = 0; c < nc; ++c) {
@@ -1486,6 +1492,8 @@ int oracle_history(const OracleCase* h, double* out, int max_rows) {
   if (out) std::memcpy(out, h->history.data(), sizeof(double) * (size_t)n * w);
   return rows;
 }
+
+void oracle_set_pow_mode(int mode) { g_pow_mode = mode; }
 
 void oracle_set_threads(int n) {
 #ifdef _OPENMP

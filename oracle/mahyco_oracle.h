@@ -42,6 +42,10 @@ int oracle_get_field(const OracleCase* h, int field, int env, int comp, double* 
  * InternalEnergy mass-avg, nb_envcells]. Returns number of rows; fills up to max_rows. */
 int oracle_history(const OracleCase* h, double* out, int max_rows);
 
+/* 0 (default): characteristic length with std::pow as the reference; 1: with the correctly
+ * rounded power used by the HIP kernels (for bit-exact comparisons). */
+void oracle_set_pow_mode(int mode);
+
 /* Number of OpenMP threads used by the oracle loops (0 = runtime default). */
 void oracle_set_threads(int n);
 

@@ -1,0 +1,105 @@
+"""The CPU oracle against the reference's own non-regression outputs (tests/golden).
+
+Tolerances (the fixtures hold 17-digit time histories and %12.5e EnSight fields):
+  * final time and every time-history time stamp: 1e-13 relative (pins the whole
+    dt sequence: CFL, 1.05 growth, deltat-max, first-step rules);
+  * EnvSummation averages: 1e-12 relative (summation order of the averages is the
+    reference's parallel one for para_4 cases);
+  * final per-cell / per-node EnSight fields: within the printed precision,
+    |d| <= 5e-6 |ref| + 1e-12 after rounding the oracle value to %12.5e.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from mahyco_amd import cases
+from mahyco_amd import types as T
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+FIXTURES = {
+    "CAS_BiSodCaseX": cases.bisod_x,
+    "CAS_BiSodCaseY": cases.bisod_y,
+    "CAS_BiSodCaseZ": cases.bisod_z,
+    "CAS_SodCaseY": lambda: cases.sod_mono("Y"),
+    "CAS_SodCaseZ": lambda: cases.sod_mono("Z"),
+    "CAS_Lag_BiSodCaseX": cases.lag_bisod_x,
+    "CAS_2D_Lag_SodCaseX": cases.lag2d_sod_x,
+    "CAS_2D_Sedov": cases.sedov_2d,
+    "CAS_bitriplepoint_minmod": cases.bitriplepoint,
+}
+TH = ["CellMass", "Pressure", "Density", "InternalEnergy"]
+ENSIGHT = {"Density": T.MH_F_DENSITY, "Pressure": T.MH_F_PRESSURE, "InternalEnergy": T.MH_F_INTERNAL_ENERGY,
+           "CellMass": T.MH_F_CELL_MASS, "PseudoViscosity": T.MH_F_PSEUDO_VISCOSITY, "Materiau": T.MH_F_MATERIAU}
+
+
+def _printed(x):
+    return np.array([float("%12.5e" % v) for v in x])
+
+
+@pytest.fixture(scope="module")
+def runs():
+    from oracle.pyoracle import OracleCase
+    return {}
+
+
+def _run(name, store):
+    from oracle.pyoracle import OracleCase
+    if name not in store:
+        o = OracleCase(FIXTURES[name]())
+        st = o.run(10 ** 7)
+        store[name] = (o, st)
+    return store[name]
+
+
+@pytest.mark.parametrize("name", list(FIXTURES))
+def test_time_history(name, runs):
+    o, st = _run(name, runs)
+    ref = np.load(os.path.join(GOLDEN, name + ".npz"))
+    assert st == T.MH_STOPPED
+    ft = float(ref["final_time"])
+    assert abs(o.time - ft) <= 1e-13 * ft, (o.time, ft)
+    th = o.history()
+    t_ref = ref["th_time"]
+    assert th.shape[0] == t_ref.shape[0]
+    np.testing.assert_allclose(th[:, 0], t_ref, rtol=1e-13, atol=0)
+    for k in range(len(ref["env_names"])):
+        for vi, v in enumerate(TH):
+            key = f"th_{v}_{k}"
+            if key not in ref:
+                continue
+            np.testing.assert_allclose(th[:, 1 + 5 * k + vi], ref[key], rtol=1e-12, atol=0, err_msg=key)
+
+
+@pytest.mark.parametrize("name", list(FIXTURES))
+def test_ensight_fields(name, runs):
+    o, st = _run(name, runs)
+    ref = np.load(os.path.join(GOLDEN, name + ".npz"))
+    for var, fid in ENSIGHT.items():
+        r = ref["cell_" + var]
+        d = np.abs(_printed(o.get(fid)) - r)
+        assert np.all(d <= 5e-6 * np.abs(r) + 1e-12), (var, float(d.max()))
+    r = ref["node_NodeMass"]
+    d = np.abs(_printed(o.get(T.MH_N_NODE_MASS)) - r)
+    assert np.all(d <= 5e-6 * np.abs(r) + 1e-12)
+    for c in range(3 if FIXTURES[name]().dim == 3 else 2):
+        r = ref[f"node_Velocity_{c}"]
+        d = np.abs(_printed(o.get(T.MH_N_VELOCITY, -1, c)) - r)
+        assert np.all(d <= 5e-6 * np.abs(r) + 1e-12), (c, float(d.max()))
+
+
+def test_lagrange_fixtures_bit_exact(runs):
+    """The Lagrange-only fixtures (sequential runs) are reproduced to the last printed bit:
+    every 17-digit time-history value is equal."""
+    for name in ("CAS_Lag_BiSodCaseX", "CAS_2D_Lag_SodCaseX"):
+        o, st = _run(name, runs)
+        ref = np.load(os.path.join(GOLDEN, name + ".npz"))
+        th = o.history()
+        assert np.array_equal(th[:, 0], ref["th_time"])
+        for k in range(len(ref["env_names"])):
+            for vi, v in enumerate(TH):
+                key = f"th_{v}_{k}"
+                if key in ref:
+                    got = np.array([float("%.16e" % x) for x in th[:, 1 + 5 * k + vi]])
+                    assert np.array_equal(got, ref[key]), key

@@ -84,6 +84,14 @@ int mahyco_update_energy_and_pressure(MhHandle* h);
 int mahyco_remap(MhHandle* h);
 int mahyco_compute_delta_t(MhHandle* h);
 
+/* IEquationOfState::applyEOS(env) (IEquationOfState.h:45-49; PerfectGasEOSService.cc:45-67,
+ * StiffenedGasEOSService.cc:43-63): pressure and sound speed of environment env from its
+ * density and internal energy.  Also run inside mahyco_remap / the Lagrange-only cycle. */
+int mahyco_apply_eos(MhHandle* h, int env);
+/* MahycoModule::computePressionMoyenne (MahycoModule.cc:1708-1735): cell pressure and sound
+ * speed of multi-environment cells.  Also run inside mahyco_remap / the Lagrange-only cycle. */
+int mahyco_compute_pression_moyenne(MhHandle* h);
+
 /* Kernel timing: when enabled, every launch is bracketed by HIP events on the
  * handle's stream.  mahyco_profile_read fills up to `max` entries of kernel
  * names (NUL-separated into `names`, `names_len` bytes), total milliseconds and

@@ -34,7 +34,23 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+RUNNER = os.path.join(OUT_DIR, "mahyco_run")
+
+
+def build_host(verbose: bool = False) -> str:
+    """C++ host interface (MahycoModule / IRemap / IEquationOfState) and its driver."""
+    host = os.path.join(HERE, "host")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+           os.path.join(host, "mahyco_module.cpp"), os.path.join(host, "main.cpp"),
+           "-L", OUT_DIR, "-lmahyco_hip", "-Wl,-rpath,$ORIGIN", "-o", RUNNER]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return RUNNER
 
 
 if __name__ == "__main__":

@@ -460,6 +460,15 @@ int validate(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const Mh
 
 }  // namespace
 
+template <int M>
+static void launch_eos_t(MhHandle* h, int env) {
+  hipLaunchKernelGGL(k_apply_eos<M>, dim3(h->nblk_c), dim3(MH_BLOCK), 0, h->stream, h->dev, env);
+}
+template <int M>
+static void launch_pm_t(MhHandle* h) {
+  hipLaunchKernelGGL(k_pression_moyenne<M>, dim3(h->nblk_c), dim3(MH_BLOCK), 0, h->stream, h->dev);
+}
+
 // ============================================================================ C ABI
 extern "C" {
 
@@ -879,6 +888,28 @@ int mahyco_compute_delta_t(MhHandle* h) {
   int st = step_phase(h, expect, 0);
   if (st == MH_OK) phase_dt(h);
   return st;
+}
+
+int mahyco_apply_eos(MhHandle* h, int env) {
+  if (!h || env < 0 || env >= h->m) return MH_ERR_INVALID;
+  switch (h->m) {
+    case 1: launch_eos_t<1>(h, env); break;
+    case 2: launch_eos_t<2>(h, env); break;
+    case 3: launch_eos_t<3>(h, env); break;
+    default: launch_eos_t<4>(h, env); break;
+  }
+  return hip_check(hipGetLastError(), "k_apply_eos");
+}
+
+int mahyco_compute_pression_moyenne(MhHandle* h) {
+  if (!h) return MH_ERR_INVALID;
+  switch (h->m) {
+    case 1: launch_pm_t<1>(h); break;
+    case 2: launch_pm_t<2>(h); break;
+    case 3: launch_pm_t<3>(h); break;
+    default: launch_pm_t<4>(h); break;
+  }
+  return hip_check(hipGetLastError(), "k_pression_moyenne");
 }
 
 int mahyco_profile_enable(MhHandle* h, int on) {

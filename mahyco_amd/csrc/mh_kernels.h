@@ -849,6 +849,50 @@ __global__ __launch_bounds__(MH_BLOCK) void k_finalize(MhDev S, const double* __
   block_min_store(dxs, S.partial);
 }
 
+// applyEOS of one environment on its member cells (PerfectGasEOSService.cc:45-67,
+// StiffenedGasEOSService.cc:43-63), as a standalone call.
+template <int M>
+__global__ __launch_bounds__(MH_BLOCK) void k_apply_eos(MhDev S, int env) {
+  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (c >= S.g.nc) return;
+  const uint8_t mk = S.mask[c];
+  if (!((mk >> env) & 1)) return;
+  if (__popc(mk) == 1) {
+    double p, cs;
+    eos(S.env[env], S.rho[c], S.e[c], p, cs);
+    S.p[c] = p;
+    S.c[c] = cs;
+  } else {
+    const long o = env * S.g.nc + c;
+    double p, cs;
+    eos(S.env[env], S.rhok[o], S.ek[o], p, cs);
+    S.pk[o] = p;
+  }
+}
+
+// computePressionMoyenne (MahycoModule.cc:1708-1735) on multi-environment cells; the
+// environment sound speeds are those of applyEOS on the stored density and energy.
+template <int M>
+__global__ __launch_bounds__(MH_BLOCK) void k_pression_moyenne(MhDev S) {
+  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (c >= S.g.nc) return;
+  const uint8_t mk = S.mask[c];
+  if (__popc(mk) == 1) return;
+  const double opt = S.vitson_max ? 1. : 0.;
+  double ps = 0., cm = 0.;
+  for (int kk = 0; kk < M; ++kk) {
+    if (!((mk >> kk) & 1)) continue;
+    const long o = kk * S.g.nc + c;
+    double p, cs;
+    eos(S.env[kk], S.rhok[o], S.ek[o], p, cs);
+    ps += S.fk[o] * S.pk[o];
+    cm = opt * ((cs < cm) ? cm : cs);
+    cm += (1. - opt) * S.fk[o] * cs;
+  }
+  S.p[c] = ps;
+  S.c[c] = cm;
+}
+
 // computeDeltaT (MahycoModule.cc:1739-1864) on the block minima, plus the time increment
 // of the iteration and the final-time stop rule.  One block.
 __global__ __launch_bounds__(MH_DT_BLOCK) void k_dt(MhDev S, int nblocks) {

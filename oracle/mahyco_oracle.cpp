@@ -1495,6 +1495,8 @@ int oracle_history(const OracleCase* h, double* out, int max_rows) {
 
 void oracle_set_pow_mode(int mode) { g_pow_mode = mode; }
 
+double oracle_flux_limiter(int id, double r) { return fluxLimiter(id, r); }
+
 void oracle_set_threads(int n) {
 #ifdef _OPENMP
   if (n > 0) omp_set_num_threads(n);

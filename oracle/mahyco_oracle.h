@@ -46,6 +46,9 @@ int oracle_history(const OracleCase* h, double* out, int max_rows);
  * rounded power used by the HIP kernels (for bit-exact comparisons). */
 void oracle_set_pow_mode(int mode);
 
+/* RemapADIService::fluxLimiter (UtilesRemap.cc:108-120) as restated by the oracle. */
+double oracle_flux_limiter(int id, double r);
+
 /* Number of OpenMP threads used by the oracle loops (0 = runtime default). */
 void oracle_set_threads(int n);
 

@@ -1,0 +1,120 @@
+"""The GPU path through its different doors gives identical results (GPU tests):
+per-entry-point calls vs fused cycles vs hipGraph replays, repeated runs, the C++ host
+interface (MahycoModule over the C ABI), and full-size properties of the 256x128x128
+bi-material case that do not need the oracle (mass, planar symmetry, positivity)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from mahyco_amd import cases
+from mahyco_amd import types as T
+from tests.helpers import bitwise_equal, fields
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _entry_point_cycle(g):
+    lib, h = g.lib, g.h
+    for fn in ("mahyco_save_values_at_n", "mahyco_update_velocity", "mahyco_apply_boundary_condition",
+               "mahyco_update_position", "mahyco_compute_geometric_values", "mahyco_update_density",
+               "mahyco_compute_artificial_viscosity", "mahyco_update_elasticity_and_plasticity",
+               "mahyco_update_energy_and_pressure"):
+        assert getattr(lib, fn)(h) == T.MH_OK, fn
+    if g.case.with_projection:
+        assert lib.mahyco_remap(h) == T.MH_OK
+    assert lib.mahyco_compute_delta_t(h) == T.MH_OK
+
+
+@pytest.mark.parametrize("make", [lambda: cases.bisod_x(30, 4, 3), cases.lag2d_sod_x,
+                                  lambda: cases.bitriplepoint()])
+def test_entry_points_equal_fused_and_graph(make):
+    from mahyco_amd.lib import GpuCase
+    case = make()
+    print(f"[paths] {case.name}", flush=True)
+    a, b, c = GpuCase(case), GpuCase(case), GpuCase(case)
+    n = 24
+    for _ in range(n):
+        _entry_point_cycle(a)
+    a.synchronize()
+    b.run(n)
+    c.run(n, graph=6)
+    fa, fb, fc = (fields(x, len(case.envs), case.dim) for x in (a, b, c))
+    for k in fa:
+        assert bitwise_equal(fa[k], fb[k]), ("entry points vs fused", k)
+        assert bitwise_equal(fb[k], fc[k]), ("fused vs graph", k)
+    assert a.time == b.time == c.time
+    # out-of-order entry point is refused
+    assert a.lib.mahyco_update_velocity(a.h) == T.MH_ERR_INVALID
+    for x in (a, b, c):
+        x.close()
+
+
+def test_graph_replays_are_stable():
+    """Several replays of the same captured graph, with eager work and synchronisations in
+    between, match an eager run cycle for cycle."""
+    from mahyco_amd.lib import GpuCase
+    case = cases.bisod_x(24, 4, 4)
+    g, e = GpuCase(case), GpuCase(case)
+    for _ in range(4):
+        g.run(6, graph=6)
+        e.run(6)
+        assert g.time == e.time
+        assert bitwise_equal(g.get(T.MH_F_DENSITY), e.get(T.MH_F_DENSITY))
+        g.run(1)
+        e.run(1)
+    g.close()
+    e.close()
+
+
+def test_repeatable():
+    from mahyco_amd.lib import GpuCase
+    case = cases.config2_bisod3d(48, 24, 24)
+    r = []
+    for _ in range(2):
+        g = GpuCase(case)
+        g.run(30, graph=6)
+        r.append(fields(g, 2, 3))
+        g.close()
+    for k in r[0]:
+        assert bitwise_equal(r[0][k], r[1][k]), k
+
+
+def test_cpp_host_interface():
+    """MahycoModule (C++) through the entry points and through fused cycles matches the
+    C ABI, and reproduces the fixture's final time of CAS_BiSodCaseX."""
+    exe = os.path.join(ROOT, "mahyco_amd", "_build", "mahyco_run")
+    assert os.path.exists(exe), "C++ driver not built"
+    out1 = subprocess.run([exe, "bisod_x"], check=True, capture_output=True, text=True, timeout=120).stdout
+    out2 = subprocess.run([exe, "bisod_x", "--fused", "--graph", "6"], check=True, capture_output=True,
+                          text=True, timeout=120).stdout
+    assert out1 == out2, (out1, out2)
+    kv = dict(x.split("=") for x in out1.split())
+    assert int(kv["iteration"]) == 183
+    assert abs(float(kv["time"]) - 0.198843560989769) <= 1e-14
+    bad = subprocess.run([exe, "nosuchcase"], capture_output=True, text=True, timeout=60)
+    assert bad.returncode == 2
+
+
+def test_full_size_bisod_properties():
+    """256x128x128 bi-material tube, 100 iterations: the solution stays planar (every y-z
+    slice equal to the first to round-off), positive, and keeps its mass to 1e-9
+    (threshold drops of remapVariables)."""
+    from mahyco_amd.lib import GpuCase
+    case = cases.config2_bisod3d()
+    g = GpuCase(case)
+    m0 = g.get(T.MH_F_CELL_MASS).sum()
+    st = g.run(100, graph=6)
+    assert st == T.MH_OK
+    assert g.iteration == 100
+    rho = g.get(T.MH_F_DENSITY).reshape(case.cell_shape)
+    p = g.get(T.MH_F_PRESSURE).reshape(case.cell_shape)
+    assert np.all(rho > 0) and np.all(p > 0)
+    dev = np.max(np.abs(rho - rho[:1, :1, :]) / rho[:1, :1, :])
+    assert dev < 1e-9, dev
+    m1 = g.get(T.MH_F_CELL_MASS).sum()
+    assert abs(m1 - m0) <= 1e-9 * m0
+    g.close()

@@ -92,6 +92,28 @@ int mahyco_apply_eos(MhHandle* h, int env);
  * speed of multi-environment cells.  Also run inside mahyco_remap / the Lagrange-only cycle. */
 int mahyco_compute_pression_moyenne(MhHandle* h);
 
+/* ---- ensembles (no reference counterpart: many independent cases on one GPU) ---------
+ * Cases of any size advance together, each with its own time step and stop rule: one
+ * launch per kernel phase covers every active case of a (dimension, environment count)
+ * group.  Admission is by HBM footprint: mahyco_ensemble_add returns MH_ERR_NOMEM when the
+ * case would exceed the budget or no slot is free.  An added case becomes active at the next
+ * iteration multiple of 6 of the ensemble clock (so all active cases share the sweep
+ * rotation).  max_iterations > 0 stops a case after that many iterations.
+ * mahyco_ensemble_poll fills per-slot status (MH_OK running, MH_STOPPED done, 2 pending,
+ * -100 empty, or a negative error), time and iteration; returns the number of active slots. */
+typedef struct MhEnsemble MhEnsemble;
+size_t mahyco_case_footprint(const MhMesh* mesh, int nb_env, const MhOptions* options);
+int mahyco_ensemble_create(int max_slots, double hbm_budget_bytes, MhEnsemble** out);
+void mahyco_ensemble_destroy(MhEnsemble* e);
+int mahyco_ensemble_add(MhEnsemble* e, const MhMesh* mesh, const MhEnvironment* envs, int nb_env,
+                        const MhOptions* options, int max_iterations, int* slot_out);
+int mahyco_ensemble_remove(MhEnsemble* e, int slot);
+int mahyco_ensemble_step(MhEnsemble* e, int ncycles, int use_graph);
+int mahyco_ensemble_synchronize(MhEnsemble* e);
+int mahyco_ensemble_poll(MhEnsemble* e, int* status, double* time, int* iteration);
+int mahyco_ensemble_get_field(MhEnsemble* e, int slot, int field, int env, int comp, double* out);
+double mahyco_ensemble_used_bytes(MhEnsemble* e);
+
 /* Kernel timing: when enabled, every launch is bracketed by HIP events on the
  * handle's stream.  mahyco_profile_read fills up to `max` entries of kernel
  * names (NUL-separated into `names`, `names_len` bytes), total milliseconds and

@@ -27,6 +27,7 @@ struct ProfRec {
 
 }  // namespace
 
+struct MhEnsemble;
 struct MhHandle {
   MhMesh mesh;
   MhOptions opt;
@@ -48,6 +49,9 @@ struct MhHandle {
   int prof_cnt[KN_COUNT] = {0};
   std::map<long, hipGraphExec_t> graphs;  // key: (cycles << 8) | (parity << 4) | sens
   std::vector<hipGraph_t> graph_defs;
+  bool owns_stream = true;
+  MhScalars* ext_sc = nullptr;  // scalars living in an ensemble's array
+  size_t bytes = 0;             // device bytes of the case
 };
 
 namespace {
@@ -75,6 +79,7 @@ int dalloc(MhHandle* h, T** p, size_t n) {
     return MH_ERR_NOMEM;
   }
   h->allocs.push_back(q);
+  h->bytes += n * sizeof(T);
   *p = (T*)q;
   return MH_OK;
 }
@@ -343,7 +348,7 @@ void prof_end(MhHandle* h, ProfRec& r) {
 
 template <int DIM, int M>
 void launch_sweeps_t(MhHandle* h) {
-  MhDev& S = h->dev;
+  const MhDev& S = h->dev;
   const int dim = DIM;
   int dirs[3], nd = 0;
   for (int i = 0; i < dim; ++i) {
@@ -352,28 +357,22 @@ void launch_sweeps_t(MhHandle* h) {
     if (ncell == 1) continue;  // RemapADIService.cc:27
     dirs[nd++] = idir;
   }
-  double* Uin = h->Ua;
-  double* Uout = h->Ub;
-  double* Pin = h->Pb;
-  double* Pout = h->Pa;
   for (int s = 0; s < nd; ++s) {
-    const int first = s == 0, last = (s == nd - 1) && !first;
+    const int first = s == 0, last = (s == nd - 1) && !first, sel = s & 1;
     const int D = dirs[s];
     if (D == 0) {
-      LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, Uin, Uout, h->dflux, first);
-      LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0>), h->nblk_n, MH_BLOCK, S, h->Ud, Pin, Pout, h->dflux, first, last);
+      LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+      LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0>), h->nblk_n, MH_BLOCK, S, sel, first, last);
     } else if (D == 1) {
-      LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, Uin, Uout, h->dflux, first);
-      LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 1>), h->nblk_n, MH_BLOCK, S, h->Ud, Pin, Pout, h->dflux, first, last);
+      LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+      LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 1>), h->nblk_n, MH_BLOCK, S, sel, first, last);
     } else {
-      LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, (DIM == 3 ? 2 : 0), M>), h->nblk_c, MH_BLOCK, S, Uin, Uout, h->dflux, first);
-      LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, (DIM == 3 ? 2 : 0)>), h->nblk_n, MH_BLOCK, S, h->Ud, Pin, Pout, h->dflux, first, last);
+      LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, (DIM == 3 ? 2 : 0), M>), h->nblk_c, MH_BLOCK, S, sel, first);
+      LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, (DIM == 3 ? 2 : 0)>), h->nblk_n, MH_BLOCK, S, sel, first, last);
     }
-    std::swap(Uin, Uout);
-    std::swap(Pin, Pout);
   }
-  if (nd <= 1) LAUNCH(h, KN_NODE_VEL, (k_node_velocity<DIM>), h->nblk_n, MH_BLOCK, S, (const double*)h->Ud, nd == 0 ? 1 : 0);
-  LAUNCH(h, KN_FINAL, (k_finalize<DIM, M>), h->nblk_c, MH_BLOCK, S, (const double*)Uin);
+  if (nd <= 1) LAUNCH(h, KN_NODE_VEL, (k_node_velocity<DIM>), h->nblk_n, MH_BLOCK, S, nd == 0 ? 1 : 0);
+  LAUNCH(h, KN_FINAL, (k_finalize<DIM, M>), h->nblk_c, MH_BLOCK, S, nd & 1);
   h->sens = (h->sens + 1) % dim;
 }
 
@@ -391,7 +390,7 @@ void launch_node_t(MhHandle* h) {
 }
 template <int DIM, int M>
 void launch_cell_t(MhHandle* h) {
-  LAUNCH(h, KN_CELL, (k_cell_lag<DIM, M>), h->nblk_c, MH_BLOCK, h->dev, h->Ua);
+  LAUNCH(h, KN_CELL, (k_cell_lag<DIM, M>), h->nblk_c, MH_BLOCK, h->dev);
 }
 
 #define MH_DISPATCH(fn, h)                                      \
@@ -410,7 +409,7 @@ void launch_cell_t(MhHandle* h) {
     }                                                           \
   } while (0)
 
-void launch_dt(MhHandle* h) { LAUNCH(h, KN_DT, k_dt, 1, MH_DT_BLOCK, h->dev, h->nblk_c); }
+void launch_dt(MhHandle* h) { LAUNCH(h, KN_DT, k_dt, 1, MH_DT_BLOCK, h->dev); }
 
 // Phase functions shared by mahyco_cycle and the per-entry-point calls.
 void phase_node(MhHandle* h) {
@@ -469,11 +468,189 @@ static void launch_pm_t(MhHandle* h) {
   hipLaunchKernelGGL(k_pression_moyenne<M>, dim3(h->nblk_c), dim3(MH_BLOCK), 0, h->stream, h->dev);
 }
 
+
+// ============================================================================ ensembles
+// Many independent cases advance together: one launch per kernel phase covers the blocks of
+// every active case of a (dimension, environment count) group.  A case is admitted only at
+// a global iteration multiple of 6, so all active cases share the coordinate parity and the
+// sweep rotation (RemapADIService.cc:25, 59-60), hence the sweep direction of each launch.
+size_t case_footprint(const MhMesh& mesh, int m, int with_proj) {
+  const long nc = (long)mesh.nx * mesh.ny * (mesh.dim == 3 ? mesh.nz : 1);
+  const long nn = (long)(mesh.nx + 1) * (mesh.ny + 1) * (mesh.dim == 3 ? mesh.nz + 1 : 1);
+  const long nv = 3 * m + 1;
+  long words = (mesh.nx + mesh.ny + mesh.nz + 3) + nc /*VE*/ + 10 * nn /*u, xl x2, mn*/ + 7 * nc +
+               6L * m * nc + 2 * nc + (nc + nn) / MH_BLOCK + 8;
+  if (with_proj) words += 2 * nv * nc + m * nc + 10 * nn;
+  return (size_t)words * 8 + (size_t)nc;
+}
+
+struct EnsGroup {
+  int dim = 0, m = 0;
+  std::vector<int> slots;
+  int2* d_cmap = nullptr;
+  int2* d_nmap = nullptr;
+  int* d_list = nullptr;
+  int ncb = 0, nnb = 0;
+};
+
+struct MhEnsembleImpl {
+  hipStream_t stream = nullptr;
+  int max_slots = 0;
+  double budget = 0.;
+  size_t used = 0;
+  std::vector<MhHandle*> slot;     // active cases
+  std::vector<MhHandle*> pending;  // admitted, activated at the next phase boundary
+  std::vector<int> pending_slot;
+  MhScalars* d_sc = nullptr;       // scalars of every slot
+  MhDev* d_tab[2] = {nullptr, nullptr};  // case states by coordinate parity
+  std::vector<EnsGroup> groups;
+  long launched = 0;
+  bool dirty = true;
+  long version = 0;
+  std::map<long, hipGraphExec_t> graphs;
+  std::vector<hipGraph_t> graph_defs;
+};
+
+namespace {
+
+void ens_free_groups(MhEnsembleImpl* E) {
+  for (auto& g : E->groups) {
+    (void)hipFree(g.d_cmap);
+    (void)hipFree(g.d_nmap);
+    (void)hipFree(g.d_list);
+  }
+  E->groups.clear();
+}
+
+void ens_drop_graphs(MhEnsembleImpl* E) {
+  for (auto& kv : E->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto g : E->graph_defs) (void)hipGraphDestroy(g);
+  E->graphs.clear();
+  E->graph_defs.clear();
+}
+
+// Rebuilds the state tables and the block maps after a change of composition.
+int ens_rebuild(MhEnsembleImpl* E) {
+  HIPCHK(hipStreamSynchronize(E->stream));
+  ens_drop_graphs(E);
+  ens_free_groups(E);
+  std::vector<MhDev> tab[2];
+  for (int p = 0; p < 2; ++p) tab[p].resize(E->max_slots);
+  for (int sl = 0; sl < E->max_slots; ++sl) {
+    MhHandle* h = E->slot[sl];
+    if (!h) continue;
+    for (int p = 0; p < 2; ++p) {
+      MhDev d = h->dev;
+      for (int k = 0; k < 3; ++k) {
+        d.xlp[k] = h->xl[p][k];
+        d.xlc[k] = h->xl[p ^ 1][k];
+      }
+      tab[p][sl] = d;
+    }
+    bool found = false;
+    for (auto& g : E->groups)
+      if (g.dim == h->g.dim && g.m == h->m) {
+        g.slots.push_back(sl);
+        found = true;
+      }
+    if (!found) {
+      EnsGroup g;
+      g.dim = h->g.dim;
+      g.m = h->m;
+      g.slots.push_back(sl);
+      E->groups.push_back(g);
+    }
+  }
+  for (int p = 0; p < 2; ++p)
+    HIPCHK(hipMemcpy(E->d_tab[p], tab[p].data(), sizeof(MhDev) * E->max_slots, hipMemcpyHostToDevice));
+  for (auto& g : E->groups) {
+    std::vector<int2> cm, nm;
+    for (int sl : g.slots) {
+      MhHandle* h = E->slot[sl];
+      for (int b = 0; b < h->nblk_c; ++b) cm.push_back(make_int2(sl, b));
+      for (int b = 0; b < h->nblk_n; ++b) nm.push_back(make_int2(sl, b));
+    }
+    g.ncb = (int)cm.size();
+    g.nnb = (int)nm.size();
+    HIPCHK(hipMalloc(&g.d_cmap, sizeof(int2) * cm.size()));
+    HIPCHK(hipMalloc(&g.d_nmap, sizeof(int2) * nm.size()));
+    HIPCHK(hipMalloc(&g.d_list, sizeof(int) * g.slots.size()));
+    HIPCHK(hipMemcpy(g.d_cmap, cm.data(), sizeof(int2) * cm.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(g.d_nmap, nm.data(), sizeof(int2) * nm.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(g.d_list, g.slots.data(), sizeof(int) * g.slots.size(), hipMemcpyHostToDevice));
+  }
+  E->dirty = false;
+  E->version++;
+  return MH_OK;
+}
+
+template <int DIM, int M>
+void ens_group_cycle(MhEnsembleImpl* E, const EnsGroup& g) {
+  const int par = (int)(E->launched & 1);
+  const MhDev* tab = E->d_tab[par];
+  hipStream_t st = E->stream;
+  hipLaunchKernelGGL((k_node_lag_batch<DIM>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap);
+  hipLaunchKernelGGL((k_cell_lag_batch<DIM, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap);
+  const int sens = (int)(E->launched % DIM);
+  for (int s = 0; s < DIM; ++s) {
+    const int D = (s + sens) % DIM, first = s == 0, last = s == DIM - 1, sel = s & 1;
+    if (D == 0) {
+      hipLaunchKernelGGL((k_sweep_cell_batch<DIM, 0, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
+      hipLaunchKernelGGL((k_sweep_node_batch<DIM, 0>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
+    } else if (D == 1) {
+      hipLaunchKernelGGL((k_sweep_cell_batch<DIM, 1, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
+      hipLaunchKernelGGL((k_sweep_node_batch<DIM, 1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
+    } else {
+      hipLaunchKernelGGL((k_sweep_cell_batch<DIM, (DIM == 3 ? 2 : 0), M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
+      hipLaunchKernelGGL((k_sweep_node_batch<DIM, (DIM == 3 ? 2 : 0)>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
+    }
+  }
+  hipLaunchKernelGGL((k_finalize_batch<DIM, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, DIM & 1);
+  hipLaunchKernelGGL(k_dt_batch, dim3((unsigned)g.slots.size()), dim3(MH_DT_BLOCK), 0, st, tab, g.d_list);
+}
+
+void ens_cycle(MhEnsembleImpl* E) {
+  for (const auto& g : E->groups) {
+    if (g.dim == 3) {
+      if (g.m == 1) ens_group_cycle<3, 1>(E, g);
+      else if (g.m == 2) ens_group_cycle<3, 2>(E, g);
+      else if (g.m == 3) ens_group_cycle<3, 3>(E, g);
+      else ens_group_cycle<3, 4>(E, g);
+    } else {
+      if (g.m == 1) ens_group_cycle<2, 1>(E, g);
+      else if (g.m == 2) ens_group_cycle<2, 2>(E, g);
+      else if (g.m == 3) ens_group_cycle<2, 3>(E, g);
+      else ens_group_cycle<2, 4>(E, g);
+    }
+  }
+  E->launched += 1;
+}
+
+// Activates the pending cases (only at a phase boundary).
+int ens_activate(MhEnsembleImpl* E) {
+  if (E->pending.empty() || (E->launched % 6) != 0) return MH_OK;
+  for (size_t i = 0; i < E->pending.size(); ++i) E->slot[E->pending_slot[i]] = E->pending[i];
+  E->pending.clear();
+  E->pending_slot.clear();
+  E->dirty = true;
+  return MH_OK;
+}
+
+}  // namespace
+
 // ============================================================================ C ABI
 extern "C" {
 
+static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const MhOptions* options,
+                       hipStream_t stream, MhScalars* ext_sc, MhHandle** out);
+
 int mahyco_create(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const MhOptions* options,
                   MhHandle** out) {
+  return create_case(mesh, envs, nb_env, options, nullptr, nullptr, out);
+}
+
+static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const MhOptions* options,
+                       hipStream_t stream, MhScalars* ext_sc, MhHandle** out) {
   if (!out) return MH_ERR_INVALID;
   *out = nullptr;
   int st = validate(mesh, envs, nb_env, options);
@@ -495,11 +672,17 @@ int mahyco_create(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, con
   g.nn = (long)g.nnx * g.nny * g.nnz;
   h->nblk_c = (int)((g.nc + MH_BLOCK - 1) / MH_BLOCK);
   h->nblk_n = (int)((g.nn + MH_BLOCK - 1) / MH_BLOCK);
-  st = hip_check(hipStreamCreate(&h->stream), "hipStreamCreate");
-  if (st != MH_OK) {
-    delete h;
-    return st;
+  if (stream) {
+    h->stream = stream;
+    h->owns_stream = false;
+  } else {
+    st = hip_check(hipStreamCreate(&h->stream), "hipStreamCreate");
+    if (st != MH_OK) {
+      delete h;
+      return st;
+    }
   }
+  h->ext_sc = ext_sc;
   HostInit I;
   st = host_init(h, I);
   if (st != MH_OK) {
@@ -565,7 +748,8 @@ int mahyco_create(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, con
   ok |= dalloc(h, &S.Vl, nc);
   ok |= dalloc(h, &S.partial, (size_t)std::max(h->nblk_c, h->nblk_n));
   ok |= dalloc(h, &S.err, 4);
-  ok |= dalloc(h, &S.sc, 1);
+  if (ext_sc) S.sc = ext_sc;
+  else ok |= dalloc(h, &S.sc, 1);
   if (options->with_projection) {
     const size_t nv = 3 * nb_env + 1;
     ok |= dalloc(h, &h->Ua, nv * nc);
@@ -585,6 +769,15 @@ int mahyco_create(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, con
   S.ay = ay;
   S.az = az;
   S.VE = VE;
+  S.Ua = h->Ua;
+  S.Ub = h->Ub;
+  S.dflux = h->dflux;
+  S.Ud = h->Ud;
+  S.Pa = h->Pa;
+  S.Pb = h->Pb;
+  S.nblk_c = h->nblk_c;
+  S.nblk_n = h->nblk_n;
+  S.max_iter = 0;
   // initial Lagrangian coordinates = Eulerian grid
   std::vector<double> xs[3];
   for (int d = 0; d < 3; ++d) xs[d].assign(nn, 0.);
@@ -647,7 +840,7 @@ void mahyco_destroy(MhHandle* h) {
     (void)hipEventDestroy(r.b);
   }
   for (void* p : h->allocs) (void)hipFree(p);
-  if (h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->stream && h->owns_stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
 
@@ -910,6 +1103,173 @@ int mahyco_compute_pression_moyenne(MhHandle* h) {
     default: launch_pm_t<4>(h); break;
   }
   return hip_check(hipGetLastError(), "k_pression_moyenne");
+}
+
+// ---- ensembles ------------------------------------------------------------------------
+size_t mahyco_case_footprint(const MhMesh* mesh, int nb_env, const MhOptions* options) {
+  if (!mesh || !options || nb_env < 1 || nb_env > MH_MAX_ENV) return 0;
+  return case_footprint(*mesh, nb_env, options->with_projection);
+}
+
+int mahyco_ensemble_create(int max_slots, double hbm_budget_bytes, MhEnsemble** out) {
+  if (!out || max_slots < 1) return MH_ERR_INVALID;
+  MhEnsembleImpl* E = new MhEnsembleImpl();
+  E->max_slots = max_slots;
+  E->budget = hbm_budget_bytes;
+  E->slot.assign(max_slots, nullptr);
+  int st = hip_check(hipStreamCreate(&E->stream), "hipStreamCreate");
+  if (st == MH_OK) st = hip_check(hipMalloc(&E->d_sc, sizeof(MhScalars) * max_slots), "hipMalloc");
+  if (st == MH_OK) st = hip_check(hipMalloc(&E->d_tab[0], sizeof(MhDev) * max_slots), "hipMalloc");
+  if (st == MH_OK) st = hip_check(hipMalloc(&E->d_tab[1], sizeof(MhDev) * max_slots), "hipMalloc");
+  if (st != MH_OK) {
+    mahyco_ensemble_destroy((MhEnsemble*)E);
+    return st;
+  }
+  *out = (MhEnsemble*)E;
+  return MH_OK;
+}
+
+void mahyco_ensemble_destroy(MhEnsemble* e) {
+  MhEnsembleImpl* E = (MhEnsembleImpl*)e;
+  if (!E) return;
+  if (E->stream) (void)hipStreamSynchronize(E->stream);
+  ens_drop_graphs(E);
+  ens_free_groups(E);
+  for (MhHandle* h : E->slot) mahyco_destroy(h);
+  for (MhHandle* h : E->pending) mahyco_destroy(h);
+  (void)hipFree(E->d_sc);
+  (void)hipFree(E->d_tab[0]);
+  (void)hipFree(E->d_tab[1]);
+  if (E->stream) (void)hipStreamDestroy(E->stream);
+  delete E;
+}
+
+int mahyco_ensemble_add(MhEnsemble* e, const MhMesh* mesh, const MhEnvironment* envs, int nb_env,
+                        const MhOptions* options, int max_iterations, int* slot_out) {
+  MhEnsembleImpl* E = (MhEnsembleImpl*)e;
+  if (!E || !slot_out || !mesh || !options) return MH_ERR_INVALID;
+  *slot_out = -1;
+  if (!options->with_projection) return MH_ERR_UNSUPPORTED;
+  if (mesh->nx < 2 || mesh->ny < 2 || (mesh->dim == 3 && mesh->nz < 2)) return MH_ERR_UNSUPPORTED;
+  const size_t need = case_footprint(*mesh, nb_env, 1);
+  if ((double)(E->used + need) > E->budget) return MH_ERR_NOMEM;  // admission by HBM footprint
+  int sl = -1;
+  for (int i = 0; i < E->max_slots && sl < 0; ++i) {
+    if (E->slot[i]) continue;
+    bool taken = false;
+    for (int q : E->pending_slot) taken |= (q == i);
+    if (!taken) sl = i;
+  }
+  if (sl < 0) return MH_ERR_NOMEM;
+  MhHandle* h = nullptr;
+  int st = create_case(mesh, envs, nb_env, options, E->stream, E->d_sc + sl, &h);
+  if (st != MH_OK) return st;
+  h->dev.max_iter = max_iterations > 0 ? max_iterations : 0;
+  h->bytes = need;
+  E->used += need;
+  E->pending.push_back(h);
+  E->pending_slot.push_back(sl);
+  *slot_out = sl;
+  return ens_activate(E);
+}
+
+int mahyco_ensemble_remove(MhEnsemble* e, int slot) {
+  MhEnsembleImpl* E = (MhEnsembleImpl*)e;
+  if (!E || slot < 0 || slot >= E->max_slots) return MH_ERR_INVALID;
+  for (size_t i = 0; i < E->pending.size(); ++i)
+    if (E->pending_slot[i] == slot) {
+      E->used -= E->pending[i]->bytes;
+      mahyco_destroy(E->pending[i]);
+      E->pending.erase(E->pending.begin() + i);
+      E->pending_slot.erase(E->pending_slot.begin() + i);
+      return MH_OK;
+    }
+  if (!E->slot[slot]) return MH_ERR_INVALID;
+  HIPCHK(hipStreamSynchronize(E->stream));
+  E->used -= E->slot[slot]->bytes;
+  mahyco_destroy(E->slot[slot]);
+  E->slot[slot] = nullptr;
+  E->dirty = true;
+  return MH_OK;
+}
+
+int mahyco_ensemble_step(MhEnsemble* e, int ncycles, int use_graph) {
+  MhEnsembleImpl* E = (MhEnsembleImpl*)e;
+  if (!E || ncycles < 0) return MH_ERR_INVALID;
+  int done = 0;
+  while (done < ncycles) {
+    int st = ens_activate(E);
+    if (st != MH_OK) return st;
+    if (E->dirty) {
+      st = ens_rebuild(E);
+      if (st != MH_OK) return st;
+    }
+    // a 6-iteration graph when no admission can happen inside it
+    if (use_graph && (E->launched % 6) == 0 && ncycles - done >= 6 && E->pending.empty()) {
+      const long key = E->version;
+      auto it = E->graphs.find(key);
+      if (it == E->graphs.end()) {
+        hipGraph_t gdef;
+        HIPCHK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+        const long l0 = E->launched;
+        for (int i = 0; i < 6; ++i) ens_cycle(E);
+        HIPCHK(hipStreamEndCapture(E->stream, &gdef));
+        E->launched = l0;
+        hipGraphExec_t ex;
+        HIPCHK(hipGraphInstantiate(&ex, gdef, nullptr, nullptr, 0));
+        E->graph_defs.push_back(gdef);
+        it = E->graphs.emplace(key, ex).first;
+      }
+      HIPCHK(hipGraphLaunch(it->second, E->stream));
+      E->launched += 6;
+      done += 6;
+      continue;
+    }
+    ens_cycle(E);
+    done += 1;
+  }
+  return hip_check(hipGetLastError(), "ensemble launch");
+}
+
+int mahyco_ensemble_synchronize(MhEnsemble* e) {
+  MhEnsembleImpl* E = (MhEnsembleImpl*)e;
+  if (!E) return MH_ERR_INVALID;
+  HIPCHK(hipStreamSynchronize(E->stream));
+  return MH_OK;
+}
+
+int mahyco_ensemble_poll(MhEnsemble* e, int* status, double* time, int* iteration) {
+  MhEnsembleImpl* E = (MhEnsembleImpl*)e;
+  if (!E) return MH_ERR_INVALID;
+  std::vector<MhScalars> sc(E->max_slots);
+  HIPCHK(hipMemcpyAsync(sc.data(), E->d_sc, sizeof(MhScalars) * E->max_slots, hipMemcpyDeviceToHost, E->stream));
+  HIPCHK(hipStreamSynchronize(E->stream));
+  int active = 0;
+  for (int i = 0; i < E->max_slots; ++i) {
+    bool live = E->slot[i] != nullptr;
+    bool pend = false;
+    for (int q : E->pending_slot) pend |= (q == i);
+    int stv = -100;  // empty slot
+    if (pend) stv = 2;  // admitted, not yet active
+    else if (live) stv = sc[i].status != 0 ? sc[i].status : (sc[i].stopped ? MH_STOPPED : MH_OK);
+    if (live) active++;
+    if (status) status[i] = stv;
+    if (time) time[i] = live ? sc[i].t : 0.;
+    if (iteration) iteration[i] = live ? sc[i].iteration : 0;
+  }
+  return active;
+}
+
+int mahyco_ensemble_get_field(MhEnsemble* e, int slot, int field, int env, int comp, double* out) {
+  MhEnsembleImpl* E = (MhEnsembleImpl*)e;
+  if (!E || slot < 0 || slot >= E->max_slots || !E->slot[slot]) return MH_ERR_INVALID;
+  HIPCHK(hipStreamSynchronize(E->stream));
+  return mahyco_get_field(E->slot[slot], field, env, comp, out);
+}
+
+double mahyco_ensemble_used_bytes(MhEnsemble* e) {
+  MhEnsembleImpl* E = (MhEnsembleImpl*)e;
+  return E ? (double)E->used : 0.;
 }
 
 int mahyco_profile_enable(MhHandle* h, int on) {

@@ -35,7 +35,7 @@ __device__ __forceinline__ double amax(double a, double b) { return (a < b) ? b 
 __device__ __forceinline__ D3 load3(double* const* a, long n) { return d3(a[0][n], a[1][n], a[2][n]); }
 
 // Block-wide minimum (Arcane math::min semantics on non-NaN input), written by thread 0.
-__device__ __forceinline__ void block_min_store(double v, double* out) {
+__device__ __forceinline__ void block_min_store(double v, double* out, int lb) {
   __shared__ double red[MH_BLOCK / 64];
   for (int off = 32; off > 0; off >>= 1) {
     double o = __shfl_xor(v, off, 64);
@@ -47,7 +47,7 @@ __device__ __forceinline__ void block_min_store(double v, double* out) {
   if (threadIdx.x == 0) {
     double r = red[0];
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = amin(r, red[w]);
-    out[blockIdx.x] = r;
+    out[lb] = r;
   }
 }
 
@@ -95,8 +95,8 @@ __device__ __forceinline__ void node_gather(const MhDev& S, int i, int j, int k,
 // SaveValuesAtN + UpdateVelocity (:438-562) + ApplyBoundaryCondition (:756-807) +
 // UpdatePosition (:730-751) for one node.
 template <int DIM>
-__global__ __launch_bounds__(MH_BLOCK) void k_node_lag(MhDev S) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_node_lag_body(const MhDev& S, int lb) {
+  const long n = lb * (long)blockDim.x + threadIdx.x;
   if (n >= S.g.nn) return;
   if (mh_halted(S.sc)) return;
   const int nnx = S.g.nnx, nny = S.g.nny;
@@ -233,9 +233,10 @@ __device__ __forceinline__ void eos(const MhEnvParam& ep, double rho, double e, 
 // With remap: writes U (V_k, V_k rho_k, V_k rho_k e_k, V q; PrepareRemap.cc:100-136), V, L.
 // Without: applies the EOS and the mean pressure and writes the CFL candidate.
 template <int DIM, int M>
-__global__ __launch_bounds__(MH_BLOCK) void k_cell_lag(MhDev S, double* __restrict__ U) {
+__device__ __forceinline__ void k_cell_lag_body(const MhDev& S, int lb) {
+  double* __restrict__ U = S.Ua;
   constexpr int NV = 3 * M + 1;
-  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long c = lb * (long)blockDim.x + threadIdx.x;
   const bool active = c < S.g.nc && !mh_halted(S.sc);
   double dxs = 1.79769313486231570815e+308;
   if (active) {
@@ -349,7 +350,7 @@ __global__ __launch_bounds__(MH_BLOCK) void k_cell_lag(MhDev S, double* __restri
     }
     S.L[c] = L;
   }
-  if (!S.with_proj) block_min_store(dxs, S.partial);
+  if (!S.with_proj) block_min_store(dxs, S.partial, lb);
 }
 
 // ----------------------------------------------------------------------------- remap sweeps
@@ -430,11 +431,12 @@ __device__ __forceinline__ void face_quant(const MhDev& S, long f0, double& area
 // computeUpwindFaceQuantitiesForProjection (:279-383) and computeUremap (:395-575).
 // Phi of the stencil is rebuilt from U (U / V before the first sweep, U / sum V_k after).
 template <int DIM, int D, int M>
-__global__ __launch_bounds__(MH_BLOCK) void k_sweep_cell(MhDev S, const double* __restrict__ Uin,
-                                                         double* __restrict__ Uout,
-                                                         double* __restrict__ dflux, int first) {
+__device__ __forceinline__ void k_sweep_cell_body(const MhDev& S, int lb, int usel, int first) {
+  const double* __restrict__ Uin = usel ? S.Ub : S.Ua;
+  double* __restrict__ Uout = usel ? S.Ua : S.Ub;
+  double* __restrict__ dflux = S.dflux;
   constexpr int NV = 3 * M + 1;
-  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long c = lb * (long)blockDim.x + threadIdx.x;
   if (c >= S.g.nc) return;
   if (mh_halted(S.sc)) return;
   const int nx = S.g.nx, ny = S.g.ny;
@@ -541,13 +543,13 @@ __global__ __launch_bounds__(MH_BLOCK) void k_sweep_cell(MhDev S, const double* 
 // Nodes with no neighbour along D keep their values.  On the last sweep the node velocity
 // u = mu / m is formed for every node (RemapADIFinal.cc:286-297).
 template <int DIM, int D>
-__global__ __launch_bounds__(MH_BLOCK) void k_sweep_node(MhDev S, double* __restrict__ Ud,
-                                                         const double* __restrict__ Pin,
-                                                         double* __restrict__ Pout,
-                                                         const double* __restrict__ dflux, int first,
-                                                         int last) {
+__device__ __forceinline__ void k_sweep_node_body(const MhDev& S, int lb, int psel, int first, int last) {
+  double* __restrict__ Ud = S.Ud;
+  const double* __restrict__ Pin = psel ? S.Pa : S.Pb;
+  double* __restrict__ Pout = psel ? S.Pb : S.Pa;
+  const double* __restrict__ dflux = S.dflux;
   constexpr int NC = DIM;  // velocity components carried
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long n = lb * (long)blockDim.x + threadIdx.x;
   if (n >= S.g.nn) return;
   if (mh_halted(S.sc)) return;
   const int nnx = S.g.nnx, nny = S.g.nny;
@@ -658,9 +660,9 @@ __global__ __launch_bounds__(MH_BLOCK) void k_sweep_node(MhDev S, double* __rest
 // u = mu / m for every node with a non-zero projected mass (RemapADIFinal.cc:286-297), for
 // cycles whose remap has a single sweep (the sweep kernel still reads u) or none.
 template <int DIM>
-__global__ __launch_bounds__(MH_BLOCK) void k_node_velocity(MhDev S, const double* __restrict__ Ud,
-                                                            int materialize) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_node_velocity_body(const MhDev& S, int lb, int materialize) {
+  const double* __restrict__ Ud = S.Ud;
+  const long n = lb * (long)blockDim.x + threadIdx.x;
   if (n >= S.g.nn) return;
   if (mh_halted(S.sc)) return;
   const long nn = S.g.nn;
@@ -682,8 +684,9 @@ __global__ __launch_bounds__(MH_BLOCK) void k_node_velocity(MhDev S, const doubl
 // computeDeltaT (MahycoModule.cc:1763-1779).  Pure cells write through the global arrays
 // only (their environment value is the global value).
 template <int DIM, int M>
-__global__ __launch_bounds__(MH_BLOCK) void k_finalize(MhDev S, const double* __restrict__ Uin) {
-  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_finalize_body(const MhDev& S, int lb, int usel) {
+  const double* __restrict__ Uin = usel ? S.Ub : S.Ua;
+  const long c = lb * (long)blockDim.x + threadIdx.x;
   const bool active = c < S.g.nc && !mh_halted(S.sc);
   double dxs = 1.79769313486231570815e+308;
   if (active) {
@@ -846,7 +849,7 @@ __global__ __launch_bounds__(MH_BLOCK) void k_finalize(MhDev S, const double* __
     const long b = i + (long)S.g.nnx * (j + (long)S.g.nny * k);
     dxs = S.L[c] / (c_g + cell_vmax<DIM>(S, b));
   }
-  block_min_store(dxs, S.partial);
+  block_min_store(dxs, S.partial, lb);
 }
 
 // applyEOS of one environment on its member cells (PerfectGasEOSService.cc:45-67,
@@ -895,7 +898,8 @@ __global__ __launch_bounds__(MH_BLOCK) void k_pression_moyenne(MhDev S) {
 
 // computeDeltaT (MahycoModule.cc:1739-1864) on the block minima, plus the time increment
 // of the iteration and the final-time stop rule.  One block.
-__global__ __launch_bounds__(MH_DT_BLOCK) void k_dt(MhDev S, int nblocks) {
+__device__ __forceinline__ void k_dt_body(const MhDev& S) {
+  const int nblocks = S.nblk_c;
   __shared__ double red[MH_DT_BLOCK / 64];
   MhScalars* sc = S.sc;
   if (mh_halted(sc)) return;
@@ -925,6 +929,36 @@ __global__ __launch_bounds__(MH_DT_BLOCK) void k_dt(MhDev S, int nblocks) {
   sc->dt = new_dt;
   const double T = S.tfinal;
   if ((t < T && t + new_dt > T) || t > T) sc->stopped = 1;
+  if (S.max_iter > 0 && sc->iteration >= S.max_iter) sc->stopped = 1;
+}
+
+// ----------------------------------------------------------------------------- wrappers
+// Single mesh: the state travels by value in the kernel arguments.  Batch: one launch
+// covers many meshes; block b works on mesh bmap[b].x, local block bmap[b].y, whose state
+// is tab[bmap[b].x] (meshes of one launch share DIM, M and the sweep direction).
+#define MH_WRAP(NAME, TPL, TARGS, PARAMS, ARGS)                                              \
+  TPL __global__ __launch_bounds__(MH_BLOCK) void NAME(MhDev S PARAMS) {                     \
+    NAME##_body TARGS(S, (int)blockIdx.x ARGS);                                              \
+  }                                                                                          \
+  TPL __global__ __launch_bounds__(MH_BLOCK) void NAME##_batch(const MhDev* __restrict__ tab, \
+                                                               const int2* __restrict__ bmap PARAMS) { \
+    const int2 bm = bmap[blockIdx.x];                                                        \
+    NAME##_body TARGS(tab[bm.x], bm.y ARGS);                                                 \
+  }
+#define MH_COMMA ,
+MH_WRAP(k_node_lag, template <int DIM>, <DIM>, , )
+MH_WRAP(k_cell_lag, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, , )
+MH_WRAP(k_sweep_cell, template <int DIM MH_COMMA int D MH_COMMA int M>, <DIM MH_COMMA D MH_COMMA M>,
+        MH_COMMA int usel MH_COMMA int first, MH_COMMA usel MH_COMMA first)
+MH_WRAP(k_sweep_node, template <int DIM MH_COMMA int D>, <DIM MH_COMMA D>,
+        MH_COMMA int psel MH_COMMA int first MH_COMMA int last, MH_COMMA psel MH_COMMA first MH_COMMA last)
+MH_WRAP(k_node_velocity, template <int DIM>, <DIM>, MH_COMMA int materialize, MH_COMMA materialize)
+MH_WRAP(k_finalize, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, MH_COMMA int usel, MH_COMMA usel)
+
+__global__ __launch_bounds__(MH_DT_BLOCK) void k_dt(MhDev S) { k_dt_body(S); }
+__global__ __launch_bounds__(MH_DT_BLOCK) void k_dt_batch(const MhDev* __restrict__ tab,
+                                                          const int* __restrict__ list) {
+  k_dt_body(tab[list[blockIdx.x]]);
 }
 
 #endif

@@ -70,6 +70,12 @@ struct MhDev {
   double* partial;                            // per-block CFL minima
   int* err;                                   // error word raised inside a cycle
   MhScalars* sc;
+  // remap buffers: U ping-pong [nv][nc], mass fluxes [m][nc], dual state [4][nn] (in
+  // place), dual phi ping-pong [3][nn]
+  double *Ua, *Ub, *dflux, *Ud, *Pa, *Pb;
+  int nblk_c, nblk_n;  // blocks of MH_BLOCK threads over cells / nodes
+  int max_iter;        // > 0: stop after this many iterations (fixed-horizon runs)
+  int pad2;
 };
 
 #endif

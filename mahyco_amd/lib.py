@@ -59,6 +59,31 @@ def load():
     lib.mahyco_profile_enable.restype = C.c_int
     lib.mahyco_profile_read.argtypes = [H, C.c_char_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]
     lib.mahyco_profile_read.restype = C.c_int
+    lib.mahyco_case_footprint.argtypes = [C.POINTER(T.MhMesh), C.c_int, C.POINTER(T.MhOptions)]
+    lib.mahyco_case_footprint.restype = C.c_size_t
+    lib.mahyco_ensemble_create.argtypes = [C.c_int, C.c_double, C.POINTER(C.c_void_p)]
+    lib.mahyco_ensemble_create.restype = C.c_int
+    lib.mahyco_ensemble_destroy.argtypes = [H]
+    lib.mahyco_ensemble_destroy.restype = None
+    lib.mahyco_ensemble_add.argtypes = [H, C.POINTER(T.MhMesh), C.c_void_p, C.c_int, C.POINTER(T.MhOptions),
+                                        C.c_int, C.POINTER(C.c_int)]
+    lib.mahyco_ensemble_add.restype = C.c_int
+    lib.mahyco_ensemble_remove.argtypes = [H, C.c_int]
+    lib.mahyco_ensemble_remove.restype = C.c_int
+    lib.mahyco_ensemble_step.argtypes = [H, C.c_int, C.c_int]
+    lib.mahyco_ensemble_step.restype = C.c_int
+    lib.mahyco_ensemble_synchronize.argtypes = [H]
+    lib.mahyco_ensemble_synchronize.restype = C.c_int
+    lib.mahyco_ensemble_poll.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.mahyco_ensemble_poll.restype = C.c_int
+    lib.mahyco_ensemble_get_field.argtypes = [H, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    lib.mahyco_ensemble_get_field.restype = C.c_int
+    lib.mahyco_ensemble_used_bytes.argtypes = [H]
+    lib.mahyco_ensemble_used_bytes.restype = C.c_double
+    lib.mahyco_apply_eos.argtypes = [H, C.c_int]
+    lib.mahyco_apply_eos.restype = C.c_int
+    lib.mahyco_compute_pression_moyenne.argtypes = [H]
+    lib.mahyco_compute_pression_moyenne.restype = C.c_int
     _lib = lib
     return lib
 
@@ -137,3 +162,85 @@ class GpuCase:
         n = self.lib.mahyco_profile_read(self.h, names, 1024, ms.ctypes.data, cnt.ctypes.data, 16)
         lst = names.raw.split(b"\0")[:n]
         return {lst[i].decode(): (float(ms[i]), int(cnt[i])) for i in range(n)}
+
+
+def footprint(case):
+    """Device bytes a case needs (the admission measure of ensembles)."""
+    lib = load()
+    m, o = case.mesh_c(), case.options_c()
+    return int(lib.mahyco_case_footprint(C.byref(m), len(case.envs), C.byref(o)))
+
+
+class Ensemble:
+    """Many independent cases advancing together (batched launches, per-case time step)."""
+
+    def __init__(self, max_slots, hbm_budget_bytes):
+        self.lib = load()
+        e = C.c_void_p()
+        st = self.lib.mahyco_ensemble_create(max_slots, float(hbm_budget_bytes), C.byref(e))
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_ensemble_create")
+        self.e = e
+        self.max_slots = max_slots
+        self._keep = {}
+        self.cases = {}
+
+    def close(self):
+        if getattr(self, "e", None):
+            self.lib.mahyco_ensemble_destroy(self.e)
+            self.e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, case, max_iterations=0):
+        """Returns the slot, or None when the budget or the slots are exhausted."""
+        m, envs, o = case.mesh_c(), case.envs_c(), case.options_c()
+        slot = C.c_int(-1)
+        st = self.lib.mahyco_ensemble_add(self.e, C.byref(m), envs, len(case.envs), C.byref(o),
+                                          int(max_iterations), C.byref(slot))
+        if st == T.MH_ERR_NOMEM:
+            return None
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_ensemble_add")
+        self._keep[slot.value] = (m, envs, o)
+        self.cases[slot.value] = case
+        return slot.value
+
+    def remove(self, slot):
+        st = self.lib.mahyco_ensemble_remove(self.e, slot)
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_ensemble_remove")
+        self._keep.pop(slot, None)
+        self.cases.pop(slot, None)
+
+    def step(self, ncycles, graph=True):
+        st = self.lib.mahyco_ensemble_step(self.e, ncycles, int(graph))
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_ensemble_step")
+
+    def synchronize(self):
+        return self.lib.mahyco_ensemble_synchronize(self.e)
+
+    def poll(self):
+        st = np.zeros(self.max_slots, dtype=np.int32)
+        tm = np.zeros(self.max_slots)
+        it = np.zeros(self.max_slots, dtype=np.int32)
+        n = self.lib.mahyco_ensemble_poll(self.e, st.ctypes.data, tm.ctypes.data, it.ctypes.data)
+        return n, st, tm, it
+
+    def get(self, slot, field, env=-1, comp=0):
+        case = self.cases[slot]
+        n = case.nb_nodes if field >= 100 else case.nb_cells
+        out = np.empty(n)
+        st = self.lib.mahyco_ensemble_get_field(self.e, slot, field, env, comp, out.ctypes.data)
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_ensemble_get_field")
+        return out
+
+    @property
+    def used_bytes(self):
+        return self.lib.mahyco_ensemble_used_bytes(self.e)

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -50,6 +51,7 @@ struct MhHandle {
   std::map<long, hipGraphExec_t> graphs;  // key: (cycles << 8) | (parity << 4) | sens
   std::vector<hipGraph_t> graph_defs;
   bool owns_stream = true;
+  bool node_tiled = true;  // 3D: z-marching node kernel (MAHYCO_NODE_KERNEL=flat selects the flat one)
   MhScalars* ext_sc = nullptr;  // scalars living in an ensemble's array
   size_t bytes = 0;             // device bytes of the case
 };
@@ -386,7 +388,13 @@ void set_parity(MhHandle* h) {
 
 template <int DIM, int M>
 void launch_node_t(MhHandle* h) {
-  LAUNCH(h, KN_NODE, (k_node_lag<DIM>), h->nblk_n, MH_BLOCK, h->dev);
+  if (DIM == 3 && h->node_tiled) {
+    const dim3 grid((h->g.nnx + MH_NT - 1) / MH_NT, (h->g.nny + MH_NT - 1) / MH_NT,
+                    (h->g.nnz + MH_NT_KC - 1) / MH_NT_KC);
+    LAUNCH(h, KN_NODE, k_node_lag3_tiled, grid, MH_BLOCK, h->dev);
+  } else {
+    LAUNCH(h, KN_NODE, (k_node_lag<DIM>), h->nblk_n, MH_BLOCK, h->dev);
+  }
 }
 template <int DIM, int M>
 void launch_cell_t(MhHandle* h) {
@@ -683,6 +691,10 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
     }
   }
   h->ext_sc = ext_sc;
+  {
+    const char* nk = getenv("MAHYCO_NODE_KERNEL");
+    h->node_tiled = !(nk && strcmp(nk, "flat") == 0);
+  }
   HostInit I;
   st = host_init(h, I);
   if (st != MH_OK) {

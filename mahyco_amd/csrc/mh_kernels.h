@@ -92,31 +92,11 @@ __device__ __forceinline__ void node_gather(const MhDev& S, int i, int j, int k,
   if (DIM == 3) fz += pr * q.z;
 }
 
-// SaveValuesAtN + UpdateVelocity (:438-562) + ApplyBoundaryCondition (:756-807) +
-// UpdatePosition (:730-751) for one node.
+// UpdateVelocity tail (u += dt_half / m F), ApplyBoundaryCondition (Vx/Vy/Vz on the
+// XMIN..ZMAX node groups, in option order) and UpdatePosition (x += dt u) for node n.
 template <int DIM>
-__device__ __forceinline__ void k_node_lag_body(const MhDev& S, int lb) {
-  const long n = lb * (long)blockDim.x + threadIdx.x;
-  if (n >= S.g.nn) return;
-  if (mh_halted(S.sc)) return;
-  const int nnx = S.g.nnx, nny = S.g.nny;
-  const int i = (int)(n % nnx);
-  const long r = n / nnx;
-  const int j = (int)(r % nny);
-  const int k = (int)(r / nny);
-  const double dt = S.sc->dt;
-  const double dth = 0.5 * (S.sc->dt_old + dt);
-  double mass = 0., fx = 0., fy = 0., fz = 0.;
-  if (DIM == 3) {
-    node_gather<3, 1, 1, 1>(S, i, j, k, mass, fx, fy, fz);
-    node_gather<3, 0, 1, 1>(S, i, j, k, mass, fx, fy, fz);
-    node_gather<3, 1, 0, 1>(S, i, j, k, mass, fx, fy, fz);
-    node_gather<3, 0, 0, 1>(S, i, j, k, mass, fx, fy, fz);
-  }
-  node_gather<DIM, 1, 1, 0>(S, i, j, k, mass, fx, fy, fz);
-  node_gather<DIM, 0, 1, 0>(S, i, j, k, mass, fx, fy, fz);
-  node_gather<DIM, 1, 0, 0>(S, i, j, k, mass, fx, fy, fz);
-  node_gather<DIM, 0, 0, 0>(S, i, j, k, mass, fx, fy, fz);
+__device__ __forceinline__ void node_finish(const MhDev& S, long n, int i, int j, int k, double dt,
+                                            double dth, double mass, double fx, double fy, double fz) {
   const double s = dth / mass;
   double ux = S.u[0][n] + s * fx;
   double uy = S.u[1][n] + s * fy;
@@ -158,6 +138,156 @@ __device__ __forceinline__ void k_node_lag_body(const MhDev& S, int lb) {
     S.xlc[2][n] = z0 + dt * uz;
   }
   S.mn[n] = mass;
+}
+
+
+// SaveValuesAtN + UpdateVelocity (:438-562) + ApplyBoundaryCondition (:756-807) +
+// UpdatePosition (:730-751) for one node.
+template <int DIM>
+__device__ __forceinline__ void k_node_lag_body(const MhDev& S, int lb) {
+  const long n = lb * (long)blockDim.x + threadIdx.x;
+  if (n >= S.g.nn) return;
+  if (mh_halted(S.sc)) return;
+  const int nnx = S.g.nnx, nny = S.g.nny;
+  const int i = (int)(n % nnx);
+  const long r = n / nnx;
+  const int j = (int)(r % nny);
+  const int k = (int)(r / nny);
+  const double dt = S.sc->dt;
+  const double dth = 0.5 * (S.sc->dt_old + dt);
+  double mass = 0., fx = 0., fy = 0., fz = 0.;
+  if (DIM == 3) {
+    node_gather<3, 1, 1, 1>(S, i, j, k, mass, fx, fy, fz);
+    node_gather<3, 0, 1, 1>(S, i, j, k, mass, fx, fy, fz);
+    node_gather<3, 1, 0, 1>(S, i, j, k, mass, fx, fy, fz);
+    node_gather<3, 0, 0, 1>(S, i, j, k, mass, fx, fy, fz);
+  }
+  node_gather<DIM, 1, 1, 0>(S, i, j, k, mass, fx, fy, fz);
+  node_gather<DIM, 0, 1, 0>(S, i, j, k, mass, fx, fy, fz);
+  node_gather<DIM, 1, 0, 0>(S, i, j, k, mass, fx, fy, fz);
+  node_gather<DIM, 0, 0, 0>(S, i, j, k, mass, fx, fy, fz);
+  node_finish<DIM>(S, n, i, j, k, dt, dth, mass, fx, fy, fz);
+}
+
+
+// 3D node kernel, z-marching tiles: a block owns 16x16 node columns over MH_NT_KC node
+// planes.  Each cell plane of the tile (17x17 cells) is processed once: its corner vectors
+// (cqs_hex, computed once per cell instead of once per corner) are multiplied by p+q and
+// kept in LDS with M/8 -- the bottom corners for the node plane it sits on, the top ones
+// for the next plane.  Every node then adds its 8 cells in cell order, as k_node_lag does,
+// so both kernels give the same bits.
+#define MH_NT 16
+#define MH_NT_KC 8
+#define MH_NT_CELLS ((MH_NT + 1) * (MH_NT + 1))
+__device__ __forceinline__ void nt_cell(const MhDev& S, int ii, int jj, int kk, double* up_or_lo, int s,
+                                        double* keep, bool top_to_lds) {
+  // corner force contributions (p + q) CQS_l and the node-mass share w M of one cell
+  const long nnx = S.g.nnx, nnxy = (long)S.g.nnx * S.g.nny;
+  const long c = ii + (long)S.g.nx * (jj + (long)S.g.ny * kk);
+  const long b = ii + nnx * (jj + (long)S.g.nny * kk);
+  D3 x[8], q[8];
+  x[0] = load3(S.xlp, b);
+  x[1] = load3(S.xlp, b + 1);
+  x[2] = load3(S.xlp, b + 1 + nnx);
+  x[3] = load3(S.xlp, b + nnx);
+  x[4] = load3(S.xlp, b + nnxy);
+  x[5] = load3(S.xlp, b + 1 + nnxy);
+  x[6] = load3(S.xlp, b + 1 + nnx + nnxy);
+  x[7] = load3(S.xlp, b + nnx + nnxy);
+  cqs_hex(x, q);
+  const double pr = S.p[c] + S.q[c];
+  const double wm = 0.125 * S.M[c];
+  for (int l = 0; l < 4; ++l) {
+    up_or_lo[(3 * l + 0) * MH_NT_CELLS + s] = pr * q[l].x;
+    up_or_lo[(3 * l + 1) * MH_NT_CELLS + s] = pr * q[l].y;
+    up_or_lo[(3 * l + 2) * MH_NT_CELLS + s] = pr * q[l].z;
+  }
+  up_or_lo[12 * MH_NT_CELLS + s] = wm;
+  for (int l = 0; l < 4; ++l) {
+    keep[3 * l + 0] = pr * q[4 + l].x;
+    keep[3 * l + 1] = pr * q[4 + l].y;
+    keep[3 * l + 2] = pr * q[4 + l].z;
+  }
+  keep[12] = wm;
+  (void)top_to_lds;
+}
+
+__device__ __forceinline__ void nt_add(const double* P, int s, int corner, bool ok, double& mass, double& fx,
+                                       double& fy, double& fz) {
+  if (!ok) return;
+  mass += P[12 * MH_NT_CELLS + s];
+  fx += P[(3 * corner + 0) * MH_NT_CELLS + s];
+  fy += P[(3 * corner + 1) * MH_NT_CELLS + s];
+  fz += P[(3 * corner + 2) * MH_NT_CELLS + s];
+}
+
+__device__ __forceinline__ void k_node_lag3_tiled_body(const MhDev& S, int bx, int by, int bz) {
+  __shared__ double lo[13 * MH_NT_CELLS];  // top corners (4..7) of the cell plane below
+  __shared__ double up[13 * MH_NT_CELLS];  // bottom corners (0..3) of the cell plane above
+  if (mh_halted(S.sc)) return;
+  const int tid = threadIdx.x;
+  const int tx = tid % MH_NT, ty = tid / MH_NT;
+  const int i0 = bx * MH_NT, j0 = by * MH_NT, k0 = bz * MH_NT_KC;
+  const int nx = S.g.nx, ny = S.g.ny, nz = S.g.nz;
+  const int kend = min(k0 + MH_NT_KC, S.g.nnz);
+  const double dt = S.sc->dt;
+  const double dth = 0.5 * (S.sc->dt_old + dt);
+  double keep[2][13];
+  // prologue: top corners of the cell plane k0 - 1
+  if (k0 >= 1) {
+    for (int r = 0; r < 2; ++r) {
+      const int sl = tid + r * MH_BLOCK;
+      if (sl >= MH_NT_CELLS) break;
+      const int ii = i0 - 1 + sl % (MH_NT + 1), jj = j0 - 1 + sl / (MH_NT + 1);
+      if (ii < 0 || ii >= nx || jj < 0 || jj >= ny) continue;
+      nt_cell(S, ii, jj, k0 - 1, up, sl, keep[r], false);
+      for (int v = 0; v < 13; ++v) lo[v * MH_NT_CELLS + sl] = keep[r][v];
+    }
+  }
+  const int i = i0 + tx, j = j0 + ty;
+  const bool node_ok = i < S.g.nnx && j < S.g.nny;
+  for (int k = k0; k < kend; ++k) {
+    // cell plane k: bottom corners to LDS now, top corners kept for node plane k + 1
+    if (k < nz) {
+      for (int r = 0; r < 2; ++r) {
+        const int sl = tid + r * MH_BLOCK;
+        if (sl >= MH_NT_CELLS) break;
+        const int ii = i0 - 1 + sl % (MH_NT + 1), jj = j0 - 1 + sl / (MH_NT + 1);
+        if (ii < 0 || ii >= nx || jj < 0 || jj >= ny) continue;
+        nt_cell(S, ii, jj, k, up, sl, keep[r], true);
+      }
+    }
+    __syncthreads();
+    if (node_ok) {
+      double mass = 0., fx = 0., fy = 0., fz = 0.;
+      const int s00 = ty * (MH_NT + 1) + tx;  // cell (i-1, j-1)
+      const bool okx0 = i >= 1, okx1 = i < nx, oky0 = j >= 1, oky1 = j < ny;
+      if (k >= 1) {  // cells k-1, corners 4..7 -> slots 0..3 of lo
+        nt_add(lo, s00, 2, okx0 && oky0, mass, fx, fy, fz);
+        nt_add(lo, s00 + 1, 3, okx1 && oky0, mass, fx, fy, fz);
+        nt_add(lo, s00 + MH_NT + 1, 1, okx0 && oky1, mass, fx, fy, fz);
+        nt_add(lo, s00 + MH_NT + 2, 0, okx1 && oky1, mass, fx, fy, fz);
+      }
+      if (k < nz) {  // cells k, corners 0..3
+        nt_add(up, s00, 2, okx0 && oky0, mass, fx, fy, fz);
+        nt_add(up, s00 + 1, 3, okx1 && oky0, mass, fx, fy, fz);
+        nt_add(up, s00 + MH_NT + 1, 1, okx0 && oky1, mass, fx, fy, fz);
+        nt_add(up, s00 + MH_NT + 2, 0, okx1 && oky1, mass, fx, fy, fz);
+      }
+      const long n = i + (long)S.g.nnx * (j + (long)S.g.nny * k);
+      node_finish<3>(S, n, i, j, k, dt, dth, mass, fx, fy, fz);
+    }
+    __syncthreads();
+    if (k < nz) {
+      for (int r = 0; r < 2; ++r) {
+        const int sl = tid + r * MH_BLOCK;
+        if (sl >= MH_NT_CELLS) break;
+        const int ii = i0 - 1 + sl % (MH_NT + 1), jj = j0 - 1 + sl / (MH_NT + 1);
+        if (ii < 0 || ii >= nx || jj < 0 || jj >= ny) continue;
+        for (int v = 0; v < 13; ++v) lo[v * MH_NT_CELLS + sl] = keep[r][v];
+      }
+    }
+  }
 }
 
 // ----------------------------------------------------------------------------- cells
@@ -954,6 +1084,10 @@ MH_WRAP(k_sweep_node, template <int DIM MH_COMMA int D>, <DIM MH_COMMA D>,
         MH_COMMA int psel MH_COMMA int first MH_COMMA int last, MH_COMMA psel MH_COMMA first MH_COMMA last)
 MH_WRAP(k_node_velocity, template <int DIM>, <DIM>, MH_COMMA int materialize, MH_COMMA materialize)
 MH_WRAP(k_finalize, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, MH_COMMA int usel, MH_COMMA usel)
+
+__global__ __launch_bounds__(MH_BLOCK) void k_node_lag3_tiled(MhDev S) {
+  k_node_lag3_tiled_body(S, blockIdx.x, blockIdx.y, blockIdx.z);
+}
 
 __global__ __launch_bounds__(MH_DT_BLOCK) void k_dt(MhDev S) { k_dt_body(S); }
 __global__ __launch_bounds__(MH_DT_BLOCK) void k_dt_batch(const MhDev* __restrict__ tab,

@@ -1,0 +1,158 @@
+"""Ensemble workloads on one GPU: a fixed batch of uneven meshes (all advancing together)
+and a served stream of hydro runs with continuous batching.
+
+Serving: runs arrive at Poisson times; each is admitted into the device ensemble as soon
+as a slot and enough HBM (its footprint) are free, keeps its state resident until its own
+final time, and is retired when done; admitted runs advance together in batched launches,
+so a run that arrives while others are on the GPU overlaps with them.
+"""
+import heapq
+import time
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import cases
+from . import types as T
+
+
+def batch_cases(n=256, seed=20261202, lo=64, hi=512, cycles=300):
+    """2D bi-material tubes with seeded states; sides log-uniform in [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        side = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        c = cases.riemann2d(side, side, rng, stiffened=False, cycles=cycles, name=f"batch{i}_{side}")
+        out.append(c)
+    return out
+
+
+def serve_requests(n=64, seed=20261202, rate=4.0, t_end=0.1, lo2=128, hi2=1024, lo3=32, hi3=160):
+    """Poisson arrivals (rate per second) of 2D / 3D stiffened-gas Riemann runs."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for i in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        if rng.random() < 0.5:
+            side = int(round(np.exp(rng.uniform(np.log(lo2), np.log(hi2)))))
+            c = cases.riemann2d(side, side, rng, stiffened=True, t_end=t_end, name=f"req{i}_2d{side}")
+        else:
+            side = int(round(np.exp(rng.uniform(np.log(lo3), np.log(hi3)))))
+            c = cases.riemann3d(side, rng, stiffened=True, t_end=t_end, name=f"req{i}_3d{side}")
+        reqs.append((t, c))
+    return reqs
+
+
+def run_batch(batch, graph=True, max_slots=None, budget=200e9, poll_every=30):
+    """All cases in one ensemble; returns (seconds, per-case iterations)."""
+    from .lib import Ensemble
+    ncyc = batch[0].max_cycles
+    E = Ensemble(max_slots or len(batch), budget)
+    slots = [E.add(c, max_iterations=ncyc) for c in batch]
+    assert all(s is not None for s in slots), "batch exceeds the HBM budget"
+    E.step(6, graph=graph)  # activation + graph capture outside the timed region? no: part of it
+    E.synchronize()
+    t0 = time.perf_counter()
+    done = 6
+    while done < ncyc:
+        k = min(poll_every, ncyc - done)
+        E.step(k, graph=graph)
+        done += k
+    E.synchronize()
+    dt = time.perf_counter() - t0
+    n, st, tm, it = E.poll()
+    E.close()
+    return dt, it[:len(batch)], st[:len(batch)]
+
+
+def run_back_to_back(batch, graph=6):
+    """Each case alone on the GPU, one after the other (the single-mesh path)."""
+    from .lib import GpuCase
+    t0 = time.perf_counter()
+    lat = []
+    for c in batch:
+        t1 = time.perf_counter()
+        g = GpuCase(c)
+        g.run(c.max_cycles, graph=graph)
+        g.close()
+        lat.append(time.perf_counter() - t1)
+    return time.perf_counter() - t0, lat
+
+
+@dataclass
+class Served:
+    arrival: float
+    admitted: float = -1.0
+    done: float = -1.0
+    iterations: int = 0
+    slot: int = -1
+
+
+def serve(reqs, budget=64e9, max_slots=64, step_cycles=6, graph=True):
+    """Continuous-batching request loop; times are wall-clock seconds from the first arrival."""
+    from .lib import Ensemble
+    E = Ensemble(max_slots, budget)
+    recs: List[Served] = [Served(a) for a, _ in reqs]
+    queue = list(range(len(reqs)))  # in arrival order
+    live = {}
+    t0 = time.perf_counter() - reqs[0][0]  # the clock starts at the first arrival
+    now = lambda: time.perf_counter() - t0
+    while queue or live:
+        t = now()
+        # admit every arrived request that fits (FIFO, by HBM footprint and slots)
+        while queue and reqs[queue[0]][0] <= t:
+            i = queue[0]
+            slot = E.add(reqs[i][1])
+            if slot is None:
+                break
+            recs[i].admitted = now()
+            recs[i].slot = slot
+            live[slot] = i
+            queue.pop(0)
+        if not live:
+            # idle until the next arrival
+            wait = reqs[queue[0]][0] - now()
+            if wait > 0:
+                time.sleep(min(wait, 0.05))
+            continue
+        E.step(step_cycles, graph=graph)
+        n, st, tm, it = E.poll()
+        for slot in list(live):
+            if st[slot] == T.MH_STOPPED or st[slot] < 0:
+                i = live.pop(slot)
+                recs[i].done = now()
+                recs[i].iterations = int(it[slot])
+                if st[slot] < 0:
+                    recs[i].iterations = -1
+                E.remove(slot)
+    E.close()
+    return recs
+
+
+def serve_back_to_back(reqs, graph=6):
+    """Baseline: the same stream served one run at a time on the single-mesh path."""
+    from .lib import GpuCase
+    t0 = time.perf_counter() - reqs[0][0]
+    recs = []
+    for a, c in reqs:
+        while time.perf_counter() - t0 < a:
+            time.sleep(0.001)
+        g = GpuCase(c)
+        while True:
+            g.queue(60, graph=graph)
+            st = g.synchronize()
+            if st != T.MH_OK:
+                break
+        r = Served(a, admitted=a, done=time.perf_counter() - t0, iterations=g.iteration)
+        g.close()
+        recs.append(r)
+    return recs
+
+
+def latency_stats(recs):
+    tts = np.array([r.done - r.arrival for r in recs])
+    span = max(r.done for r in recs) - min(r.arrival for r in recs)
+    return {"runs": len(recs), "runs_per_s": len(recs) / span, "p50_s": float(np.percentile(tts, 50)),
+            "p99_s": float(np.percentile(tts, 99)), "span_s": span}

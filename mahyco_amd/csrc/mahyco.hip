@@ -678,8 +678,9 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
   g.nnz = g.dim == 3 ? g.nz + 1 : 1;
   g.nc = (long)g.nx * g.ny * g.nz;
   g.nn = (long)g.nnx * g.nny * g.nnz;
-  h->nblk_c = (int)((g.nc + MH_BLOCK - 1) / MH_BLOCK);
-  h->nblk_n = (int)((g.nn + MH_BLOCK - 1) / MH_BLOCK);
+  // block counts are multiples of 8 (the XCD count) for the XCD-contiguous block order
+  h->nblk_c = (int)(((g.nc + MH_BLOCK - 1) / MH_BLOCK + 7) / 8 * 8);
+  h->nblk_n = (int)(((g.nn + MH_BLOCK - 1) / MH_BLOCK + 7) / 8 * 8);
   if (stream) {
     h->stream = stream;
     h->owns_stream = false;

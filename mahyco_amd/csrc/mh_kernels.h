@@ -607,21 +607,20 @@ __device__ __forceinline__ void k_sweep_cell_body(const MhDev& S, int lb, int us
 #pragma unroll
   for (int f = 0; f < 4; ++f) dxf[f] = (valid[f] && valid[f + 1]) ? xc[f + 1] - xc[f] : 0.;
   // the two faces of the cell along D: s = 0 back (outer normal -e_D), s = 1 front (+e_D)
-  bool inner[2];
+  bool inner[2], upprev[2];
   double area[2], vf[2], hup[2];
-  int up[2];
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
     inner[s] = s == 0 ? (a > 0) : (a < na - 1);
     area[s] = 0.;
     vf[s] = 0.;
     hup[s] = 0.;
-    up[s] = s;
+    upprev[s] = false;
     if (!inner[s]) continue;
     double xf;
     face_quant<DIM, D>(S, b + s * nD, area[s], xf, vf[s]);
-    up[s] = (vf[s] * dxf[s + 1] > 0.0) ? s - 1 : s;  // offset of the upwind cell
-    hup[s] = (double)S.order2 * (xf - xc[up[s] + 2]);
+    upprev[s] = vf[s] * dxf[s + 1] > 0.0;  // upwind cell: the one behind the face
+    hup[s] = (double)S.order2 * (xf - (upprev[s] ? xc[s + 1] : xc[s + 2]));
   }
   // cell faces in Arcane local order: 3D min face first; 2D x: max face first
   const int s0 = (DIM == 2 && D == 0) ? 1 : 0, s1 = 1 - s0;
@@ -648,7 +647,10 @@ __device__ __forceinline__ void k_sweep_cell_body(const MhDev& S, int lb, int us
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
       if (!inner[s]) continue;
-      const double pf = phi[up[s] + 2] + hup[s] * grad[up[s] + 1];
+      // Phi_face = Phi_up + h grad_up, up = offset s-1 (behind) or s (ahead)
+      const double pu = upprev[s] ? phi[s + 1] : phi[s + 2];
+      const double gu = upprev[s] ? grad[s] : grad[s + 1];
+      const double pf = pu + hup[s] * gu;
       flux[s] = od[s] * vf[s] * area[s] * dt * pf;
     }
     double ff = 0.;
@@ -1066,9 +1068,14 @@ __device__ __forceinline__ void k_dt_body(const MhDev& S) {
 // Single mesh: the state travels by value in the kernel arguments.  Batch: one launch
 // covers many meshes; block b works on mesh bmap[b].x, local block bmap[b].y, whose state
 // is tab[bmap[b].x] (meshes of one launch share DIM, M and the sweep direction).
+// Blocks are dispatched round-robin over the 8 XCDs, each with its own L2.  Single-mesh
+// grids are a multiple of 8 blocks and block b works on block (b % 8) * (grid / 8) + b / 8,
+// so each XCD sweeps one contiguous slab and the neighbours of a stencil stay in its L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) { return (b & 7) * (nb >> 3) + (b >> 3); }
+
 #define MH_WRAP(NAME, TPL, TARGS, PARAMS, ARGS)                                              \
   TPL __global__ __launch_bounds__(MH_BLOCK) void NAME(MhDev S PARAMS) {                     \
-    NAME##_body TARGS(S, (int)blockIdx.x ARGS);                                              \
+    NAME##_body TARGS(S, xcd_block((int)blockIdx.x, (int)gridDim.x) ARGS);                   \
   }                                                                                          \
   TPL __global__ __launch_bounds__(MH_BLOCK) void NAME##_batch(const MhDev* __restrict__ tab, \
                                                                const int2* __restrict__ bmap PARAMS) { \

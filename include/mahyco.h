@@ -92,6 +92,14 @@ int mahyco_apply_eos(MhHandle* h, int env);
  * speed of multi-environment cells.  Also run inside mahyco_remap / the Lagrange-only cycle. */
 int mahyco_compute_pression_moyenne(MhHandle* h);
 
+/* TimeHistory module with the EnvSummation service (time_history/EnvSummationService.cc:77-169):
+ * every `period` iterations the device appends a row [time, then per environment: mean CellMass,
+ * mean Pressure, volume-weighted Density, mass-weighted InternalEnergy, member count] without
+ * host synchronisation.  Enable before the first iteration; read returns the number of rows
+ * written and copies up to max_rows rows. */
+int mahyco_history_enable(MhHandle* h, int period, int max_rows);
+int mahyco_history_read(MhHandle* h, double* rows, int max_rows);
+
 /* ---- ensembles (no reference counterpart: many independent cases on one GPU) ---------
  * Cases of any size advance together, each with its own time step and stop rule: one
  * launch per kernel phase covers every active case of a (dimension, environment count)

@@ -428,8 +428,15 @@ void phase_cell(MhHandle* h) { MH_DISPATCH(launch_cell_t, h); }
 void phase_remap(MhHandle* h) {
   if (h->opt.with_projection) MH_DISPATCH(launch_sweeps_t, h);
 }
+template <int DIM, int M>
+void launch_hist_t(MhHandle* h) {
+  hipLaunchKernelGGL(k_hist_part<M>, dim3(h->nblk_c), dim3(MH_BLOCK), 0, h->stream, h->dev);
+  hipLaunchKernelGGL(k_hist_final<M>, dim3(1), dim3(MH_BLOCK), 0, h->stream, h->dev);
+}
+
 void phase_dt(MhHandle* h) {
   launch_dt(h);
+  if (h->dev.hist_period > 0) MH_DISPATCH(launch_hist_t, h);
   h->launched += 1;
 }
 
@@ -1283,6 +1290,35 @@ int mahyco_ensemble_get_field(MhEnsemble* e, int slot, int field, int env, int c
 double mahyco_ensemble_used_bytes(MhEnsemble* e) {
   MhEnsembleImpl* E = (MhEnsembleImpl*)e;
   return E ? (double)E->used : 0.;
+}
+
+int mahyco_history_enable(MhHandle* h, int period, int max_rows) {
+  if (!h || period < 0 || max_rows < 0) return MH_ERR_INVALID;
+  if (h->launched != 0) return MH_ERR_INVALID;  // before the first iteration
+  if (period == 0) {
+    h->dev.hist_period = 0;
+    return MH_OK;
+  }
+  const int w = 1 + 5 * h->m;
+  int ok = MH_OK;
+  ok |= dalloc(h, &h->dev.hist, (size_t)max_rows * w);
+  ok |= dalloc(h, &h->dev.hist_part, (size_t)h->nblk_c * 6 * h->m);
+  if (ok != MH_OK) return MH_ERR_NOMEM;
+  h->dev.hist_period = period;
+  h->dev.hist_max = max_rows;
+  return MH_OK;
+}
+
+int mahyco_history_read(MhHandle* h, double* rows, int max_rows) {
+  if (!h) return MH_ERR_INVALID;
+  MhScalars sc;
+  int st = read_scalars(h, &sc);
+  if (st != MH_OK) return st;
+  if (h->dev.hist_period <= 0) return 0;
+  const int n = std::min(sc.hist_rows, max_rows);
+  if (rows && n > 0)
+    HIPCHK(hipMemcpy(rows, h->dev.hist, sizeof(double) * (size_t)n * (1 + 5 * h->m), hipMemcpyDeviceToHost));
+  return sc.hist_rows;
 }
 
 int mahyco_profile_enable(MhHandle* h, int on) {

@@ -1064,6 +1064,92 @@ __device__ __forceinline__ void k_dt_body(const MhDev& S) {
   if (S.max_iter > 0 && sc->iteration >= S.max_iter) sc->stopped = 1;
 }
 
+
+// ----------------------------------------------------------------------------- time history
+// EnvSummationService::write (time_history/EnvSummationService.cc:77-169) every hist_period
+// iterations: per environment the plain mean of M_k and p_k over its cells, the
+// volume-weighted density and the mass-weighted internal energy.  Runs after k_dt; both
+// kernels do nothing unless the iteration just completed is a new multiple of the period.
+__device__ __forceinline__ bool hist_due(const MhDev& S) {
+  const MhScalars* sc = S.sc;
+  return S.hist_period > 0 && sc->status == 0 && sc->iteration > 0 &&
+         (sc->iteration % S.hist_period) == 0 && sc->iteration != sc->hist_last && sc->hist_rows < S.hist_max;
+}
+
+template <int M>
+__device__ __forceinline__ void k_hist_part_body(const MhDev& S, int lb) {
+  __shared__ double red[6 * M][MH_BLOCK / 64];
+  if (!hist_due(S)) return;
+  const long c = lb * (long)blockDim.x + threadIdx.x;
+  double acc[6 * M];
+  for (int v = 0; v < 6 * M; ++v) acc[v] = 0.;
+  if (c < S.g.nc) {
+    const uint8_t mk = S.mask[c];
+    const bool pure = __popc(mk) == 1;
+    const double V = S.with_proj ? S.VE[c] : S.Vl[c];
+    const long nc = S.g.nc;
+    for (int k = 0; k < M; ++k) {
+      if (!((mk >> k) & 1)) continue;
+      const long o = k * nc + c;
+      const double Mk = pure ? S.M[c] : S.Mk[o];
+      const double pk = pure ? S.p[c] : S.pk[o];
+      const double rk = pure ? S.rho[c] : S.rhok[o];
+      const double ek = pure ? S.e[c] : S.ek[o];
+      const double Vk = pure ? V : S.fk[o] * V;
+      acc[6 * k + 0] = 1.;
+      acc[6 * k + 1] = Mk;
+      acc[6 * k + 2] = pk;
+      acc[6 * k + 3] = rk * Vk;
+      acc[6 * k + 4] = Vk;
+      acc[6 * k + 5] = ek * Mk;
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int v = 0; v < 6 * M; ++v) {
+    double x = acc[v];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if (lane == 0) red[v][wid] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6 * M) {
+    double x = 0.;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) x += red[threadIdx.x][w];
+    S.hist_part[(long)lb * 6 * M + threadIdx.x] = x;
+  }
+}
+
+template <int M>
+__device__ __forceinline__ void k_hist_final_body(const MhDev& S) {
+  if (!hist_due(S)) return;
+  MhScalars* sc = S.sc;
+  const int v = threadIdx.x;
+  __shared__ double tot[6 * M];
+  if (v < 6 * M) {
+    double x = 0.;
+    for (int b = 0; b < S.nblk_c; ++b) x += S.hist_part[(long)b * 6 * M + v];
+    tot[v] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double* row = S.hist + (long)sc->hist_rows * (1 + 5 * M);
+  row[0] = sc->t;
+  for (int k = 0; k < M; ++k) {
+    const double cnt = tot[6 * k + 0];
+    row[1 + 5 * k + 0] = cnt > 0. ? tot[6 * k + 1] / cnt : 0.;
+    row[1 + 5 * k + 1] = cnt > 0. ? tot[6 * k + 2] / cnt : 0.;
+    row[1 + 5 * k + 2] = cnt > 0. ? tot[6 * k + 3] / tot[6 * k + 4] : 0.;
+    row[1 + 5 * k + 3] = cnt > 0. ? tot[6 * k + 5] / tot[6 * k + 1] : 0.;
+    row[1 + 5 * k + 4] = cnt;
+  }
+  sc->hist_rows = sc->hist_rows + 1;
+  sc->hist_last = sc->iteration;
+}
+
+template <int M>
+__global__ __launch_bounds__(MH_BLOCK) void k_hist_part(MhDev S) { k_hist_part_body<M>(S, (int)blockIdx.x); }
+template <int M>
+__global__ __launch_bounds__(MH_BLOCK) void k_hist_final(MhDev S) { k_hist_final_body<M>(S); }
+
 // ----------------------------------------------------------------------------- wrappers
 // Single mesh: the state travels by value in the kernel arguments.  Batch: one launch
 // covers many meshes; block b works on mesh bmap[b].x, local block bmap[b].y, whose state

@@ -38,7 +38,8 @@ struct MhScalars {
   int iteration;
   int stopped;  // final time reached: later iterations are no-ops
   int status;   // 0 or a negative MhStatus: later iterations are no-ops
-  int pad;
+  int hist_rows;  // time-history rows written
+  int hist_last;  // iteration of the last row
 };
 
 struct MhDev {
@@ -75,7 +76,12 @@ struct MhDev {
   double *Ua, *Ub, *dflux, *Ud, *Pa, *Pb;
   int nblk_c, nblk_n;  // blocks of MH_BLOCK threads over cells / nodes
   int max_iter;        // > 0: stop after this many iterations (fixed-horizon runs)
-  int pad2;
+  int hist_period;     // > 0: EnvSummation rows every hist_period iterations
+  double* hist;        // rows of 1 + 5m doubles (time, then per environment: mean M_k,
+                       // mean p_k, volume-weighted rho_k, mass-weighted e_k, member count)
+  double* hist_part;   // per-block partial sums, 6m doubles per block
+  int hist_max;        // capacity in rows
+  int pad3;
 };
 
 #endif

@@ -80,6 +80,10 @@ def load():
     lib.mahyco_ensemble_get_field.restype = C.c_int
     lib.mahyco_ensemble_used_bytes.argtypes = [H]
     lib.mahyco_ensemble_used_bytes.restype = C.c_double
+    lib.mahyco_history_enable.argtypes = [H, C.c_int, C.c_int]
+    lib.mahyco_history_enable.restype = C.c_int
+    lib.mahyco_history_read.argtypes = [H, C.c_void_p, C.c_int]
+    lib.mahyco_history_read.restype = C.c_int
     lib.mahyco_apply_eos.argtypes = [H, C.c_int]
     lib.mahyco_apply_eos.restype = C.c_int
     lib.mahyco_compute_pression_moyenne.argtypes = [H]
@@ -151,6 +155,20 @@ class GpuCase:
         if st != T.MH_OK:
             raise MahycoError(st, "mahyco_get_field")
         return out
+
+    def history_enable(self, period=10, max_rows=4096):
+        st = self.lib.mahyco_history_enable(self.h, period, max_rows)
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_history_enable")
+        self._hist_max = max_rows
+
+    def history(self):
+        """EnvSummation rows, same layout as the oracle's."""
+        w = 1 + 5 * len(self.case.envs)
+        n = self.lib.mahyco_history_read(self.h, None, 0)
+        buf = np.zeros(max(n, 1) * w)
+        self.lib.mahyco_history_read(self.h, buf.ctypes.data, n)
+        return buf[:n * w].reshape(n, w)
 
     def profile(self, on=True):
         self.lib.mahyco_profile_enable(self.h, int(on))

@@ -51,7 +51,8 @@ struct MhHandle {
   std::map<long, hipGraphExec_t> graphs;  // key: (cycles << 8) | (parity << 4) | sens
   std::vector<hipGraph_t> graph_defs;
   bool owns_stream = true;
-  bool node_tiled = true;  // 3D: z-marching node kernel (MAHYCO_NODE_KERNEL=flat selects the flat one)
+  bool node_tiled = true;   // 3D: z-marching node kernel (MAHYCO_NODE_KERNEL=flat selects the flat one)
+  bool sweep_march = true;  // y/z sweeps: marching cell kernel (MAHYCO_SWEEP=flat selects the flat one)
   MhScalars* ext_sc = nullptr;  // scalars living in an ensemble's array
   size_t bytes = 0;             // device bytes of the case
 };
@@ -348,6 +349,14 @@ void prof_end(MhHandle* h, ProfRec& r) {
     if ((h)->prof) prof_end((h), _r);                                    \
   } while (0)
 
+// blocks of the marching sweep along D (one thread per MH_MARCH_K cells), a multiple of 8
+int march_blocks(MhHandle* h, int D) {
+  const long na = D == 1 ? h->g.ny : h->g.nz;
+  const long nt = D == 1 ? (h->g.dim == 3 ? h->g.nz : 1) : h->g.ny;
+  const long threads = (long)h->g.nx * nt * ((na + MH_MARCH_K - 1) / MH_MARCH_K);
+  return (int)(((threads + MH_BLOCK - 1) / MH_BLOCK + 7) / 8 * 8);
+}
+
 template <int DIM, int M>
 void launch_sweeps_t(MhHandle* h) {
   const MhDev& S = h->dev;
@@ -366,10 +375,20 @@ void launch_sweeps_t(MhHandle* h) {
       LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
       LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0>), h->nblk_n, MH_BLOCK, S, sel, first, last);
     } else if (D == 1) {
-      LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+      if (h->sweep_march) {
+        LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell_march<DIM, 1, M>), march_blocks(h, 1), MH_BLOCK, S, sel, first);
+      } else {
+        LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+      }
       LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 1>), h->nblk_n, MH_BLOCK, S, sel, first, last);
     } else {
-      LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, (DIM == 3 ? 2 : 0), M>), h->nblk_c, MH_BLOCK, S, sel, first);
+      if constexpr (DIM == 3) {
+        if (h->sweep_march) {
+          LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell_march<3, 2, M>), march_blocks(h, 2), MH_BLOCK, S, sel, first);
+        } else {
+          LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<3, 2, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+        }
+      }
       LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, (DIM == 3 ? 2 : 0)>), h->nblk_n, MH_BLOCK, S, sel, first, last);
     }
   }
@@ -702,6 +721,8 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
   {
     const char* nk = getenv("MAHYCO_NODE_KERNEL");
     h->node_tiled = !(nk && strcmp(nk, "flat") == 0);
+    const char* sw = getenv("MAHYCO_SWEEP");
+    h->sweep_march = !(sw && strcmp(sw, "flat") == 0);
   }
   HostInit I;
   st = host_init(h, I);

@@ -28,6 +28,11 @@
 
 __device__ __forceinline__ bool mh_halted(const MhScalars* sc) { return sc->status != 0 || sc->stopped != 0; }
 
+// Blocks are dispatched round-robin over the 8 XCDs, each with its own L2.  Single-mesh
+// grids are a multiple of 8 blocks and block b works on block (b % 8) * (grid / 8) + b / 8,
+// so each XCD sweeps one contiguous slab and the neighbours of a stencil stay in its L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) { return (b & 7) * (nb >> 3) + (b >> 3); }
+
 // Arcane math::min / math::max
 __device__ __forceinline__ double amin(double a, double b) { return (a < b) ? a : b; }
 __device__ __forceinline__ double amax(double a, double b) { return (a < b) ? b : a; }
@@ -668,6 +673,149 @@ __device__ __forceinline__ void k_sweep_cell_body(const MhDev& S, int lb, int us
   }
 }
 
+
+// The same sweep for D = y or z, marching: a thread walks MH_MARCH_K consecutive cells along
+// D (consecutive threads are consecutive in x, so every load is coalesced) with a register
+// pipeline, so Phi, each face gradient, each limited slope and each face flux is formed
+// once instead of once per stencil member.  The step for cell a brings in cell a+2, forms
+// the gradient of face a+3/2, the slope of cell a+1 and the flux of face a+1/2, then
+// updates cell a with the fluxes of faces a-1/2 (carried, negated) and a+1/2.  Same
+// formulas in the same order as k_sweep_cell_body, hence the same bits.
+#define MH_MARCH_K 16
+template <int DIM, int D, int M>
+__global__ __launch_bounds__(MH_BLOCK) void k_sweep_cell_march(MhDev S, int usel, int first) {
+  constexpr int NV = 3 * M + 1;
+  const int nx = S.g.nx, ny = S.g.ny, nz = S.g.nz;
+  const int na = D == 1 ? ny : nz;
+  const int nt = (D == 1) ? (DIM == 3 ? nz : 1) : ny;
+  const int nchunk = (na + MH_MARCH_K - 1) / MH_MARCH_K;
+  const long tid = (long)xcd_block((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
+  if (tid >= (long)nx * nt * nchunk) return;
+  if (mh_halted(S.sc)) return;
+  const double* __restrict__ Uin = usel ? S.Ub : S.Ua;
+  double* __restrict__ Uout = usel ? S.Ua : S.Ub;
+  double* __restrict__ dflux = S.dflux;
+  const int i = (int)(tid % nx);
+  const long r = tid / nx;
+  const int t = (int)(r % nt);
+  const int ch = (int)(r / nt);
+  const int a0 = ch * MH_MARCH_K, a1 = min(a0 + MH_MARCH_K, na);
+  const long nc = S.g.nc;
+  const long nnx = S.g.nnx, nny = S.g.nny;
+  const long sD = D == 1 ? (long)nx : (long)nx * ny;
+  const long c_base = D == 1 ? i + (long)nx * ny * t : i + (long)nx * t;
+  const long nD = D == 1 ? nnx : nnx * nny;
+  const long n_base = D == 1 ? i + nnx * nny * t : i + nnx * t;
+  const double dt = S.sc->dt;
+  const double o2 = (double)S.order2;
+  // pipeline state: cells a (0) and a+1 (1); face a+1/2 (01); slope of cell a; flux of face a-1/2
+  double P0[NV], P1[NV], G01[NV], S0[NV], FB[NV];
+  double xc0 = 0., xc1 = 0., dx01 = 0.;
+  bool v0 = false, v1 = false;
+  for (int v = 0; v < NV; ++v) {
+    P0[v] = 0.;
+    P1[v] = 0.;
+    G01[v] = 0.;
+    S0[v] = 0.;
+    FB[v] = 0.;
+  }
+  {  // cell a0 - 2 enters as "a+1" of the first step (a = a0 - 3)
+    const int ap = a0 - 2;
+    v1 = ap >= 0 && ap < na;
+    if (v1) {
+      const long cc = c_base + ap * sD;
+      double den;
+      if (first) den = S.Vl[cc];
+      else {
+        den = 0.;
+        for (int k = 0; k < M; ++k) den += Uin[k * nc + cc];
+      }
+      for (int v = 0; v < NV; ++v) P1[v] = Uin[v * nc + cc] / den;
+      xc1 = cell_centre_d<DIM, D>(S, n_base + ap * nD);
+    }
+  }
+  for (int a = a0 - 3; a < a1; ++a) {
+    // cell a+2
+    const int ap = a + 2;
+    const bool vp = ap >= 0 && ap < na;
+    double P2[NV], xc2 = 0.;
+    if (vp) {
+      const long cc = c_base + ap * sD;
+      double den;
+      if (first) den = S.Vl[cc];
+      else {
+        den = 0.;
+        for (int k = 0; k < M; ++k) den += Uin[k * nc + cc];
+      }
+      for (int v = 0; v < NV; ++v) P2[v] = Uin[v * nc + cc] / den;
+      xc2 = cell_centre_d<DIM, D>(S, n_base + ap * nD);
+    } else {
+      for (int v = 0; v < NV; ++v) P2[v] = 0.;
+    }
+    // face a+3/2: gradient; cell a+1: limited slope
+    const bool f12 = v1 && vp;
+    const double dx12 = f12 ? xc2 - xc1 : 0.;
+    double G12[NV], S1[NV];
+    for (int v = 0; v < NV; ++v) {
+      G12[v] = f12 ? (P2[v] - P1[v]) / dx12 : 0.;
+      double g = 0.;
+      if (S.order2 && v1) {
+        const double gb = G01[v], gf = G12[v];
+        if (gb != 0.) g += 0.5 * (mh_flux_limiter(S.limiter, gf / gb) * gb);
+        if (gf != 0.) g += 0.5 * (mh_flux_limiter(S.limiter, gb / gf) * gf);
+      }
+      S1[v] = g;
+    }
+    // face a+1/2: upwind value and flux (outer normal +e_D for cell a)
+    const bool f01 = v0 && v1;
+    double FF[NV];
+    if (f01) {
+      double area, xf, vf;
+      face_quant<DIM, D>(S, n_base + (a + 1) * nD, area, xf, vf);
+      const bool upprev = vf * dx01 > 0.0;
+      const double h = o2 * (xf - (upprev ? xc0 : xc1));
+      for (int v = 0; v < NV; ++v) {
+        const double pf = (upprev ? P0[v] : P1[v]) + h * (upprev ? S0[v] : S1[v]);
+        FF[v] = 1. * vf * area * dt * pf;
+      }
+    } else {
+      for (int v = 0; v < NV; ++v) FF[v] = 0.;
+    }
+    // cell a: U -= (flux of face a-1/2) + (flux of face a+1/2)
+    if (a >= a0 && v0) {
+      const long c = c_base + a * sD;
+      const bool inb = a > 0, inf = a < na - 1;
+      for (int v = 0; v < NV; ++v) {
+        double ff = 0.;
+        if (inb) ff += FB[v];
+        if (inf) ff += FF[v];
+        double un = Uin[v * nc + c] - ff;
+        if (v >= M && v < 2 * M) {
+          double dd = 0.;
+          if (inb) dd += 0.5 * FB[v] * -1.;
+          if (inf) dd += 0.5 * FF[v] * 1.;
+          dflux[(v - M) * nc + c] = dd;
+        }
+        if (v >= M && v < 3 * M && un < 0.) un = 0.;
+        Uout[v * nc + c] = un;
+      }
+    }
+    // shift the pipeline by one cell
+    for (int v = 0; v < NV; ++v) {
+      P0[v] = P1[v];
+      P1[v] = P2[v];
+      G01[v] = G12[v];
+      S0[v] = S1[v];
+      FB[v] = -FF[v];
+    }
+    xc0 = xc1;
+    xc1 = xc2;
+    dx01 = dx12;
+    v0 = v1;
+    v1 = vp;
+  }
+}
+
 // One ADI direction, dual part for one node: limited nodal velocity gradients
 // (computeDualGradPhi UtilesRemap.cc:19-72, computeAndLimitGradPhiDual :369-400), back/front
 // mass fluxes gathered from the cells behind/ahead in face order, env order
@@ -1154,10 +1302,7 @@ __global__ __launch_bounds__(MH_BLOCK) void k_hist_final(MhDev S) { k_hist_final
 // Single mesh: the state travels by value in the kernel arguments.  Batch: one launch
 // covers many meshes; block b works on mesh bmap[b].x, local block bmap[b].y, whose state
 // is tab[bmap[b].x] (meshes of one launch share DIM, M and the sweep direction).
-// Blocks are dispatched round-robin over the 8 XCDs, each with its own L2.  Single-mesh
-// grids are a multiple of 8 blocks and block b works on block (b % 8) * (grid / 8) + b / 8,
-// so each XCD sweeps one contiguous slab and the neighbours of a stencil stay in its L2.
-__device__ __forceinline__ int xcd_block(int b, int nb) { return (b & 7) * (nb >> 3) + (b >> 3); }
+// Single-mesh grids use the XCD-contiguous block order of xcd_block.
 
 #define MH_WRAP(NAME, TPL, TARGS, PARAMS, ARGS)                                              \
   TPL __global__ __launch_bounds__(MH_BLOCK) void NAME(MhDev S PARAMS) {                     \

@@ -135,3 +135,23 @@ def test_node_kernel_tiled_equals_flat(monkeypatch, shape):
         g.close()
     for k in res["flat"]:
         assert bitwise_equal(res["flat"][k], res["tiled"][k]), k
+
+
+@pytest.mark.parametrize("make", [lambda: cases.config2_bisod3d(13, 37, 21), lambda: cases.config2_bisod3d(9, 2, 40),
+                                  lambda: cases.config2_bisod3d(7, 17, 3), lambda: cases.bitriplepoint(),
+                                  lambda: cases.sedov_2d()])
+def test_sweep_march_equals_flat(monkeypatch, make):
+    """The marching y/z sweep kernel and the flat one give identical bits (sizes cover
+    partial chunks, chunks starting past the third cell and sweeps 2 or 3 cells long)."""
+    from mahyco_amd.lib import GpuCase
+    res = {}
+    for mode in ("flat", "march"):
+        monkeypatch.setenv("MAHYCO_SWEEP", mode)
+        c = make()
+        print(f"[paths] sweep {mode} {c.name}", flush=True)
+        g = GpuCase(c)
+        g.run(18)
+        res[mode] = fields(g, len(c.envs), c.dim)
+        g.close()
+    for k in res["flat"]:
+        assert bitwise_equal(res["flat"][k], res["march"][k]), k

@@ -35,6 +35,8 @@ def kernel_bytes(kind, nc, nn, m, d):
     """Algorithmic bytes of one launch of each kernel (every field read or written once;
     neighbour reuse is free; the Eulerian grid is rebuilt from indices)."""
     nv = 3 * m + 1
+    if kind.startswith("k_sweep_"):
+        kind = kind[:-2]  # same bytes for every direction
     words = {
         # read u, x_prev (d each) and M, p, q per cell; write u, x_new (d each) and m_node
         "k_node_lag": nn * (4 * d + 1) + nc * 3,
@@ -50,6 +52,20 @@ def kernel_bytes(kind, nc, nn, m, d):
         "k_dt": 0,
     }
     return 8.0 * words[kind]
+
+
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of `kernel` at this mesh from the FETCH_SIZE + WRITE_SIZE profile
+    summary kept in profiles/ (scaled by a plain copy's counters), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_bisod3d.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("mesh") != [args.nx, args.ny, args.nz]:
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k["bytes_per_launch"]
 
 
 def cpu_baseline(case_fn, cycles, threads):
@@ -80,6 +96,8 @@ def main():
     ap.add_argument("--cpu-cycles", type=int, default=4)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run one plain device copy of MIB MiB (scale reference for HBM counters)")
     ap.add_argument("--mode", choices=["plain", "batched", "serve"], default="plain",
                     help="plain: single-mesh cycle (default); batched: 256 uneven 2D meshes, 300 "
                          "iterations each; serve: Poisson stream with continuous batching")
@@ -120,17 +138,22 @@ def main():
     print(f"[bench] timed {args.steps} iterations: {ms_per_step:.3f} ms/iteration", file=sys.stderr, flush=True)
 
     # per-kernel HIP-event durations on the library stream over a further profiled region
-    g.profile(True)
-    g.queue(args.profile_steps)
-    g.synchronize()
-    prof = g.profile_read()
-    g.profile(False)
-    kern = {k: {"ms_avg": v[0] / v[1], "launches": v[1], "ms_total": v[0]} for k, v in prof.items() if v[1]}
-    dominant = max(kern, key=lambda k: kern[k]["ms_total"])
-    b = kernel_bytes(dominant, nc, nn, m, d)
-    achieved = b / (kern[dominant]["ms_avg"] * 1e-3) / 1e9
-    for k, v in kern.items():
-        v["alg_GBs"] = kernel_bytes(k, nc, nn, m, d) / (v["ms_avg"] * 1e-3) / 1e9 if v["ms_avg"] > 0 else None
+    kern, roof = {}, None
+    if args.profile_steps > 0:
+        g.profile(True)
+        g.queue(args.profile_steps)
+        g.synchronize()
+        prof = g.profile_read()
+        g.profile(False)
+        kern = {k: {"ms_avg": v[0] / v[1], "launches": v[1], "ms_total": v[0]} for k, v in prof.items() if v[1]}
+        for k, v in kern.items():
+            v["alg_GBs"] = kernel_bytes(k, nc, nn, m, d) / (v["ms_avg"] * 1e-3) / 1e9 if v["ms_avg"] > 0 else None
+        dominant = max(kern, key=lambda k: kern[k]["ms_total"] * (0 if k == "k_dt" else 1))
+        b = kernel_bytes(dominant, nc, nn, m, d)
+        achieved = b / (kern[dominant]["ms_avg"] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant, args), "kernel": dominant,
+                "alg_bytes_per_launch": b}
     alg_cycle = 8.0 * algorithmic_words(nc, nn, m, d)
     cyc_gbs = alg_cycle / (ms_per_step * 1e-3) / 1e9
 
@@ -150,20 +173,34 @@ def main():
         "config": {"workload": "bisod3d_lagrange_adi_remap", "mesh": [args.nx, args.ny, args.nz],
                    "cells": nc, "environments": m, "limiter": "superbee", "order": 2, "cfl": 0.3,
                    "graph_cycles": args.graph},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant,
-                     "alg_bytes_per_launch": b},
+        "roofline": roof,
         "roofline_cycle": {"alg_bytes_per_cycle": alg_cycle, "achieved_GBs": cyc_gbs,
                            "frac": cyc_gbs / HBM_PEAK_GBS},
         "kernels": kern,
     }
     g.close()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     if not args.no_cpu:
         small = lambda: cases.config2_bisod3d(args.nx, args.ny, args.nz)
         out["cpu_baseline"] = cpu_baseline(small, args.cpu_cycles, args.cpu_threads)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
+
+
+def copy_calibration(mib):
+    """One device-to-device copy of known size: HBM counters of a profiled run are scaled by
+    what they read for it (bytes moved = 2 x size)."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"[bench] copy calibration: {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
+    del a, b
 
 
 def bench_batched(args):

@@ -17,9 +17,20 @@
 
 namespace {
 
-const char* kKernelNames[] = {"k_node_lag",   "k_cell_lag", "k_sweep_cell", "k_sweep_node",
-                              "k_node_velocity", "k_finalize", "k_dt"};
-enum { KN_NODE = 0, KN_CELL, KN_SWEEP_CELL, KN_SWEEP_NODE, KN_NODE_VEL, KN_FINAL, KN_DT, KN_COUNT };
+// profiled kernel groups (sweeps separately per direction: they run different code)
+const char* kKernelNames[] = {"k_node_lag",     "k_cell_lag",     "k_sweep_cell_x",  "k_sweep_cell_y",
+                              "k_sweep_cell_z", "k_sweep_node_x", "k_sweep_node_y",  "k_sweep_node_z",
+                              "k_node_velocity", "k_finalize",    "k_dt"};
+enum {
+  KN_NODE = 0,
+  KN_CELL,
+  KN_SWEEP_CELL,  // + direction
+  KN_SWEEP_NODE = KN_SWEEP_CELL + 3,
+  KN_NODE_VEL = KN_SWEEP_NODE + 3,
+  KN_FINAL,
+  KN_DT,
+  KN_COUNT
+};
 
 struct ProfRec {
   int kind;
@@ -372,24 +383,24 @@ void launch_sweeps_t(MhHandle* h) {
     const int first = s == 0, last = (s == nd - 1) && !first, sel = s & 1;
     const int D = dirs[s];
     if (D == 0) {
-      LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-      LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+      LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+      LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node<DIM, 0>), h->nblk_n, MH_BLOCK, S, sel, first, last);
     } else if (D == 1) {
       if (h->sweep_march) {
-        LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell_march<DIM, 1, M>), march_blocks(h, 1), MH_BLOCK, S, sel, first);
+        LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell_march<DIM, 1, M>), march_blocks(h, 1), MH_BLOCK, S, sel, first);
       } else {
-        LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+        LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, sel, first);
       }
-      LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 1>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+      LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node<DIM, 1>), h->nblk_n, MH_BLOCK, S, sel, first, last);
     } else {
       if constexpr (DIM == 3) {
         if (h->sweep_march) {
-          LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell_march<3, 2, M>), march_blocks(h, 2), MH_BLOCK, S, sel, first);
+          LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell_march<3, 2, M>), march_blocks(h, 2), MH_BLOCK, S, sel, first);
         } else {
-          LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<3, 2, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+          LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell<3, 2, M>), h->nblk_c, MH_BLOCK, S, sel, first);
         }
       }
-      LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, (DIM == 3 ? 2 : 0)>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+      LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node<DIM, (DIM == 3 ? 2 : 0)>), h->nblk_n, MH_BLOCK, S, sel, first, last);
     }
   }
   if (nd <= 1) LAUNCH(h, KN_NODE_VEL, (k_node_velocity<DIM>), h->nblk_n, MH_BLOCK, S, nd == 0 ? 1 : 0);

@@ -190,17 +190,23 @@ def main():
 
 
 def copy_calibration(mib):
-    """One device-to-device copy of known size: HBM counters of a profiled run are scaled by
-    what they read for it (bytes moved = 2 x size)."""
-    import torch
-    n = mib * (1 << 20) // 8
-    a = torch.ones(n, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    b.copy_(a)
-    torch.cuda.synchronize()
+    """One device-to-device copy of known size through the HIP runtime (its blit kernel):
+    HBM counters of a profiled run are scaled by what they read for it (mib MiB read,
+    mib MiB written)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")  # the runtime the library is linked with
+    n = mib * (1 << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    ok = (hip.hipMemset(a, 1, C.c_size_t(n)) == 0 and hip.hipDeviceSynchronize() == 0
+          and hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0 and hip.hipDeviceSynchronize() == 0)  # 3: device to device
+    hip.hipFree(a)
+    hip.hipFree(b)
+    if not ok:
+        raise RuntimeError("copy calibration failed")
     print(f"[bench] copy calibration: {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
-    del a, b
 
 
 def bench_batched(args):

@@ -479,10 +479,9 @@ __device__ __forceinline__ void k_cell_lag_body(const MhDev& S, int lb) {
         dxs = L / (cmix + cell_vmax<DIM>(S, b));
       }
     }
-    if (S.with_proj) {
+    if (S.with_proj)
       for (int v = 0; v < NV; ++v) U[v * nc + c] = u_out[v];
-      S.Vl[c] = V;
-    }
+    S.Vl[c] = V;
     S.L[c] = L;
   }
   if (!S.with_proj) block_min_store(dxs, S.partial, lb);
@@ -1110,9 +1109,14 @@ __device__ __forceinline__ void k_finalize_body(const MhDev& S, int lb, int usel
         S.Mk[o] = Mk[kk];
       }
     }
-    double fg = 0.;
-    for (int kk = 0; kk < M; ++kk)
-      if ((nm >> kk) & 1) fg = f[kk];
+    // the global volume fraction is the environment value of a single-environment cell;
+    // a mixed cell keeps the global value it had (Arcane leaves it untouched)
+    if (pure) {
+      double fg = 0.;
+      for (int kk = 0; kk < M; ++kk)
+        if ((nm >> kk) & 1) fg = f[kk];
+      S.fg[c] = fg;
+    }
     S.mask[c] = nm;
     S.rho[c] = rho_g;
     S.e[c] = e_g;
@@ -1120,7 +1124,6 @@ __device__ __forceinline__ void k_finalize_body(const MhDev& S, int lb, int usel
     S.c[c] = c_g;
     S.q[c] = q_g;
     S.M[c] = M_g;
-    S.fg[c] = fg;
     const int nx = S.g.nx, ny = S.g.ny;
     const int i = (int)(c % nx);
     const long r = c / nx;

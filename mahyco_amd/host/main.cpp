@@ -3,6 +3,7 @@
 // entry points (or fused cycles) and prints the final time and field checksums.
 //   mahyco_run <case> [--cycles N] [--fused] [--graph K]
 // cases: bisod_x  sod2d_lagrange  sedov2d  bisod3d:<nx>x<ny>x<nz>
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -95,7 +96,14 @@ int main(int argc, char** argv) {
     mod->computeCellMass();
     mod->computeNodeMass();
     if (fused) {
-      mod->cycle(cycles, graph);
+      // queued in chunks, each followed by a status check: iterations after the stop are
+      // no-ops on the device, but an unbounded request must not queue them all
+      const int chunk = graph > 0 ? 10 * graph : 60;
+      for (int done = 0; done < cycles;) {
+        const int k = std::min(chunk, cycles - done);
+        if (mod->cycle(k, graph) != MH_OK) break;
+        done += k;
+      }
     } else {
       for (int it = 0; it < cycles; ++it)
         if (!mod->iterate()) break;

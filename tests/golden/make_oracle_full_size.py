@@ -1,0 +1,48 @@
+"""Digests of the oracle's state after full-size runs, for bitwise GPU checks at full size.
+
+The oracle (oracle/, the CPU restatement of the reference) runs the two single-mesh
+workloads at their full sizes -- the 3D bi-material tube 256x128x128 for 100 iterations
+and the 3D Sedov blast 192^3 for 200 iterations -- with the correctly rounded cube root
+that the GPU library uses.  The fixture keeps, per parity field, the SHA-256 of the
+float64 array (global cell / node order), its sum, and the x-profile at j = k = 0, plus
+time, time step and iteration as exact hex floats.  A GPU run that reproduces every digest
+is bitwise identical to the oracle over the whole mesh.
+
+    python tests/golden/make_oracle_full_size.py [bisod3d] [sedov3d]   (minutes, 8 threads)
+"""
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from mahyco_amd import cases  # noqa: E402
+from tests.helpers import summarize  # noqa: E402
+
+WORKLOADS = {
+    "bisod3d": (lambda: cases.config2_bisod3d(256, 128, 128), 100),
+    "sedov3d": (lambda: cases.config3_sedov3d(192, 200), 200),
+}
+
+
+def main(names):
+    from oracle.pyoracle import OracleCase
+    for name in names:
+        make, cycles = WORKLOADS[name]
+        case = make()
+        t0 = time.time()
+        o = OracleCase(case, pow_mode=1, threads=os.cpu_count() or 1)
+        st = o.run(cycles)
+        res = {"workload": case.name, "cycles": cycles, "status": int(st), "iteration": o.iteration,
+               "time": float(o.time).hex(), "deltat": float(o.deltat).hex(), "fields": summarize(o, case)}
+        o.close()
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), f"oracle_{name}_full.json")
+        with open(path, "w") as fh:
+            json.dump(res, fh, indent=0)
+        print(f"{name}: {cycles} iterations, status {st}, {time.time() - t0:.0f} s -> {path}", flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:] or list(WORKLOADS))

@@ -1,4 +1,6 @@
 """Shared helpers of the test suite: field lists and comparisons."""
+import hashlib
+
 import numpy as np
 
 from mahyco_amd import types as T
@@ -49,3 +51,20 @@ def bitwise_equal(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     return a.shape == b.shape and bool(np.all((a == b) | (np.isnan(a) & np.isnan(b))))
+
+
+def digest(arr):
+    """SHA-256 of a float64 array's bytes."""
+    a = np.ascontiguousarray(arr, dtype=np.float64)
+    return hashlib.sha256(a.tobytes()).hexdigest()
+
+
+def summarize(sim, case):
+    """Per parity field: digest, exact sum and the x-profile at j = k = 0 (hex floats)."""
+    f = fields(sim, len(case.envs), case.dim)
+    nx = case.cell_shape[-1]
+    out = {}
+    for k, v in f.items():
+        row = v[:nx] if v.size == case.nb_cells else v[:nx + 1]
+        out[k] = {"sha256": digest(v), "sum": float(np.sum(v)).hex(), "row": [float(x).hex() for x in row]}
+    return out

@@ -63,7 +63,7 @@ struct MhHandle {
   std::vector<hipGraph_t> graph_defs;
   bool owns_stream = true;
   bool node_tiled = true;   // 3D: z-marching node kernel (MAHYCO_NODE_KERNEL=flat selects the flat one)
-  bool sweep_march = true;  // y/z sweeps: marching cell kernel (MAHYCO_SWEEP=flat selects the flat one)
+  bool sweep_march = true;  // row-segment x sweep and marching y/z sweeps (MAHYCO_SWEEP=flat: one thread per cell)
   MhScalars* ext_sc = nullptr;  // scalars living in an ensemble's array
   size_t bytes = 0;             // device bytes of the case
 };
@@ -368,6 +368,13 @@ int march_blocks(MhHandle* h, int D) {
   return (int)(((threads + MH_BLOCK - 1) / MH_BLOCK + 7) / 8 * 8);
 }
 
+// blocks of the row-segment x sweep (one wavefront per MH_ROW_OUT cells of a row), a multiple of 8
+int row_blocks(MhHandle* h) {
+  const long waves = (long)h->g.ny * h->g.nz * ((h->g.nx + MH_ROW_OUT - 1) / MH_ROW_OUT);
+  const long wpb = MH_BLOCK / 64;
+  return (int)(((waves + wpb - 1) / wpb + 7) / 8 * 8);
+}
+
 template <int DIM, int M>
 void launch_sweeps_t(MhHandle* h) {
   const MhDev& S = h->dev;
@@ -383,7 +390,11 @@ void launch_sweeps_t(MhHandle* h) {
     const int first = s == 0, last = (s == nd - 1) && !first, sel = s & 1;
     const int D = dirs[s];
     if (D == 0) {
-      LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+      if (h->sweep_march) {
+        LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell_row<DIM, M>), row_blocks(h), MH_BLOCK, S, sel, first);
+      } else {
+        LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+      }
       LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node<DIM, 0>), h->nblk_n, MH_BLOCK, S, sel, first, last);
     } else if (D == 1) {
       if (h->sweep_march) {

@@ -673,6 +673,98 @@ __device__ __forceinline__ void k_sweep_cell_body(const MhDev& S, int lb, int us
 }
 
 
+// The x sweep with one wavefront per row segment: lane l holds cell i = 60 s - 2 + l of a
+// row, so the stencil neighbours along x are the neighbouring lanes.  Phi, the gradient of
+// the back face, the limited slope and the flux of the back face are formed once per cell
+// and passed to the neighbour with a lane shuffle (the front-face value of cell i is the
+// back-face value of cell i+1).  Lanes 2..61 write their cell; the two lanes at each end
+// only feed the stencil.  Same formulas in the same order as k_sweep_cell_body.
+#define MH_ROW_OUT 60
+template <int DIM, int M>
+__global__ __launch_bounds__(MH_BLOCK) void k_sweep_cell_row(MhDev S, int usel, int first) {
+  constexpr int NV = 3 * M + 1;
+  const int nx = S.g.nx;
+  const long rows = (long)S.g.ny * S.g.nz;
+  const int wpr = (nx + MH_ROW_OUT - 1) / MH_ROW_OUT;
+  const int lane = threadIdx.x & 63;
+  const long gw = (long)xcd_block((int)blockIdx.x, (int)gridDim.x) * (MH_BLOCK / 64) + (threadIdx.x >> 6);
+  if (gw >= rows * wpr) return;  // whole wavefront
+  if (mh_halted(S.sc)) return;
+  const double* __restrict__ Uin = usel ? S.Ub : S.Ua;
+  double* __restrict__ Uout = usel ? S.Ua : S.Ub;
+  double* __restrict__ dflux = S.dflux;
+  const long row = gw / wpr;
+  const int i = (int)(gw % wpr) * MH_ROW_OUT - 2 + lane;
+  const bool valid = i >= 0 && i < nx;
+  const bool validp = i - 1 >= 0 && i - 1 < nx;
+  const bool out = valid && lane >= 2 && lane < 2 + MH_ROW_OUT;
+  const long nc = S.g.nc;
+  const long c = row * nx + i;
+  const int j = (int)(row % S.g.ny);
+  const int k = (int)(row / S.g.ny);
+  const long b = i + (long)S.g.nnx * (j + (long)S.g.nny * k);
+  const double dt = S.sc->dt;
+  double den = 1., xc = 0.;
+  if (valid) {
+    if (first) {
+      den = S.Vl[c];
+    } else {
+      double sv = 0.;
+      for (int kk = 0; kk < M; ++kk) sv += Uin[kk * nc + c];
+      den = sv;
+    }
+    xc = cell_centre_d<DIM, 0>(S, b);
+  }
+  const double xcp = __shfl_up(xc, 1);
+  const double dxb = (valid && validp) ? xc - xcp : 0.;
+  // back face (node column i): inner when 0 < i < nx
+  const bool inb = valid && i > 0;
+  const bool inf = valid && i < nx - 1;
+  double area = 0., xf = 0., vf = 0.;
+  if (inb) face_quant<DIM, 0>(S, b, area, xf, vf);
+  const bool upprev = vf * dxb > 0.0;
+  const double hb = (double)S.order2 * (xf - (upprev ? xcp : xc));
+  for (int v = 0; v < NV; ++v) {
+    const double uv = valid ? Uin[v * nc + c] : 0.;
+    const double phi = valid ? uv / den : 0.;
+    const double phip = __shfl_up(phi, 1);
+    const double gb = (valid && validp) ? (phi - phip) / dxb : 0.;
+    const double gf = __shfl_down(gb, 1);
+    double g = 0.;
+    if (S.order2 && valid) {
+      if (gb != 0.) g += 0.5 * (mh_flux_limiter(S.limiter, gf / gb) * gb);
+      if (gf != 0.) g += 0.5 * (mh_flux_limiter(S.limiter, gb / gf) * gf);
+    }
+    const double gp = __shfl_up(g, 1);
+    double F = 0.;  // flux of the back face with outer normal +e_x (for cell i-1)
+    if (inb) {
+      const double pf = (upprev ? phip : phi) + hb * (upprev ? gp : g);
+      F = 1. * vf * area * dt * pf;
+    }
+    const double Ff = __shfl_down(F, 1);
+    const double fb = -F;  // the back face seen from cell i (outer normal -e_x)
+    double ff = 0.;
+    double dd = 0.;
+    if (DIM == 2) {  // 2D: the max-x face comes first in the cell's face order
+      if (inf) ff += Ff;
+      if (inb) ff += fb;
+      if (inf) dd += 0.5 * Ff * 1.;
+      if (inb) dd += 0.5 * fb * -1.;
+    } else {
+      if (inb) ff += fb;
+      if (inf) ff += Ff;
+      if (inb) dd += 0.5 * fb * -1.;
+      if (inf) dd += 0.5 * Ff * 1.;
+    }
+    if (out) {
+      double un = uv - ff;
+      if (v >= M && v < 2 * M) dflux[(v - M) * nc + c] = dd;
+      if (v >= M && v < 3 * M && un < 0.) un = 0.;
+      Uout[v * nc + c] = un;
+    }
+  }
+}
+
 // The same sweep for D = y or z, marching: a thread walks MH_MARCH_K consecutive cells along
 // D (consecutive threads are consecutive in x, so every load is coalesced) with a register
 // pipeline, so Phi, each face gradient, each limited slope and each face flux is formed

@@ -63,7 +63,8 @@ struct MhHandle {
   std::vector<hipGraph_t> graph_defs;
   bool owns_stream = true;
   bool node_tiled = true;   // 3D: z-marching node kernel (MAHYCO_NODE_KERNEL=flat selects the flat one)
-  bool sweep_march = true;  // row-segment x sweep and marching y/z sweeps (MAHYCO_SWEEP=flat: one thread per cell)
+  bool sweep_march = true;
+  int march_groups = 4;     // slot groups of the marching sweep (MAHYCO_MARCH_GROUPS=1 or 4)  // row-segment x sweep and marching y/z sweeps (MAHYCO_SWEEP=flat: one thread per cell)
   MhScalars* ext_sc = nullptr;  // scalars living in an ensemble's array
   size_t bytes = 0;             // device bytes of the case
 };
@@ -360,12 +361,13 @@ void prof_end(MhHandle* h, ProfRec& r) {
     if ((h)->prof) prof_end((h), _r);                                    \
   } while (0)
 
-// blocks of the marching sweep along D (one thread per MH_MARCH_K cells), a multiple of 8
+// blocks of the marching sweep along D (one thread per MH_MARCH_K cells and slot group), a multiple of 8
 int march_blocks(MhHandle* h, int D) {
   const long na = D == 1 ? h->g.ny : h->g.nz;
   const long nt = D == 1 ? (h->g.dim == 3 ? h->g.nz : 1) : h->g.ny;
-  const long threads = (long)h->g.nx * nt * ((na + MH_MARCH_K - 1) / MH_MARCH_K);
-  return (int)(((threads + MH_BLOCK - 1) / MH_BLOCK + 7) / 8 * 8);
+  const long cols = (long)h->g.nx * nt * ((na + MH_MARCH_K - 1) / MH_MARCH_K);
+  const long cpb = MH_BLOCK / h->march_groups;
+  return (int)(((cols + cpb - 1) / cpb + 7) / 8 * 8);
 }
 
 // blocks of the row-segment x sweep (one wavefront per MH_ROW_OUT cells of a row), a multiple of 8
@@ -373,6 +375,28 @@ int row_blocks(MhHandle* h) {
   const long waves = (long)h->g.ny * h->g.nz * ((h->g.nx + MH_ROW_OUT - 1) / MH_ROW_OUT);
   const long wpb = MH_BLOCK / 64;
   return (int)(((waves + wpb - 1) / wpb + 7) / 8 * 8);
+}
+
+// blocks of the dual sweeps: row segments of 60 nodes along x, or node lines marched in
+// chunks of MH_MARCH_K along y / z; multiples of 8
+int node_row_blocks(MhHandle* h) {
+  const long waves = (long)h->g.nny * h->g.nnz * ((h->g.nnx + MH_ROW_OUT - 1) / MH_ROW_OUT);
+  const long wpb = MH_BLOCK / 64;
+  return (int)(((waves + wpb - 1) / wpb + 7) / 8 * 8);
+}
+int node_march_blocks(MhHandle* h, int D) {
+  const long na = D == 1 ? h->g.ny : h->g.nz;
+  const long nt = D == 1 ? (h->g.dim == 3 ? h->g.nnz : 1) : h->g.nny;
+  const long threads = (long)h->g.nnx * nt * ((na + 1 + MH_MARCH_K - 1) / MH_MARCH_K);
+  return (int)(((threads + MH_BLOCK - 1) / MH_BLOCK + 7) / 8 * 8);
+}
+
+template <int DIM, int D, int M>
+void launch_march(MhHandle* h, int sel, int first) {
+  const MhDev& S = h->dev;
+  const int nb = march_blocks(h, D);
+  if (h->march_groups == 4) LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell_march<DIM, D, M, 4>), nb, MH_BLOCK, S, sel, first);
+  else LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell_march<DIM, D, M, 1>), nb, MH_BLOCK, S, sel, first);
 }
 
 template <int DIM, int M>
@@ -395,23 +419,28 @@ void launch_sweeps_t(MhHandle* h) {
       } else {
         LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
       }
-      LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node<DIM, 0>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+      if (h->sweep_march) LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node_row<DIM>), node_row_blocks(h), MH_BLOCK, S, sel, first, last);
+      else LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node<DIM, 0>), h->nblk_n, MH_BLOCK, S, sel, first, last);
     } else if (D == 1) {
       if (h->sweep_march) {
-        LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell_march<DIM, 1, M>), march_blocks(h, 1), MH_BLOCK, S, sel, first);
+        launch_march<DIM, 1, M>(h, sel, first);
       } else {
         LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, sel, first);
       }
-      LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node<DIM, 1>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+      if (h->sweep_march) LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node_march<DIM, 1>), node_march_blocks(h, 1), MH_BLOCK, S, sel, first, last);
+      else LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node<DIM, 1>), h->nblk_n, MH_BLOCK, S, sel, first, last);
     } else {
       if constexpr (DIM == 3) {
         if (h->sweep_march) {
-          LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell_march<3, 2, M>), march_blocks(h, 2), MH_BLOCK, S, sel, first);
+          launch_march<3, 2, M>(h, sel, first);
         } else {
           LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell<3, 2, M>), h->nblk_c, MH_BLOCK, S, sel, first);
         }
       }
-      LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node<DIM, (DIM == 3 ? 2 : 0)>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+      if constexpr (DIM == 3) {
+        if (h->sweep_march) LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node_march<3, 2>), node_march_blocks(h, 2), MH_BLOCK, S, sel, first, last);
+        else LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node<3, 2>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+      }
     }
   }
   if (nd <= 1) LAUNCH(h, KN_NODE_VEL, (k_node_velocity<DIM>), h->nblk_n, MH_BLOCK, S, nd == 0 ? 1 : 0);
@@ -745,6 +774,8 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
     h->node_tiled = !(nk && strcmp(nk, "flat") == 0);
     const char* sw = getenv("MAHYCO_SWEEP");
     h->sweep_march = !(sw && strcmp(sw, "flat") == 0);
+    const char* mg = getenv("MAHYCO_MARCH_GROUPS");
+    h->march_groups = (mg && atoi(mg) == 1) ? 1 : 4;
   }
   HostInit I;
   st = host_init(h, I);

@@ -770,17 +770,23 @@ __global__ __launch_bounds__(MH_BLOCK) void k_sweep_cell_row(MhDev S, int usel, 
 // pipeline, so Phi, each face gradient, each limited slope and each face flux is formed
 // once instead of once per stencil member.  The step for cell a brings in cell a+2, forms
 // the gradient of face a+3/2, the slope of cell a+1 and the flux of face a+1/2, then
-// updates cell a with the fluxes of faces a-1/2 (carried, negated) and a+1/2.  Same
-// formulas in the same order as k_sweep_cell_body, hence the same bits.
+// updates cell a with the fluxes of faces a-1/2 (carried, negated) and a+1/2.  With G > 1
+// the G wavefronts of a block share the same columns and split the remap slots
+// (wavefront g takes slots g, g+G, ...), which cuts the registers per thread so that more
+// wavefronts hide the memory latency.  Same formulas in the same order as
+// k_sweep_cell_body, hence the same bits.
 #define MH_MARCH_K 16
-template <int DIM, int D, int M>
+template <int DIM, int D, int M, int G>
 __global__ __launch_bounds__(MH_BLOCK) void k_sweep_cell_march(MhDev S, int usel, int first) {
   constexpr int NV = 3 * M + 1;
+  constexpr int NQ = (NV + G - 1) / G;  // slots per thread
+  constexpr int CPB = MH_BLOCK / G;     // columns per block
   const int nx = S.g.nx, ny = S.g.ny, nz = S.g.nz;
   const int na = D == 1 ? ny : nz;
   const int nt = (D == 1) ? (DIM == 3 ? nz : 1) : ny;
   const int nchunk = (na + MH_MARCH_K - 1) / MH_MARCH_K;
-  const long tid = (long)xcd_block((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
+  const int grp = (int)threadIdx.x / CPB;
+  const long tid = (long)xcd_block((int)blockIdx.x, (int)gridDim.x) * CPB + (threadIdx.x % CPB);
   if (tid >= (long)nx * nt * nchunk) return;
   if (mh_halted(S.sc)) return;
   const double* __restrict__ Uin = usel ? S.Ub : S.Ua;
@@ -800,91 +806,94 @@ __global__ __launch_bounds__(MH_BLOCK) void k_sweep_cell_march(MhDev S, int usel
   const double dt = S.sc->dt;
   const double o2 = (double)S.order2;
   // pipeline state: cells a (0) and a+1 (1); face a+1/2 (01); slope of cell a; flux of face a-1/2
-  double P0[NV], P1[NV], G01[NV], S0[NV], FB[NV];
+  double P0[NQ], P1[NQ], G01[NQ], S0[NQ], FB[NQ];
   double xc0 = 0., xc1 = 0., dx01 = 0.;
   bool v0 = false, v1 = false;
-  for (int v = 0; v < NV; ++v) {
-    P0[v] = 0.;
-    P1[v] = 0.;
-    G01[v] = 0.;
-    S0[v] = 0.;
-    FB[v] = 0.;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    P0[q] = 0.;
+    P1[q] = 0.;
+    G01[q] = 0.;
+    S0[q] = 0.;
+    FB[q] = 0.;
   }
+  // Phi of the slots of this thread at cell ap (and its centre along D)
+  auto load_cell = [&](int ap, double* P, double& xc) {
+    const long cc = c_base + ap * sD;
+    double den;
+    if (first) den = S.Vl[cc];
+    else {
+      den = 0.;
+      for (int k = 0; k < M; ++k) den += Uin[k * nc + cc];
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int v = grp + G * q;
+      P[q] = v < NV ? Uin[v * nc + cc] / den : 0.;
+    }
+    xc = cell_centre_d<DIM, D>(S, n_base + ap * nD);
+  };
   {  // cell a0 - 2 enters as "a+1" of the first step (a = a0 - 3)
     const int ap = a0 - 2;
     v1 = ap >= 0 && ap < na;
-    if (v1) {
-      const long cc = c_base + ap * sD;
-      double den;
-      if (first) den = S.Vl[cc];
-      else {
-        den = 0.;
-        for (int k = 0; k < M; ++k) den += Uin[k * nc + cc];
-      }
-      for (int v = 0; v < NV; ++v) P1[v] = Uin[v * nc + cc] / den;
-      xc1 = cell_centre_d<DIM, D>(S, n_base + ap * nD);
-    }
+    if (v1) load_cell(ap, P1, xc1);
   }
   for (int a = a0 - 3; a < a1; ++a) {
     // cell a+2
     const int ap = a + 2;
     const bool vp = ap >= 0 && ap < na;
-    double P2[NV], xc2 = 0.;
-    if (vp) {
-      const long cc = c_base + ap * sD;
-      double den;
-      if (first) den = S.Vl[cc];
-      else {
-        den = 0.;
-        for (int k = 0; k < M; ++k) den += Uin[k * nc + cc];
-      }
-      for (int v = 0; v < NV; ++v) P2[v] = Uin[v * nc + cc] / den;
-      xc2 = cell_centre_d<DIM, D>(S, n_base + ap * nD);
-    } else {
-      for (int v = 0; v < NV; ++v) P2[v] = 0.;
-    }
+    double P2[NQ], xc2 = 0.;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) P2[q] = 0.;
+    if (vp) load_cell(ap, P2, xc2);
     // face a+3/2: gradient; cell a+1: limited slope
     const bool f12 = v1 && vp;
     const double dx12 = f12 ? xc2 - xc1 : 0.;
-    double G12[NV], S1[NV];
-    for (int v = 0; v < NV; ++v) {
-      G12[v] = f12 ? (P2[v] - P1[v]) / dx12 : 0.;
+    double G12[NQ], S1[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      G12[q] = f12 ? (P2[q] - P1[q]) / dx12 : 0.;
       double g = 0.;
       if (S.order2 && v1) {
-        const double gb = G01[v], gf = G12[v];
+        const double gb = G01[q], gf = G12[q];
         if (gb != 0.) g += 0.5 * (mh_flux_limiter(S.limiter, gf / gb) * gb);
         if (gf != 0.) g += 0.5 * (mh_flux_limiter(S.limiter, gb / gf) * gf);
       }
-      S1[v] = g;
+      S1[q] = g;
     }
     // face a+1/2: upwind value and flux (outer normal +e_D for cell a)
     const bool f01 = v0 && v1;
-    double FF[NV];
+    double FF[NQ];
     if (f01) {
       double area, xf, vf;
       face_quant<DIM, D>(S, n_base + (a + 1) * nD, area, xf, vf);
       const bool upprev = vf * dx01 > 0.0;
       const double h = o2 * (xf - (upprev ? xc0 : xc1));
-      for (int v = 0; v < NV; ++v) {
-        const double pf = (upprev ? P0[v] : P1[v]) + h * (upprev ? S0[v] : S1[v]);
-        FF[v] = 1. * vf * area * dt * pf;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const double pf = (upprev ? P0[q] : P1[q]) + h * (upprev ? S0[q] : S1[q]);
+        FF[q] = 1. * vf * area * dt * pf;
       }
     } else {
-      for (int v = 0; v < NV; ++v) FF[v] = 0.;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) FF[q] = 0.;
     }
     // cell a: U -= (flux of face a-1/2) + (flux of face a+1/2)
     if (a >= a0 && v0) {
       const long c = c_base + a * sD;
       const bool inb = a > 0, inf = a < na - 1;
-      for (int v = 0; v < NV; ++v) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int v = grp + G * q;
+        if (v >= NV) continue;
         double ff = 0.;
-        if (inb) ff += FB[v];
-        if (inf) ff += FF[v];
+        if (inb) ff += FB[q];
+        if (inf) ff += FF[q];
         double un = Uin[v * nc + c] - ff;
         if (v >= M && v < 2 * M) {
           double dd = 0.;
-          if (inb) dd += 0.5 * FB[v] * -1.;
-          if (inf) dd += 0.5 * FF[v] * 1.;
+          if (inb) dd += 0.5 * FB[q] * -1.;
+          if (inf) dd += 0.5 * FF[q] * 1.;
           dflux[(v - M) * nc + c] = dd;
         }
         if (v >= M && v < 3 * M && un < 0.) un = 0.;
@@ -892,16 +901,267 @@ __global__ __launch_bounds__(MH_BLOCK) void k_sweep_cell_march(MhDev S, int usel
       }
     }
     // shift the pipeline by one cell
-    for (int v = 0; v < NV; ++v) {
-      P0[v] = P1[v];
-      P1[v] = P2[v];
-      G01[v] = G12[v];
-      S0[v] = S1[v];
-      FB[v] = -FF[v];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      P0[q] = P1[q];
+      P1[q] = P2[q];
+      G01[q] = G12[q];
+      S0[q] = S1[q];
+      FB[q] = -FF[q];
     }
     xc0 = xc1;
     xc1 = xc2;
     dx01 = dx12;
+    v0 = v1;
+    v1 = vp;
+  }
+}
+
+// Mass flux through the dual faces between node position a and a+1 along D: the sum, in
+// the order of k_sweep_node_body, of the cell mass fluxes of the 2^(d-1) cells at
+// position a around the node line (it1, it2) -- the "front" flux of node a and the "back"
+// flux of node a+1 (DualPhaseRemap.cc:75-108).  0 when position a holds no cells.
+template <int DIM, int D>
+__device__ __forceinline__ double dual_edge_flux(const MhDev& S, int a, int it1, int it2) {
+  const int na = D == 0 ? S.g.nx : D == 1 ? S.g.ny : S.g.nz;
+  if (a < 0 || a >= na) return 0.;
+  const double w = DIM == 3 ? 0.25 : 0.5;
+  const long nc = S.g.nc;
+  const int n1 = D == 0 ? S.g.ny : S.g.nx;
+  const int n2 = D == 2 ? S.g.ny : S.g.nz;
+  const long st1 = D == 0 ? (long)S.g.nx : 1;
+  const long st2 = D == 2 ? (long)S.g.nx : (long)S.g.nx * S.g.ny;
+  const long cbase = D == 0 ? (long)a : D == 1 ? (long)a * S.g.nx : (long)a * S.g.nx * S.g.ny;
+  const int lo2 = DIM == 3 ? it2 - 1 : 0, hi2 = DIM == 3 ? it2 : 0;
+  double f = 0.;
+  for (int p2 = lo2; p2 <= hi2; ++p2) {
+    if (DIM == 3 && (p2 < 0 || p2 >= n2)) continue;
+    for (int p1 = it1 - 1; p1 <= it1; ++p1) {
+      if (p1 < 0 || p1 >= n1) continue;
+      const long cf = cbase + p1 * st1 + (DIM == 3 ? p2 * st2 : 0);
+      for (int kk = 0; kk < S.m; ++kk) f += w * S.dflux[kk * nc + cf];
+    }
+  }
+  return f;
+}
+
+// Limited nodal gradient from the edge gradients behind (gb) and ahead (gf)
+// (computeAndLimitGradPhiDual, UtilesRemap.cc:369-400), in the order of k_sweep_node_body.
+__device__ __forceinline__ double dual_limited(const MhDev& S, double gb, double gf) {
+  double g = 0.;
+  if (gb != 0.) g += 0.5 * mh_flux_limiter(S.limiter, gf / gb) * gb;
+  if (gf != 0.) g += 0.5 * mh_flux_limiter(S.limiter, gb / gf) * gf;
+  return g;
+}
+
+// Dual update of one node from its neighbourhood along D (DualPhaseRemap.cc:110-257):
+// Pm/P0/Pp, Xm/X0/Xp, Gm/G0/Gp are phi, coordinate and limited gradient of the nodes a-1,
+// a, a+1; back/front the dual mass fluxes.  Writes Ud, phi and, on the last sweep, u.
+template <int DIM, int D>
+__device__ __forceinline__ void dual_node_update(const MhDev& S, long n, double* U, const double* Pm,
+                                                 const double* P0, const double* Pp, double Xm, double X0,
+                                                 double Xp, const double* Gm, const double* G0,
+                                                 const double* Gp, double back, double front, double* Pout,
+                                                 int last) {
+  constexpr int NC = DIM;
+  const long nn = S.g.nn;
+  const double dt = S.sc->dt;
+  U[NC] += back - front;
+  const double o2 = (double)S.order2;
+  const double ufront = 0.5 * (P0[D] + Pp[D]);
+  const int sf = ufront > 0 ? 1 : -1;
+  const bool fo = front < 0.;
+  const double uback = 0.5 * (Pm[D] + P0[D]);
+  const int sb = uback > 0 ? 1 : -1;
+  const bool bo = back > 0.;
+#pragma unroll
+  for (int cc = 0; cc < NC; ++cc) {
+    const double FU = (fo ? Pp[cc] : P0[cc]) + o2 * (0.5 * (fo ? Gp[cc] : G0[cc]) * (sf * (Xp - X0) - dt * ufront));
+    const double BU = (bo ? Pm[cc] : P0[cc]) + o2 * (0.5 * (bo ? Gm[cc] : G0[cc]) * (sb * (X0 - Xm) - dt * uback));
+    U[cc] += back * BU - front * FU;
+  }
+#pragma unroll
+  for (int cc = 0; cc < NC; ++cc)
+    if (fabs(U[cc]) < 1.e-300) U[cc] = 0.;
+#pragma unroll
+  for (int cc = 0; cc <= NC; ++cc) S.Ud[cc * nn + n] = U[cc];
+#pragma unroll
+  for (int cc = 0; cc < NC; ++cc) {
+    const double ph = U[cc] / U[NC];
+    Pout[cc * nn + n] = ph;
+    if (last && U[NC] != 0.) S.u[cc][n] = ph;
+  }
+}
+
+// A node with no neighbour along D keeps its dual state (k_sweep_node_body).
+template <int DIM>
+__device__ __forceinline__ void dual_node_keep(const MhDev& S, long n, const double* U, const double* P,
+                                               double* Pout, int last) {
+  constexpr int NC = DIM;
+  const long nn = S.g.nn;
+#pragma unroll
+  for (int cc = 0; cc <= NC; ++cc) S.Ud[cc * nn + n] = U[cc];
+#pragma unroll
+  for (int cc = 0; cc < NC; ++cc) Pout[cc * nn + n] = P[cc];
+  if (last && U[NC] != 0.)
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc) S.u[cc][n] = U[cc] / U[NC];
+}
+
+// Dual x sweep with one wavefront per 60 nodes of a node row (lane shuffles carry the
+// neighbours' phi, coordinates, edge gradients and limited gradients; the front flux of
+// node i is the back flux of node i+1).  Lanes 2..61 write their node.
+template <int DIM>
+__global__ __launch_bounds__(MH_BLOCK) void k_sweep_node_row(MhDev S, int psel, int first, int last) {
+  constexpr int NC = DIM;
+  const int nnx = S.g.nnx, na = S.g.nx;
+  const long rows = (long)S.g.nny * S.g.nnz;
+  const int wpr = (nnx + MH_ROW_OUT - 1) / MH_ROW_OUT;
+  const int lane = threadIdx.x & 63;
+  const long gw = (long)xcd_block((int)blockIdx.x, (int)gridDim.x) * (MH_BLOCK / 64) + (threadIdx.x >> 6);
+  if (gw >= rows * wpr) return;  // whole wavefront
+  if (mh_halted(S.sc)) return;
+  const double* __restrict__ Pin = psel ? S.Pa : S.Pb;
+  double* __restrict__ Pout = psel ? S.Pb : S.Pa;
+  const long nn = S.g.nn;
+  const long row = gw / wpr;
+  const int i = (int)(gw % wpr) * MH_ROW_OUT - 2 + lane;
+  const bool valid = i >= 0 && i <= na;
+  const bool validp = i - 1 >= 0 && i - 1 <= na;
+  const bool out = valid && lane >= 2 && lane < 2 + MH_ROW_OUT;
+  const int j = (int)(row % S.g.nny);
+  const int k = (int)(row / S.g.nny);
+  const long n = row * nnx + i;
+  double P[NC], X = 0.;
+#pragma unroll
+  for (int cc = 0; cc < NC; ++cc) P[cc] = valid ? (first ? S.u[cc][n] : Pin[cc * nn + n]) : 0.;
+  if (valid) X = S.xlc[0][n];
+  const bool inner = valid && i >= 1 && i <= na - 1;
+  const double Xm = __shfl_up(X, 1), Xp = __shfl_down(X, 1);
+  double Pm[NC], Pp[NC], G0[NC], Gm[NC], Gp[NC];
+#pragma unroll
+  for (int cc = 0; cc < NC; ++cc) {
+    Pm[cc] = __shfl_up(P[cc], 1);
+    Pp[cc] = __shfl_down(P[cc], 1);
+    const double eb = (valid && validp) ? (P[cc] - Pm[cc]) / (X - Xm) : 0.;  // edge i-1/2
+    const double ef = __shfl_down(eb, 1);                                    // edge i+1/2
+    G0[cc] = (S.order2 && inner) ? dual_limited(S, eb, ef) : 0.;
+    Gm[cc] = __shfl_up(G0[cc], 1);
+    Gp[cc] = __shfl_down(G0[cc], 1);
+  }
+  const double front = valid ? dual_edge_flux<DIM, 0>(S, i, j, k) : 0.;
+  const double back = __shfl_up(front, 1);
+  if (!out) return;
+  double U[NC + 1];
+  if (first) {
+    const double mass = S.mn[n];
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc) U[cc] = mass * S.u[cc][n];
+    U[NC] = mass;
+  } else {
+#pragma unroll
+    for (int cc = 0; cc <= NC; ++cc) U[cc] = S.Ud[cc * nn + n];
+  }
+  if (!inner) {
+    dual_node_keep<DIM>(S, n, U, P, Pout, last);
+    return;
+  }
+  dual_node_update<DIM, 0>(S, n, U, Pm, P, Pp, Xm, X, Xp, Gm, G0, Gp, back, front, Pout, last);
+}
+
+// Dual y / z sweep, marching: a thread walks MH_MARCH_K nodes of a node line along D with
+// phi, coordinates, edge gradients, limited gradients and the edge mass flux carried in
+// registers (each formed once per node).  Same formulas and order as k_sweep_node_body.
+template <int DIM, int D>
+__global__ __launch_bounds__(MH_BLOCK) void k_sweep_node_march(MhDev S, int psel, int first, int last) {
+  constexpr int NC = DIM;
+  const int nnx = S.g.nnx, nny = S.g.nny;
+  const int na = D == 1 ? S.g.ny : S.g.nz;  // nodes 0..na along D
+  const int nt = (D == 1) ? (DIM == 3 ? S.g.nnz : 1) : nny;
+  const int nchunk = (na + 1 + MH_MARCH_K - 1) / MH_MARCH_K;
+  const long tid = (long)xcd_block((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
+  if (tid >= (long)nnx * nt * nchunk) return;
+  if (mh_halted(S.sc)) return;
+  const double* __restrict__ Pin = psel ? S.Pa : S.Pb;
+  double* __restrict__ Pout = psel ? S.Pb : S.Pa;
+  const long nn = S.g.nn;
+  const int i = (int)(tid % nnx);
+  const long r = tid / nnx;
+  const int t = (int)(r % nt);
+  const int ch = (int)(r / nt);
+  const int a0 = ch * MH_MARCH_K, a1 = min(a0 + MH_MARCH_K, na + 1);
+  const long nD = D == 1 ? (long)nnx : (long)nnx * nny;
+  const long n_base = D == 1 ? i + (long)nnx * nny * t : i + (long)nnx * t;
+  // transverse node indices of the line (for the cell gather)
+  const int it1 = i, it2 = t;
+  auto load_node = [&](int ap, double* P, double& X) {
+    const long m2 = n_base + ap * nD;
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc) P[cc] = first ? S.u[cc][m2] : Pin[cc * nn + m2];
+    X = S.xlc[D][m2];
+  };
+  double Pm[NC], P0[NC], P1[NC], Gm[NC], G0[NC], e01[NC];
+  double Xm = 0., X0 = 0., X1 = 0., Fb = 0.;
+  bool v0 = false, v1 = false;
+#pragma unroll
+  for (int cc = 0; cc < NC; ++cc) {
+    Pm[cc] = 0.;
+    P0[cc] = 0.;
+    P1[cc] = 0.;
+    Gm[cc] = 0.;
+    G0[cc] = 0.;
+    e01[cc] = 0.;
+  }
+  {  // node a0 - 2 enters as "a+1" of the first step (a = a0 - 3)
+    const int ap = a0 - 2;
+    v1 = ap >= 0 && ap <= na;
+    if (v1) load_node(ap, P1, X1);
+  }
+  for (int a = a0 - 3; a < a1; ++a) {
+    const int ap = a + 2;
+    const bool vp = ap >= 0 && ap <= na;
+    double P2[NC], X2 = 0.;
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc) P2[cc] = 0.;
+    if (vp) load_node(ap, P2, X2);
+    // edge a+3/2 gradient; limited gradient of node a+1
+    const bool in1 = S.order2 && a + 1 >= 1 && a + 1 <= na - 1;
+    double e12[NC], G1[NC];
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc) {
+      e12[cc] = (v1 && vp) ? (P2[cc] - P1[cc]) / (X2 - X1) : 0.;
+      G1[cc] = in1 ? dual_limited(S, e01[cc], e12[cc]) : 0.;
+    }
+    // mass flux of edge a+1/2 (front of node a)
+    const double Ff = dual_edge_flux<DIM, D>(S, a, it1, it2);
+    if (a >= a0 && v0) {
+      const long n = n_base + a * nD;
+      double U[NC + 1];
+      if (first) {
+        const double mass = S.mn[n];
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) U[cc] = mass * S.u[cc][n];
+        U[NC] = mass;
+      } else {
+#pragma unroll
+        for (int cc = 0; cc <= NC; ++cc) U[cc] = S.Ud[cc * nn + n];
+      }
+      if (a >= 1 && a <= na - 1) dual_node_update<DIM, D>(S, n, U, Pm, P0, P1, Xm, X0, X1, Gm, G0, G1, Fb, Ff, Pout, last);
+      else dual_node_keep<DIM>(S, n, U, P0, Pout, last);
+    }
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc) {
+      Pm[cc] = P0[cc];
+      P0[cc] = P1[cc];
+      P1[cc] = P2[cc];
+      Gm[cc] = G0[cc];
+      G0[cc] = G1[cc];
+      e01[cc] = e12[cc];
+    }
+    Xm = X0;
+    X0 = X1;
+    X1 = X2;
+    Fb = Ff;
     v0 = v1;
     v1 = vp;
   }

@@ -173,9 +173,10 @@ def test_node_kernel_tiled_equals_flat(monkeypatch, shape):
                                   lambda: cases.config2_bisod3d(7, 17, 3), lambda: cases.config2_bisod3d(130, 5, 4),
                                   lambda: cases.bitriplepoint(), lambda: cases.sedov_2d()])
 def test_sweep_march_equals_flat(monkeypatch, make):
-    """The row-segment x sweep and the marching y/z sweeps give the same bits as the
-    one-thread-per-cell sweeps (sizes cover partial chunks and row segments, several
-    segments per row, chunks starting past the third cell and sweeps 2 or 3 cells long)."""
+    """The row-segment x sweeps and the marching y/z sweeps (cell and dual parts) give the
+    same bits as the one-thread-per-item sweeps (sizes cover partial chunks and row
+    segments, several segments per row, chunks starting past the third item and sweeps 2
+    or 3 cells long)."""
     from mahyco_amd.lib import GpuCase
     res = {}
     for mode in ("flat", "march"):

@@ -32,6 +32,9 @@ enum {
   KN_COUNT
 };
 
+// sweep variants: one thread per cell / node; x rows + y/z marching; x rows + y/z LDS tiles
+enum { MH_SWEEP_FLAT = 0, MH_SWEEP_MARCH = 1, MH_SWEEP_TILE = 2 };
+
 struct ProfRec {
   int kind;
   hipEvent_t a, b;
@@ -63,8 +66,8 @@ struct MhHandle {
   std::vector<hipGraph_t> graph_defs;
   bool owns_stream = true;
   bool node_tiled = true;   // 3D: z-marching node kernel (MAHYCO_NODE_KERNEL=flat selects the flat one)
-  bool sweep_march = true;
-  int march_groups = 4;     // slot groups of the marching sweep (MAHYCO_MARCH_GROUPS=1 or 4)  // row-segment x sweep and marching y/z sweeps (MAHYCO_SWEEP=flat: one thread per cell)
+  int sweep_mode = MH_SWEEP_TILE;  // MAHYCO_SWEEP = flat | march | tile
+  int march_groups = 4;            // slot groups of the marching sweep (MAHYCO_MARCH_GROUPS=1 or 4)
   MhScalars* ext_sc = nullptr;  // scalars living in an ensemble's array
   size_t bytes = 0;             // device bytes of the case
 };
@@ -391,12 +394,37 @@ int node_march_blocks(MhHandle* h, int D) {
   return (int)(((threads + MH_BLOCK - 1) / MH_BLOCK + 7) / 8 * 8);
 }
 
+// blocks of the LDS-tiled y / z sweep (64 cells along x by MH_TILE_L along D), a multiple of 8
+int tile_blocks(MhHandle* h, int D) {
+  const long na = D == 1 ? h->g.ny : h->g.nz;
+  const long nt = D == 1 ? (h->g.dim == 3 ? h->g.nz : 1) : h->g.ny;
+  const long tiles = (long)((h->g.nx + 63) / 64) * nt * ((na + MH_TILE_L - 1) / MH_TILE_L);
+  return (int)((tiles + 7) / 8 * 8);
+}
+
+// cell and dual parts of a y or z sweep in the selected variant
 template <int DIM, int D, int M>
-void launch_march(MhHandle* h, int sel, int first) {
+void launch_sweep_yz(MhHandle* h, int sel, int first, int last) {
   const MhDev& S = h->dev;
-  const int nb = march_blocks(h, D);
-  if (h->march_groups == 4) LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell_march<DIM, D, M, 4>), nb, MH_BLOCK, S, sel, first);
-  else LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell_march<DIM, D, M, 1>), nb, MH_BLOCK, S, sel, first);
+  const int kc = KN_SWEEP_CELL + D, kn = KN_SWEEP_NODE + D;
+  switch (h->sweep_mode) {
+    case MH_SWEEP_FLAT:
+      LAUNCH(h, kc, (k_sweep_cell<DIM, D, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+      break;
+    case MH_SWEEP_MARCH:
+      if (h->march_groups == 4)
+        LAUNCH(h, kc, (k_sweep_cell_march<DIM, D, M, 4>), march_blocks(h, D), MH_BLOCK, S, sel, first);
+      else
+        LAUNCH(h, kc, (k_sweep_cell_march<DIM, D, M, 1>), march_blocks(h, D), MH_BLOCK, S, sel, first);
+      break;
+    default:
+      LAUNCH(h, kc, (k_sweep_cell_tile<DIM, D, M>), tile_blocks(h, D), MH_BLOCK, S, sel, first);
+      break;
+  }
+  if (h->sweep_mode == MH_SWEEP_MARCH)
+    LAUNCH(h, kn, (k_sweep_node_march<DIM, D>), node_march_blocks(h, D), MH_BLOCK, S, sel, first, last);
+  else
+    LAUNCH(h, kn, (k_sweep_node<DIM, D>), h->nblk_n, MH_BLOCK, S, sel, first, last);
 }
 
 template <int DIM, int M>
@@ -413,34 +441,16 @@ void launch_sweeps_t(MhHandle* h) {
   for (int s = 0; s < nd; ++s) {
     const int first = s == 0, last = (s == nd - 1) && !first, sel = s & 1;
     const int D = dirs[s];
+    const int mode = h->sweep_mode;
     if (D == 0) {
-      if (h->sweep_march) {
-        LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell_row<DIM, M>), row_blocks(h), MH_BLOCK, S, sel, first);
-      } else {
-        LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-      }
-      if (h->sweep_march) LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node_row<DIM>), node_row_blocks(h), MH_BLOCK, S, sel, first, last);
-      else LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node<DIM, 0>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+      if (mode == MH_SWEEP_FLAT) LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+      else LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell_row<DIM, M>), row_blocks(h), MH_BLOCK, S, sel, first);
+      if (mode == MH_SWEEP_FLAT) LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+      else LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node_row<DIM>), node_row_blocks(h), MH_BLOCK, S, sel, first, last);
     } else if (D == 1) {
-      if (h->sweep_march) {
-        launch_march<DIM, 1, M>(h, sel, first);
-      } else {
-        LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-      }
-      if (h->sweep_march) LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node_march<DIM, 1>), node_march_blocks(h, 1), MH_BLOCK, S, sel, first, last);
-      else LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node<DIM, 1>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+      launch_sweep_yz<DIM, 1, M>(h, sel, first, last);
     } else {
-      if constexpr (DIM == 3) {
-        if (h->sweep_march) {
-          launch_march<3, 2, M>(h, sel, first);
-        } else {
-          LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell<3, 2, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-        }
-      }
-      if constexpr (DIM == 3) {
-        if (h->sweep_march) LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node_march<3, 2>), node_march_blocks(h, 2), MH_BLOCK, S, sel, first, last);
-        else LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node<3, 2>), h->nblk_n, MH_BLOCK, S, sel, first, last);
-      }
+      if constexpr (DIM == 3) launch_sweep_yz<3, 2, M>(h, sel, first, last);
     }
   }
   if (nd <= 1) LAUNCH(h, KN_NODE_VEL, (k_node_velocity<DIM>), h->nblk_n, MH_BLOCK, S, nd == 0 ? 1 : 0);
@@ -773,7 +783,7 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
     const char* nk = getenv("MAHYCO_NODE_KERNEL");
     h->node_tiled = !(nk && strcmp(nk, "flat") == 0);
     const char* sw = getenv("MAHYCO_SWEEP");
-    h->sweep_march = !(sw && strcmp(sw, "flat") == 0);
+    h->sweep_mode = !sw ? MH_SWEEP_TILE : !strcmp(sw, "flat") ? MH_SWEEP_FLAT : !strcmp(sw, "march") ? MH_SWEEP_MARCH : MH_SWEEP_TILE;
     const char* mg = getenv("MAHYCO_MARCH_GROUPS");
     h->march_groups = (mg && atoi(mg) == 1) ? 1 : 4;
   }

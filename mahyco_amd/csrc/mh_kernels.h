@@ -1167,6 +1167,150 @@ __global__ __launch_bounds__(MH_BLOCK) void k_sweep_node_march(MhDev S, int psel
   }
 }
 
+// The same sweep for D = y or z, LDS tiles: a block of 64 x 4 threads owns 64 cells along
+// x by MH_TILE_L cells along D (plus two halo cells on each side).  Every cell's Phi, every
+// face gradient, every limited slope and every face flux of the tile is formed once, by
+// the thread owning that row, and shared through LDS (four barriers per remap slot).
+// Wavefront w holds rows w, w+4, ... of the tile, 64 consecutive cells in x, so every
+// global access is coalesced.  Same formulas in the same order as k_sweep_cell_body.
+#define MH_TILE_L 16
+#define MH_TILE_R (MH_TILE_L + 4)  // rows held, halos included
+#define MH_TILE_Q (MH_TILE_R / 4)  // rows per thread
+template <int DIM, int D, int M>
+__global__ __launch_bounds__(MH_BLOCK) void k_sweep_cell_tile(MhDev S, int usel, int first) {
+  constexpr int NV = 3 * M + 1;
+  __shared__ double sphi[MH_TILE_R][64];
+  __shared__ double sgf[MH_TILE_R][64];  // face gradients, then face fluxes
+  __shared__ double ssl[MH_TILE_R][64];  // cell centres, then limited slopes
+  const int nx = S.g.nx, ny = S.g.ny, nz = S.g.nz;
+  const int na = D == 1 ? ny : nz;
+  const int nt = (D == 1) ? (DIM == 3 ? nz : 1) : ny;
+  const int tilesx = (nx + 63) / 64;
+  const long ntiles = (long)tilesx * nt * ((na + MH_TILE_L - 1) / MH_TILE_L);
+  const long lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  if (lb >= ntiles) return;  // whole block
+  if (mh_halted(S.sc)) return;
+  const double* __restrict__ Uin = usel ? S.Ub : S.Ua;
+  double* __restrict__ Uout = usel ? S.Ua : S.Ub;
+  double* __restrict__ dflux = S.dflux;
+  const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+  const int i = (int)(lb % tilesx) * 64 + lx;
+  const long rest = lb / tilesx;
+  const int t = (int)(rest % nt);
+  const int abase = (int)(rest / nt) * MH_TILE_L - 2;  // cell along D of tile row 0
+  const long nc = S.g.nc;
+  const long nnx = S.g.nnx, nny = S.g.nny;
+  const long sD = D == 1 ? (long)nx : (long)nx * ny;
+  const long c_base = D == 1 ? i + (long)nx * ny * t : i + (long)nx * t;
+  const long nD = D == 1 ? nnx : nnx * nny;
+  const long n_base = D == 1 ? i + nnx * nny * t : i + nnx * t;
+  const double dt = S.sc->dt;
+  const double o2 = (double)S.order2;
+  // rows r = ly + 4q of this thread: cell validity, Phi denominator, Delta x of face r
+  // (between rows r-1 and r) and the upwind data of that face
+  bool val[MH_TILE_Q], inner[MH_TILE_Q], up[MH_TILE_Q];
+  double den[MH_TILE_Q], dx[MH_TILE_Q], w[MH_TILE_Q], hh[MH_TILE_Q], xcq[MH_TILE_Q];
+#pragma unroll
+  for (int q = 0; q < MH_TILE_Q; ++q) {
+    const int a = abase + ly + 4 * q;
+    val[q] = i < nx && a >= 0 && a < na;
+    den[q] = 1.;
+    xcq[q] = 0.;
+    if (val[q]) {
+      const long cc = c_base + a * sD;
+      if (first) den[q] = S.Vl[cc];
+      else {
+        double sv = 0.;
+        for (int k = 0; k < M; ++k) sv += Uin[k * nc + cc];
+        den[q] = sv;
+      }
+      xcq[q] = cell_centre_d<DIM, D>(S, n_base + a * nD);
+    }
+    ssl[ly + 4 * q][lx] = xcq[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MH_TILE_Q; ++q) {
+    const int r = ly + 4 * q, a = abase + r;
+    const bool valp = r >= 1 && i < nx && a - 1 >= 0 && a - 1 < na;
+    const double xcp = r >= 1 ? ssl[r - 1][lx] : 0.;
+    dx[q] = (val[q] && valp) ? xcq[q] - xcp : 0.;
+    inner[q] = val[q] && valp;  // face between cells a-1 and a
+    up[q] = false;
+    w[q] = 0.;
+    hh[q] = 0.;
+    if (inner[q] && r >= 2 && r <= MH_TILE_R - 2) {
+      double area, xf, vf;
+      face_quant<DIM, D>(S, n_base + a * nD, area, xf, vf);
+      up[q] = vf * dx[q] > 0.0;
+      hh[q] = o2 * (xf - (up[q] ? xcp : xcq[q]));
+      w[q] = 1. * vf * area * dt;
+    }
+  }
+  for (int v = 0; v < NV; ++v) {
+    // barriers: the previous slot read sphi and ssl before its last barrier, and reads
+    // sgf (fluxes) before the first barrier of this slot
+    double uq[MH_TILE_Q];
+#pragma unroll
+    for (int q = 0; q < MH_TILE_Q; ++q) {
+      const int a = abase + ly + 4 * q;
+      uq[q] = val[q] ? Uin[v * nc + c_base + a * sD] : 0.;
+      sphi[ly + 4 * q][lx] = val[q] ? uq[q] / den[q] : 0.;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MH_TILE_Q; ++q) {
+      const int r = ly + 4 * q;
+      sgf[r][lx] = (r >= 1 && inner[q]) ? (sphi[r][lx] - sphi[r - 1][lx]) / dx[q] : 0.;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MH_TILE_Q; ++q) {
+      const int r = ly + 4 * q;
+      double g = 0.;
+      if (S.order2 && val[q] && r >= 1 && r <= MH_TILE_R - 2) {
+        const double gb = sgf[r][lx], gf = sgf[r + 1][lx];
+        if (gb != 0.) g += 0.5 * (mh_flux_limiter(S.limiter, gf / gb) * gb);
+        if (gf != 0.) g += 0.5 * (mh_flux_limiter(S.limiter, gb / gf) * gf);
+      }
+      ssl[r][lx] = g;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MH_TILE_Q; ++q) {  // the gradients were all read before the barrier
+      const int r = ly + 4 * q;
+      double f = 0.;
+      if (inner[q] && r >= 2 && r <= MH_TILE_R - 2) {
+        const double pf = up[q] ? sphi[r - 1][lx] + hh[q] * ssl[r - 1][lx] : sphi[r][lx] + hh[q] * ssl[r][lx];
+        f = w[q] * pf;
+      }
+      sgf[r][lx] = f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MH_TILE_Q; ++q) {
+      const int r = ly + 4 * q, a = abase + r;
+      if (!val[q] || r < 2 || r >= 2 + MH_TILE_L) continue;
+      const bool inb = a > 0, inf = a < na - 1;
+      const double fb = -sgf[r][lx];
+      const double Ff = sgf[r + 1][lx];
+      double ff = 0.;
+      if (inb) ff += fb;
+      if (inf) ff += Ff;
+      const long c = c_base + a * sD;
+      double un = uq[q] - ff;
+      if (v >= M && v < 2 * M) {
+        double dd = 0.;
+        if (inb) dd += 0.5 * fb * -1.;
+        if (inf) dd += 0.5 * Ff * 1.;
+        dflux[(v - M) * nc + c] = dd;
+      }
+      if (v >= M && v < 3 * M && un < 0.) un = 0.;
+      Uout[v * nc + c] = un;
+    }
+  }
+}
+
 // One ADI direction, dual part for one node: limited nodal velocity gradients
 // (computeDualGradPhi UtilesRemap.cc:19-72, computeAndLimitGradPhiDual :369-400), back/front
 // mass fluxes gathered from the cells behind/ahead in face order, env order

@@ -173,13 +173,13 @@ def test_node_kernel_tiled_equals_flat(monkeypatch, shape):
                                   lambda: cases.config2_bisod3d(7, 17, 3), lambda: cases.config2_bisod3d(130, 5, 4),
                                   lambda: cases.bitriplepoint(), lambda: cases.sedov_2d()])
 def test_sweep_march_equals_flat(monkeypatch, make):
-    """The row-segment x sweeps and the marching y/z sweeps (cell and dual parts) give the
-    same bits as the one-thread-per-item sweeps (sizes cover partial chunks and row
+    """The row-segment x sweeps, the marching y/z sweeps and the LDS-tiled y/z sweeps give
+    the same bits as the one-thread-per-item sweeps (sizes cover partial chunks and row
     segments, several segments per row, chunks starting past the third item and sweeps 2
     or 3 cells long)."""
     from mahyco_amd.lib import GpuCase
     res = {}
-    for mode in ("flat", "march"):
+    for mode in ("flat", "march", "tile"):
         monkeypatch.setenv("MAHYCO_SWEEP", mode)
         c = make()
         print(f"[paths] sweep {mode} {c.name}", flush=True)
@@ -188,4 +188,5 @@ def test_sweep_march_equals_flat(monkeypatch, make):
         res[mode] = fields(g, len(c.envs), c.dim)
         g.close()
     for k in res["flat"]:
-        assert bitwise_equal(res["flat"][k], res["march"][k]), k
+        assert bitwise_equal(res["flat"][k], res["march"][k]), ("march", k)
+        assert bitwise_equal(res["flat"][k], res["tile"][k]), ("tile", k)

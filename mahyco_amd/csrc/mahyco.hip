@@ -32,8 +32,8 @@ enum {
   KN_COUNT
 };
 
-// sweep variants: one thread per cell / node; x rows + y/z marching; x rows + y/z LDS tiles
-enum { MH_SWEEP_FLAT = 0, MH_SWEEP_MARCH = 1, MH_SWEEP_TILE = 2 };
+// sweep variants: one thread per cell / node; row segments along x and LDS tiles along y / z
+enum { MH_SWEEP_FLAT = 0, MH_SWEEP_TILE = 1 };
 
 struct ProfRec {
   int kind;
@@ -66,8 +66,7 @@ struct MhHandle {
   std::vector<hipGraph_t> graph_defs;
   bool owns_stream = true;
   bool node_tiled = true;   // 3D: z-marching node kernel (MAHYCO_NODE_KERNEL=flat selects the flat one)
-  int sweep_mode = MH_SWEEP_TILE;  // MAHYCO_SWEEP = flat | march | tile
-  int march_groups = 4;            // slot groups of the marching sweep (MAHYCO_MARCH_GROUPS=1 or 4)
+  int sweep_mode = MH_SWEEP_TILE;  // MAHYCO_SWEEP=flat selects one thread per cell / node
   MhScalars* ext_sc = nullptr;  // scalars living in an ensemble's array
   size_t bytes = 0;             // device bytes of the case
 };
@@ -364,15 +363,6 @@ void prof_end(MhHandle* h, ProfRec& r) {
     if ((h)->prof) prof_end((h), _r);                                    \
   } while (0)
 
-// blocks of the marching sweep along D (one thread per MH_MARCH_K cells and slot group), a multiple of 8
-int march_blocks(MhHandle* h, int D) {
-  const long na = D == 1 ? h->g.ny : h->g.nz;
-  const long nt = D == 1 ? (h->g.dim == 3 ? h->g.nz : 1) : h->g.ny;
-  const long cols = (long)h->g.nx * nt * ((na + MH_MARCH_K - 1) / MH_MARCH_K);
-  const long cpb = MH_BLOCK / h->march_groups;
-  return (int)(((cols + cpb - 1) / cpb + 7) / 8 * 8);
-}
-
 // blocks of the row-segment x sweep (one wavefront per MH_ROW_OUT cells of a row), a multiple of 8
 int row_blocks(MhHandle* h) {
   const long waves = (long)h->g.ny * h->g.nz * ((h->g.nx + MH_ROW_OUT - 1) / MH_ROW_OUT);
@@ -380,18 +370,11 @@ int row_blocks(MhHandle* h) {
   return (int)(((waves + wpb - 1) / wpb + 7) / 8 * 8);
 }
 
-// blocks of the dual sweeps: row segments of 60 nodes along x, or node lines marched in
-// chunks of MH_MARCH_K along y / z; multiples of 8
+// blocks of the row-segment dual x sweep (one wavefront per MH_ROW_OUT nodes of a node row), a multiple of 8
 int node_row_blocks(MhHandle* h) {
   const long waves = (long)h->g.nny * h->g.nnz * ((h->g.nnx + MH_ROW_OUT - 1) / MH_ROW_OUT);
   const long wpb = MH_BLOCK / 64;
   return (int)(((waves + wpb - 1) / wpb + 7) / 8 * 8);
-}
-int node_march_blocks(MhHandle* h, int D) {
-  const long na = D == 1 ? h->g.ny : h->g.nz;
-  const long nt = D == 1 ? (h->g.dim == 3 ? h->g.nnz : 1) : h->g.nny;
-  const long threads = (long)h->g.nnx * nt * ((na + 1 + MH_MARCH_K - 1) / MH_MARCH_K);
-  return (int)(((threads + MH_BLOCK - 1) / MH_BLOCK + 7) / 8 * 8);
 }
 
 // blocks of the LDS-tiled y / z sweep (64 cells along x by MH_TILE_L along D), a multiple of 8
@@ -402,29 +385,16 @@ int tile_blocks(MhHandle* h, int D) {
   return (int)((tiles + 7) / 8 * 8);
 }
 
-// cell and dual parts of a y or z sweep in the selected variant
+// cell and dual parts of a y or z sweep in the selected variant (the dual part is one
+// thread per node in both)
 template <int DIM, int D, int M>
 void launch_sweep_yz(MhHandle* h, int sel, int first, int last) {
   const MhDev& S = h->dev;
-  const int kc = KN_SWEEP_CELL + D, kn = KN_SWEEP_NODE + D;
-  switch (h->sweep_mode) {
-    case MH_SWEEP_FLAT:
-      LAUNCH(h, kc, (k_sweep_cell<DIM, D, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-      break;
-    case MH_SWEEP_MARCH:
-      if (h->march_groups == 4)
-        LAUNCH(h, kc, (k_sweep_cell_march<DIM, D, M, 4>), march_blocks(h, D), MH_BLOCK, S, sel, first);
-      else
-        LAUNCH(h, kc, (k_sweep_cell_march<DIM, D, M, 1>), march_blocks(h, D), MH_BLOCK, S, sel, first);
-      break;
-    default:
-      LAUNCH(h, kc, (k_sweep_cell_tile<DIM, D, M>), tile_blocks(h, D), MH_BLOCK, S, sel, first);
-      break;
-  }
-  if (h->sweep_mode == MH_SWEEP_MARCH)
-    LAUNCH(h, kn, (k_sweep_node_march<DIM, D>), node_march_blocks(h, D), MH_BLOCK, S, sel, first, last);
+  if (h->sweep_mode == MH_SWEEP_FLAT)
+    LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell<DIM, D, M>), h->nblk_c, MH_BLOCK, S, sel, first);
   else
-    LAUNCH(h, kn, (k_sweep_node<DIM, D>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+    LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell_tile<DIM, D, M>), tile_blocks(h, D), MH_BLOCK, S, sel, first);
+  LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node<DIM, D>), h->nblk_n, MH_BLOCK, S, sel, first, last);
 }
 
 template <int DIM, int M>
@@ -783,9 +753,7 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
     const char* nk = getenv("MAHYCO_NODE_KERNEL");
     h->node_tiled = !(nk && strcmp(nk, "flat") == 0);
     const char* sw = getenv("MAHYCO_SWEEP");
-    h->sweep_mode = !sw ? MH_SWEEP_TILE : !strcmp(sw, "flat") ? MH_SWEEP_FLAT : !strcmp(sw, "march") ? MH_SWEEP_MARCH : MH_SWEEP_TILE;
-    const char* mg = getenv("MAHYCO_MARCH_GROUPS");
-    h->march_groups = (mg && atoi(mg) == 1) ? 1 : 4;
+    h->sweep_mode = (sw && strcmp(sw, "flat") == 0) ? MH_SWEEP_FLAT : MH_SWEEP_TILE;
   }
   HostInit I;
   st = host_init(h, I);

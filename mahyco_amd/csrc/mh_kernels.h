@@ -175,18 +175,18 @@ __device__ __forceinline__ void k_node_lag_body(const MhDev& S, int lb) {
 }
 
 
-// 3D node kernel, z-marching tiles: a block owns 16x16 node columns over MH_NT_KC node
-// planes.  Each cell plane of the tile (17x17 cells) is processed once: its corner vectors
-// (cqs_hex, computed once per cell instead of once per corner) are multiplied by p+q and
-// kept in LDS with M/8 -- the bottom corners for the node plane it sits on, the top ones
-// for the next plane.  Every node then adds its 8 cells in cell order, as k_node_lag does,
-// so both kernels give the same bits.
-#define MH_NT 16
+// 3D node kernel, z-marching tiles: a block owns 15x15 node columns over MH_NT_KC node
+// planes, so each cell plane of the tile is 16x16 cells, one per thread.  Each cell plane
+// is processed once: its corner vectors (cqs_hex, computed once per cell instead of once
+// per corner) are multiplied by p+q and kept in LDS with M/8 -- the bottom corners for
+// the node plane it sits on, the top ones (held in registers meanwhile) for the next
+// plane.  Every node then adds its 8 cells in cell order, as k_node_lag does, so both
+// kernels give the same bits.
+#define MH_NT 15
 #define MH_NT_KC 8
-#define MH_NT_CELLS ((MH_NT + 1) * (MH_NT + 1))
-__device__ __forceinline__ void nt_cell(const MhDev& S, int ii, int jj, int kk, double* up_or_lo, int s,
-                                        double* keep, bool top_to_lds) {
-  // corner force contributions (p + q) CQS_l and the node-mass share w M of one cell
+#define MH_NT_CELLS ((MH_NT + 1) * (MH_NT + 1))  // = MH_BLOCK
+__device__ __forceinline__ void nt_cell(const MhDev& S, int ii, int jj, int kk, double* lds, int s, double* keep) {
+  // corner force contributions (p + q) CQS_l and the node-mass share M/8 of one cell
   const long nnx = S.g.nnx, nnxy = (long)S.g.nnx * S.g.nny;
   const long c = ii + (long)S.g.nx * (jj + (long)S.g.ny * kk);
   const long b = ii + nnx * (jj + (long)S.g.nny * kk);
@@ -203,18 +203,17 @@ __device__ __forceinline__ void nt_cell(const MhDev& S, int ii, int jj, int kk, 
   const double pr = S.p[c] + S.q[c];
   const double wm = 0.125 * S.M[c];
   for (int l = 0; l < 4; ++l) {
-    up_or_lo[(3 * l + 0) * MH_NT_CELLS + s] = pr * q[l].x;
-    up_or_lo[(3 * l + 1) * MH_NT_CELLS + s] = pr * q[l].y;
-    up_or_lo[(3 * l + 2) * MH_NT_CELLS + s] = pr * q[l].z;
+    lds[(3 * l + 0) * MH_NT_CELLS + s] = pr * q[l].x;
+    lds[(3 * l + 1) * MH_NT_CELLS + s] = pr * q[l].y;
+    lds[(3 * l + 2) * MH_NT_CELLS + s] = pr * q[l].z;
   }
-  up_or_lo[12 * MH_NT_CELLS + s] = wm;
+  lds[12 * MH_NT_CELLS + s] = wm;
   for (int l = 0; l < 4; ++l) {
     keep[3 * l + 0] = pr * q[4 + l].x;
     keep[3 * l + 1] = pr * q[4 + l].y;
     keep[3 * l + 2] = pr * q[4 + l].z;
   }
   keep[12] = wm;
-  (void)top_to_lds;
 }
 
 __device__ __forceinline__ void nt_add(const double* P, int s, int corner, bool ok, double& mass, double& fx,
@@ -227,41 +226,32 @@ __device__ __forceinline__ void nt_add(const double* P, int s, int corner, bool 
 }
 
 __device__ __forceinline__ void k_node_lag3_tiled_body(const MhDev& S, int bx, int by, int bz) {
+  static_assert(MH_NT_CELLS == MH_BLOCK, "one cell of the tile plane per thread");
   __shared__ double lo[13 * MH_NT_CELLS];  // top corners (4..7) of the cell plane below
   __shared__ double up[13 * MH_NT_CELLS];  // bottom corners (0..3) of the cell plane above
   if (mh_halted(S.sc)) return;
   const int tid = threadIdx.x;
-  const int tx = tid % MH_NT, ty = tid / MH_NT;
   const int i0 = bx * MH_NT, j0 = by * MH_NT, k0 = bz * MH_NT_KC;
   const int nx = S.g.nx, ny = S.g.ny, nz = S.g.nz;
   const int kend = min(k0 + MH_NT_KC, S.g.nnz);
   const double dt = S.sc->dt;
   const double dth = 0.5 * (S.sc->dt_old + dt);
-  double keep[2][13];
+  // this thread's cell column of the tile: (i0 - 1 + tid % 16, j0 - 1 + tid / 16)
+  const int ci = i0 - 1 + tid % (MH_NT + 1), cj = j0 - 1 + tid / (MH_NT + 1);
+  const bool cell_ok = ci >= 0 && ci < nx && cj >= 0 && cj < ny;
+  double keep[13];
   // prologue: top corners of the cell plane k0 - 1
-  if (k0 >= 1) {
-    for (int r = 0; r < 2; ++r) {
-      const int sl = tid + r * MH_BLOCK;
-      if (sl >= MH_NT_CELLS) break;
-      const int ii = i0 - 1 + sl % (MH_NT + 1), jj = j0 - 1 + sl / (MH_NT + 1);
-      if (ii < 0 || ii >= nx || jj < 0 || jj >= ny) continue;
-      nt_cell(S, ii, jj, k0 - 1, up, sl, keep[r], false);
-      for (int v = 0; v < 13; ++v) lo[v * MH_NT_CELLS + sl] = keep[r][v];
-    }
+  if (k0 >= 1 && cell_ok) {
+    nt_cell(S, ci, cj, k0 - 1, up, tid, keep);
+    for (int v = 0; v < 13; ++v) lo[v * MH_NT_CELLS + tid] = keep[v];
   }
-  const int i = i0 + tx, j = j0 + ty;
-  const bool node_ok = i < S.g.nnx && j < S.g.nny;
+  // this thread's node column: (i0 + tid % 15, j0 + tid / 15) for tid < 225
+  const int i = i0 + tid % MH_NT, j = j0 + tid / MH_NT;
+  const bool node_ok = tid < MH_NT * MH_NT && i < S.g.nnx && j < S.g.nny;
+  const int tx = tid % MH_NT, ty = tid / MH_NT;
   for (int k = k0; k < kend; ++k) {
     // cell plane k: bottom corners to LDS now, top corners kept for node plane k + 1
-    if (k < nz) {
-      for (int r = 0; r < 2; ++r) {
-        const int sl = tid + r * MH_BLOCK;
-        if (sl >= MH_NT_CELLS) break;
-        const int ii = i0 - 1 + sl % (MH_NT + 1), jj = j0 - 1 + sl / (MH_NT + 1);
-        if (ii < 0 || ii >= nx || jj < 0 || jj >= ny) continue;
-        nt_cell(S, ii, jj, k, up, sl, keep[r], true);
-      }
-    }
+    if (k < nz && cell_ok) nt_cell(S, ci, cj, k, up, tid, keep);
     __syncthreads();
     if (node_ok) {
       double mass = 0., fx = 0., fy = 0., fz = 0.;
@@ -283,15 +273,8 @@ __device__ __forceinline__ void k_node_lag3_tiled_body(const MhDev& S, int bx, i
       node_finish<3>(S, n, i, j, k, dt, dth, mass, fx, fy, fz);
     }
     __syncthreads();
-    if (k < nz) {
-      for (int r = 0; r < 2; ++r) {
-        const int sl = tid + r * MH_BLOCK;
-        if (sl >= MH_NT_CELLS) break;
-        const int ii = i0 - 1 + sl % (MH_NT + 1), jj = j0 - 1 + sl / (MH_NT + 1);
-        if (ii < 0 || ii >= nx || jj < 0 || jj >= ny) continue;
-        for (int v = 0; v < 13; ++v) lo[v * MH_NT_CELLS + sl] = keep[r][v];
-      }
-    }
+    if (k < nz && cell_ok)
+      for (int v = 0; v < 13; ++v) lo[v * MH_NT_CELLS + tid] = keep[v];
   }
 }
 
@@ -765,158 +748,6 @@ __global__ __launch_bounds__(MH_BLOCK) void k_sweep_cell_row(MhDev S, int usel, 
   }
 }
 
-// The same sweep for D = y or z, marching: a thread walks MH_MARCH_K consecutive cells along
-// D (consecutive threads are consecutive in x, so every load is coalesced) with a register
-// pipeline, so Phi, each face gradient, each limited slope and each face flux is formed
-// once instead of once per stencil member.  The step for cell a brings in cell a+2, forms
-// the gradient of face a+3/2, the slope of cell a+1 and the flux of face a+1/2, then
-// updates cell a with the fluxes of faces a-1/2 (carried, negated) and a+1/2.  With G > 1
-// the G wavefronts of a block share the same columns and split the remap slots
-// (wavefront g takes slots g, g+G, ...), which cuts the registers per thread so that more
-// wavefronts hide the memory latency.  Same formulas in the same order as
-// k_sweep_cell_body, hence the same bits.
-#define MH_MARCH_K 16
-template <int DIM, int D, int M, int G>
-__global__ __launch_bounds__(MH_BLOCK) void k_sweep_cell_march(MhDev S, int usel, int first) {
-  constexpr int NV = 3 * M + 1;
-  constexpr int NQ = (NV + G - 1) / G;  // slots per thread
-  constexpr int CPB = MH_BLOCK / G;     // columns per block
-  const int nx = S.g.nx, ny = S.g.ny, nz = S.g.nz;
-  const int na = D == 1 ? ny : nz;
-  const int nt = (D == 1) ? (DIM == 3 ? nz : 1) : ny;
-  const int nchunk = (na + MH_MARCH_K - 1) / MH_MARCH_K;
-  const int grp = (int)threadIdx.x / CPB;
-  const long tid = (long)xcd_block((int)blockIdx.x, (int)gridDim.x) * CPB + (threadIdx.x % CPB);
-  if (tid >= (long)nx * nt * nchunk) return;
-  if (mh_halted(S.sc)) return;
-  const double* __restrict__ Uin = usel ? S.Ub : S.Ua;
-  double* __restrict__ Uout = usel ? S.Ua : S.Ub;
-  double* __restrict__ dflux = S.dflux;
-  const int i = (int)(tid % nx);
-  const long r = tid / nx;
-  const int t = (int)(r % nt);
-  const int ch = (int)(r / nt);
-  const int a0 = ch * MH_MARCH_K, a1 = min(a0 + MH_MARCH_K, na);
-  const long nc = S.g.nc;
-  const long nnx = S.g.nnx, nny = S.g.nny;
-  const long sD = D == 1 ? (long)nx : (long)nx * ny;
-  const long c_base = D == 1 ? i + (long)nx * ny * t : i + (long)nx * t;
-  const long nD = D == 1 ? nnx : nnx * nny;
-  const long n_base = D == 1 ? i + nnx * nny * t : i + nnx * t;
-  const double dt = S.sc->dt;
-  const double o2 = (double)S.order2;
-  // pipeline state: cells a (0) and a+1 (1); face a+1/2 (01); slope of cell a; flux of face a-1/2
-  double P0[NQ], P1[NQ], G01[NQ], S0[NQ], FB[NQ];
-  double xc0 = 0., xc1 = 0., dx01 = 0.;
-  bool v0 = false, v1 = false;
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    P0[q] = 0.;
-    P1[q] = 0.;
-    G01[q] = 0.;
-    S0[q] = 0.;
-    FB[q] = 0.;
-  }
-  // Phi of the slots of this thread at cell ap (and its centre along D)
-  auto load_cell = [&](int ap, double* P, double& xc) {
-    const long cc = c_base + ap * sD;
-    double den;
-    if (first) den = S.Vl[cc];
-    else {
-      den = 0.;
-      for (int k = 0; k < M; ++k) den += Uin[k * nc + cc];
-    }
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int v = grp + G * q;
-      P[q] = v < NV ? Uin[v * nc + cc] / den : 0.;
-    }
-    xc = cell_centre_d<DIM, D>(S, n_base + ap * nD);
-  };
-  {  // cell a0 - 2 enters as "a+1" of the first step (a = a0 - 3)
-    const int ap = a0 - 2;
-    v1 = ap >= 0 && ap < na;
-    if (v1) load_cell(ap, P1, xc1);
-  }
-  for (int a = a0 - 3; a < a1; ++a) {
-    // cell a+2
-    const int ap = a + 2;
-    const bool vp = ap >= 0 && ap < na;
-    double P2[NQ], xc2 = 0.;
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) P2[q] = 0.;
-    if (vp) load_cell(ap, P2, xc2);
-    // face a+3/2: gradient; cell a+1: limited slope
-    const bool f12 = v1 && vp;
-    const double dx12 = f12 ? xc2 - xc1 : 0.;
-    double G12[NQ], S1[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      G12[q] = f12 ? (P2[q] - P1[q]) / dx12 : 0.;
-      double g = 0.;
-      if (S.order2 && v1) {
-        const double gb = G01[q], gf = G12[q];
-        if (gb != 0.) g += 0.5 * (mh_flux_limiter(S.limiter, gf / gb) * gb);
-        if (gf != 0.) g += 0.5 * (mh_flux_limiter(S.limiter, gb / gf) * gf);
-      }
-      S1[q] = g;
-    }
-    // face a+1/2: upwind value and flux (outer normal +e_D for cell a)
-    const bool f01 = v0 && v1;
-    double FF[NQ];
-    if (f01) {
-      double area, xf, vf;
-      face_quant<DIM, D>(S, n_base + (a + 1) * nD, area, xf, vf);
-      const bool upprev = vf * dx01 > 0.0;
-      const double h = o2 * (xf - (upprev ? xc0 : xc1));
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const double pf = (upprev ? P0[q] : P1[q]) + h * (upprev ? S0[q] : S1[q]);
-        FF[q] = 1. * vf * area * dt * pf;
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) FF[q] = 0.;
-    }
-    // cell a: U -= (flux of face a-1/2) + (flux of face a+1/2)
-    if (a >= a0 && v0) {
-      const long c = c_base + a * sD;
-      const bool inb = a > 0, inf = a < na - 1;
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const int v = grp + G * q;
-        if (v >= NV) continue;
-        double ff = 0.;
-        if (inb) ff += FB[q];
-        if (inf) ff += FF[q];
-        double un = Uin[v * nc + c] - ff;
-        if (v >= M && v < 2 * M) {
-          double dd = 0.;
-          if (inb) dd += 0.5 * FB[q] * -1.;
-          if (inf) dd += 0.5 * FF[q] * 1.;
-          dflux[(v - M) * nc + c] = dd;
-        }
-        if (v >= M && v < 3 * M && un < 0.) un = 0.;
-        Uout[v * nc + c] = un;
-      }
-    }
-    // shift the pipeline by one cell
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      P0[q] = P1[q];
-      P1[q] = P2[q];
-      G01[q] = G12[q];
-      S0[q] = S1[q];
-      FB[q] = -FF[q];
-    }
-    xc0 = xc1;
-    xc1 = xc2;
-    dx01 = dx12;
-    v0 = v1;
-    v1 = vp;
-  }
-}
-
 // Mass flux through the dual faces between node position a and a+1 along D: the sum, in
 // the order of k_sweep_node_body, of the cell mass fluxes of the 2^(d-1) cells at
 // position a around the node line (it1, it2) -- the "front" flux of node a and the "back"
@@ -1067,104 +898,6 @@ __global__ __launch_bounds__(MH_BLOCK) void k_sweep_node_row(MhDev S, int psel, 
     return;
   }
   dual_node_update<DIM, 0>(S, n, U, Pm, P, Pp, Xm, X, Xp, Gm, G0, Gp, back, front, Pout, last);
-}
-
-// Dual y / z sweep, marching: a thread walks MH_MARCH_K nodes of a node line along D with
-// phi, coordinates, edge gradients, limited gradients and the edge mass flux carried in
-// registers (each formed once per node).  Same formulas and order as k_sweep_node_body.
-template <int DIM, int D>
-__global__ __launch_bounds__(MH_BLOCK) void k_sweep_node_march(MhDev S, int psel, int first, int last) {
-  constexpr int NC = DIM;
-  const int nnx = S.g.nnx, nny = S.g.nny;
-  const int na = D == 1 ? S.g.ny : S.g.nz;  // nodes 0..na along D
-  const int nt = (D == 1) ? (DIM == 3 ? S.g.nnz : 1) : nny;
-  const int nchunk = (na + 1 + MH_MARCH_K - 1) / MH_MARCH_K;
-  const long tid = (long)xcd_block((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
-  if (tid >= (long)nnx * nt * nchunk) return;
-  if (mh_halted(S.sc)) return;
-  const double* __restrict__ Pin = psel ? S.Pa : S.Pb;
-  double* __restrict__ Pout = psel ? S.Pb : S.Pa;
-  const long nn = S.g.nn;
-  const int i = (int)(tid % nnx);
-  const long r = tid / nnx;
-  const int t = (int)(r % nt);
-  const int ch = (int)(r / nt);
-  const int a0 = ch * MH_MARCH_K, a1 = min(a0 + MH_MARCH_K, na + 1);
-  const long nD = D == 1 ? (long)nnx : (long)nnx * nny;
-  const long n_base = D == 1 ? i + (long)nnx * nny * t : i + (long)nnx * t;
-  // transverse node indices of the line (for the cell gather)
-  const int it1 = i, it2 = t;
-  auto load_node = [&](int ap, double* P, double& X) {
-    const long m2 = n_base + ap * nD;
-#pragma unroll
-    for (int cc = 0; cc < NC; ++cc) P[cc] = first ? S.u[cc][m2] : Pin[cc * nn + m2];
-    X = S.xlc[D][m2];
-  };
-  double Pm[NC], P0[NC], P1[NC], Gm[NC], G0[NC], e01[NC];
-  double Xm = 0., X0 = 0., X1 = 0., Fb = 0.;
-  bool v0 = false, v1 = false;
-#pragma unroll
-  for (int cc = 0; cc < NC; ++cc) {
-    Pm[cc] = 0.;
-    P0[cc] = 0.;
-    P1[cc] = 0.;
-    Gm[cc] = 0.;
-    G0[cc] = 0.;
-    e01[cc] = 0.;
-  }
-  {  // node a0 - 2 enters as "a+1" of the first step (a = a0 - 3)
-    const int ap = a0 - 2;
-    v1 = ap >= 0 && ap <= na;
-    if (v1) load_node(ap, P1, X1);
-  }
-  for (int a = a0 - 3; a < a1; ++a) {
-    const int ap = a + 2;
-    const bool vp = ap >= 0 && ap <= na;
-    double P2[NC], X2 = 0.;
-#pragma unroll
-    for (int cc = 0; cc < NC; ++cc) P2[cc] = 0.;
-    if (vp) load_node(ap, P2, X2);
-    // edge a+3/2 gradient; limited gradient of node a+1
-    const bool in1 = S.order2 && a + 1 >= 1 && a + 1 <= na - 1;
-    double e12[NC], G1[NC];
-#pragma unroll
-    for (int cc = 0; cc < NC; ++cc) {
-      e12[cc] = (v1 && vp) ? (P2[cc] - P1[cc]) / (X2 - X1) : 0.;
-      G1[cc] = in1 ? dual_limited(S, e01[cc], e12[cc]) : 0.;
-    }
-    // mass flux of edge a+1/2 (front of node a)
-    const double Ff = dual_edge_flux<DIM, D>(S, a, it1, it2);
-    if (a >= a0 && v0) {
-      const long n = n_base + a * nD;
-      double U[NC + 1];
-      if (first) {
-        const double mass = S.mn[n];
-#pragma unroll
-        for (int cc = 0; cc < NC; ++cc) U[cc] = mass * S.u[cc][n];
-        U[NC] = mass;
-      } else {
-#pragma unroll
-        for (int cc = 0; cc <= NC; ++cc) U[cc] = S.Ud[cc * nn + n];
-      }
-      if (a >= 1 && a <= na - 1) dual_node_update<DIM, D>(S, n, U, Pm, P0, P1, Xm, X0, X1, Gm, G0, G1, Fb, Ff, Pout, last);
-      else dual_node_keep<DIM>(S, n, U, P0, Pout, last);
-    }
-#pragma unroll
-    for (int cc = 0; cc < NC; ++cc) {
-      Pm[cc] = P0[cc];
-      P0[cc] = P1[cc];
-      P1[cc] = P2[cc];
-      Gm[cc] = G0[cc];
-      G0[cc] = G1[cc];
-      e01[cc] = e12[cc];
-    }
-    Xm = X0;
-    X0 = X1;
-    X1 = X2;
-    Fb = Ff;
-    v0 = v1;
-    v1 = vp;
-  }
 }
 
 // The same sweep for D = y or z, LDS tiles: a block of 64 x 4 threads owns 64 cells along

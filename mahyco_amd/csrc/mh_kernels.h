@@ -707,8 +707,10 @@ __global__ __launch_bounds__(MH_BLOCK) void k_sweep_cell_row(MhDev S, int usel, 
   if (inb) face_quant<DIM, 0>(S, b, area, xf, vf);
   const bool upprev = vf * dxb > 0.0;
   const double hb = (double)S.order2 * (xf - (upprev ? xcp : xc));
+  double unext = valid ? Uin[c] : 0.;  // slot loads run one slot ahead
   for (int v = 0; v < NV; ++v) {
-    const double uv = valid ? Uin[v * nc + c] : 0.;
+    const double uv = unext;
+    if (v + 1 < NV) unext = valid ? Uin[(v + 1) * nc + c] : 0.;
     const double phi = valid ? uv / den : 0.;
     const double phip = __shfl_up(phi, 1);
     const double gb = (valid && validp) ? (phi - phip) / dxb : 0.;
@@ -910,7 +912,10 @@ __global__ __launch_bounds__(MH_BLOCK) void k_sweep_node_row(MhDev S, int psel, 
 #define MH_TILE_R (MH_TILE_L + 4)  // rows held, halos included
 #define MH_TILE_Q (MH_TILE_R / 4)  // rows per thread
 template <int DIM, int D, int M>
-__global__ __launch_bounds__(MH_BLOCK) void k_sweep_cell_tile(MhDev S, int usel, int first) {
+#ifndef MH_TILE_MIN_WAVES
+#define MH_TILE_MIN_WAVES 4  // waves per SIMD the register allocation must allow
+#endif
+__global__ __launch_bounds__(MH_BLOCK, MH_TILE_MIN_WAVES) void k_sweep_cell_tile(MhDev S, int usel, int first) {
   constexpr int NV = 3 * M + 1;
   __shared__ double sphi[MH_TILE_R][64];
   __shared__ double sgf[MH_TILE_R][64];  // face gradients, then face fluxes
@@ -980,6 +985,9 @@ __global__ __launch_bounds__(MH_BLOCK) void k_sweep_cell_tile(MhDev S, int usel,
       w[q] = 1. * vf * area * dt;
     }
   }
+  double un_next[MH_TILE_Q];  // slot loads run one slot ahead
+#pragma unroll
+  for (int q = 0; q < MH_TILE_Q; ++q) un_next[q] = val[q] ? Uin[c_base + (abase + ly + 4 * q) * sD] : 0.;
   for (int v = 0; v < NV; ++v) {
     // barriers: the previous slot read sphi and ssl before its last barrier, and reads
     // sgf (fluxes) before the first barrier of this slot
@@ -987,7 +995,8 @@ __global__ __launch_bounds__(MH_BLOCK) void k_sweep_cell_tile(MhDev S, int usel,
 #pragma unroll
     for (int q = 0; q < MH_TILE_Q; ++q) {
       const int a = abase + ly + 4 * q;
-      uq[q] = val[q] ? Uin[v * nc + c_base + a * sD] : 0.;
+      uq[q] = un_next[q];
+      if (v + 1 < NV) un_next[q] = val[q] ? Uin[(v + 1) * nc + c_base + a * sD] : 0.;
       sphi[ly + 4 * q][lx] = val[q] ? uq[q] / den[q] : 0.;
     }
     __syncthreads();

@@ -11,7 +11,7 @@ import numpy as np
 from . import types as T
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "libmahyco_hip.so")
+LIB_PATH = os.environ.get("MAHYCO_HIP_LIB") or os.path.join(_HERE, "_build", "libmahyco_hip.so")
 _lib = None
 
 

@@ -68,19 +68,30 @@ def measured_traffic(kernel, args):
     return None if k is None else k["bytes_per_launch"]
 
 
-def cpu_baseline(case_fn, cycles, threads):
-    """The oracle (C++ restatement of the reference, OpenMP) on the host cores."""
+def cpu_baseline(case_fn, cycles, threads, single_cycles=1):
+    """The oracle (C++ restatement of the reference, OpenMP) on the host cores: `cycles`
+    iterations on `threads` threads, then `single_cycles` on one thread."""
     from oracle.pyoracle import OracleCase
     case = case_fn()
+    nc = case.nb_cells
     o = OracleCase(case, pow_mode=0, threads=threads)
     o.run(1)  # first iteration outside the timed region (page-in)
     t0 = time.perf_counter()
     o.run(cycles)
     dt = time.perf_counter() - t0
-    nc = case.nb_cells
     o.close()
-    return {"value": nc * cycles / dt, "unit": "cell-cycles/s", "cores": threads, "kind": "port",
-            "sample": f"{case.name}: {cycles} iterations after 1 untimed, {dt:.1f} s, OMP threads={threads}"}
+    out = {"value": nc * cycles / dt, "unit": "cell-cycles/s", "cores": threads, "kind": "port",
+           "sample": f"{case.name}: {cycles} iterations after 1 untimed, {dt:.1f} s, OMP threads={threads}"}
+    if single_cycles > 0:
+        o = OracleCase(case, pow_mode=0, threads=1)
+        o.run(1)
+        t0 = time.perf_counter()
+        o.run(single_cycles)
+        d1 = time.perf_counter() - t0
+        o.close()
+        out["one_thread"] = {"value": nc * single_cycles / d1, "unit": "cell-cycles/s", "cores": 1,
+                             "sample": f"{single_cycles} iteration(s) after 1 untimed, {d1:.1f} s"}
+    return out
 
 
 def main():
@@ -95,6 +106,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=12)
     ap.add_argument("--cpu-cycles", type=int, default=4)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-single-cycles", type=int, default=1, help="iterations timed on one host thread")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (scale reference for HBM counters)")
@@ -183,7 +195,7 @@ def main():
         copy_calibration(args.copy_calibration)
     if not args.no_cpu:
         small = lambda: cases.config2_bisod3d(args.nx, args.ny, args.nz)
-        out["cpu_baseline"] = cpu_baseline(small, args.cpu_cycles, args.cpu_threads)
+        out["cpu_baseline"] = cpu_baseline(small, args.cpu_cycles, args.cpu_threads, args.cpu_single_cycles)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

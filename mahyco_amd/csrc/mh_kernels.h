@@ -1545,8 +1545,14 @@ __global__ __launch_bounds__(MH_BLOCK) void k_hist_final(MhDev S) { k_hist_final
 // is tab[bmap[b].x] (meshes of one launch share DIM, M and the sweep direction).
 // Single-mesh grids use the XCD-contiguous block order of xcd_block.
 
-#define MH_WRAP(NAME, TPL, TARGS, PARAMS, ARGS)                                              \
-  TPL __global__ __launch_bounds__(MH_BLOCK) void NAME(MhDev S PARAMS) {                     \
+#ifndef MH_CELL_MIN_WAVES
+#define MH_CELL_MIN_WAVES 1  // register budget of k_cell_lag, in waves per SIMD
+#endif
+#ifndef MH_NODE_MIN_WAVES
+#define MH_NODE_MIN_WAVES 1  // register budget of k_node_lag3_tiled, in waves per SIMD
+#endif
+#define MH_WRAP(NAME, WAVES, TPL, TARGS, PARAMS, ARGS)                                      \
+  TPL __global__ __launch_bounds__(MH_BLOCK, WAVES) void NAME(MhDev S PARAMS) {              \
     NAME##_body TARGS(S, xcd_block((int)blockIdx.x, (int)gridDim.x) ARGS);                   \
   }                                                                                          \
   TPL __global__ __launch_bounds__(MH_BLOCK) void NAME##_batch(const MhDev* __restrict__ tab, \
@@ -1555,16 +1561,16 @@ __global__ __launch_bounds__(MH_BLOCK) void k_hist_final(MhDev S) { k_hist_final
     NAME##_body TARGS(tab[bm.x], bm.y ARGS);                                                 \
   }
 #define MH_COMMA ,
-MH_WRAP(k_node_lag, template <int DIM>, <DIM>, , )
-MH_WRAP(k_cell_lag, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, , )
-MH_WRAP(k_sweep_cell, template <int DIM MH_COMMA int D MH_COMMA int M>, <DIM MH_COMMA D MH_COMMA M>,
+MH_WRAP(k_node_lag, 1, template <int DIM>, <DIM>, , )
+MH_WRAP(k_cell_lag, MH_CELL_MIN_WAVES, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, , )
+MH_WRAP(k_sweep_cell, 1, template <int DIM MH_COMMA int D MH_COMMA int M>, <DIM MH_COMMA D MH_COMMA M>,
         MH_COMMA int usel MH_COMMA int first, MH_COMMA usel MH_COMMA first)
-MH_WRAP(k_sweep_node, template <int DIM MH_COMMA int D>, <DIM MH_COMMA D>,
+MH_WRAP(k_sweep_node, 1, template <int DIM MH_COMMA int D>, <DIM MH_COMMA D>,
         MH_COMMA int psel MH_COMMA int first MH_COMMA int last, MH_COMMA psel MH_COMMA first MH_COMMA last)
-MH_WRAP(k_node_velocity, template <int DIM>, <DIM>, MH_COMMA int materialize, MH_COMMA materialize)
-MH_WRAP(k_finalize, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, MH_COMMA int usel, MH_COMMA usel)
+MH_WRAP(k_node_velocity, 1, template <int DIM>, <DIM>, MH_COMMA int materialize, MH_COMMA materialize)
+MH_WRAP(k_finalize, 1, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, MH_COMMA int usel, MH_COMMA usel)
 
-__global__ __launch_bounds__(MH_BLOCK) void k_node_lag3_tiled(MhDev S) {
+__global__ __launch_bounds__(MH_BLOCK, MH_NODE_MIN_WAVES) void k_node_lag3_tiled(MhDev S) {
   k_node_lag3_tiled_body(S, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 

@@ -104,9 +104,9 @@ def main():
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--graph", type=int, default=6, help="cycles per hipGraph (0: eager launches)")
     ap.add_argument("--profile-steps", type=int, default=12)
-    ap.add_argument("--cpu-cycles", type=int, default=4)
+    ap.add_argument("--cpu-cycles", type=int, default=20)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--cpu-single-cycles", type=int, default=1, help="iterations timed on one host thread")
+    ap.add_argument("--cpu-single-cycles", type=int, default=4, help="iterations timed on one host thread")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (scale reference for HBM counters)")

@@ -439,7 +439,7 @@ void set_parity(MhHandle* h) {
 template <int DIM, int M>
 void launch_node_t(MhHandle* h) {
   if (DIM == 3 && h->node_tiled) {
-    const dim3 grid((h->g.nnx + MH_NT - 1) / MH_NT, (h->g.nny + MH_NT - 1) / MH_NT,
+    const dim3 grid((h->g.nnx + MH_NTX - 1) / MH_NTX, (h->g.nny + MH_NTY - 1) / MH_NTY,
                     (h->g.nnz + MH_NT_KC - 1) / MH_NT_KC);
     LAUNCH(h, KN_NODE, k_node_lag3_tiled, grid, MH_BLOCK, h->dev);
   } else {

@@ -175,16 +175,20 @@ __device__ __forceinline__ void k_node_lag_body(const MhDev& S, int lb) {
 }
 
 
-// 3D node kernel, z-marching tiles: a block owns 15x15 node columns over MH_NT_KC node
-// planes, so each cell plane of the tile is 16x16 cells, one per thread.  Each cell plane
+// 3D node kernel, z-marching tiles: a block owns MH_NTX x MH_NTY node columns over
+// MH_NT_KC node planes, so each cell plane of the tile is (MH_NTX+1) x (MH_NTY+1) = 256
+// cells, one per thread.  Each cell plane
 // is processed once: its corner vectors (cqs_hex, computed once per cell instead of once
 // per corner) are multiplied by p+q and kept in LDS with M/8 -- the bottom corners for
 // the node plane it sits on, the top ones (held in registers meanwhile) for the next
 // plane.  Every node then adds its 8 cells in cell order, as k_node_lag does, so both
 // kernels give the same bits.
-#define MH_NT 15
+#ifndef MH_NTX
+#define MH_NTX 31  // node columns of a tile along x
+#endif
+#define MH_NTY (MH_BLOCK / (MH_NTX + 1) - 1)  // along y: (MH_NTX + 1)(MH_NTY + 1) cells = MH_BLOCK
 #define MH_NT_KC 8
-#define MH_NT_CELLS ((MH_NT + 1) * (MH_NT + 1))  // = MH_BLOCK
+#define MH_NT_CELLS ((MH_NTX + 1) * (MH_NTY + 1))  // = MH_BLOCK
 __device__ __forceinline__ void nt_cell(const MhDev& S, int ii, int jj, int kk, double* lds, int s, double* keep) {
   // corner force contributions (p + q) CQS_l and the node-mass share M/8 of one cell
   const long nnx = S.g.nnx, nnxy = (long)S.g.nnx * S.g.nny;
@@ -231,13 +235,13 @@ __device__ __forceinline__ void k_node_lag3_tiled_body(const MhDev& S, int bx, i
   __shared__ double up[13 * MH_NT_CELLS];  // bottom corners (0..3) of the cell plane above
   if (mh_halted(S.sc)) return;
   const int tid = threadIdx.x;
-  const int i0 = bx * MH_NT, j0 = by * MH_NT, k0 = bz * MH_NT_KC;
+  const int i0 = bx * MH_NTX, j0 = by * MH_NTY, k0 = bz * MH_NT_KC;
   const int nx = S.g.nx, ny = S.g.ny, nz = S.g.nz;
   const int kend = min(k0 + MH_NT_KC, S.g.nnz);
   const double dt = S.sc->dt;
   const double dth = 0.5 * (S.sc->dt_old + dt);
-  // this thread's cell column of the tile: (i0 - 1 + tid % 16, j0 - 1 + tid / 16)
-  const int ci = i0 - 1 + tid % (MH_NT + 1), cj = j0 - 1 + tid / (MH_NT + 1);
+  // this thread's cell column of the tile
+  const int ci = i0 - 1 + tid % (MH_NTX + 1), cj = j0 - 1 + tid / (MH_NTX + 1);
   const bool cell_ok = ci >= 0 && ci < nx && cj >= 0 && cj < ny;
   double keep[13];
   // prologue: top corners of the cell plane k0 - 1
@@ -245,29 +249,29 @@ __device__ __forceinline__ void k_node_lag3_tiled_body(const MhDev& S, int bx, i
     nt_cell(S, ci, cj, k0 - 1, up, tid, keep);
     for (int v = 0; v < 13; ++v) lo[v * MH_NT_CELLS + tid] = keep[v];
   }
-  // this thread's node column: (i0 + tid % 15, j0 + tid / 15) for tid < 225
-  const int i = i0 + tid % MH_NT, j = j0 + tid / MH_NT;
-  const bool node_ok = tid < MH_NT * MH_NT && i < S.g.nnx && j < S.g.nny;
-  const int tx = tid % MH_NT, ty = tid / MH_NT;
+  // this thread's node column (threads below MH_NTX * MH_NTY)
+  const int i = i0 + tid % MH_NTX, j = j0 + tid / MH_NTX;
+  const bool node_ok = tid < MH_NTX * MH_NTY && i < S.g.nnx && j < S.g.nny;
+  const int tx = tid % MH_NTX, ty = tid / MH_NTX;
   for (int k = k0; k < kend; ++k) {
     // cell plane k: bottom corners to LDS now, top corners kept for node plane k + 1
     if (k < nz && cell_ok) nt_cell(S, ci, cj, k, up, tid, keep);
     __syncthreads();
     if (node_ok) {
       double mass = 0., fx = 0., fy = 0., fz = 0.;
-      const int s00 = ty * (MH_NT + 1) + tx;  // cell (i-1, j-1)
+      const int s00 = ty * (MH_NTX + 1) + tx;  // cell (i-1, j-1)
       const bool okx0 = i >= 1, okx1 = i < nx, oky0 = j >= 1, oky1 = j < ny;
       if (k >= 1) {  // cells k-1, corners 4..7 -> slots 0..3 of lo
         nt_add(lo, s00, 2, okx0 && oky0, mass, fx, fy, fz);
         nt_add(lo, s00 + 1, 3, okx1 && oky0, mass, fx, fy, fz);
-        nt_add(lo, s00 + MH_NT + 1, 1, okx0 && oky1, mass, fx, fy, fz);
-        nt_add(lo, s00 + MH_NT + 2, 0, okx1 && oky1, mass, fx, fy, fz);
+        nt_add(lo, s00 + MH_NTX + 1, 1, okx0 && oky1, mass, fx, fy, fz);
+        nt_add(lo, s00 + MH_NTX + 2, 0, okx1 && oky1, mass, fx, fy, fz);
       }
       if (k < nz) {  // cells k, corners 0..3
         nt_add(up, s00, 2, okx0 && oky0, mass, fx, fy, fz);
         nt_add(up, s00 + 1, 3, okx1 && oky0, mass, fx, fy, fz);
-        nt_add(up, s00 + MH_NT + 1, 1, okx0 && oky1, mass, fx, fy, fz);
-        nt_add(up, s00 + MH_NT + 2, 0, okx1 && oky1, mass, fx, fy, fz);
+        nt_add(up, s00 + MH_NTX + 1, 1, okx0 && oky1, mass, fx, fy, fz);
+        nt_add(up, s00 + MH_NTX + 2, 0, okx1 && oky1, mass, fx, fy, fz);
       }
       const long n = i + (long)S.g.nnx * (j + (long)S.g.nny * k);
       node_finish<3>(S, n, i, j, k, dt, dth, mass, fx, fy, fz);

@@ -187,5 +187,7 @@ def test_sweep_rows_and_tiles_equal_flat(monkeypatch, make):
         g.run(18)
         res[mode] = fields(g, len(c.envs), c.dim)
         g.close()
-    for k in res["flat"]:
-        assert bitwise_equal(res["flat"][k], res["tile"][k]), k
+    ref = res["flat"]
+    for key, r in res.items():
+        for k in ref:
+            assert bitwise_equal(ref[k], r[k]), (key, k)

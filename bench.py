@@ -250,6 +250,7 @@ def bench_serve(args):
     """Poisson stream of 2D (128^2..1024^2) and 3D (32^3..160^3) stiffened-gas runs to t=0.1,
     admitted by HBM footprint into a continuously batched ensemble."""
     from mahyco_amd import serve
+    from mahyco_amd import types as T
     reqs = serve.serve_requests(args.runs, seed=args.seed, rate=args.rate)
     print(f"[bench] serve: {len(reqs)} requests at {args.rate}/s", file=sys.stderr, flush=True)
     recs = serve.serve(reqs, budget=96e9, max_slots=64)
@@ -260,7 +261,8 @@ def bench_serve(args):
            "config": {"workload": "served_stream_2d3d_stiffened", "runs": len(reqs), "t_end": 0.1,
                       "hbm_budget_GB": 96},
            "p50_s": stats["p50_s"], "p99_s": stats["p99_s"], "span_s": stats["span_s"],
-           "failed": sum(1 for r in recs if r.iterations < 0)}
+           "failed": [{"run": reqs[i][1].name, "status": T.STATUS_NAMES.get(r.status, r.status)}
+                      for i, r in enumerate(recs) if r.iterations < 0]}
     print(json.dumps(out))
 
 

@@ -120,6 +120,7 @@ double axis_coord(const MhMesh& mesh, int d, int i, int n) {
 // ComputeCellMass (:221-237).  The node mass is formed by the first node kernel.
 struct HostInit {
   std::vector<double> ax, ay, az, VE, rho, p, c, q, M, e, fg, fk, chik, rhok, ek, pk, Mk;
+  std::vector<double> chig;  // global mass fraction: 1 everywhere, as Arcane starts it
   std::vector<uint8_t> mask;
   std::vector<double> u[3];
 };
@@ -148,6 +149,7 @@ int host_init(MhHandle* h, HostInit& I) {
   I.fg.assign(nc, 1.);
   I.fk.assign(ne, 1.);
   I.chik.assign(ne, 1.);
+  I.chig.assign(nc, 1.);
   I.rhok.assign(ne, 0.);
   I.ek.assign(ne, 0.);
   I.pk.assign(ne, 0.);
@@ -544,7 +546,7 @@ size_t case_footprint(const MhMesh& mesh, int m, int with_proj) {
   const long nn = (long)(mesh.nx + 1) * (mesh.ny + 1) * (mesh.dim == 3 ? mesh.nz + 1 : 1);
   const long nv = 3 * m + 1;
   long words = (mesh.nx + mesh.ny + mesh.nz + 3) + nc /*VE*/ + 10 * nn /*u, xl x2, mn*/ + 7 * nc +
-               6L * m * nc + 2 * nc + (nc + nn) / MH_BLOCK + 8;
+               6L * m * nc + 3 * nc + 2 * ((nc + nn) / MH_BLOCK + 8);
   if (with_proj) words += 2 * nv * nc + m * nc + 10 * nn;
   return (size_t)words * 8 + (size_t)nc;
 }
@@ -812,13 +814,14 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
   ok |= dalloc(h, &S.mask, nc);
   ok |= dalloc(h, &S.fk, ne);
   ok |= dalloc(h, &S.chik, ne);
+  ok |= dalloc(h, &S.chig, nc);
   ok |= dalloc(h, &S.rhok, ne);
   ok |= dalloc(h, &S.ek, ne);
   ok |= dalloc(h, &S.pk, ne);
   ok |= dalloc(h, &S.Mk, ne);
   ok |= dalloc(h, &S.L, nc);
   ok |= dalloc(h, &S.Vl, nc);
-  ok |= dalloc(h, &S.partial, (size_t)std::max(h->nblk_c, h->nblk_n));
+  ok |= dalloc(h, &S.partial, 2 * (size_t)std::max(h->nblk_c, h->nblk_n));  // CFL partials: 2 per block
   ok |= dalloc(h, &S.err, 4);
   if (ext_sc) S.sc = ext_sc;
   else ok |= dalloc(h, &S.sc, 1);
@@ -887,6 +890,7 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
   ok |= upload(h, S.mask, I.mask);
   ok |= upload(h, S.fk, I.fk);
   ok |= upload(h, S.chik, I.chik);
+  ok |= upload(h, S.chig, I.chig);
   ok |= upload(h, S.rhok, I.rhok);
   ok |= upload(h, S.ek, I.ek);
   ok |= upload(h, S.pk, I.pk);
@@ -1072,10 +1076,12 @@ int mahyco_get_field(MhHandle* h, int field, int env, int comp, double* out) {
     if ((st = d2h(S.fk, fk.data(), (long)h->m * nc)) != MH_OK) return st;
     const bool lagv = !h->opt.with_projection && sc.iteration > 0;
     if ((st = d2h(lagv ? S.Vl : S.VE, V.data(), nc)) != MH_OK) return st;
-    std::vector<double> chik;
+    std::vector<double> chik, chig;
     if (field == MH_F_MASS_FRACTION) {
       chik.resize((size_t)h->m * nc);
+      chig.resize(nc);
       if ((st = d2h(S.chik, chik.data(), (long)h->m * nc)) != MH_OK) return st;
+      if ((st = d2h(S.chig, chig.data(), nc)) != MH_OK) return st;
     }
     for (long c = 0; c < nc; ++c) {
       const uint8_t b = mask[c];
@@ -1095,7 +1101,7 @@ int mahyco_get_field(MhHandle* h, int field, int env, int comp, double* out) {
         else if ((b >> env) & 1) val = pure ? fg[c] : fk[env * nc + c];
       } else {
         if (env < 0) val = 0.;
-        else if ((b >> env) & 1) val = pure ? 1. : chik[env * nc + c];
+        else if ((b >> env) & 1) val = pure ? chig[c] : chik[env * nc + c];
       }
       out[c] = val;
     }

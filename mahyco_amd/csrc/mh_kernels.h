@@ -40,19 +40,68 @@ __device__ __forceinline__ double amax(double a, double b) { return (a < b) ? b 
 __device__ __forceinline__ D3 load3(double* const* a, long n) { return d3(a[0][n], a[1][n], a[2][n]); }
 
 // Block-wide minimum (Arcane math::min semantics on non-NaN input), written by thread 0.
-__device__ __forceinline__ void block_min_store(double v, double* out, int lb) {
-  __shared__ double red[MH_BLOCK / 64];
-  for (int off = 32; off > 0; off >>= 1) {
-    double o = __shfl_xor(v, off, 64);
-    v = amin(v, o);
+// CFL minimum with the reference's sequential semantics.  computeDeltaT folds
+// min = math::min(min, dx) over the cells in order (MahycoModule.cc:1777), and math::min(a,
+// b) = a < b ? a : b turns NaN at a NaN candidate (an emptied cell's sound speed) and
+// restarts at the next cell: the result is the minimum over the cells after the last NaN
+// candidate, NaN if the last cell is one.  A run of cells therefore reduces to (nan: a NaN
+// was seen, m: minimum after the last NaN, NaN if there is no value there), combined in
+// cell order; (NaN, no NaN seen) is the empty run.  The reference's start value (the
+// largest double) is applied once, at the end.
+struct MhCflMin {
+  double m;
+  int nan;
+};
+__device__ __forceinline__ double min_skip_nan(double a, double b) { return (b < a || a != a) ? b : a; }
+__device__ __forceinline__ MhCflMin cfl_leaf(double x, bool active) {
+  MhCflMin r;
+  r.nan = active && x != x;
+  r.m = active ? x : __longlong_as_double(0x7ff8000000000000LL);  // inactive: the empty run
+  return r;
+}
+__device__ __forceinline__ MhCflMin cfl_combine(MhCflMin l, MhCflMin r) {
+  if (r.nan) return r;
+  MhCflMin o;
+  o.nan = l.nan;
+  o.m = min_skip_nan(l.m, r.m);
+  return o;
+}
+// in-order reduction over a wavefront (lane 0 ends with lanes 0..63 combined)
+__device__ __forceinline__ MhCflMin cfl_wave(MhCflMin v) {
+  const int lane = threadIdx.x & 63;
+  for (int off = 1; off < 64; off <<= 1) {
+    MhCflMin o;
+    o.m = __shfl_down(v.m, off, 64);
+    o.nan = __shfl_down(v.nan, off, 64);
+    if (lane + off < 64) v = cfl_combine(v, o);
   }
+  return v;
+}
+
+// Per-block CFL partial of the cells of block lb, in cell order: out[2 lb] = minimum after
+// the last NaN, out[2 lb + 1] = 1 if a NaN was seen.
+__device__ __forceinline__ void block_min_store(double v, bool active, double* out, int lb) {
+  __shared__ double rm[MH_BLOCK / 64];
+  __shared__ int rn[MH_BLOCK / 64];
+  MhCflMin r = cfl_wave(cfl_leaf(v, active));
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) red[wid] = v;
+  if (lane == 0) {
+    rm[wid] = r.m;
+    rn[wid] = r.nan;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double r = red[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = amin(r, red[w]);
-    out[lb] = r;
+    MhCflMin t;
+    t.m = rm[0];
+    t.nan = rn[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      MhCflMin o;
+      o.m = rm[w];
+      o.nan = rn[w];
+      t = cfl_combine(t, o);
+    }
+    out[2 * lb] = t.m;
+    out[2 * lb + 1] = t.nan ? 1. : 0.;
   }
 }
 
@@ -471,7 +520,7 @@ __device__ __forceinline__ void k_cell_lag_body(const MhDev& S, int lb) {
     S.Vl[c] = V;
     S.L[c] = L;
   }
-  if (!S.with_proj) block_min_store(dxs, S.partial, lb);
+  if (!S.with_proj) block_min_store(dxs, active, S.partial, lb);
 }
 
 // ----------------------------------------------------------------------------- remap sweeps
@@ -1271,10 +1320,19 @@ __device__ __forceinline__ void k_finalize_body(const MhDev& S, int lb, int usel
           if ((nm >> kk) & 1) chi[kk] *= unsurfm;
       }
     } else {
-      // the reference keeps the previous mass fractions of such (vacuum) cells; they are
-      // not stored for pure cells here, so the case is reported
-      S.err[0] = MH_ERR_UNSUPPORTED;
-      for (int kk = 0; kk < M; ++kk) chi[kk] = 0.;
+      // (near-)empty cell: the reference keeps the mass fractions it holds after the
+      // membership change -- the global value for a single-environment cell (taken from
+      // the remaining environment's partial value when the cell was mixed), the partial
+      // values for a mixed cell (0 for environments that have just entered; all partial
+      // values restart from 0 and the global value when a pure cell turns mixed)
+      const int on = __popc(om), nnb = __popc(nm);
+      for (int kk = 0; kk < M; ++kk) {
+        chi[kk] = 0.;
+        if (!((nm >> kk) & 1)) continue;
+        if (nnb == 1) chi[kk] = on == 1 ? S.chig[c] : S.chik[kk * nc + c];
+        else if (on == 1) chi[kk] = ((om >> kk) & 1) ? S.chig[c] : 0.;
+        else chi[kk] = ((om >> kk) & 1) ? S.chik[kk * nc + c] : 0.;
+      }
     }
     double density = 0.;
     for (int kk = 0; kk < M; ++kk) {
@@ -1351,13 +1409,27 @@ __device__ __forceinline__ void k_finalize_body(const MhDev& S, int lb, int usel
         S.Mk[o] = Mk[kk];
       }
     }
-    // the global volume fraction is the environment value of a single-environment cell;
-    // a mixed cell keeps the global value it had (Arcane leaves it untouched)
+    // the global volume and mass fractions are the environment values of a
+    // single-environment cell; a mixed cell keeps the global values it had (Arcane leaves
+    // them untouched).  Partial mass fractions follow Arcane's membership change: all
+    // restart from 0 when a pure cell turns mixed, and the environment that replaces the
+    // only one of a pure cell starts from 0
     if (pure) {
-      double fg = 0.;
+      double fg = 0., cg = 0.;
       for (int kk = 0; kk < M; ++kk)
-        if ((nm >> kk) & 1) fg = f[kk];
+        if ((nm >> kk) & 1) {
+          fg = f[kk];
+          cg = chi[kk];
+        }
       S.fg[c] = fg;
+      S.chig[c] = cg;
+    }
+    if (nm != om && __popc(om) == 1) {  // a mixed cell's old partial values are kept
+      for (int kk = 0; kk < M; ++kk) {
+        const bool now = (nm >> kk) & 1, was = (om >> kk) & 1;
+        if (now && !pure) continue;  // written below with the cell's partial values
+        if (pure ? (now && !was) : !now) S.chik[kk * nc + c] = 0.;
+      }
     }
     S.mask[c] = nm;
     S.rho[c] = rho_g;
@@ -1374,7 +1446,7 @@ __device__ __forceinline__ void k_finalize_body(const MhDev& S, int lb, int usel
     const long b = i + (long)S.g.nnx * (j + (long)S.g.nny * k);
     dxs = S.L[c] / (c_g + cell_vmax<DIM>(S, b));
   }
-  block_min_store(dxs, S.partial, lb);
+  block_min_store(dxs, active, S.partial, lb);
 }
 
 // applyEOS of one environment on its member cells (PerfectGasEOSService.cc:45-67,
@@ -1425,17 +1497,36 @@ __global__ __launch_bounds__(MH_BLOCK) void k_pression_moyenne(MhDev S) {
 // of the iteration and the final-time stop rule.  One block.
 __device__ __forceinline__ void k_dt_body(const MhDev& S) {
   const int nblocks = S.nblk_c;
-  __shared__ double red[MH_DT_BLOCK / 64];
+  __shared__ double rm[MH_DT_BLOCK / 64];
+  __shared__ int rn[MH_DT_BLOCK / 64];
   MhScalars* sc = S.sc;
   if (mh_halted(sc)) return;
-  double v = 1.79769313486231570815e+308;
-  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) v = amin(v, S.partial[b]);
-  for (int off = 32; off > 0; off >>= 1) v = amin(v, __shfl_xor(v, off, 64));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  // block partials in block (= cell) order: thread t folds a contiguous run of them
+  const int per = (nblocks + (int)blockDim.x - 1) / (int)blockDim.x;
+  MhCflMin v = cfl_leaf(0., false);
+  for (int b = threadIdx.x * per; b < min(nblocks, ((int)threadIdx.x + 1) * per); ++b) {
+    MhCflMin o;
+    o.m = S.partial[2 * b];
+    o.nan = S.partial[2 * b + 1] != 0.;
+    v = cfl_combine(v, o);
+  }
+  v = cfl_wave(v);
+  if ((threadIdx.x & 63) == 0) {
+    rm[threadIdx.x >> 6] = v.m;
+    rn[threadIdx.x >> 6] = v.nan;
+  }
   __syncthreads();
   if (threadIdx.x != 0) return;
-  double mn = red[0];
-  for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mn = amin(mn, red[w]);
+  MhCflMin tot;
+  tot.m = rm[0];
+  tot.nan = rn[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+    MhCflMin o;
+    o.m = rm[w];
+    o.nan = rn[w];
+    tot = cfl_combine(tot, o);
+  }
+  const double mn = tot.nan ? tot.m : amin(1.79769313486231570815e+308, tot.m);
   const double dt = sc->dt;
   const double t = sc->t + dt;
   double new_dt = S.cfl * mn;

@@ -67,6 +67,7 @@ struct MhDev {
   double *rho, *p, *c, *q, *M, *e, *fg;
   uint8_t* mask;                              // environment membership bits
   double *fk, *chik, *rhok, *ek, *pk, *Mk;  // [m][nc], mixed cells only
+  double* chig;                               // global mass fraction (single-environment cells)
   double *L, *Vl;                             // characteristic length, Lagrangian volume
   double* partial;                            // per-block CFL minima
   int* err;                                   // error word raised inside a cycle

@@ -88,6 +88,7 @@ class Served:
     done: float = -1.0
     iterations: int = 0
     slot: int = -1
+    status: int = 0
 
 
 def serve(reqs, budget=64e9, max_slots=64, step_cycles=6, graph=True):
@@ -124,6 +125,7 @@ def serve(reqs, budget=64e9, max_slots=64, step_cycles=6, graph=True):
                 i = live.pop(slot)
                 recs[i].done = now()
                 recs[i].iterations = int(it[slot])
+                recs[i].status = int(st[slot])
                 if st[slot] < 0:
                     recs[i].iterations = -1
                 E.remove(slot)

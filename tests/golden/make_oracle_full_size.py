@@ -1,14 +1,16 @@
-"""Digests of the oracle's state after full-size runs, for bitwise GPU checks at full size.
+"""Digests of the oracle's state after long runs, for bitwise GPU checks over whole meshes.
 
 The oracle (oracle/, the CPU restatement of the reference) runs the two single-mesh
 workloads at their full sizes -- the 3D bi-material tube 256x128x128 for 100 iterations
-and the 3D Sedov blast 192^3 for 200 iterations -- with the correctly rounded cube root
-that the GPU library uses.  The fixture keeps, per parity field, the SHA-256 of the
+and the 3D Sedov blast 192^3 for 200 iterations -- and two runs of the served stream whose
+cells empty out, to their final time, with the correctly rounded cube root that the GPU
+library uses.  The fixture keeps, per parity field, the SHA-256 of the
 float64 array (global cell / node order), its sum, and the x-profile at j = k = 0, plus
 time, time step and iteration as exact hex floats.  A GPU run that reproduces every digest
 is bitwise identical to the oracle over the whole mesh.
 
-    python tests/golden/make_oracle_full_size.py [bisod3d] [sedov3d]   (minutes, 8 threads)
+    python tests/golden/make_oracle_full_size.py [bisod3d] [sedov3d] [served114] [served162]
+    (minutes on 8 threads)
 """
 import json
 import os
@@ -21,9 +23,19 @@ sys.path.insert(0, ROOT)
 from mahyco_amd import cases  # noqa: E402
 from tests.helpers import summarize  # noqa: E402
 
+def _served(i):
+    """Request i of the seeded served stream (stiffened-gas Riemann runs to t = 0.1)."""
+    from mahyco_amd import serve
+    return serve.serve_requests(256, seed=20261202, rate=8.0)[i][1]
+
+
 WORKLOADS = {
     "bisod3d": (lambda: cases.config2_bisod3d(256, 128, 128), 100),
     "sedov3d": (lambda: cases.config3_sedov3d(192, 200), 200),
+    # two served runs whose cells empty out (total mass below the remap threshold squared):
+    # the mass fractions then keep their values across the membership change
+    "served114": (lambda: _served(114), 400),
+    "served162": (lambda: _served(162), 400),
 }
 
 

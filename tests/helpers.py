@@ -54,8 +54,10 @@ def bitwise_equal(a, b):
 
 
 def digest(arr):
-    """SHA-256 of a float64 array's bytes."""
+    """SHA-256 of a float64 array's bytes, every NaN written as the same quiet NaN (NaN
+    payloads and signs differ between the host CPU and the GPU; the values are NaN on both)."""
     a = np.ascontiguousarray(arr, dtype=np.float64)
+    a = np.where(np.isnan(a), np.float64(np.nan), a)
     return hashlib.sha256(a.tobytes()).hexdigest()
 
 

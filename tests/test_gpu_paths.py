@@ -118,16 +118,24 @@ def test_full_size_bisod_properties():
     g.close()
 
 
+def _served(i):
+    from mahyco_amd import serve
+    return serve.serve_requests(256, seed=20261202, rate=8.0)[i][1]
+
+
 FULL_SIZE = {
     "bisod3d": (lambda: cases.config2_bisod3d(256, 128, 128), 100),
     "sedov3d": (lambda: cases.config3_sedov3d(192, 200), 200),
+    "served114": (lambda: _served(114), 400),  # cells that empty out: kept mass fractions
+    "served162": (lambda: _served(162), 400),
 }
 
 
 @pytest.mark.parametrize("name", sorted(FULL_SIZE))
 def test_full_size_bitwise_vs_oracle(name):
-    """Full-size single-mesh workloads against the oracle's digests (SHA-256 of every parity
-    field over the whole mesh, exact time and time step; tests/golden/make_oracle_full_size.py)."""
+    """Full-size single-mesh workloads and two served runs with emptying cells against the
+    oracle's digests (SHA-256 of every parity field over the whole mesh, exact time and
+    time step; tests/golden/make_oracle_full_size.py)."""
     from mahyco_amd.lib import GpuCase
     path = os.path.join(ROOT, "tests", "golden", f"oracle_{name}_full.json")
     if not os.path.exists(path):

@@ -452,7 +452,6 @@ template <int DIM, int M>
 void launch_cell_t(MhHandle* h) {
   LAUNCH(h, KN_CELL, (k_cell_lag<DIM, M>), h->nblk_c, MH_BLOCK, h->dev);
 }
-
 #define MH_DISPATCH(fn, h)                                      \
   do {                                                          \
     const int _d = (h)->g.dim, _m = (h)->m;                     \

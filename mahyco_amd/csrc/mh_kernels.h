@@ -66,9 +66,14 @@ __device__ __forceinline__ MhCflMin cfl_combine(MhCflMin l, MhCflMin r) {
   o.m = min_skip_nan(l.m, r.m);
   return o;
 }
-// in-order reduction over a wavefront (lane 0 ends with lanes 0..63 combined)
+// in-order reduction over a wavefront (lane 0 ends with lanes 0..63 combined); without a
+// NaN in the wavefront it is a plain minimum, in any order
 __device__ __forceinline__ MhCflMin cfl_wave(MhCflMin v) {
   const int lane = threadIdx.x & 63;
+  if (__ballot(v.nan) == 0) {
+    for (int off = 32; off > 0; off >>= 1) v.m = min_skip_nan(v.m, __shfl_xor(v.m, off, 64));
+    return v;
+  }
   for (int off = 1; off < 64; off <<= 1) {
     MhCflMin o;
     o.m = __shfl_down(v.m, off, 64);
@@ -236,25 +241,47 @@ __device__ __forceinline__ void k_node_lag_body(const MhDev& S, int lb) {
 #define MH_NTX 31  // node columns of a tile along x
 #endif
 #define MH_NTY (MH_BLOCK / (MH_NTX + 1) - 1)  // along y: (MH_NTX + 1)(MH_NTY + 1) cells = MH_BLOCK
-#define MH_NT_KC 8
+#ifndef MH_NT_KC
+#define MH_NT_KC 8  // node planes per tile
+#endif
 #define MH_NT_CELLS ((MH_NTX + 1) * (MH_NTY + 1))  // = MH_BLOCK
-__device__ __forceinline__ void nt_cell(const MhDev& S, int ii, int jj, int kk, double* lds, int s, double* keep) {
-  // corner force contributions (p + q) CQS_l and the node-mass share M/8 of one cell
-  const long nnx = S.g.nnx, nnxy = (long)S.g.nnx * S.g.nny;
+// What one cell of a column needs beyond its lower face: the 4 nodes of its upper face
+// (previous Lagrangian positions) and p, q, M.  The lower face is the upper face of the
+// cell below, carried in registers.
+struct NtCellIn {
+  D3 top[4];
+  double p, q, M;
+};
+__device__ __forceinline__ void nt_load_face(const MhDev& S, int ii, int jj, int kn, D3* f) {
+  const long nnx = S.g.nnx;
+  const long b = ii + nnx * (jj + (long)S.g.nny * kn);
+  f[0] = load3(S.xlp, b);
+  f[1] = load3(S.xlp, b + 1);
+  f[2] = load3(S.xlp, b + 1 + nnx);
+  f[3] = load3(S.xlp, b + nnx);
+}
+__device__ __forceinline__ void nt_load_cell(const MhDev& S, int ii, int jj, int kk, NtCellIn& in) {
   const long c = ii + (long)S.g.nx * (jj + (long)S.g.ny * kk);
-  const long b = ii + nnx * (jj + (long)S.g.nny * kk);
+  nt_load_face(S, ii, jj, kk + 1, in.top);
+  in.p = S.p[c];
+  in.q = S.q[c];
+  in.M = S.M[c];
+}
+// corner force contributions (p + q) CQS_l and the node-mass share M/8 of one cell: lower
+// corners to LDS slot s, upper corners to keep
+__device__ __forceinline__ void nt_cell(const D3* bot, const NtCellIn& in, double* lds, int s, double* keep) {
   D3 x[8], q[8];
-  x[0] = load3(S.xlp, b);
-  x[1] = load3(S.xlp, b + 1);
-  x[2] = load3(S.xlp, b + 1 + nnx);
-  x[3] = load3(S.xlp, b + nnx);
-  x[4] = load3(S.xlp, b + nnxy);
-  x[5] = load3(S.xlp, b + 1 + nnxy);
-  x[6] = load3(S.xlp, b + 1 + nnx + nnxy);
-  x[7] = load3(S.xlp, b + nnx + nnxy);
+  x[0] = bot[0];
+  x[1] = bot[1];
+  x[2] = bot[2];
+  x[3] = bot[3];
+  x[4] = in.top[0];
+  x[5] = in.top[1];
+  x[6] = in.top[2];
+  x[7] = in.top[3];
   cqs_hex(x, q);
-  const double pr = S.p[c] + S.q[c];
-  const double wm = 0.125 * S.M[c];
+  const double pr = in.p + in.q;
+  const double wm = 0.125 * in.M;
   for (int l = 0; l < 4; ++l) {
     lds[(3 * l + 0) * MH_NT_CELLS + s] = pr * q[l].x;
     lds[(3 * l + 1) * MH_NT_CELLS + s] = pr * q[l].y;
@@ -293,10 +320,22 @@ __device__ __forceinline__ void k_node_lag3_tiled_body(const MhDev& S, int bx, i
   const int ci = i0 - 1 + tid % (MH_NTX + 1), cj = j0 - 1 + tid / (MH_NTX + 1);
   const bool cell_ok = ci >= 0 && ci < nx && cj >= 0 && cj < ny;
   double keep[13];
-  // prologue: top corners of the cell plane k0 - 1
-  if (k0 >= 1 && cell_ok) {
-    nt_cell(S, ci, cj, k0 - 1, up, tid, keep);
-    for (int v = 0; v < 13; ++v) lo[v * MH_NT_CELLS + tid] = keep[v];
+  // the thread's cell column: lower face carried from plane to plane, the next cell's
+  // loads issued one plane ahead
+  D3 bot[4];
+  NtCellIn cur, nxt;
+  const int kc_end = min(kend, nz);  // cell planes k0 .. kc_end-1
+  if (cell_ok) {
+    if (k0 >= 1) {  // prologue: top corners of the cell plane k0 - 1
+      nt_load_face(S, ci, cj, k0 - 1, bot);
+      nt_load_cell(S, ci, cj, k0 - 1, cur);
+      nt_cell(bot, cur, up, tid, keep);
+      for (int v = 0; v < 13; ++v) lo[v * MH_NT_CELLS + tid] = keep[v];
+      for (int l = 0; l < 4; ++l) bot[l] = cur.top[l];
+    } else {
+      nt_load_face(S, ci, cj, k0, bot);
+    }
+    if (k0 < kc_end) nt_load_cell(S, ci, cj, k0, nxt);
   }
   // this thread's node column (threads below MH_NTX * MH_NTY)
   const int i = i0 + tid % MH_NTX, j = j0 + tid / MH_NTX;
@@ -304,7 +343,12 @@ __device__ __forceinline__ void k_node_lag3_tiled_body(const MhDev& S, int bx, i
   const int tx = tid % MH_NTX, ty = tid / MH_NTX;
   for (int k = k0; k < kend; ++k) {
     // cell plane k: bottom corners to LDS now, top corners kept for node plane k + 1
-    if (k < nz && cell_ok) nt_cell(S, ci, cj, k, up, tid, keep);
+    if (k < nz && cell_ok) {
+      cur = nxt;
+      if (k + 1 < kc_end) nt_load_cell(S, ci, cj, k + 1, nxt);
+      nt_cell(bot, cur, up, tid, keep);
+      for (int l = 0; l < 4; ++l) bot[l] = cur.top[l];
+    }
     __syncthreads();
     if (node_ok) {
       double mass = 0., fx = 0., fy = 0., fz = 0.;
@@ -403,22 +447,15 @@ __device__ __forceinline__ void eos(const MhEnvParam& ep, double rho, double e, 
 // ComputeArtificialViscosity (:386-426) + UpdateEnergyAndPressure (:1430-1480) for one cell.
 // With remap: writes U (V_k, V_k rho_k, V_k rho_k e_k, V q; PrepareRemap.cc:100-136), V, L.
 // Without: applies the EOS and the mean pressure and writes the CFL candidate.
+// The cell's work once its node coordinates x (at this cycle's Lagrangian positions) are
+// in registers; returns the CFL candidate of the Lagrange-only loop.
 template <int DIM, int M>
-__device__ __forceinline__ void k_cell_lag_body(const MhDev& S, int lb) {
+__device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, const D3* x) {
   double* __restrict__ U = S.Ua;
   constexpr int NV = 3 * M + 1;
-  const long c = lb * (long)blockDim.x + threadIdx.x;
-  const bool active = c < S.g.nc && !mh_halted(S.sc);
   double dxs = 1.79769313486231570815e+308;
-  if (active) {
-    const int nx = S.g.nx, ny = S.g.ny;
-    const int i = (int)(c % nx);
-    const long r = c / nx;
-    const int j = (int)(r % ny);
-    const int k = (int)(r / ny);
-    const long b = i + (long)S.g.nnx * (j + (long)S.g.nny * k);
-    D3 x[8], qv[8];
-    cell_nodes<DIM>(S, b, S.xlc, x);
+  {
+    D3 qv[8];
     double V;
     if (DIM == 3) {
       cqs_hex(x, qv);
@@ -519,6 +556,25 @@ __device__ __forceinline__ void k_cell_lag_body(const MhDev& S, int lb) {
       for (int v = 0; v < NV; ++v) U[v * nc + c] = u_out[v];
     S.Vl[c] = V;
     S.L[c] = L;
+  }
+  return dxs;
+}
+
+template <int DIM, int M>
+__device__ __forceinline__ void k_cell_lag_body(const MhDev& S, int lb) {
+  const long c = lb * (long)blockDim.x + threadIdx.x;
+  const bool active = c < S.g.nc && !mh_halted(S.sc);
+  double dxs = 1.79769313486231570815e+308;
+  if (active) {
+    const int nx = S.g.nx, ny = S.g.ny;
+    const int i = (int)(c % nx);
+    const long r = c / nx;
+    const int j = (int)(r % ny);
+    const int k = (int)(r / ny);
+    const long b = i + (long)S.g.nnx * (j + (long)S.g.nny * k);
+    D3 x[8];
+    cell_nodes<DIM>(S, b, S.xlc, x);
+    dxs = cell_lag_core<DIM, M>(S, c, b, x);
   }
   if (!S.with_proj) block_min_store(dxs, active, S.partial, lb);
 }
@@ -1501,26 +1557,32 @@ __device__ __forceinline__ void k_dt_body(const MhDev& S) {
   __shared__ int rn[MH_DT_BLOCK / 64];
   MhScalars* sc = S.sc;
   if (mh_halted(sc)) return;
-  // block partials in block (= cell) order: thread t folds a contiguous run of them
-  const int per = (nblocks + (int)blockDim.x - 1) / (int)blockDim.x;
-  MhCflMin v = cfl_leaf(0., false);
-  for (int b = threadIdx.x * per; b < min(nblocks, ((int)threadIdx.x + 1) * per); ++b) {
-    MhCflMin o;
-    o.m = S.partial[2 * b];
-    o.nan = S.partial[2 * b + 1] != 0.;
-    v = cfl_combine(v, o);
+  // block partials in block (= cell) order: wavefront w folds a contiguous run of them,
+  // 64 at a time (coalesced), then the wavefronts are combined in order
+  const int nw = (int)(blockDim.x >> 6), lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int chunk = ((nblocks + nw - 1) / nw + 63) / 64 * 64;
+  const int bbeg = wid * chunk, bend = min(nblocks, bbeg + chunk);
+  MhCflMin acc = cfl_leaf(0., false);
+  for (int b0 = bbeg; b0 < bend; b0 += 64) {
+    const int b = b0 + lane;
+    MhCflMin o = cfl_leaf(0., false);
+    if (b < bend) {
+      o.m = S.partial[2 * b];
+      o.nan = S.partial[2 * b + 1] != 0.;
+    }
+    o = cfl_wave(o);
+    acc = cfl_combine(acc, o);  // lane 0 holds the run's value; other lanes are ignored
   }
-  v = cfl_wave(v);
-  if ((threadIdx.x & 63) == 0) {
-    rm[threadIdx.x >> 6] = v.m;
-    rn[threadIdx.x >> 6] = v.nan;
+  if (lane == 0) {
+    rm[wid] = acc.m;
+    rn[wid] = acc.nan;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
   MhCflMin tot;
   tot.m = rm[0];
   tot.nan = rn[0];
-  for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+  for (int w = 1; w < nw; ++w) {
     MhCflMin o;
     o.m = rm[w];
     o.nan = rn[w];

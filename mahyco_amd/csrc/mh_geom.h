@@ -60,6 +60,41 @@ MH_HD void cqs_hex(const D3* x, D3* q) {
   q[7] = t * (5. * (n2a08 + n2a12 + n4a12 + n4a11 + n6a11 + n6a08) + (n2a04 + n2a05 + n4a09 + n4a10 + n6a07 + n6a03));
 }
 
+// All eight corner vectors of a hexahedron, handed to f(l, q_l) as soon as each is
+// complete.  Faces are formed in the order 1, 2, 3, 5, 6, 4 so that face 1 (corners 0..3)
+// retires before face 4 (corners 4..7) is formed: five faces are live at most instead of
+// six.  Every term and sum is the one of cqs_hex, so each q_l has the same bits.
+template <class F>
+MH_HD void cqs_hex_each(const D3* x, F&& f) {
+  const double t = 1. / 12.;
+  const D3 c0 = hex_fc0(x);
+  const D3 n1a04 = hcross(x[0] - c0, x[3] - c0), n1a03 = hcross(x[3] - c0, x[2] - c0);
+  const D3 n1a02 = hcross(x[2] - c0, x[1] - c0), n1a01 = hcross(x[1] - c0, x[0] - c0);
+  const D3 c1 = hex_fc1(x);
+  const D3 n2a05 = hcross(x[0] - c1, x[4] - c1), n2a12 = hcross(x[4] - c1, x[7] - c1);
+  const D3 n2a08 = hcross(x[7] - c1, x[3] - c1), n2a04 = hcross(x[3] - c1, x[0] - c1);
+  const D3 c2 = hex_fc2(x);
+  const D3 n3a01 = hcross(x[0] - c2, x[1] - c2), n3a06 = hcross(x[1] - c2, x[5] - c2);
+  const D3 n3a09 = hcross(x[5] - c2, x[4] - c2), n3a05 = hcross(x[4] - c2, x[0] - c2);
+  f(0, t * (5. * (n1a01 + n1a04 + n2a04 + n2a05 + n3a05 + n3a01) + (n1a02 + n1a03 + n2a08 + n2a12 + n3a06 + n3a09)));
+  const D3 c4 = hex_fc4(x);
+  const D3 n5a02 = hcross(x[1] - c4, x[2] - c4), n5a07 = hcross(x[2] - c4, x[6] - c4);
+  const D3 n5a10 = hcross(x[6] - c4, x[5] - c4), n5a06 = hcross(x[5] - c4, x[1] - c4);
+  f(1, t * (5. * (n1a01 + n1a02 + n3a01 + n3a06 + n5a06 + n5a02) + (n1a04 + n1a03 + n3a09 + n3a05 + n5a10 + n5a07)));
+  const D3 c5 = hex_fc5(x);
+  const D3 n6a03 = hcross(x[2] - c5, x[3] - c5), n6a08 = hcross(x[3] - c5, x[7] - c5);
+  const D3 n6a11 = hcross(x[7] - c5, x[6] - c5), n6a07 = hcross(x[6] - c5, x[2] - c5);
+  f(2, t * (5. * (n1a02 + n1a03 + n5a07 + n5a02 + n6a07 + n6a03) + (n1a01 + n1a04 + n5a06 + n5a10 + n6a11 + n6a08)));
+  f(3, t * (5. * (n1a03 + n1a04 + n2a08 + n2a04 + n6a08 + n6a03) + (n1a01 + n1a02 + n2a05 + n2a12 + n6a07 + n6a11)));
+  const D3 c3 = hex_fc3(x);
+  const D3 n4a09 = hcross(x[4] - c3, x[5] - c3), n4a10 = hcross(x[5] - c3, x[6] - c3);
+  const D3 n4a11 = hcross(x[6] - c3, x[7] - c3), n4a12 = hcross(x[7] - c3, x[4] - c3);
+  f(4, t * (5. * (n2a05 + n2a12 + n3a05 + n3a09 + n4a09 + n4a12) + (n2a08 + n2a04 + n3a01 + n3a06 + n4a10 + n4a11)));
+  f(5, t * (5. * (n3a06 + n3a09 + n4a09 + n4a10 + n5a10 + n5a06) + (n3a01 + n3a05 + n4a12 + n4a11 + n5a07 + n5a02)));
+  f(6, t * (5. * (n4a11 + n4a10 + n5a10 + n5a07 + n6a07 + n6a11) + (n4a12 + n4a09 + n5a06 + n5a02 + n6a03 + n6a08)));
+  f(7, t * (5. * (n2a08 + n2a12 + n4a12 + n4a11 + n6a11 + n6a08) + (n2a04 + n2a05 + n4a09 + n4a10 + n6a07 + n6a03)));
+}
+
 // One corner vector of a hexahedron: only the three faces around corner L are formed;
 // each term is computed exactly as in cqs_hex, so the result has the same bits.
 template <int L>

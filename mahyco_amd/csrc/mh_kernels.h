@@ -270,7 +270,7 @@ __device__ __forceinline__ void nt_load_cell(const MhDev& S, int ii, int jj, int
 // corner force contributions (p + q) CQS_l and the node-mass share M/8 of one cell: lower
 // corners to LDS slot s, upper corners to keep
 __device__ __forceinline__ void nt_cell(const D3* bot, const NtCellIn& in, double* lds, int s, double* keep) {
-  D3 x[8], q[8];
+  D3 x[8];
   x[0] = bot[0];
   x[1] = bot[1];
   x[2] = bot[2];
@@ -279,20 +279,20 @@ __device__ __forceinline__ void nt_cell(const D3* bot, const NtCellIn& in, doubl
   x[5] = in.top[1];
   x[6] = in.top[2];
   x[7] = in.top[3];
-  cqs_hex(x, q);
   const double pr = in.p + in.q;
   const double wm = 0.125 * in.M;
-  for (int l = 0; l < 4; ++l) {
-    lds[(3 * l + 0) * MH_NT_CELLS + s] = pr * q[l].x;
-    lds[(3 * l + 1) * MH_NT_CELLS + s] = pr * q[l].y;
-    lds[(3 * l + 2) * MH_NT_CELLS + s] = pr * q[l].z;
-  }
+  cqs_hex_each(x, [&](int l, const D3& q) {
+    if (l < 4) {
+      lds[(3 * l + 0) * MH_NT_CELLS + s] = pr * q.x;
+      lds[(3 * l + 1) * MH_NT_CELLS + s] = pr * q.y;
+      lds[(3 * l + 2) * MH_NT_CELLS + s] = pr * q.z;
+    } else {
+      keep[3 * (l - 4) + 0] = pr * q.x;
+      keep[3 * (l - 4) + 1] = pr * q.y;
+      keep[3 * (l - 4) + 2] = pr * q.z;
+    }
+  });
   lds[12 * MH_NT_CELLS + s] = wm;
-  for (int l = 0; l < 4; ++l) {
-    keep[3 * l + 0] = pr * q[4 + l].x;
-    keep[3 * l + 1] = pr * q[4 + l].y;
-    keep[3 * l + 2] = pr * q[4 + l].z;
-  }
   keep[12] = wm;
 }
 
@@ -1041,10 +1041,12 @@ __global__ __launch_bounds__(MH_BLOCK, MH_TILE_MIN_WAVES) void k_sweep_cell_tile
   double* __restrict__ Uout = usel ? S.Ua : S.Ub;
   double* __restrict__ dflux = S.dflux;
   const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
-  const int i = (int)(lb % tilesx) * 64 + lx;
-  const long rest = lb / tilesx;
-  const int t = (int)(rest % nt);
-  const int abase = (int)(rest / nt) * MH_TILE_L - 2;  // cell along D of tile row 0
+  // tiles along D vary fastest: the tiles sharing a halo run side by side on one XCD
+  const int ntd = (na + MH_TILE_L - 1) / MH_TILE_L;
+  const int abase = (int)(lb % ntd) * MH_TILE_L - 2;  // cell along D of tile row 0
+  const long rest = lb / ntd;
+  const int i = (int)(rest % tilesx) * 64 + lx;
+  const int t = (int)(rest / tilesx);
   const long nc = S.g.nc;
   const long nnx = S.g.nnx, nny = S.g.nny;
   const long sD = D == 1 ? (long)nx : (long)nx * ny;
@@ -1557,36 +1559,58 @@ __device__ __forceinline__ void k_dt_body(const MhDev& S) {
   __shared__ int rn[MH_DT_BLOCK / 64];
   MhScalars* sc = S.sc;
   if (mh_halted(sc)) return;
-  // block partials in block (= cell) order: wavefront w folds a contiguous run of them,
-  // 64 at a time (coalesced), then the wavefronts are combined in order
   const int nw = (int)(blockDim.x >> 6), lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int chunk = ((nblocks + nw - 1) / nw + 63) / 64 * 64;
-  const int bbeg = wid * chunk, bend = min(nblocks, bbeg + chunk);
-  MhCflMin acc = cfl_leaf(0., false);
-  for (int b0 = bbeg; b0 < bend; b0 += 64) {
-    const int b = b0 + lane;
-    MhCflMin o = cfl_leaf(0., false);
-    if (b < bend) {
-      o.m = S.partial[2 * b];
-      o.nan = S.partial[2 * b + 1] != 0.;
-    }
-    o = cfl_wave(o);
-    acc = cfl_combine(acc, o);  // lane 0 holds the run's value; other lanes are ignored
-  }
-  if (lane == 0) {
-    rm[wid] = acc.m;
-    rn[wid] = acc.nan;
-  }
+  __shared__ int any_nan;
+  // without a NaN candidate anywhere the fold is a plain minimum, in any order
+  if (threadIdx.x == 0) any_nan = 0;
   __syncthreads();
-  if (threadIdx.x != 0) return;
+  double pm = __longlong_as_double(0x7ff8000000000000LL);
+  int pn = 0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+    pm = min_skip_nan(pm, S.partial[2 * b]);
+    pn |= S.partial[2 * b + 1] != 0.;
+  }
+  for (int off = 32; off > 0; off >>= 1) pm = min_skip_nan(pm, __shfl_xor(pm, off, 64));
+  if (__ballot(pn) != 0 && lane == 0) any_nan = 1;
+  if (lane == 0) rm[wid] = pm;
+  __syncthreads();
   MhCflMin tot;
-  tot.m = rm[0];
-  tot.nan = rn[0];
-  for (int w = 1; w < nw; ++w) {
-    MhCflMin o;
-    o.m = rm[w];
-    o.nan = rn[w];
-    tot = cfl_combine(tot, o);
+  if (!any_nan) {
+    if (threadIdx.x != 0) return;
+    tot.nan = 0;
+    tot.m = rm[0];
+    for (int w = 1; w < nw; ++w) tot.m = min_skip_nan(tot.m, rm[w]);
+  } else {
+    // block partials in block (= cell) order: wavefront w folds a contiguous run of them,
+    // 64 at a time, then the wavefronts are combined in order
+    __syncthreads();
+    const int chunk = ((nblocks + nw - 1) / nw + 63) / 64 * 64;
+    const int bbeg = wid * chunk, bend = min(nblocks, bbeg + chunk);
+    MhCflMin acc = cfl_leaf(0., false);
+    for (int b0 = bbeg; b0 < bend; b0 += 64) {
+      const int b = b0 + lane;
+      MhCflMin o = cfl_leaf(0., false);
+      if (b < bend) {
+        o.m = S.partial[2 * b];
+        o.nan = S.partial[2 * b + 1] != 0.;
+      }
+      o = cfl_wave(o);
+      acc = cfl_combine(acc, o);  // lane 0 holds the run's value; other lanes are ignored
+    }
+    if (lane == 0) {
+      rm[wid] = acc.m;
+      rn[wid] = acc.nan;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    tot.m = rm[0];
+    tot.nan = rn[0];
+    for (int w = 1; w < nw; ++w) {
+      MhCflMin o;
+      o.m = rm[w];
+      o.nan = rn[w];
+      tot = cfl_combine(tot, o);
+    }
   }
   const double mn = tot.nan ? tot.m : amin(1.79769313486231570815e+308, tot.m);
   const double dt = sc->dt;

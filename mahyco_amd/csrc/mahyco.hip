@@ -357,6 +357,17 @@ void prof_end(MhHandle* h, ProfRec& r) {
   h->prof_pending.push_back(r);
 }
 
+// Runs the statement with LIM, a compile-time limiter id: 0 minmod, 1 superbee, 2 vanLeer,
+// 3 none (first order, or an id without a classical limiter: the slopes are 0).
+#define MH_LIMSW(h, ...)                                                  \
+  do {                                                                    \
+    const int _lim = (h)->dev.order2 ? (h)->dev.limiter : 3;              \
+    if (_lim == 0) { constexpr int LIM = 0; __VA_ARGS__; }                \
+    else if (_lim == 1) { constexpr int LIM = 1; __VA_ARGS__; }           \
+    else if (_lim == 2) { constexpr int LIM = 2; __VA_ARGS__; }           \
+    else { constexpr int LIM = 3; __VA_ARGS__; }                          \
+  } while (0)
+
 #define LAUNCH(h, kind, kern, grid, block, ...)                          \
   do {                                                                   \
     ProfRec _r;                                                          \
@@ -393,10 +404,10 @@ template <int DIM, int D, int M>
 void launch_sweep_yz(MhHandle* h, int sel, int first, int last) {
   const MhDev& S = h->dev;
   if (h->sweep_mode == MH_SWEEP_FLAT)
-    LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell<DIM, D, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+    MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell<DIM, D, M, LIM>), h->nblk_c, MH_BLOCK, S, sel, first));
   else
-    LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell_tile<DIM, D, M>), tile_blocks(h, D), MH_BLOCK, S, sel, first);
-  LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node<DIM, D>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+    MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell_tile<DIM, D, M, LIM>), tile_blocks(h, D), MH_BLOCK, S, sel, first));
+  MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node<DIM, D, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
 }
 
 template <int DIM, int M>
@@ -415,10 +426,13 @@ void launch_sweeps_t(MhHandle* h) {
     const int D = dirs[s];
     const int mode = h->sweep_mode;
     if (D == 0) {
-      if (mode == MH_SWEEP_FLAT) LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-      else LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell_row<DIM, M>), row_blocks(h), MH_BLOCK, S, sel, first);
-      if (mode == MH_SWEEP_FLAT) LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0>), h->nblk_n, MH_BLOCK, S, sel, first, last);
-      else LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node_row<DIM>), node_row_blocks(h), MH_BLOCK, S, sel, first, last);
+      if (mode == MH_SWEEP_FLAT) {
+        MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, 0, M, LIM>), h->nblk_c, MH_BLOCK, S, sel, first));
+        MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
+      } else {
+        MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell_row<DIM, M, LIM>), row_blocks(h), MH_BLOCK, S, sel, first));
+        MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node_row<DIM, LIM>), node_row_blocks(h), MH_BLOCK, S, sel, first, last));
+      }
     } else if (D == 1) {
       launch_sweep_yz<DIM, 1, M>(h, sel, first, last);
     } else {
@@ -661,14 +675,14 @@ void ens_group_cycle(MhEnsembleImpl* E, const EnsGroup& g) {
   for (int s = 0; s < DIM; ++s) {
     const int D = (s + sens) % DIM, first = s == 0, last = s == DIM - 1, sel = s & 1;
     if (D == 0) {
-      hipLaunchKernelGGL((k_sweep_cell_batch<DIM, 0, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
-      hipLaunchKernelGGL((k_sweep_node_batch<DIM, 0>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
+      hipLaunchKernelGGL((k_sweep_cell_batch<DIM, 0, M, -1>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
+      hipLaunchKernelGGL((k_sweep_node_batch<DIM, 0, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
     } else if (D == 1) {
-      hipLaunchKernelGGL((k_sweep_cell_batch<DIM, 1, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
-      hipLaunchKernelGGL((k_sweep_node_batch<DIM, 1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
+      hipLaunchKernelGGL((k_sweep_cell_batch<DIM, 1, M, -1>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
+      hipLaunchKernelGGL((k_sweep_node_batch<DIM, 1, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
     } else {
-      hipLaunchKernelGGL((k_sweep_cell_batch<DIM, (DIM == 3 ? 2 : 0), M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
-      hipLaunchKernelGGL((k_sweep_node_batch<DIM, (DIM == 3 ? 2 : 0)>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
+      hipLaunchKernelGGL((k_sweep_cell_batch<DIM, (DIM == 3 ? 2 : 0), M, -1>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
+      hipLaunchKernelGGL((k_sweep_node_batch<DIM, (DIM == 3 ? 2 : 0), -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
     }
   }
   hipLaunchKernelGGL((k_finalize_batch<DIM, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, DIM & 1);

@@ -195,4 +195,11 @@ MH_HD double mh_flux_limiter(int id, double r) {
   return 0.0;
 }
 
+// The limiter with its id known at compile time (LIM >= 0: the branches on the id fold
+// away) or read at run time (LIM < 0, batched launches over cases with different limiters).
+template <int LIM>
+MH_HD double mh_limiter(int id, double r) {
+  return mh_flux_limiter(LIM < 0 ? id : LIM, r);
+}
+
 #endif

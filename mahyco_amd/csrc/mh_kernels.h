@@ -656,7 +656,7 @@ __device__ __forceinline__ void face_quant(const MhDev& S, long f0, double& area
 // computeGradPhiCell + computeAndLimitGradPhi (:161-267, UtilesRemap.cc:325-357),
 // computeUpwindFaceQuantitiesForProjection (:279-383) and computeUremap (:395-575).
 // Phi of the stencil is rebuilt from U (U / V before the first sweep, U / sum V_k after).
-template <int DIM, int D, int M>
+template <int DIM, int D, int M, int LIM>
 __device__ __forceinline__ void k_sweep_cell_body(const MhDev& S, int lb, int usel, int first) {
   const double* __restrict__ Uin = usel ? S.Ub : S.Ua;
   double* __restrict__ Uout = usel ? S.Ua : S.Ub;
@@ -734,8 +734,8 @@ __device__ __forceinline__ void k_sweep_cell_body(const MhDev& S, int lb, int us
       double g = 0.;
       if (S.order2 && valid[o + 1]) {
         const double gb = gface[o], gf = gface[o + 1];
-        if (gb != 0.) g += 0.5 * (mh_flux_limiter(S.limiter, gf / gb) * gb);
-        if (gf != 0.) g += 0.5 * (mh_flux_limiter(S.limiter, gb / gf) * gf);
+        if (gb != 0.) g += 0.5 * (mh_limiter<LIM>(S.limiter, gf / gb) * gb);
+        if (gf != 0.) g += 0.5 * (mh_limiter<LIM>(S.limiter, gb / gf) * gf);
       }
       grad[o] = g;
     }
@@ -772,7 +772,7 @@ __device__ __forceinline__ void k_sweep_cell_body(const MhDev& S, int lb, int us
 // back-face value of cell i+1).  Lanes 2..61 write their cell; the two lanes at each end
 // only feed the stencil.  Same formulas in the same order as k_sweep_cell_body.
 #define MH_ROW_OUT 60
-template <int DIM, int M>
+template <int DIM, int M, int LIM>
 __global__ __launch_bounds__(MH_BLOCK) void k_sweep_cell_row(MhDev S, int usel, int first) {
   constexpr int NV = 3 * M + 1;
   const int nx = S.g.nx;
@@ -826,8 +826,8 @@ __global__ __launch_bounds__(MH_BLOCK) void k_sweep_cell_row(MhDev S, int usel, 
     const double gf = __shfl_down(gb, 1);
     double g = 0.;
     if (S.order2 && valid) {
-      if (gb != 0.) g += 0.5 * (mh_flux_limiter(S.limiter, gf / gb) * gb);
-      if (gf != 0.) g += 0.5 * (mh_flux_limiter(S.limiter, gb / gf) * gf);
+      if (gb != 0.) g += 0.5 * (mh_limiter<LIM>(S.limiter, gf / gb) * gb);
+      if (gf != 0.) g += 0.5 * (mh_limiter<LIM>(S.limiter, gb / gf) * gf);
     }
     const double gp = __shfl_up(g, 1);
     double F = 0.;  // flux of the back face with outer normal +e_x (for cell i-1)
@@ -889,10 +889,11 @@ __device__ __forceinline__ double dual_edge_flux(const MhDev& S, int a, int it1,
 
 // Limited nodal gradient from the edge gradients behind (gb) and ahead (gf)
 // (computeAndLimitGradPhiDual, UtilesRemap.cc:369-400), in the order of k_sweep_node_body.
+template <int LIM>
 __device__ __forceinline__ double dual_limited(const MhDev& S, double gb, double gf) {
   double g = 0.;
-  if (gb != 0.) g += 0.5 * mh_flux_limiter(S.limiter, gf / gb) * gb;
-  if (gf != 0.) g += 0.5 * mh_flux_limiter(S.limiter, gb / gf) * gf;
+  if (gb != 0.) g += 0.5 * mh_limiter<LIM>(S.limiter, gf / gb) * gb;
+  if (gf != 0.) g += 0.5 * mh_limiter<LIM>(S.limiter, gb / gf) * gf;
   return g;
 }
 
@@ -953,7 +954,7 @@ __device__ __forceinline__ void dual_node_keep(const MhDev& S, long n, const dou
 // Dual x sweep with one wavefront per 60 nodes of a node row (lane shuffles carry the
 // neighbours' phi, coordinates, edge gradients and limited gradients; the front flux of
 // node i is the back flux of node i+1).  Lanes 2..61 write their node.
-template <int DIM>
+template <int DIM, int LIM>
 __global__ __launch_bounds__(MH_BLOCK) void k_sweep_node_row(MhDev S, int psel, int first, int last) {
   constexpr int NC = DIM;
   const int nnx = S.g.nnx, na = S.g.nx;
@@ -987,7 +988,7 @@ __global__ __launch_bounds__(MH_BLOCK) void k_sweep_node_row(MhDev S, int psel, 
     Pp[cc] = __shfl_down(P[cc], 1);
     const double eb = (valid && validp) ? (P[cc] - Pm[cc]) / (X - Xm) : 0.;  // edge i-1/2
     const double ef = __shfl_down(eb, 1);                                    // edge i+1/2
-    G0[cc] = (S.order2 && inner) ? dual_limited(S, eb, ef) : 0.;
+    G0[cc] = (S.order2 && inner) ? dual_limited<LIM>(S, eb, ef) : 0.;
     Gm[cc] = __shfl_up(G0[cc], 1);
     Gp[cc] = __shfl_down(G0[cc], 1);
   }
@@ -1020,7 +1021,7 @@ __global__ __launch_bounds__(MH_BLOCK) void k_sweep_node_row(MhDev S, int psel, 
 #define MH_TILE_L 16
 #define MH_TILE_R (MH_TILE_L + 4)  // rows held, halos included
 #define MH_TILE_Q (MH_TILE_R / 4)  // rows per thread
-template <int DIM, int D, int M>
+template <int DIM, int D, int M, int LIM>
 #ifndef MH_TILE_MIN_WAVES
 #define MH_TILE_MIN_WAVES 4  // waves per SIMD the register allocation must allow
 #endif
@@ -1123,8 +1124,8 @@ __global__ __launch_bounds__(MH_BLOCK, MH_TILE_MIN_WAVES) void k_sweep_cell_tile
       double g = 0.;
       if (S.order2 && val[q] && r >= 1 && r <= MH_TILE_R - 2) {
         const double gb = sgf[r][lx], gf = sgf[r + 1][lx];
-        if (gb != 0.) g += 0.5 * (mh_flux_limiter(S.limiter, gf / gb) * gb);
-        if (gf != 0.) g += 0.5 * (mh_flux_limiter(S.limiter, gb / gf) * gf);
+        if (gb != 0.) g += 0.5 * (mh_limiter<LIM>(S.limiter, gf / gb) * gb);
+        if (gf != 0.) g += 0.5 * (mh_limiter<LIM>(S.limiter, gb / gf) * gf);
       }
       ssl[r][lx] = g;
     }
@@ -1170,7 +1171,7 @@ __global__ __launch_bounds__(MH_BLOCK, MH_TILE_MIN_WAVES) void k_sweep_cell_tile
 // (DualPhaseRemap.cc:75-108), upwind momentum update (:110-239), filter and phi (:241-257).
 // Nodes with no neighbour along D keep their values.  On the last sweep the node velocity
 // u = mu / m is formed for every node (RemapADIFinal.cc:286-297).
-template <int DIM, int D>
+template <int DIM, int D, int LIM>
 __device__ __forceinline__ void k_sweep_node_body(const MhDev& S, int lb, int psel, int first, int last) {
   double* __restrict__ Ud = S.Ud;
   const double* __restrict__ Pin = psel ? S.Pa : S.Pb;
@@ -1229,8 +1230,8 @@ __device__ __forceinline__ void k_sweep_node_body(const MhDev& S, int lb, int ps
       if (in) {
         const double gf = (P[o + 3][cc] - P[o + 2][cc]) / (X[o + 3] - X[o + 2]);
         const double gb = (P[o + 2][cc] - P[o + 1][cc]) / (X[o + 2] - X[o + 1]);
-        if (gb != 0.) g += 0.5 * mh_flux_limiter(S.limiter, gf / gb) * gb;
-        if (gf != 0.) g += 0.5 * mh_flux_limiter(S.limiter, gb / gf) * gf;
+        if (gb != 0.) g += 0.5 * mh_limiter<LIM>(S.limiter, gf / gb) * gb;
+        if (gf != 0.) g += 0.5 * mh_limiter<LIM>(S.limiter, gb / gf) * gf;
       }
       G[o + 1][cc] = g;
     }
@@ -1744,9 +1745,9 @@ __global__ __launch_bounds__(MH_BLOCK) void k_hist_final(MhDev S) { k_hist_final
 #define MH_COMMA ,
 MH_WRAP(k_node_lag, 1, template <int DIM>, <DIM>, , )
 MH_WRAP(k_cell_lag, MH_CELL_MIN_WAVES, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, , )
-MH_WRAP(k_sweep_cell, 1, template <int DIM MH_COMMA int D MH_COMMA int M>, <DIM MH_COMMA D MH_COMMA M>,
+MH_WRAP(k_sweep_cell, 1, template <int DIM MH_COMMA int D MH_COMMA int M MH_COMMA int LIM>, <DIM MH_COMMA D MH_COMMA M MH_COMMA LIM>,
         MH_COMMA int usel MH_COMMA int first, MH_COMMA usel MH_COMMA first)
-MH_WRAP(k_sweep_node, 1, template <int DIM MH_COMMA int D>, <DIM MH_COMMA D>,
+MH_WRAP(k_sweep_node, 1, template <int DIM MH_COMMA int D MH_COMMA int LIM>, <DIM MH_COMMA D MH_COMMA LIM>,
         MH_COMMA int psel MH_COMMA int first MH_COMMA int last, MH_COMMA psel MH_COMMA first MH_COMMA last)
 MH_WRAP(k_node_velocity, 1, template <int DIM>, <DIM>, MH_COMMA int materialize, MH_COMMA materialize)
 MH_WRAP(k_finalize, 1, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, MH_COMMA int usel, MH_COMMA usel)

@@ -571,6 +571,12 @@ struct EnsGroup {
   int2* d_nmap = nullptr;
   int* d_list = nullptr;
   int ncb = 0, nnb = 0;
+  // block maps of the row-segment x sweeps and the tiled y / z cell sweeps
+  bool rows = false;
+  int2* d_rcmap = nullptr;
+  int2* d_rnmap = nullptr;
+  int2* d_tmap[2] = {nullptr, nullptr};
+  int nrcb = 0, nrnb = 0, ntb[2] = {0, 0};
 };
 
 struct MhEnsembleImpl {
@@ -598,6 +604,10 @@ void ens_free_groups(MhEnsembleImpl* E) {
     (void)hipFree(g.d_cmap);
     (void)hipFree(g.d_nmap);
     (void)hipFree(g.d_list);
+    (void)hipFree(g.d_rcmap);
+    (void)hipFree(g.d_rnmap);
+    (void)hipFree(g.d_tmap[0]);
+    (void)hipFree(g.d_tmap[1]);
   }
   E->groups.clear();
 }
@@ -658,6 +668,31 @@ int ens_rebuild(MhEnsembleImpl* E) {
     HIPCHK(hipMemcpy(g.d_cmap, cm.data(), sizeof(int2) * cm.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(g.d_nmap, nm.data(), sizeof(int2) * nm.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(g.d_list, g.slots.data(), sizeof(int) * g.slots.size(), hipMemcpyHostToDevice));
+    // rows and tiles when every case of the group runs them
+    g.rows = true;
+    for (int sl : g.slots) g.rows = g.rows && E->slot[sl]->sweep_mode == MH_SWEEP_TILE;
+    if (g.rows) {
+      std::vector<int2> rc, rn, t[2];
+      for (int sl : g.slots) {
+        MhHandle* h = E->slot[sl];
+        for (int b = 0, n = row_blocks(h); b < n; ++b) rc.push_back(make_int2(sl, b));
+        for (int b = 0, n = node_row_blocks(h); b < n; ++b) rn.push_back(make_int2(sl, b));
+        for (int D = 1; D < g.dim; ++D)
+          for (int b = 0, n = tile_blocks(h, D); b < n; ++b) t[D - 1].push_back(make_int2(sl, b));
+      }
+      auto put = [](const std::vector<int2>& v, int2** d, int* n) -> int {
+        *n = (int)v.size();
+        if (v.empty()) return MH_OK;
+        HIPCHK(hipMalloc(d, sizeof(int2) * v.size()));
+        HIPCHK(hipMemcpy(*d, v.data(), sizeof(int2) * v.size(), hipMemcpyHostToDevice));
+        return MH_OK;
+      };
+      int st;
+      if ((st = put(rc, &g.d_rcmap, &g.nrcb)) != MH_OK) return st;
+      if ((st = put(rn, &g.d_rnmap, &g.nrnb)) != MH_OK) return st;
+      for (int q = 0; q < 2; ++q)
+        if ((st = put(t[q], &g.d_tmap[q], &g.ntb[q])) != MH_OK) return st;
+    }
   }
   E->dirty = false;
   E->version++;
@@ -674,7 +709,17 @@ void ens_group_cycle(MhEnsembleImpl* E, const EnsGroup& g) {
   const int sens = (int)(E->launched % DIM);
   for (int s = 0; s < DIM; ++s) {
     const int D = (s + sens) % DIM, first = s == 0, last = s == DIM - 1, sel = s & 1;
-    if (D == 0) {
+    if (g.rows && D == 0) {
+      hipLaunchKernelGGL((k_sweep_cell_row_batch<DIM, M, -1>), dim3(g.nrcb), dim3(MH_BLOCK), 0, st, tab, g.d_rcmap, sel, first);
+      hipLaunchKernelGGL((k_sweep_node_row_batch<DIM, -1>), dim3(g.nrnb), dim3(MH_BLOCK), 0, st, tab, g.d_rnmap, sel, first, last);
+    } else if (g.rows && D == 1) {
+      hipLaunchKernelGGL((k_sweep_cell_tile_batch<DIM, 1, M, -1>), dim3(g.ntb[0]), dim3(MH_BLOCK), 0, st, tab, g.d_tmap[0], sel, first);
+      hipLaunchKernelGGL((k_sweep_node_batch<DIM, 1, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
+    } else if (g.rows) {
+      if constexpr (DIM == 3)
+        hipLaunchKernelGGL((k_sweep_cell_tile_batch<3, 2, M, -1>), dim3(g.ntb[1]), dim3(MH_BLOCK), 0, st, tab, g.d_tmap[1], sel, first);
+      hipLaunchKernelGGL((k_sweep_node_batch<DIM, (DIM == 3 ? 2 : 0), -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
+    } else if (D == 0) {
       hipLaunchKernelGGL((k_sweep_cell_batch<DIM, 0, M, -1>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
       hipLaunchKernelGGL((k_sweep_node_batch<DIM, 0, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
     } else if (D == 1) {

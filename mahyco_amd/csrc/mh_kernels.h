@@ -773,13 +773,13 @@ __device__ __forceinline__ void k_sweep_cell_body(const MhDev& S, int lb, int us
 // only feed the stencil.  Same formulas in the same order as k_sweep_cell_body.
 #define MH_ROW_OUT 60
 template <int DIM, int M, int LIM>
-__global__ __launch_bounds__(MH_BLOCK) void k_sweep_cell_row(MhDev S, int usel, int first) {
+__device__ __forceinline__ void k_sweep_cell_row_body(const MhDev& S, int lb, int usel, int first) {
   constexpr int NV = 3 * M + 1;
   const int nx = S.g.nx;
   const long rows = (long)S.g.ny * S.g.nz;
   const int wpr = (nx + MH_ROW_OUT - 1) / MH_ROW_OUT;
   const int lane = threadIdx.x & 63;
-  const long gw = (long)xcd_block((int)blockIdx.x, (int)gridDim.x) * (MH_BLOCK / 64) + (threadIdx.x >> 6);
+  const long gw = (long)lb * (MH_BLOCK / 64) + (threadIdx.x >> 6);
   if (gw >= rows * wpr) return;  // whole wavefront
   if (mh_halted(S.sc)) return;
   const double* __restrict__ Uin = usel ? S.Ub : S.Ua;
@@ -955,13 +955,13 @@ __device__ __forceinline__ void dual_node_keep(const MhDev& S, long n, const dou
 // neighbours' phi, coordinates, edge gradients and limited gradients; the front flux of
 // node i is the back flux of node i+1).  Lanes 2..61 write their node.
 template <int DIM, int LIM>
-__global__ __launch_bounds__(MH_BLOCK) void k_sweep_node_row(MhDev S, int psel, int first, int last) {
+__device__ __forceinline__ void k_sweep_node_row_body(const MhDev& S, int lb, int psel, int first, int last) {
   constexpr int NC = DIM;
   const int nnx = S.g.nnx, na = S.g.nx;
   const long rows = (long)S.g.nny * S.g.nnz;
   const int wpr = (nnx + MH_ROW_OUT - 1) / MH_ROW_OUT;
   const int lane = threadIdx.x & 63;
-  const long gw = (long)xcd_block((int)blockIdx.x, (int)gridDim.x) * (MH_BLOCK / 64) + (threadIdx.x >> 6);
+  const long gw = (long)lb * (MH_BLOCK / 64) + (threadIdx.x >> 6);
   if (gw >= rows * wpr) return;  // whole wavefront
   if (mh_halted(S.sc)) return;
   const double* __restrict__ Pin = psel ? S.Pa : S.Pb;
@@ -1021,11 +1021,11 @@ __global__ __launch_bounds__(MH_BLOCK) void k_sweep_node_row(MhDev S, int psel, 
 #define MH_TILE_L 16
 #define MH_TILE_R (MH_TILE_L + 4)  // rows held, halos included
 #define MH_TILE_Q (MH_TILE_R / 4)  // rows per thread
-template <int DIM, int D, int M, int LIM>
 #ifndef MH_TILE_MIN_WAVES
 #define MH_TILE_MIN_WAVES 4  // waves per SIMD the register allocation must allow
 #endif
-__global__ __launch_bounds__(MH_BLOCK, MH_TILE_MIN_WAVES) void k_sweep_cell_tile(MhDev S, int usel, int first) {
+template <int DIM, int D, int M, int LIM>
+__device__ __forceinline__ void k_sweep_cell_tile_body(const MhDev& S, int lb_, int usel, int first) {
   constexpr int NV = 3 * M + 1;
   __shared__ double sphi[MH_TILE_R][64];
   __shared__ double sgf[MH_TILE_R][64];  // face gradients, then face fluxes
@@ -1035,7 +1035,7 @@ __global__ __launch_bounds__(MH_BLOCK, MH_TILE_MIN_WAVES) void k_sweep_cell_tile
   const int nt = (D == 1) ? (DIM == 3 ? nz : 1) : ny;
   const int tilesx = (nx + 63) / 64;
   const long ntiles = (long)tilesx * nt * ((na + MH_TILE_L - 1) / MH_TILE_L);
-  const long lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const long lb = lb_;
   if (lb >= ntiles) return;  // whole block
   if (mh_halted(S.sc)) return;
   const double* __restrict__ Uin = usel ? S.Ub : S.Ua;
@@ -1751,6 +1751,13 @@ MH_WRAP(k_sweep_node, 1, template <int DIM MH_COMMA int D MH_COMMA int LIM>, <DI
         MH_COMMA int psel MH_COMMA int first MH_COMMA int last, MH_COMMA psel MH_COMMA first MH_COMMA last)
 MH_WRAP(k_node_velocity, 1, template <int DIM>, <DIM>, MH_COMMA int materialize, MH_COMMA materialize)
 MH_WRAP(k_finalize, 1, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, MH_COMMA int usel, MH_COMMA usel)
+MH_WRAP(k_sweep_cell_row, 1, template <int DIM MH_COMMA int M MH_COMMA int LIM>, <DIM MH_COMMA M MH_COMMA LIM>,
+        MH_COMMA int usel MH_COMMA int first, MH_COMMA usel MH_COMMA first)
+MH_WRAP(k_sweep_cell_tile, MH_TILE_MIN_WAVES,
+        template <int DIM MH_COMMA int D MH_COMMA int M MH_COMMA int LIM>, <DIM MH_COMMA D MH_COMMA M MH_COMMA LIM>,
+        MH_COMMA int usel MH_COMMA int first, MH_COMMA usel MH_COMMA first)
+MH_WRAP(k_sweep_node_row, 1, template <int DIM MH_COMMA int LIM>, <DIM MH_COMMA LIM>,
+        MH_COMMA int psel MH_COMMA int first MH_COMMA int last, MH_COMMA psel MH_COMMA first MH_COMMA last)
 
 __global__ __launch_bounds__(MH_BLOCK, MH_NODE_MIN_WAVES) void k_node_lag3_tiled(MhDev S) {
   k_node_lag3_tiled_body(S, blockIdx.x, blockIdx.y, blockIdx.z);

@@ -100,6 +100,20 @@ int mahyco_compute_pression_moyenne(MhHandle* h);
 int mahyco_history_enable(MhHandle* h, int period, int max_rows);
 int mahyco_history_read(MhHandle* h, double* rows, int max_rows);
 
+/* Post-processing output of the case's <output> variables (Donnees.arc <output> block,
+ * written by Arcane's EnSight Gold post-processor into output/depouillement and compared
+ * byte for byte by NONREGRESSION/launch_test.sh:74-83): appends the current state as the
+ * next EnSight Gold ASCII step under dir (ensight.case, ensight.geo/ensight.geoNNNNNN,
+ * <Var>/<Var>NNNNNN).  variables are reference names (CellMass, Pressure, Density, Velocity,
+ * NodeMass, InternalEnergy, PseudoViscosity, Materiau, SoundSpeed, FracVol, CellVolume,
+ * CaracteristicLength); every step of a case uses the same list. */
+int mahyco_ensight_write(MhHandle* h, const char* dir, const char* const* variables, int nb_variables);
+/* The EnvSummation time history as Arcane writes it into output/courbes/gnuplot
+ * (compared by NONREGRESSION/launch_test.sh:85-91): files <Var>_<environment name> for
+ * CellMass, Pressure, Density and InternalEnergy, one "%.16E %.16E" line per row.
+ * env_names holds one name per environment. */
+int mahyco_history_write_gnuplot(MhHandle* h, const char* dir, const char* const* env_names);
+
 /* ---- ensembles (no reference counterpart: many independent cases on one GPU) ---------
  * Cases of any size advance together, each with its own time step and stop rule: one
  * launch per kernel phase covers every active case of a (dimension, environment count)

@@ -14,6 +14,7 @@
 
 #include "../../include/mahyco.h"
 #include "mh_kernels.h"
+#include "mh_output.h"
 
 namespace {
 
@@ -67,6 +68,8 @@ struct MhHandle {
   bool owns_stream = true;
   bool node_tiled = true;   // 3D: z-marching node kernel (MAHYCO_NODE_KERNEL=flat selects the flat one)
   int sweep_mode = MH_SWEEP_TILE;  // MAHYCO_SWEEP=flat selects one thread per cell / node
+  std::vector<const mh_output::VarSpec*> ens_vars;  // EnSight variables of this case's outputs
+  std::vector<double> ens_times;                    // times of the EnSight steps written
   MhScalars* ext_sc = nullptr;  // scalars living in an ensemble's array
   size_t bytes = 0;             // device bytes of the case
 };
@@ -1435,6 +1438,44 @@ int mahyco_history_read(MhHandle* h, double* rows, int max_rows) {
   if (rows && n > 0)
     HIPCHK(hipMemcpy(rows, h->dev.hist, sizeof(double) * (size_t)n * (1 + 5 * h->m), hipMemcpyDeviceToHost));
   return sc.hist_rows;
+}
+
+int mahyco_ensight_write(MhHandle* h, const char* dir, const char* const* variables, int nb_variables) {
+  if (!h || !dir || (!variables && nb_variables > 0) || nb_variables < 0) return MH_ERR_INVALID;
+  std::vector<const mh_output::VarSpec*> vars;
+  for (int i = 0; i < nb_variables; ++i) {
+    const mh_output::VarSpec* s = variables[i] ? mh_output::find_var(variables[i]) : nullptr;
+    if (!s) return MH_ERR_INVALID;
+    vars.push_back(s);
+  }
+  if (!h->ens_times.empty() && vars != h->ens_vars) return MH_ERR_INVALID;  // one variable list per case
+  const std::string d(dir);
+  if (!mh_output::make_dir(d) || !mh_output::make_dir(d + "/ensight.geo")) return MH_ERR_INVALID;
+  for (const auto* s : vars)
+    if (!mh_output::make_dir(d + "/" + s->name)) return MH_ERR_INVALID;
+  const int step = (int)h->ens_times.size();
+  const std::string sn = mh_output::step_name(step);
+  const int n[3] = {h->g.nx, h->g.ny, h->g.nz};
+  int st = mh_output::write_geometry(h, h->g.dim, n, d + "/ensight.geo/ensight.geo" + sn);
+  for (size_t i = 0; st == MH_OK && i < vars.size(); ++i)
+    st = mh_output::write_variable(h, h->g.dim, *vars[i], d + "/" + vars[i]->name + "/" + vars[i]->name + sn);
+  if (st != MH_OK) return st;
+  h->ens_vars = vars;
+  h->ens_times.push_back(mahyco_time(h));
+  return mh_output::write_case(d, h->ens_vars, h->ens_times);
+}
+
+int mahyco_history_write_gnuplot(MhHandle* h, const char* dir, const char* const* env_names) {
+  if (!h || !dir || !env_names) return MH_ERR_INVALID;
+  for (int k = 0; k < h->m; ++k)
+    if (!env_names[k]) return MH_ERR_INVALID;
+  const int n = mahyco_history_read(h, nullptr, 0);
+  if (n < 0) return n;
+  std::vector<double> rows((size_t)std::max(n, 1) * (1 + 5 * h->m));
+  const int got = mahyco_history_read(h, rows.data(), n);
+  if (got < 0) return got;
+  if (!mh_output::make_dir(dir)) return MH_ERR_INVALID;
+  return mh_output::write_gnuplot(dir, rows, n, h->m, env_names);
 }
 
 int mahyco_profile_enable(MhHandle* h, int on) {

@@ -84,6 +84,10 @@ def load():
     lib.mahyco_history_enable.restype = C.c_int
     lib.mahyco_history_read.argtypes = [H, C.c_void_p, C.c_int]
     lib.mahyco_history_read.restype = C.c_int
+    lib.mahyco_ensight_write.argtypes = [H, C.c_char_p, C.POINTER(C.c_char_p), C.c_int]
+    lib.mahyco_ensight_write.restype = C.c_int
+    lib.mahyco_history_write_gnuplot.argtypes = [H, C.c_char_p, C.POINTER(C.c_char_p)]
+    lib.mahyco_history_write_gnuplot.restype = C.c_int
     lib.mahyco_apply_eos.argtypes = [H, C.c_int]
     lib.mahyco_apply_eos.restype = C.c_int
     lib.mahyco_compute_pression_moyenne.argtypes = [H]
@@ -169,6 +173,20 @@ class GpuCase:
         buf = np.zeros(max(n, 1) * w)
         self.lib.mahyco_history_read(self.h, buf.ctypes.data, n)
         return buf[:n * w].reshape(n, w)
+
+    def ensight_write(self, directory, variables):
+        """Appends the current state as the next EnSight Gold step under directory."""
+        arr = (C.c_char_p * len(variables))(*[v.encode() for v in variables])
+        st = self.lib.mahyco_ensight_write(self.h, str(directory).encode(), arr, len(variables))
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_ensight_write")
+
+    def history_write_gnuplot(self, directory, env_names):
+        """Writes the EnvSummation histories as <Var>_<environment> gnuplot files."""
+        arr = (C.c_char_p * len(env_names))(*[v.encode() for v in env_names])
+        st = self.lib.mahyco_history_write_gnuplot(self.h, str(directory).encode(), arr)
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_history_write_gnuplot")
 
     def profile(self, on=True):
         self.lib.mahyco_profile_enable(self.h, int(on))

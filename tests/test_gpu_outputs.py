@@ -1,0 +1,72 @@
+"""On-disk outputs against the files the reference wrote for the same cases (GPU tests).
+
+The reference's non-regression suite compares its EnSight post-processing and gnuplot time
+histories with diff -r (NONREGRESSION/launch_test.sh:74-96).  Here the GPU run of three
+cases writes both with mahyco_ensight_write / mahyco_history_write_gnuplot and they are
+compared with the reference's own files (tests/golden/outputs, packed unchanged by
+tests/golden/make_output_fixtures.py): the EnSight files byte for byte, the time histories
+line for line in the same format, with the values at the bars of the fixture tests (time
+stamps 1e-13, EnvSummation values 1e-12 relative: the last of their 17 printed digits
+depend on summation order)."""
+import os
+import re
+import tarfile
+
+import numpy as np
+import pytest
+
+from tests.test_oracle_fixtures import FIXTURES, GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+OUTPUTS = os.path.join(GOLDEN, "outputs")
+CASES = ["CAS_BiSodCaseX", "CAS_2D_Lag_SodCaseX", "CAS_2D_Sedov"]
+LINE = re.compile(r"^-?\d\.\d{16}E[+-]\d{2} -?\d\.\d{16}E[+-]\d{2}$")
+
+
+def _files(root):
+    out = {}
+    for d, _, fs in os.walk(root):
+        for f in fs:
+            p = os.path.join(d, f)
+            out[os.path.relpath(p, root)] = p
+    return out
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_outputs_match_reference_files(name, tmp_path):
+    from mahyco_amd import types as T
+    from mahyco_amd.lib import GpuCase
+    print(f"[outputs] {name}", flush=True)
+    ref = tmp_path / "ref"
+    with tarfile.open(os.path.join(OUTPUTS, name + ".tar.gz")) as t:
+        t.extractall(ref)
+    case_text = (ref / "depouillement" / "ensight.case").read_text()
+    variables = re.findall(r"^(?:scalar|vector) per \w+:\s+1\s+(\w+) ", case_text, re.M)
+    env_names = [str(e) for e in np.load(os.path.join(GOLDEN, name + ".npz"))["env_names"]]
+    g = GpuCase(FIXTURES[name]())
+    g.history_enable(10, 4096)
+    st = T.MH_OK
+    while st == T.MH_OK:
+        st = g.run(500, graph=6)
+    assert st == T.MH_STOPPED
+    out = tmp_path / "out"
+    g.ensight_write(out / "depouillement", variables)
+    g.history_write_gnuplot(out / "courbes" / "gnuplot", env_names)
+    g.close()
+    mine, theirs = _files(out), _files(ref)
+    assert sorted(mine) == sorted(theirs)
+    for rel in sorted(theirs):
+        a = open(mine[rel]).read().splitlines()
+        b = open(theirs[rel]).read().splitlines()
+        assert len(a) == len(b), rel
+        if rel.startswith("depouillement"):
+            bad = [i for i, (x, y) in enumerate(zip(a, b)) if x != y]
+            assert not bad, (rel, len(bad), [(a[i], b[i]) for i in bad[:4]])
+            assert open(mine[rel], "rb").read() == open(theirs[rel], "rb").read(), rel
+        else:
+            assert all(LINE.match(x) for x in a), rel
+            va = np.array([[float(v) for v in x.split()] for x in a])
+            vb = np.array([[float(v) for v in x.split()] for x in b])
+            np.testing.assert_allclose(va[:, 0], vb[:, 0], rtol=1e-13, atol=0, err_msg=rel)
+            np.testing.assert_allclose(va[:, 1], vb[:, 1], rtol=1e-12, atol=0, err_msg=rel)

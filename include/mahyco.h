@@ -114,6 +114,15 @@ int mahyco_ensight_write(MhHandle* h, const char* dir, const char* const* variab
  * env_names holds one name per environment. */
 int mahyco_history_write_gnuplot(MhHandle* h, const char* dir, const char* const* env_names);
 
+/* Protection / reprise (Arcane checkpoint service of the <arcane-checkpoint> block, restart
+ * entry point MahycoModule::hydroContinueInit, MahycoModule.cc:273-307): write saves the
+ * whole device state of the case between two iterations; read restores it into a case
+ * created from the same description (mesh, options, environments, and the same
+ * mahyco_history_enable call), after which the run continues bit for bit as if it had not
+ * stopped.  A file of another case is refused with MH_ERR_INVALID and the case left as is. */
+int mahyco_checkpoint_write(MhHandle* h, const char* path);
+int mahyco_checkpoint_read(MhHandle* h, const char* path);
+
 /* ---- ensembles (no reference counterpart: many independent cases on one GPU) ---------
  * Cases of any size advance together, each with its own time step and stop rule: one
  * launch per kernel phase covers every active case of a (dimension, environment count)

@@ -53,6 +53,7 @@ struct MhHandle {
   MhDev dev;
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
+  std::vector<size_t> alloc_bytes;  // size of each allocation (checkpoints)
   double* xl[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   double *Ua = nullptr, *Ub = nullptr, *dflux = nullptr, *Ud = nullptr, *Pa = nullptr, *Pb = nullptr;
   long launched = 0;  // iterations queued by the host
@@ -99,6 +100,7 @@ int dalloc(MhHandle* h, T** p, size_t n) {
     return MH_ERR_NOMEM;
   }
   h->allocs.push_back(q);
+  h->alloc_bytes.push_back(n * sizeof(T));
   h->bytes += n * sizeof(T);
   *p = (T*)q;
   return MH_OK;
@@ -1476,6 +1478,116 @@ int mahyco_history_write_gnuplot(MhHandle* h, const char* dir, const char* const
   if (got < 0) return got;
   if (!mh_output::make_dir(dir)) return MH_ERR_INVALID;
   return mh_output::write_gnuplot(dir, rows, n, h->m, env_names);
+}
+
+// Checkpoint file: a header (magic, layout version, the case's mesh, options and
+// environments, counts and host-side cycle state) followed by every device allocation of
+// the case in allocation order.  Restoring needs a case created with the same description
+// (and the same time-history capacity), whose allocations therefore match one for one.
+static const char kCkptMagic[8] = {'M', 'H', 'C', 'K', 'P', 'T', '0', '1'};
+// field-by-field equality of case descriptions (struct padding is not compared)
+static bool same_mesh(const MhMesh& a, const MhMesh& b) {
+  bool eq = a.dim == b.dim && a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.coord_rule == b.coord_rule;
+  for (int d = 0; d < 3; ++d) eq = eq && a.origin[d] == b.origin[d] && a.length[d] == b.length[d];
+  return eq;
+}
+static bool same_env(const MhEnvironment& a, const MhEnvironment& b) {
+  bool eq = a.eos == b.eos && a.adiabatic_cst == b.adiabatic_cst && a.limit_tension == b.limit_tension &&
+            a.specific_heat == b.specific_heat && a.linear_pseudo_coeff == b.linear_pseudo_coeff &&
+            a.quadratic_pseudo_coeff == b.quadratic_pseudo_coeff && a.densite_initiale == b.densite_initiale &&
+            a.pression_initiale == b.pression_initiale;
+  for (int d = 0; d < 3; ++d) eq = eq && a.vitesse_initiale[d] == b.vitesse_initiale[d];
+  return eq;
+}
+static bool same_options(const MhOptions& a, const MhOptions& b) {
+  bool eq = a.cfl == b.cfl && a.threshold == b.threshold && a.deltat_init == b.deltat_init &&
+            a.deltat_min == b.deltat_min && a.deltat_max == b.deltat_max && a.final_time == b.final_time &&
+            a.with_projection == b.with_projection && a.calcul_vitson_max == b.calcul_vitson_max &&
+            a.nb_bc == b.nb_bc && a.ordre_projection == b.ordre_projection &&
+            a.projection_limiteur_id == b.projection_limiteur_id && a.remap_threshold == b.remap_threshold &&
+            a.cas_model == b.cas_model && a.cas_test == b.cas_test && a.interface_position == b.interface_position &&
+            a.interface_axis == b.interface_axis && a.sedov_deposit == b.sedov_deposit;
+  for (int i = 0; eq && i < a.nb_bc && i < MH_MAX_BC; ++i)
+    eq = a.bc[i].surface == b.bc[i].surface && a.bc[i].type == b.bc[i].type && a.bc[i].value == b.bc[i].value;
+  return eq;
+}
+struct CkptHead {
+  char magic[8];
+  int version, m;
+  MhMesh mesh;
+  MhOptions opt;
+  MhEnvironment env[MH_MAX_ENV];
+  long nc, nn, launched;
+  int sens, hist_period, hist_max, max_iter;
+  long nallocs;
+};
+
+int mahyco_checkpoint_write(MhHandle* h, const char* path) {
+  if (!h || !path || h->ext_sc || h->phase != 0) return MH_ERR_INVALID;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  CkptHead hd;
+  memset(&hd, 0, sizeof hd);
+  memcpy(hd.magic, kCkptMagic, 8);
+  hd.version = 1;
+  hd.m = h->m;
+  hd.mesh = h->mesh;
+  hd.opt = h->opt;
+  for (int k = 0; k < h->m; ++k) hd.env[k] = h->env[k];
+  hd.nc = h->g.nc;
+  hd.nn = h->g.nn;
+  hd.launched = h->launched;
+  hd.sens = h->sens;
+  hd.hist_period = h->dev.hist_period;
+  hd.hist_max = h->dev.hist_max;
+  hd.max_iter = h->dev.max_iter;
+  hd.nallocs = (long)h->allocs.size();
+  FILE* f = fopen(path, "wb");
+  if (!f) return MH_ERR_INVALID;
+  bool ok = fwrite(&hd, sizeof hd, 1, f) == 1;
+  std::vector<char> buf;
+  for (size_t i = 0; ok && i < h->allocs.size(); ++i) {
+    const uint64_t n = h->alloc_bytes[i];
+    buf.resize(n);
+    ok = hipMemcpy(buf.data(), h->allocs[i], n, hipMemcpyDeviceToHost) == hipSuccess && fwrite(&n, sizeof n, 1, f) == 1 &&
+         fwrite(buf.data(), 1, n, f) == n;
+  }
+  ok = (fclose(f) == 0) && ok;
+  return ok ? MH_OK : MH_ERR_INVALID;
+}
+
+int mahyco_checkpoint_read(MhHandle* h, const char* path) {
+  if (!h || !path || h->ext_sc || h->phase != 0) return MH_ERR_INVALID;
+  FILE* f = fopen(path, "rb");
+  if (!f) return MH_ERR_INVALID;
+  CkptHead hd;
+  bool ok = fread(&hd, sizeof hd, 1, f) == 1 && memcmp(hd.magic, kCkptMagic, 8) == 0 && hd.version == 1 &&
+            hd.m == h->m && same_mesh(hd.mesh, h->mesh) && same_options(hd.opt, h->opt) && hd.nc == h->g.nc &&
+            hd.nn == h->g.nn &&
+            hd.hist_period == h->dev.hist_period && hd.hist_max == h->dev.hist_max &&
+            hd.nallocs == (long)h->allocs.size();
+  for (int k = 0; ok && k < h->m; ++k) ok = same_env(hd.env[k], h->env[k]);
+  std::vector<std::vector<char>> data(ok ? h->allocs.size() : 0);
+  for (size_t i = 0; ok && i < h->allocs.size(); ++i) {
+    uint64_t n = 0;
+    ok = fread(&n, sizeof n, 1, f) == 1 && n == h->alloc_bytes[i];
+    if (ok) {
+      data[i].resize(n);
+      ok = fread(data[i].data(), 1, n, f) == n;
+    }
+  }
+  fclose(f);
+  if (!ok) return MH_ERR_INVALID;  // nothing of the case has been changed
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (size_t i = 0; i < h->allocs.size(); ++i)
+    HIPCHK(hipMemcpy(h->allocs[i], data[i].data(), data[i].size(), hipMemcpyHostToDevice));
+  h->launched = hd.launched;
+  h->sens = hd.sens;
+  h->dev.max_iter = hd.max_iter;
+  for (auto& kv : h->graphs) (void)hipGraphExecDestroy(kv.second);  // they hold the old kernel arguments
+  for (auto gdef : h->graph_defs) (void)hipGraphDestroy(gdef);
+  h->graphs.clear();
+  h->graph_defs.clear();
+  return MH_OK;
 }
 
 int mahyco_profile_enable(MhHandle* h, int on) {

@@ -88,6 +88,9 @@ def load():
     lib.mahyco_ensight_write.restype = C.c_int
     lib.mahyco_history_write_gnuplot.argtypes = [H, C.c_char_p, C.POINTER(C.c_char_p)]
     lib.mahyco_history_write_gnuplot.restype = C.c_int
+    for name in ("mahyco_checkpoint_write", "mahyco_checkpoint_read"):
+        getattr(lib, name).argtypes = [H, C.c_char_p]
+        getattr(lib, name).restype = C.c_int
     lib.mahyco_apply_eos.argtypes = [H, C.c_int]
     lib.mahyco_apply_eos.restype = C.c_int
     lib.mahyco_compute_pression_moyenne.argtypes = [H]
@@ -187,6 +190,16 @@ class GpuCase:
         st = self.lib.mahyco_history_write_gnuplot(self.h, str(directory).encode(), arr)
         if st != T.MH_OK:
             raise MahycoError(st, "mahyco_history_write_gnuplot")
+
+    def checkpoint_write(self, path):
+        st = self.lib.mahyco_checkpoint_write(self.h, str(path).encode())
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_checkpoint_write")
+
+    def checkpoint_read(self, path):
+        st = self.lib.mahyco_checkpoint_read(self.h, str(path).encode())
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_checkpoint_read")
 
     def profile(self, on=True):
         self.lib.mahyco_profile_enable(self.h, int(on))

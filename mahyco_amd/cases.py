@@ -173,6 +173,13 @@ def sod_mono(axis="Y") -> Case:
                 bcs=list(WALLS_3D))
 
 
+def ordre3(make, name) -> Case:
+    """The NONREGRESSION/CAS_*_ordre_3 variants: the same case with <ordre-projection>3."""
+    c = make()
+    c.name, c.ordre_projection = name, 3
+    return c
+
+
 def lag_bisod_x() -> Case:
     """NONREGRESSION/CAS_Lag_BiSodCaseX/Donnees.arc (3D Lagrange only)."""
     c = bisod_x()

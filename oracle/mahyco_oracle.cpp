@@ -118,6 +118,8 @@ struct OracleCase {
   std::vector<double> u_lagrange, phi_lagrange, grad_phi, grad_phi_face, phi_face, dual_phi_flux;
   std::vector<double> u_dual, phi_dual, dual_grad_phi;  // [nn][5]
   std::vector<double> deltax_lagrange, face_normal_velocity;
+  std::vector<double> h_cell_lagrange;       // cell width along the sweep (order 3)
+  std::vector<int> est_mixte, est_pure;      // mixed-cell indicators (order 3); Arcane starts them at 0
   std::vector<R3> face_length_lagrange;
   std::vector<double> back_flux_mass, front_flux_mass;
   int sens_projection = 0;
@@ -167,6 +169,8 @@ struct OracleCase {
   void computeUpwindFaceQuantitiesForProjection(int idir);
   void computeUremap(int idir);
   void computeDualUremap(int idir);
+  int dirNeighbor(long c, int idir, bool back) const;
+  void setMixedFlags(long c, const double* f);
   int remapVariables();
   void setMembership(const std::vector<uint8_t>& newmask);
   int computeDeltaT();
@@ -968,8 +972,10 @@ void OracleCase::computeFaceQuantitesForRemap() {
   }
 }
 
-// RemapADIService::computeGradPhiFace, RemapADIService.cc:119-149 (inner faces only;
-// h_cell is used by the G/pente-borne/order-3 paths only and is not formed).
+// RemapADIService::computeGradPhiFace, RemapADIService.cc:119-149 (inner faces only).
+// h_cell (the sum over the cell's inner faces of the sweep of |x_f - x_c|) is formed
+// for order 3, its only user here; with at most two terms per cell starting from 0 the
+// face order of the sum does not matter.
 void OracleCase::computeGradPhiFace(int idir) {
 #pragma omp parallel for schedule(static)
   for (long f = face_off[idir]; f < face_off[idir] + face_cnt[idir]; ++f) {
@@ -979,6 +985,90 @@ void OracleCase::computeGradPhiFace(int idir) {
     for (int v = 0; v < nv; ++v)
       grad_phi_face[f * nv + v] = (phi_lagrange[(long)cf * nv + v] - phi_lagrange[(long)cb * nv + v]) / deltax_lagrange[f];
   }
+  if (opt.ordre_projection == 3) {
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < nc; ++c) {
+      double h = 0.;
+      for (int l = 0; l < nfpc; ++l) {
+        const int f = cell_faces[c * nfpc + l];
+        if (face_dir[f] != idir || face_prev[f] < 0 || face_next[f] < 0) continue;
+        h += normL2(face_coord[f] - cell_coord[c]);
+      }
+      h_cell_lagrange[c] = h;
+    }
+  }
+}
+
+// Remap-ordre-3.cc:16-112: evaluate_grad, evaluate_ystar, evaluate_fm / evaluate_fp,
+// define_interval, intersection.
+static inline double o3_grad(double hm, double h0, double hp, double ym, double y0, double yp) {
+  return h0 / (hm + h0 + hp) * ((2. * hm + h0) / (h0 + hp) * (yp - y0) + (h0 + 2. * hp) / (hm + h0) * (y0 - ym));
+}
+static inline double o3_ystar(double hmm, double hm, double hp, double hpp, double ymm, double ym, double yp, double ypp,
+                              double gradm, double gradp) {
+  const double tmp1 = (2. * hp * hm) / (hm + hp) * ((hmm + hm) / (2. * hm + hp) - (hpp + hp) / (2. * hp + hm)) * (yp - ym);
+  const double tmp2 = -hm * (hmm + hm) / (2. * hm + hp) * gradp + hp * (hp + hpp) / (hm + 2. * hp) * gradm;
+  return ym + hm / (hm + hp) * (yp - ym) + 1. / (hmm + hm + hp + hpp) * (tmp1 + tmp2);
+}
+static inline double o3_fm(double x, double dx, double up, double du, double u6) {
+  return up - 0.5 * x / dx * (du - (1. - 2. / 3. * x / dx) * u6);
+}
+static inline double o3_fp(double x, double dx, double um, double du, double u6) {
+  return um + 0.5 * x / dx * (du - (1. - 2. / 3. * x / dx) * u6);
+}
+// RemapADIService::ComputeFluxOrdre3, Remap-ordre-3.cc:127-189 (face value with TVD clip)
+static double flux_ordre3(double ymmm, double ymm, double ym, double yp, double ypp, double yppp, double hmmm,
+                          double hmm, double hm, double hp, double hpp, double hppp, double vdt) {
+  const double gradmm = o3_grad(hmmm, hmm, hm, ymmm, ymm, ym);
+  const double gradm = o3_grad(hmm, hm, hp, ymm, ym, yp);
+  const double gradp = o3_grad(hm, hp, hpp, ym, yp, ypp);
+  const double gradpp = o3_grad(hp, hpp, hppp, yp, ypp, yppp);
+  const double ystarm = o3_ystar(hmmm, hmm, hm, hp, ymmm, ymm, ym, yp, gradmm, gradm);
+  const double ystar = o3_ystar(hmm, hm, hp, hpp, ymm, ym, yp, ypp, gradm, gradp);
+  const double ystarp = o3_ystar(hm, hp, hpp, hppp, ym, yp, ypp, yppp, gradp, gradpp);
+  const double ym_m = ystarm, ym_p = ystar, yp_m = ystar, yp_p = ystarp;
+  const double grad_m = ym_p - ym_m, grad_p = yp_p - yp_m;
+  const double ym6 = 6. * (ym - 0.5 * (ym_m + ym_p));
+  const double yp6 = 6. * (yp - 0.5 * (yp_m + yp_p));
+  double flux;
+  if (vdt > 0.) flux = o3_fm(vdt, hm, ym_p, grad_m, ym6);
+  else if (vdt < 0.) flux = o3_fp(-vdt, hp, yp_m, grad_p, yp6);
+  else return 0.;  // vdt == 0 (a NaN vdt leaves flux undefined in the reference; not reached)
+  const double num = vdt / hm, nup = vdt / hp;
+  const double ym_ym = ym + (1. - num) / num * (ym - ymm);
+  const double yp_ym = yp - (1. + nup) / nup * (yp - ypp);
+  double I1[2], I2[2];
+  I1[0] = amin(ym, yp);
+  I1[1] = amax(ym, yp);
+  if (vdt >= 0.) {
+    I2[0] = amin(ym, ym_ym);
+    I2[1] = amax(ym, ym_ym);
+  } else {
+    I2[0] = amin(yp, yp_ym);
+    I2[1] = amax(yp, yp_ym);
+  }
+  double lo, hi;
+  if ((I1[1] < I2[0]) || (I2[1] < I1[0])) {
+    lo = 0.;
+    hi = 0.;
+  } else {
+    lo = amax(I1[0], I2[0]);
+    hi = amin(I1[1], I2[1]);
+  }
+  if (flux < lo) flux = lo;
+  if (flux > hi) flux = hi;
+  return flux;
+}
+
+// Neighbour of cell c across its back (back = true) or front face along idir, -1 at the boundary.
+int OracleCase::dirNeighbor(long c, int idir, bool back) const {
+  for (int l = 0; l < nfpc; ++l) {
+    const int f = cell_faces[c * nfpc + l];
+    if (face_dir[f] != idir) continue;
+    if (back && face_next[f] == c) return face_prev[f];
+    if (!back && face_prev[f] == c) return face_next[f];
+  }
+  return -1;
 }
 
 // RemapADIService::fluxLimiter, UtilesRemap.cc:108-120.
@@ -1017,9 +1107,55 @@ void OracleCase::computeGradPhiCell(int idir) {
   }
 }
 
-// RemapADIService::computeUpwindFaceQuantitiesForProjection :279-383 (order <= 2, no pente-borne).
+// RemapADIService::computeUpwindFaceQuantitiesForProjection :279-383 (no pente-borne):
+// order <= 2 upwind values; order 3 (:319-379) the 6-cell flux of ComputeFluxOrdre3 with
+// neighbours clamped at the boundary, then, when the two cells are not pure cells of one
+// environment, M_k and E_k face values rebuilt from the upwind cell's ratios.
 void OracleCase::computeUpwindFaceQuantitiesForProjection(int idir) {
   std::fill(phi_face.begin(), phi_face.end(), 0.0);
+  if (opt.ordre_projection == 3) {
+#pragma omp parallel for schedule(static)
+    for (long f = face_off[idir]; f < face_off[idir] + face_cnt[idir]; ++f) {
+      const int cb = face_prev[f], cf = face_next[f];
+      if (cb < 0 || cf < 0) continue;
+      int cbb = cb, cbbb = cb, cff = cf, cfff = cf;
+      const int pb = dirNeighbor(cb, idir, true);
+      if (pb >= 0) {
+        cbb = pb;
+        cbbb = cbb;
+        const int pbb = dirNeighbor(cbb, idir, true);
+        if (pbb >= 0) cbbb = pbb;
+      }
+      const int nf_ = dirNeighbor(cf, idir, false);
+      if (nf_ >= 0) {
+        cff = nf_;
+        cfff = cff;
+        const int nff = dirNeighbor(cff, idir, false);
+        if (nff >= 0) cfff = nff;
+      }
+      const double vdt = face_normal_velocity[f] * global_deltat;
+      const double* P[6] = {&phi_lagrange[(long)cbbb * nv], &phi_lagrange[(long)cbb * nv], &phi_lagrange[(long)cb * nv],
+                            &phi_lagrange[(long)cf * nv], &phi_lagrange[(long)cff * nv], &phi_lagrange[(long)cfff * nv]};
+      for (int v = 0; v < nv; ++v)
+        phi_face[f * nv + v] = flux_ordre3(P[0][v], P[1][v], P[2][v], P[3][v], P[4][v], P[5][v], h_cell_lagrange[cbbb],
+                                           h_cell_lagrange[cbb], h_cell_lagrange[cb], h_cell_lagrange[cf],
+                                           h_cell_lagrange[cff], h_cell_lagrange[cfff], vdt);
+      const bool voisinage_pure = est_mixte[cb] == 0 && est_mixte[cf] == 0 && est_pure[cb] == est_pure[cf];
+      if (!voisinage_pure) {
+        double* F = &phi_face[f * nv];
+        for (int k = 0; k < m; ++k) {
+          if (vdt > 0. && P[2][k] != 0. && P[2][m + k] != 0.) {
+            F[m + k] = (P[2][m + k] / P[2][k]) * F[k];
+            F[2 * m + k] = (P[2][2 * m + k] / P[2][m + k]) * F[m + k];
+          } else if (vdt < 0. && P[3][k] != 0. && P[3][m + k] != 0.) {
+            F[m + k] = (P[3][m + k] / P[3][k]) * F[k];
+            F[2 * m + k] = (P[3][2 * m + k] / P[3][m + k]) * F[m + k];
+          }
+        }
+      }
+    }
+    return;
+  }
   const int order2 = opt.ordre_projection - 1;
 #pragma omp parallel for schedule(static)
   for (long f = face_off[idir]; f < face_off[idir] + face_cnt[idir]; ++f) {
@@ -1064,7 +1200,22 @@ void OracleCase::computeUremap(int idir) {
     double somme_volume = 0.;
     for (int k = 0; k < m; ++k) somme_volume += U[k];
     for (int v = 0; v < nv; ++v) phi_lagrange[c * nv + v] = U[v] / somme_volume;
+    setMixedFlags(c, &phi_lagrange[c * nv]);  // :559-573
   }
+}
+
+// est_mixte / est_pure from the volume fractions (RemapADIService.cc:559-573,
+// RemapADIFinal.cc:124-145): mixed if more than one is positive, else the index of the
+// positive one (-1 if none).
+void OracleCase::setMixedFlags(long c, const double* f) {
+  int cnt = 0, pure = -1;
+  for (int k = 0; k < m; ++k)
+    if (f[k] > 0.) {
+      cnt++;
+      pure = k;
+    }
+  est_mixte[c] = cnt > 1 ? 1 : 0;
+  est_pure[c] = cnt > 1 ? -1 : pure;
 }
 
 // RemapADIService::computeDualUremap, DualPhaseRemap.cc:38-258, with
@@ -1228,6 +1379,11 @@ int OracleCase::remapVariables() {
     if (somme_frac != 0.) unsursomme_frac = 1. / somme_frac;
     for (int k = 0; k < m; ++k)
       if (member(k, c)) ev(fracvol, k, c) *= unsursomme_frac;
+    {  // RemapADIFinal.cc:124-145
+      double fr[MH_MAX_ENV];
+      for (int k = 0; k < m; ++k) fr[k] = member(k, c) ? ev(fracvol, k, c) : 0.;
+      setMixedFlags(c, fr);
+    }
     if (masset > thr * thr) {
       const double unsurmasset = 1. / masset;
       double fmasset = 0.;
@@ -1384,7 +1540,7 @@ OracleCase* oracle_create(const MhMesh* mesh, const MhEnvironment* envs, int nb_
   if (!mesh || !envs || !opt || nb_env < 1 || nb_env > MH_MAX_ENV) return nullptr;
   if (mesh->dim != 2 && mesh->dim != 3) return nullptr;
   if (mesh->nx < 1 || mesh->ny < 1 || (mesh->dim == 3 && mesh->nz < 1)) return nullptr;
-  if (opt->ordre_projection < 1 || opt->ordre_projection > 2) return nullptr;
+  if (opt->ordre_projection < 1 || opt->ordre_projection > 3) return nullptr;
   OracleCase* h = new OracleCase();
   h->m = nb_env;
   for (int k = 0; k < nb_env; ++k) h->env[k] = envs[k];
@@ -1422,6 +1578,9 @@ OracleCase* oracle_create(const MhMesh* mesh, const MhEnvironment* envs, int nb_
   h->phi_dual.assign(nn * 5, 0.);
   h->dual_grad_phi.assign(nn * 5, 0.);
   h->deltax_lagrange.assign(nf, 0.);
+  h->h_cell_lagrange.assign(h->nc, 0.);
+  h->est_mixte.assign(h->nc, 0);
+  h->est_pure.assign(h->nc, 0);
   h->face_normal_velocity.assign(nf, 0.);
   h->face_length_lagrange.assign(nf, {0., 0., 0.});
   h->back_flux_mass.assign(nn, 0.);

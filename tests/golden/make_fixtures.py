@@ -36,6 +36,12 @@ CASES = {
     "CAS_2D_Lag_SodCaseX": (2, (100, 5, 1), (0, 0, 0), (1., .1, 0.)),
     "CAS_2D_Sedov": (2, (60, 60, 1), (0, 0, 0), (1.2, 1.2, 0.)),
     "CAS_bitriplepoint_minmod": (2, (70, 30, 1), (0, 0, 0), (.07, .03, 0.)),
+    "CAS_BiSodCaseX_ordre_3": (3, (100, 5, 5), (0, 0, 0), (1., .1, .1)),
+    "CAS_BiSodCaseY_ordre_3": (3, (5, 100, 5), (0, 0, 0), (.1, 1., .1)),
+    "CAS_BiSodCaseZ_ordre_3": (3, (5, 5, 100), (0, 0, 0), (.1, .1, 1.)),
+    "CAS_SodCaseX_ordre_3": (3, (100, 5, 5), (0, 0, 0), (1., .1, .1)),
+    "CAS_SodCaseY_ordre_3": (3, (5, 100, 5), (0, 0, 0), (.1, 1., .1)),
+    "CAS_SodCaseZ_ordre_3": (3, (5, 5, 100), (0, 0, 0), (.1, .1, 1.)),
 }
 TH_VARS = ["CellMass", "Pressure", "Density", "InternalEnergy"]
 CELL_VARS = ["Density", "Pressure", "InternalEnergy", "CellMass", "PseudoViscosity", "Materiau"]

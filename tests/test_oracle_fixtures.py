@@ -28,6 +28,13 @@ FIXTURES = {
     "CAS_2D_Lag_SodCaseX": cases.lag2d_sod_x,
     "CAS_2D_Sedov": cases.sedov_2d,
     "CAS_bitriplepoint_minmod": cases.bitriplepoint,
+    # order-3 remap (Remap-ordre-3.cc); the BiSod X variant ran on 4 subdomains
+    "CAS_BiSodCaseX_ordre_3": lambda: cases.ordre3(cases.bisod_x, "CAS_BiSodCaseX_ordre_3"),
+    "CAS_BiSodCaseY_ordre_3": lambda: cases.ordre3(cases.bisod_y, "CAS_BiSodCaseY_ordre_3"),
+    "CAS_BiSodCaseZ_ordre_3": lambda: cases.ordre3(cases.bisod_z, "CAS_BiSodCaseZ_ordre_3"),
+    "CAS_SodCaseX_ordre_3": lambda: cases.ordre3(lambda: cases.sod_mono("X"), "CAS_SodCaseX_ordre_3"),
+    "CAS_SodCaseY_ordre_3": lambda: cases.ordre3(lambda: cases.sod_mono("Y"), "CAS_SodCaseY_ordre_3"),
+    "CAS_SodCaseZ_ordre_3": lambda: cases.ordre3(lambda: cases.sod_mono("Z"), "CAS_SodCaseZ_ordre_3"),
 }
 TH = ["CellMass", "Pressure", "Density", "InternalEnergy"]
 ENSIGHT = {"Density": T.MH_F_DENSITY, "Pressure": T.MH_F_PRESSURE, "InternalEnergy": T.MH_F_INTERNAL_ENERGY,

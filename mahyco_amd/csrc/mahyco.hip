@@ -21,7 +21,7 @@ namespace {
 // profiled kernel groups (sweeps separately per direction: they run different code)
 const char* kKernelNames[] = {"k_node_lag",     "k_cell_lag",     "k_sweep_cell_x",  "k_sweep_cell_y",
                               "k_sweep_cell_z", "k_sweep_node_x", "k_sweep_node_y",  "k_sweep_node_z",
-                              "k_node_velocity", "k_finalize",    "k_dt"};
+                              "k_node_velocity", "k_finalize",    "k_dt",            "k_hcell"};
 enum {
   KN_NODE = 0,
   KN_CELL,
@@ -30,6 +30,7 @@ enum {
   KN_NODE_VEL = KN_SWEEP_NODE + 3,
   KN_FINAL,
   KN_DT,
+  KN_HCELL,  // cell widths of the order-3 remap
   KN_COUNT
 };
 
@@ -426,11 +427,25 @@ void launch_sweeps_t(MhHandle* h) {
     if (ncell == 1) continue;  // RemapADIService.cc:27
     dirs[nd++] = idir;
   }
+  const bool o3 = S.order2 == 2;
+  if (o3 && nd > 0) LAUNCH(h, KN_HCELL, (k_hcell<DIM>), h->nblk_c, MH_BLOCK, S);
   for (int s = 0; s < nd; ++s) {
     const int first = s == 0, last = (s == nd - 1) && !first, sel = s & 1;
     const int D = dirs[s];
     const int mode = h->sweep_mode;
-    if (D == 0) {
+    if (o3) {  // order 3: one thread per cell for the cell part, node part as selected
+      if (D == 0) LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell3<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+      else if (D == 1) LAUNCH(h, KN_SWEEP_CELL + 1, (k_sweep_cell3<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+      else if constexpr (DIM == 3) LAUNCH(h, KN_SWEEP_CELL + 2, (k_sweep_cell3<3, 2, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+      if (D == 0 && mode != MH_SWEEP_FLAT)
+        MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node_row<DIM, LIM>), node_row_blocks(h), MH_BLOCK, S, sel, first, last));
+      else if (D == 0)
+        MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
+      else if (D == 1)
+        MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE + 1, (k_sweep_node<DIM, 1, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
+      else if constexpr (DIM == 3)
+        MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE + 2, (k_sweep_node<3, 2, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
+    } else if (D == 0) {
       if (mode == MH_SWEEP_FLAT) {
         MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, 0, M, LIM>), h->nblk_c, MH_BLOCK, S, sel, first));
         MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
@@ -527,7 +542,7 @@ int validate(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const Mh
   if (!mesh || !envs || !o || nb_env < 1 || nb_env > MH_MAX_ENV) return MH_ERR_INVALID;
   if (mesh->dim != 2 && mesh->dim != 3) return MH_ERR_INVALID;
   if (mesh->nx < 1 || mesh->ny < 1 || (mesh->dim == 3 && mesh->nz < 1)) return MH_ERR_INVALID;
-  if (o->ordre_projection < 1 || o->ordre_projection > 2) return MH_ERR_UNSUPPORTED;
+  if (o->ordre_projection < 1 || o->ordre_projection > 3) return MH_ERR_UNSUPPORTED;
   if (o->nb_bc < 0 || o->nb_bc > MH_MAX_BC) return MH_ERR_INVALID;
   for (int b = 0; b < o->nb_bc; ++b)
     if (o->bc[b].surface < 0 || o->bc[b].surface > 5 || o->bc[b].type < 0 || o->bc[b].type > 2) return MH_ERR_INVALID;
@@ -571,6 +586,7 @@ size_t case_footprint(const MhMesh& mesh, int m, int with_proj) {
 
 struct EnsGroup {
   int dim = 0, m = 0;
+  bool o3 = false;  // order-3 remap
   std::vector<int> slots;
   int2* d_cmap = nullptr;
   int2* d_nmap = nullptr;
@@ -644,7 +660,7 @@ int ens_rebuild(MhEnsembleImpl* E) {
     }
     bool found = false;
     for (auto& g : E->groups)
-      if (g.dim == h->g.dim && g.m == h->m) {
+      if (g.dim == h->g.dim && g.m == h->m && g.o3 == (h->dev.order2 == 2)) {
         g.slots.push_back(sl);
         found = true;
       }
@@ -652,6 +668,7 @@ int ens_rebuild(MhEnsembleImpl* E) {
       EnsGroup g;
       g.dim = h->g.dim;
       g.m = h->m;
+      g.o3 = h->dev.order2 == 2;
       g.slots.push_back(sl);
       E->groups.push_back(g);
     }
@@ -674,7 +691,7 @@ int ens_rebuild(MhEnsembleImpl* E) {
     HIPCHK(hipMemcpy(g.d_nmap, nm.data(), sizeof(int2) * nm.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(g.d_list, g.slots.data(), sizeof(int) * g.slots.size(), hipMemcpyHostToDevice));
     // rows and tiles when every case of the group runs them
-    g.rows = true;
+    g.rows = !g.o3;
     for (int sl : g.slots) g.rows = g.rows && E->slot[sl]->sweep_mode == MH_SWEEP_TILE;
     if (g.rows) {
       std::vector<int2> rc, rn, t[2];
@@ -712,9 +729,24 @@ void ens_group_cycle(MhEnsembleImpl* E, const EnsGroup& g) {
   hipLaunchKernelGGL((k_node_lag_batch<DIM>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap);
   hipLaunchKernelGGL((k_cell_lag_batch<DIM, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap);
   const int sens = (int)(E->launched % DIM);
+  if (g.o3) hipLaunchKernelGGL((k_hcell_batch<DIM>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap);
   for (int s = 0; s < DIM; ++s) {
     const int D = (s + sens) % DIM, first = s == 0, last = s == DIM - 1, sel = s & 1;
-    if (g.rows && D == 0) {
+    if (g.o3) {
+      if (D == 0)
+        hipLaunchKernelGGL((k_sweep_cell3_batch<DIM, 0, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
+      else if (D == 1)
+        hipLaunchKernelGGL((k_sweep_cell3_batch<DIM, 1, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
+      else if constexpr (DIM == 3)
+        hipLaunchKernelGGL((k_sweep_cell3_batch<3, 2, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
+    }
+    if (g.o3 && D == 0) {
+      hipLaunchKernelGGL((k_sweep_node_batch<DIM, 0, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
+    } else if (g.o3 && D == 1) {
+      hipLaunchKernelGGL((k_sweep_node_batch<DIM, 1, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
+    } else if (g.o3) {
+      hipLaunchKernelGGL((k_sweep_node_batch<DIM, (DIM == 3 ? 2 : 0), -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
+    } else if (g.rows && D == 0) {
       hipLaunchKernelGGL((k_sweep_cell_row_batch<DIM, M, -1>), dim3(g.nrcb), dim3(MH_BLOCK), 0, st, tab, g.d_rcmap, sel, first);
       hipLaunchKernelGGL((k_sweep_node_row_batch<DIM, -1>), dim3(g.nrnb), dim3(MH_BLOCK), 0, st, tab, g.d_rnmap, sel, first, last);
     } else if (g.rows && D == 1) {
@@ -896,6 +928,10 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
     ok |= dalloc(h, &h->Ud, (size_t)4 * nn);
     ok |= dalloc(h, &h->Pa, (size_t)3 * nn);
     ok |= dalloc(h, &h->Pb, (size_t)3 * nn);
+    if (options->ordre_projection == 3) {
+      ok |= dalloc(h, &S.hc, (size_t)g.dim * nc);
+      ok |= dalloc(h, &S.est, (size_t)nc);
+    }
   } else {
     ok |= dalloc(h, &h->Ua, 1);
   }
@@ -959,6 +995,11 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
   ok |= upload(h, S.pk, I.pk);
   ok |= upload(h, S.Mk, I.Mk);
   ok |= upload(h, S.err, err);
+  std::vector<int8_t> est0;  // order-3 flags: Arcane starts est_mixte and est_pure at 0
+  if (S.est) {
+    est0.assign(nc, 0);
+    ok |= upload(h, S.est, est0);
+  }
   ok |= hip_check(hipMemcpyAsync(S.sc, &sc, sizeof(sc), hipMemcpyHostToDevice, h->stream), "upload scalars");
   ok |= hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
   if (ok != MH_OK) {

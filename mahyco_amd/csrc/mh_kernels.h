@@ -764,6 +764,265 @@ __device__ __forceinline__ void k_sweep_cell_body(const MhDev& S, int lb, int us
   }
 }
 
+// ----------------------------------------------------------------------------- order-3 remap
+// Lagrangian centre of the face of direction D whose first node is f0, all three components,
+// summed over the face nodes in Arcane order as computeFaceQuantitesForRemap does
+// (PrepareRemap.cc:18-59): 0 + w x_0 + w x_1 (+ w x_2 + w x_3), w = 1 / nodes.
+template <int DIM, int D>
+__device__ __forceinline__ D3 face_centre(const MhDev& S, long f0) {
+  const long nnx = S.g.nnx, nnxy = (long)S.g.nnx * S.g.nny;
+  double r[3] = {0., 0., 0.};
+  if (DIM == 3) {
+    long nd[4];
+    if (D == 0) { nd[0] = f0; nd[1] = f0 + nnx; nd[2] = f0 + nnx + nnxy; nd[3] = f0 + nnxy; }
+    else if (D == 1) { nd[0] = f0; nd[1] = f0 + 1; nd[2] = f0 + 1 + nnxy; nd[3] = f0 + nnxy; }
+    else { nd[0] = f0; nd[1] = f0 + 1; nd[2] = f0 + 1 + nnx; nd[3] = f0 + nnx; }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      double s = 0.;
+#pragma unroll
+      for (int l = 0; l < 4; ++l) s += 0.25 * S.xlc[q][nd[l]];
+      r[q] = s;
+    }
+  } else {
+    const long n1 = D == 0 ? f0 + nnx : f0 + 1;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      double s = 0.;
+      s += 0.5 * S.xlc[q][f0];
+      s += 0.5 * S.xlc[q][n1];
+      r[q] = s;
+    }
+  }
+  return d3(r[0], r[1], r[2]);
+}
+
+__device__ __forceinline__ double norm3(D3 d) { return sqrt(d.x * d.x + d.y * d.y + d.z * d.z); }
+
+// Cell widths of the order-3 remap, every direction at once (computeGradPhiFace,
+// RemapADIService.cc:119-149): h_D(c) = sum over the inner faces of c normal to D of
+// |x_f - x_c|, with the Lagrangian face and cell centres (full 3-vectors).  The Lagrangian
+// mesh does not move during the remap, so one launch before the first sweep serves all of them.
+template <int DIM>
+__device__ __forceinline__ void k_hcell_body(const MhDev& S, int lb) {
+  const long c = lb * (long)blockDim.x + threadIdx.x;
+  if (c >= S.g.nc) return;
+  if (mh_halted(S.sc)) return;
+  const int nx = S.g.nx, ny = S.g.ny;
+  const int i = (int)(c % nx);
+  const long r = c / nx;
+  const int j = (int)(r % ny);
+  const int k = (int)(r / ny);
+  const long b = i + (long)S.g.nnx * (j + (long)S.g.nny * k);
+  const long nnx = S.g.nnx, nnxy = (long)S.g.nnx * S.g.nny;
+  const D3 xc = d3(cell_centre_d<DIM, 0>(S, b), cell_centre_d<DIM, 1>(S, b),
+                   DIM == 3 ? cell_centre_d<DIM, (DIM == 3 ? 2 : 0)>(S, b) : 0.);
+  const long nc = S.g.nc;
+  {
+    double h = 0.;
+    if (i > 0) h += norm3(face_centre<DIM, 0>(S, b) - xc);
+    if (i < nx - 1) h += norm3(face_centre<DIM, 0>(S, b + 1) - xc);
+    S.hc[c] = h;
+  }
+  {
+    double h = 0.;
+    if (j > 0) h += norm3(face_centre<DIM, 1>(S, b) - xc);
+    if (j < ny - 1) h += norm3(face_centre<DIM, 1>(S, b + nnx) - xc);
+    S.hc[nc + c] = h;
+  }
+  if (DIM == 3) {
+    double h = 0.;
+    if (k > 0) h += norm3(face_centre<DIM, (DIM == 3 ? 2 : 0)>(S, b) - xc);
+    if (k < S.g.nz - 1) h += norm3(face_centre<DIM, (DIM == 3 ? 2 : 0)>(S, b + nnxy) - xc);
+    S.hc[2 * nc + c] = h;
+  }
+}
+
+// Remap-ordre-3.cc:16-112: evaluate_grad, evaluate_ystar, evaluate_fm / evaluate_fp (the
+// piecewise-parabolic reconstruction on a non-uniform 1D stencil).
+__device__ __forceinline__ double o3_grad(double hm, double h0, double hp, double ym, double y0, double yp) {
+  return h0 / (hm + h0 + hp) * ((2. * hm + h0) / (h0 + hp) * (yp - y0) + (h0 + 2. * hp) / (hm + h0) * (y0 - ym));
+}
+__device__ __forceinline__ double o3_ystar(double hmm, double hm, double hp, double hpp, double ymm, double ym,
+                                           double yp, double ypp, double gradm, double gradp) {
+  const double t1 = (2. * hp * hm) / (hm + hp) * ((hmm + hm) / (2. * hm + hp) - (hpp + hp) / (2. * hp + hm)) * (yp - ym);
+  const double t2 = -hm * (hmm + hm) / (2. * hm + hp) * gradp + hp * (hp + hpp) / (hm + 2. * hp) * gradm;
+  return ym + hm / (hm + hp) * (yp - ym) + 1. / (hmm + hm + hp + hpp) * (t1 + t2);
+}
+__device__ __forceinline__ double o3_fm(double x, double dx, double up, double du, double u6) {
+  return up - 0.5 * x / dx * (du - (1. - 2. / 3. * x / dx) * u6);
+}
+__device__ __forceinline__ double o3_fp(double x, double dx, double um, double du, double u6) {
+  return um + 0.5 * x / dx * (du - (1. - 2. / 3. * x / dx) * u6);
+}
+
+// ComputeFluxOrdre3 (Remap-ordre-3.cc:127-189): face value from the six cells y[0..5]
+// around the face (y[2] behind, y[3] ahead) of widths h[0..5], for the displacement
+// vdt = u_face dt, clipped to the interval the upwind cell and its neighbours allow.
+__device__ __forceinline__ double flux_ordre3(const double* y, const double* h, double vdt) {
+  const double gmm = o3_grad(h[0], h[1], h[2], y[0], y[1], y[2]);
+  const double gm = o3_grad(h[1], h[2], h[3], y[1], y[2], y[3]);
+  const double gp = o3_grad(h[2], h[3], h[4], y[2], y[3], y[4]);
+  const double gpp = o3_grad(h[3], h[4], h[5], y[3], y[4], y[5]);
+  const double ysm = o3_ystar(h[0], h[1], h[2], h[3], y[0], y[1], y[2], y[3], gmm, gm);
+  const double ys = o3_ystar(h[1], h[2], h[3], h[4], y[1], y[2], y[3], y[4], gm, gp);
+  const double ysp = o3_ystar(h[2], h[3], h[4], h[5], y[2], y[3], y[4], y[5], gp, gpp);
+  const double ym = y[2], yp = y[3], hm = h[2], hp = h[3];
+  double flux;
+  if (vdt > 0.) flux = o3_fm(vdt, hm, ys, ys - ysm, 6. * (ym - 0.5 * (ysm + ys)));
+  else if (vdt < 0.) flux = o3_fp(-vdt, hp, ys, ysp - ys, 6. * (yp - 0.5 * (ys + ysp)));
+  else return 0.;
+  const double num = vdt / hm, nup = vdt / hp;
+  const double ym_ym = ym + (1. - num) / num * (ym - y[1]);
+  const double yp_ym = yp - (1. + nup) / nup * (yp - y[4]);
+  const double a0 = amin(ym, yp), a1 = amax(ym, yp);
+  double b0, b1;
+  if (vdt >= 0.) {
+    b0 = amin(ym, ym_ym);
+    b1 = amax(ym, ym_ym);
+  } else {
+    b0 = amin(yp, yp_ym);
+    b1 = amax(yp, yp_ym);
+  }
+  double lo, hi;
+  if ((a1 < b0) || (b1 < a0)) {
+    lo = 0.;
+    hi = 0.;
+  } else {
+    lo = amax(a0, b0);
+    hi = amin(a1, b1);
+  }
+  if (flux < lo) flux = lo;
+  if (flux > hi) flux = hi;
+  return flux;
+}
+
+// One ADI direction, cell part, order 3 (computeUpwindFaceQuantitiesForProjection,
+// RemapADIService.cc:319-379, then computeUremap :395-575): the face value of each
+// variable comes from the six cells around the face along D, clamped at the mesh boundary
+// (ComputeFluxOrdre3).  Across a face whose two cells are not pure cells of one
+// environment, the mass and energy face values are rebuilt from the upwind cell's
+// ratios.  The flags of the two cells come from the sweep input (volume fractions > 0,
+// :559-573), or in the first sweep of a cycle from the flags remapVariables left
+// (RemapADIFinal.cc:124-145).  Phi as in k_sweep_cell_body.
+template <int DIM, int D, int M>
+__device__ __forceinline__ void k_sweep_cell3_body(const MhDev& S, int lb, int usel, int first) {
+  const double* __restrict__ Uin = usel ? S.Ub : S.Ua;
+  double* __restrict__ Uout = usel ? S.Ua : S.Ub;
+  constexpr int NV = 3 * M + 1;
+  const long c = lb * (long)blockDim.x + threadIdx.x;
+  if (c >= S.g.nc) return;
+  if (mh_halted(S.sc)) return;
+  const int nx = S.g.nx, ny = S.g.ny;
+  const int i = (int)(c % nx);
+  const long r = c / nx;
+  const int j = (int)(r % ny);
+  const int k = (int)(r / ny);
+  const int a = D == 0 ? i : D == 1 ? j : k;
+  const int na = D == 0 ? nx : D == 1 ? ny : S.g.nz;
+  const long sD = D == 0 ? 1 : D == 1 ? (long)nx : (long)nx * ny;
+  const long nD = D == 0 ? 1 : D == 1 ? (long)S.g.nnx : (long)S.g.nnx * S.g.nny;
+  const long b = i + (long)S.g.nnx * (j + (long)S.g.nny * k);
+  const long nc = S.g.nc;
+  const double dt = S.sc->dt;
+  // stencil: offsets -3..3 along D, clamped to the mesh (index o + 3)
+  long cs[7];
+  double den[7], hs[7];
+#pragma unroll
+  for (int o = 0; o < 7; ++o) {
+    const int ao = min(max(a + o - 3, 0), na - 1);
+    cs[o] = c + (ao - a) * sD;
+    if (first) {
+      den[o] = S.Vl[cs[o]];
+    } else {
+      double sv = 0.;
+      for (int kk = 0; kk < M; ++kk) sv += Uin[kk * nc + cs[o]];
+      den[o] = sv;
+    }
+    hs[o] = S.hc[D * nc + cs[o]];
+  }
+  // the two faces of the cell along D: s = 0 back (cells at offsets -1, 0), s = 1 front (0, +1)
+  bool inner[2];
+  double area[2], vf[2], vdt[2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    inner[s] = s == 0 ? (a > 0) : (a < na - 1);
+    area[s] = 0.;
+    vf[s] = 0.;
+    vdt[s] = 0.;
+    if (!inner[s]) continue;
+    double xf;
+    face_quant<DIM, D>(S, b + s * nD, area[s], xf, vf[s]);
+    vdt[s] = vf[s] * dt;
+  }
+  // flags of the cells at offsets -1, 0, +1
+  int code[3] = {0, 0, 0}, cnt[3] = {0, 0, 0}, pur[3] = {-1, -1, -1};
+  if (first) {
+#pragma unroll
+    for (int o = 0; o < 3; ++o) code[o] = S.est[cs[o + 2]];
+  }
+  bool vpure[2] = {true, true};
+  double fv[2][M], pv[2][M], fm[2][M], pm[2][M];
+  bool cor[2][M];
+  const int s0 = (DIM == 2 && D == 0) ? 1 : 0, s1 = 1 - s0;
+  const double od[2] = {-1., 1.};
+  for (int v = 0; v < NV; ++v) {
+    if (v == M) {
+      if (!first) {
+#pragma unroll
+        for (int o = 0; o < 3; ++o) code[o] = cnt[o] > 1 ? MH_EST_MIXED : pur[o];
+      }
+#pragma unroll
+      for (int s = 0; s < 2; ++s) vpure[s] = code[s] == code[s + 1] && code[s] != MH_EST_MIXED;
+    }
+    double phi[7];
+#pragma unroll
+    for (int o = 0; o < 7; ++o) phi[o] = Uin[v * nc + cs[o]] / den[o];
+    if (v < M && !first) {
+#pragma unroll
+      for (int o = 0; o < 3; ++o)
+        if (phi[o + 2] > 0.) {
+          cnt[o]++;
+          pur[o] = v;
+        }
+    }
+    double flux[2] = {0., 0.};
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      if (!inner[s]) continue;
+      double pf = flux_ordre3(&phi[s], &hs[s], vdt[s]);
+      // upwind cell of the face: offset s-1 (vdt > 0) or s (vdt < 0)
+      const double pu = vdt[s] > 0. ? phi[s + 2] : phi[s + 3];
+      if (v < M) {
+        fv[s][v] = pf;
+        pv[s][v] = pu;
+      } else if (v < 2 * M) {
+        const int e = v - M;
+        cor[s][e] = !vpure[s] && (vdt[s] > 0. || vdt[s] < 0.) && pv[s][e] != 0. && pu != 0.;
+        if (cor[s][e]) pf = (pu / pv[s][e]) * fv[s][e];
+        fm[s][e] = pf;
+        pm[s][e] = pu;
+      } else if (v < 3 * M) {
+        const int e = v - 2 * M;
+        if (cor[s][e]) pf = (pu / pm[s][e]) * fm[s][e];
+      }
+      flux[s] = od[s] * vf[s] * area[s] * dt * pf;
+    }
+    double ff = 0.;
+    if (inner[s0]) ff += flux[s0];
+    if (inner[s1]) ff += flux[s1];
+    double un = Uin[v * nc + c] - ff;
+    if (v >= M && v < 2 * M) {
+      double dd = 0.;
+      if (inner[s0]) dd += 0.5 * flux[s0] * od[s0];
+      if (inner[s1]) dd += 0.5 * flux[s1] * od[s1];
+      S.dflux[(v - M) * nc + c] = dd;
+    }
+    if (v >= M && v < 3 * M && un < 0.) un = 0.;  // computeUremap clamps, :465-485
+    Uout[v * nc + c] = un;
+  }
+}
+
 
 // The x sweep with one wavefront per row segment: lane l holds cell i = 60 s - 2 + l of a
 // row, so the stencil neighbours along x are the neighbouring lanes.  Phi, the gradient of
@@ -1363,6 +1622,15 @@ __device__ __forceinline__ void k_finalize_body(const MhDev& S, int lb, int usel
     if (somme_frac != 0.) unsursomme = 1. / somme_frac;
     for (int kk = 0; kk < M; ++kk)
       if ((nm >> kk) & 1) f[kk] *= unsursomme;
+    if (S.est) {  // mixed-cell flag for the order-3 remap of the next cycle (:124-145)
+      int cnt = 0, pu = -1;
+      for (int kk = 0; kk < M; ++kk)
+        if (f[kk] > 0.) {
+          cnt++;
+          pu = kk;
+        }
+      S.est[c] = (int8_t)(cnt > 1 ? MH_EST_MIXED : pu);
+    }
     if (masset > thr * thr) {
       const double unsurmasset = 1. / masset;
       double fm = 0.;
@@ -1758,6 +2026,9 @@ MH_WRAP(k_sweep_cell_tile, MH_TILE_MIN_WAVES,
         MH_COMMA int usel MH_COMMA int first, MH_COMMA usel MH_COMMA first)
 MH_WRAP(k_sweep_node_row, 1, template <int DIM MH_COMMA int LIM>, <DIM MH_COMMA LIM>,
         MH_COMMA int psel MH_COMMA int first MH_COMMA int last, MH_COMMA psel MH_COMMA first MH_COMMA last)
+MH_WRAP(k_hcell, 1, template <int DIM>, <DIM>, , )
+MH_WRAP(k_sweep_cell3, 1, template <int DIM MH_COMMA int D MH_COMMA int M>, <DIM MH_COMMA D MH_COMMA M>,
+        MH_COMMA int usel MH_COMMA int first, MH_COMMA usel MH_COMMA first)
 
 __global__ __launch_bounds__(MH_BLOCK, MH_NODE_MIN_WAVES) void k_node_lag3_tiled(MhDev S) {
   k_node_lag3_tiled_body(S, blockIdx.x, blockIdx.y, blockIdx.z);

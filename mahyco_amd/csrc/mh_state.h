@@ -17,6 +17,8 @@
 
 #include "../../include/mahyco_types.h"
 
+#define MH_EST_MIXED 99  // MhDev::est of a cell with more than one positive volume fraction
+
 struct MhGrid {
   int dim, nx, ny, nz;  // nz = 1 in 2D
   int nnx, nny, nnz;    // node counts (nnz = 1 in 2D)
@@ -75,6 +77,11 @@ struct MhDev {
   // remap buffers: U ping-pong [nv][nc], mass fluxes [m][nc], dual state [4][nn] (in
   // place), dual phi ping-pong [3][nn]
   double *Ua, *Ub, *dflux, *Ud, *Pa, *Pb;
+  // order-3 remap only (null otherwise): cell widths per direction [dim][nc], and the
+  // mixed-cell flag remapVariables leaves per cell (MH_EST_MIXED, else the index of the one
+  // environment with a positive volume fraction, -1 for none)
+  double* hc;
+  int8_t* est;
   int nblk_c, nblk_n;  // blocks of MH_BLOCK threads over cells / nodes
   int max_iter;        // > 0: stop after this many iterations (fixed-horizon runs)
   int hist_period;     // > 0: EnvSummation rows every hist_period iterations

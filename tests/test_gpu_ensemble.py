@@ -10,7 +10,7 @@ from mahyco_amd import types as T
 pytestmark = pytest.mark.gpu
 
 
-def _mix(seed, n=10, stiff=False, dim3=False):
+def _mix(seed, n=10, stiff=False, dim3=False, o3=False):
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
@@ -20,6 +20,8 @@ def _mix(seed, n=10, stiff=False, dim3=False):
         else:
             nx = int(np.exp(rng.uniform(np.log(16), np.log(48))))
             c = cases.riemann2d(nx, nx, rng, stiffened=stiff, name=f"m{i}")
+        if o3 and i % 3 == 0:
+            c.ordre_projection = 3  # order-3 remap: a batch group of its own
         out.append(c)
     return out
 
@@ -36,11 +38,12 @@ def _alone(case, ncycles):
     return res
 
 
-@pytest.mark.parametrize("graph,stiff,dim3", [(True, False, False), (False, True, True)])
-def test_alone_equals_mixed(graph, stiff, dim3):
+@pytest.mark.parametrize("graph,stiff,dim3,o3", [(True, False, False, False), (False, True, True, False),
+                                                (True, True, True, True)])
+def test_alone_equals_mixed(graph, stiff, dim3, o3):
     from mahyco_amd.lib import Ensemble
-    print(f"[ensemble] graph={graph} stiff={stiff} dim3={dim3}", flush=True)
-    mix = _mix(3 if graph else 4, stiff=stiff, dim3=dim3)
+    print(f"[ensemble] graph={graph} stiff={stiff} dim3={dim3} o3={o3}", flush=True)
+    mix = _mix((3 if graph else 4) + (2 if o3 else 0), stiff=stiff, dim3=dim3, o3=o3)
     ncyc = 30
     E = Ensemble(16, 4e9)
     slots = [E.add(c, max_iterations=ncyc) for c in mix]

@@ -42,6 +42,14 @@ CASES = {
     "riemann2d_perfect_gas": (lambda: _riemann2d(False, 11), 150),
     "riemann2d_stiffened_gas": (lambda: _riemann2d(True, 12), 150),
     "riemann3d_stiffened_gas": (lambda: _riemann3d(13), 60),
+    # order-3 remap (Remap-ordre-3.cc): the reference's two- and one-material tubes, a 3D
+    # tube in all directions, and Riemann problems with mixed cells (mass / energy face
+    # values rebuilt from the upwind ratios)
+    "bisod_x_ordre3_fixture": (lambda: cases.ordre3(cases.bisod_x, "bisod_x_o3"), 400),
+    "sod_z_ordre3_fixture": (lambda: cases.ordre3(lambda: cases.sod_mono("Z"), "sod_z_o3"), 120),
+    "bisod3d_ordre3_32x16x16": (lambda: cases.ordre3(lambda: cases.config2_bisod3d(32, 16, 16), "b3d_o3"), 100),
+    "riemann2d_stiffened_ordre3": (lambda: cases.ordre3(lambda: _riemann2d(True, 14), "r2d_o3"), 150),
+    "riemann3d_stiffened_ordre3": (lambda: cases.ordre3(lambda: _riemann3d(15), "r3d_o3"), 60),
 }
 
 

@@ -180,7 +180,8 @@ def test_node_kernel_tiled_equals_flat(monkeypatch, shape):
 @pytest.mark.parametrize("make", [lambda: cases.config2_bisod3d(13, 37, 21), lambda: cases.config2_bisod3d(9, 2, 40),
                                   lambda: cases.config2_bisod3d(7, 17, 3), lambda: cases.config2_bisod3d(130, 5, 4),
                                   lambda: cases.config2_bisod3d(70, 35, 18), lambda: cases.bitriplepoint(),
-                                  lambda: cases.sedov_2d()])
+                                  lambda: cases.sedov_2d(),
+                                  lambda: cases.ordre3(lambda: cases.config2_bisod3d(70, 35, 18), "b3d_o3")])
 def test_sweep_rows_and_tiles_equal_flat(monkeypatch, make):
     """The row-segment x sweeps (cell and dual parts) and the LDS-tiled y/z cell sweeps
     give the same bits as the one-thread-per-item sweeps (sizes cover partial tiles and row

@@ -194,6 +194,42 @@ def lag2d_sod_x() -> Case:
                 deltat_init=1e-5, deltat_min=1e-8, deltat_max=1e-4, final_time=.2, bcs=list(WALLS_2D))
 
 
+def lag_bisod(axis) -> Case:
+    """NONREGRESSION/CAS_Lag_BiSodCase{X,Y,Z}/Donnees.arc (3D Lagrange only, two materials)."""
+    c = {"X": bisod_x, "Y": bisod_y, "Z": bisod_z}[axis]()
+    c.name, c.with_projection, c.deltat_max = "CAS_Lag_BiSodCase" + axis, False, 1e-4
+    return c
+
+
+def lag_sod(axis) -> Case:
+    """NONREGRESSION/CAS_Lag_SodCase{X,Y,Z}/Donnees.arc (3D Lagrange only, mono Sod)."""
+    c = sod_mono(axis)
+    c.name, c.with_projection, c.deltat_max = "CAS_Lag_SodCase" + axis, False, 1e-4
+    return c
+
+
+def lag2d(axis, bi) -> Case:
+    """NONREGRESSION/CAS_2D_Lag_{Bi,}SodCase{X,Y,Z}/Donnees.arc: Lagrange only on a 2D mesh
+    along X or Y (100x5, 5x100), and on a 5x1x100 3D mesh for Z."""
+    t = {"X": 3, "Y": 4, "Z": 5}[axis] + (10 if bi else 0)
+    name = "CAS_2D_Lag_" + ("BiSod" if bi else "Sod") + "Case" + axis
+    if axis == "Z":
+        dim, n, ln = 3, (5, 1, 100), (.1, .1, 1.)
+    else:
+        dim, n, ln = 2, ((100, 5, 1) if axis == "X" else (5, 100, 1)), ((1., .1, 0.) if axis == "X" else (.1, 1., 0.))
+    return Case(name, dim, n, (0., 0., 0.), ln, [_pg(1., 1.), _pg(.125, .1)], T.MH_CASE_SOD, t,
+                with_projection=False, deltat_init=1e-5, deltat_min=1e-8, deltat_max=1e-4, final_time=.2,
+                bcs=list(WALLS_3D))
+
+
+def sedov_3d_fixture() -> Case:
+    """NONREGRESSION/CAS_Sedov/Donnees.arc (3D ADI remap, 60^3 on [0,1.2]^3, to t = 1e-3)."""
+    return Case("CAS_Sedov", 3, (60, 60, 60), (0., 0., 0.), (1.2, 1.2, 1.2),
+                [_pg(1., 0.0979264e-5), _pg(1., 1.)], T.MH_CASE_SEDOV, 1,
+                deltat_init=1e-5, deltat_min=1e-8, deltat_max=0.01, final_time=1e-3,
+                bcs=[("XMIN", "Vx", 0.), ("YMIN", "Vy", 0.), ("ZMIN", "Vz", 0.)])
+
+
 def sedov_2d() -> Case:
     """NONREGRESSION/CAS_2D_Sedov/Donnees.arc (2D ADI remap, 60x60 on [0,1.2]^2)."""
     return Case("CAS_2D_Sedov", 2, (60, 60, 1), (0., 0., 0.), (1.2, 1.2, 0.),

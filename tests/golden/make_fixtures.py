@@ -42,7 +42,25 @@ CASES = {
     "CAS_SodCaseX_ordre_3": (3, (100, 5, 5), (0, 0, 0), (1., .1, .1)),
     "CAS_SodCaseY_ordre_3": (3, (5, 100, 5), (0, 0, 0), (.1, 1., .1)),
     "CAS_SodCaseZ_ordre_3": (3, (5, 5, 100), (0, 0, 0), (.1, .1, 1.)),
+    "CAS_Lag_BiSodCaseY": (3, (5, 100, 5), (0, 0, 0), (.1, 1., .1)),
+    "CAS_Lag_BiSodCaseZ": (3, (5, 5, 100), (0, 0, 0), (.1, .1, 1.)),
+    "CAS_Lag_SodCaseX": (3, (100, 5, 5), (0, 0, 0), (1., .1, .1)),
+    "CAS_Lag_SodCaseY": (3, (5, 100, 5), (0, 0, 0), (.1, 1., .1)),
+    "CAS_Lag_SodCaseZ": (3, (5, 5, 100), (0, 0, 0), (.1, .1, 1.)),
+    "CAS_2D_Lag_BiSodCaseX": (2, (100, 5, 1), (0, 0, 0), (1., .1, 0.)),
+    "CAS_2D_Lag_BiSodCaseY": (2, (5, 100, 1), (0, 0, 0), (.1, 1., 0.)),
+    "CAS_2D_Lag_SodCaseY": (2, (5, 100, 1), (0, 0, 0), (.1, 1., 0.)),
+    "CAS_2D_Lag_SodCaseZ": (3, (5, 1, 100), (0, 0, 0), (.1, .1, 1.)),
+    "CAS_Sedov": (3, (60, 60, 60), (0, 0, 0), (1.2, 1.2, 1.2)),
 }
+# Runs whose output holds no geometry: parts are the subdomains of the Cartesian generator
+# in rank order (x fastest), each with its cells in local order (x fastest); their node
+# fields are not kept (a part's node order is not recoverable without its geometry).
+NSD_BLOCKS = {"CAS_Sedov": (2, 2, 2)}
+# Lagrange-only runs on several subdomains: the final mesh has moved along the tube axis,
+# so cells and nodes are placed by their rank along that axis within each line.
+MOVING_AXIS = {"CAS_Lag_BiSodCaseZ": 2, "CAS_Lag_SodCaseX": 0, "CAS_2D_Lag_BiSodCaseX": 0,
+               "CAS_2D_Lag_BiSodCaseY": 1, "CAS_2D_Lag_SodCaseY": 1}
 TH_VARS = ["CellMass", "Pressure", "Density", "InternalEnergy"]
 CELL_VARS = ["Density", "Pressure", "InternalEnergy", "CellMass", "PseudoViscosity", "Materiau"]
 
@@ -88,6 +106,57 @@ def _read_var(path, nper, ncomp=1):
     return out
 
 
+def _block_maps(nsd, n):
+    """Global cell / node ids of the parts of a 3D run on nsd subdomains without geometry."""
+    cell_maps, node_maps = [], []
+    b = [n[a] // nsd[a] for a in range(3)]
+    for r in range(nsd[0] * nsd[1] * nsd[2]):
+        p = (r % nsd[0], (r // nsd[0]) % nsd[1], r // (nsd[0] * nsd[1]))
+        k, j, i = np.meshgrid(*[np.arange(p[a] * b[a], (p[a] + 1) * b[a]) for a in (2, 1, 0)], indexing="ij")
+        cell_maps.append((i + n[0] * (j + n[1] * k)).ravel())
+        k, j, i = np.meshgrid(*[np.arange(p[a] * b[a], (p[a] + 1) * b[a] + 1) for a in (2, 1, 0)], indexing="ij")
+        node_maps.append((i + (n[0] + 1) * (j + (n[1] + 1) * k)).ravel())
+    return cell_maps, node_maps
+
+
+def _ranked_maps(parts, ax, dim, n, origin, dx):
+    """Global cell / node ids of every part when the mesh moved along axis ax only: the
+    other coordinates give the line, the rank of the coordinate along ax within the line
+    (over all parts) gives the index along it."""
+    nx, ny, nz = n[0], n[1], (n[2] if dim == 3 else 1)
+    nn_ = [nx + 1, ny + 1, nz + 1]
+
+    def ids(points, cells):
+        pts = np.concatenate(points)
+        owner = np.concatenate([np.full(len(p), q) for q, p in enumerate(points)])
+        other = [a for a in range(dim) if a != ax]
+        idx = np.zeros((len(pts), 3), dtype=int)
+        for a in other:
+            idx[:, a] = (np.floor((pts[:, a] - origin[a]) / dx[a]) if cells
+                         else np.rint((pts[:, a] - origin[a]) / dx[a])).astype(int)
+        key = idx[:, other[0]] + (10 ** 6) * (idx[:, other[1]] if len(other) > 1 else 0)
+        order = np.lexsort((pts[:, ax], key))
+        rank = np.empty(len(pts), dtype=int)
+        sk = key[order]
+        start = np.r_[0, np.flatnonzero(np.diff(sk)) + 1]
+        line_start = np.repeat(start, np.diff(np.r_[start, len(sk)]))
+        rank[order] = np.arange(len(pts)) - line_start
+        idx[:, ax] = rank
+        dims = [nx, ny, nz] if cells else nn_
+        gid = idx[:, 0] + dims[0] * idx[:, 1] + (dims[0] * dims[1] * idx[:, 2] if dim == 3 else 0)
+        return [gid[owner == q] for q in range(len(points))]
+
+    cen = [xyz[conn].mean(axis=1) for xyz, conn, _ in parts]
+    cell_maps = ids(cen, True)
+    # nodes shared by two subdomains appear in both parts: rank the distinct positions
+    allxyz = np.concatenate([xyz for xyz, _, _ in parts])
+    uniq, inv = np.unique(np.round(allxyz, 12), axis=0, return_inverse=True)
+    gu = ids([uniq], False)[0]
+    sizes = np.cumsum([0] + [len(xyz) for xyz, _, _ in parts])
+    node_maps = [gu[inv.ravel()[sizes[q]:sizes[q + 1]]] for q in range(len(parts))]
+    return cell_maps, node_maps
+
+
 def convert(root, case, outdir):
     dim, n, origin, length = CASES[case]
     nx, ny, nz = n
@@ -110,13 +179,20 @@ def convert(root, case, outdir):
             a = np.atleast_2d(np.loadtxt(fn))
             res[f"th_{v}_{k}"] = a[:, 1]
             times = a[:, 0]
-    res["th_time"] = times
+    res["th_time"] = times if times is not None else np.zeros(0)  # some runs wrote no history
     # geometry -> global index maps
-    geo_dir = os.path.join(dep, "ensight.geo")
-    parts = _read_geo(os.path.join(geo_dir, sorted(os.listdir(geo_dir))[-1]))
     dx = [length[a] / n[a] if n[a] else 1. for a in range(3)]
-    cell_maps, node_maps = [], []
-    single = len(parts) == 1  # sequential run: one part in global order (Lagrangian meshes move)
+    geo_dir = os.path.join(dep, "ensight.geo")
+    if case in NSD_BLOCKS:
+        cell_maps, node_maps = _block_maps(NSD_BLOCKS[case], (nx, ny, nz))
+        parts = []
+    else:
+        parts = _read_geo(os.path.join(geo_dir, sorted(os.listdir(geo_dir))[-1]))
+        cell_maps, node_maps = [], []
+    single = len(parts) == 1 and not cell_maps  # sequential run: one part in global order (Lagrangian meshes move)
+    if not single and case in MOVING_AXIS:
+        cell_maps, node_maps = _ranked_maps(parts, MOVING_AXIS[case], dim, n, origin, dx)
+        parts = []
     for xyz, conn, _ in parts:
         if single:
             cell_maps.append(np.arange(len(conn)))
@@ -139,6 +215,9 @@ def convert(root, case, outdir):
             g[cm] = pv
         assert not np.isnan(g).any(), (case, v)
         res[f"cell_{v}"] = g
+    if case in NSD_BLOCKS:  # node order within a part is not recoverable without geometry
+        np.savez_compressed(os.path.join(outdir, case + ".npz"), **res)
+        return
     vd = os.path.join(dep, "NodeMass")
     vals = _read_var(os.path.join(vd, sorted(os.listdir(vd))[-1]), [len(m) for m in node_maps])
     g = np.full(nn, np.nan)
@@ -146,6 +225,9 @@ def convert(root, case, outdir):
         g[nm] = pv
     res["node_NodeMass"] = g
     vd = os.path.join(dep, "Velocity")
+    if not os.path.isdir(vd):
+        np.savez_compressed(os.path.join(outdir, case + ".npz"), **res)
+        return
     vals = _read_var(os.path.join(vd, sorted(os.listdir(vd))[-1]), [len(m) for m in node_maps], 3)
     for c in range(3):
         g = np.full(nn, np.nan)
@@ -158,6 +240,6 @@ def convert(root, case, outdir):
 if __name__ == "__main__":
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/NONREGRESSION"
     outdir = os.path.dirname(os.path.abspath(__file__))
-    for c in CASES:
+    for c in sys.argv[2:] or CASES:
         convert(root, c, outdir)
         print("wrote", c)

@@ -43,9 +43,19 @@ enum MhSurface { MH_XMIN = 0, MH_XMAX = 1, MH_YMIN = 2, MH_YMAX = 3, MH_ZMIN = 4
 /* TypesMahyco::eBoundaryCondition (TypesMahyco.h:16-31), velocity subset. */
 enum MhBcType { MH_BC_VX = 0, MH_BC_VY = 1, MH_BC_VZ = 2 };
 
-/* TypesMahyco limiteur (TypesMahyco.h:96-105), classical subset.  Any other id is
- * first order (fluxLimiter returns 0, UtilesRemap.cc:108-120). */
-enum MhLimiter { MH_MINMOD = 0, MH_SUPERBEE = 1, MH_VANLEER = 2 };
+/* TypesMahyco limiteur (TypesMahyco.h:96-105): classical limiters (fluxLimiter,
+ * UtilesRemap.cc:108-120) and the generalised ones for unequal cell widths (fluxLimiterG,
+ * UtilesRemap.cc:130-183). */
+enum MhLimiter {
+  MH_MINMOD = 0,
+  MH_SUPERBEE = 1,
+  MH_VANLEER = 2,
+  MH_MINMOD_G = 3,
+  MH_SUPERBEE_G = 4,
+  MH_VANLEER_G = 5,
+  MH_ARITHMETIC_G = 6,
+  MH_ULTRABEE_G = 7
+};
 
 /* Case models (IInitialisations services). */
 enum MhCaseModel {
@@ -97,7 +107,7 @@ typedef struct MhOptions {
   int nb_bc;
   MhBoundaryCondition bc[MH_MAX_BC];
   /* remap (RemapADI.axl) */
-  int ordre_projection;       /* 1 or 2 */
+  int ordre_projection;       /* 1, 2 or 3 */
   int projection_limiteur_id; /* MhLimiter */
   double remap_threshold;     /* threshold, default 1e-10 */
   /* case model */
@@ -106,6 +116,12 @@ typedef struct MhOptions {
   double interface_position;/* MH_CASE_RIEMANN: interface coordinate */
   int interface_axis;       /* MH_CASE_RIEMANN: 0,1,2 */
   double sedov_deposit;     /* SEDOV total energy deposit (reference constant 0.244816) */
+  /* remap variants (RemapADI.axl:181-205; all default 0) */
+  int projection_pente_borne;      /* projection-pente-borne: fluxes integrated over the cell (order 2) */
+  int pente_borne_debar_fix;       /* projection-pente-borne-debar-fix: 0, 1 (mean value), 2 (cell value) */
+  int pente_borne_mixte;           /* projection-pente-borne-mixte: pure-neighbourhood variant */
+  int projection_limiteur_pure_id; /* projection-limiteur-pure-id (MhLimiter) */
+  int projection_simple_pente;     /* projection-simple-pente */
 } MhOptions;
 
 /* Field identifiers for get/set.  Cell fields take an environment index

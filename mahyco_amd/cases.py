@@ -66,6 +66,11 @@ class Case:
     sedov_deposit: float = 0.244816
     coord_rule: int = T.MH_COORD_DIV
     max_cycles: Optional[int] = None
+    pente_borne: bool = False
+    debar_fix: int = 0
+    pente_borne_mixte: bool = False
+    limiter_pure: int = T.MH_MINMOD
+    simple_pente: bool = False
 
     @property
     def nb_cells(self) -> int:
@@ -126,6 +131,11 @@ class Case:
         o.interface_position = self.interface_position
         o.interface_axis = self.interface_axis
         o.sedov_deposit = self.sedov_deposit
+        o.projection_pente_borne = int(self.pente_borne)
+        o.pente_borne_debar_fix = self.debar_fix
+        o.pente_borne_mixte = int(self.pente_borne_mixte)
+        o.projection_limiteur_pure_id = self.limiter_pure
+        o.projection_simple_pente = int(self.simple_pente)
         return o
 
 
@@ -194,6 +204,14 @@ def lag2d_sod_x() -> Case:
                 deltat_init=1e-5, deltat_min=1e-8, deltat_max=1e-4, final_time=.2, bcs=list(WALLS_2D))
 
 
+def pente_borne(make, name, debar, limiter=T.MH_MINMOD) -> Case:
+    """The NONREGRESSION/CAS_BiSodCase*_pente_borne_* variants: fluxes integrated over the cell
+    (projection-pente-borne) with the given debar fix and limiter."""
+    c = make()
+    c.name, c.pente_borne, c.debar_fix, c.limiter = name, True, debar, limiter
+    return c
+
+
 def lag_bisod(axis) -> Case:
     """NONREGRESSION/CAS_Lag_BiSodCase{X,Y,Z}/Donnees.arc (3D Lagrange only, two materials)."""
     c = {"X": bisod_x, "Y": bisod_y, "Z": bisod_z}[axis]()
@@ -245,6 +263,18 @@ def bitriplepoint() -> Case:
                 threshold=1e-16, deltat_init=1e-9, deltat_min=1e-10, deltat_max=1e-5,
                 final_time=.041, limiter=T.MH_VANLEER,
                 bcs=[("XMIN", "Vx", 0.), ("XMAX", "Vx", 0.), ("YMIN", "Vy", 0.), ("YMAX", "Vy", 0.)])
+
+
+def mono_pointtriple(simple_pente=False) -> Case:
+    """NONREGRESSION/CAS_MonoPointtriple_pb{,_sp}_mixte_superbeeG_superbeeG/Donnees.arc: the
+    triple point with pente-borne fluxes, the pure-neighbourhood variant, debar fix 1 and the
+    generalised superbee limiter (optionally in simple-pente form), remap threshold 1e-16."""
+    c = bitriplepoint()
+    c.name = "CAS_MonoPointtriple_pb_" + ("sp_" if simple_pente else "") + "mixte_superbeeG_superbeeG"
+    c.pente_borne, c.debar_fix, c.pente_borne_mixte, c.simple_pente = True, 1, True, simple_pente
+    c.limiter = c.limiter_pure = T.MH_SUPERBEE_G
+    c.remap_threshold = 1e-16
+    return c
 
 
 # ----------------------------------------------------------------------------- workloads

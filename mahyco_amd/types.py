@@ -32,6 +32,7 @@ MH_BC_VX, MH_BC_VY, MH_BC_VZ = range(3)
 BC_TYPES = {"Vx": MH_BC_VX, "Vy": MH_BC_VY, "Vz": MH_BC_VZ}
 
 MH_MINMOD, MH_SUPERBEE, MH_VANLEER = 0, 1, 2
+MH_MINMOD_G, MH_SUPERBEE_G, MH_VANLEER_G, MH_ARITHMETIC_G, MH_ULTRABEE_G = 3, 4, 5, 6, 7
 
 MH_CASE_SOD, MH_CASE_SEDOV, MH_CASE_RIEMANN, MH_CASE_POINTTRIPLE = range(4)
 MH_COORD_MUL, MH_COORD_DIV = 0, 1
@@ -85,4 +86,7 @@ class MhOptions(C.Structure):
                 ("bc", MhBoundaryCondition * MH_MAX_BC), ("ordre_projection", C.c_int),
                 ("projection_limiteur_id", C.c_int), ("remap_threshold", C.c_double),
                 ("cas_model", C.c_int), ("cas_test", C.c_int), ("interface_position", C.c_double),
-                ("interface_axis", C.c_int), ("sedov_deposit", C.c_double)]
+                ("interface_axis", C.c_int), ("sedov_deposit", C.c_double),
+                ("projection_pente_borne", C.c_int), ("pente_borne_debar_fix", C.c_int),
+                ("pente_borne_mixte", C.c_int), ("projection_limiteur_pure_id", C.c_int),
+                ("projection_simple_pente", C.c_int)]

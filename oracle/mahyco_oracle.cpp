@@ -118,8 +118,9 @@ struct OracleCase {
   std::vector<double> u_lagrange, phi_lagrange, grad_phi, grad_phi_face, phi_face, dual_phi_flux;
   std::vector<double> u_dual, phi_dual, dual_grad_phi;  // [nn][5]
   std::vector<double> deltax_lagrange, face_normal_velocity;
-  std::vector<double> h_cell_lagrange;       // cell width along the sweep (order 3)
-  std::vector<int> est_mixte, est_pure;      // mixed-cell indicators (order 3); Arcane starts them at 0
+  std::vector<double> h_cell_lagrange;       // cell width along the sweep
+  std::vector<int> est_mixte, est_pure;      // mixed-cell indicators; Arcane starts them at 0
+  std::vector<double> dphi_ar, dphi_av;      // pente-borne fluxes through the back / front face [nc][nv]
   std::vector<R3> face_length_lagrange;
   std::vector<double> back_flux_mass, front_flux_mass;
   int sens_projection = 0;
@@ -924,7 +925,15 @@ void OracleCase::computeVariablesForRemap() {
       U[m + k] = V * rho;
       U[2 * m + k] = V * rho * ev(internal_energy, k, c);
       U[3 * m] = cell_volume.g[c] * pseudo.g[c];
-      for (int v = 0; v < nv; ++v) phi_lagrange[c * nv + v] = U[v] / cell_volume.g[c];
+      if (opt.projection_pente_borne) {  // :138-155: the environment's own values
+        double* P = &phi_lagrange[c * nv];
+        P[k] = ev(fracvol, k, c);
+        P[m + k] = rho;
+        P[2 * m + k] = ev(internal_energy, k, c);
+        P[3 * m] = pseudo.g[c];
+      } else {
+        for (int v = 0; v < nv; ++v) phi_lagrange[c * nv + v] = U[v] / cell_volume.g[c];
+      }
     }
   }
 #pragma omp parallel for schedule(static)
@@ -973,9 +982,8 @@ void OracleCase::computeFaceQuantitesForRemap() {
 }
 
 // RemapADIService::computeGradPhiFace, RemapADIService.cc:119-149 (inner faces only).
-// h_cell (the sum over the cell's inner faces of the sweep of |x_f - x_c|) is formed
-// for order 3, its only user here; with at most two terms per cell starting from 0 the
-// face order of the sum does not matter.
+// h_cell is the sum over the cell's inner faces of the sweep of |x_f - x_c|; with at most
+// two terms per cell starting from 0 the face order of the sum does not matter.
 void OracleCase::computeGradPhiFace(int idir) {
 #pragma omp parallel for schedule(static)
   for (long f = face_off[idir]; f < face_off[idir] + face_cnt[idir]; ++f) {
@@ -985,7 +993,7 @@ void OracleCase::computeGradPhiFace(int idir) {
     for (int v = 0; v < nv; ++v)
       grad_phi_face[f * nv + v] = (phi_lagrange[(long)cf * nv + v] - phi_lagrange[(long)cb * nv + v]) / deltax_lagrange[f];
   }
-  if (opt.ordre_projection == 3) {
+  {
 #pragma omp parallel for schedule(static)
     for (long c = 0; c < nc; ++c) {
       double h = 0.;
@@ -1082,27 +1090,225 @@ static inline double fluxLimiter(int id, double r) {
   return 0.0;
 }
 
-// RemapADIService::computeGradPhiCell :161-267 + computeAndLimitGradPhi UtilesRemap.cc:325-357
-// (classical limiters; pente-borne off).
+// RemapADIService::fluxLimiterG, UtilesRemap.cc:130-183: limited slope for unequal widths
+// h0 (cell), hplus / hmoins (front / back neighbour); ids MH_MINMOD_G..MH_ULTRABEE_G.
+static double fluxLimiterG(int id, double gradplus, double gradmoins, double y0, double yplus, double ymoins,
+                           double h0, double hplus, double hmoins, bool simple_pente) {
+  (void)y0;
+  double grady = 0.;
+  if (gradplus * gradmoins < 0.0) return 0.;
+  if (id == MH_MINMOD_G) {
+    if ((yplus - ymoins) > 0.) grady = std::min(std::fabs(gradplus), std::fabs(gradmoins));
+    else grady = -std::min(std::fabs(gradplus), std::fabs(gradmoins));
+  } else if (id == MH_SUPERBEE_G) {
+    if ((yplus - ymoins) > 0.) grady = std::max(std::fabs(gradplus), std::fabs(gradmoins));
+    else grady = -std::max(std::fabs(gradplus), std::fabs(gradmoins));
+  } else if (id == MH_VANLEER_G) {
+    const double lambdaplus = (h0 / 2. + hplus) / (h0 + hplus + hmoins);
+    const double lambdamoins = (h0 / 2. + hmoins) / (h0 + hplus + hmoins);
+    if ((lambdaplus * gradplus + lambdamoins * gradmoins) != 0.)
+      grady = gradplus * gradmoins / (lambdaplus * gradplus + lambdamoins * gradmoins);
+    else
+      grady = 0.;
+  } else if (id == MH_ULTRABEE_G) {
+    grady = (yplus - ymoins) / h0;
+  } else if (id == MH_ARITHMETIC_G) {
+    const double lambdaplus = (h0 / 2. + hplus) / (h0 + hplus + hmoins);
+    const double lambdamoins = (h0 / 2. + hmoins) / (h0 + hplus + hmoins);
+    grady = lambdamoins * gradplus + lambdaplus * gradmoins;
+  }
+  if (simple_pente) {  // :166-180
+    const double gradMplus = gradplus * (h0 + hplus) / h0;
+    const double gradMmoins = gradmoins * (h0 + hmoins) / h0;
+    const double gradM = std::min(std::fabs(gradMplus), std::fabs(gradMmoins));
+    double gradyl;
+    if ((yplus - ymoins) > 0.) gradyl = std::min(std::fabs(gradM), std::fabs(grady));
+    else gradyl = -std::min(std::fabs(gradM), std::fabs(grady));
+    grady = gradyl;
+  }
+  return grady;
+}
+
+// computeAndLimitGradPhi, UtilesRemap.cc:325-357: classical limiters from the two face
+// gradients, generalised ones (id >= MH_MINMOD_G) from the three cell values and widths.
+static double limitedGrad(int id, double gb, double gf, double y0, double yplus, double ymoins, double h0,
+                          double hplus, double hmoins, bool simple_pente) {
+  if (id < MH_MINMOD_G) {
+    double g = 0.;
+    if (gb != 0.) g += 0.5 * (fluxLimiter(id, gf / gb) * gb);
+    if (gf != 0.) g += 0.5 * (fluxLimiter(id, gb / gf) * gf);
+    return g;
+  }
+  return fluxLimiterG(id, gf, gb, y0, yplus, ymoins, h0, hplus, hmoins, simple_pente);
+}
+
+// ---------------------------------------------------------------- pente-borne (UtilesRemap.cc)
+// `abs` of a Real in computeFluxPP / INTY / computexgxd.
+static inline double ref_abs(double x) { return std::fabs(x); }
+
+// INTY, UtilesRemap.cc:86-97: integral from x0 to X of the segment (x0, y0)-(x1, y1).
+static inline double inty(double X, double x0, double y0, double x1, double y1, double thr) {
+  double flux = 0.;
+  if (ref_abs(x1 - x0) > thr) {
+    const double eta = amin(amax(0., (X - x0) / (x1 - x0)), 1.);
+    flux = (y0 + 0.5 * eta * (y1 - y0)) * (x1 - x0) * eta;
+  }
+  return flux;
+}
+
+// computeY0, UtilesRemap.cc:194-258: monotonicity thresholds y0plus (type 0), y0moins (type 1).
+static double computeY0(int id, double y0, double yplus, double ymoins, double h0, double hplus, double hmoins,
+                        int type, bool simple_pente) {
+  (void)y0;
+  double y0plus = 0., y0moins = 0.;
+  if (id == MH_MINMOD_G || id == MH_MINMOD) {
+    y0plus = yplus;
+    y0moins = ymoins;
+  } else if (id == MH_SUPERBEE_G || id == MH_SUPERBEE) {
+    y0plus = ((h0 + hmoins) * yplus + h0 * ymoins) / (2 * h0 + hmoins);
+    y0moins = ((h0 + hplus) * ymoins + h0 * yplus) / (2 * h0 + hplus);
+  } else if (id == MH_VANLEER_G || id == MH_VANLEER) {
+    const double a = amin(yplus, ymoins);
+    const double b = amax(yplus, ymoins);
+    double xplus = (h0 * h0 + 3 * h0 * hmoins + 2 * hmoins * hmoins) * yplus;
+    double xmoins = (h0 * h0 + 3 * h0 * hplus + 2 * hplus * hplus) * ymoins;
+    xplus += (h0 * h0 - h0 * hplus - 2 * hplus * hplus + 2 * h0 * hmoins) * ymoins;
+    xmoins += (h0 * h0 - h0 * hmoins - 2 * hmoins * hmoins + 2 * h0 * hplus) * yplus;
+    xplus /= (2 * h0 * h0 + 5 * h0 * hmoins + 2 * hmoins * hmoins - h0 * hplus - 2 * hplus * hplus);
+    xmoins /= (2 * h0 * h0 + 5 * h0 * hplus + 2 * hplus * hplus - h0 * hmoins - 2 * hmoins * hmoins);
+    y0plus = amin(amax(xplus, a), b);
+    y0moins = amin(amax(xmoins, a), b);
+  } else if (id == MH_ULTRABEE_G) {
+    y0plus = (yplus + ymoins) / 2.;
+    y0moins = (yplus + ymoins) / 2.;
+  } else if (id == MH_ARITHMETIC_G) {
+    y0plus = ((h0 + hmoins + hplus) * yplus + h0 * ymoins) / (2 * h0 + hmoins + hplus);
+    y0moins = ((h0 + hmoins + hplus) * ymoins + h0 * yplus) / (2 * h0 + hmoins + hplus);
+  }
+  if (simple_pente) {
+    y0plus = yplus;
+    y0moins = ymoins;
+  }
+  return type == 0 ? y0plus : y0moins;
+}
+
+// One pente-borne flux (computeFluxPP / computeFluxPPPure, UtilesRemap.cc:425-832, per
+// variable): the three-piece reconstruction of computexgxd (:269-290) and computeygyd
+// (:302-318) integrated over the part of the cell that leaves through the back (type 0)
+// or front (type 1) face during the step, ppv = positive part of the outgoing velocity x dt.
+static double pbFlux(int type, double y0, double yplus, double ymoins, double h0, double hplus, double hmoins,
+                     double grady, double ppv, int limY, double thr, bool simple_pente) {
+  const double y0plus = computeY0(limY, y0, yplus, ymoins, h0, hplus, hmoins, 0, simple_pente);
+  const double y0moins = computeY0(limY, y0, yplus, ymoins, h0, hplus, hmoins, 1, simple_pente);
+  double xplus = 1.;
+  if (ref_abs(y0plus - yplus) > thr) xplus = (y0 - yplus) / (y0plus - yplus) - 1. / 2.;
+  double xmoins = 1.;
+  if (ref_abs(y0moins - ymoins) > thr) xmoins = (y0 - ymoins) / (y0moins - ymoins) - 1. / 2.;
+  const double xd = +h0 * amin(amax(xplus, -1. / 2.), 1. / 2.);
+  const double xg = -h0 * amin(amax(xmoins, -1. / 2.), 1. / 2.);
+  const double xtd = y0 + h0 / 2 * grady;
+  const double xtg = y0 - h0 / 2 * grady;
+  const double ad = amin(yplus, 2. * y0moins - ymoins);
+  const double bd = amax(yplus, 2. * y0moins - ymoins);
+  const double ag = amin(ymoins, 2. * y0plus - yplus);
+  const double bg = amax(ymoins, 2. * y0plus - yplus);
+  const double yd = amin(amax(xtd, ad), bd);
+  const double yg = amin(amax(xtg, ag), bg);
+  double f1, f1m, f2, f2m, f3, f3m;
+  if (type == 0) {
+    f1m = inty(-h0 / 2. + ppv, -h0 / 2., ymoins, xg, yg, thr);
+    f1 = inty(-h0 / 2., -h0 / 2., ymoins, xg, yg, thr);
+    f2m = inty(-h0 / 2. + ppv, xg, yg, xd, yd, thr);
+    f2 = inty(-h0 / 2., xg, yg, xd, yd, thr);
+    f3m = inty(-h0 / 2. + ppv, xd, yd, h0 / 2., yplus, thr);
+    f3 = inty(-h0 / 2., xd, yd, h0 / 2., yplus, thr);
+  } else {
+    f1 = inty(h0 / 2. - ppv, -h0 / 2., ymoins, xg, yg, thr);
+    f1m = inty(h0 / 2., -h0 / 2., ymoins, xg, yg, thr);
+    f2 = inty(h0 / 2. - ppv, xg, yg, xd, yd, thr);
+    f2m = inty(h0 / 2., xg, yg, xd, yd, thr);
+    f3 = inty(h0 / 2. - ppv, xd, yd, h0 / 2., yplus, thr);
+    f3m = inty(h0 / 2., xd, yd, h0 / 2., yplus, thr);
+  }
+  double F = std::max(((f1m - f1) + (f2m - f2) + (f3m - f3)), 0.);
+  if (((yplus - y0) * (ymoins - y0)) >= 0.) F = y0 * ppv;  // formula 16
+  return F;
+}
+
+// RemapADIService::computeGradPhiCell :161-267 + computeAndLimitGradPhi UtilesRemap.cc:325-357:
+// limited cell slopes (boundary cells are their own missing neighbour) and, with
+// pente-borne, the fluxes through the back and front faces (computeFluxPP, or
+// computeFluxPPPure for a pure neighbourhood with projection-pente-borne-mixte).
 void OracleCase::computeGradPhiCell(int idir) {
   std::fill(grad_phi.begin(), grad_phi.end(), 0.0);
+  const bool pb = opt.projection_pente_borne != 0;
+  if (pb) {
+    std::fill(dphi_ar.begin(), dphi_ar.end(), 0.0);
+    std::fill(dphi_av.begin(), dphi_av.end(), 0.0);
+  }
   if (opt.ordre_projection <= 1) return;
-  const int lim = opt.projection_limiteur_id;
+  const R3 dirproj{0.5 * (1 - idir) * (2 - idir), 1.0 * idir * (2 - idir), -0.5 * idir * (1 - idir)};
+  const double deltat = global_deltat;
+  const double thr = opt.remap_threshold;
+  const bool sp = opt.projection_simple_pente != 0;
 #pragma omp parallel for schedule(static)
   for (long c = 0; c < nc; ++c) {
     long backFace = -1, frontFace = -1;
+    int lb = -1, lf = -1;
+    long bc = c, fc = c;
     for (int l = 0; l < nfpc; ++l) {
       int f = cell_faces[c * nfpc + l];
       if (face_dir[f] != idir) continue;
-      if (face_prev[f] == c) frontFace = f;
-      else if (face_next[f] == c) backFace = f;
+      if (face_prev[f] == c) {
+        frontFace = f;
+        lf = l;
+        if (face_next[f] >= 0) fc = face_next[f];
+      } else if (face_next[f] == c) {
+        backFace = f;
+        lb = l;
+        if (face_prev[f] >= 0) bc = face_prev[f];
+      }
     }
-    for (int v = 0; v < nv; ++v) {
-      double gb = grad_phi_face[backFace * nv + v], gf = grad_phi_face[frontFace * nv + v];
-      double g = 0.;
-      if (gb != 0.) g += 0.5 * (fluxLimiter(lim, gf / gb) * gb);
-      if (gf != 0.) g += 0.5 * (fluxLimiter(lim, gb / gf) * gf);
-      grad_phi[c * nv + v] = g;
+    const bool vpure = opt.pente_borne_mixte && est_mixte[c] == 0 && est_mixte[fc] == 0 && est_mixte[bc] == 0 &&
+                       est_pure[c] == est_pure[fc] && est_pure[c] == est_pure[bc];
+    int lim = opt.projection_limiteur_id;
+    if (opt.projection_limiteur_pure_id == 1 && vpure) lim = opt.projection_limiteur_pure_id;
+    const double* P = &phi_lagrange[c * nv];
+    const double* Pf = &phi_lagrange[fc * nv];
+    const double* Pb = &phi_lagrange[bc * nv];
+    const double h0 = h_cell_lagrange[c], hp = h_cell_lagrange[fc], hm = h_cell_lagrange[bc];
+    for (int v = 0; v < nv; ++v)
+      grad_phi[c * nv + v] = limitedGrad(lim, grad_phi_face[backFace * nv + v], grad_phi_face[frontFace * nv + v],
+                                         P[v], Pf[v], Pb[v], h0, hp, hm, sp);
+    if (!pb) continue;
+    const int limY = vpure ? opt.projection_limiteur_pure_id : opt.projection_limiteur_id;
+    for (int type = 0; type < 2; ++type) {
+      const long face = type == 0 ? backFace : frontFace;
+      const double vout = dot(outer_face_normal[c * nfpc + (type == 0 ? lb : lf)], dirproj) * face_normal_velocity[face];
+      const double ppv = 0.5 * (vout + ref_abs(vout)) * deltat;
+      double* F = type == 0 ? &dphi_ar[c * nv] : &dphi_av[c * nv];
+      for (int v = 0; v < nv; ++v)
+        F[v] = pbFlux(type, P[v], Pf[v], Pb[v], h0, hp, hm, grad_phi[c * nv + v], ppv, limY, thr, sp);
+      // debar fixes (:577-623 mixed neighbourhood, :782-826 pure)
+      if (opt.pente_borne_debar_fix == 1) {
+        double sfv = 0.;
+        for (int k = 0; k < m; ++k) sfv += F[k];
+        if (std::fabs(sfv) > thr) {
+          for (int k = 0; k < m; ++k) {
+            if (!vpure) F[m + k] = (F[m + k] / sfv) * F[k];
+            F[2 * m + k] = (F[2 * m + k] / sfv) * F[m + k];
+          }
+        } else {
+          for (int v = 0; v < nv; ++v) F[v] = 0.;
+        }
+      } else if (opt.pente_borne_debar_fix == 2) {
+        for (int k = 0; k < m; ++k) {
+          if (!vpure) F[m + k] = P[m + k] * F[k];
+          F[2 * m + k] = P[2 * m + k] * F[m + k];
+        }
+      }
+      if (ppv == 0.)
+        for (int v = 0; v < nv; ++v) F[v] = 0.;
     }
   }
 }
@@ -1156,6 +1362,15 @@ void OracleCase::computeUpwindFaceQuantitiesForProjection(int idir) {
     }
     return;
   }
+  if (opt.projection_pente_borne) {  // :312-317: flux out of cb minus flux out of cf
+#pragma omp parallel for schedule(static)
+    for (long f = face_off[idir]; f < face_off[idir] + face_cnt[idir]; ++f) {
+      int cb = face_prev[f], cf = face_next[f];
+      if (cb < 0 || cf < 0) continue;
+      for (int v = 0; v < nv; ++v) phi_face[f * nv + v] = dphi_av[(long)cb * nv + v] - dphi_ar[(long)cf * nv + v];
+    }
+    return;
+  }
   const int order2 = opt.ordre_projection - 1;
 #pragma omp parallel for schedule(static)
   for (long f = face_off[idir]; f < face_off[idir] + face_cnt[idir]; ++f) {
@@ -1172,6 +1387,8 @@ void OracleCase::computeUpwindFaceQuantitiesForProjection(int idir) {
 void OracleCase::computeUremap(int idir) {
   const R3 dirproj{0.5 * (1 - idir) * (2 - idir), 1.0 * idir * (2 - idir), -0.5 * idir * (1 - idir)};
   const double deltat = global_deltat;
+  const bool pb = opt.projection_pente_borne != 0;
+  const double thr = opt.remap_threshold;
   std::fill(dual_phi_flux.begin(), dual_phi_flux.end(), 0.0);
 #pragma omp parallel for schedule(static)
   for (long c = 0; c < nc; ++c) {
@@ -1185,7 +1402,8 @@ void OracleCase::computeUremap(int idir) {
         const R3 outer = outer_face_normal[c * nfpc + l];
         const double od = dot(outer, dirproj);
         for (int v = 0; v < nv; ++v) {
-          double flux = od * fnv * len * deltat * phi_face[(long)f * nv + v];
+          // pente-borne: phi_face already holds the integrated flux
+          double flux = pb ? od * len * phi_face[(long)f * nv + v] : od * fnv * len * deltat * phi_face[(long)f * nv + v];
           flux_face[v] += flux;
           dual_phi_flux[c * nv + v] += 0.5 * flux * comp(outer, idir);
         }
@@ -1193,14 +1411,29 @@ void OracleCase::computeUremap(int idir) {
     }
     double* U = &u_lagrange[c * nv];
     for (int v = 0; v < nv; ++v) U[v] -= flux_face[v];
+    if (pb)  // :441-463: tiny partial volumes, masses or energies drop the environment's three
+      for (int k = 0; k < m; ++k) {
+        if (U[k] < 1. * thr) U[k] = U[m + k] = U[2 * m + k] = 0.;
+        if (U[m + k] < 1. * thr) U[k] = U[m + k] = U[2 * m + k] = 0.;
+        if (U[2 * m + k] < 1. * thr) U[k] = U[m + k] = U[2 * m + k] = 0.;
+      }
     for (int k = 0; k < m; ++k) {
       if (U[m + k] < 0.) U[m + k] = 0.;
       if (U[2 * m + k] < 0.) U[2 * m + k] = 0.;
     }
     double somme_volume = 0.;
     for (int k = 0; k < m; ++k) somme_volume += U[k];
-    for (int v = 0; v < nv; ++v) phi_lagrange[c * nv + v] = U[v] / somme_volume;
-    setMixedFlags(c, &phi_lagrange[c * nv]);  // :559-573
+    double* P = &phi_lagrange[c * nv];
+    if (pb) {  // :493-538: volume fraction, density, specific energy; the pseudo keeps its value
+      for (int k = 0; k < m; ++k) {
+        P[k] = U[k] / somme_volume;
+        P[m + k] = U[k] != 0. ? U[m + k] / U[k] : 0.;
+      }
+      for (int k = 0; k < m; ++k) P[2 * m + k] = U[m + k] != 0. ? U[2 * m + k] / U[m + k] : 0.;
+    } else {
+      for (int v = 0; v < nv; ++v) P[v] = U[v] / somme_volume;
+    }
+    setMixedFlags(c, P);  // :559-573
   }
 }
 
@@ -1233,12 +1466,31 @@ void OracleCase::computeDualUremap(int idir) {
       if (back < 0 || front < 0) continue;
       const double xn = comp(node_coord[n], idir);
       const double xf = comp(node_coord[front], idir), xb = comp(node_coord[back], idir);
+      // dual cell widths (UtilesRemap.cc:50-61); with neither second neighbour (two
+      // cells along the sweep) both outer widths are 0
+      const int bb = prv[back], ff = nxt[front];
+      const double h0 = 0.5 * (xf - xb);
+      double hm = 0., hp = 0.;
+      if (bb < 0) {
+        if (ff >= 0) hp = 0.5 * (comp(node_coord[ff], idir) - xn);
+      } else if (ff < 0) {
+        hm = 0.5 * (xn - comp(node_coord[bb], idir));
+      } else {
+        hm = 0.5 * (xn - comp(node_coord[bb], idir));
+        hp = 0.5 * (comp(node_coord[ff], idir) - xn);
+      }
       for (int v = 0; v < 3; ++v) {
         double gf = (phi_dual[front * 5 + v] - phi_dual[n * 5 + v]) / (xf - xn);
         double gb = (phi_dual[n * 5 + v] - phi_dual[back * 5 + v]) / (xn - xb);
-        double g = 0.;
-        if (gb != 0.) g += 0.5 * fluxLimiter(lim, gf / gb) * gb;
-        if (gf != 0.) g += 0.5 * fluxLimiter(lim, gb / gf) * gf;
+        double g;
+        if (lim < MH_MINMOD_G) {  // computeAndLimitGradPhiDual :369-400
+          g = 0.;
+          if (gb != 0.) g += 0.5 * fluxLimiter(lim, gf / gb) * gb;
+          if (gf != 0.) g += 0.5 * fluxLimiter(lim, gb / gf) * gf;
+        } else {
+          g = fluxLimiterG(lim, gf, gb, phi_dual[n * 5 + v], phi_dual[front * 5 + v], phi_dual[back * 5 + v], h0, hp,
+                           hm, opt.projection_simple_pente != 0);
+        }
         dual_grad_phi[n * 5 + v] = g;
       }
     }
@@ -1541,6 +1793,9 @@ OracleCase* oracle_create(const MhMesh* mesh, const MhEnvironment* envs, int nb_
   if (mesh->dim != 2 && mesh->dim != 3) return nullptr;
   if (mesh->nx < 1 || mesh->ny < 1 || (mesh->dim == 3 && mesh->nz < 1)) return nullptr;
   if (opt->ordre_projection < 1 || opt->ordre_projection > 3) return nullptr;
+  if (opt->projection_limiteur_id < 0 || opt->projection_limiteur_id > MH_ULTRABEE_G) return nullptr;
+  if (opt->projection_pente_borne && (opt->pente_borne_debar_fix < 0 || opt->pente_borne_debar_fix > 2)) return nullptr;
+  if (opt->projection_pente_borne && opt->ordre_projection == 3) return nullptr;
   OracleCase* h = new OracleCase();
   h->m = nb_env;
   for (int k = 0; k < nb_env; ++k) h->env[k] = envs[k];
@@ -1579,6 +1834,10 @@ OracleCase* oracle_create(const MhMesh* mesh, const MhEnvironment* envs, int nb_
   h->dual_grad_phi.assign(nn * 5, 0.);
   h->deltax_lagrange.assign(nf, 0.);
   h->h_cell_lagrange.assign(h->nc, 0.);
+  if (opt->projection_pente_borne) {
+    h->dphi_ar.assign(h->nc * h->nv, 0.);
+    h->dphi_av.assign(h->nc * h->nv, 0.);
+  }
   h->est_mixte.assign(h->nc, 0);
   h->est_pure.assign(h->nc, 0);
   h->face_normal_velocity.assign(nf, 0.);

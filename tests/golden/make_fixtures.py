@@ -52,6 +52,13 @@ CASES = {
     "CAS_2D_Lag_SodCaseY": (2, (5, 100, 1), (0, 0, 0), (.1, 1., 0.)),
     "CAS_2D_Lag_SodCaseZ": (3, (5, 1, 100), (0, 0, 0), (.1, .1, 1.)),
     "CAS_Sedov": (3, (60, 60, 60), (0, 0, 0), (1.2, 1.2, 1.2)),
+    "CAS_BiSodCaseX_pente_borne_minmod_debar_1": (3, (100, 5, 5), (0, 0, 0), (1., .1, .1)),
+    "CAS_BiSodCaseX_pente_borne_minmod_debar_2": (3, (100, 5, 5), (0, 0, 0), (1., .1, .1)),
+    "CAS_BiSodCaseX_pente_borne_superbee_debar_2": (3, (100, 5, 5), (0, 0, 0), (1., .1, .1)),
+    "CAS_BiSodCaseY_pente_borne_minmod_debar_2": (3, (5, 100, 5), (0, 0, 0), (.1, 1., .1)),
+    "CAS_BiSodCaseY_pente_borne_superbee_debar_2": (3, (5, 100, 5), (0, 0, 0), (.1, 1., .1)),
+    "CAS_MonoPointtriple_pb_mixte_superbeeG_superbeeG": (2, (70, 30, 1), (0, 0, 0), (.07, .03, 0.)),
+    "CAS_MonoPointtriple_pb_sp_mixte_superbeeG_superbeeG": (2, (70, 30, 1), (0, 0, 0), (.07, .03, 0.)),
 }
 # Runs whose output holds no geometry: parts are the subdomains of the Cartesian generator
 # in rank order (x fastest), each with its cells in local order (x fastest); their node

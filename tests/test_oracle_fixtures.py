@@ -51,11 +51,47 @@ FIXTURES = {
     # (CAS_Lag_BiSodCaseZ) both match; the run wrote no time history to narrow it down)
     # 3D Sedov blast, 60^3 on 8 subdomains
     "CAS_Sedov": cases.sedov_3d_fixture,
+    # pente-borne fluxes (UtilesRemap.cc:425-832) with the two debar fixes
+    "CAS_BiSodCaseX_pente_borne_minmod_debar_1":
+        lambda: cases.pente_borne(cases.bisod_x, "CAS_BiSodCaseX_pente_borne_minmod_debar_1", 1),
+    "CAS_BiSodCaseX_pente_borne_minmod_debar_2":
+        lambda: cases.pente_borne(cases.bisod_x, "CAS_BiSodCaseX_pente_borne_minmod_debar_2", 2),
+    "CAS_BiSodCaseX_pente_borne_superbee_debar_2":
+        lambda: cases.pente_borne(cases.bisod_x, "CAS_BiSodCaseX_pente_borne_superbee_debar_2", 2, T.MH_SUPERBEE),
+    "CAS_BiSodCaseY_pente_borne_minmod_debar_2":
+        lambda: cases.pente_borne(cases.bisod_y, "CAS_BiSodCaseY_pente_borne_minmod_debar_2", 2),
+    "CAS_BiSodCaseY_pente_borne_superbee_debar_2":
+        lambda: cases.pente_borne(cases.bisod_y, "CAS_BiSodCaseY_pente_borne_superbee_debar_2", 2, T.MH_SUPERBEE),
+    # pente-borne on the 3-material triple point with the generalised superbee limiter
+    "CAS_MonoPointtriple_pb_mixte_superbeeG_superbeeG": lambda: cases.mono_pointtriple(False),
+    "CAS_MonoPointtriple_pb_sp_mixte_superbeeG_superbeeG": lambda: cases.mono_pointtriple(True),
+    # (CAS_BiSodCaseZ_pente_borne_superbee_debar_2 is left out: its output holds four EnSight
+    # steps, the last at t = 0.1973 < final-time, and no time history; the oracle reaches the
+    # times of the first two steps at iterations 50 and 100 exactly, but the stored fields
+    # there do not hold the dataset's initial condition -- density 1 against 0.125 over the
+    # upper half of the tube at t = 0.0021 -- so the files come from another setup)
 }
 TH = ["CellMass", "Pressure", "Density", "InternalEnergy"]
 # The averages of the 60^3 Sedov run are sums of 216000 positive terms formed per
 # subdomain (8) and then reduced: any other order stays within (n - 1) eps = 2.4e-11.
 TH_RTOL = {"CAS_Sedov": 2.5e-11}
+# Ill-conditioned averages.  The pressure average of an environment is a plain mean that
+# includes cells holding 1e-16 of it, and the triple point with the generalised superbee
+# slopes is chaotic: a one-ulp change of deltat-init moves the oracle's own averages past
+# 1e-12 from row ~10 (pressure) and ~130 (the others) on, and its final fields everywhere
+# behind the shock (test_chaotic_fixture_is_chaotic).  Per run and variable: (rows compared,
+# relative tolerance); final EnSight fields are compared only for runs not in CHAOTIC.
+TH_LIMITS = {
+    "CAS_MonoPointtriple_pb_sp_mixte_superbeeG_superbeeG": {"Pressure": (None, 1e-10)},
+    "CAS_MonoPointtriple_pb_mixte_superbeeG_superbeeG": {"Pressure": (40, 1e-12), "CellMass": (120, 1e-12),
+                                                         "Density": (120, 1e-12), "InternalEnergy": (120, 1e-12)},
+}
+CHAOTIC = {"CAS_MonoPointtriple_pb_mixte_superbeeG_superbeeG"}
+
+
+def th_bar(name, var):
+    """(rows, rtol) of the time-history comparison of variable var for run name."""
+    return TH_LIMITS.get(name, {}).get(var, (None, TH_RTOL.get(name, 1e-12)))
 ENSIGHT = {"Density": T.MH_F_DENSITY, "Pressure": T.MH_F_PRESSURE, "InternalEnergy": T.MH_F_INTERNAL_ENERGY,
            "CellMass": T.MH_F_CELL_MASS, "PseudoViscosity": T.MH_F_PSEUDO_VISCOSITY, "Materiau": T.MH_F_MATERIAU}
 
@@ -97,11 +133,11 @@ def test_time_history(name, runs):
             key = f"th_{v}_{k}"
             if key not in ref:
                 continue
-            np.testing.assert_allclose(th[:, 1 + 5 * k + vi], ref[key], rtol=TH_RTOL.get(name, 1e-12), atol=0,
-                                       err_msg=key)
+            rows, rtol = th_bar(name, v)
+            np.testing.assert_allclose(th[:rows, 1 + 5 * k + vi], ref[key][:rows], rtol=rtol, atol=0, err_msg=key)
 
 
-@pytest.mark.parametrize("name", list(FIXTURES))
+@pytest.mark.parametrize("name", [n for n in FIXTURES if n not in CHAOTIC])
 def test_ensight_fields(name, runs):
     o, st = _run(name, runs)
     ref = np.load(os.path.join(GOLDEN, name + ".npz"))
@@ -138,3 +174,25 @@ def test_lagrange_fixtures_bit_exact(runs):
                 if key in ref:
                     got = np.array([float("%.16e" % x) for x in th[:, 1 + 5 * k + vi]])
                     assert np.array_equal(got, ref[key]), key
+
+
+def test_chaotic_fixture_is_chaotic():
+    """The run left out of the final-field comparison is chaotic: changing deltat-init by one
+    ulp moves the oracle's own final averages by far more than any comparison bar, while the
+    simple-pente variant of the same case stays within 1e-10."""
+    from oracle.pyoracle import OracleCase
+
+    def final_density_avg(name, bump):
+        c = FIXTURES[name]()
+        if bump:
+            c.deltat_init = float(np.nextafter(c.deltat_init, 1.0))
+        o = OracleCase(c)
+        o.run(10 ** 7)
+        return o.history()[-1, 1 + 5 * 1 + 2]  # density average of environment 1
+
+    name = "CAS_MonoPointtriple_pb_mixte_superbeeG_superbeeG"
+    a, b = final_density_avg(name, False), final_density_avg(name, True)
+    assert abs(a - b) > 1e-6 * abs(a), (a, b)
+    name = "CAS_MonoPointtriple_pb_sp_mixte_superbeeG_superbeeG"
+    a, b = final_density_avg(name, False), final_density_avg(name, True)
+    assert abs(a - b) <= 1e-10 * abs(a), (a, b)

@@ -428,12 +428,25 @@ void launch_sweeps_t(MhHandle* h) {
     dirs[nd++] = idir;
   }
   const bool o3 = S.order2 == 2;
-  if (o3 && nd > 0) LAUNCH(h, KN_HCELL, (k_hcell<DIM>), h->nblk_c, MH_BLOCK, S);
+  // pente-borne or generalised limiters: one thread per cell and node, limiter read at run time
+  const bool gen = !o3 && (S.pb || (S.order2 && S.limiter >= MH_MINMOD_G));
+  if (S.hc && nd > 0) LAUNCH(h, KN_HCELL, (k_hcell<DIM>), h->nblk_c, MH_BLOCK, S);
   for (int s = 0; s < nd; ++s) {
     const int first = s == 0, last = (s == nd - 1) && !first, sel = s & 1;
     const int D = dirs[s];
     const int mode = h->sweep_mode;
-    if (o3) {  // order 3: one thread per cell for the cell part, node part as selected
+    if (gen) {
+      if (D == 0) {
+        LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell_gen<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+        LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0, MH_LIM_ANY>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+      } else if (D == 1) {
+        LAUNCH(h, KN_SWEEP_CELL + 1, (k_sweep_cell_gen<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+        LAUNCH(h, KN_SWEEP_NODE + 1, (k_sweep_node<DIM, 1, MH_LIM_ANY>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+      } else if constexpr (DIM == 3) {
+        LAUNCH(h, KN_SWEEP_CELL + 2, (k_sweep_cell_gen<3, 2, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+        LAUNCH(h, KN_SWEEP_NODE + 2, (k_sweep_node<3, 2, MH_LIM_ANY>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+      }
+    } else if (o3) {  // order 3: one thread per cell for the cell part, node part as selected
       if (D == 0) LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell3<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
       else if (D == 1) LAUNCH(h, KN_SWEEP_CELL + 1, (k_sweep_cell3<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, sel, first);
       else if constexpr (DIM == 3) LAUNCH(h, KN_SWEEP_CELL + 2, (k_sweep_cell3<3, 2, M>), h->nblk_c, MH_BLOCK, S, sel, first);
@@ -543,6 +556,11 @@ int validate(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const Mh
   if (mesh->dim != 2 && mesh->dim != 3) return MH_ERR_INVALID;
   if (mesh->nx < 1 || mesh->ny < 1 || (mesh->dim == 3 && mesh->nz < 1)) return MH_ERR_INVALID;
   if (o->ordre_projection < 1 || o->ordre_projection > 3) return MH_ERR_UNSUPPORTED;
+  if (o->projection_limiteur_id < 0 || o->projection_limiteur_id > MH_ULTRABEE_G) return MH_ERR_INVALID;
+  if (o->projection_limiteur_pure_id < 0 || o->projection_limiteur_pure_id > MH_ULTRABEE_G) return MH_ERR_INVALID;
+  if (o->projection_pente_borne && (o->pente_borne_debar_fix < 0 || o->pente_borne_debar_fix > 2))
+    return MH_ERR_INVALID;
+  if (o->projection_pente_borne && o->ordre_projection == 3) return MH_ERR_UNSUPPORTED;
   if (o->nb_bc < 0 || o->nb_bc > MH_MAX_BC) return MH_ERR_INVALID;
   for (int b = 0; b < o->nb_bc; ++b)
     if (o->bc[b].surface < 0 || o->bc[b].surface > 5 || o->bc[b].type < 0 || o->bc[b].type > 2) return MH_ERR_INVALID;
@@ -584,9 +602,16 @@ size_t case_footprint(const MhMesh& mesh, int m, int with_proj) {
   return (size_t)words * 8 + (size_t)nc;
 }
 
+// cases whose sweeps take the pente-borne / generalised-limiter kernels
+bool ens_gen(const MhHandle* h) {
+  const MhDev& S = h->dev;
+  return S.order2 != 2 && (S.pb || (S.order2 && S.limiter >= MH_MINMOD_G));
+}
+
 struct EnsGroup {
   int dim = 0, m = 0;
-  bool o3 = false;  // order-3 remap
+  bool o3 = false;   // order-3 remap
+  bool gen = false;  // pente-borne or generalised limiters
   std::vector<int> slots;
   int2* d_cmap = nullptr;
   int2* d_nmap = nullptr;
@@ -660,7 +685,7 @@ int ens_rebuild(MhEnsembleImpl* E) {
     }
     bool found = false;
     for (auto& g : E->groups)
-      if (g.dim == h->g.dim && g.m == h->m && g.o3 == (h->dev.order2 == 2)) {
+      if (g.dim == h->g.dim && g.m == h->m && g.o3 == (h->dev.order2 == 2) && g.gen == ens_gen(h)) {
         g.slots.push_back(sl);
         found = true;
       }
@@ -669,6 +694,7 @@ int ens_rebuild(MhEnsembleImpl* E) {
       g.dim = h->g.dim;
       g.m = h->m;
       g.o3 = h->dev.order2 == 2;
+      g.gen = ens_gen(h);
       g.slots.push_back(sl);
       E->groups.push_back(g);
     }
@@ -691,7 +717,7 @@ int ens_rebuild(MhEnsembleImpl* E) {
     HIPCHK(hipMemcpy(g.d_nmap, nm.data(), sizeof(int2) * nm.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(g.d_list, g.slots.data(), sizeof(int) * g.slots.size(), hipMemcpyHostToDevice));
     // rows and tiles when every case of the group runs them
-    g.rows = !g.o3;
+    g.rows = !g.o3 && !g.gen;
     for (int sl : g.slots) g.rows = g.rows && E->slot[sl]->sweep_mode == MH_SWEEP_TILE;
     if (g.rows) {
       std::vector<int2> rc, rn, t[2];
@@ -729,9 +755,25 @@ void ens_group_cycle(MhEnsembleImpl* E, const EnsGroup& g) {
   hipLaunchKernelGGL((k_node_lag_batch<DIM>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap);
   hipLaunchKernelGGL((k_cell_lag_batch<DIM, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap);
   const int sens = (int)(E->launched % DIM);
-  if (g.o3) hipLaunchKernelGGL((k_hcell_batch<DIM>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap);
+  if (g.o3 || g.gen) hipLaunchKernelGGL((k_hcell_batch<DIM>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap);
   for (int s = 0; s < DIM; ++s) {
     const int D = (s + sens) % DIM, first = s == 0, last = s == DIM - 1, sel = s & 1;
+    if (g.gen) {
+      if (D == 0) {
+        hipLaunchKernelGGL((k_sweep_cell_gen_batch<DIM, 0, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
+        hipLaunchKernelGGL((k_sweep_node_batch<DIM, 0, MH_LIM_ANY>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel,
+                           first, last);
+      } else if (D == 1) {
+        hipLaunchKernelGGL((k_sweep_cell_gen_batch<DIM, 1, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
+        hipLaunchKernelGGL((k_sweep_node_batch<DIM, 1, MH_LIM_ANY>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel,
+                           first, last);
+      } else if constexpr (DIM == 3) {
+        hipLaunchKernelGGL((k_sweep_cell_gen_batch<3, 2, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
+        hipLaunchKernelGGL((k_sweep_node_batch<3, 2, MH_LIM_ANY>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel,
+                           first, last);
+      }
+      continue;
+    }
     if (g.o3) {
       if (D == 0)
         hipLaunchKernelGGL((k_sweep_cell3_batch<DIM, 0, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
@@ -877,6 +919,12 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
   S.tfinal = options->final_time;
   S.order2 = options->ordre_projection - 1;
   S.limiter = options->projection_limiteur_id;
+  S.pb = options->projection_pente_borne ? 1 : 0;
+  S.debar = options->pente_borne_debar_fix;
+  S.pb_mixte = options->pente_borne_mixte ? 1 : 0;
+  S.lim_pure = options->projection_limiteur_pure_id;
+  S.simple_pente = options->projection_simple_pente ? 1 : 0;
+  S.nsweep = (g.nx > 1) + (g.ny > 1) + (g.dim == 3 && g.nz > 1);
   S.with_proj = options->with_projection ? 1 : 0;
   S.vitson_max = options->calcul_vitson_max ? 1 : 0;
   S.nbc = options->nb_bc;
@@ -928,10 +976,11 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
     ok |= dalloc(h, &h->Ud, (size_t)4 * nn);
     ok |= dalloc(h, &h->Pa, (size_t)3 * nn);
     ok |= dalloc(h, &h->Pb, (size_t)3 * nn);
-    if (options->ordre_projection == 3) {
+    const bool gen_lim = options->ordre_projection == 2 && options->projection_limiteur_id >= MH_MINMOD_G;
+    if (options->ordre_projection == 3 || options->projection_pente_borne || gen_lim)
       ok |= dalloc(h, &S.hc, (size_t)g.dim * nc);
-      ok |= dalloc(h, &S.est, (size_t)nc);
-    }
+    if (options->ordre_projection == 3 || options->pente_borne_mixte) ok |= dalloc(h, &S.est, (size_t)nc);
+    if (options->projection_pente_borne) ok |= dalloc(h, &S.phi0, (size_t)nv * nc);
   } else {
     ok |= dalloc(h, &h->Ua, 1);
   }
@@ -995,10 +1044,15 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
   ok |= upload(h, S.pk, I.pk);
   ok |= upload(h, S.Mk, I.Mk);
   ok |= upload(h, S.err, err);
-  std::vector<int8_t> est0;  // order-3 flags: Arcane starts est_mixte and est_pure at 0
+  std::vector<int8_t> est0;  // mixed-cell flags: Arcane starts est_mixte and est_pure at 0
   if (S.est) {
     est0.assign(nc, 0);
     ok |= upload(h, S.est, est0);
+  }
+  std::vector<double> phi00;  // pente-borne phi: Arcane starts m_phi_lagrange at 0
+  if (S.phi0) {
+    phi00.assign((size_t)S.nv * nc, 0.);
+    ok |= upload(h, S.phi0, phi00);
   }
   ok |= hip_check(hipMemcpyAsync(S.sc, &sc, sizeof(sc), hipMemcpyHostToDevice, h->stream), "upload scalars");
   ok |= hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
@@ -1545,6 +1599,10 @@ static bool same_options(const MhOptions& a, const MhOptions& b) {
             a.deltat_min == b.deltat_min && a.deltat_max == b.deltat_max && a.final_time == b.final_time &&
             a.with_projection == b.with_projection && a.calcul_vitson_max == b.calcul_vitson_max &&
             a.nb_bc == b.nb_bc && a.ordre_projection == b.ordre_projection &&
+            a.projection_pente_borne == b.projection_pente_borne && a.pente_borne_debar_fix == b.pente_borne_debar_fix &&
+            a.pente_borne_mixte == b.pente_borne_mixte &&
+            a.projection_limiteur_pure_id == b.projection_limiteur_pure_id &&
+            a.projection_simple_pente == b.projection_simple_pente &&
             a.projection_limiteur_id == b.projection_limiteur_id && a.remap_threshold == b.remap_threshold &&
             a.cas_model == b.cas_model && a.cas_test == b.cas_test && a.interface_position == b.interface_position &&
             a.interface_axis == b.interface_axis && a.sedov_deposit == b.sedov_deposit;

@@ -195,11 +195,15 @@ MH_HD double mh_flux_limiter(int id, double r) {
   return 0.0;
 }
 
+// LIM value of kernels that read the limiter id at run time and also take the generalised
+// limiters (ids >= MH_MINMOD_G, handled by the caller).
+#define MH_LIM_ANY 8
+
 // The limiter with its id known at compile time (LIM >= 0: the branches on the id fold
 // away) or read at run time (LIM < 0, batched launches over cases with different limiters).
 template <int LIM>
 MH_HD double mh_limiter(int id, double r) {
-  return mh_flux_limiter(LIM < 0 ? id : LIM, r);
+  return mh_flux_limiter((LIM < 0 || LIM == MH_LIM_ANY) ? id : LIM, r);
 }
 
 #endif

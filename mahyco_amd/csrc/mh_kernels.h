@@ -491,6 +491,12 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
             u_out[2 * M + kk] = V * rho * e;
           }
         u_out[3 * M] = V * q;
+        if (S.pb) {  // pente-borne phi of the first sweep (PrepareRemap.cc:138-155)
+          S.phi0[ke * nc + c] = S.fg[c];
+          S.phi0[(M + ke) * nc + c] = rho;
+          S.phi0[(2 * M + ke) * nc + c] = e;
+          S.phi0[3 * M * nc + c] = q;
+        }
       } else {
         double p, cc;
         eos(ep, rho, e, p, cc);
@@ -529,6 +535,15 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
           u_out[2 * M + kk] = Vk[kk] * rk[kk] * ek[kk];
         }
         u_out[3 * M] = V * qsum;
+        if (S.pb) {
+          for (int kk = 0; kk < M; ++kk) {
+            if (!((mk >> kk) & 1)) continue;
+            S.phi0[kk * nc + c] = fk[kk];
+            S.phi0[(M + kk) * nc + c] = rk[kk];
+            S.phi0[(2 * M + kk) * nc + c] = ek[kk];
+          }
+          S.phi0[3 * M * nc + c] = qsum;
+        }
       } else {
         double ps = 0., cmix = 0.;
         const double opt = S.vitson_max ? 1. : 0.;
@@ -1024,6 +1039,382 @@ __device__ __forceinline__ void k_sweep_cell3_body(const MhDev& S, int lb, int u
 }
 
 
+// ----------------------------------------------------------------------------- generalised
+// limiters and pente-borne fluxes (UtilesRemap.cc), in the reference's operation order.
+// std::min / std::max as the reference's host code has them.
+__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+
+// fluxLimiterG (UtilesRemap.cc:130-183): slope from the front / back gradients gp, gm and
+// the neighbour values yp, ym for widths h0 (cell), hp / hm (front / back neighbour).
+__device__ __forceinline__ double flux_limiter_g(int id, double gp, double gm, double yp, double ym, double h0,
+                                                 double hp, double hm, int simple_pente) {
+  double grady = 0.;
+  if (gp * gm < 0.0) return 0.;
+  if (id == MH_MINMOD_G) {
+    grady = (yp - ym) > 0. ? smin(fabs(gp), fabs(gm)) : -smin(fabs(gp), fabs(gm));
+  } else if (id == MH_SUPERBEE_G) {
+    grady = (yp - ym) > 0. ? smax(fabs(gp), fabs(gm)) : -smax(fabs(gp), fabs(gm));
+  } else if (id == MH_VANLEER_G) {
+    const double lp = (h0 / 2. + hp) / (h0 + hp + hm);
+    const double lm = (h0 / 2. + hm) / (h0 + hp + hm);
+    const double den = lp * gp + lm * gm;
+    grady = den != 0. ? gp * gm / den : 0.;
+  } else if (id == MH_ULTRABEE_G) {
+    grady = (yp - ym) / h0;
+  } else if (id == MH_ARITHMETIC_G) {
+    const double lp = (h0 / 2. + hp) / (h0 + hp + hm);
+    const double lm = (h0 / 2. + hm) / (h0 + hp + hm);
+    grady = lm * gp + lp * gm;
+  }
+  if (simple_pente) {
+    const double gMp = gp * (h0 + hp) / h0;
+    const double gMm = gm * (h0 + hm) / h0;
+    const double gM = smin(fabs(gMp), fabs(gMm));
+    grady = (yp - ym) > 0. ? smin(fabs(gM), fabs(grady)) : -smin(fabs(gM), fabs(grady));
+  }
+  return grady;
+}
+
+// computeAndLimitGradPhi (UtilesRemap.cc:325-357): classical limiters from the back / front
+// face gradients gb, gf; generalised ones (id >= MH_MINMOD_G) from values and widths.
+__device__ __forceinline__ double limited_grad(int id, double gb, double gf, double yp, double ym, double h0,
+                                               double hp, double hm, int simple_pente) {
+  if (id < MH_MINMOD_G) {
+    double g = 0.;
+    if (gb != 0.) g += 0.5 * (mh_flux_limiter(id, gf / gb) * gb);
+    if (gf != 0.) g += 0.5 * (mh_flux_limiter(id, gb / gf) * gf);
+    return g;
+  }
+  return flux_limiter_g(id, gf, gb, yp, ym, h0, hp, hm, simple_pente);
+}
+
+// INTY (UtilesRemap.cc:86-97): integral from x0 to X of the segment (x0, y0)-(x1, y1).
+__device__ __forceinline__ double inty(double X, double x0, double y0, double x1, double y1, double thr) {
+  double flux = 0.;
+  if (fabs(x1 - x0) > thr) {
+    const double eta = amin(amax(0., (X - x0) / (x1 - x0)), 1.);
+    flux = (y0 + 0.5 * eta * (y1 - y0)) * (x1 - x0) * eta;
+  }
+  return flux;
+}
+
+// computeY0 (UtilesRemap.cc:194-258): monotonicity thresholds y0plus, y0moins.
+__device__ __forceinline__ void compute_y0(int id, double yplus, double ymoins, double h0, double hplus, double hmoins,
+                                           int simple_pente, double& y0plus, double& y0moins) {
+  y0plus = 0.;
+  y0moins = 0.;
+  if (id == MH_MINMOD_G || id == MH_MINMOD) {
+    y0plus = yplus;
+    y0moins = ymoins;
+  } else if (id == MH_SUPERBEE_G || id == MH_SUPERBEE) {
+    y0plus = ((h0 + hmoins) * yplus + h0 * ymoins) / (2 * h0 + hmoins);
+    y0moins = ((h0 + hplus) * ymoins + h0 * yplus) / (2 * h0 + hplus);
+  } else if (id == MH_VANLEER_G || id == MH_VANLEER) {
+    const double a = amin(yplus, ymoins);
+    const double b = amax(yplus, ymoins);
+    double xplus = (h0 * h0 + 3 * h0 * hmoins + 2 * hmoins * hmoins) * yplus;
+    double xmoins = (h0 * h0 + 3 * h0 * hplus + 2 * hplus * hplus) * ymoins;
+    xplus += (h0 * h0 - h0 * hplus - 2 * hplus * hplus + 2 * h0 * hmoins) * ymoins;
+    xmoins += (h0 * h0 - h0 * hmoins - 2 * hmoins * hmoins + 2 * h0 * hplus) * yplus;
+    xplus /= (2 * h0 * h0 + 5 * h0 * hmoins + 2 * hmoins * hmoins - h0 * hplus - 2 * hplus * hplus);
+    xmoins /= (2 * h0 * h0 + 5 * h0 * hplus + 2 * hplus * hplus - h0 * hmoins - 2 * hmoins * hmoins);
+    y0plus = amin(amax(xplus, a), b);
+    y0moins = amin(amax(xmoins, a), b);
+  } else if (id == MH_ULTRABEE_G) {
+    y0plus = (yplus + ymoins) / 2.;
+    y0moins = (yplus + ymoins) / 2.;
+  } else if (id == MH_ARITHMETIC_G) {
+    y0plus = ((h0 + hmoins + hplus) * yplus + h0 * ymoins) / (2 * h0 + hmoins + hplus);
+    y0moins = ((h0 + hmoins + hplus) * ymoins + h0 * yplus) / (2 * h0 + hmoins + hplus);
+  }
+  if (simple_pente) {
+    y0plus = yplus;
+    y0moins = ymoins;
+  }
+}
+
+// One pente-borne flux of one variable (computeFluxPP / computeFluxPPPure,
+// UtilesRemap.cc:425-832): the three-piece reconstruction of computexgxd (:269-290) and
+// computeygyd (:302-318) integrated over the part of the cell that leaves through its back
+// (type 0) or front (type 1) face; ppv = positive part of the outgoing velocity x dt.
+__device__ __forceinline__ double pb_flux(int type, double y0, double yplus, double ymoins, double h0, double hplus,
+                                          double hmoins, double grady, double ppv, int limY, double thr,
+                                          int simple_pente) {
+  double y0plus, y0moins;
+  compute_y0(limY, yplus, ymoins, h0, hplus, hmoins, simple_pente, y0plus, y0moins);
+  double xplus = 1.;
+  if (fabs(y0plus - yplus) > thr) xplus = (y0 - yplus) / (y0plus - yplus) - 1. / 2.;
+  double xmoins = 1.;
+  if (fabs(y0moins - ymoins) > thr) xmoins = (y0 - ymoins) / (y0moins - ymoins) - 1. / 2.;
+  const double xd = +h0 * amin(amax(xplus, -1. / 2.), 1. / 2.);
+  const double xg = -h0 * amin(amax(xmoins, -1. / 2.), 1. / 2.);
+  const double xtd = y0 + h0 / 2 * grady;
+  const double xtg = y0 - h0 / 2 * grady;
+  const double ad = amin(yplus, 2. * y0moins - ymoins);
+  const double bd = amax(yplus, 2. * y0moins - ymoins);
+  const double ag = amin(ymoins, 2. * y0plus - yplus);
+  const double bg = amax(ymoins, 2. * y0plus - yplus);
+  const double yd = amin(amax(xtd, ad), bd);
+  const double yg = amin(amax(xtg, ag), bg);
+  double f1, f1m, f2, f2m, f3, f3m;
+  if (type == 0) {
+    f1m = inty(-h0 / 2. + ppv, -h0 / 2., ymoins, xg, yg, thr);
+    f1 = inty(-h0 / 2., -h0 / 2., ymoins, xg, yg, thr);
+    f2m = inty(-h0 / 2. + ppv, xg, yg, xd, yd, thr);
+    f2 = inty(-h0 / 2., xg, yg, xd, yd, thr);
+    f3m = inty(-h0 / 2. + ppv, xd, yd, h0 / 2., yplus, thr);
+    f3 = inty(-h0 / 2., xd, yd, h0 / 2., yplus, thr);
+  } else {
+    f1 = inty(h0 / 2. - ppv, -h0 / 2., ymoins, xg, yg, thr);
+    f1m = inty(h0 / 2., -h0 / 2., ymoins, xg, yg, thr);
+    f2 = inty(h0 / 2. - ppv, xg, yg, xd, yd, thr);
+    f2m = inty(h0 / 2., xg, yg, xd, yd, thr);
+    f3 = inty(h0 / 2. - ppv, xd, yd, h0 / 2., yplus, thr);
+    f3m = inty(h0 / 2., xd, yd, h0 / 2., yplus, thr);
+  }
+  double F = smax(((f1m - f1) + (f2m - f2) + (f3m - f3)), 0.);
+  if (((yplus - y0) * (ymoins - y0)) >= 0.) F = y0 * ppv;  // formula 16
+  return F;
+}
+
+// One ADI direction, cell part, for the remap variants at order <= 2: runtime classical or
+// generalised limiters (the limiter of a pure neighbourhood when projection-limiteur-pure-id
+// is 1, RemapADIService.cc:205-209), and pente-borne fluxes (:213-264, :312-317, :424-428,
+// :441-463, :493-538): phi is (volume fraction, density, specific energy, pseudo), each
+// face carries the flux leaving the cell behind it minus the flux leaving the cell ahead,
+// each computed from that cell and its two neighbours, then adjusted by the debar fix
+// (UtilesRemap.cc:577-623, :782-826).  Boundary cells are their own missing neighbour.
+template <int DIM, int D, int M>
+__device__ __forceinline__ void k_sweep_cell_gen_body(const MhDev& S, int lb, int usel, int first) {
+  const double* __restrict__ Uin = usel ? S.Ub : S.Ua;
+  double* __restrict__ Uout = usel ? S.Ua : S.Ub;
+  constexpr int NV = 3 * M + 1;
+  const long c = lb * (long)blockDim.x + threadIdx.x;
+  if (c >= S.g.nc) return;
+  if (mh_halted(S.sc)) return;
+  const int nx = S.g.nx, ny = S.g.ny;
+  const int i = (int)(c % nx);
+  const long r = c / nx;
+  const int j = (int)(r % ny);
+  const int k = (int)(r / ny);
+  const int a = D == 0 ? i : D == 1 ? j : k;
+  const int na = D == 0 ? nx : D == 1 ? ny : S.g.nz;
+  const long sD = D == 0 ? 1 : D == 1 ? (long)nx : (long)nx * ny;
+  const long nD = D == 0 ? 1 : D == 1 ? (long)S.g.nnx : (long)S.g.nnx * S.g.nny;
+  const long b = i + (long)S.g.nnx * (j + (long)S.g.nny * k);
+  const long nc = S.g.nc;
+  const double dt = S.sc->dt;
+  const bool pb = S.pb != 0;
+  const double thr = S.thr_remap;
+  const int sp = S.simple_pente;
+  // stencil offsets -2..2 (index o + 2), clamped to the mesh
+  long cs[5];
+  bool valid[5];
+  double den[5], xc[5], hs[5];
+#pragma unroll
+  for (int o = 0; o < 5; ++o) {
+    const int bb = a + o - 2;
+    valid[o] = bb >= 0 && bb < na;
+    const int ao = min(max(bb, 0), na - 1);
+    cs[o] = c + (ao - a) * sD;
+    hs[o] = S.hc[D * nc + cs[o]];
+    xc[o] = valid[o] ? cell_centre_d<DIM, D>(S, b + (o - 2) * nD) : 0.;
+    if (first && !pb) {
+      den[o] = S.Vl[cs[o]];
+    } else {
+      double sv = 0.;
+      if (!first)
+        for (int kk = 0; kk < M; ++kk) sv += Uin[kk * nc + cs[o]];
+      den[o] = sv;
+    }
+  }
+  double dxf[4];
+#pragma unroll
+  for (int f = 0; f < 4; ++f) dxf[f] = (valid[f] && valid[f + 1]) ? xc[f + 1] - xc[f] : 0.;
+  // the two faces of the cell along D: s = 0 back (outer normal -e_D), s = 1 front (+e_D)
+  bool inner[2], upprev[2];
+  double area[2], vf[2], hup[2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    inner[s] = s == 0 ? (a > 0) : (a < na - 1);
+    area[s] = 0.;
+    vf[s] = 0.;
+    hup[s] = 0.;
+    upprev[s] = false;
+    if (!inner[s]) continue;
+    double xf;
+    face_quant<DIM, D>(S, b + s * nD, area[s], xf, vf[s]);
+    upprev[s] = vf[s] * dxf[s + 1] > 0.0;
+    hup[s] = (double)S.order2 * (xf - (upprev[s] ? xc[s + 1] : xc[s + 2]));
+  }
+  // pure neighbourhoods of the cells at offsets -1, 0, +1 (index x = 1, 2, 3)
+  bool vpure[5] = {false, false, false, false, false};
+  if (S.pb_mixte) {
+    int code[5];
+#pragma unroll
+    for (int o = 0; o < 5; ++o) {
+      if (first) {
+        code[o] = S.est[cs[o]];
+      } else {
+        int cnt = 0, pu = -1;
+        for (int kk = 0; kk < M; ++kk)
+          if (Uin[kk * nc + cs[o]] / den[o] > 0.) {
+            cnt++;
+            pu = kk;
+          }
+        code[o] = cnt > 1 ? MH_EST_MIXED : pu;
+      }
+    }
+#pragma unroll
+    for (int x = 1; x <= 3; ++x)
+      vpure[x] = code[x] != MH_EST_MIXED && code[x - 1] != MH_EST_MIXED && code[x + 1] != MH_EST_MIXED &&
+                 code[x] == code[x + 1] && code[x] == code[x - 1];
+  }
+  int lim[5], limY[5];
+#pragma unroll
+  for (int x = 0; x < 5; ++x) {
+    lim[x] = (S.lim_pure == 1 && vpure[x]) ? S.lim_pure : S.limiter;
+    limY[x] = vpure[x] ? S.lim_pure : S.limiter;
+  }
+  // pente-borne combinations q: (cell -1, front), (cell 0, back), (cell 0, front), (cell +1, back)
+  const int qx[4] = {1, 2, 2, 3}, qt[4] = {1, 0, 1, 0};
+  double ppv[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int s = q < 2 ? 0 : 1;
+    const double vout = qt[q] == 1 ? vf[s] : -vf[s];
+    ppv[q] = 0.5 * (vout + fabs(vout)) * dt;
+  }
+  const bool order2 = S.order2 != 0;
+  // phi and limited slopes of variable v at the five stencil cells
+  auto load_phi = [&](int v, double* phi) {
+#pragma unroll
+    for (int o = 0; o < 5; ++o) {
+      const long cc = cs[o];
+      double p;
+      if (!pb) {
+        p = Uin[v * nc + cc] / den[o];
+      } else if (first || v == 3 * M) {
+        p = S.phi0[v * nc + cc];
+      } else if (v < M) {
+        p = Uin[v * nc + cc] / den[o];
+      } else {
+        const double d = Uin[(v - M) * nc + cc];
+        p = d != 0. ? Uin[v * nc + cc] / d : 0.;
+      }
+      phi[o] = p;
+    }
+  };
+  auto slopes = [&](const double* phi, double* gr) {
+    double gface[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) gface[f] = (valid[f] && valid[f + 1]) ? (phi[f + 1] - phi[f]) / dxf[f] : 0.;
+#pragma unroll
+    for (int x = 1; x <= 3; ++x)
+      gr[x] = order2 ? limited_grad(lim[x], gface[x - 1], gface[x], phi[x + 1], phi[x - 1], hs[x], hs[x + 1], hs[x - 1], sp)
+                     : 0.;
+  };
+  auto pb_raw = [&](const double* phi, const double* gr, int q) {
+    const int x = qx[q];
+    return pb_flux(qt[q], phi[x], phi[x + 1], phi[x - 1], hs[x], hs[x + 1], hs[x - 1], gr[x], ppv[q], limY[x], thr, sp);
+  };
+  // pente-borne volume fluxes first: the debar fix and the zero tests need their sums
+  double Fv[4][M], Fm[4][M], sfv[4] = {0., 0., 0., 0.};
+  bool zero[4] = {true, true, true, true};
+  if (pb && order2) {
+    for (int v = 0; v < M; ++v) {
+      double phi[5], gr[5];
+      load_phi(v, phi);
+      slopes(phi, gr);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        Fv[q][v] = inner[q < 2 ? 0 : 1] ? pb_raw(phi, gr, q) : 0.;
+        sfv[q] += Fv[q][v];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) zero[q] = ppv[q] == 0. || (S.debar == 1 && !(fabs(sfv[q]) > thr));
+  }
+  const int s0 = (DIM == 2 && D == 0) ? 1 : 0, s1 = 1 - s0;
+  const double od[2] = {-1., 1.};
+  double un[NV];
+  for (int v = 0; v < NV; ++v) {
+    double phi[5], gr[5];
+    double face[2] = {0., 0.};
+    if (pb) {
+      double F[4] = {0., 0., 0., 0.};
+      if (order2) {
+        if (v >= M) {
+          load_phi(v, phi);
+          slopes(phi, gr);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (!inner[q < 2 ? 0 : 1] || zero[q]) continue;
+          const int x = qx[q];
+          if (v < M) {
+            F[q] = Fv[q][v];
+          } else if (v < 2 * M) {
+            const int e = v - M;
+            double f = pb_raw(phi, gr, q);
+            if (!vpure[x]) {
+              if (S.debar == 1) f = (f / sfv[q]) * Fv[q][e];
+              else if (S.debar == 2) f = phi[x] * Fv[q][e];
+            }
+            F[q] = Fm[q][e] = f;
+          } else if (v < 3 * M) {
+            const int e = v - 2 * M;
+            double f = pb_raw(phi, gr, q);
+            if (S.debar == 1) f = (f / sfv[q]) * Fm[q][e];
+            else if (S.debar == 2) f = phi[x] * Fm[q][e];
+            F[q] = f;
+          } else {
+            F[q] = pb_raw(phi, gr, q);
+          }
+        }
+      }
+      face[0] = F[0] - F[1];
+      face[1] = F[2] - F[3];
+    } else {
+      load_phi(v, phi);
+      slopes(phi, gr);
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int up = upprev[s] ? s + 1 : s + 2;
+        face[s] = phi[up] + hup[s] * gr[up];
+      }
+    }
+    double flux[2] = {0., 0.};
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      if (!inner[s]) continue;
+      flux[s] = pb ? od[s] * area[s] * face[s] : od[s] * vf[s] * area[s] * dt * face[s];
+    }
+    double ff = 0.;
+    if (inner[s0]) ff += flux[s0];
+    if (inner[s1]) ff += flux[s1];
+    un[v] = Uin[v * nc + c] - ff;
+    if (v >= M && v < 2 * M) {
+      double dd = 0.;
+      if (inner[s0]) dd += 0.5 * flux[s0] * od[s0];
+      if (inner[s1]) dd += 0.5 * flux[s1] * od[s1];
+      S.dflux[(v - M) * nc + c] = dd;
+    }
+  }
+  if (pb)  // tiny partial volumes, masses or energies drop the environment (:441-463)
+    for (int kk = 0; kk < M; ++kk) {
+      if (un[kk] < 1. * thr) un[kk] = un[M + kk] = un[2 * M + kk] = 0.;
+      if (un[M + kk] < 1. * thr) un[kk] = un[M + kk] = un[2 * M + kk] = 0.;
+      if (un[2 * M + kk] < 1. * thr) un[kk] = un[M + kk] = un[2 * M + kk] = 0.;
+    }
+  for (int v = 0; v < NV; ++v) {
+    double w = un[v];
+    if (v >= M && v < 3 * M && w < 0.) w = 0.;  // computeUremap clamps, :465-485
+    Uout[v * nc + c] = w;
+  }
+}
+
 // The x sweep with one wavefront per row segment: lane l holds cell i = 60 s - 2 + l of a
 // row, so the stencil neighbours along x are the neighbouring lanes.  Phi, the gradient of
 // the back face, the limited slope and the flux of the back face are formed once per cell
@@ -1480,17 +1871,40 @@ __device__ __forceinline__ void k_sweep_node_body(const MhDev& S, int lb, int ps
     }
   }
   double G[3][NC];
+  // coordinates two and three nodes away, for the dual widths of generalised limiters
+  double Xmm = 0., Xpp = 0.;
+  if (LIM == MH_LIM_ANY) {
+    if (a - 3 >= 0) Xmm = S.xlc[D][n - 3 * nD];
+    if (a + 3 <= na) Xpp = S.xlc[D][n + 3 * nD];
+  }
 #pragma unroll
   for (int o = -1; o <= 1; ++o) {
     const int bb = a + o;
     const bool in = S.order2 && bb >= 1 && bb <= na - 1;
+    double h0 = 0., hm = 0., hp = 0.;
+    if (LIM == MH_LIM_ANY && in) {  // dual cell widths (UtilesRemap.cc:50-61)
+      const double xbb = o == -1 ? Xmm : X[o], xff = o == 1 ? Xpp : X[o + 4];
+      h0 = 0.5 * (X[o + 3] - X[o + 1]);
+      if (bb - 2 < 0) {
+        if (bb + 2 <= na) hp = 0.5 * (xff - X[o + 2]);
+      } else if (bb + 2 > na) {
+        hm = 0.5 * (X[o + 2] - xbb);
+      } else {
+        hm = 0.5 * (X[o + 2] - xbb);
+        hp = 0.5 * (xff - X[o + 2]);
+      }
+    }
     for (int cc = 0; cc < NC; ++cc) {
       double g = 0.;
       if (in) {
         const double gf = (P[o + 3][cc] - P[o + 2][cc]) / (X[o + 3] - X[o + 2]);
         const double gb = (P[o + 2][cc] - P[o + 1][cc]) / (X[o + 2] - X[o + 1]);
-        if (gb != 0.) g += 0.5 * mh_limiter<LIM>(S.limiter, gf / gb) * gb;
-        if (gf != 0.) g += 0.5 * mh_limiter<LIM>(S.limiter, gb / gf) * gf;
+        if (LIM == MH_LIM_ANY && S.limiter >= MH_MINMOD_G) {
+          g = flux_limiter_g(S.limiter, gf, gb, P[o + 3][cc], P[o + 1][cc], h0, hp, hm, S.simple_pente);
+        } else {
+          if (gb != 0.) g += 0.5 * mh_limiter<LIM>(S.limiter, gf / gb) * gb;
+          if (gf != 0.) g += 0.5 * mh_limiter<LIM>(S.limiter, gb / gf) * gf;
+        }
       }
       G[o + 1][cc] = g;
     }
@@ -1582,6 +1996,16 @@ __device__ __forceinline__ void k_finalize_body(const MhDev& S, int lb, int usel
     const double thr = S.thr_remap;
     double U[3 * M + 1];
     for (int v = 0; v <= 3 * M; ++v) U[v] = Uin[v * nc + c];
+    if (S.pb && S.nsweep > 0) {  // phi the last sweep left (RemapADIService.cc:493-538): what the
+                                  // next cycle's first sweep sees for non-member environments
+      double sv = 0.;
+      for (int kk = 0; kk < M; ++kk) sv += U[kk];
+      for (int kk = 0; kk < M; ++kk) {
+        S.phi0[kk * nc + c] = U[kk] / sv;
+        S.phi0[(M + kk) * nc + c] = U[kk] != 0. ? U[M + kk] / U[kk] : 0.;
+      }
+      for (int kk = 0; kk < M; ++kk) S.phi0[(2 * M + kk) * nc + c] = U[M + kk] != 0. ? U[2 * M + kk] / U[M + kk] : 0.;
+    }
     const double VE = S.VE[c];
     // environment membership (:29-78)
     const uint8_t om = S.mask[c];
@@ -2027,6 +2451,8 @@ MH_WRAP(k_sweep_cell_tile, MH_TILE_MIN_WAVES,
 MH_WRAP(k_sweep_node_row, 1, template <int DIM MH_COMMA int LIM>, <DIM MH_COMMA LIM>,
         MH_COMMA int psel MH_COMMA int first MH_COMMA int last, MH_COMMA psel MH_COMMA first MH_COMMA last)
 MH_WRAP(k_hcell, 1, template <int DIM>, <DIM>, , )
+MH_WRAP(k_sweep_cell_gen, 1, template <int DIM MH_COMMA int D MH_COMMA int M>, <DIM MH_COMMA D MH_COMMA M>,
+        MH_COMMA int usel MH_COMMA int first, MH_COMMA usel MH_COMMA first)
 MH_WRAP(k_sweep_cell3, 1, template <int DIM MH_COMMA int D MH_COMMA int M>, <DIM MH_COMMA D MH_COMMA M>,
         MH_COMMA int usel MH_COMMA int first, MH_COMMA usel MH_COMMA first)
 

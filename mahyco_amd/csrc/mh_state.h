@@ -77,11 +77,21 @@ struct MhDev {
   // remap buffers: U ping-pong [nv][nc], mass fluxes [m][nc], dual state [4][nn] (in
   // place), dual phi ping-pong [3][nn]
   double *Ua, *Ub, *dflux, *Ud, *Pa, *Pb;
-  // order-3 remap only (null otherwise): cell widths per direction [dim][nc], and the
-  // mixed-cell flag remapVariables leaves per cell (MH_EST_MIXED, else the index of the one
-  // environment with a positive volume fraction, -1 for none)
+  // cell widths per direction [dim][nc] (order 3, pente-borne, generalised limiters), and
+  // the mixed-cell flag remapVariables leaves per cell (order 3, pente-borne-mixte):
+  // MH_EST_MIXED, else the index of the one environment with a positive volume fraction,
+  // -1 for none.  Null when unused.
   double* hc;
   int8_t* est;
+  // pente-borne remap: phi of the first sweep [3m+1][nc] (the environments' own values for
+  // members, the last sweep's values otherwise)
+  double* phi0;
+  int pb;            // projection-pente-borne
+  int debar;         // projection-pente-borne-debar-fix (0, 1, 2)
+  int pb_mixte;      // projection-pente-borne-mixte
+  int lim_pure;      // projection-limiteur-pure-id
+  int simple_pente;  // projection-simple-pente
+  int nsweep;        // directions with more than one cell
   int nblk_c, nblk_n;  // blocks of MH_BLOCK threads over cells / nodes
   int max_iter;        // > 0: stop after this many iterations (fixed-horizon runs)
   int hist_period;     // > 0: EnvSummation rows every hist_period iterations

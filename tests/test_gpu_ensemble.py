@@ -10,7 +10,7 @@ from mahyco_amd import types as T
 pytestmark = pytest.mark.gpu
 
 
-def _mix(seed, n=10, stiff=False, dim3=False, o3=False):
+def _mix(seed, n=10, stiff=False, dim3=False, o3=False, pb=False):
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
@@ -22,6 +22,9 @@ def _mix(seed, n=10, stiff=False, dim3=False, o3=False):
             c = cases.riemann2d(nx, nx, rng, stiffened=stiff, name=f"m{i}")
         if o3 and i % 3 == 0:
             c.ordre_projection = 3  # order-3 remap: a batch group of its own
+        if pb and i % 3 == 1:  # pente-borne and generalised limiters: another group
+            c.pente_borne, c.debar_fix, c.pente_borne_mixte = True, 1 + i % 2, True
+            c.limiter = T.MH_SUPERBEE_G if i % 2 else T.MH_MINMOD
         out.append(c)
     return out
 
@@ -43,7 +46,7 @@ def _alone(case, ncycles):
 def test_alone_equals_mixed(graph, stiff, dim3, o3):
     from mahyco_amd.lib import Ensemble
     print(f"[ensemble] graph={graph} stiff={stiff} dim3={dim3} o3={o3}", flush=True)
-    mix = _mix((3 if graph else 4) + (2 if o3 else 0), stiff=stiff, dim3=dim3, o3=o3)
+    mix = _mix((3 if graph else 4) + (2 if o3 else 0), stiff=stiff, dim3=dim3, o3=o3, pb=o3)
     ncyc = 30
     E = Ensemble(16, 4e9)
     slots = [E.add(c, max_iterations=ncyc) for c in mix]

@@ -29,6 +29,13 @@ def _riemann3d(seed, n=14):
     return cases.riemann3d(n, rng, stiffened=True, t_end=1e30, name=f"riemann3d_{seed}")
 
 
+def _variant(c, **kw):
+    for key, v in kw.items():
+        setattr(c, key, v)
+    c.name += "_" + "_".join(f"{key}{v}" for key, v in kw.items())
+    return c
+
+
 CASES = {
     "bisod_x_fixture": (cases.bisod_x, 400),       # runs to the final time (183 iterations)
     "bisod_y_fixture": (cases.bisod_y, 80),
@@ -50,6 +57,21 @@ CASES = {
     "bisod3d_ordre3_32x16x16": (lambda: cases.ordre3(lambda: cases.config2_bisod3d(32, 16, 16), "b3d_o3"), 100),
     "riemann2d_stiffened_ordre3": (lambda: cases.ordre3(lambda: _riemann2d(True, 14), "r2d_o3"), 150),
     "riemann3d_stiffened_ordre3": (lambda: cases.ordre3(lambda: _riemann3d(15), "r3d_o3"), 60),
+    # pente-borne fluxes (UtilesRemap.cc:425-832) and generalised limiters (:130-183)
+    "bisod_x_pente_borne_debar1": (lambda: cases.pente_borne(cases.bisod_x, "pb1", 1), 400),
+    "bisod_y_pente_borne_superbee_debar2": (lambda: cases.pente_borne(cases.bisod_y, "pb2", 2, T.MH_SUPERBEE), 120),
+    "bisod_z_pente_borne_vanleer_debar0": (lambda: cases.pente_borne(cases.bisod_z, "pb0", 0, T.MH_VANLEER), 120),
+    "pointtriple_pb_superbeeG_fixture": (lambda: cases.mono_pointtriple(False), 400),
+    "pointtriple_pb_sp_superbeeG_fixture": (lambda: cases.mono_pointtriple(True), 400),
+    "riemann3d_pb_mixte_purelim1_vanleerG": (lambda: _variant(_riemann3d(16), pente_borne=True, debar_fix=1,
+                                                              pente_borne_mixte=True, limiter=T.MH_VANLEER_G,
+                                                              limiter_pure=T.MH_SUPERBEE), 60),
+    "riemann2d_minmodG": (lambda: _variant(_riemann2d(True, 17), limiter=T.MH_MINMOD_G), 150),
+    "riemann2d_vanleerG_simple_pente": (lambda: _variant(_riemann2d(True, 18), limiter=T.MH_VANLEER_G,
+                                                         simple_pente=True), 150),
+    "riemann2d_arithmeticG_pb": (lambda: _variant(_riemann2d(False, 19), limiter=T.MH_ARITHMETIC_G,
+                                                  pente_borne=True, debar_fix=2), 150),
+    "riemann3d_ultrabeeG": (lambda: _variant(_riemann3d(20), limiter=T.MH_ULTRABEE_G), 60),
 }
 
 

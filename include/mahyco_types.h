@@ -122,6 +122,9 @@ typedef struct MhOptions {
   int pente_borne_mixte;           /* projection-pente-borne-mixte: pure-neighbourhood variant */
   int projection_limiteur_pure_id; /* projection-limiteur-pure-id (MhLimiter) */
   int projection_simple_pente;     /* projection-simple-pente */
+  /* Lagrange energy update variants (Mahyco.axl:476-488; default 0) */
+  int with_newton;        /* with-newton: Newton iterations on the energy equation */
+  int pression_explicite; /* pression-explicite: explicit pressure work */
 } MhOptions;
 
 /* Field identifiers for get/set.  Cell fields take an environment index

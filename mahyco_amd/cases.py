@@ -71,6 +71,8 @@ class Case:
     pente_borne_mixte: bool = False
     limiter_pure: int = T.MH_MINMOD
     simple_pente: bool = False
+    with_newton: bool = False
+    pression_explicite: bool = False
 
     @property
     def nb_cells(self) -> int:
@@ -136,6 +138,8 @@ class Case:
         o.pente_borne_mixte = int(self.pente_borne_mixte)
         o.projection_limiteur_pure_id = self.limiter_pure
         o.projection_simple_pente = int(self.simple_pente)
+        o.with_newton = int(self.with_newton)
+        o.pression_explicite = int(self.pression_explicite)
         return o
 
 
@@ -209,6 +213,13 @@ def pente_borne(make, name, debar, limiter=T.MH_MINMOD) -> Case:
     (projection-pente-borne) with the given debar fix and limiter."""
     c = make()
     c.name, c.pente_borne, c.debar_fix, c.limiter = name, True, debar, limiter
+    return c
+
+
+def lag2d_sod_x_newton() -> Case:
+    """NONREGRESSION/CAS_2D_Lag_SodCaseX_newton/Donnees.arc (energy by Newton iterations)."""
+    c = lag2d_sod_x()
+    c.name, c.with_newton = "CAS_2D_Lag_SodCaseX_newton", True
     return c
 
 

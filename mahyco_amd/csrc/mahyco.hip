@@ -925,6 +925,9 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
   S.lim_pure = options->projection_limiteur_pure_id;
   S.simple_pente = options->projection_simple_pente ? 1 : 0;
   S.nsweep = (g.nx > 1) + (g.ny > 1) + (g.dim == 3 && g.nz > 1);
+  S.newton = options->with_newton ? 1 : 0;
+  S.p_explicit = options->pression_explicite ? 1 : 0;
+  S.thr_module = options->threshold;
   S.with_proj = options->with_projection ? 1 : 0;
   S.vitson_max = options->calcul_vitson_max ? 1 : 0;
   S.nbc = options->nb_bc;
@@ -1602,7 +1605,8 @@ static bool same_options(const MhOptions& a, const MhOptions& b) {
             a.projection_pente_borne == b.projection_pente_borne && a.pente_borne_debar_fix == b.pente_borne_debar_fix &&
             a.pente_borne_mixte == b.pente_borne_mixte &&
             a.projection_limiteur_pure_id == b.projection_limiteur_pure_id &&
-            a.projection_simple_pente == b.projection_simple_pente &&
+            a.projection_simple_pente == b.projection_simple_pente && a.with_newton == b.with_newton &&
+            a.pression_explicite == b.pression_explicite &&
             a.projection_limiteur_id == b.projection_limiteur_id && a.remap_threshold == b.remap_threshold &&
             a.cas_model == b.cas_model && a.cas_test == b.cas_test && a.interface_position == b.interface_position &&
             a.interface_axis == b.interface_axis && a.sedov_deposit == b.sedov_deposit;

@@ -431,6 +431,29 @@ __device__ __forceinline__ double gp_energy(double e_n, double p, double rho, do
   return numer / denom;
 }
 
+// Energy of one environment after the Lagrange step: updateEnergyAndPressureforGP (closed
+// form above), updateEnergyAndPressurebyNewton (MahycoModule.cc:1506-1540: Newton on fvnr,
+// MahycoModule.h:378-386, with the EOS of applyOneCellEOS at each iterate) or
+// updateEnergyAndPressureExplicite (:1484-1502).  Newton's last pressure and sound speed are
+// replaced by the EOS call that follows the update (or the remap), so only e is returned.
+__device__ __forceinline__ double lag_energy(const MhDev& S, const MhEnvParam& ep, double e_n, double p_n, double rho,
+                                             double rho_n, double q) {
+  if (S.p_explicit) return e_n - (p_n + q) * (1.0 / rho - 1.0 / rho_n);
+  if (!S.newton) return gp_energy(e_n, p_n, rho, rho_n, q, ep.gamma);
+  double pseudo = 0.;
+  if (q * (1.0 / rho - 1.0 / rho_n) < 0.) pseudo = q;
+  const double g = ep.gamma;
+  double e = e_n, p = p_n;
+  for (int i = 0; i < 50; ++i) {
+    if (!(fabs(e - e_n + 0.5 * (p + p_n + 2. * pseudo) * (1. / rho - 1. / rho_n)) >= S.thr_module)) break;
+    p = ep.stiffened ? ((g - 1.) * rho * e) - (g * ep.pinf) : (g - 1.) * rho * e;
+    const double dpde = (g - 1.) * rho;
+    const double f = e - e_n + 0.5 * (p + p_n + 2. * pseudo) * (1. / rho - 1. / rho_n);
+    e = e - f / (1. + 0.5 * dpde * (1. / rho - 1. / rho_n));
+  }
+  return e;
+}
+
 // applyEOS (PerfectGasEOSService.cc:45-67, StiffenedGasEOSService.cc:43-63)
 __device__ __forceinline__ void eos(const MhEnvParam& ep, double rho, double e, double& p, double& c) {
   const double g = ep.gamma;
@@ -482,7 +505,7 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
       const MhEnvParam& ep = S.env[ke];
       double q = 0.;
       if (divu < 0.0) q = 1. / tau * (-ep.a1 * L * cs * divu + ep.a2 * L * L * divu * divu);
-      const double e = gp_energy(S.e[c], S.p[c], rho, rho_n, q, ep.gamma);
+      const double e = lag_energy(S, ep, S.e[c], S.p[c], rho, rho_n, q);
       if (S.with_proj) {
         for (int kk = 0; kk < M; ++kk)
           if (kk == ke) {
@@ -521,7 +544,7 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
         const double tk = 0.5 * (1.0 / rkn + 1.0 / rk[kk]);
         qk[kk] = 0.;
         if (divu < 0.0) qk[kk] = 1. / tk * (-ep.a1 * L * cs * divu + ep.a2 * L * L * divu * divu);
-        ek[kk] = gp_energy(S.ek[o], S.pk[o], rk[kk], rkn, qk[kk], ep.gamma);
+        ek[kk] = lag_energy(S, ep, S.ek[o], S.pk[o], rk[kk], rkn, qk[kk]);
       }
       for (int kk = 0; kk < M; ++kk)
         if ((mk >> kk) & 1) qsum += qk[kk] * fk[kk];

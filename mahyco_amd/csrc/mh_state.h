@@ -92,6 +92,9 @@ struct MhDev {
   int lim_pure;      // projection-limiteur-pure-id
   int simple_pente;  // projection-simple-pente
   int nsweep;        // directions with more than one cell
+  int newton;        // with-newton: energy of the Lagrange step by Newton iterations
+  int p_explicit;    // pression-explicite
+  double thr_module; // module threshold (Newton tolerance)
   int nblk_c, nblk_n;  // blocks of MH_BLOCK threads over cells / nodes
   int max_iter;        // > 0: stop after this many iterations (fixed-horizon runs)
   int hist_period;     // > 0: EnvSummation rows every hist_period iterations

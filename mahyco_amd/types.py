@@ -89,4 +89,5 @@ class MhOptions(C.Structure):
                 ("interface_axis", C.c_int), ("sedov_deposit", C.c_double),
                 ("projection_pente_borne", C.c_int), ("pente_borne_debar_fix", C.c_int),
                 ("pente_borne_mixte", C.c_int), ("projection_limiteur_pure_id", C.c_int),
-                ("projection_simple_pente", C.c_int)]
+                ("projection_simple_pente", C.c_int), ("with_newton", C.c_int),
+                ("pression_explicite", C.c_int)]

@@ -815,16 +815,49 @@ void OracleCase::computeArtificialViscosity() {
 
 // MahycoModule::updateEnergyAndPressure :1430-1480 -> updateEnergyAndPressureforGP :1623-1659
 // (no CSTS, pseudo-centree off; the gamma of the env EOS for StiffenedGas too).
+// MahycoModule::fvnr / fvnrderiv, MahycoModule.h:378-386: the energy equation of the
+// Lagrange step and its derivative in e.
+static inline double fvnr(double e, double p, double en, double qnn1, double pn, double rn1, double rn) {
+  return e - en + 0.5 * (p + pn + 2. * qnn1) * (1. / rn1 - 1. / rn);
+}
+static inline double fvnrderiv(double dpde, double rn1, double rn) { return 1. + 0.5 * dpde * (1. / rn1 - 1. / rn); }
+
+// updateEnergyAndPressure :1430-1480 with updateEnergyAndPressureforGP :1622-1660 (non-CSTS,
+// pseudo not centred), updateEnergyAndPressurebyNewton :1506-1540 and
+// updateEnergyAndPressureExplicite :1484-1502.  Newton leaves the pressure and sound speed
+// of its last EOS call, which the EOS call after the update (or after the remap) replaces,
+// so only the energy is kept here.
 void OracleCase::updateEnergyAndPressure() {
   for (int k = 0; k < m; ++k) {
     const double adiabatic_cst = env[k].adiabatic_cst;
+    const double pi = env[k].limit_tension;
+    const bool sg = env[k].eos == MH_EOS_STIFFENED_GAS;
 #pragma omp parallel for schedule(static)
     for (long c = 0; c < nc; ++c) {
       if (!member(k, c)) continue;
-      double pseudo_v = 0.;
       const double rho = ev(density, k, c), rhon = ev(density_n, k, c);
       const double q = ev(pseudo, k, c);
+      if (opt.pression_explicite) {
+        ev(internal_energy, k, c) -= (ev(pressure_n, k, c) + q) * (1.0 / rho - 1.0 / rhon);
+        continue;
+      }
+      double pseudo_v = 0.;
       if (q * (1.0 / rho - 1.0 / rhon) < 0.) pseudo_v = q;
+      if (opt.with_newton) {
+        const double rn = rhon, pn = ev(pressure_n, k, c), en = ev(internal_energy_n, k, c);
+        double e = ev(internal_energy, k, c), p = pn, dpde = 0.;
+        int i = 0;
+        while (i < 50 && std::fabs(fvnr(e, p, en, pseudo_v, pn, rho, rn)) >= opt.threshold) {
+          // applyOneCellEOS (PerfectGasEOSService.cc:71-88, StiffenedGasEOSService.cc:67-83)
+          p = sg ? ((adiabatic_cst - 1.) * rho * e) - (adiabatic_cst * pi) : (adiabatic_cst - 1.) * rho * e;
+          dpde = (adiabatic_cst - 1.) * rho;
+          const double enew = e - fvnr(e, p, en, pseudo_v, pn, rho, rn) / fvnrderiv(dpde, rho, rn);
+          e = enew;
+          i++;
+        }
+        ev(internal_energy, k, c) = e;
+        continue;
+      }
       const double denom = 1 + 0.5 * (adiabatic_cst - 1.0) * rho * (1.0 / rho - 1.0 / rhon);
       const double numer = ev(internal_energy_n, k, c) - (0.5 * ev(pressure, k, c) + pseudo_v) * (1.0 / rho - 1.0 / rhon);
       ev(internal_energy, k, c) = numer / denom;

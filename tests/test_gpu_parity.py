@@ -72,6 +72,11 @@ CASES = {
     "riemann2d_arithmeticG_pb": (lambda: _variant(_riemann2d(False, 19), limiter=T.MH_ARITHMETIC_G,
                                                   pente_borne=True, debar_fix=2), 150),
     "riemann3d_ultrabeeG": (lambda: _variant(_riemann3d(20), limiter=T.MH_ULTRABEE_G), 60),
+    # energy of the Lagrange step by Newton iterations or with explicit pressure work
+    "sod2d_lagrange_newton_fixture": (cases.lag2d_sod_x_newton, 3000),
+    "riemann3d_stiffened_newton": (lambda: _variant(_riemann3d(21), with_newton=True), 60),
+    "riemann2d_explicit_pressure": (lambda: _variant(_riemann2d(True, 22), pression_explicite=True), 150),
+    "sod2d_lagrange_explicit_pressure": (lambda: _variant(cases.config1_sod2d_lagrange(), pression_explicite=True), 800),
 }
 
 

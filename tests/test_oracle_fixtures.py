@@ -65,6 +65,8 @@ FIXTURES = {
     # pente-borne on the 3-material triple point with the generalised superbee limiter
     "CAS_MonoPointtriple_pb_mixte_superbeeG_superbeeG": lambda: cases.mono_pointtriple(False),
     "CAS_MonoPointtriple_pb_sp_mixte_superbeeG_superbeeG": lambda: cases.mono_pointtriple(True),
+    # energy of the Lagrange step by Newton iterations (MahycoModule.cc:1506-1540)
+    "CAS_2D_Lag_SodCaseX_newton": cases.lag2d_sod_x_newton,
     # (CAS_BiSodCaseZ_pente_borne_superbee_debar_2 is left out: its output holds four EnSight
     # steps, the last at t = 0.1973 < final-time, and no time history; the oracle reaches the
     # times of the first two steps at iterations 50 and 100 exactly, but the stored fields
@@ -163,7 +165,8 @@ def test_lagrange_fixtures_bit_exact(runs):
     every 17-digit time-history value is equal."""
     # (the sequential tubes along y, CAS_Lag_BiSodCaseY and CAS_Lag_SodCaseY, differ in the
     # last printed digit of a few averages and are held to the 1e-12 bar only)
-    for name in ("CAS_Lag_BiSodCaseX", "CAS_2D_Lag_SodCaseX", "CAS_Lag_SodCaseZ", "CAS_2D_Lag_SodCaseZ"):
+    for name in ("CAS_Lag_BiSodCaseX", "CAS_2D_Lag_SodCaseX", "CAS_Lag_SodCaseZ", "CAS_2D_Lag_SodCaseZ",
+                 "CAS_2D_Lag_SodCaseX_newton"):
         o, st = _run(name, runs)
         ref = np.load(os.path.join(GOLDEN, name + ".npz"))
         th = o.history()

@@ -167,3 +167,71 @@ def test_first_order_remap_is_donor_cell():
     o.run(80)
     rho = o.get(T.MH_F_DENSITY).reshape(c.cell_shape)[0, 0, :]
     assert np.all(rho > 0.12) and np.all(rho < 1.01)
+
+
+def test_newton_energy_converges_to_closed_form():
+    """with-newton: for a perfect gas the energy equation of the Lagrange step (fvnr,
+    MahycoModule.h:378-386) is linear in e and its root is the closed form of
+    updateEnergyAndPressureforGP; Newton iterated to the module threshold lands within
+    rounding of it every cycle, so whole runs agree to ~1e-9 (a dropped or mis-signed term
+    in fvnr or fvnrderiv moves them by far more, or stops the iteration early)."""
+    from oracle.pyoracle import OracleCase
+    res = []
+    for newton in (False, True):
+        c = cases.riemann2d(30, 6, np.random.default_rng(4), stiffened=False)
+        c.with_projection, c.with_newton = False, newton
+        o = OracleCase(c)
+        o.run(120)
+        res.append((o.get(T.MH_F_INTERNAL_ENERGY), o.get(T.MH_F_PRESSURE), o.time))
+        o.close()
+    (e0, p0, t0), (e1, p1, t1) = res
+    assert abs(t1 - t0) <= 1e-12 * t0
+    np.testing.assert_allclose(e1, e0, rtol=1e-8)
+    np.testing.assert_allclose(p1, p0, rtol=1e-8)
+
+
+def test_newton_energy_stiffened_gas_root():
+    """with-newton on a stiffened gas, one Lagrange step from the initial state: the energy
+    equation e - e_n + (p(e) + p_n + 2 q)/2 (1/rho - 1/rho_n) = 0 with
+    p(e) = (gamma - 1) rho e - gamma pi is linear, so each pure cell's energy is its root
+    (e_n - (p_n + 2 q - gamma pi)/2 dv) / (1 + (gamma - 1) rho dv / 2), q counted only when
+    it works against compression (q dv < 0)."""
+    from oracle.pyoracle import OracleCase
+    c = cases.riemann2d(30, 6, np.random.default_rng(5), stiffened=True)
+    c.with_projection, c.with_newton = False, True
+    o = OracleCase(c)
+    rho_n, p_n, e_n = o.get(T.MH_F_DENSITY), o.get(T.MH_F_PRESSURE), o.get(T.MH_F_INTERNAL_ENERGY)
+    mask = o.get(T.MH_F_ENV_MASK).astype(int)
+    o.run(1)
+    rho, q, e = o.get(T.MH_F_DENSITY), o.get(T.MH_F_PSEUDO_VISCOSITY), o.get(T.MH_F_INTERNAL_ENERGY)
+    o.close()
+    pure = (mask == 1) | (mask == 2)
+    env = np.where(mask == 2, 1, 0)
+    gamma = np.array([c.envs[k].adiabatic_cst for k in env])
+    pi = np.array([c.envs[k].limit_tension for k in env])
+    dv = 1.0 / rho - 1.0 / rho_n
+    qw = np.where(q * dv < 0., q, 0.)
+    root = (e_n - 0.5 * (p_n + 2 * qw - gamma * pi) * dv) / (1 + 0.5 * (gamma - 1) * rho * dv)
+    assert pure.sum() > 100
+    np.testing.assert_allclose(e[pure], root[pure], rtol=1e-9, atol=1e-9)
+
+
+def test_explicit_pressure_is_consistent_first_order():
+    """pression-explicite (updateEnergyAndPressureExplicite, MahycoModule.cc:1484-1502)
+    replaces the implicit p^{n+1} work by p^n: the two schemes solve the same equations, so
+    on a smooth compression their difference after a fixed time shrinks with the time step
+    (first order), while a wrong sign or a missing pseudo term would leave an O(1) gap."""
+    from oracle.pyoracle import OracleCase
+    gaps = []
+    for dtmax in (4e-4, 1e-4):
+        out = []
+        for explicit in (False, True):
+            c = cases.lag2d_sod_x()
+            c.deltat_max, c.final_time, c.pression_explicite = dtmax, 0.04, explicit
+            o = OracleCase(c)
+            o.run(10 ** 6)
+            out.append(o.get(T.MH_F_INTERNAL_ENERGY))
+            o.close()
+        gaps.append(np.abs(out[1] - out[0]).mean())
+    assert gaps[1] < 0.5 * gaps[0], gaps
+    assert gaps[1] < 1e-2

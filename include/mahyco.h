@@ -123,6 +123,12 @@ int mahyco_history_write_gnuplot(MhHandle* h, const char* dir, const char* const
 int mahyco_checkpoint_write(MhHandle* h, const char* path);
 int mahyco_checkpoint_read(MhHandle* h, const char* path);
 
+/* Restart entry point MahycoModule::hydroContinueInit (MahycoModule.cc:273-306), called after
+ * mahyco_checkpoint_read to continue as the reference does: the continued run projects only
+ * 3m+5 remap slots (:288), so with two or more environments the pseudo-viscosity slot 7m is
+ * not swept and V q keeps its Lagrangian value through every later remap. */
+int mahyco_continue_init(MhHandle* h);
+
 /* ---- ensembles (no reference counterpart: many independent cases on one GPU) ---------
  * Cases of any size advance together, each with its own time step and stop rule: one
  * launch per kernel phase covers every active case of a (dimension, environment count)

@@ -1658,6 +1658,14 @@ int mahyco_checkpoint_write(MhHandle* h, const char* path) {
   return ok ? MH_OK : MH_ERR_INVALID;
 }
 
+// Captured graphs hold the kernel arguments (the MhDev copy) of their capture.
+static void drop_graphs(MhHandle* h) {
+  for (auto& kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto gdef : h->graph_defs) (void)hipGraphDestroy(gdef);
+  h->graphs.clear();
+  h->graph_defs.clear();
+}
+
 int mahyco_checkpoint_read(MhHandle* h, const char* path) {
   if (!h || !path || h->ext_sc || h->phase != 0) return MH_ERR_INVALID;
   FILE* f = fopen(path, "rb");
@@ -1686,10 +1694,14 @@ int mahyco_checkpoint_read(MhHandle* h, const char* path) {
   h->launched = hd.launched;
   h->sens = hd.sens;
   h->dev.max_iter = hd.max_iter;
-  for (auto& kv : h->graphs) (void)hipGraphExecDestroy(kv.second);  // they hold the old kernel arguments
-  for (auto gdef : h->graph_defs) (void)hipGraphDestroy(gdef);
-  h->graphs.clear();
-  h->graph_defs.clear();
+  drop_graphs(h);  // they hold the old kernel arguments
+  return MH_OK;
+}
+
+int mahyco_continue_init(MhHandle* h) {
+  if (!h || h->ext_sc || h->phase != 0) return MH_ERR_INVALID;
+  h->dev.q_frozen = (h->dev.with_proj && 7 * h->m >= 3 * h->m + 5) ? 1 : 0;
+  drop_graphs(h);
   return MH_OK;
 }
 

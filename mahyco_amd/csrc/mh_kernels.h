@@ -514,6 +514,7 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
             u_out[2 * M + kk] = V * rho * e;
           }
         u_out[3 * M] = V * q;
+        if (S.q_frozen) S.q[c] = q;  // the Lagrangian pseudo, for the remap's unswept V q slot
         if (S.pb) {  // pente-borne phi of the first sweep (PrepareRemap.cc:138-155)
           S.phi0[ke * nc + c] = S.fg[c];
           S.phi0[(M + ke) * nc + c] = rho;
@@ -558,6 +559,7 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
           u_out[2 * M + kk] = Vk[kk] * rk[kk] * ek[kk];
         }
         u_out[3 * M] = V * qsum;
+        if (S.q_frozen) S.q[c] = qsum;
         if (S.pb) {
           for (int kk = 0; kk < M; ++kk) {
             if (!((mk >> kk) & 1)) continue;
@@ -2130,7 +2132,7 @@ __device__ __forceinline__ void k_finalize_body(const MhDev& S, int lb, int usel
       if (!((nm >> kk) & 1)) continue;
       Mk[kk] = chi[kk] * M_g;
       if (pure) M_g = Mk[kk];
-      qk[kk] = U[3 * M] * unsurvol;
+      qk[kk] = (S.q_frozen ? S.Vl[c] * S.q[c] : U[3 * M]) * unsurvol;
       if (pure) rho_g = dk[kk];
       energie += chi[kk] * ek[kk];
       pseudo += f[kk] * qk[kk];

@@ -95,6 +95,9 @@ struct MhDev {
   int newton;        // with-newton: energy of the Lagrange step by Newton iterations
   int p_explicit;    // pression-explicite
   double thr_module; // module threshold (Newton tolerance)
+  int q_frozen;      // run continued from a dump with m >= 2: V q keeps its Lagrangian value
+                     // through the remap (hydroContinueInit, MahycoModule.cc:288)
+  int pad4;
   int nblk_c, nblk_n;  // blocks of MH_BLOCK threads over cells / nodes
   int max_iter;        // > 0: stop after this many iterations (fixed-horizon runs)
   int hist_period;     // > 0: EnvSummation rows every hist_period iterations

@@ -91,6 +91,8 @@ def load():
     for name in ("mahyco_checkpoint_write", "mahyco_checkpoint_read"):
         getattr(lib, name).argtypes = [H, C.c_char_p]
         getattr(lib, name).restype = C.c_int
+    lib.mahyco_continue_init.argtypes = [H]
+    lib.mahyco_continue_init.restype = C.c_int
     lib.mahyco_apply_eos.argtypes = [H, C.c_int]
     lib.mahyco_apply_eos.restype = C.c_int
     lib.mahyco_compute_pression_moyenne.argtypes = [H]
@@ -200,6 +202,12 @@ class GpuCase:
         st = self.lib.mahyco_checkpoint_read(self.h, str(path).encode())
         if st != T.MH_OK:
             raise MahycoError(st, "mahyco_checkpoint_read")
+
+    def continue_init(self):
+        """Continue as the reference does after a restart (hydroContinueInit)."""
+        st = self.lib.mahyco_continue_init(self.h)
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_continue_init")
 
     def profile(self, on=True):
         self.lib.mahyco_profile_enable(self.h, int(on))

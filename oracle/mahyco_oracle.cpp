@@ -121,6 +121,10 @@ struct OracleCase {
   std::vector<double> h_cell_lagrange;       // cell width along the sweep
   std::vector<int> est_mixte, est_pure;      // mixed-cell indicators; Arcane starts them at 0
   std::vector<double> dphi_ar, dphi_av;      // pente-borne fluxes through the back / front face [nc][nv]
+  // A run continued from a dump projects only its first 3m+5 remap slots
+  // (hydroContinueInit, MahycoModule.cc:288): with m >= 2 the pseudo-viscosity slot 7m is
+  // not swept, so V q keeps its Lagrangian value through the remap.
+  bool q_frozen = false;
   std::vector<R3> face_length_lagrange;
   std::vector<double> back_flux_mass, front_flux_mass;
   int sens_projection = 0;
@@ -1437,6 +1441,7 @@ void OracleCase::computeUremap(int idir) {
         for (int v = 0; v < nv; ++v) {
           // pente-borne: phi_face already holds the integrated flux
           double flux = pb ? od * len * phi_face[(long)f * nv + v] : od * fnv * len * deltat * phi_face[(long)f * nv + v];
+          if (q_frozen && v == 3 * m) flux = 0.;
           flux_face[v] += flux;
           dual_phi_flux[c * nv + v] += 0.5 * flux * comp(outer, idir);
         }
@@ -1464,7 +1469,8 @@ void OracleCase::computeUremap(int idir) {
       }
       for (int k = 0; k < m; ++k) P[2 * m + k] = U[m + k] != 0. ? U[2 * m + k] / U[m + k] : 0.;
     } else {
-      for (int v = 0; v < nv; ++v) P[v] = U[v] / somme_volume;
+      for (int v = 0; v < nv; ++v)
+        if (!(q_frozen && v == 3 * m)) P[v] = U[v] / somme_volume;
     }
     setMixedFlags(c, P);  // :559-573
   }
@@ -1884,6 +1890,10 @@ OracleCase* oracle_create(const MhMesh* mesh, const MhEnvironment* envs, int nb_
 }
 
 void oracle_destroy(OracleCase* h) { delete h; }
+
+// hydroContinueInit (MahycoModule.cc:273-306): the state continues as dumped; the remap
+// then carries 3m+5 slots, which leaves the pseudo-viscosity slot out when m >= 2.
+void oracle_continue(OracleCase* h) { h->q_frozen = 7 * h->m >= 3 * h->m + 5; }
 
 int oracle_run(OracleCase* h, int max_cycles) {
   for (int i = 0; i < max_cycles; ++i) {

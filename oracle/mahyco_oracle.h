@@ -26,6 +26,8 @@ void oracle_destroy(OracleCase* h);
 /* Runs up to max_cycles iterations of the time loop (MahycoLoop or MahycoLagrangeLoop).
  * Returns MH_OK, MH_STOPPED (final time reached) or a negative MhStatus. */
 int oracle_run(OracleCase* h, int max_cycles);
+/* The run continues as after a restart from a dump (hydroContinueInit, MahycoModule.cc:273-306). */
+void oracle_continue(OracleCase* h);
 
 double oracle_time(const OracleCase* h);
 double oracle_deltat(const OracleCase* h);

@@ -33,6 +33,8 @@ def load():
     lib.oracle_destroy.argtypes = [C.c_void_p]
     lib.oracle_run.argtypes = [C.c_void_p, C.c_int]
     lib.oracle_run.restype = C.c_int
+    lib.oracle_continue.argtypes = [C.c_void_p]
+    lib.oracle_continue.restype = None
     for n in ("oracle_time", "oracle_deltat"):
         getattr(lib, n).restype = C.c_double
         getattr(lib, n).argtypes = [C.c_void_p]
@@ -80,6 +82,10 @@ class OracleCase:
 
     def run(self, ncycles):
         return self.lib.oracle_run(self.h, ncycles)
+
+    def continue_after_restart(self):
+        """From here on the run behaves as one continued from a dump (hydroContinueInit)."""
+        self.lib.oracle_continue(self.h)
 
     @property
     def time(self):

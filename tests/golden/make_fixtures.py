@@ -59,6 +59,7 @@ CASES = {
     "CAS_BiSodCaseY_pente_borne_superbee_debar_2": (3, (5, 100, 5), (0, 0, 0), (.1, 1., .1)),
     "CAS_MonoPointtriple_pb_mixte_superbeeG_superbeeG": (2, (70, 30, 1), (0, 0, 0), (.07, .03, 0.)),
     "CAS_2D_Lag_SodCaseX_newton": (2, (100, 5, 1), (0, 0, 0), (1., .1, 0.)),
+    "CAS_SodCaseX": (3, (100, 5, 5), (0, 0, 0), (1., .1, .1)),
     "CAS_MonoPointtriple_pb_sp_mixte_superbeeG_superbeeG": (2, (70, 30, 1), (0, 0, 0), (.07, .03, 0.)),
 }
 # Runs whose output holds no geometry: parts are the subdomains of the Cartesian generator

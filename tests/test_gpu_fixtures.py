@@ -8,19 +8,27 @@ import numpy as np
 import pytest
 
 from mahyco_amd import types as T
-from tests.test_oracle_fixtures import CHAOTIC, ENSIGHT, FIXTURES, GOLDEN, TH, _printed, th_bar
+from tests.test_oracle_fixtures import CHAOTIC, ENSIGHT, FIXTURES, GOLDEN, RESTART_AFTER, TH, _printed, th_bar
 
 pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("name", list(FIXTURES))
-def test_gpu_matches_reference_fixture(name):
+def test_gpu_matches_reference_fixture(name, tmp_path):
     from mahyco_amd.lib import GpuCase
     print(f"[gpu fixture] {name}", flush=True)
     ref = np.load(os.path.join(GOLDEN, name + ".npz"))
     case = FIXTURES[name]()
     g = GpuCase(case)
     g.history_enable(10, 1024)
+    if name in RESTART_AFTER:  # stop, dump, restore into a new case, continue as the reference
+        g.run(RESTART_AFTER[name], graph=6)
+        g.checkpoint_write(tmp_path / "dump")
+        g.close()
+        g = GpuCase(case)
+        g.history_enable(10, 1024)
+        g.checkpoint_read(tmp_path / "dump")
+        g.continue_init()
     st = T.MH_OK
     while st == T.MH_OK:
         st = g.run(500, graph=6)

@@ -110,3 +110,44 @@ def test_checkpoint_restart_is_bitwise(make, tmp_path):
     assert d.iteration == 0
     for x in (a, c, d):
         x.close()
+
+
+@pytest.mark.parametrize("nenv", [1, 2])
+def test_continue_init_matches_oracle(nenv, tmp_path):
+    """mahyco_checkpoint_read + mahyco_continue_init continue as the reference's restart does
+    (hydroContinueInit, MahycoModule.cc:288: 3m+5 projected slots): bitwise equal to the
+    oracle continued the same way; with one environment the pseudo slot is still projected
+    and the run equals an uninterrupted one."""
+    import numpy as np
+    from mahyco_amd import cases
+    from mahyco_amd.lib import GpuCase
+    from oracle.pyoracle import OracleCase
+    from tests.helpers import bitwise_equal, fields
+    if nenv == 1:
+        case = cases.config3_sedov3d(12)
+    else:
+        case = cases.riemann3d(12, np.random.default_rng(32), stiffened=True, t_end=1e30, name="restart2")
+    print(f"[continue] m={len(case.envs)}", flush=True)
+    g = GpuCase(case)
+    g.run(7, graph=6)
+    g.checkpoint_write(tmp_path / "dump")
+    g.close()
+    g = GpuCase(case)
+    g.checkpoint_read(tmp_path / "dump")
+    g.continue_init()
+    g.run(40, graph=6)
+    o = OracleCase(case, pow_mode=1)
+    o.run(7)
+    o.continue_after_restart()
+    o.run(40)
+    assert g.iteration == o.iteration and g.time == o.time
+    fg, fo = fields(g, len(case.envs), 3), fields(o, len(case.envs), 3)
+    for k in fo:
+        assert bitwise_equal(fg[k], fo[k]), k
+    if nenv == 1:
+        u = OracleCase(case, pow_mode=1)
+        u.run(47)
+        fu = fields(u, 1, 3)
+        for k in fo:
+            assert bitwise_equal(fu[k], fo[k]), k
+    g.close()

@@ -35,7 +35,7 @@ enum MhStatus {
 };
 
 /* Equation-of-state services (IEquationOfState implementations). */
-enum MhEos { MH_EOS_PERFECT_GAS = 0, MH_EOS_STIFFENED_GAS = 1 };
+enum MhEos { MH_EOS_PERFECT_GAS = 0, MH_EOS_STIFFENED_GAS = 1, MH_EOS_FICTIF = 2 /* eos/fictif/FictifEOSService.cc */ };
 
 /* Face groups built by PrepareFaceGroup (MahycoAnnexe.cc:149-235). */
 enum MhSurface { MH_XMIN = 0, MH_XMAX = 1, MH_YMIN = 2, MH_YMAX = 3, MH_ZMIN = 4, MH_ZMAX = 5 };
@@ -62,7 +62,8 @@ enum MhCaseModel {
   MH_CASE_SOD = 0,         /* casTest/SOD/SODService.cc; cas_test 3,4,5 (mono) 13,14,15 (bi) */
   MH_CASE_SEDOV = 1,       /* casTest/SEDOV/SEDOVService.cc; cas_test 1 (mono) */
   MH_CASE_RIEMANN = 2,     /* two-state shock tube with geometric-overlap volume fractions */
-  MH_CASE_POINTTRIPLE = 3  /* casTest/POINTTRIPLE/POINTTRIPLEService.cc; cas_test 12 (3 envs) */
+  MH_CASE_POINTTRIPLE = 3, /* casTest/POINTTRIPLE/POINTTRIPLEService.cc; cas_test 12 (3 envs) */
+  MH_CASE_RIDER = 4        /* casTest/RIDER/RIDERService.cc; cas_test 23-30 (mono), 31-38 (bubble in 2 envs) */
 };
 
 /* Node coordinate convention of the Cartesian generator. */
@@ -86,6 +87,10 @@ typedef struct MhEnvironment {
   double densite_initiale;       /* densite-initiale */
   double pression_initiale;      /* pression-initiale */
   double vitesse_initiale[3];    /* vitesse-initiale */
+  double energie_initiale;       /* energie-initiale (Mahyco.axl:295, default 0) */
+  /* Fictif EOS (eos/fictif/FictifEOS.axl): p = valeur-debut-pression +
+   * valeur-dependance-temps t after tdebut-pression, 0 after tfin-pression */
+  double fictif_tdebut, fictif_tfin, fictif_p0, fictif_dpdt;
 } MhEnvironment;
 
 typedef struct MhBoundaryCondition {
@@ -125,6 +130,13 @@ typedef struct MhOptions {
   /* Lagrange energy update variants (Mahyco.axl:476-488; default 0) */
   int with_newton;        /* with-newton: Newton iterations on the energy equation */
   int pression_explicite; /* pression-explicite: explicit pressure work */
+  /* advection mode and the RIDER case service */
+  int sans_lagrange;         /* sans-lagrange: prescribed velocity, remap without the dual phase */
+  int rider_reverse;         /* RIDER reverse-option: velocity x cos(pi t parametre) */
+  double rider_parametre;    /* RIDER parametre */
+  double rider_rayon_bulle;  /* RIDER rayon-bulle (reference default 0.15) */
+  int rider_densite_lineaire;/* RIDER densite-interieure-initiale-lineaire */
+  int pad_rider;
 } MhOptions;
 
 /* Field identifiers for get/set.  Cell fields take an environment index

@@ -23,6 +23,11 @@ class Env:
     linear_pseudo_coeff: float = 0.5
     quadratic_pseudo_coeff: float = 1.2
     vitesse_initiale: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    energie_initiale: float = 0.0
+    fictif_tdebut: float = 0.0
+    fictif_tfin: float = 0.0
+    fictif_p0: float = 0.0
+    fictif_dpdt: float = 0.0
 
     def to_c(self) -> T.MhEnvironment:
         e = T.MhEnvironment()
@@ -36,6 +41,9 @@ class Env:
         e.pression_initiale = self.pression_initiale
         for i in range(3):
             e.vitesse_initiale[i] = self.vitesse_initiale[i]
+        e.energie_initiale = self.energie_initiale
+        e.fictif_tdebut, e.fictif_tfin = self.fictif_tdebut, self.fictif_tfin
+        e.fictif_p0, e.fictif_dpdt = self.fictif_p0, self.fictif_dpdt
         return e
 
 
@@ -73,6 +81,11 @@ class Case:
     simple_pente: bool = False
     with_newton: bool = False
     pression_explicite: bool = False
+    sans_lagrange: bool = False
+    rider_reverse: bool = False
+    rider_parametre: float = 0.0
+    rider_rayon_bulle: float = 0.15
+    rider_densite_lineaire: bool = False
 
     @property
     def nb_cells(self) -> int:
@@ -140,6 +153,11 @@ class Case:
         o.projection_simple_pente = int(self.simple_pente)
         o.with_newton = int(self.with_newton)
         o.pression_explicite = int(self.pression_explicite)
+        o.sans_lagrange = int(self.sans_lagrange)
+        o.rider_reverse = int(self.rider_reverse)
+        o.rider_parametre = self.rider_parametre
+        o.rider_rayon_bulle = self.rider_rayon_bulle
+        o.rider_densite_lineaire = int(self.rider_densite_lineaire)
         return o
 
 
@@ -285,6 +303,30 @@ def mono_pointtriple(simple_pente=False) -> Case:
     c.pente_borne, c.debar_fix, c.pente_borne_mixte, c.simple_pente = True, 1, True, simple_pente
     c.limiter = c.limiter_pure = T.MH_SUPERBEE_G
     c.remap_threshold = 1e-16
+    return c
+
+
+RIDER_TESTS = {"MonoRiderTx": 23, "MonoRiderTy": 24, "MonoRiderT45": 25, "MonoRiderRotation": 26,
+               "MonoRiderVortex": 27, "MonoRiderDeformation": 28, "MonoRiderVortexTimeReverse": 29,
+               "MonoRiderDeformationTimeReverse": 30, "RiderTx": 31, "RiderTy": 32, "RiderT45": 33,
+               "RiderRotation": 34, "RiderVortex": 35, "RiderDeformation": 36, "RiderVortexTimeReverse": 37,
+               "RiderDeformationTimeReverse": 38}
+
+
+def _fictif(rho):
+    return Env(densite_initiale=rho, pression_initiale=0., eos=T.MH_EOS_FICTIF, fictif_tdebut=0., fictif_tfin=10.)
+
+
+def rider(name, cas_test, final_time, reverse=None, bcs=None) -> Case:
+    """NONREGRESSION/CAS_*Rider*/Donnees.arc: pure advection (sans-lagrange) of a bubble on a
+    50x50x1 mesh of [0,1]^2 x [0,0.1], two Fictif environments (Vide rho 0, Bulle rho 1),
+    constant time step 1e-3; reverse = parametre of the reverse option."""
+    c = Case(name, 3, (50, 50, 1), (0., 0., 0.), (1., 1., .1), [_fictif(0.), _fictif(1.)], T.MH_CASE_RIDER,
+             cas_test, deltat_init=1e-3, deltat_min=1e-8, deltat_max=1e-4, final_time=final_time,
+             bcs=list(bcs or []))
+    c.sans_lagrange = True
+    if reverse is not None:
+        c.rider_reverse, c.rider_parametre = True, reverse
     return c
 
 

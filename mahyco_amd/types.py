@@ -24,6 +24,7 @@ STATUS_NAMES = {
 
 MH_EOS_PERFECT_GAS = 0
 MH_EOS_STIFFENED_GAS = 1
+MH_EOS_FICTIF = 2
 
 MH_XMIN, MH_XMAX, MH_YMIN, MH_YMAX, MH_ZMIN, MH_ZMAX = range(6)
 SURFACES = {"XMIN": MH_XMIN, "XMAX": MH_XMAX, "YMIN": MH_YMIN, "YMAX": MH_YMAX,
@@ -34,7 +35,7 @@ BC_TYPES = {"Vx": MH_BC_VX, "Vy": MH_BC_VY, "Vz": MH_BC_VZ}
 MH_MINMOD, MH_SUPERBEE, MH_VANLEER = 0, 1, 2
 MH_MINMOD_G, MH_SUPERBEE_G, MH_VANLEER_G, MH_ARITHMETIC_G, MH_ULTRABEE_G = 3, 4, 5, 6, 7
 
-MH_CASE_SOD, MH_CASE_SEDOV, MH_CASE_RIEMANN, MH_CASE_POINTTRIPLE = range(4)
+MH_CASE_SOD, MH_CASE_SEDOV, MH_CASE_RIEMANN, MH_CASE_POINTTRIPLE, MH_CASE_RIDER = range(5)
 MH_COORD_MUL, MH_COORD_DIV = 0, 1
 
 # MhField
@@ -72,7 +73,9 @@ class MhEnvironment(C.Structure):
     _fields_ = [("eos", C.c_int), ("adiabatic_cst", C.c_double), ("limit_tension", C.c_double),
                 ("specific_heat", C.c_double), ("linear_pseudo_coeff", C.c_double),
                 ("quadratic_pseudo_coeff", C.c_double), ("densite_initiale", C.c_double),
-                ("pression_initiale", C.c_double), ("vitesse_initiale", C.c_double * 3)]
+                ("pression_initiale", C.c_double), ("vitesse_initiale", C.c_double * 3),
+                ("energie_initiale", C.c_double), ("fictif_tdebut", C.c_double), ("fictif_tfin", C.c_double),
+                ("fictif_p0", C.c_double), ("fictif_dpdt", C.c_double)]
 
 
 class MhBoundaryCondition(C.Structure):
@@ -90,4 +93,6 @@ class MhOptions(C.Structure):
                 ("projection_pente_borne", C.c_int), ("pente_borne_debar_fix", C.c_int),
                 ("pente_borne_mixte", C.c_int), ("projection_limiteur_pure_id", C.c_int),
                 ("projection_simple_pente", C.c_int), ("with_newton", C.c_int),
-                ("pression_explicite", C.c_int)]
+                ("pression_explicite", C.c_int), ("sans_lagrange", C.c_int), ("rider_reverse", C.c_int),
+                ("rider_parametre", C.c_double), ("rider_rayon_bulle", C.c_double),
+                ("rider_densite_lineaire", C.c_int), ("pad_rider", C.c_int)]

@@ -175,6 +175,9 @@ struct OracleCase {
   void computeUremap(int idir);
   void computeDualUremap(int idir);
   int dirNeighbor(long c, int idir, bool back) const;
+  void riderRadii(long c, bool low_centre, double& rmin, double& rmax) const;
+  void updateVelocityWithoutLagrange();
+  void applyFictif(int k);
   void setMixedFlags(long c, const double* f);
   int remapVariables();
   void setMembership(const std::vector<uint8_t>& newmask);
@@ -363,8 +366,8 @@ void OracleCase::hydroStartInit() {
   std::fill(pseudo.g.begin(), pseudo.g.end(), 0.0);
   std::fill(pseudo.p.begin(), pseudo.p.end(), 0.0);
   initMatAndVar();  // casModel()->initVar :110-140 (membership was set by initMat)
-  // :143-152 initEOS per environment
-  for (int k = 0; k < m; ++k) {
+  // :143-152 initEOS per environment, then the mixed-cell averages (not in advection mode)
+  for (int k = 0; k < m && !opt.sans_lagrange; ++k) {
     const MhEnvironment& e = env[k];
     const double gamma = e.adiabatic_cst;
     for (long c = 0; c < nc; ++c) {
@@ -381,7 +384,7 @@ void OracleCase::hydroStartInit() {
     }
   }
   // :154-170 mixed-cell averages (density and pressure are not zeroed first)
-  for (long c = 0; c < nc; ++c) {
+  for (long c = 0; c < nc && !opt.sans_lagrange; ++c) {
     if (nbenv[c] == 1) continue;
     internal_energy.g[c] = 0.;
     for (int k = 0; k < m; ++k) {
@@ -427,6 +430,14 @@ void OracleCase::hydroStartInitEnvAndMat() {
       if (opt.cas_test != 2) {
         if (cell_coord[c].x <= 0.01) mat = 0.;
         else mat = (cell_coord[c].y <= 0.015) ? 1. : 2.;
+      }
+    } else if (opt.cas_model == MH_CASE_RIDER) {  // RIDERService::initMat :19-53
+      if (opt.cas_test >= 31) {  // the Mono* tests (23-30) put every cell in environment 0
+        double rmin, rmax;
+        riderRadii(c, opt.cas_test < 34, rmin, rmax);
+        const double rb = opt.rider_rayon_bulle;
+        if (rmax < rb) mat = 1.;
+        else if ((rmax >= rb) && (rmin < rb)) mat = (rb - rmin) / (rmax - rmin);
       }
     } else if (opt.cas_model == MH_CASE_RIEMANN) {
       int a = opt.interface_axis;
@@ -553,9 +564,90 @@ void OracleCase::initMatAndVar() {
       }
     }
   }
+  if (opt.cas_model == MH_CASE_RIDER) {  // RIDERService::initVarMono :58-158, initVar :163-290
+    const bool mono = opt.cas_test <= 30;
+    const bool lin = opt.rider_densite_lineaire != 0;
+    const double rb = opt.rider_rayon_bulle;
+    const R3 Xb = (opt.cas_test < (mono ? 26 : 34)) ? R3{0.20, 0.20, 0.} : R3{0.50, 0.75, 0.};
+    const double croissance = lin ? 1. / rb : 1.;
+    for (long c = 0; c < nc; ++c) {
+      double rmin, rmax;
+      riderRadii(c, opt.cas_test < (mono ? 26 : 34), rmin, rmax);
+      density.g[c] = env[0].densite_initiale;
+      pressure.g[c] = env[0].pression_initiale;
+      internal_energy.g[c] = env[0].energie_initiale;
+      const double r = std::sqrt((cell_coord[c].x - Xb.x) * (cell_coord[c].x - Xb.x) +
+                                 (cell_coord[c].y - Xb.y) * (cell_coord[c].y - Xb.y));
+      const double r_cell = lin ? r : rmax;
+      if (r_cell < rb) {
+        density.g[c] = env[1].densite_initiale;
+        if (lin) density.g[c] = env[1].densite_initiale * std::min(croissance * 2 * (rb - r), 1.);
+        pressure.g[c] = env[1].pression_initiale;
+      } else if ((rmax >= rb) && (rmin < rb)) {
+        if (!lin) {
+          const double frac_b = (rb - rmin) / (rmax - rmin);
+          if (!mono)
+            for (int k = 0; k < m && k < 2; ++k) {
+              if (!member(k, c)) continue;
+              ev(density, k, c) = env[k == 0 ? 1 : 0].densite_initiale;
+              ev(fracvol, k, c) = k == 0 ? frac_b : 1. - frac_b;
+              ev(mass_fraction, k, c) = k == 0 ? frac_b : 1. - frac_b;
+              ev(pressure, k, c) = 0.;
+            }
+          density.g[c] = frac_b * env[1].densite_initiale;
+          pressure.g[c] = env[1].pression_initiale;
+        }
+      }
+    }
+    const R3 cc{0.5, 0.5, 0.};
+    const double Pi = 3.14159265359;  // TypesMahyco.h:42
+    const int t = mono ? opt.cas_test : opt.cas_test - 8;  // the same velocity field per Mono/bi pair
+    for (long n = 0; n < nn; ++n) {
+      R3 u{0., 0., 0.};
+      const R3 x = node_coord[n];
+      if (t == 23) u.x = 1.;
+      if (t == 24) u.y = 1.;
+      if (t == 25) u = R3{1.0, 1.0, 0.0};
+      if (t == 26) {
+        const R3 dd = x - cc;
+        const double theta = std::atan2(dd.y, dd.x);
+        const double r = std::sqrt(dd.x * dd.x + dd.y * dd.y);
+        const double omega = 4. * Pi;
+        u.x = -r * omega * std::sin(omega * 0. + theta);
+        u.y = r * omega * std::cos(omega * 0. + theta);
+      }
+      if (t == 27 || t == 29) {
+        const R3 dd = x;
+        u.x = -2. * std::cos(Pi * dd.y) * std::sin(Pi * dd.y) * std::sin(Pi * dd.x) * std::sin(Pi * dd.x);
+        u.y = 2. * std::cos(Pi * dd.x) * std::sin(Pi * dd.x) * std::sin(Pi * dd.y) * std::sin(Pi * dd.y);
+      }
+      if (t == 28 || t == 30) {
+        const R3 dd = x + cc;
+        u.x = std::sin(4. * Pi * dd.x) * std::sin(4. * Pi * dd.y);
+        u.y = std::cos(4. * Pi * dd.x) * std::cos(4. * Pi * dd.y);
+      }
+      velocity[n] = u;
+    }
+    velocity_n = velocity;
+    return;
+  }
   for (long n = 0; n < nn; ++n) {
     const double* v0 = env[0].vitesse_initiale;
     velocity[n] = (opt.cas_model == MH_CASE_SEDOV) ? R3{0., 0., 0.} : R3{v0[0], v0[1], v0[2]};
+  }
+}
+
+// Smallest and largest distance from the bubble centre (0.2, 0.2) or (0.5, 0.75) to the
+// nodes of cell c, in the plane (RIDERService.cc:33-41).
+void OracleCase::riderRadii(long c, bool low_centre, double& rmin, double& rmax) const {
+  const R3 Xb = low_centre ? R3{0.20, 0.20, 0.} : R3{0.50, 0.75, 0.};
+  rmin = 10.;
+  rmax = 0.;
+  for (int l = 0; l < nnpc; ++l) {
+    const R3 x = node_coord[cell_nodes[c * nnpc + l]];
+    const double rnode = std::sqrt((x.x - Xb.x) * (x.x - Xb.x) + (x.y - Xb.y) * (x.y - Xb.y));
+    rmin = std::min(rmin, rnode);
+    rmax = std::max(rmax, rnode);
   }
 }
 
@@ -648,7 +740,21 @@ void OracleCase::saveValuesAtN() {
   density_n = density;
   internal_energy_n = internal_energy;
   cqs_n = cqs;
-  velocity_n = velocity;
+  if (!opt.sans_lagrange) velocity_n = velocity;  // :373-375
+}
+
+// MahycoModule::updateVelocityWithoutLagrange, MahycoModule.cc:702-718: the prescribed
+// velocity, optionally reversed in time by cos(pi t parametre).
+void OracleCase::updateVelocityWithoutLagrange() {
+  const double option_real = opt.rider_reverse ? 1. : 0.;
+  const double factor = opt.rider_parametre;
+  const double Pi = 3.14159265359;  // TypesMahyco.h:42
+  const double cs = std::cos(Pi * global_time * factor);
+  for (long n = 0; n < nn; ++n) {  // u_n (1 - option) + (option u_n) cos, per component
+    const R3 u = velocity_n[n];
+    velocity[n] = R3{u.x * (1. - option_real) + option_real * u.x * cs, u.y * (1. - option_real) + option_real * u.y * cs,
+                     u.z * (1. - option_real) + option_real * u.z * cs};
+  }
 }
 
 // MahycoModule::updateVelocity, MahycoModule.cc:438-562 (non-CSTS; elasto force of
@@ -679,7 +785,9 @@ void OracleCase::applyBoundaryCondition() {
 void OracleCase::updatePosition() {
   const double deltat = global_deltat;
 #pragma omp parallel for schedule(static)
-  for (long n = 0; n < nn; ++n) node_coord[n] += deltat * velocity[n];
+  for (long n = 0; n < nn; ++n)
+    if (!opt.sans_lagrange || nc_off[n + 1] - nc_off[n] == 4)  // :737 (advection: 4-cell nodes only)
+      node_coord[n] += deltat * velocity[n];
   const double one_over_nbnode = dim == 2 ? .25 : .125;
 #pragma omp parallel for schedule(static)
   for (long c = 0; c < nc; ++c) {
@@ -931,7 +1039,7 @@ int OracleCase::remap() {
     computeGradPhiCell(idir);
     computeUpwindFaceQuantitiesForProjection(idir);
     computeUremap(idir);
-    computeDualUremap(idir);
+    if (!opt.sans_lagrange) computeDualUremap(idir);  // withDualProjection (PrepareRemap.cc:211)
   }
   sens_projection = (sens_projection + 1) % dim;
   node_coord = node_coord_0;  // isEuler() == "Full"
@@ -944,9 +1052,25 @@ int OracleCase::remap() {
       if (member(k, c)) mt += k * ev(fracvol, k, c);
     materiau[c] = mt;
   }
-  for (int k = 0; k < m; ++k) applyEOS(k);
-  computePressionMoyenne();
+  for (int k = 0; k < m; ++k) {
+    if (env[k].eos == MH_EOS_FICTIF) applyFictif(k);
+    else applyEOS(k);
+  }
+  if (!opt.sans_lagrange) computePressionMoyenne();  // returns at once in advection mode (:1711)
   return MH_OK;
+}
+
+// FictifEOSService::applyEOS, eos/fictif/FictifEOSService.cc:45-75: a pressure that only
+// depends on time, a unit sound speed (no influence on the step) and no internal energy.
+void OracleCase::applyFictif(int k) {
+  const MhEnvironment& e = env[k];
+  for (long c = 0; c < nc; ++c) {
+    if (!member(k, c)) continue;
+    if (global_time > e.fictif_tdebut) ev(pressure, k, c) = e.fictif_p0 + e.fictif_dpdt * global_time;
+    if (global_time > e.fictif_tfin) ev(pressure, k, c) = 0.;
+    ev(sound_speed, k, c) = 1.;
+    ev(internal_energy, k, c) = 0.;
+  }
 }
 
 // PrepareRemap.cc:94-193 (slots V_k, M_k, E_k and V*q; dual slots u, m, KE).
@@ -1449,7 +1573,7 @@ void OracleCase::computeUremap(int idir) {
     }
     double* U = &u_lagrange[c * nv];
     for (int v = 0; v < nv; ++v) U[v] -= flux_face[v];
-    if (pb)  // :441-463: tiny partial volumes, masses or energies drop the environment's three
+    if (pb && !opt.sans_lagrange)  // :441-463 (with the dual phase): tiny partial volumes, masses or energies drop the environment's three
       for (int k = 0; k < m; ++k) {
         if (U[k] < 1. * thr) U[k] = U[m + k] = U[2 * m + k] = 0.;
         if (U[m + k] < 1. * thr) U[k] = U[m + k] = U[2 * m + k] = 0.;
@@ -1631,7 +1755,7 @@ int OracleCase::remapVariables() {
     for (long c = 0; c < nc; ++c) {
       const double* U = &u_lagrange[c * nv];
       const bool was = member(k, c);
-      if (((U[k] / euler_volume[c]) < thr || (U[m + k] == 0.)) && was) {
+      if (((U[k] / euler_volume[c]) < thr || (U[m + k] == 0. && !opt.sans_lagrange)) && was) {
         newmask[c] &= (uint8_t)~(1u << k);
       } else if ((U[k] / euler_volume[c]) > thr && !was) {
         double volt = 0.;
@@ -1729,14 +1853,16 @@ int OracleCase::remapVariables() {
     }
   }
   if (err) return MH_ERR_NAN_REMAP;
-  // :286-297 node velocity from the dual projection
+  // :286-297 node velocity from the dual projection (none in advection mode)
+  if (!opt.sans_lagrange) {
 #pragma omp parallel for schedule(static)
-  for (long n = 0; n < nn; ++n) {
-    const double mp = u_dual[n * 5 + 3];
-    if (mp != 0.) {
-      velocity[n].x = u_dual[n * 5 + 0] / mp;
-      velocity[n].y = u_dual[n * 5 + 1] / mp;
-      velocity[n].z = u_dual[n * 5 + 2] / mp;
+    for (long n = 0; n < nn; ++n) {
+      const double mp = u_dual[n * 5 + 3];
+      if (mp != 0.) {
+        velocity[n].x = u_dual[n * 5 + 0] / mp;
+        velocity[n].y = u_dual[n * 5 + 1] / mp;
+        velocity[n].z = u_dual[n * 5 + 2] / mp;
+      }
     }
   }
   computeNodeMass();  // :299-306
@@ -1753,6 +1879,12 @@ int OracleCase::remapVariables() {
 int OracleCase::computeDeltaT() {
   global_old_deltat = global_deltat;
   old_deltat = global_old_deltat;
+  if (opt.sans_lagrange) {  // :1751-1753: the step stays deltat-init
+    global_deltat = opt.deltat_init;
+    const double stop_time = opt.final_time;
+    if ((global_time < stop_time && (global_time + global_deltat) > stop_time) || global_time > stop_time) stopped = true;
+    return MH_OK;
+  }
   double minimum_aux = 1.79769313486231570815e+308;
   std::vector<double> dxs(nc);
 #pragma omp parallel for schedule(static)
@@ -1806,14 +1938,17 @@ int OracleCase::cycle() {
   global_iteration += 1;
   global_time += global_deltat;
   saveValuesAtN();
-  updateVelocity();
+  if (opt.sans_lagrange) updateVelocityWithoutLagrange();
+  else updateVelocity();
   applyBoundaryCondition();
   updatePosition();
   int st = computeGeometricValues();
   if (st != MH_OK) return st;
-  updateDensity();
-  computeArtificialViscosity();
-  updateEnergyAndPressure();
+  if (!opt.sans_lagrange) {  // skipped in advection mode (:1271, :389, :1433)
+    updateDensity();
+    computeArtificialViscosity();
+    updateEnergyAndPressure();
+  }
   if (opt.with_projection) {
     st = remap();
     if (st != MH_OK) return st;

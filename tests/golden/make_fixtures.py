@@ -60,6 +60,24 @@ CASES = {
     "CAS_MonoPointtriple_pb_mixte_superbeeG_superbeeG": (2, (70, 30, 1), (0, 0, 0), (.07, .03, 0.)),
     "CAS_2D_Lag_SodCaseX_newton": (2, (100, 5, 1), (0, 0, 0), (1., .1, 0.)),
     "CAS_SodCaseX": (3, (100, 5, 5), (0, 0, 0), (1., .1, .1)),
+    "CAS_MonoRiderTx": (3, (50, 50, 1), (0, 0, 0), (1., 1., .1)),
+    "CAS_MonoRiderTy": (3, (50, 50, 1), (0, 0, 0), (1., 1., .1)),
+    "CAS_MonoRiderT45": (3, (50, 50, 1), (0, 0, 0), (1., 1., .1)),
+    "CAS_MonoRiderRotation": (3, (50, 50, 1), (0, 0, 0), (1., 1., .1)),
+    "CAS_MonoRiderVortex": (3, (50, 50, 1), (0, 0, 0), (1., 1., .1)),
+    "CAS_MonoRiderDeformation": (3, (50, 50, 1), (0, 0, 0), (1., 1., .1)),
+    "CAS_MonoRiderVortexTimeReverse": (3, (50, 50, 1), (0, 0, 0), (1., 1., .1)),
+    "CAS_MonoRiderDeformationTimeReverse": (3, (50, 50, 1), (0, 0, 0), (1., 1., .1)),
+    "CAS_RiderTx": (3, (50, 50, 1), (0, 0, 0), (1., 1., .1)),
+    "CAS_RiderTy": (3, (50, 50, 1), (0, 0, 0), (1., 1., .1)),
+    "CAS_RiderT45": (3, (50, 50, 1), (0, 0, 0), (1., 1., .1)),
+    "CAS_RiderRotation": (3, (50, 50, 1), (0, 0, 0), (1., 1., .1)),
+    "CAS_RiderVortex": (3, (50, 50, 1), (0, 0, 0), (1., 1., .1)),
+    "CAS_RiderDeformation": (3, (50, 50, 1), (0, 0, 0), (1., 1., .1)),
+    "CAS_RiderVortexTimeReverse": (3, (50, 50, 1), (0, 0, 0), (1., 1., .1)),
+    "CAS_RiderDeformationTimeReverse": (3, (50, 50, 1), (0, 0, 0), (1., 1., .1)),
+    "CAS_MonoRiderTy_pb_simple_superbee": (3, (50, 50, 1), (0, 0, 0), (1., 1., .1)),
+    "CAS_MonoRiderTy_pb_mixte_superbee_vanleer": (3, (50, 50, 1), (0, 0, 0), (1., 1., .1)),
     "CAS_MonoPointtriple_pb_sp_mixte_superbeeG_superbeeG": (2, (70, 30, 1), (0, 0, 0), (.07, .03, 0.)),
 }
 # Runs whose output holds no geometry: parts are the subdomains of the Cartesian generator

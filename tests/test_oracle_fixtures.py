@@ -18,6 +18,22 @@ from mahyco_amd import types as T
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
+RIDER_FT = {t: (3. if "Vortex" in t else 1.) if "Reverse" in t or "Vortex" in t else .5 for t in cases.RIDER_TESTS}
+RIDER_FT.update({"MonoRiderVortexTimeReverse": 4., "RiderVortexTimeReverse": 4.})
+RIDER_REV = {"MonoRiderVortexTimeReverse": .25, "MonoRiderDeformationTimeReverse": 1., "RiderVortexTimeReverse": .25,
+             "RiderDeformationTimeReverse": 1.}
+
+
+def _pb_rider(name, mixte):
+    """CAS_MonoRiderTy_pb_*: the bubble advected along y (cas-test 32) with pente-borne
+    fluxes, debar fix 2, superbee; the mixte variant takes vanLeer in pure neighbourhoods."""
+    c = cases.rider(name, 32, .5)
+    c.pente_borne, c.debar_fix, c.limiter = True, 2, T.MH_SUPERBEE
+    if mixte:
+        c.pente_borne_mixte, c.limiter_pure = True, T.MH_VANLEER
+    return c
+
+
 FIXTURES = {
     "CAS_BiSodCaseX": cases.bisod_x,
     "CAS_BiSodCaseY": cases.bisod_y,
@@ -67,6 +83,13 @@ FIXTURES = {
     "CAS_MonoPointtriple_pb_sp_mixte_superbeeG_superbeeG": lambda: cases.mono_pointtriple(True),
     # energy of the Lagrange step by Newton iterations (MahycoModule.cc:1506-1540)
     "CAS_2D_Lag_SodCaseX_newton": cases.lag2d_sod_x_newton,
+    # pure advection (sans-lagrange) of the RIDER bubble, one and two environments
+    **{f"CAS_{t}": (lambda t=t: cases.rider(f"CAS_{t}", cases.RIDER_TESTS[t], RIDER_FT[t], RIDER_REV.get(t)))
+       for t in cases.RIDER_TESTS if t not in ("RiderVortex", "RiderVortexTimeReverse")},
+    "CAS_RiderVortex": lambda: cases.rider("CAS_RiderVortex", 27, 3.),
+    "CAS_RiderVortexTimeReverse": lambda: cases.rider("CAS_RiderVortexTimeReverse", 37, 4., .25),
+    "CAS_MonoRiderTy_pb_simple_superbee": lambda: _pb_rider("CAS_MonoRiderTy_pb_simple_superbee", False),
+    "CAS_MonoRiderTy_pb_mixte_superbee_vanleer": lambda: _pb_rider("CAS_MonoRiderTy_pb_mixte_superbee_vanleer", True),
     # protection-reprise: 10 iterations, a dump, then the continued run (launch_test.sh:25-38)
     "CAS_SodCaseX": lambda: cases.sod_mono("X"),
     # (CAS_BiSodCaseZ_pente_borne_superbee_debar_2 is left out: its output holds four EnSight
@@ -170,12 +193,13 @@ def test_ensight_fields(name, runs):
 
 
 def test_lagrange_fixtures_bit_exact(runs):
-    """The Lagrange-only fixtures (sequential runs) are reproduced to the last printed bit:
+    """Sequential Lagrange-only and advection fixtures are reproduced to the last printed bit:
     every 17-digit time-history value is equal."""
     # (the sequential tubes along y, CAS_Lag_BiSodCaseY and CAS_Lag_SodCaseY, differ in the
     # last printed digit of a few averages and are held to the 1e-12 bar only)
     for name in ("CAS_Lag_BiSodCaseX", "CAS_2D_Lag_SodCaseX", "CAS_Lag_SodCaseZ", "CAS_2D_Lag_SodCaseZ",
-                 "CAS_2D_Lag_SodCaseX_newton"):
+                 "CAS_2D_Lag_SodCaseX_newton", "CAS_MonoRiderTx", "CAS_MonoRiderTy", "CAS_MonoRiderT45",
+                 "CAS_MonoRiderDeformation"):
         o, st = _run(name, runs)
         ref = np.load(os.path.join(GOLDEN, name + ".npz"))
         th = o.history()

@@ -57,6 +57,8 @@ struct MhHandle {
   std::vector<size_t> alloc_bytes;  // size of each allocation (checkpoints)
   double* xl[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   double *Ua = nullptr, *Ub = nullptr, *dflux = nullptr, *Ud = nullptr, *Pa = nullptr, *Pb = nullptr;
+  double* u0[3] = {nullptr, nullptr, nullptr};  // advection mode: prescribed velocity
+  double* rev_cos = nullptr;                     // advection mode: reverse factor per iteration
   long launched = 0;  // iterations queued by the host
   int sens = 0;       // m_sens_projection of the next iteration
   int nblk_c = 0, nblk_n = 0;
@@ -129,7 +131,23 @@ struct HostInit {
   std::vector<double> chig;  // global mass fraction: 1 everywhere, as Arcane starts it
   std::vector<uint8_t> mask;
   std::vector<double> u[3];
+  std::vector<double> rev_cos;  // advection mode: cos(pi t_n parametre) per iteration
 };
+
+// RIDERService (casTest/RIDER/RIDERService.cc) constants: the bubble centre of the
+// translations (0.2, 0.2) or of the rotations and later tests (0.5, 0.75), and the
+// reference's Pi (TypesMahyco.h:42).
+const double kRiderPi = 3.14159265359;
+void rider_radii(const D3* x, int nnode, bool low_centre, double& rmin, double& rmax) {
+  const double xb = low_centre ? 0.20 : 0.50, yb = low_centre ? 0.20 : 0.75;
+  rmin = 10.;
+  rmax = 0.;
+  for (int l = 0; l < nnode; ++l) {
+    const double rnode = std::sqrt((x[l].x - xb) * (x[l].x - xb) + (x[l].y - yb) * (x[l].y - yb));
+    rmin = std::min(rmin, rnode);
+    rmax = std::max(rmax, rnode);
+  }
+}
 
 int host_init(MhHandle* h, HostInit& I) {
   const MhGrid& g = h->g;
@@ -163,6 +181,7 @@ int host_init(MhHandle* h, HostInit& I) {
   I.mask.assign(nc, 1);
   for (int d = 0; d < 3; ++d) I.u[d].assign(nn, 0.);
   std::vector<double> xcx(nc), xcy(nc), xcz(nc), f1(nc, 0.);
+  std::vector<double> rider_frac(nc, -1.);  // RIDER cells cut by the bubble: its fraction
   const double w = dim == 3 ? .125 : .25;
   for (int k = 0; k < g.nz; ++k)
     for (int j = 0; j < g.ny; ++j)
@@ -207,6 +226,35 @@ int host_init(MhHandle* h, HostInit& I) {
           if (opt.cas_test != 2) {
             if (xc.x <= 0.01) mat = 0.;
             else mat = (xc.y <= 0.015) ? 1. : 2.;
+          }
+        } else if (opt.cas_model == MH_CASE_RIDER) {  // RIDERService::initMat :19-53
+          if (opt.cas_test >= 31) {
+            double rmin, rmax;
+            rider_radii(x, nnode, opt.cas_test < 34, rmin, rmax);
+            const double rb = opt.rider_rayon_bulle;
+            if (rmax < rb) mat = 1.;
+            else if ((rmax >= rb) && (rmin < rb)) mat = (rb - rmin) / (rmax - rmin);
+          }
+          // initVar (initVarMono :58-158, initVar :163-290), per cell
+          const bool mono = opt.cas_test <= 30;
+          const bool lin = opt.rider_densite_lineaire != 0;
+          const double rb = opt.rider_rayon_bulle;
+          double rmin, rmax;
+          rider_radii(x, nnode, opt.cas_test < (mono ? 26 : 34), rmin, rmax);
+          const double xb = opt.cas_test < (mono ? 26 : 34) ? 0.20 : 0.50;
+          const double yb = opt.cas_test < (mono ? 26 : 34) ? 0.20 : 0.75;
+          const double croissance = lin ? 1. / rb : 1.;
+          I.rho[c] = h->env[0].densite_initiale;
+          I.p[c] = h->env[0].pression_initiale;
+          I.e[c] = h->env[0].energie_initiale;
+          const double r = std::sqrt((xc.x - xb) * (xc.x - xb) + (xc.y - yb) * (xc.y - yb));
+          const double r_cell = lin ? r : rmax;
+          if (r_cell < rb) {
+            I.rho[c] = h->env[1].densite_initiale;
+            if (lin) I.rho[c] = h->env[1].densite_initiale * std::min(croissance * 2 * (rb - r), 1.);
+            I.p[c] = h->env[1].pression_initiale;
+          } else if ((rmax >= rb) && (rmin < rb) && !lin) {
+            rider_frac[c] = (rb - rmin) / (rmax - rmin);
           }
         } else if (opt.cas_model == MH_CASE_RIEMANN) {
           const int a = opt.interface_axis;
@@ -285,6 +333,29 @@ int host_init(MhHandle* h, HostInit& I) {
         I.rhok[s * nc + c] = h->env[s].densite_initiale;
         I.pk[s * nc + c] = h->env[s].pression_initiale;
       }
+    } else if (opt.cas_model == MH_CASE_RIDER) {
+      const double frac_b = rider_frac[c];
+      if (frac_b >= 0.) {  // environment 0 takes the bubble density and fraction (RIDERService.cc:252-266)
+        if (opt.cas_test >= 31)
+          for (int k = 0; k < m && k < 2; ++k) {
+            if (!((bits >> k) & 1)) continue;
+            const double rk = h->env[k == 0 ? 1 : 0].densite_initiale;
+            const double fr = k == 0 ? frac_b : 1. - frac_b;
+            if (pure) {  // the environment value is the cell value
+              I.rho[c] = rk;
+              I.fg[c] = fr;
+              I.chig[c] = fr;
+              I.p[c] = 0.;
+            } else {
+              I.rhok[k * nc + c] = rk;
+              I.fk[k * nc + c] = fr;
+              I.chik[k * nc + c] = fr;
+              I.pk[k * nc + c] = 0.;
+            }
+          }
+        I.rho[c] = frac_b * h->env[1].densite_initiale;
+        I.p[c] = h->env[1].pression_initiale;
+      }
     } else if (opt.cas_model == MH_CASE_RIEMANN) {
       if (pure) {
         const int k = bits == 1 ? 0 : 1;
@@ -307,10 +378,61 @@ int host_init(MhHandle* h, HostInit& I) {
     }
   }
   const double* v0 = h->env[0].vitesse_initiale;
-  if (opt.cas_model != MH_CASE_SEDOV)
+  if (opt.cas_model == MH_CASE_RIDER) {  // RIDERService.cc:134-157, 267-289 (the same field per Mono/bi pair)
+    const int t = opt.cas_test <= 30 ? opt.cas_test : opt.cas_test - 8;
+    const double Pi = kRiderPi;
+    for (long n = 0; n < nn; ++n) {
+      const int i = (int)(n % g.nnx);
+      const long r = n / g.nnx;
+      const int j = (int)(r % g.nny), k = (int)(r / g.nny);
+      const D3 x = node(i, j, k);
+      double ux = 0., uy = 0.;
+      if (t == 23) ux = 1.;
+      if (t == 24) uy = 1.;
+      if (t == 25) ux = uy = 1.;
+      // one sincos per argument: GCC merges the sin/cos pairs of RIDERService.cc into sincos
+      // calls at -O1 and above, and glibc's sincos can differ from its sin by an ulp
+      double s0, c0, s1, c1;
+      if (t == 26) {
+        const double dx = x.x - 0.5, dy = x.y - 0.5;
+        const double theta = std::atan2(dy, dx);
+        const double rr = std::sqrt(dx * dx + dy * dy);
+        const double omega = 4. * Pi;
+        ::sincos(omega * 0. + theta, &s0, &c0);
+        ux = -rr * omega * s0;
+        uy = rr * omega * c0;
+      }
+      if (t == 27 || t == 29) {
+        ::sincos(Pi * x.x, &s0, &c0);
+        ::sincos(Pi * x.y, &s1, &c1);
+        ux = -2. * c1 * s1 * s0 * s0;
+        uy = 2. * c0 * s0 * s1 * s1;
+      }
+      if (t == 28 || t == 30) {
+        const double dx = x.x + 0.5, dy = x.y + 0.5;
+        ::sincos(4. * Pi * dx, &s0, &c0);
+        ::sincos(4. * Pi * dy, &s1, &c1);
+        ux = s0 * s1;
+        uy = c0 * c1;
+      }
+      I.u[0][n] = ux;
+      I.u[1][n] = uy;
+      I.u[2][n] = 0.;
+    }
+  } else if (opt.cas_model != MH_CASE_SEDOV)
     for (int d = 0; d < 3; ++d) std::fill(I.u[d].begin(), I.u[d].end(), dim == 2 && d == 2 ? 0. : v0[d]);
-  // initEOS per environment (PerfectGasEOSService.cc:12-37, StiffenedGasEOSService.cc:14-33)
-  for (long c = 0; c < nc; ++c) {
+  if (opt.sans_lagrange) {  // cos(pi t parametre) at t_n = t_(n-1) + deltat-init, as the device accumulates it
+    const long nmax = (long)std::ceil(opt.final_time / opt.deltat_init) + 8;
+    I.rev_cos.resize(nmax);
+    double t = 0.;
+    for (long it = 0; it < nmax; ++it) {
+      t = t + opt.deltat_init;
+      I.rev_cos[it] = std::cos(kRiderPi * t * opt.rider_parametre);
+    }
+  }
+  // initEOS per environment (PerfectGasEOSService.cc:12-37, StiffenedGasEOSService.cc:14-33),
+  // not in advection mode (MahycoModule.cc:143)
+  for (long c = 0; c < nc && !opt.sans_lagrange; ++c) {
     const uint8_t bits = I.mask[c];
     const bool pure = __builtin_popcount(bits) == 1;
     for (int k = 0; k < m; ++k) {
@@ -347,7 +469,6 @@ int host_init(MhHandle* h, HostInit& I) {
     if (__builtin_popcount(bits) != 1)
       for (int k = 0; k < m; ++k)
         if ((bits >> k) & 1) I.Mk[k * nc + c] = I.chik[k * nc + c] * I.M[c];
-    I.fg[c] = 1.;
   }
   return MH_OK;
 }
@@ -382,6 +503,12 @@ void prof_end(MhHandle* h, ProfRec& r) {
     if ((h)->prof) prof_end((h), _r);                                    \
   } while (0)
 
+// a dual (node) sweep kernel: none in advection mode (withDualProjection, PrepareRemap.cc:211)
+#define LAUNCH_DUAL(h, ...)                           \
+  do {                                                \
+    if (!(h)->dev.sans_lagrange) LAUNCH(h, __VA_ARGS__); \
+  } while (0)
+
 // blocks of the row-segment x sweep (one wavefront per MH_ROW_OUT cells of a row), a multiple of 8
 int row_blocks(MhHandle* h) {
   const long waves = (long)h->g.ny * h->g.nz * ((h->g.nx + MH_ROW_OUT - 1) / MH_ROW_OUT);
@@ -413,7 +540,7 @@ void launch_sweep_yz(MhHandle* h, int sel, int first, int last) {
     MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell<DIM, D, M, LIM>), h->nblk_c, MH_BLOCK, S, sel, first));
   else
     MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell_tile<DIM, D, M, LIM>), tile_blocks(h, D), MH_BLOCK, S, sel, first));
-  MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE + D, (k_sweep_node<DIM, D, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
+  MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE + D, (k_sweep_node<DIM, D, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
 }
 
 template <int DIM, int M>
@@ -438,33 +565,33 @@ void launch_sweeps_t(MhHandle* h) {
     if (gen) {
       if (D == 0) {
         LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell_gen<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-        LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0, MH_LIM_ANY>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+        LAUNCH_DUAL(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0, MH_LIM_ANY>), h->nblk_n, MH_BLOCK, S, sel, first, last);
       } else if (D == 1) {
         LAUNCH(h, KN_SWEEP_CELL + 1, (k_sweep_cell_gen<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-        LAUNCH(h, KN_SWEEP_NODE + 1, (k_sweep_node<DIM, 1, MH_LIM_ANY>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+        LAUNCH_DUAL(h, KN_SWEEP_NODE + 1, (k_sweep_node<DIM, 1, MH_LIM_ANY>), h->nblk_n, MH_BLOCK, S, sel, first, last);
       } else if constexpr (DIM == 3) {
         LAUNCH(h, KN_SWEEP_CELL + 2, (k_sweep_cell_gen<3, 2, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-        LAUNCH(h, KN_SWEEP_NODE + 2, (k_sweep_node<3, 2, MH_LIM_ANY>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+        LAUNCH_DUAL(h, KN_SWEEP_NODE + 2, (k_sweep_node<3, 2, MH_LIM_ANY>), h->nblk_n, MH_BLOCK, S, sel, first, last);
       }
     } else if (o3) {  // order 3: one thread per cell for the cell part, node part as selected
       if (D == 0) LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell3<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
       else if (D == 1) LAUNCH(h, KN_SWEEP_CELL + 1, (k_sweep_cell3<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, sel, first);
       else if constexpr (DIM == 3) LAUNCH(h, KN_SWEEP_CELL + 2, (k_sweep_cell3<3, 2, M>), h->nblk_c, MH_BLOCK, S, sel, first);
       if (D == 0 && mode != MH_SWEEP_FLAT)
-        MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node_row<DIM, LIM>), node_row_blocks(h), MH_BLOCK, S, sel, first, last));
+        MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE, (k_sweep_node_row<DIM, LIM>), node_row_blocks(h), MH_BLOCK, S, sel, first, last));
       else if (D == 0)
-        MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
+        MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
       else if (D == 1)
-        MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE + 1, (k_sweep_node<DIM, 1, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
+        MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE + 1, (k_sweep_node<DIM, 1, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
       else if constexpr (DIM == 3)
-        MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE + 2, (k_sweep_node<3, 2, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
+        MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE + 2, (k_sweep_node<3, 2, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
     } else if (D == 0) {
       if (mode == MH_SWEEP_FLAT) {
         MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, 0, M, LIM>), h->nblk_c, MH_BLOCK, S, sel, first));
-        MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
+        MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
       } else {
         MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell_row<DIM, M, LIM>), row_blocks(h), MH_BLOCK, S, sel, first));
-        MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node_row<DIM, LIM>), node_row_blocks(h), MH_BLOCK, S, sel, first, last));
+        MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE, (k_sweep_node_row<DIM, LIM>), node_row_blocks(h), MH_BLOCK, S, sel, first, last));
       }
     } else if (D == 1) {
       launch_sweep_yz<DIM, 1, M>(h, sel, first, last);
@@ -472,7 +599,7 @@ void launch_sweeps_t(MhHandle* h) {
       if constexpr (DIM == 3) launch_sweep_yz<3, 2, M>(h, sel, first, last);
     }
   }
-  if (nd <= 1) LAUNCH(h, KN_NODE_VEL, (k_node_velocity<DIM>), h->nblk_n, MH_BLOCK, S, nd == 0 ? 1 : 0);
+  if (nd <= 1 && !S.sans_lagrange) LAUNCH(h, KN_NODE_VEL, (k_node_velocity<DIM>), h->nblk_n, MH_BLOCK, S, nd == 0 ? 1 : 0);
   LAUNCH(h, KN_FINAL, (k_finalize<DIM, M>), h->nblk_c, MH_BLOCK, S, nd & 1);
   h->sens = (h->sens + 1) % dim;
 }
@@ -487,7 +614,9 @@ void set_parity(MhHandle* h) {
 
 template <int DIM, int M>
 void launch_node_t(MhHandle* h) {
-  if (DIM == 3 && h->node_tiled) {
+  if (h->dev.sans_lagrange) {
+    LAUNCH(h, KN_NODE, (k_node_adv<DIM>), h->nblk_n, MH_BLOCK, h->dev);
+  } else if (DIM == 3 && h->node_tiled) {
     const dim3 grid((h->g.nnx + MH_NTX - 1) / MH_NTX, (h->g.nny + MH_NTY - 1) / MH_NTY,
                     (h->g.nnz + MH_NT_KC - 1) / MH_NT_KC);
     LAUNCH(h, KN_NODE, k_node_lag3_tiled, grid, MH_BLOCK, h->dev);
@@ -564,11 +693,17 @@ int validate(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const Mh
   if (o->nb_bc < 0 || o->nb_bc > MH_MAX_BC) return MH_ERR_INVALID;
   for (int b = 0; b < o->nb_bc; ++b)
     if (o->bc[b].surface < 0 || o->bc[b].surface > 5 || o->bc[b].type < 0 || o->bc[b].type > 2) return MH_ERR_INVALID;
-  if (o->cas_model < 0 || o->cas_model > 3) return MH_ERR_UNSUPPORTED;
+  if (o->cas_model < 0 || o->cas_model > 4) return MH_ERR_UNSUPPORTED;
+  if (o->cas_model == MH_CASE_RIDER && (o->cas_test < 23 || o->cas_test > 38)) return MH_ERR_UNSUPPORTED;
+  if (o->cas_model == MH_CASE_RIDER && o->cas_test >= 31 && nb_env < 2) return MH_ERR_INVALID;
+  if (o->sans_lagrange && (!o->with_projection || !(o->deltat_init > 0.) ||
+                           !(o->final_time / o->deltat_init <= 1e7)))
+    return MH_ERR_UNSUPPORTED;  // the reverse factors are tabulated per iteration up to the final time
   if (o->cas_model == MH_CASE_RIEMANN && nb_env < 2) return MH_ERR_INVALID;
   if (o->cas_model == MH_CASE_POINTTRIPLE && nb_env < 3) return MH_ERR_INVALID;
   for (int k = 0; k < nb_env; ++k)
-    if (envs[k].eos != MH_EOS_PERFECT_GAS && envs[k].eos != MH_EOS_STIFFENED_GAS) return MH_ERR_UNSUPPORTED;
+    if (envs[k].eos != MH_EOS_PERFECT_GAS && envs[k].eos != MH_EOS_STIFFENED_GAS && envs[k].eos != MH_EOS_FICTIF)
+      return MH_ERR_UNSUPPORTED;
   const long nc = (long)mesh->nx * mesh->ny * (mesh->dim == 3 ? mesh->nz : 1);
   const long nn = (long)(mesh->nx + 1) * (mesh->ny + 1) * (mesh->dim == 3 ? mesh->nz + 1 : 1);
   if (nc > (1L << 31) || nn > (1L << 31)) return MH_ERR_INVALID;
@@ -911,6 +1046,11 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
     S.env[k].a1 = envs[k].linear_pseudo_coeff;
     S.env[k].a2 = envs[k].quadratic_pseudo_coeff;
     S.env[k].stiffened = envs[k].eos == MH_EOS_STIFFENED_GAS;
+    S.env[k].fictif = envs[k].eos == MH_EOS_FICTIF;
+    S.env[k].ft0 = envs[k].fictif_tdebut;
+    S.env[k].ft1 = envs[k].fictif_tfin;
+    S.env[k].fp0 = envs[k].fictif_p0;
+    S.env[k].fdpdt = envs[k].fictif_dpdt;
   }
   S.cfl = options->cfl;
   S.thr_remap = options->remap_threshold;
@@ -928,6 +1068,9 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
   S.newton = options->with_newton ? 1 : 0;
   S.p_explicit = options->pression_explicite ? 1 : 0;
   S.thr_module = options->threshold;
+  S.sans_lagrange = options->sans_lagrange ? 1 : 0;
+  S.rider_reverse = options->rider_reverse ? 1 : 0;
+  S.dt_init = options->deltat_init;
   S.with_proj = options->with_projection ? 1 : 0;
   S.vitson_max = options->calcul_vitson_max ? 1 : 0;
   S.nbc = options->nb_bc;
@@ -984,6 +1127,10 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
       ok |= dalloc(h, &S.hc, (size_t)g.dim * nc);
     if (options->ordre_projection == 3 || options->pente_borne_mixte) ok |= dalloc(h, &S.est, (size_t)nc);
     if (options->projection_pente_borne) ok |= dalloc(h, &S.phi0, (size_t)nv * nc);
+    if (options->sans_lagrange) {  // prescribed velocity and the reverse factors
+      for (int d = 0; d < 3; ++d) ok |= dalloc(h, &h->u0[d], nn);
+      ok |= dalloc(h, &h->rev_cos, I.rev_cos.size());
+    }
   } else {
     ok |= dalloc(h, &h->Ua, 1);
   }
@@ -1051,6 +1198,15 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
   if (S.est) {
     est0.assign(nc, 0);
     ok |= upload(h, S.est, est0);
+  }
+  if (h->rev_cos) {
+    for (int d = 0; d < 3; ++d) {
+      ok |= upload(h, h->u0[d], I.u[d]);
+      S.u0[d] = h->u0[d];
+    }
+    ok |= upload(h, h->rev_cos, I.rev_cos);
+    S.rev_cos = h->rev_cos;
+    S.rev_n = (long)I.rev_cos.size();
   }
   std::vector<double> phi00;  // pente-borne phi: Arcane starts m_phi_lagrange at 0
   if (S.phi0) {
@@ -1388,7 +1544,7 @@ int mahyco_ensemble_add(MhEnsemble* e, const MhMesh* mesh, const MhEnvironment* 
   MhEnsembleImpl* E = (MhEnsembleImpl*)e;
   if (!E || !slot_out || !mesh || !options) return MH_ERR_INVALID;
   *slot_out = -1;
-  if (!options->with_projection) return MH_ERR_UNSUPPORTED;
+  if (!options->with_projection || options->sans_lagrange) return MH_ERR_UNSUPPORTED;
   if (mesh->nx < 2 || mesh->ny < 2 || (mesh->dim == 3 && mesh->nz < 2)) return MH_ERR_UNSUPPORTED;
   const size_t need = case_footprint(*mesh, nb_env, 1);
   if ((double)(E->used + need) > E->budget) return MH_ERR_NOMEM;  // admission by HBM footprint

@@ -229,6 +229,54 @@ __device__ __forceinline__ void k_node_lag_body(const MhDev& S, int lb) {
 }
 
 
+// Advection mode (sans-lagrange), node part: updateVelocityWithoutLagrange
+// (MahycoModule.cc:702-718), u = u0 (1 - r) + (r u0) cos(pi t parametre) with r = 1 under
+// the reverse option, the boundary conditions, and updatePosition (:730-751), which moves
+// only the nodes shared by exactly four cells.
+template <int DIM>
+__device__ __forceinline__ void k_node_adv_body(const MhDev& S, int lb) {
+  const long n = lb * (long)blockDim.x + threadIdx.x;
+  if (n >= S.g.nn) return;
+  if (mh_halted(S.sc)) return;
+  const int nnx = S.g.nnx, nny = S.g.nny;
+  const int i = (int)(n % nnx);
+  const long r = n / nnx;
+  const int j = (int)(r % nny);
+  const int k = (int)(r / nny);
+  const double dt = S.sc->dt;
+  const long it = S.sc->iteration;
+  const double cs = S.rev_cos[it < S.rev_n ? it : S.rev_n - 1];
+  const double opt = S.rider_reverse ? 1. : 0.;
+  double u[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double a = S.u0[d][n];
+    u[d] = a * (1. - opt) + opt * a * cs;
+  }
+  for (int b = 0; b < S.nbc; ++b) {
+    const int sf = S.bc_surf[b];
+    bool on;
+    switch (sf) {
+      case MH_XMIN: on = (i == 0); break;
+      case MH_XMAX: on = (i == S.g.nx); break;
+      case MH_YMIN: on = (j == 0); break;
+      case MH_YMAX: on = (j == S.g.ny); break;
+      case MH_ZMIN: on = DIM == 3 ? (k == 0) : true; break;
+      default: on = DIM == 3 ? (k == S.g.nz) : true; break;
+    }
+    if (on) u[S.bc_type[b]] = S.bc_val[b];
+  }
+  const int cells = ((i > 0) + (i < S.g.nx)) * ((j > 0) + (j < S.g.ny)) * (DIM == 3 ? (k > 0) + (k < S.g.nz) : 1);
+  const bool moves = cells == 4;
+  const double x0[3] = {S.ax[i], S.ay[j], DIM == 3 ? S.az[k] : 0.};
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    S.u[d][n] = u[d];
+    S.xlc[d][n] = moves ? x0[d] + dt * u[d] : x0[d];
+  }
+  if (DIM == 2) S.u[2][n] = u[2];
+}
+
 // 3D node kernel, z-marching tiles: a block owns MH_NTX x MH_NTY node columns over
 // MH_NT_KC node planes, so each cell plane of the tile is (MH_NTX+1) x (MH_NTY+1) = 256
 // cells, one per thread.  Each cell plane
@@ -466,6 +514,16 @@ __device__ __forceinline__ void eos(const MhEnvParam& ep, double rho, double e, 
   }
 }
 
+// FictifEOSService::applyEOS (eos/fictif/FictifEOSService.cc:45-75): the pressure only
+// depends on the time t of the iteration, the sound speed is 1 and the caller zeroes the
+// internal energy.
+__device__ __forceinline__ void fictif_eos(const MhEnvParam& ep, double t, double pprev, double& p, double& c) {
+  p = pprev;
+  if (t > ep.ft0) p = ep.fp0 + ep.fdpdt * t;
+  if (t > ep.ft1) p = 0.;
+  c = 1.;
+}
+
 // ComputeGeometricValues (:1097-1253) + UpdateDensity (:1267-1316) +
 // ComputeArtificialViscosity (:386-426) + UpdateEnergyAndPressure (:1430-1480) for one cell.
 // With remap: writes U (V_k, V_k rho_k, V_k rho_k e_k, V q; PrepareRemap.cc:100-136), V, L.
@@ -489,6 +547,46 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
     }
     if (V < 0.) S.err[0] = MH_ERR_NEG_VOLUME;
     const double L = DIM == 3 ? mh_pow_third(V) : mh_pow_half(V);
+    if (S.sans_lagrange) {
+      // advection mode: no density, pseudo or energy update (MahycoModule.cc:1271, 389,
+      // 1433); the remap vector of PrepareRemap.cc:100-155 from the stored state
+      const uint8_t mk = S.mask[c];
+      const long nc = S.g.nc;
+      double u_out[NV];
+      for (int v = 0; v < NV; ++v) u_out[v] = 0.;
+      if (__popc(mk) == 1) {
+        const int ke = __ffs(mk) - 1;
+        const double rho = S.rho[c], e = S.e[c];
+        u_out[ke] = V;
+        u_out[M + ke] = V * rho;
+        u_out[2 * M + ke] = V * rho * e;
+        if (S.pb) {
+          S.phi0[ke * nc + c] = S.fg[c];
+          S.phi0[(M + ke) * nc + c] = rho;
+          S.phi0[(2 * M + ke) * nc + c] = e;
+        }
+      } else {
+        for (int kk = 0; kk < M; ++kk) {
+          if (!((mk >> kk) & 1)) continue;
+          const long o = kk * nc + c;
+          const double Vk = S.fk[o] * V;
+          u_out[kk] = Vk;
+          u_out[M + kk] = Vk * S.rhok[o];
+          u_out[2 * M + kk] = Vk * S.rhok[o] * S.ek[o];
+          if (S.pb) {
+            S.phi0[o] = S.fk[o];
+            S.phi0[(M + kk) * nc + c] = S.rhok[o];
+            S.phi0[(2 * M + kk) * nc + c] = S.ek[o];
+          }
+        }
+      }
+      u_out[3 * M] = V * S.q[c];
+      if (S.pb && mk) S.phi0[3 * M * nc + c] = S.q[c];
+      for (int v = 0; v < NV; ++v) U[v * nc + c] = u_out[v];
+      S.Vl[c] = V;
+      S.L[c] = L;
+      return dxs;
+    }
     const double dt = S.sc->dt;
     const uint8_t mk = S.mask[c];
     const double cs = S.c[c];
@@ -1427,7 +1525,7 @@ __device__ __forceinline__ void k_sweep_cell_gen_body(const MhDev& S, int lb, in
       S.dflux[(v - M) * nc + c] = dd;
     }
   }
-  if (pb)  // tiny partial volumes, masses or energies drop the environment (:441-463)
+  if (pb && !S.sans_lagrange)  // with the dual phase, tiny partial volumes, masses or energies drop the environment (:441-463)
     for (int kk = 0; kk < M; ++kk) {
       if (un[kk] < 1. * thr) un[kk] = un[M + kk] = un[2 * M + kk] = 0.;
       if (un[M + kk] < 1. * thr) un[kk] = un[M + kk] = un[2 * M + kk] = 0.;
@@ -2037,7 +2135,7 @@ __device__ __forceinline__ void k_finalize_body(const MhDev& S, int lb, int usel
     uint8_t nm = om;
     for (int kk = 0; kk < M; ++kk) {
       const bool was = (om >> kk) & 1;
-      if (((U[kk] / VE) < thr || (U[M + kk] == 0.)) && was) {
+      if (((U[kk] / VE) < thr || (U[M + kk] == 0. && !S.sans_lagrange)) && was) {
         nm &= (uint8_t)~(1u << kk);
       } else if ((U[kk] / VE) > thr && !was) {
         double volt = 0.;
@@ -2157,15 +2255,25 @@ __device__ __forceinline__ void k_finalize_body(const MhDev& S, int lb, int usel
     // applyEOS per environment, computePressionMoyenne (MahycoModule.cc:1708-1735)
     double p_g = 0., c_g = 0.;
     const double opt = S.vitson_max ? 1. : 0.;
+    const double tnow = S.sc->t + S.sc->dt;  // m_global_time of this iteration
     for (int kk = 0; kk < M; ++kk) {
       if (!((nm >> kk) & 1)) continue;
-      eos(S.env[kk], pure ? rho_g : dk[kk], ek[kk], pk[kk], ck[kk]);
+      if (S.env[kk].fictif) {
+        fictif_eos(S.env[kk], tnow, pure ? S.p[c] : S.pk[kk * nc + c], pk[kk], ck[kk]);
+        ek[kk] = 0.;
+        if (pure) e_g = 0.;
+      } else {
+        eos(S.env[kk], pure ? rho_g : dk[kk], ek[kk], pk[kk], ck[kk]);
+      }
       if (pure) {
         p_g = pk[kk];
         c_g = ck[kk];
       }
     }
-    if (!pure) {
+    if (!pure && S.sans_lagrange) {  // computePressionMoyenne returns at once (:1711)
+      p_g = S.p[c];
+      c_g = S.c[c];
+    } else if (!pure) {
       p_g = 0.;
       c_g = 0.;
       for (int kk = 0; kk < M; ++kk) {
@@ -2174,6 +2282,8 @@ __device__ __forceinline__ void k_finalize_body(const MhDev& S, int lb, int usel
         c_g = opt * ((ck[kk] < c_g) ? c_g : ck[kk]);
         c_g += (1. - opt) * f[kk] * ck[kk];
       }
+    }
+    if (!pure) {
       for (int kk = 0; kk < M; ++kk) {
         if (!((nm >> kk) & 1)) continue;
         const long o = kk * nc + c;
@@ -2233,15 +2343,26 @@ __global__ __launch_bounds__(MH_BLOCK) void k_apply_eos(MhDev S, int env) {
   if (c >= S.g.nc) return;
   const uint8_t mk = S.mask[c];
   if (!((mk >> env) & 1)) return;
+  const double tnow = S.sc->t + S.sc->dt;
   if (__popc(mk) == 1) {
     double p, cs;
-    eos(S.env[env], S.rho[c], S.e[c], p, cs);
+    if (S.env[env].fictif) {
+      fictif_eos(S.env[env], tnow, S.p[c], p, cs);
+      S.e[c] = 0.;
+    } else {
+      eos(S.env[env], S.rho[c], S.e[c], p, cs);
+    }
     S.p[c] = p;
     S.c[c] = cs;
   } else {
     const long o = env * S.g.nc + c;
     double p, cs;
-    eos(S.env[env], S.rhok[o], S.ek[o], p, cs);
+    if (S.env[env].fictif) {
+      fictif_eos(S.env[env], tnow, S.pk[o], p, cs);
+      S.ek[o] = 0.;
+    } else {
+      eos(S.env[env], S.rhok[o], S.ek[o], p, cs);
+    }
     S.pk[o] = p;
   }
 }
@@ -2337,7 +2458,8 @@ __device__ __forceinline__ void k_dt_body(const MhDev& S) {
   new_dt = amin(new_dt, 1.05 * dt);
   new_dt = amin(new_dt, S.dtmax);
   int status = S.err[0];
-  if (status == 0 && new_dt < S.dtmin) status = MH_ERR_DT_MIN;
+  if (S.sans_lagrange) new_dt = S.dt_init;  // advection mode keeps deltat-init (:1751-1753)
+  else if (status == 0 && new_dt < S.dtmin) status = MH_ERR_DT_MIN;
   sc->t = t;
   sc->iteration = sc->iteration + 1;
   sc->dxmin = mn;
@@ -2476,6 +2598,7 @@ MH_WRAP(k_sweep_cell_tile, MH_TILE_MIN_WAVES,
 MH_WRAP(k_sweep_node_row, 1, template <int DIM MH_COMMA int LIM>, <DIM MH_COMMA LIM>,
         MH_COMMA int psel MH_COMMA int first MH_COMMA int last, MH_COMMA psel MH_COMMA first MH_COMMA last)
 MH_WRAP(k_hcell, 1, template <int DIM>, <DIM>, , )
+MH_WRAP(k_node_adv, 1, template <int DIM>, <DIM>, , )
 MH_WRAP(k_sweep_cell_gen, 1, template <int DIM MH_COMMA int D MH_COMMA int M>, <DIM MH_COMMA D MH_COMMA M>,
         MH_COMMA int usel MH_COMMA int first, MH_COMMA usel MH_COMMA first)
 MH_WRAP(k_sweep_cell3, 1, template <int DIM MH_COMMA int D MH_COMMA int M>, <DIM MH_COMMA D MH_COMMA M>,

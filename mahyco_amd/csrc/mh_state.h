@@ -28,7 +28,8 @@ struct MhGrid {
 struct MhEnvParam {
   double gamma, pinf, a1, a2;
   int stiffened;
-  int pad;
+  int fictif;                  // Fictif EOS (advection tests): p(t), c = 1, e = 0
+  double ft0, ft1, fp0, fdpdt; // its tdebut / tfin-pression, valeur-debut-pression, valeur-dependance-temps
 };
 
 // Time-loop scalars kept on the device so that cycles run without host round trips.
@@ -97,7 +98,14 @@ struct MhDev {
   double thr_module; // module threshold (Newton tolerance)
   int q_frozen;      // run continued from a dump with m >= 2: V q keeps its Lagrangian value
                      // through the remap (hydroContinueInit, MahycoModule.cc:288)
-  int pad4;
+  // advection mode (sans-lagrange): prescribed node velocity u0 [3][nn] scaled by
+  // rev_cos[iteration] = cos(pi t parametre) with the reverse option, constant step
+  int sans_lagrange;
+  int rider_reverse;
+  double dt_init;
+  const double* u0[3];
+  const double* rev_cos;
+  long rev_n;
   int nblk_c, nblk_n;  // blocks of MH_BLOCK threads over cells / nodes
   int max_iter;        // > 0: stop after this many iterations (fixed-horizon runs)
   int hist_period;     // > 0: EnvSummation rows every hist_period iterations

@@ -608,23 +608,31 @@ void OracleCase::initMatAndVar() {
       if (t == 23) u.x = 1.;
       if (t == 24) u.y = 1.;
       if (t == 25) u = R3{1.0, 1.0, 0.0};
+      // sin and cos of one argument come from one sincos call, as GCC compiles the pairs of
+      // RIDERService.cc:141-160 at -O1 and above (glibc's sincos and sin can differ by an ulp)
+      double s0, c0, s1, c1;
       if (t == 26) {
         const R3 dd = x - cc;
         const double theta = std::atan2(dd.y, dd.x);
         const double r = std::sqrt(dd.x * dd.x + dd.y * dd.y);
         const double omega = 4. * Pi;
-        u.x = -r * omega * std::sin(omega * 0. + theta);
-        u.y = r * omega * std::cos(omega * 0. + theta);
+        ::sincos(omega * 0. + theta, &s0, &c0);
+        u.x = -r * omega * s0;
+        u.y = r * omega * c0;
       }
       if (t == 27 || t == 29) {
         const R3 dd = x;
-        u.x = -2. * std::cos(Pi * dd.y) * std::sin(Pi * dd.y) * std::sin(Pi * dd.x) * std::sin(Pi * dd.x);
-        u.y = 2. * std::cos(Pi * dd.x) * std::sin(Pi * dd.x) * std::sin(Pi * dd.y) * std::sin(Pi * dd.y);
+        ::sincos(Pi * dd.x, &s0, &c0);
+        ::sincos(Pi * dd.y, &s1, &c1);
+        u.x = -2. * c1 * s1 * s0 * s0;
+        u.y = 2. * c0 * s0 * s1 * s1;
       }
       if (t == 28 || t == 30) {
         const R3 dd = x + cc;
-        u.x = std::sin(4. * Pi * dd.x) * std::sin(4. * Pi * dd.y);
-        u.y = std::cos(4. * Pi * dd.x) * std::cos(4. * Pi * dd.y);
+        ::sincos(4. * Pi * dd.x, &s0, &c0);
+        ::sincos(4. * Pi * dd.y, &s1, &c1);
+        u.x = s0 * s1;
+        u.y = c0 * c1;
       }
       velocity[n] = u;
     }

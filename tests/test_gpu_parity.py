@@ -77,6 +77,14 @@ CASES = {
     "riemann3d_stiffened_newton": (lambda: _variant(_riemann3d(21), with_newton=True), 60),
     "riemann2d_explicit_pressure": (lambda: _variant(_riemann2d(True, 22), pression_explicite=True), 150),
     "sod2d_lagrange_explicit_pressure": (lambda: _variant(cases.config1_sod2d_lagrange(), pression_explicite=True), 800),
+    # advection mode (sans-lagrange) with the RIDER bubble: one and two environments, the
+    # reverse option, pente-borne with the pure-neighbourhood limiter
+    "monorider_rotation_fixture": (lambda: cases.rider("mr_rot", 26, .5), 500),
+    "rider_deformation_reverse_fixture": (lambda: cases.rider("r_def_rev", 38, 1., 1.), 1000),
+    "rider_vortex_reverse_fixture": (lambda: cases.rider("r_vort_rev", 37, 4., .25), 600),
+    "rider_ty_pb_mixte_fixture": (lambda: _variant(cases.rider("r_ty_pb", 32, .5), pente_borne=True, debar_fix=2,
+                                                   limiter=T.MH_SUPERBEE, pente_borne_mixte=True,
+                                                   limiter_pure=T.MH_VANLEER), 500),
 }
 
 

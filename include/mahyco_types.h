@@ -63,7 +63,8 @@ enum MhCaseModel {
   MH_CASE_SEDOV = 1,       /* casTest/SEDOV/SEDOVService.cc; cas_test 1 (mono) */
   MH_CASE_RIEMANN = 2,     /* two-state shock tube with geometric-overlap volume fractions */
   MH_CASE_POINTTRIPLE = 3, /* casTest/POINTTRIPLE/POINTTRIPLEService.cc; cas_test 12 (3 envs) */
-  MH_CASE_RIDER = 4        /* casTest/RIDER/RIDERService.cc; cas_test 23-30 (mono), 31-38 (bubble in 2 envs) */
+  MH_CASE_RIDER = 4,       /* casTest/RIDER/RIDERService.cc; cas_test 23-30 (mono), 31-38 (bubble in 2 envs) */
+  MH_CASE_ADVECTION = 5    /* casTest/ADVECTION/ADVECTIONService.cc; cas_test 7-10 (mono), 18-21 (square in 2 envs) */
 };
 
 /* Node coordinate convention of the Cartesian generator. */
@@ -132,10 +133,11 @@ typedef struct MhOptions {
   int pression_explicite; /* pression-explicite: explicit pressure work */
   /* advection mode and the RIDER case service */
   int sans_lagrange;         /* sans-lagrange: prescribed velocity, remap without the dual phase */
-  int rider_reverse;         /* RIDER reverse-option: velocity x cos(pi t parametre) */
-  double rider_parametre;    /* RIDER parametre */
-  double rider_rayon_bulle;  /* RIDER rayon-bulle (reference default 0.15) */
-  int rider_densite_lineaire;/* RIDER densite-interieure-initiale-lineaire */
+  /* options of the RIDER and ADVECTION case services (the same names in RIDER.axl and ADVECTION.axl) */
+  int rider_reverse;         /* reverse-option: velocity x cos(pi t parametre) */
+  double rider_parametre;    /* parametre */
+  double rider_rayon_bulle;  /* rayon-bulle (reference default 0.15 for RIDER, 0.25 for ADVECTION) */
+  int rider_densite_lineaire;/* densite-interieure-initiale-lineaire */
   int pad_rider;
 } MhOptions;
 

@@ -330,6 +330,27 @@ def rider(name, cas_test, final_time, reverse=None, bcs=None) -> Case:
     return c
 
 
+ADVECTION_TESTS = {"MonoAdvectionTx": 7, "MonoAdvectionTy": 8, "MonoAdvectionT45": 9, "MonoAdvectionRotation": 10,
+                   "AdvectionTx": 18, "AdvectionTy": 19, "AdvectionT45": 20, "AdvectionRotation": 21}
+
+
+def advection(name, cas_test, final_time, n=50, dt=1e-3, ordre=2, reverse=None, lineaire=False, bcs=None) -> Case:
+    """ADVECTIONService (casTest/ADVECTION): pure advection (sans-lagrange) of a square of side
+    2 rayon-bulle (0.25) centred at (0.5, 0.5), or (0.5, 0.75) for the rotations, on an n x n x 1
+    mesh of [0,1]^2 x [0,0.1]; two Fictif environments (rho 0 outside, 1 inside).  The
+    reference holds no dataset for this service; the layout is the RIDER datasets'."""
+    c = Case(name, 3, (n, n, 1), (0., 0., 0.), (1., 1., .1), [_fictif(0.), _fictif(1.)], T.MH_CASE_ADVECTION,
+             cas_test, deltat_init=dt, deltat_min=1e-8, deltat_max=1e-4, final_time=final_time,
+             bcs=list(bcs or []))
+    c.ordre_projection = ordre
+    c.sans_lagrange = True
+    c.rider_rayon_bulle = .25
+    c.rider_densite_lineaire = lineaire
+    if reverse is not None:
+        c.rider_reverse, c.rider_parametre = True, reverse
+    return c
+
+
 # ----------------------------------------------------------------------------- workloads
 def config1_sod2d_lagrange() -> Case:
     """2D Sod 200x10 on [0,1]x[0,0.1], single material, Lagrange only, CFL 0.3, t=0.2."""

@@ -256,6 +256,12 @@ int host_init(MhHandle* h, HostInit& I) {
           } else if ((rmax >= rb) && (rmin < rb) && !lin) {
             rider_frac[c] = (rb - rmin) / (rmax - rmin);
           }
+        } else if (opt.cas_model == MH_CASE_ADVECTION) {  // ADVECTIONService::initMat :11-65
+          if (opt.cas_test >= 18) {  // the Mono* tests (7-10) put every cell in environment 0
+            const double yb = opt.cas_test < 21 ? 0.50 : 0.75;
+            const double rb = 0.25;  // a constant here, the rayon-bulle option in initVar
+            if (std::fabs(xc.x - 0.50) < rb && std::fabs(xc.y - yb) < rb) mat = 1.;
+          }
         } else if (opt.cas_model == MH_CASE_RIEMANN) {
           const int a = opt.interface_axis;
           const D3 lo3 = x[0], hi3 = a == 0 ? x[1] : a == 1 ? x[3] : x[4];
@@ -356,6 +362,20 @@ int host_init(MhHandle* h, HostInit& I) {
         I.rho[c] = frac_b * h->env[1].densite_initiale;
         I.p[c] = h->env[1].pression_initiale;
       }
+    } else if (opt.cas_model == MH_CASE_ADVECTION) {  // ADVECTIONService::initVarMono :67-145, initVar :146-246
+      // a square of half side rayon-bulle, no mixed cells, internal energy left at 0
+      const bool lin = opt.rider_densite_lineaire != 0;
+      const double rb = opt.rider_rayon_bulle;
+      const double xb = 0.50, yb = opt.cas_test < (opt.cas_test <= 10 ? 10 : 21) ? 0.50 : 0.75;
+      const double croissance = lin ? 1. / rb : 1.;
+      I.rho[c] = h->env[0].densite_initiale;
+      I.p[c] = h->env[0].pression_initiale;
+      const double r = std::sqrt((xcx[c] - xb) * (xcx[c] - xb) + (xcy[c] - yb) * (xcy[c] - yb));
+      if (std::fabs(xcx[c] - xb) < rb && std::fabs(xcy[c] - yb) < rb) {
+        I.rho[c] = h->env[1].densite_initiale;
+        if (lin) I.rho[c] = h->env[1].densite_initiale * croissance * r;
+        I.p[c] = h->env[1].pression_initiale;
+      }
     } else if (opt.cas_model == MH_CASE_RIEMANN) {
       if (pure) {
         const int k = bits == 1 ? 0 : 1;
@@ -378,8 +398,12 @@ int host_init(MhHandle* h, HostInit& I) {
     }
   }
   const double* v0 = h->env[0].vitesse_initiale;
-  if (opt.cas_model == MH_CASE_RIDER) {  // RIDERService.cc:134-157, 267-289 (the same field per Mono/bi pair)
-    const int t = opt.cas_test <= 30 ? opt.cas_test : opt.cas_test - 8;
+  if (opt.cas_model == MH_CASE_RIDER || opt.cas_model == MH_CASE_ADVECTION) {
+    // RIDERService.cc:134-157, 267-289 and ADVECTIONService.cc:131-144, 233-246, numbered as
+    // RIDER's fields: 23 Tx, 24 Ty, 25 T45, 26 rotation, 27/29 vortex, 28/30 deformation
+    const int t = opt.cas_model == MH_CASE_ADVECTION ? (opt.cas_test <= 10 ? opt.cas_test : opt.cas_test - 11) + 16
+                  : opt.cas_test <= 30                ? opt.cas_test
+                                                      : opt.cas_test - 8;
     const double Pi = kRiderPi;
     for (long n = 0; n < nn; ++n) {
       const int i = (int)(n % g.nnx);
@@ -693,7 +717,10 @@ int validate(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const Mh
   if (o->nb_bc < 0 || o->nb_bc > MH_MAX_BC) return MH_ERR_INVALID;
   for (int b = 0; b < o->nb_bc; ++b)
     if (o->bc[b].surface < 0 || o->bc[b].surface > 5 || o->bc[b].type < 0 || o->bc[b].type > 2) return MH_ERR_INVALID;
-  if (o->cas_model < 0 || o->cas_model > 4) return MH_ERR_UNSUPPORTED;
+  if (o->cas_model < 0 || o->cas_model > 5) return MH_ERR_UNSUPPORTED;
+  if (o->cas_model == MH_CASE_ADVECTION && !((o->cas_test >= 7 && o->cas_test <= 10) || (o->cas_test >= 18 && o->cas_test <= 21)))
+    return MH_ERR_UNSUPPORTED;
+  if (o->cas_model == MH_CASE_ADVECTION && nb_env < 2) return MH_ERR_INVALID;  // the square takes env 1's density
   if (o->cas_model == MH_CASE_RIDER && (o->cas_test < 23 || o->cas_test > 38)) return MH_ERR_UNSUPPORTED;
   if (o->cas_model == MH_CASE_RIDER && o->cas_test >= 31 && nb_env < 2) return MH_ERR_INVALID;
   if (o->sans_lagrange && (!o->with_projection || !(o->deltat_init > 0.) ||

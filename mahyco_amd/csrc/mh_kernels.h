@@ -524,6 +524,20 @@ __device__ __forceinline__ void fictif_eos(const MhEnvParam& ep, double t, doubl
   c = 1.;
 }
 
+// A cell that no environment occupies gets no remap values from computeVariablesForRemap
+// (PrepareRemap.cc:100-166 loop over environment cells): its U is 0 and its Phi is the one
+// the last sweep of the previous remap left, U / sum V_k (RemapADIService.cc:493-538).
+// Without pente-borne the remap vector written for such a cell holds that Phi, read by the
+// first sweep with a unit denominator, and the sweep starts its own U from 0.
+template <int M>
+__device__ __forceinline__ void empty_cell_phi(const MhDev& S, long c, double* u_out) {
+  const long nc = S.g.nc;
+  const double* __restrict__ Ulast = (S.nsweep & 1) ? S.Ub : S.Ua;
+  double sv = 0.;
+  for (int kk = 0; kk < M; ++kk) sv += Ulast[kk * nc + c];
+  for (int v = 0; v <= 3 * M; ++v) u_out[v] = Ulast[v * nc + c] / sv;
+}
+
 // ComputeGeometricValues (:1097-1253) + UpdateDensity (:1267-1316) +
 // ComputeArtificialViscosity (:386-426) + UpdateEnergyAndPressure (:1430-1480) for one cell.
 // With remap: writes U (V_k, V_k rho_k, V_k rho_k e_k, V q; PrepareRemap.cc:100-136), V, L.
@@ -582,6 +596,7 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
       }
       u_out[3 * M] = V * S.q[c];
       if (S.pb && mk) S.phi0[3 * M * nc + c] = S.q[c];
+      if (!mk && !S.pb && S.nsweep > 0) empty_cell_phi<M>(S, c, u_out);
       for (int v = 0; v < NV; ++v) U[v * nc + c] = u_out[v];
       S.Vl[c] = V;
       S.L[c] = L;
@@ -665,7 +680,7 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
             S.phi0[(M + kk) * nc + c] = rk[kk];
             S.phi0[(2 * M + kk) * nc + c] = ek[kk];
           }
-          S.phi0[3 * M * nc + c] = qsum;
+          if (mk) S.phi0[3 * M * nc + c] = qsum;  // an empty cell keeps the last sweep's
         }
       } else {
         double ps = 0., cmix = 0.;
@@ -690,6 +705,7 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
         dxs = L / (cmix + cell_vmax<DIM>(S, b));
       }
     }
+    if (S.with_proj && !mk && !S.pb && S.nsweep > 0) empty_cell_phi<M>(S, c, u_out);
     if (S.with_proj)
       for (int v = 0; v < NV; ++v) U[v * nc + c] = u_out[v];
     S.Vl[c] = V;
@@ -803,6 +819,7 @@ __device__ __forceinline__ void k_sweep_cell_body(const MhDev& S, int lb, int us
   const long c = lb * (long)blockDim.x + threadIdx.x;
   if (c >= S.g.nc) return;
   if (mh_halted(S.sc)) return;
+  const bool own = !(first && S.mask[c] == 0);  // an empty cell's U starts from 0 (its Phi is stale)
   const int nx = S.g.nx, ny = S.g.ny;
   const int i = (int)(c % nx);
   const long r = c / nx;
@@ -827,7 +844,7 @@ __device__ __forceinline__ void k_sweep_cell_body(const MhDev& S, int lb, int us
     if (valid[o + 2]) {
       const long cc = c + o * sD;
       if (first) {
-        den[o + 2] = S.Vl[cc];
+        den[o + 2] = S.mask[cc] ? S.Vl[cc] : 1.;  // an empty cell holds Phi itself
       } else {
         double sv = 0.;
         for (int kk = 0; kk < M; ++kk) sv += Uin[kk * nc + cc];
@@ -890,7 +907,7 @@ __device__ __forceinline__ void k_sweep_cell_body(const MhDev& S, int lb, int us
     double ff = 0.;
     if (inner[s0]) ff += flux[s0];
     if (inner[s1]) ff += flux[s1];
-    double un = Uin[v * nc + c] - ff;
+    double un = (own ? Uin[v * nc + c] : 0.) - ff;
     if (v >= M && v < 2 * M) {
       double dd = 0.;
       if (inner[s0]) dd += 0.5 * flux[s0] * od[s0];
@@ -1051,6 +1068,7 @@ __device__ __forceinline__ void k_sweep_cell3_body(const MhDev& S, int lb, int u
   const long c = lb * (long)blockDim.x + threadIdx.x;
   if (c >= S.g.nc) return;
   if (mh_halted(S.sc)) return;
+  const bool own = !(first && S.mask[c] == 0);  // an empty cell's U starts from 0 (its Phi is stale)
   const int nx = S.g.nx, ny = S.g.ny;
   const int i = (int)(c % nx);
   const long r = c / nx;
@@ -1071,7 +1089,7 @@ __device__ __forceinline__ void k_sweep_cell3_body(const MhDev& S, int lb, int u
     const int ao = min(max(a + o - 3, 0), na - 1);
     cs[o] = c + (ao - a) * sD;
     if (first) {
-      den[o] = S.Vl[cs[o]];
+      den[o] = S.mask[cs[o]] ? S.Vl[cs[o]] : 1.;  // an empty cell holds Phi itself
     } else {
       double sv = 0.;
       for (int kk = 0; kk < M; ++kk) sv += Uin[kk * nc + cs[o]];
@@ -1149,7 +1167,7 @@ __device__ __forceinline__ void k_sweep_cell3_body(const MhDev& S, int lb, int u
     double ff = 0.;
     if (inner[s0]) ff += flux[s0];
     if (inner[s1]) ff += flux[s1];
-    double un = Uin[v * nc + c] - ff;
+    double un = (own ? Uin[v * nc + c] : 0.) - ff;
     if (v >= M && v < 2 * M) {
       double dd = 0.;
       if (inner[s0]) dd += 0.5 * flux[s0] * od[s0];
@@ -1316,6 +1334,7 @@ __device__ __forceinline__ void k_sweep_cell_gen_body(const MhDev& S, int lb, in
   const long c = lb * (long)blockDim.x + threadIdx.x;
   if (c >= S.g.nc) return;
   if (mh_halted(S.sc)) return;
+  const bool own = !(first && S.mask[c] == 0);  // an empty cell's U starts from 0 (its Phi is stale)
   const int nx = S.g.nx, ny = S.g.ny;
   const int i = (int)(c % nx);
   const long r = c / nx;
@@ -1344,7 +1363,7 @@ __device__ __forceinline__ void k_sweep_cell_gen_body(const MhDev& S, int lb, in
     hs[o] = S.hc[D * nc + cs[o]];
     xc[o] = valid[o] ? cell_centre_d<DIM, D>(S, b + (o - 2) * nD) : 0.;
     if (first && !pb) {
-      den[o] = S.Vl[cs[o]];
+      den[o] = S.mask[cs[o]] ? S.Vl[cs[o]] : 1.;  // an empty cell holds Phi itself
     } else {
       double sv = 0.;
       if (!first)
@@ -1517,7 +1536,7 @@ __device__ __forceinline__ void k_sweep_cell_gen_body(const MhDev& S, int lb, in
     double ff = 0.;
     if (inner[s0]) ff += flux[s0];
     if (inner[s1]) ff += flux[s1];
-    un[v] = Uin[v * nc + c] - ff;
+    un[v] = (own ? Uin[v * nc + c] : 0.) - ff;
     if (v >= M && v < 2 * M) {
       double dd = 0.;
       if (inner[s0]) dd += 0.5 * flux[s0] * od[s0];
@@ -1565,6 +1584,7 @@ __device__ __forceinline__ void k_sweep_cell_row_body(const MhDev& S, int lb, in
   const bool out = valid && lane >= 2 && lane < 2 + MH_ROW_OUT;
   const long nc = S.g.nc;
   const long c = row * nx + i;
+  const bool own = !(first && valid && S.mask[c] == 0);  // an empty cell's U starts from 0
   const int j = (int)(row % S.g.ny);
   const int k = (int)(row / S.g.ny);
   const long b = i + (long)S.g.nnx * (j + (long)S.g.nny * k);
@@ -1572,7 +1592,7 @@ __device__ __forceinline__ void k_sweep_cell_row_body(const MhDev& S, int lb, in
   double den = 1., xc = 0.;
   if (valid) {
     if (first) {
-      den = S.Vl[c];
+      den = S.mask[c] ? S.Vl[c] : 1.;  // an empty cell holds Phi itself
     } else {
       double sv = 0.;
       for (int kk = 0; kk < M; ++kk) sv += Uin[kk * nc + c];
@@ -1624,7 +1644,7 @@ __device__ __forceinline__ void k_sweep_cell_row_body(const MhDev& S, int lb, in
       if (inf) dd += 0.5 * Ff * 1.;
     }
     if (out) {
-      double un = uv - ff;
+      double un = (own ? uv : 0.) - ff;
       if (v >= M && v < 2 * M) dflux[(v - M) * nc + c] = dd;
       if (v >= M && v < 3 * M && un < 0.) un = 0.;
       Uout[v * nc + c] = un;
@@ -1841,7 +1861,7 @@ __device__ __forceinline__ void k_sweep_cell_tile_body(const MhDev& S, int lb_, 
     xcq[q] = 0.;
     if (val[q]) {
       const long cc = c_base + a * sD;
-      if (first) den[q] = S.Vl[cc];
+      if (first) den[q] = S.mask[cc] ? S.Vl[cc] : 1.;  // an empty cell holds Phi itself
       else {
         double sv = 0.;
         for (int k = 0; k < M; ++k) sv += Uin[k * nc + cc];
@@ -1925,7 +1945,7 @@ __device__ __forceinline__ void k_sweep_cell_tile_body(const MhDev& S, int lb_, 
       if (inb) ff += fb;
       if (inf) ff += Ff;
       const long c = c_base + a * sD;
-      double un = uq[q] - ff;
+      double un = (first && !S.mask[c] ? 0. : uq[q]) - ff;
       if (v >= M && v < 2 * M) {
         double dd = 0.;
         if (inb) dd += 0.5 * fb * -1.;

@@ -35,7 +35,7 @@ BC_TYPES = {"Vx": MH_BC_VX, "Vy": MH_BC_VY, "Vz": MH_BC_VZ}
 MH_MINMOD, MH_SUPERBEE, MH_VANLEER = 0, 1, 2
 MH_MINMOD_G, MH_SUPERBEE_G, MH_VANLEER_G, MH_ARITHMETIC_G, MH_ULTRABEE_G = 3, 4, 5, 6, 7
 
-MH_CASE_SOD, MH_CASE_SEDOV, MH_CASE_RIEMANN, MH_CASE_POINTTRIPLE, MH_CASE_RIDER = range(5)
+MH_CASE_SOD, MH_CASE_SEDOV, MH_CASE_RIEMANN, MH_CASE_POINTTRIPLE, MH_CASE_RIDER, MH_CASE_ADVECTION = range(6)
 MH_COORD_MUL, MH_COORD_DIV = 0, 1
 
 # MhField

@@ -439,6 +439,12 @@ void OracleCase::hydroStartInitEnvAndMat() {
         if (rmax < rb) mat = 1.;
         else if ((rmax >= rb) && (rmin < rb)) mat = (rb - rmin) / (rmax - rmin);
       }
+    } else if (opt.cas_model == MH_CASE_ADVECTION) {  // ADVECTIONService::initMat :11-65
+      if (opt.cas_test >= 18) {  // the Mono* tests (7-10) put every cell in environment 0
+        const R3 Xb = opt.cas_test < 21 ? R3{0.50, 0.50, 0.} : R3{0.50, 0.75, 0.};
+        const double rb = 0.25;  // a constant here, the rayon-bulle option in initVar
+        if (std::fabs(cell_coord[c].x - Xb.x) < rb && std::fabs(cell_coord[c].y - Xb.y) < rb) mat = 1.;
+      }
     } else if (opt.cas_model == MH_CASE_RIEMANN) {
       int a = opt.interface_axis;
       int n0 = cell_nodes[c * nnpc + 0];
@@ -599,9 +605,37 @@ void OracleCase::initMatAndVar() {
         }
       }
     }
+  }
+  if (opt.cas_model == MH_CASE_ADVECTION) {  // ADVECTIONService::initVarMono :67-145, initVar :146-246
+    // a square of half side rayon-bulle; no mixed cells (the cut-cell branch is commented
+    // out in the reference); the internal energy is left as allocated (0)
+    const bool mono = opt.cas_test <= 10;
+    const bool lin = opt.rider_densite_lineaire != 0;
+    const double rb = opt.rider_rayon_bulle;
+    const R3 Xb = (opt.cas_test < (mono ? 10 : 21)) ? R3{0.50, 0.50, 0.} : R3{0.50, 0.75, 0.};
+    const double croissance = lin ? 1. / rb : 1.;
+    for (long c = 0; c < nc; ++c) {
+      density.g[c] = env[0].densite_initiale;
+      pressure.g[c] = env[0].pression_initiale;
+      const double r = std::sqrt((cell_coord[c].x - Xb.x) * (cell_coord[c].x - Xb.x) +
+                                 (cell_coord[c].y - Xb.y) * (cell_coord[c].y - Xb.y));
+      const double rx = std::fabs(cell_coord[c].x - Xb.x), ry = std::fabs(cell_coord[c].y - Xb.y);
+      if ((rx < rb) && (ry < rb)) {
+        density.g[c] = env[1].densite_initiale;
+        if (lin) density.g[c] = env[1].densite_initiale * croissance * r;
+        pressure.g[c] = env[1].pression_initiale;
+      }
+    }
+  }
+  if (opt.cas_model == MH_CASE_RIDER || opt.cas_model == MH_CASE_ADVECTION) {
     const R3 cc{0.5, 0.5, 0.};
     const double Pi = 3.14159265359;  // TypesMahyco.h:42
-    const int t = mono ? opt.cas_test : opt.cas_test - 8;  // the same velocity field per Mono/bi pair
+    // one numbering of the velocity fields: 23 Tx, 24 Ty, 25 T45, 26 rotation, 27/29 vortex,
+    // 28/30 deformation (RIDER Mono/bi pairs share a field; ADVECTION 7-10 and 18-21 are
+    // its first four, ADVECTIONService.cc:131-144, 233-246)
+    const int t = opt.cas_model == MH_CASE_ADVECTION ? (opt.cas_test <= 10 ? opt.cas_test : opt.cas_test - 11) + 16
+                  : opt.cas_test <= 30                ? opt.cas_test
+                                                      : opt.cas_test - 8;
     for (long n = 0; n < nn; ++n) {
       R3 u{0., 0., 0.};
       const R3 x = node_coord[n];

@@ -85,6 +85,15 @@ CASES = {
     "rider_ty_pb_mixte_fixture": (lambda: _variant(cases.rider("r_ty_pb", 32, .5), pente_borne=True, debar_fix=2,
                                                    limiter=T.MH_SUPERBEE, pente_borne_mixte=True,
                                                    limiter_pure=T.MH_VANLEER), 500),
+    # advection mode with the ADVECTION square: the reversed rotation, the linear density at
+    # Courant number just below 1 (cells emptied of both environments), order 3, pente-borne
+    # with a generalised limiter
+    "monoadvection_rotation_reverse": (lambda: cases.advection("ma_rot", 10, .25, n=64, reverse=2.), 250),
+    "advection_t45_lineaire_courant1": (lambda: cases.advection("a_t45", 20, 1., n=50, dt=(1. - 2. ** -30) / 50,
+                                                                ordre=1, lineaire=True), 8),
+    "advection_rotation_ordre3": (lambda: cases.advection("a_rot_o3", 21, .1, n=48, ordre=3), 100),
+    "advection_ty_pb_superbeeG": (lambda: _variant(cases.advection("a_ty_pbg", 19, .2, n=40, dt=2e-3),
+                                                   pente_borne=True, debar_fix=1, limiter=T.MH_SUPERBEE_G), 100),
 }
 
 

@@ -138,7 +138,10 @@ typedef struct MhOptions {
   double rider_parametre;    /* parametre */
   double rider_rayon_bulle;  /* rayon-bulle (reference default 0.15 for RIDER, 0.25 for ADVECTION) */
   int rider_densite_lineaire;/* densite-interieure-initiale-lineaire */
-  int pad_rider;
+  /* schema-csts (Mahyco.axl:396-399): velocity taken back to n-1/2 before the Lagrange step and on to
+     n+1 before the remap, energy from the corner-force work at n and n+1 (MahycoModule.cc:451-453,
+     574-690, 1661-1697; PrepareRemap.cc:210, 236-240) */
+  int schema_csts;
 } MhOptions;
 
 /* Field identifiers for get/set.  Cell fields take an environment index

@@ -81,6 +81,7 @@ class Case:
     simple_pente: bool = False
     with_newton: bool = False
     pression_explicite: bool = False
+    schema_csts: bool = False
     sans_lagrange: bool = False
     rider_reverse: bool = False
     rider_parametre: float = 0.0
@@ -153,6 +154,7 @@ class Case:
         o.projection_simple_pente = int(self.simple_pente)
         o.with_newton = int(self.with_newton)
         o.pression_explicite = int(self.pression_explicite)
+        o.schema_csts = int(self.schema_csts)
         o.sans_lagrange = int(self.sans_lagrange)
         o.rider_reverse = int(self.rider_reverse)
         o.rider_parametre = self.rider_parametre

@@ -21,7 +21,8 @@ namespace {
 // profiled kernel groups (sweeps separately per direction: they run different code)
 const char* kKernelNames[] = {"k_node_lag",     "k_cell_lag",     "k_sweep_cell_x",  "k_sweep_cell_y",
                               "k_sweep_cell_z", "k_sweep_node_x", "k_sweep_node_y",  "k_sweep_node_z",
-                              "k_node_velocity", "k_finalize",    "k_dt",            "k_hcell"};
+                              "k_node_velocity", "k_finalize",    "k_dt",            "k_hcell",
+                              "k_node_fwd"};
 enum {
   KN_NODE = 0,
   KN_CELL,
@@ -31,6 +32,7 @@ enum {
   KN_FINAL,
   KN_DT,
   KN_HCELL,  // cell widths of the order-3 remap
+  KN_NODE_FWD,  // CSTS forward velocity step before the remap
   KN_COUNT
 };
 
@@ -581,6 +583,7 @@ void launch_sweeps_t(MhHandle* h) {
   const bool o3 = S.order2 == 2;
   // pente-borne or generalised limiters: one thread per cell and node, limiter read at run time
   const bool gen = !o3 && (S.pb || (S.order2 && S.limiter >= MH_MINMOD_G));
+  if (S.csts) LAUNCH(h, KN_NODE_FWD, (k_node_fwd<DIM>), h->nblk_n, MH_BLOCK, S);  // PrepareRemap.cc:210
   if (S.hc && nd > 0) LAUNCH(h, KN_HCELL, (k_hcell<DIM>), h->nblk_c, MH_BLOCK, S);
   for (int s = 0; s < nd; ++s) {
     const int first = s == 0, last = (s == nd - 1) && !first, sel = s & 1;
@@ -726,6 +729,12 @@ int validate(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const Mh
   if (o->sans_lagrange && (!o->with_projection || !(o->deltat_init > 0.) ||
                            !(o->final_time / o->deltat_init <= 1e7)))
     return MH_ERR_UNSUPPORTED;  // the reverse factors are tabulated per iteration up to the final time
+  // CSTS: single-environment remap runs with the closed-form energy (with Newton the reference
+  // reads p before setting it, MahycoModule.cc:1595-1601; explicit pressure applies the EOS
+  // before the forward step)
+  if (o->schema_csts && (nb_env != 1 || !o->with_projection || o->with_newton || o->pression_explicite ||
+                         o->sans_lagrange))
+    return MH_ERR_UNSUPPORTED;
   if (o->cas_model == MH_CASE_RIEMANN && nb_env < 2) return MH_ERR_INVALID;
   if (o->cas_model == MH_CASE_POINTTRIPLE && nb_env < 3) return MH_ERR_INVALID;
   for (int k = 0; k < nb_env; ++k)
@@ -1097,6 +1106,7 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
   S.thr_module = options->threshold;
   S.sans_lagrange = options->sans_lagrange ? 1 : 0;
   S.rider_reverse = options->rider_reverse ? 1 : 0;
+  S.csts = options->schema_csts ? 1 : 0;
   S.dt_init = options->deltat_init;
   S.with_proj = options->with_projection ? 1 : 0;
   S.vitson_max = options->calcul_vitson_max ? 1 : 0;
@@ -1158,6 +1168,10 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
       for (int d = 0; d < 3; ++d) ok |= dalloc(h, &h->u0[d], nn);
       ok |= dalloc(h, &h->rev_cos, I.rev_cos.size());
     }
+    if (options->schema_csts) {  // saved velocity, pseudo-viscosity of the previous cycle
+      for (int d = 0; d < 3; ++d) ok |= dalloc(h, &S.un[d], nn);
+      ok |= dalloc(h, &S.qm1, nc);
+    }
   } else {
     ok |= dalloc(h, &h->Ua, 1);
   }
@@ -1204,6 +1218,11 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
     ok |= upload(h, S.u[d], I.u[d]);
     ok |= upload(h, h->xl[0][d], xs[d]);
     ok |= upload(h, h->xl[1][d], xs[d]);
+  }
+  if (S.csts) {  // m_pseudo_viscosity_nmoins1 starts at 0; the saved velocity is written before use
+    const std::vector<double> zc(nc, 0.), zn(nn, 0.);
+    ok |= upload(h, S.qm1, zc);
+    for (int d = 0; d < 3; ++d) ok |= upload(h, S.un[d], zn);
   }
   ok |= upload(h, S.rho, I.rho);
   ok |= upload(h, S.p, I.p);
@@ -1571,7 +1590,7 @@ int mahyco_ensemble_add(MhEnsemble* e, const MhMesh* mesh, const MhEnvironment* 
   MhEnsembleImpl* E = (MhEnsembleImpl*)e;
   if (!E || !slot_out || !mesh || !options) return MH_ERR_INVALID;
   *slot_out = -1;
-  if (!options->with_projection || options->sans_lagrange) return MH_ERR_UNSUPPORTED;
+  if (!options->with_projection || options->sans_lagrange || options->schema_csts) return MH_ERR_UNSUPPORTED;
   if (mesh->nx < 2 || mesh->ny < 2 || (mesh->dim == 3 && mesh->nz < 2)) return MH_ERR_UNSUPPORTED;
   const size_t need = case_footprint(*mesh, nb_env, 1);
   if ((double)(E->used + need) > E->budget) return MH_ERR_NOMEM;  // admission by HBM footprint
@@ -1792,7 +1811,10 @@ static bool same_options(const MhOptions& a, const MhOptions& b) {
             a.pression_explicite == b.pression_explicite &&
             a.projection_limiteur_id == b.projection_limiteur_id && a.remap_threshold == b.remap_threshold &&
             a.cas_model == b.cas_model && a.cas_test == b.cas_test && a.interface_position == b.interface_position &&
-            a.interface_axis == b.interface_axis && a.sedov_deposit == b.sedov_deposit;
+            a.interface_axis == b.interface_axis && a.sedov_deposit == b.sedov_deposit &&
+            a.sans_lagrange == b.sans_lagrange && a.rider_reverse == b.rider_reverse &&
+            a.rider_parametre == b.rider_parametre && a.rider_rayon_bulle == b.rider_rayon_bulle &&
+            a.rider_densite_lineaire == b.rider_densite_lineaire && a.schema_csts == b.schema_csts;
   for (int i = 0; eq && i < a.nb_bc && i < MH_MAX_BC; ++i)
     eq = a.bc[i].surface == b.bc[i].surface && a.bc[i].type == b.bc[i].type && a.bc[i].value == b.bc[i].value;
   return eq;

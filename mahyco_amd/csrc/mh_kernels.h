@@ -113,9 +113,12 @@ __device__ __forceinline__ void block_min_store(double v, bool active, double* o
 // ----------------------------------------------------------------------------- nodes
 // Corner L of cell (ii,jj,kk) seen from node (i,j,k); accumulates mass and force in the
 // order of a cell-order scatter (MahycoModule.cc:247-262 and :455-462).
-template <int DIM, int DI, int DJ, int DK>
+// FWD: the corner vectors at this cycle's Lagrangian positions (updateVelocityForward,
+// :638-690) instead of the previous cycle's.
+template <int DIM, int DI, int DJ, int DK, int FWD = 0>
 __device__ __forceinline__ void node_gather(const MhDev& S, int i, int j, int k, double& mass,
                                             double& fx, double& fy, double& fz) {
+  double* const* X = FWD ? S.xlc : S.xlp;
   const int ii = i - DI, jj = j - DJ, kk = k - DK;
   if (ii < 0 || ii >= S.g.nx || jj < 0 || jj >= S.g.ny) return;
   if (DIM == 3 && (kk < 0 || kk >= S.g.nz)) return;
@@ -129,21 +132,21 @@ __device__ __forceinline__ void node_gather(const MhDev& S, int i, int j, int k,
   D3 q;
   if (DIM == 3) {
     D3 x[8];
-    x[0] = load3(S.xlp, b);
-    x[1] = load3(S.xlp, b + 1);
-    x[2] = load3(S.xlp, b + 1 + nnx);
-    x[3] = load3(S.xlp, b + nnx);
-    x[4] = load3(S.xlp, b + nnxy);
-    x[5] = load3(S.xlp, b + 1 + nnxy);
-    x[6] = load3(S.xlp, b + 1 + nnx + nnxy);
-    x[7] = load3(S.xlp, b + nnx + nnxy);
+    x[0] = load3(X, b);
+    x[1] = load3(X, b + 1);
+    x[2] = load3(X, b + 1 + nnx);
+    x[3] = load3(X, b + nnx);
+    x[4] = load3(X, b + nnxy);
+    x[5] = load3(X, b + 1 + nnxy);
+    x[6] = load3(X, b + 1 + nnx + nnxy);
+    x[7] = load3(X, b + nnx + nnxy);
     q = cqs_hex_corner<L>(x);
   } else {
     D3 x[4];
-    x[0] = d3(S.xlp[0][b], S.xlp[1][b], 0.);
-    x[1] = d3(S.xlp[0][b + 1], S.xlp[1][b + 1], 0.);
-    x[2] = d3(S.xlp[0][b + 1 + nnx], S.xlp[1][b + 1 + nnx], 0.);
-    x[3] = d3(S.xlp[0][b + nnx], S.xlp[1][b + nnx], 0.);
+    x[0] = d3(X[0][b], X[1][b], 0.);
+    x[1] = d3(X[0][b + 1], X[1][b + 1], 0.);
+    x[2] = d3(X[0][b + 1 + nnx], X[1][b + 1 + nnx], 0.);
+    x[3] = d3(X[0][b + nnx], X[1][b + nnx], 0.);
     q = cqs_quad_corner<L & 3>(x);
   }
   fx += pr * q.x;
@@ -157,9 +160,19 @@ template <int DIM>
 __device__ __forceinline__ void node_finish(const MhDev& S, long n, int i, int j, int k, double dt,
                                             double dth, double mass, double fx, double fy, double fz) {
   const double s = dth / mass;
-  double ux = S.u[0][n] + s * fx;
-  double uy = S.u[1][n] + s * fy;
-  double uz = DIM == 3 ? S.u[2][n] + s * fz : 0.;
+  double ux = S.u[0][n], uy = S.u[1][n], uz = DIM == 3 ? S.u[2][n] : 0.;
+  if (S.csts) {  // SaveValuesAtN of the velocity, then updateVelocityBackward (:574-627)
+    S.un[0][n] = ux;
+    S.un[1][n] = uy;
+    if (DIM == 3) S.un[2][n] = uz;
+    const double sb = (-0.5 * S.sc->dt_old) / mass;
+    ux = ux + sb * fx;
+    uy = uy + sb * fy;
+    if (DIM == 3) uz = uz + sb * fz;
+  }
+  ux = ux + s * fx;
+  uy = uy + s * fy;
+  if (DIM == 3) uz = uz + s * fz;
   for (int b = 0; b < S.nbc; ++b) {
     const int sf = S.bc_surf[b];
     bool on;
@@ -228,6 +241,35 @@ __device__ __forceinline__ void k_node_lag_body(const MhDev& S, int lb) {
   node_finish<DIM>(S, n, i, j, k, dt, dth, mass, fx, fy, fz);
 }
 
+
+// updateVelocityForward (:638-690), before a CSTS remap: u += (dt/2) / m F with the force of
+// this cycle's pressure array, pseudo-viscosity and corner vectors (no boundary condition).
+template <int DIM>
+__device__ __forceinline__ void k_node_fwd_body(const MhDev& S, int lb) {
+  const long n = lb * (long)blockDim.x + threadIdx.x;
+  if (n >= S.g.nn) return;
+  if (mh_halted(S.sc)) return;
+  const int nnx = S.g.nnx, nny = S.g.nny;
+  const int i = (int)(n % nnx);
+  const long r = n / nnx;
+  const int j = (int)(r % nny);
+  const int k = (int)(r / nny);
+  double mass = 0., fx = 0., fy = 0., fz = 0.;
+  if (DIM == 3) {
+    node_gather<3, 1, 1, 1, 1>(S, i, j, k, mass, fx, fy, fz);
+    node_gather<3, 0, 1, 1, 1>(S, i, j, k, mass, fx, fy, fz);
+    node_gather<3, 1, 0, 1, 1>(S, i, j, k, mass, fx, fy, fz);
+    node_gather<3, 0, 0, 1, 1>(S, i, j, k, mass, fx, fy, fz);
+  }
+  node_gather<DIM, 1, 1, 0, 1>(S, i, j, k, mass, fx, fy, fz);
+  node_gather<DIM, 0, 1, 0, 1>(S, i, j, k, mass, fx, fy, fz);
+  node_gather<DIM, 1, 0, 0, 1>(S, i, j, k, mass, fx, fy, fz);
+  node_gather<DIM, 0, 0, 0, 1>(S, i, j, k, mass, fx, fy, fz);
+  const double s = (0.5 * S.sc->dt) / S.mn[n];
+  S.u[0][n] = S.u[0][n] + s * fx;
+  S.u[1][n] = S.u[1][n] + s * fy;
+  if (DIM == 3) S.u[2][n] = S.u[2][n] + s * fz;
+}
 
 // Advection mode (sans-lagrange), node part: updateVelocityWithoutLagrange
 // (MahycoModule.cc:702-718), u = u0 (1 - r) + (r u0) cos(pi t parametre) with r = 1 under
@@ -524,6 +566,39 @@ __device__ __forceinline__ void fictif_eos(const MhEnvParam& ep, double t, doubl
   c = 1.;
 }
 
+// updateEnergyAndPressureforGP, CSTS branch (MahycoModule.cc:1661-1697), for a
+// single-environment cell: the work of the corner forces at n (previous cycle's corner
+// vectors, saved velocity) and n+1 (this cycle's corner vectors qv, velocity at n+1/2).
+template <int DIM>
+__device__ __forceinline__ double csts_energy(const MhDev& S, const MhEnvParam& ep, long c, long b, const D3* qv,
+                                              double rho, double q) {
+  constexpr int NN = DIM == 3 ? 8 : 4;
+  const long nnx = S.g.nnx, nnxy = (long)S.g.nnx * S.g.nny;
+  const long off[8] = {0, 1, 1 + nnx, nnx, nnxy, 1 + nnxy, 1 + nnx + nnxy, nnx + nnxy};
+  D3 xp[8], qn[8];
+  for (int l = 0; l < NN; ++l)
+    xp[l] = d3(S.xlp[0][b + off[l]], S.xlp[1][b + off[l]], DIM == 3 ? S.xlp[2][b + off[l]] : 0.);
+  if (DIM == 3) cqs_hex(xp, qn);
+  else cqs_quad(xp, qn);
+  const double dtn = S.sc->dt, dto = S.sc->dt_old;
+  double cn1 = 0., cn = 0., cdn = 0., cvon = 0.;
+  for (int l = 0; l < NN; ++l) {
+    const long n = b + off[l];
+    const D3 v = d3(S.u[0][n], S.u[1][n], DIM == 3 ? S.u[2][n] : 0.);
+    const D3 vn = d3(S.un[0][n], S.un[1][n], DIM == 3 ? S.un[2][n] : 0.);
+    const D3 dv = v - vn;
+    cn1 += (v.x * qv[l].x + v.y * qv[l].y + v.z * qv[l].z) * dtn;
+    cn += (v.x * qn[l].x + v.y * qn[l].y + v.z * qn[l].z) * dtn;
+    cdn += (dv.x * qn[l].x + dv.y * qn[l].y + dv.z * qn[l].z) * (dtn - dto);
+    cvon += (vn.x * qn[l].x + vn.y * qn[l].y + vn.z * qn[l].z) * dto;
+  }
+  const double M = S.M[c], pn = S.p[c], qnn = S.q[c], qnm1 = S.qm1[c];
+  const double denom = 1 + 0.5 * (ep.gamma - 1.0) * rho * cn1 / M;
+  const double numer = S.e[c] - (0.5 * (pn + qnn) * cn / M) - (0.5 * q * cn1 / M) + (0.25 * (pn + qnn) * cdn / M) -
+                       (0.5 * (qnn - qnm1) * cvon / M);
+  return numer / denom;
+}
+
 // A cell that no environment occupies gets no remap values from computeVariablesForRemap
 // (PrepareRemap.cc:100-166 loop over environment cells): its U is 0 and its Phi is the one
 // the last sweep of the previous remap left, U / sum V_k (RemapADIService.cc:493-538).
@@ -618,7 +693,7 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
       const MhEnvParam& ep = S.env[ke];
       double q = 0.;
       if (divu < 0.0) q = 1. / tau * (-ep.a1 * L * cs * divu + ep.a2 * L * L * divu * divu);
-      const double e = lag_energy(S, ep, S.e[c], S.p[c], rho, rho_n, q);
+      const double e = S.csts ? csts_energy<DIM>(S, ep, c, b, qv, rho, q) : lag_energy(S, ep, S.e[c], S.p[c], rho, rho_n, q);
       if (S.with_proj) {
         for (int kk = 0; kk < M; ++kk)
           if (kk == ke) {
@@ -628,6 +703,10 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
           }
         u_out[3 * M] = V * q;
         if (S.q_frozen) S.q[c] = q;  // the Lagrangian pseudo, for the remap's unswept V q slot
+        if (S.csts) {  // SaveValuesAtN of the pseudo; this cycle's for the forward velocity step
+          S.qm1[c] = S.q[c];
+          S.q[c] = q;
+        }
         if (S.pb) {  // pente-borne phi of the first sweep (PrepareRemap.cc:138-155)
           S.phi0[ke * nc + c] = S.fg[c];
           S.phi0[(M + ke) * nc + c] = rho;
@@ -783,10 +862,15 @@ __device__ __forceinline__ void face_quant(const MhDev& S, long f0, double& area
     xf = s;
     const double* ud = S.u[D];
     double us = 0.;
-    us += ud[nd[0]];
-    us += ud[nd[1]];
-    us += ud[nd[2]];
-    us += ud[nd[3]];
+    if (S.csts) {  // the mean of n and n+1 (PrepareRemap.cc:236-240)
+      const double* vd = S.un[D];
+      for (int l = 0; l < 4; ++l) us += 0.5 * (ud[nd[l]] + vd[nd[l]]);
+    } else {
+      us += ud[nd[0]];
+      us += ud[nd[1]];
+      us += ud[nd[2]];
+      us += ud[nd[3]];
+    }
     vf = 0.25 * us;
   } else {
     const long n1 = D == 0 ? f0 + nnx : f0 + 1;
@@ -800,8 +884,14 @@ __device__ __forceinline__ void face_quant(const MhDev& S, long f0, double& area
     xf = s;
     const double* ud = S.u[D];
     double us = 0.;
-    us += ud[f0];
-    us += ud[n1];
+    if (S.csts) {
+      const double* vd = S.un[D];
+      us += 0.5 * (ud[f0] + vd[f0]);
+      us += 0.5 * (ud[n1] + vd[n1]);
+    } else {
+      us += ud[f0];
+      us += ud[n1];
+    }
     vf = 0.5 * us;
   }
 }
@@ -2619,6 +2709,7 @@ MH_WRAP(k_sweep_node_row, 1, template <int DIM MH_COMMA int LIM>, <DIM MH_COMMA 
         MH_COMMA int psel MH_COMMA int first MH_COMMA int last, MH_COMMA psel MH_COMMA first MH_COMMA last)
 MH_WRAP(k_hcell, 1, template <int DIM>, <DIM>, , )
 MH_WRAP(k_node_adv, 1, template <int DIM>, <DIM>, , )
+MH_WRAP(k_node_fwd, 1, template <int DIM>, <DIM>, , )
 MH_WRAP(k_sweep_cell_gen, 1, template <int DIM MH_COMMA int D MH_COMMA int M>, <DIM MH_COMMA D MH_COMMA M>,
         MH_COMMA int usel MH_COMMA int first, MH_COMMA usel MH_COMMA first)
 MH_WRAP(k_sweep_cell3, 1, template <int DIM MH_COMMA int D MH_COMMA int M>, <DIM MH_COMMA D MH_COMMA M>,

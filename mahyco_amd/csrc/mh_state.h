@@ -102,6 +102,11 @@ struct MhDev {
   // rev_cos[iteration] = cos(pi t parametre) with the reverse option, constant step
   int sans_lagrange;
   int rider_reverse;
+  // schema-csts (single environment): the velocity saved at the start of the cycle [3][nn]
+  // and the pseudo-viscosity of the previous cycle [nc]
+  int csts;
+  double* un[3];
+  double* qm1;
   double dt_init;
   const double* u0[3];
   const double* rev_cos;

@@ -157,6 +157,7 @@ struct OracleCase {
 
   void saveValuesAtN();
   void updateVelocity();
+  void updateVelocityForward();
   void applyBoundaryCondition();
   void updatePosition();
   int computeGeometricValues();
@@ -799,10 +800,13 @@ void OracleCase::updateVelocityWithoutLagrange() {
   }
 }
 
-// MahycoModule::updateVelocity, MahycoModule.cc:438-562 (non-CSTS; elasto force of
-// NoModel adds +-0; gravity 0; anti-drift off with projection).
+// MahycoModule::updateVelocity, MahycoModule.cc:438-562 (elasto force of NoModel adds +-0;
+// gravity 0; anti-drift off with projection).  With CSTS and a remap, updateVelocityBackward
+// (:574-627) first takes the projected velocity back to n-1/2 with the same force.
 void OracleCase::updateVelocity() {
   const double dt = 0.5 * (global_old_deltat + global_deltat);
+  const bool backward = opt.schema_csts && opt.with_projection;  // :451-453
+  const double dtb = -0.5 * global_old_deltat;
 #pragma omp parallel for schedule(static)
   for (long n = 0; n < nn; ++n) {
     R3 F{0., 0., 0.};
@@ -810,6 +814,24 @@ void OracleCase::updateVelocity() {
       int c = nc_cell[a];
       double pr = pressure_n.g[c] + pseudo_n.g[c];
       F += pr * cqs_n[(long)c * nnpc + nc_local[a]];
+    }
+    if (backward) velocity[n] += (dtb / node_mass[n]) * F;
+    velocity[n] += (dt / node_mass[n]) * F;
+  }
+}
+
+// MahycoModule::updateVelocityForward, MahycoModule.cc:638-690: before a CSTS remap, the
+// velocity goes on from n+1/2 to n+1 with the force of this cycle's pressure array (not yet
+// updated by the EOS when remapping), pseudo-viscosity and corner vectors.
+void OracleCase::updateVelocityForward() {
+  const double dt = 0.5 * global_deltat;
+#pragma omp parallel for schedule(static)
+  for (long n = 0; n < nn; ++n) {
+    R3 F{0., 0., 0.};
+    for (int a = nc_off[n]; a < nc_off[n + 1]; ++a) {
+      int c = nc_cell[a];
+      double pr = pressure.g[c] + pseudo.g[c];
+      F += pr * cqs[(long)c * nnpc + nc_local[a]];
     }
     velocity[n] += (dt / node_mass[n]) * F;
   }
@@ -995,6 +1017,25 @@ void OracleCase::updateEnergyAndPressure() {
         ev(internal_energy, k, c) -= (ev(pressure_n, k, c) + q) * (1.0 / rho - 1.0 / rhon);
         continue;
       }
+      if (opt.schema_csts) {  // updateEnergyAndPressureforGP, CSTS branch (:1661-1697)
+        const double dtn = global_deltat, dto = global_old_deltat;
+        double cn1 = 0., cn = 0., cdn = 0., cvon = 0.;
+        for (int l = 0; l < nnpc; ++l) {
+          const long n = cell_nodes[c * nnpc + l];
+          const R3 C = cqs[c * nnpc + l], Cn = cqs_n[c * nnpc + l];
+          cn1 += dot(velocity[n], C) * dtn;
+          cn += dot(velocity[n], Cn) * dtn;
+          cdn += dot(velocity[n] - velocity_n[n], Cn) * (dtn - dto);
+          cvon += dot(velocity_n[n], Cn) * dto;
+        }
+        const double M = ev(cell_mass, k, c);
+        const double pn = ev(pressure_n, k, c), qn = ev(pseudo_n, k, c), qnm1 = ev(pseudo_nm1, k, c);
+        const double denom = 1 + 0.5 * (adiabatic_cst - 1.0) * rho * cn1 / M;
+        const double numer = ev(internal_energy, k, c) - (0.5 * (pn + qn) * cn / M) - (0.5 * q * cn1 / M) +
+                             (0.25 * (pn + qn) * cdn / M) - (0.5 * (qn - qnm1) * cvon / M);
+        ev(internal_energy, k, c) = numer / denom;
+        continue;
+      }
       double pseudo_v = 0.;
       if (q * (1.0 / rho - 1.0 / rhon) < 0.) pseudo_v = q;
       if (opt.with_newton) {
@@ -1071,6 +1112,7 @@ void OracleCase::computePressionMoyenne() {
 // ============================================================================ remap
 // MahycoModule::remap, PrepareRemap.cc:203-251.
 int OracleCase::remap() {
+  if (opt.schema_csts) updateVelocityForward();  // PrepareRemap.cc:210
   computeVariablesForRemap();
   computeFaceQuantitesForRemap();
   // RemapADIService::appliRemap, RemapADIService.cc:15-90
@@ -1177,7 +1219,8 @@ void OracleCase::computeFaceQuantitesForRemap() {
     R3 vm{0., 0., 0.}, fc{0., 0., 0.};
     for (int l = 0; l < face_nbnode[f]; ++l) {
       fc += one_over_nbnode * node_coord[fn[l]];
-      vm += velocity[fn[l]];
+      if (opt.schema_csts) vm += 0.5 * (velocity[fn[l]] + velocity_n[fn[l]]);  // PrepareRemap.cc:236-240
+      else vm += velocity[fn[l]];
     }
     face_coord[f] = fc;
     face_normal_velocity[f] = dot(one_over_nbnode * vm, face_normal[f]);
@@ -2012,6 +2055,9 @@ OracleCase* oracle_create(const MhMesh* mesh, const MhEnvironment* envs, int nb_
   if (opt->projection_limiteur_id < 0 || opt->projection_limiteur_id > MH_ULTRABEE_G) return nullptr;
   if (opt->projection_pente_borne && (opt->pente_borne_debar_fix < 0 || opt->pente_borne_debar_fix > 2)) return nullptr;
   if (opt->projection_pente_borne && opt->ordre_projection == 3) return nullptr;
+  // CSTS with Newton reads p before setting it (MahycoModule.cc:1595-1601); with explicit
+  // pressure the EOS runs before the forward velocity step; advection mode has no Lagrange step
+  if (opt->schema_csts && (opt->with_newton || opt->pression_explicite || opt->sans_lagrange)) return nullptr;
   OracleCase* h = new OracleCase();
   h->m = nb_env;
   for (int k = 0; k < nb_env; ++k) h->env[k] = envs[k];

@@ -72,8 +72,15 @@ def test_outputs_match_reference_files(name, tmp_path):
             np.testing.assert_allclose(va[:, 1], vb[:, 1], rtol=1e-12, atol=0, err_msg=rel)
 
 
+def _sedov3d_csts():
+    from mahyco_amd import cases
+    c = cases.config3_sedov3d(16)
+    c.schema_csts = True
+    return c
+
+
 @pytest.mark.parametrize("make", [lambda: FIXTURES["CAS_BiSodCaseX"](), lambda: FIXTURES["CAS_bitriplepoint_minmod"](),
-                                  lambda: FIXTURES["CAS_2D_Lag_SodCaseX"]()])
+                                  lambda: FIXTURES["CAS_2D_Lag_SodCaseX"](), _sedov3d_csts])
 def test_checkpoint_restart_is_bitwise(make, tmp_path):
     """Run 23 iterations, checkpoint, continue 31 in a fresh case restored from the file:
     every field, the time, the time step and the time history equal an uninterrupted run of

@@ -29,6 +29,12 @@ def _riemann3d(seed, n=14):
     return cases.riemann3d(n, rng, stiffened=True, t_end=1e30, name=f"riemann3d_{seed}")
 
 
+def _sedov2d_mono():
+    c = cases.sedov_2d()
+    c.envs = c.envs[:1]
+    return c
+
+
 def _variant(c, **kw):
     for key, v in kw.items():
         setattr(c, key, v)
@@ -94,6 +100,11 @@ CASES = {
     "advection_rotation_ordre3": (lambda: cases.advection("a_rot_o3", 21, .1, n=48, ordre=3), 100),
     "advection_ty_pb_superbeeG": (lambda: _variant(cases.advection("a_ty_pbg", 19, .2, n=40, dt=2e-3),
                                                    pente_borne=True, debar_fix=1, limiter=T.MH_SUPERBEE_G), 100),
+    # CSTS (single environment): backward / forward velocity steps, corner-work energy, mean
+    # face velocities; 2D and 3D, order 2 and 3, with and without the tiled kernels' paths
+    "sedov3d_csts_20": (lambda: _variant(cases.config3_sedov3d(20), schema_csts=True), 100),
+    "sod2d_remap_csts": (lambda: _variant(cases.config1_sod2d_lagrange(), with_projection=True, schema_csts=True), 600),
+    "sedov2d_csts_ordre3": (lambda: _variant(_sedov2d_mono(), schema_csts=True, ordre_projection=3), 200),
 }
 
 
@@ -142,3 +153,16 @@ def test_tolerance_vs_reference_pow(name):
     uo = np.stack([o.get(T.MH_N_VELOCITY, -1, c) for c in range(case.dim)])
     assert rel_err_scaled(ug, uo) <= TOL  # relative to the largest velocity component
     g.close()
+
+
+def test_csts_unsupported_combinations_refused():
+    """CSTS runs on one environment with a remap and the closed-form energy; the other
+    combinations are refused at creation."""
+    from mahyco_amd.lib import GpuCase
+    bad = [_variant(cases.config3_sedov3d(8), schema_csts=True, with_newton=True),
+           _variant(cases.config3_sedov3d(8), schema_csts=True, pression_explicite=True),
+           _variant(cases.config1_sod2d_lagrange(), schema_csts=True),
+           _variant(cases.config2_bisod3d(8, 4, 4), schema_csts=True)]
+    for c in bad:
+        with pytest.raises(Exception):
+            GpuCase(c)

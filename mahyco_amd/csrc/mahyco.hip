@@ -653,6 +653,12 @@ void launch_node_t(MhHandle* h) {
 }
 template <int DIM, int M>
 void launch_cell_t(MhHandle* h) {
+  if constexpr (M == 1) {
+    if (h->dev.csts) {
+      LAUNCH(h, KN_CELL, (k_cell_lag_csts<DIM>), h->nblk_c, MH_BLOCK, h->dev);
+      return;
+    }
+  }
   LAUNCH(h, KN_CELL, (k_cell_lag<DIM, M>), h->nblk_c, MH_BLOCK, h->dev);
 }
 #define MH_DISPATCH(fn, h)                                      \

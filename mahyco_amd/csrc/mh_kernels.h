@@ -619,7 +619,7 @@ __device__ __forceinline__ void empty_cell_phi(const MhDev& S, long c, double* u
 // Without: applies the EOS and the mean pressure and writes the CFL candidate.
 // The cell's work once its node coordinates x (at this cycle's Lagrangian positions) are
 // in registers; returns the CFL candidate of the Lagrange-only loop.
-template <int DIM, int M>
+template <int DIM, int M, int CSTS = 0>
 __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, const D3* x) {
   double* __restrict__ U = S.Ua;
   constexpr int NV = 3 * M + 1;
@@ -693,7 +693,9 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
       const MhEnvParam& ep = S.env[ke];
       double q = 0.;
       if (divu < 0.0) q = 1. / tau * (-ep.a1 * L * cs * divu + ep.a2 * L * L * divu * divu);
-      const double e = S.csts ? csts_energy<DIM>(S, ep, c, b, qv, rho, q) : lag_energy(S, ep, S.e[c], S.p[c], rho, rho_n, q);
+      double e;
+      if constexpr (CSTS) e = csts_energy<DIM>(S, ep, c, b, qv, rho, q);
+      else e = lag_energy(S, ep, S.e[c], S.p[c], rho, rho_n, q);
       if (S.with_proj) {
         for (int kk = 0; kk < M; ++kk)
           if (kk == ke) {
@@ -703,7 +705,7 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
           }
         u_out[3 * M] = V * q;
         if (S.q_frozen) S.q[c] = q;  // the Lagrangian pseudo, for the remap's unswept V q slot
-        if (S.csts) {  // SaveValuesAtN of the pseudo; this cycle's for the forward velocity step
+        if (CSTS) {  // SaveValuesAtN of the pseudo; this cycle's for the forward velocity step
           S.qm1[c] = S.q[c];
           S.q[c] = q;
         }
@@ -793,7 +795,7 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
   return dxs;
 }
 
-template <int DIM, int M>
+template <int DIM, int M, int CSTS = 0>
 __device__ __forceinline__ void k_cell_lag_body(const MhDev& S, int lb) {
   const long c = lb * (long)blockDim.x + threadIdx.x;
   const bool active = c < S.g.nc && !mh_halted(S.sc);
@@ -807,9 +809,15 @@ __device__ __forceinline__ void k_cell_lag_body(const MhDev& S, int lb) {
     const long b = i + (long)S.g.nnx * (j + (long)S.g.nny * k);
     D3 x[8];
     cell_nodes<DIM>(S, b, S.xlc, x);
-    dxs = cell_lag_core<DIM, M>(S, c, b, x);
+    dxs = cell_lag_core<DIM, M, CSTS>(S, c, b, x);
   }
   if (!S.with_proj) block_min_store(dxs, active, S.partial, lb);
+}
+// CSTS (one environment, with remap): its own instance, so that the other cell kernels do
+// not carry the corner-work registers
+template <int DIM>
+__device__ __forceinline__ void k_cell_lag_csts_body(const MhDev& S, int lb) {
+  k_cell_lag_body<DIM, 1, 1>(S, lb);
 }
 
 // ----------------------------------------------------------------------------- remap sweeps
@@ -1943,6 +1951,7 @@ __device__ __forceinline__ void k_sweep_cell_tile_body(const MhDev& S, int lb_, 
   // (between rows r-1 and r) and the upwind data of that face
   bool val[MH_TILE_Q], inner[MH_TILE_Q], up[MH_TILE_Q];
   double den[MH_TILE_Q], dx[MH_TILE_Q], w[MH_TILE_Q], hh[MH_TILE_Q], xcq[MH_TILE_Q];
+  unsigned ownm = 0;  // bit q: the cell's own U enters its update
 #pragma unroll
   for (int q = 0; q < MH_TILE_Q; ++q) {
     const int a = abase + ly + 4 * q;
@@ -1951,8 +1960,12 @@ __device__ __forceinline__ void k_sweep_cell_tile_body(const MhDev& S, int lb_, 
     xcq[q] = 0.;
     if (val[q]) {
       const long cc = c_base + a * sD;
-      if (first) den[q] = S.mask[cc] ? S.Vl[cc] : 1.;  // an empty cell holds Phi itself
-      else {
+      if (first) {
+        const bool empty = !S.mask[cc];  // an empty cell holds Phi itself and its U starts from 0
+        den[q] = empty ? 1. : S.Vl[cc];
+        if (!empty) ownm |= 1u << q;
+      } else {
+        ownm |= 1u << q;
         double sv = 0.;
         for (int k = 0; k < M; ++k) sv += Uin[k * nc + cc];
         den[q] = sv;
@@ -2035,7 +2048,7 @@ __device__ __forceinline__ void k_sweep_cell_tile_body(const MhDev& S, int lb_, 
       if (inb) ff += fb;
       if (inf) ff += Ff;
       const long c = c_base + a * sD;
-      double un = (first && !S.mask[c] ? 0. : uq[q]) - ff;
+      double un = (((ownm >> q) & 1u) ? uq[q] : 0.) - ff;
       if (v >= M && v < 2 * M) {
         double dd = 0.;
         if (inb) dd += 0.5 * fb * -1.;
@@ -2694,6 +2707,7 @@ __global__ __launch_bounds__(MH_BLOCK) void k_hist_final(MhDev S) { k_hist_final
 #define MH_COMMA ,
 MH_WRAP(k_node_lag, 1, template <int DIM>, <DIM>, , )
 MH_WRAP(k_cell_lag, MH_CELL_MIN_WAVES, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, , )
+MH_WRAP(k_cell_lag_csts, MH_CELL_MIN_WAVES, template <int DIM>, <DIM>, , )
 MH_WRAP(k_sweep_cell, 1, template <int DIM MH_COMMA int D MH_COMMA int M MH_COMMA int LIM>, <DIM MH_COMMA D MH_COMMA M MH_COMMA LIM>,
         MH_COMMA int usel MH_COMMA int first, MH_COMMA usel MH_COMMA first)
 MH_WRAP(k_sweep_node, 1, template <int DIM MH_COMMA int D MH_COMMA int LIM>, <DIM MH_COMMA D MH_COMMA LIM>,

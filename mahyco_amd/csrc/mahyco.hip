@@ -653,13 +653,8 @@ void launch_node_t(MhHandle* h) {
 }
 template <int DIM, int M>
 void launch_cell_t(MhHandle* h) {
-  if constexpr (M == 1) {
-    if (h->dev.csts) {
-      LAUNCH(h, KN_CELL, (k_cell_lag_csts<DIM>), h->nblk_c, MH_BLOCK, h->dev);
-      return;
-    }
-  }
-  LAUNCH(h, KN_CELL, (k_cell_lag<DIM, M>), h->nblk_c, MH_BLOCK, h->dev);
+  if (h->dev.csts) LAUNCH(h, KN_CELL, (k_cell_lag_csts<DIM, M>), h->nblk_c, MH_BLOCK, h->dev);
+  else LAUNCH(h, KN_CELL, (k_cell_lag<DIM, M>), h->nblk_c, MH_BLOCK, h->dev);
 }
 #define MH_DISPATCH(fn, h)                                      \
   do {                                                          \
@@ -735,11 +730,10 @@ int validate(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const Mh
   if (o->sans_lagrange && (!o->with_projection || !(o->deltat_init > 0.) ||
                            !(o->final_time / o->deltat_init <= 1e7)))
     return MH_ERR_UNSUPPORTED;  // the reverse factors are tabulated per iteration up to the final time
-  // CSTS: single-environment remap runs with the closed-form energy (with Newton the reference
-  // reads p before setting it, MahycoModule.cc:1595-1601; explicit pressure applies the EOS
-  // before the forward step)
-  if (o->schema_csts && (nb_env != 1 || !o->with_projection || o->with_newton || o->pression_explicite ||
-                         o->sans_lagrange))
+  // CSTS: remap runs with the closed-form energy (with Newton the reference reads p before
+  // setting it, MahycoModule.cc:1595-1601; explicit pressure applies the EOS before the
+  // forward step)
+  if (o->schema_csts && (!o->with_projection || o->with_newton || o->pression_explicite || o->sans_lagrange))
     return MH_ERR_UNSUPPORTED;
   if (o->cas_model == MH_CASE_RIEMANN && nb_env < 2) return MH_ERR_INVALID;
   if (o->cas_model == MH_CASE_POINTTRIPLE && nb_env < 3) return MH_ERR_INVALID;
@@ -1174,9 +1168,12 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
       for (int d = 0; d < 3; ++d) ok |= dalloc(h, &h->u0[d], nn);
       ok |= dalloc(h, &h->rev_cos, I.rev_cos.size());
     }
-    if (options->schema_csts) {  // saved velocity, pseudo-viscosity of the previous cycle
+    if (options->schema_csts) {  // saved velocity, pseudo-viscosity history, mask before the last remap
       for (int d = 0; d < 3; ++d) ok |= dalloc(h, &S.un[d], nn);
       ok |= dalloc(h, &S.qm1, nc);
+      ok |= dalloc(h, &S.qe, nc);
+      ok |= dalloc(h, &S.qem1, nc);
+      ok |= dalloc(h, &S.mkm1, nc);
     }
   } else {
     ok |= dalloc(h, &h->Ua, 1);
@@ -1225,9 +1222,14 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
     ok |= upload(h, h->xl[0][d], xs[d]);
     ok |= upload(h, h->xl[1][d], xs[d]);
   }
-  if (S.csts) {  // m_pseudo_viscosity_nmoins1 starts at 0; the saved velocity is written before use
-    const std::vector<double> zc(nc, 0.), zn(nn, 0.);
+  std::vector<double> zc, zn;  // kept until the stream is synchronised below (asynchronous uploads)
+  if (S.csts) {  // the pseudo-viscosities at n and n-1 start at 0; the saved velocity is written before use
+    zc.assign(nc, 0.);
+    zn.assign(nn, 0.);
     ok |= upload(h, S.qm1, zc);
+    ok |= upload(h, S.qe, zc);
+    ok |= upload(h, S.qem1, zc);
+    ok |= upload(h, S.mkm1, I.mask);
     for (int d = 0; d < 3; ++d) ok |= upload(h, S.un[d], zn);
   }
   ok |= upload(h, S.rho, I.rho);

@@ -566,12 +566,12 @@ __device__ __forceinline__ void fictif_eos(const MhEnvParam& ep, double t, doubl
   c = 1.;
 }
 
-// updateEnergyAndPressureforGP, CSTS branch (MahycoModule.cc:1661-1697), for a
-// single-environment cell: the work of the corner forces at n (previous cycle's corner
-// vectors, saved velocity) and n+1 (this cycle's corner vectors qv, velocity at n+1/2).
+// updateEnergyAndPressureforGP, CSTS branch (MahycoModule.cc:1661-1697): the work of the
+// corner forces at n (previous cycle's corner vectors, saved velocity) and n+1 (this cycle's
+// corner vectors qv, velocity at n+1/2), one set per cell ...
 template <int DIM>
-__device__ __forceinline__ double csts_energy(const MhDev& S, const MhEnvParam& ep, long c, long b, const D3* qv,
-                                              double rho, double q) {
+__device__ __forceinline__ void csts_work(const MhDev& S, long b, const D3* qv, double& cn1, double& cn, double& cdn,
+                                          double& cvon) {
   constexpr int NN = DIM == 3 ? 8 : 4;
   const long nnx = S.g.nnx, nnxy = (long)S.g.nnx * S.g.nny;
   const long off[8] = {0, 1, 1 + nnx, nnx, nnxy, 1 + nnxy, 1 + nnx + nnxy, nnx + nnxy};
@@ -581,7 +581,10 @@ __device__ __forceinline__ double csts_energy(const MhDev& S, const MhEnvParam& 
   if (DIM == 3) cqs_hex(xp, qn);
   else cqs_quad(xp, qn);
   const double dtn = S.sc->dt, dto = S.sc->dt_old;
-  double cn1 = 0., cn = 0., cdn = 0., cvon = 0.;
+  cn1 = 0.;
+  cn = 0.;
+  cdn = 0.;
+  cvon = 0.;
   for (int l = 0; l < NN; ++l) {
     const long n = b + off[l];
     const D3 v = d3(S.u[0][n], S.u[1][n], DIM == 3 ? S.u[2][n] : 0.);
@@ -592,11 +595,26 @@ __device__ __forceinline__ double csts_energy(const MhDev& S, const MhEnvParam& 
     cdn += (dv.x * qn[l].x + dv.y * qn[l].y + dv.z * qn[l].z) * (dtn - dto);
     cvon += (vn.x * qn[l].x + vn.y * qn[l].y + vn.z * qn[l].z) * dto;
   }
-  const double M = S.M[c], pn = S.p[c], qnn = S.q[c], qnm1 = S.qm1[c];
+}
+// ... priced per environment with its own mass, density, pressure and pseudo-viscosities
+__device__ __forceinline__ double csts_e(const MhEnvParam& ep, double e, double M, double rho, double pn, double qn,
+                                         double qnm1, double q, double cn1, double cn, double cdn, double cvon) {
   const double denom = 1 + 0.5 * (ep.gamma - 1.0) * rho * cn1 / M;
-  const double numer = S.e[c] - (0.5 * (pn + qnn) * cn / M) - (0.5 * q * cn1 / M) + (0.25 * (pn + qnn) * cdn / M) -
-                       (0.5 * (qnn - qnm1) * cvon / M);
+  const double numer = e - (0.5 * (pn + qn) * cn / M) - (0.5 * q * cn1 / M) + (0.25 * (pn + qn) * cdn / M) -
+                       (0.5 * (qn - qnm1) * cvon / M);
   return numer / denom;
+}
+// q^(n-1) of environment k in cell c (m_pseudo_viscosity_nmoins1, MahycoModule.cc:347, 360),
+// through Arcane's membership changes at the last remap (mask before it: mkm1): an
+// environment that entered has a new partial value (0); one that was alone in the cell takes
+// the cell value (qm1); one of a mixed cell keeps its partial value, the remap-before-last's
+// U_q / V (qem1).  A cell alone in one environment now reads the cell value.
+__device__ __forceinline__ double csts_qnm1(const MhDev& S, long c, int k, bool mixed_now) {
+  const uint8_t pm = S.mkm1[c];
+  const bool was = (pm >> k) & 1;
+  if (__popc(pm) >= 2) return was ? S.qem1[c] : 0.;
+  if (mixed_now) return was ? S.qm1[c] : 0.;
+  return S.qm1[c];
 }
 
 // A cell that no environment occupies gets no remap values from computeVariablesForRemap
@@ -694,8 +712,13 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
       double q = 0.;
       if (divu < 0.0) q = 1. / tau * (-ep.a1 * L * cs * divu + ep.a2 * L * L * divu * divu);
       double e;
-      if constexpr (CSTS) e = csts_energy<DIM>(S, ep, c, b, qv, rho, q);
-      else e = lag_energy(S, ep, S.e[c], S.p[c], rho, rho_n, q);
+      if constexpr (CSTS) {
+        double cn1, cn, cdn, cvon;
+        csts_work<DIM>(S, b, qv, cn1, cn, cdn, cvon);
+        e = csts_e(ep, S.e[c], S.M[c], rho, S.p[c], S.q[c], csts_qnm1(S, c, ke, false), q, cn1, cn, cdn, cvon);
+      } else {
+        e = lag_energy(S, ep, S.e[c], S.p[c], rho, rho_n, q);
+      }
       if (S.with_proj) {
         for (int kk = 0; kk < M; ++kk)
           if (kk == ke) {
@@ -728,6 +751,8 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
     } else {
       double fk[M], rk[M], qk[M], ek[M], Vk[M];
       double qsum = 0., esum = 0.;
+      double cn1 = 0., cn = 0., cdn = 0., cvon = 0.;
+      if constexpr (CSTS) csts_work<DIM>(S, b, qv, cn1, cn, cdn, cvon);
       for (int kk = 0; kk < M; ++kk) {
         if (!((mk >> kk) & 1)) continue;
         const MhEnvParam& ep = S.env[kk];
@@ -739,7 +764,11 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
         const double tk = 0.5 * (1.0 / rkn + 1.0 / rk[kk]);
         qk[kk] = 0.;
         if (divu < 0.0) qk[kk] = 1. / tk * (-ep.a1 * L * cs * divu + ep.a2 * L * L * divu * divu);
-        ek[kk] = lag_energy(S, ep, S.ek[o], S.pk[o], rk[kk], rkn, qk[kk]);
+        if constexpr (CSTS)
+          ek[kk] = csts_e(ep, S.ek[o], S.Mk[o], rk[kk], S.pk[o], S.qe[c], csts_qnm1(S, c, kk, true), qk[kk], cn1, cn,
+                          cdn, cvon);
+        else
+          ek[kk] = lag_energy(S, ep, S.ek[o], S.pk[o], rk[kk], rkn, qk[kk]);
       }
       for (int kk = 0; kk < M; ++kk)
         if ((mk >> kk) & 1) qsum += qk[kk] * fk[kk];
@@ -754,6 +783,10 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
         }
         u_out[3 * M] = V * qsum;
         if (S.q_frozen) S.q[c] = qsum;
+        if (CSTS) {  // as for a single-environment cell
+          S.qm1[c] = S.q[c];
+          S.q[c] = qsum;
+        }
         if (S.pb) {
           for (int kk = 0; kk < M; ++kk) {
             if (!((mk >> kk) & 1)) continue;
@@ -813,11 +846,11 @@ __device__ __forceinline__ void k_cell_lag_body(const MhDev& S, int lb) {
   }
   if (!S.with_proj) block_min_store(dxs, active, S.partial, lb);
 }
-// CSTS (one environment, with remap): its own instance, so that the other cell kernels do
-// not carry the corner-work registers
-template <int DIM>
+// CSTS (with remap): its own instance, so that the other cell kernels do not carry the
+// corner-work registers
+template <int DIM, int M>
 __device__ __forceinline__ void k_cell_lag_csts_body(const MhDev& S, int lb) {
-  k_cell_lag_body<DIM, 1, 1>(S, lb);
+  k_cell_lag_body<DIM, M, 1>(S, lb);
 }
 
 // ----------------------------------------------------------------------------- remap sweeps
@@ -2441,6 +2474,11 @@ __device__ __forceinline__ void k_finalize_body(const MhDev& S, int lb, int usel
       }
     }
     S.mask[c] = nm;
+    if (S.csts) {  // CSTS history: mixed cells' partial pseudo-viscosity, the mask before this remap
+      S.qem1[c] = S.qe[c];
+      S.qe[c] = (S.q_frozen ? S.Vl[c] * S.q[c] : U[3 * M]) * unsurvol;
+      S.mkm1[c] = om;
+    }
     S.rho[c] = rho_g;
     S.e[c] = e_g;
     S.p[c] = p_g;
@@ -2707,7 +2745,7 @@ __global__ __launch_bounds__(MH_BLOCK) void k_hist_final(MhDev S) { k_hist_final
 #define MH_COMMA ,
 MH_WRAP(k_node_lag, 1, template <int DIM>, <DIM>, , )
 MH_WRAP(k_cell_lag, MH_CELL_MIN_WAVES, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, , )
-MH_WRAP(k_cell_lag_csts, MH_CELL_MIN_WAVES, template <int DIM>, <DIM>, , )
+MH_WRAP(k_cell_lag_csts, MH_CELL_MIN_WAVES, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, , )
 MH_WRAP(k_sweep_cell, 1, template <int DIM MH_COMMA int D MH_COMMA int M MH_COMMA int LIM>, <DIM MH_COMMA D MH_COMMA M MH_COMMA LIM>,
         MH_COMMA int usel MH_COMMA int first, MH_COMMA usel MH_COMMA first)
 MH_WRAP(k_sweep_node, 1, template <int DIM MH_COMMA int D MH_COMMA int LIM>, <DIM MH_COMMA D MH_COMMA LIM>,

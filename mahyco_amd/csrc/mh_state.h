@@ -102,11 +102,14 @@ struct MhDev {
   // rev_cos[iteration] = cos(pi t parametre) with the reverse option, constant step
   int sans_lagrange;
   int rider_reverse;
-  // schema-csts (single environment): the velocity saved at the start of the cycle [3][nn]
-  // and the pseudo-viscosity of the previous cycle [nc]
+  // schema-csts: the velocity saved at the start of the cycle [3][nn]; per cell, the
+  // pseudo-viscosity of the previous cycle (qm1), the remap's U_q / V of the last and the
+  // previous remap (qe, qem1: the partial pseudo-viscosity of every environment of a mixed
+  // cell) and the environment mask before the last remap (mkm1)
   int csts;
   double* un[3];
-  double* qm1;
+  double *qm1, *qe, *qem1;
+  uint8_t* mkm1;
   double dt_init;
   const double* u0[3];
   const double* rev_cos;

@@ -730,11 +730,9 @@ int validate(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const Mh
   if (o->sans_lagrange && (!o->with_projection || !(o->deltat_init > 0.) ||
                            !(o->final_time / o->deltat_init <= 1e7)))
     return MH_ERR_UNSUPPORTED;  // the reverse factors are tabulated per iteration up to the final time
-  // CSTS: remap runs with the closed-form energy (with Newton the reference reads p before
-  // setting it, MahycoModule.cc:1595-1601; explicit pressure applies the EOS before the
-  // forward step)
-  if (o->schema_csts && (!o->with_projection || o->with_newton || o->pression_explicite || o->sans_lagrange))
-    return MH_ERR_UNSUPPORTED;
+  // CSTS: the closed-form energy (with Newton the reference reads p before setting it,
+  // MahycoModule.cc:1595-1601; explicit pressure applies the EOS before the forward step)
+  if (o->schema_csts && (o->with_newton || o->pression_explicite || o->sans_lagrange)) return MH_ERR_UNSUPPORTED;
   if (o->cas_model == MH_CASE_RIEMANN && nb_env < 2) return MH_ERR_INVALID;
   if (o->cas_model == MH_CASE_POINTTRIPLE && nb_env < 3) return MH_ERR_INVALID;
   for (int k = 0; k < nb_env; ++k)
@@ -1168,15 +1166,19 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
       for (int d = 0; d < 3; ++d) ok |= dalloc(h, &h->u0[d], nn);
       ok |= dalloc(h, &h->rev_cos, I.rev_cos.size());
     }
-    if (options->schema_csts) {  // saved velocity, pseudo-viscosity history, mask before the last remap
-      for (int d = 0; d < 3; ++d) ok |= dalloc(h, &S.un[d], nn);
-      ok |= dalloc(h, &S.qm1, nc);
-      ok |= dalloc(h, &S.qe, nc);
-      ok |= dalloc(h, &S.qem1, nc);
-      ok |= dalloc(h, &S.mkm1, nc);
-    }
   } else {
     ok |= dalloc(h, &h->Ua, 1);
+  }
+  if (options->schema_csts) {  // saved velocity; pseudo-viscosity history (per cell with remap, per environment without)
+    for (int d = 0; d < 3; ++d) ok |= dalloc(h, &S.un[d], nn);
+    ok |= dalloc(h, &S.qm1, nc);
+    ok |= dalloc(h, &S.qe, nc);
+    ok |= dalloc(h, &S.qem1, nc);
+    ok |= dalloc(h, &S.mkm1, nc);
+    if (!options->with_projection) {
+      ok |= dalloc(h, &S.qkn, ne);
+      ok |= dalloc(h, &S.qknm1, ne);
+    }
   }
   if (ok != MH_OK) {
     mahyco_destroy(h);
@@ -1222,7 +1224,7 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
     ok |= upload(h, h->xl[0][d], xs[d]);
     ok |= upload(h, h->xl[1][d], xs[d]);
   }
-  std::vector<double> zc, zn;  // kept until the stream is synchronised below (asynchronous uploads)
+  std::vector<double> zc, zn, ze;  // kept until the stream is synchronised below (asynchronous uploads)
   if (S.csts) {  // the pseudo-viscosities at n and n-1 start at 0; the saved velocity is written before use
     zc.assign(nc, 0.);
     zn.assign(nn, 0.);
@@ -1231,6 +1233,11 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
     ok |= upload(h, S.qem1, zc);
     ok |= upload(h, S.mkm1, I.mask);
     for (int d = 0; d < 3; ++d) ok |= upload(h, S.un[d], zn);
+    if (!options->with_projection) {
+      ze.assign(ne, 0.);
+      ok |= upload(h, S.qkn, ze);
+      ok |= upload(h, S.qknm1, ze);
+    }
   }
   ok |= upload(h, S.rho, I.rho);
   ok |= upload(h, S.p, I.p);

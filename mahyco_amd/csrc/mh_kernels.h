@@ -161,10 +161,12 @@ __device__ __forceinline__ void node_finish(const MhDev& S, long n, int i, int j
                                             double dth, double mass, double fx, double fy, double fz) {
   const double s = dth / mass;
   double ux = S.u[0][n], uy = S.u[1][n], uz = DIM == 3 ? S.u[2][n] : 0.;
-  if (S.csts) {  // SaveValuesAtN of the velocity, then updateVelocityBackward (:574-627)
+  if (S.csts) {  // SaveValuesAtN of the velocity
     S.un[0][n] = ux;
     S.un[1][n] = uy;
     if (DIM == 3) S.un[2][n] = uz;
+  }
+  if (S.csts && S.with_proj) {  // updateVelocityBackward (:451-453, 574-627)
     const double sb = (-0.5 * S.sc->dt_old) / mass;
     ux = ux + sb * fx;
     uy = uy + sb * fy;
@@ -741,6 +743,7 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
       } else {
         double p, cc;
         eos(ep, rho, e, p, cc);
+        if (CSTS) S.qm1[c] = S.q[c];
         S.rho[c] = rho;
         S.e[c] = e;
         S.p[c] = p;
@@ -764,10 +767,16 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
         const double tk = 0.5 * (1.0 / rkn + 1.0 / rk[kk]);
         qk[kk] = 0.;
         if (divu < 0.0) qk[kk] = 1. / tk * (-ep.a1 * L * cs * divu + ep.a2 * L * L * divu * divu);
-        if constexpr (CSTS)
-          ek[kk] = csts_e(ep, S.ek[o], S.Mk[o], rk[kk], S.pk[o], S.qe[c], csts_qnm1(S, c, kk, true), qk[kk], cn1, cn,
-                          cdn, cvon);
-        else
+        if constexpr (CSTS) {
+          // with remap: the history of the remap (qe, csts_qnm1); without: the environment's own
+          const double qn = S.with_proj ? S.qe[c] : S.qkn[o];
+          const double qnm1 = S.with_proj ? csts_qnm1(S, c, kk, true) : S.qknm1[o];
+          ek[kk] = csts_e(ep, S.ek[o], S.Mk[o], rk[kk], S.pk[o], qn, qnm1, qk[kk], cn1, cn, cdn, cvon);
+          if (!S.with_proj) {
+            S.qknm1[o] = qn;
+            S.qkn[o] = qk[kk];
+          }
+        } else
           ek[kk] = lag_energy(S, ep, S.ek[o], S.pk[o], rk[kk], rkn, qk[kk]);
       }
       for (int kk = 0; kk < M; ++kk)
@@ -811,6 +820,7 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
           cmix = opt * ((cc < cmix) ? cmix : cc);  // std::max(c_k, c)
           cmix += (1. - opt) * fk[kk] * cc;
         }
+        if (CSTS) S.qm1[c] = S.q[c];
         S.rho[c] = rho;
         S.e[c] = esum;
         S.p[c] = ps;

@@ -110,6 +110,7 @@ struct MhDev {
   double* un[3];
   double *qm1, *qe, *qem1;
   uint8_t* mkm1;
+  double *qkn, *qknm1;  // without remap: each environment's pseudo-viscosity at n and n-1 [m][nc]
   double dt_init;
   const double* u0[3];
   const double* rev_cos;

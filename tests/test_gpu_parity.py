@@ -109,6 +109,10 @@ CASES = {
     "bisod3d_csts_32x16x16": (lambda: _variant(cases.config2_bisod3d(32, 16, 16), schema_csts=True), 100),
     "riemann2d_stiffened_csts": (lambda: _variant(_riemann2d(True, 23), schema_csts=True), 150),
     "bitriplepoint_csts": (lambda: _variant(cases.bitriplepoint(), schema_csts=True), 300),
+    # CSTS without remap: no backward step, each environment's own pseudo-viscosity history
+    "sod2d_lagrange_csts": (lambda: _variant(cases.config1_sod2d_lagrange(), schema_csts=True), 800),
+    "riemann2d_lagrange_csts": (lambda: _variant(_riemann2d(True, 24), with_projection=False, schema_csts=True), 150),
+    "riemann3d_lagrange_csts": (lambda: _variant(_riemann3d(25), with_projection=False, schema_csts=True), 60),
 }
 
 
@@ -160,12 +164,11 @@ def test_tolerance_vs_reference_pow(name):
 
 
 def test_csts_unsupported_combinations_refused():
-    """CSTS runs with a remap and the closed-form energy; the other combinations are refused
-    at creation."""
+    """CSTS runs with the closed-form energy; Newton, explicit pressure and advection mode are
+    refused at creation."""
     from mahyco_amd.lib import GpuCase
     bad = [_variant(cases.config3_sedov3d(8), schema_csts=True, with_newton=True),
            _variant(cases.config3_sedov3d(8), schema_csts=True, pression_explicite=True),
-           _variant(cases.config1_sod2d_lagrange(), schema_csts=True),
            _variant(cases.advection("a", 7, .1, n=10), schema_csts=True)]
     for c in bad:
         with pytest.raises(Exception):

@@ -2180,6 +2180,14 @@ int oracle_history(const OracleCase* h, double* out, int max_rows) {
 void oracle_set_pow_mode(int mode) { g_pow_mode = mode; }
 
 double oracle_flux_limiter(int id, double r) { return fluxLimiter(id, r); }
+double oracle_flux_limiter_g(int id, double gradplus, double gradmoins, double y0, double yplus, double ymoins,
+                             double h0, double hplus, double hmoins, int simple_pente) {
+  return fluxLimiterG(id, gradplus, gradmoins, y0, yplus, ymoins, h0, hplus, hmoins, simple_pente != 0);
+}
+double oracle_compute_y0(int id, double y0, double yplus, double ymoins, double h0, double hplus, double hmoins,
+                         int type, int simple_pente) {
+  return computeY0(id, y0, yplus, ymoins, h0, hplus, hmoins, type, simple_pente != 0);
+}
 
 void oracle_set_threads(int n) {
 #ifdef _OPENMP

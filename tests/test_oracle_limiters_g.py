@@ -1,0 +1,131 @@
+"""The generalised limiters of the oracle (fluxLimiterG and computeY0, UtilesRemap.cc:130-258)
+against what the mathematics fixes.  Only superbeeG appears in a reference output; the other
+four ids are pinned here:
+* opposite one-sided slopes give no slope (formula 16);
+* on equal widths the slopes reduce to textbook ones: minmod and maximum modulus of the two
+  slopes, the harmonic mean 2 g+ g- / (g+ + g-) (so vanLeerG = g- phi_vanLeer(g+/g-), the
+  classical limiter pinned in test_oracle_invariants), the arithmetic mean, and the full
+  difference g+ + g- for ultrabeeG;
+* the weighted means return a common slope unchanged and do not depend on the length unit;
+* mirror symmetry: reflecting the stencil (swapping the + and - sides, negating the slopes)
+  negates the slope and swaps the two thresholds, which catches a transposed width;
+* simple-pente keeps the slope under both one-sided bounds and its thresholds are the
+  neighbour values."""
+import ctypes as C
+import itertools
+
+import numpy as np
+import pytest
+
+from mahyco_amd import types as T
+
+G_IDS = (T.MH_MINMOD_G, T.MH_SUPERBEE_G, T.MH_VANLEER_G, T.MH_ARITHMETIC_G, T.MH_ULTRABEE_G)
+
+
+@pytest.fixture(scope="module")
+def olib():
+    from oracle.pyoracle import load
+    lib = load()
+    lib.oracle_flux_limiter.restype = C.c_double
+    lib.oracle_flux_limiter.argtypes = [C.c_int, C.c_double]
+    lib.oracle_flux_limiter_g.restype = C.c_double
+    lib.oracle_flux_limiter_g.argtypes = [C.c_int] + [C.c_double] * 8 + [C.c_int]
+    lib.oracle_compute_y0.restype = C.c_double
+    lib.oracle_compute_y0.argtypes = [C.c_int] + [C.c_double] * 6 + [C.c_int, C.c_int]
+    return lib
+
+
+def _stencil(rng, h=None):
+    """Three cell values ym < y0 < yp (or decreasing) and widths; the one-sided slopes are the
+    differences over the half-sums of widths, as the remap forms them."""
+    h0, hp, hm = (h, h, h) if h else rng.uniform(0.2, 3.0, 3)
+    y = np.sort(rng.uniform(-2, 2, 3))
+    if rng.random() < 0.5:
+        y = y[::-1]
+    ym, y0, yp = y
+    gp = (yp - y0) / (0.5 * (h0 + hp))
+    gm = (y0 - ym) / (0.5 * (h0 + hm))
+    return gp, gm, y0, yp, ym, h0, hp, hm
+
+
+def test_opposite_slopes_give_zero(olib):
+    for lid in G_IDS:
+        for gp, gm in ((1., -2.), (-0.5, 3.), (1e-150, -1e-150)):
+            assert olib.oracle_flux_limiter_g(lid, gp, gm, 0., 1., -1., 1., 1., 1., 0) == 0.
+
+
+def test_equal_widths_reduce_to_textbook_slopes(olib):
+    rng = np.random.default_rng(1)
+    for _ in range(300):
+        gp, gm, y0, yp, ym, h0, hp, hm = _stencil(rng, h=0.37)
+        args = (gp, gm, y0, yp, ym, h0, hp, hm, 0)
+        s = np.sign(yp - ym)
+        assert olib.oracle_flux_limiter_g(T.MH_MINMOD_G, *args) == s * min(abs(gp), abs(gm))
+        assert olib.oracle_flux_limiter_g(T.MH_SUPERBEE_G, *args) == s * max(abs(gp), abs(gm))
+        vl = olib.oracle_flux_limiter_g(T.MH_VANLEER_G, *args)
+        np.testing.assert_allclose(vl, 2 * gp * gm / (gp + gm), rtol=1e-14)
+        np.testing.assert_allclose(vl, gm * olib.oracle_flux_limiter(T.MH_VANLEER, gp / gm), rtol=1e-13)
+        np.testing.assert_allclose(olib.oracle_flux_limiter_g(T.MH_ARITHMETIC_G, *args), 0.5 * (gp + gm), rtol=1e-14)
+        np.testing.assert_allclose(olib.oracle_flux_limiter_g(T.MH_ULTRABEE_G, *args), gp + gm, rtol=1e-13)
+
+
+def test_weighted_means_keep_a_common_slope_and_ignore_the_unit(olib):
+    rng = np.random.default_rng(2)
+    for lid in (T.MH_VANLEER_G, T.MH_ARITHMETIC_G):
+        for _ in range(200):
+            h0, hp, hm = rng.uniform(0.1, 5.0, 3)
+            g = rng.uniform(-3, 3)
+            np.testing.assert_allclose(olib.oracle_flux_limiter_g(lid, g, g, 0., g, -g, h0, hp, hm, 0), g, rtol=1e-14)
+            gp, gm = rng.uniform(0.1, 2.0, 2)
+            a = olib.oracle_flux_limiter_g(lid, gp, gm, 0., 1., -1., h0, hp, hm, 0)
+            b = olib.oracle_flux_limiter_g(lid, gp, gm, 0., 1., -1., 8 * h0, 8 * hp, 8 * hm, 0)
+            assert a == b  # widths enter through ratios only (scaling by 8 is exact)
+
+
+def test_mirror_symmetry(olib):
+    rng = np.random.default_rng(3)
+    for lid, sp in itertools.product(G_IDS, (0, 1)):
+        for _ in range(200):
+            gp, gm, y0, yp, ym, h0, hp, hm = _stencil(rng)
+            a = olib.oracle_flux_limiter_g(lid, gp, gm, y0, yp, ym, h0, hp, hm, sp)
+            b = olib.oracle_flux_limiter_g(lid, -gm, -gp, y0, ym, yp, h0, hm, hp, sp)
+            np.testing.assert_allclose(b, -a, rtol=1e-13, atol=1e-300)
+            for t in (0, 1):
+                c = olib.oracle_compute_y0(lid, y0, yp, ym, h0, hp, hm, t, sp)
+                d = olib.oracle_compute_y0(lid, y0, ym, yp, h0, hm, hp, 1 - t, sp)
+                np.testing.assert_allclose(d, c, rtol=1e-13, atol=1e-300)
+
+
+def test_thresholds_closed_forms(olib):
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        y0, yp, ym = rng.uniform(-2, 2, 3)
+        h = 0.7
+        y0p = lambda lid, sp=0: olib.oracle_compute_y0(lid, y0, yp, ym, h, h, h, 0, sp)
+        y0m = lambda lid, sp=0: olib.oracle_compute_y0(lid, y0, yp, ym, h, h, h, 1, sp)
+        assert (y0p(T.MH_MINMOD_G), y0m(T.MH_MINMOD_G)) == (yp, ym)
+        assert y0p(T.MH_ULTRABEE_G) == y0m(T.MH_ULTRABEE_G) == (yp + ym) / 2.
+        np.testing.assert_allclose(y0p(T.MH_SUPERBEE_G), (2 * yp + ym) / 3, rtol=1e-13, atol=1e-15)
+        np.testing.assert_allclose(y0p(T.MH_ARITHMETIC_G), (3 * yp + ym) / 4, rtol=1e-13, atol=1e-15)
+        np.testing.assert_allclose(y0p(T.MH_VANLEER_G), yp, rtol=1e-13, atol=1e-15)
+        np.testing.assert_allclose(y0m(T.MH_VANLEER_G), ym, rtol=1e-13, atol=1e-15)
+        for lid in G_IDS:  # simple-pente: the neighbour values
+            assert (y0p(lid, 1), y0m(lid, 1)) == (yp, ym)
+        # the vanLeer thresholds stay between the neighbours on any widths
+        hh = rng.uniform(0.1, 4.0, 3)
+        for t in (0, 1):
+            v = olib.oracle_compute_y0(T.MH_VANLEER_G, y0, yp, ym, *hh, t, 0)
+            assert min(yp, ym) <= v <= max(yp, ym)
+
+
+def test_simple_pente_bounds(olib):
+    rng = np.random.default_rng(5)
+    for lid in G_IDS:
+        for _ in range(200):
+            gp, gm, y0, yp, ym, h0, hp, hm = _stencil(rng)
+            free = olib.oracle_flux_limiter_g(lid, gp, gm, y0, yp, ym, h0, hp, hm, 0)
+            lim = olib.oracle_flux_limiter_g(lid, gp, gm, y0, yp, ym, h0, hp, hm, 1)
+            bound = min(abs(gp) * (h0 + hp) / h0, abs(gm) * (h0 + hm) / h0)
+            assert abs(lim) <= bound * (1 + 1e-15) and abs(lim) <= abs(free) * (1 + 1e-15)
+            if gp * gm > 0:
+                assert np.sign(lim) == np.sign(yp - ym) or lim == 0.

@@ -550,6 +550,11 @@ int node_row_blocks(MhHandle* h) {
 }
 
 // blocks of the LDS-tiled y / z sweep (64 cells along x by MH_TILE_L along D), a multiple of 8
+// k_sweep_node_z: one block per (row segment of an xy node plane, z plane)
+int node_z_blocks(MhHandle* h) {
+  const long segs = ((long)h->g.nnx * h->g.nny + MH_BLOCK - 1) / MH_BLOCK;
+  return (int)((segs * h->g.nnz + 7) / 8 * 8);
+}
 int tile_blocks(MhHandle* h, int D) {
   const long na = D == 1 ? h->g.ny : h->g.nz;
   const long nt = D == 1 ? (h->g.dim == 3 ? h->g.nz : 1) : h->g.ny;
@@ -566,6 +571,12 @@ void launch_sweep_yz(MhHandle* h, int sel, int first, int last) {
     MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell<DIM, D, M, LIM>), h->nblk_c, MH_BLOCK, S, sel, first));
   else
     MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell_tile<DIM, D, M, LIM>), tile_blocks(h, D), MH_BLOCK, S, sel, first));
+  if constexpr (DIM == 3 && D == 2) {
+    if (h->sweep_mode != MH_SWEEP_FLAT) {  // z dual sweep: blocks walk z fastest
+      MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE + 2, (k_sweep_node_z<LIM>), node_z_blocks(h), MH_BLOCK, S, sel, first, last));
+      return;
+    }
+  }
   MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE + D, (k_sweep_node<DIM, D, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
 }
 

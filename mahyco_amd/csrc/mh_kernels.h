@@ -2111,14 +2111,12 @@ __device__ __forceinline__ void k_sweep_cell_tile_body(const MhDev& S, int lb_, 
 // Nodes with no neighbour along D keep their values.  On the last sweep the node velocity
 // u = mu / m is formed for every node (RemapADIFinal.cc:286-297).
 template <int DIM, int D, int LIM>
-__device__ __forceinline__ void k_sweep_node_body(const MhDev& S, int lb, int psel, int first, int last) {
+__device__ __forceinline__ void k_sweep_node_at(const MhDev& S, long n, int psel, int first, int last) {
   double* __restrict__ Ud = S.Ud;
   const double* __restrict__ Pin = psel ? S.Pa : S.Pb;
   double* __restrict__ Pout = psel ? S.Pb : S.Pa;
   const double* __restrict__ dflux = S.dflux;
   constexpr int NC = DIM;  // velocity components carried
-  const long n = lb * (long)blockDim.x + threadIdx.x;
-  if (n >= S.g.nn) return;
   if (mh_halted(S.sc)) return;
   const int nnx = S.g.nnx, nny = S.g.nny;
   const int i = (int)(n % nnx);
@@ -2246,6 +2244,29 @@ __device__ __forceinline__ void k_sweep_node_body(const MhDev& S, int lb, int ps
     Pout[cc * nn + n] = ph;
     if (last && U[NC] != 0.) S.u[cc][n] = ph;
   }
+}
+
+template <int DIM, int D, int LIM>
+__device__ __forceinline__ void k_sweep_node_body(const MhDev& S, int lb, int psel, int first, int last) {
+  const long n = lb * (long)blockDim.x + threadIdx.x;
+  if (n >= S.g.nn) return;
+  k_sweep_node_at<DIM, D, LIM>(S, n, psel, first, last);
+}
+
+// The z dual sweep of one mesh with blocks walking z fastest over row segments of the
+// xy-plane: the +-2 planes of a node's stencil are read by blocks in flight at the same
+// time on the same XCD (xcd_block), instead of blocks a plane (~nnx nny / 256 blocks) apart.
+// Grid: ceil(nnx nny / MH_BLOCK) * nnz blocks, rounded up to a multiple of 8.
+template <int LIM>
+__global__ __launch_bounds__(MH_BLOCK) void k_sweep_node_z(MhDev S, int psel, int first, int last) {
+  const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const long nnxy = (long)S.g.nnx * S.g.nny;
+  const int nnz = S.g.nnz;
+  const long seg = lb / nnz;
+  const int z = lb - (int)(seg * nnz);
+  const long off = seg * MH_BLOCK + threadIdx.x;
+  if (off >= nnxy) return;  // also the rounding blocks: seg beyond the plane's segments
+  k_sweep_node_at<3, 2, LIM>(S, z * nnxy + off, psel, first, last);
 }
 
 // u = mu / m for every node with a non-zero projected mass (RemapADIFinal.cc:286-297), for

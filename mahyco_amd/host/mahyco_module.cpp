@@ -41,6 +41,18 @@ MhEnvironment StiffenedGasEOS::description() const {
   e.specific_heat = o_.specific_heat;
   return e;
 }
+MhEnvironment FictifEOS::description() const {
+  MhEnvironment e;
+  std::memset(&e, 0, sizeof(e));
+  e.eos = MH_EOS_FICTIF;
+  e.adiabatic_cst = 1.;
+  e.fictif_tdebut = o_.tdebut_pression;
+  e.fictif_tfin = o_.tfin_pression;
+  e.fictif_p0 = o_.valeur_debut_pression;
+  e.fictif_dpdt = o_.valeur_dependance_temps;
+  return e;
+}
+void FictifEOS::applyEOS(int env) { module_->check(mahyco_apply_eos(module_->h_, env), "FictifEOS::applyEOS"); }
 // initEOS runs inside mahyco_create (HydroStartInit, MahycoModule.cc:143-152).
 void PerfectGasEOS::initEOS(int) {}
 void StiffenedGasEOS::initEOS(int) {}
@@ -62,10 +74,9 @@ void StiffenedGasEOS::Endommagement(int) {
 }
 
 // ---------------------------------------------------------------- RemapADI
-void RemapADI::appliRemap(int, int withDualProjection, int, int) {
-  if (!withDualProjection) throw MahycoError(MH_ERR_UNSUPPORTED, "RemapADI::appliRemap without dual projection");
-  module_->check(mahyco_remap(module_->h_), "RemapADI::appliRemap");
-}
+// withDualProjection is 0 in advection mode only (PrepareRemap.cc:211), which the device
+// sequence knows from the options it was created with.
+void RemapADI::appliRemap(int, int, int, int) { module_->check(mahyco_remap(module_->h_), "RemapADI::appliRemap"); }
 
 // ---------------------------------------------------------------- MahycoModule
 MahycoModule::MahycoModule(const MhMesh& mesh, std::vector<EnvironmentConfig> envs, MahycoOptions opt,
@@ -86,14 +97,13 @@ void MahycoModule::check(int st, const char* where) const {
 }
 
 void MahycoModule::hydroStartInit() {
-  if (opt_.schema_csts || opt_.pseudo_centree)
-    throw MahycoError(MH_ERR_UNSUPPORTED, "schema-csts / pseudo-centree");
+  if (opt_.pseudo_centree) throw MahycoError(MH_ERR_UNSUPPORTED, "pseudo-centree");
   if (opt_.longueur_caracteristique != "racine-cubique-volume")
     throw MahycoError(MH_ERR_UNSUPPORTED, "longueur-caracteristique " + opt_.longueur_caracteristique);
   RemapADI* adi = dynamic_cast<RemapADI*>(remap_.get());
   if (!adi) throw MahycoError(MH_ERR_UNSUPPORTED, "remap service other than RemapADI");
-  if (adi->isEuler() != "Full" || adi->hasProjectionPenteBorne() || adi->hasConservationEnergieTotale())
-    throw MahycoError(MH_ERR_UNSUPPORTED, "RemapADI options outside the classical Full scheme");
+  if (adi->isEuler() != "Full" || adi->hasConservationEnergieTotale())
+    throw MahycoError(MH_ERR_UNSUPPORTED, "RemapADI: is-euler-scheme other than Full, or total-energy conservation");
   std::vector<MhEnvironment> ev;
   for (auto& e : envs_) {
     MhEnvironment d = e.eos->description();
@@ -102,6 +112,7 @@ void MahycoModule::hydroStartInit() {
     for (int i = 0; i < 3; ++i) d.vitesse_initiale[i] = e.vitesse_initiale[i];
     d.linear_pseudo_coeff = e.linear_pseudo_coeff;
     d.quadratic_pseudo_coeff = e.quadratic_pseudo_coeff;
+    d.energie_initiale = e.energie_initiale;
     ev.push_back(d);
   }
   MhOptions o;
@@ -117,9 +128,23 @@ void MahycoModule::hydroStartInit() {
   if (opt_.boundary_condition.size() > MH_MAX_BC) throw MahycoError(MH_ERR_INVALID, "too many boundary conditions");
   o.nb_bc = (int)opt_.boundary_condition.size();
   for (int b = 0; b < o.nb_bc; ++b) o.bc[b] = opt_.boundary_condition[b];
+  const RemapADIOptions& ro = adi->options();
   o.ordre_projection = adi->getOrdreProjection();
-  o.projection_limiteur_id = adi->options().projection_limiteur_id;
-  o.remap_threshold = adi->options().threshold;
+  o.projection_limiteur_id = ro.projection_limiteur_id;
+  o.projection_limiteur_pure_id = ro.projection_limiteur_pure_id;
+  o.projection_pente_borne = ro.projection_pente_borne;
+  o.pente_borne_debar_fix = ro.projection_pente_borne_debar_fix;
+  o.pente_borne_mixte = ro.projection_pente_borne_mixte;
+  o.projection_simple_pente = ro.projection_simple_pente;
+  o.remap_threshold = ro.threshold;
+  o.with_newton = opt_.with_newton;
+  o.pression_explicite = opt_.pression_explicite;
+  o.schema_csts = opt_.schema_csts;
+  o.sans_lagrange = opt_.sans_lagrange;
+  o.rider_reverse = opt_.cas.reverse_option;
+  o.rider_parametre = opt_.cas.parametre;
+  o.rider_rayon_bulle = opt_.cas.rayon_bulle;
+  o.rider_densite_lineaire = opt_.cas.densite_interieure_initiale_lineaire;
   o.cas_model = opt_.cas_model;
   o.cas_test = opt_.cas_test;
   o.interface_position = opt_.interface_position;
@@ -130,6 +155,13 @@ void MahycoModule::hydroStartInit() {
   check(mahyco_create(&mesh_, ev.data(), (int)ev.size(), &o, &h_), "MahycoModule::hydroStartInit");
 }
 
+void MahycoModule::hydroContinueInit() { check(mahyco_continue_init(h_), "hydroContinueInit"); }
+void MahycoModule::writeCheckpoint(const std::string& path) {
+  check(mahyco_checkpoint_write(h_, path.c_str()), "writeCheckpoint");
+}
+void MahycoModule::readCheckpoint(const std::string& path) {
+  check(mahyco_checkpoint_read(h_, path.c_str()), "readCheckpoint");
+}
 void MahycoModule::saveValuesAtN() { check(mahyco_save_values_at_n(h_), "saveValuesAtN"); }
 void MahycoModule::updateVelocity() { check(mahyco_update_velocity(h_), "updateVelocity"); }
 void MahycoModule::applyBoundaryCondition() { check(mahyco_apply_boundary_condition(h_), "applyBoundaryCondition"); }

@@ -5,6 +5,7 @@
 //   IEquationOfState        src/eos/IEquationOfState.h:25-66
 //   PerfectGasEOSService    src/eos/perfectgas/PerfectGasEOSService.cc
 //   StiffenedGasEOSService  src/eos/stiffenedgas/StiffenedGasEOSService.cc
+//   FictifEOSService        src/eos/fictif/FictifEOSService.cc
 //   IRemap / RemapADI       src/Remap/IRemap.h:31-64, src/Remap/RemapADIService.cc
 //   MahycoModule            src/MahycoModule.h, entry points src/Mahyco.axl:244-261
 // An environment is designated by its index.  Where the reference calls exit(1) the
@@ -88,15 +89,38 @@ class StiffenedGasEOS : public IEquationOfState {
   StiffenedGasOptions o_;
 };
 
+struct FictifOptions {  // FictifEOS.axl: a pressure that depends on time only
+  double tdebut_pression = 0., tfin_pression = 0.;
+  double valeur_debut_pression = 0., valeur_dependance_temps = 0.;
+};
+
+class FictifEOS : public IEquationOfState {
+ public:
+  explicit FictifEOS(FictifOptions o = FictifOptions()) : o_(o) {}
+  void initEOS(int) override {}  // not called in advection mode (MahycoModule.cc:143)
+  void ReinitEOS(int) override {}
+  void applyEOS(int env) override;
+  void Endommagement(int) override {}
+  double getAdiabaticCst(int) override { return 1.; }
+  double getdensityDamageThresold(int) override { return 0.; }
+  MhEnvironment description() const override;
+
+ private:
+  FictifOptions o_;
+};
+
 // ---------------------------------------------------------------- IRemap
-struct RemapADIOptions {  // RemapADI.axl:165-209 (classical subset)
+struct RemapADIOptions {  // RemapADI.axl:165-209
   double threshold = 1.e-10;
   std::string is_euler_scheme = "Full";
   int ordre_projection = 2;
   bool conservation_energie_totale = false;
   bool projection_pente_borne = false;
+  int projection_pente_borne_debar_fix = 0;
+  bool projection_pente_borne_mixte = false;
   bool projection_simple_pente = false;
   int projection_limiteur_id = 0;
+  int projection_limiteur_pure_id = 0;
 };
 
 class IRemap {
@@ -143,6 +167,14 @@ struct EnvironmentConfig {  // <environment> of Mahyco.axl
   double vitesse_initiale[3] = {0., 0., 0.};
   double linear_pseudo_coeff = 0.5, quadratic_pseudo_coeff = 1.2;
   std::shared_ptr<IEquationOfState> eos;
+  double energie_initiale = 0.;
+};
+
+struct CaseOptions {  // options of the RIDER / ADVECTION case services (RIDER.axl, ADVECTION.axl)
+  bool reverse_option = false;
+  double parametre = 0.;
+  double rayon_bulle = 0.15;
+  bool densite_interieure_initiale_lineaire = false;
 };
 
 struct MahycoOptions {  // Mahyco.axl:263-557 (subset of this path)
@@ -150,12 +182,14 @@ struct MahycoOptions {  // Mahyco.axl:263-557 (subset of this path)
   double deltat_init = 1e-5, deltat_min = 1e-8, deltat_max = 0.01, final_time = 0.2;
   bool with_projection = true, calcul_vitson_max = true;
   bool schema_csts = false, pseudo_centree = false;
+  bool with_newton = false, pression_explicite = false, sans_lagrange = false;
   std::string longueur_caracteristique = "racine-cubique-volume";
   std::vector<MhBoundaryCondition> boundary_condition;
   int cas_model = MH_CASE_SOD, cas_test = 13;
   double interface_position = 0.5;
   int interface_axis = 0;
   double sedov_deposit = 0.244816;
+  CaseOptions cas;
 };
 
 class MahycoModule {
@@ -168,7 +202,9 @@ class MahycoModule {
 
   // init entry points
   void hydroStartInit();
-  void hydroContinueInit() {}  // restart is not part of this path
+  void hydroContinueInit();   // after readCheckpoint: continue as the reference does
+  void writeCheckpoint(const std::string& path);
+  void readCheckpoint(const std::string& path);
   void computeCellMass() {}    // done by hydroStartInit on the device
   void computeNodeMass() {}    // formed by the first node kernel
   // compute-loop entry points (Mahyco.config:23-36)
@@ -206,6 +242,7 @@ class MahycoModule {
   MhHandle* h_ = nullptr;
   friend class PerfectGasEOS;
   friend class StiffenedGasEOS;
+  friend class FictifEOS;
   friend class RemapADI;
 };
 

@@ -2,7 +2,8 @@
 // builds one named case through MahycoModule, runs the time loop through the
 // entry points (or fused cycles) and prints the final time and field checksums.
 //   mahyco_run <case> [--cycles N] [--fused] [--graph K]
-// cases: bisod_x  sod2d_lagrange  sedov2d  bisod3d:<nx>x<ny>x<nz>
+// cases: bisod_x  bisod_x_csts  sod2d_lagrange  sedov2d  bisod3d:<nx>x<ny>x<nz>
+//        monorider_rotation  rider_deformation_reverse (advection of the RIDER bubble)
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -29,12 +30,40 @@ static std::unique_ptr<MahycoModule> make_case(const std::string& name) {
   auto walls = [&](int dim) {
     for (int s = 0; s < 2 * dim; ++s) opt.boundary_condition.push_back({s, s / 2, 0.0});
   };
-  if (name == "bisod_x" || name.rfind("bisod3d", 0) == 0) {  // NONREGRESSION/CAS_BiSodCaseX
+  auto fictif = [] {
+    FictifOptions o;
+    o.tfin_pression = 10.;
+    return std::make_shared<FictifEOS>(o);
+  };
+  if (name == "monorider_rotation" || name == "rider_deformation_reverse") {
+    // NONREGRESSION/CAS_MonoRiderRotation, CAS_RiderDeformationTimeReverse: sans-lagrange
+    mesh.dim = 3;
+    mesh.nx = 50; mesh.ny = 50; mesh.nz = 1;
+    mesh.length[0] = 1.; mesh.length[1] = 1.; mesh.length[2] = .1;
+    envs[0] = {"Vide", 0., 0., {0, 0, 0}, 0.5, 1.2, fictif()};
+    envs[1] = {"Bulle", 1., 0., {0, 0, 0}, 0.5, 1.2, fictif()};
+    opt.cas_model = MH_CASE_RIDER;
+    opt.sans_lagrange = true;
+    opt.deltat_init = 1e-3;
+    opt.deltat_max = 1e-4;
+    if (name == "monorider_rotation") {
+      opt.cas_test = 26;
+      opt.final_time = .5;
+    } else {
+      opt.cas_test = 38;
+      opt.final_time = 1.;
+      opt.cas.reverse_option = true;
+      opt.cas.parametre = 1.;
+    }
+    return std::unique_ptr<MahycoModule>(new MahycoModule(mesh, envs, opt, std::make_shared<RemapADI>(ro)));
+  }
+  if (name == "bisod_x" || name == "bisod_x_csts" || name.rfind("bisod3d", 0) == 0) {  // NONREGRESSION/CAS_BiSodCaseX
+    opt.schema_csts = name == "bisod_x_csts";
     mesh.dim = 3;
     mesh.nx = 100; mesh.ny = 5; mesh.nz = 5;
     mesh.length[0] = 1.; mesh.length[1] = .1; mesh.length[2] = .1;
     double g1 = 1.4;
-    if (name != "bisod_x") {  // 3D bi-material tube on [0,1]^3, superbee
+    if (name.rfind("bisod3d", 0) == 0) {  // 3D bi-material tube on [0,1]^3, superbee
       if (sscanf(name.c_str(), "bisod3d:%dx%dx%d", &mesh.nx, &mesh.ny, &mesh.nz) != 3) return nullptr;
       mesh.length[1] = mesh.length[2] = 1.;
       g1 = 1.67;

@@ -101,6 +101,40 @@ def test_cpp_host_interface():
     assert bad.returncode == 2
 
 
+@pytest.mark.parametrize("name,make", [
+    ("monorider_rotation", lambda: cases.rider("mr_rot", 26, .5)),
+    ("rider_deformation_reverse", lambda: cases.rider("r_def_rev", 38, 1., 1.)),
+    ("bisod_x_csts", lambda: _csts(cases.bisod_x())),
+])
+def test_cpp_host_interface_variants(name, make):
+    """The C++ MahycoModule with the Fictif EOS and the RIDER case service in advection mode,
+    and with CSTS: entry points and fused cycles agree with each other and with the C ABI
+    driven from Python (same iteration, time and field sums to the last bit)."""
+    from mahyco_amd.lib import GpuCase
+    exe = os.path.join(ROOT, "mahyco_amd", "_build", "mahyco_run")
+    print(f"[cpp host] {name}", flush=True)
+    out1 = subprocess.run([exe, name], check=True, capture_output=True, text=True, timeout=120).stdout
+    out2 = subprocess.run([exe, name, "--fused", "--graph", "6"], check=True, capture_output=True,
+                          text=True, timeout=120).stdout
+    assert out1 == out2, (out1, out2)
+    kv = dict(x.split("=") for x in out1.split())
+    g = GpuCase(make())
+    st = T.MH_OK
+    while st == T.MH_OK:
+        st = g.run(500, graph=6)
+    assert st == T.MH_STOPPED
+    assert int(kv["iteration"]) == g.iteration
+    assert float(kv["time"]) == g.time
+    assert float(kv["sum_rho"]) == float(np.cumsum(g.get(T.MH_F_DENSITY))[-1])
+    assert float(kv["sum_e"]) == float(np.cumsum(g.get(T.MH_F_INTERNAL_ENERGY))[-1])
+    g.close()
+
+
+def _csts(c):
+    c.schema_csts = True
+    return c
+
+
 def test_full_size_bisod_properties():
     """256x128x128 bi-material tube, 100 iterations: the solution stays planar (every y-z
     slice equal to the first to round-off) and positive."""

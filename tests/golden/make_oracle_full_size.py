@@ -9,7 +9,7 @@ float64 array (global cell / node order), its sum, and the x-profile at j = k = 
 time, time step and iteration as exact hex floats.  A GPU run that reproduces every digest
 is bitwise identical to the oracle over the whole mesh.
 
-    python tests/golden/make_oracle_full_size.py [bisod3d] [sedov3d] [served114] [served162]
+    python tests/golden/make_oracle_full_size.py [bisod3d] [sedov3d] [served114] [served162] [bisod3d_csts] [rider512]
     (minutes on 8 threads)
 """
 import json
@@ -36,7 +36,23 @@ WORKLOADS = {
     # the mass fractions then keep their values across the membership change
     "served114": (lambda: _served(114), 400),
     "served162": (lambda: _served(162), 400),
+    # the CSTS scheme on the full bi-material tube, and RIDER advection (reversed
+    # deformation, cells emptied of both environments) on a 512 x 512 mesh
+    "bisod3d_csts": (lambda: _csts(cases.config2_bisod3d(256, 128, 128)), 100),
+    "rider512": (lambda: _rider512(), 300),
 }
+
+
+def _csts(c):
+    c.schema_csts = True
+    c.name += "_csts"
+    return c
+
+
+def _rider512():
+    c = cases.rider("rider_deformation_reverse_512", 38, 1., 1.)
+    c.n = (512, 512, 1)
+    return c
 
 
 def main(names):

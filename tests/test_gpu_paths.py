@@ -162,7 +162,15 @@ FULL_SIZE = {
     "sedov3d": (lambda: cases.config3_sedov3d(192, 200), 200),
     "served114": (lambda: _served(114), 400),  # cells that empty out: kept mass fractions
     "served162": (lambda: _served(162), 400),
+    "bisod3d_csts": (lambda: _csts(cases.config2_bisod3d(256, 128, 128)), 100),  # the CSTS scheme
+    "rider512": (lambda: _rider512(), 300),  # advection, reversed deformation, emptied cells
 }
+
+
+def _rider512():
+    c = cases.rider("rider_deformation_reverse_512", 38, 1., 1.)
+    c.n = (512, 512, 1)
+    return c
 
 
 @pytest.mark.parametrize("name", sorted(FULL_SIZE))

@@ -163,7 +163,7 @@ FULL_SIZE = {
     "served114": (lambda: _served(114), 400),  # cells that empty out: kept mass fractions
     "served162": (lambda: _served(162), 400),
     "bisod3d_csts": (lambda: _csts(cases.config2_bisod3d(256, 128, 128)), 100),  # the CSTS scheme
-    "rider512": (lambda: _rider512(), 300),  # advection, reversed deformation, emptied cells
+    "rider512": (lambda: _rider512(), 300),  # advection, reversed deformation, ~2e4 mixed cells
 }
 
 

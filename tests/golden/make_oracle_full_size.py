@@ -37,7 +37,7 @@ WORKLOADS = {
     "served114": (lambda: _served(114), 400),
     "served162": (lambda: _served(162), 400),
     # the CSTS scheme on the full bi-material tube, and RIDER advection (reversed
-    # deformation, cells emptied of both environments) on a 512 x 512 mesh
+    # deformation, about 2e4 mixed cells) on a 512 x 512 mesh
     "bisod3d_csts": (lambda: _csts(cases.config2_bisod3d(256, 128, 128)), 100),
     "rider512": (lambda: _rider512(), 300),
 }

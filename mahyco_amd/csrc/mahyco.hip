@@ -772,14 +772,26 @@ static void launch_pm_t(MhHandle* h) {
 // every active case of a (dimension, environment count) group.  A case is admitted only at
 // a global iteration multiple of 6, so all active cases share the coordinate parity and the
 // sweep rotation (RemapADIService.cc:25, 59-60), hence the sweep direction of each launch.
-size_t case_footprint(const MhMesh& mesh, int m, int with_proj) {
+size_t case_footprint(const MhMesh& mesh, int m, const MhOptions& o) {
   const long nc = (long)mesh.nx * mesh.ny * (mesh.dim == 3 ? mesh.nz : 1);
   const long nn = (long)(mesh.nx + 1) * (mesh.ny + 1) * (mesh.dim == 3 ? mesh.nz + 1 : 1);
   const long nv = 3 * m + 1;
   long words = (mesh.nx + mesh.ny + mesh.nz + 3) + nc /*VE*/ + 10 * nn /*u, xl x2, mn*/ + 7 * nc +
                6L * m * nc + 3 * nc + 2 * ((nc + nn) / MH_BLOCK + 8);
-  if (with_proj) words += 2 * nv * nc + m * nc + 10 * nn;
-  return (size_t)words * 8 + (size_t)nc;
+  long bytes = nc;  // mask
+  if (o.with_projection) {
+    words += 2 * nv * nc + m * nc + 10 * nn;
+    const bool gen_lim = o.ordre_projection == 2 && o.projection_limiteur_id >= MH_MINMOD_G;
+    if (o.ordre_projection == 3 || o.projection_pente_borne || gen_lim) words += (long)mesh.dim * nc;  // hc
+    if (o.ordre_projection == 3 || o.pente_borne_mixte) bytes += nc;                                  // est
+    if (o.projection_pente_borne) words += nv * nc;                                                   // phi0
+    if (o.sans_lagrange && o.deltat_init > 0.) words += 3 * nn + (long)(o.final_time / o.deltat_init) + 8;
+  }
+  if (o.schema_csts) {  // saved velocity, pseudo-viscosity history
+    words += 3 * nn + 3 * nc + (o.with_projection ? 0 : 2L * m * nc);
+    bytes += nc;
+  }
+  return (size_t)words * 8 + (size_t)bytes;
 }
 
 // cases whose sweeps take the pente-borne / generalised-limiter kernels
@@ -792,6 +804,8 @@ struct EnsGroup {
   int dim = 0, m = 0;
   bool o3 = false;   // order-3 remap
   bool gen = false;  // pente-borne or generalised limiters
+  bool csts = false; // schema-csts
+  bool adv = false;  // advection mode (sans-lagrange): prescribed node velocity, no dual sweeps
   std::vector<int> slots;
   int2* d_cmap = nullptr;
   int2* d_nmap = nullptr;
@@ -865,7 +879,8 @@ int ens_rebuild(MhEnsembleImpl* E) {
     }
     bool found = false;
     for (auto& g : E->groups)
-      if (g.dim == h->g.dim && g.m == h->m && g.o3 == (h->dev.order2 == 2) && g.gen == ens_gen(h)) {
+      if (g.dim == h->g.dim && g.m == h->m && g.o3 == (h->dev.order2 == 2) && g.gen == ens_gen(h) &&
+          g.csts == (h->dev.csts != 0) && g.adv == (h->dev.sans_lagrange != 0)) {
         g.slots.push_back(sl);
         found = true;
       }
@@ -875,6 +890,8 @@ int ens_rebuild(MhEnsembleImpl* E) {
       g.m = h->m;
       g.o3 = h->dev.order2 == 2;
       g.gen = ens_gen(h);
+      g.csts = h->dev.csts != 0;
+      g.adv = h->dev.sans_lagrange != 0;
       g.slots.push_back(sl);
       E->groups.push_back(g);
     }
@@ -932,8 +949,11 @@ void ens_group_cycle(MhEnsembleImpl* E, const EnsGroup& g) {
   const int par = (int)(E->launched & 1);
   const MhDev* tab = E->d_tab[par];
   hipStream_t st = E->stream;
-  hipLaunchKernelGGL((k_node_lag_batch<DIM>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap);
-  hipLaunchKernelGGL((k_cell_lag_batch<DIM, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap);
+  if (g.adv) hipLaunchKernelGGL((k_node_adv_batch<DIM>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap);
+  else hipLaunchKernelGGL((k_node_lag_batch<DIM>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap);
+  if (g.csts) hipLaunchKernelGGL((k_cell_lag_csts_batch<DIM, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap);
+  else hipLaunchKernelGGL((k_cell_lag_batch<DIM, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap);
+  if (g.csts) hipLaunchKernelGGL((k_node_fwd_batch<DIM>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap);
   const int sens = (int)(E->launched % DIM);
   if (g.o3 || g.gen) hipLaunchKernelGGL((k_hcell_batch<DIM>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap);
   for (int s = 0; s < DIM; ++s) {
@@ -941,15 +961,15 @@ void ens_group_cycle(MhEnsembleImpl* E, const EnsGroup& g) {
     if (g.gen) {
       if (D == 0) {
         hipLaunchKernelGGL((k_sweep_cell_gen_batch<DIM, 0, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
-        hipLaunchKernelGGL((k_sweep_node_batch<DIM, 0, MH_LIM_ANY>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel,
+        if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, 0, MH_LIM_ANY>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel,
                            first, last);
       } else if (D == 1) {
         hipLaunchKernelGGL((k_sweep_cell_gen_batch<DIM, 1, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
-        hipLaunchKernelGGL((k_sweep_node_batch<DIM, 1, MH_LIM_ANY>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel,
+        if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, 1, MH_LIM_ANY>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel,
                            first, last);
       } else if constexpr (DIM == 3) {
         hipLaunchKernelGGL((k_sweep_cell_gen_batch<3, 2, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
-        hipLaunchKernelGGL((k_sweep_node_batch<3, 2, MH_LIM_ANY>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel,
+        if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<3, 2, MH_LIM_ANY>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel,
                            first, last);
       }
       continue;
@@ -963,30 +983,30 @@ void ens_group_cycle(MhEnsembleImpl* E, const EnsGroup& g) {
         hipLaunchKernelGGL((k_sweep_cell3_batch<3, 2, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
     }
     if (g.o3 && D == 0) {
-      hipLaunchKernelGGL((k_sweep_node_batch<DIM, 0, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
+      if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, 0, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
     } else if (g.o3 && D == 1) {
-      hipLaunchKernelGGL((k_sweep_node_batch<DIM, 1, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
+      if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, 1, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
     } else if (g.o3) {
-      hipLaunchKernelGGL((k_sweep_node_batch<DIM, (DIM == 3 ? 2 : 0), -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
+      if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, (DIM == 3 ? 2 : 0), -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
     } else if (g.rows && D == 0) {
       hipLaunchKernelGGL((k_sweep_cell_row_batch<DIM, M, -1>), dim3(g.nrcb), dim3(MH_BLOCK), 0, st, tab, g.d_rcmap, sel, first);
-      hipLaunchKernelGGL((k_sweep_node_row_batch<DIM, -1>), dim3(g.nrnb), dim3(MH_BLOCK), 0, st, tab, g.d_rnmap, sel, first, last);
+      if (!g.adv) hipLaunchKernelGGL((k_sweep_node_row_batch<DIM, -1>), dim3(g.nrnb), dim3(MH_BLOCK), 0, st, tab, g.d_rnmap, sel, first, last);
     } else if (g.rows && D == 1) {
       hipLaunchKernelGGL((k_sweep_cell_tile_batch<DIM, 1, M, -1>), dim3(g.ntb[0]), dim3(MH_BLOCK), 0, st, tab, g.d_tmap[0], sel, first);
-      hipLaunchKernelGGL((k_sweep_node_batch<DIM, 1, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
+      if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, 1, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
     } else if (g.rows) {
       if constexpr (DIM == 3)
         hipLaunchKernelGGL((k_sweep_cell_tile_batch<3, 2, M, -1>), dim3(g.ntb[1]), dim3(MH_BLOCK), 0, st, tab, g.d_tmap[1], sel, first);
-      hipLaunchKernelGGL((k_sweep_node_batch<DIM, (DIM == 3 ? 2 : 0), -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
+      if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, (DIM == 3 ? 2 : 0), -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
     } else if (D == 0) {
       hipLaunchKernelGGL((k_sweep_cell_batch<DIM, 0, M, -1>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
-      hipLaunchKernelGGL((k_sweep_node_batch<DIM, 0, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
+      if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, 0, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
     } else if (D == 1) {
       hipLaunchKernelGGL((k_sweep_cell_batch<DIM, 1, M, -1>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
-      hipLaunchKernelGGL((k_sweep_node_batch<DIM, 1, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
+      if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, 1, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
     } else {
       hipLaunchKernelGGL((k_sweep_cell_batch<DIM, (DIM == 3 ? 2 : 0), M, -1>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
-      hipLaunchKernelGGL((k_sweep_node_batch<DIM, (DIM == 3 ? 2 : 0), -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
+      if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, (DIM == 3 ? 2 : 0), -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
     }
   }
   hipLaunchKernelGGL((k_finalize_batch<DIM, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, DIM & 1);
@@ -1575,7 +1595,7 @@ int mahyco_compute_pression_moyenne(MhHandle* h) {
 // ---- ensembles ------------------------------------------------------------------------
 size_t mahyco_case_footprint(const MhMesh* mesh, int nb_env, const MhOptions* options) {
   if (!mesh || !options || nb_env < 1 || nb_env > MH_MAX_ENV) return 0;
-  return case_footprint(*mesh, nb_env, options->with_projection);
+  return case_footprint(*mesh, nb_env, *options);
 }
 
 int mahyco_ensemble_create(int max_slots, double hbm_budget_bytes, MhEnsemble** out) {
@@ -1616,9 +1636,9 @@ int mahyco_ensemble_add(MhEnsemble* e, const MhMesh* mesh, const MhEnvironment* 
   MhEnsembleImpl* E = (MhEnsembleImpl*)e;
   if (!E || !slot_out || !mesh || !options) return MH_ERR_INVALID;
   *slot_out = -1;
-  if (!options->with_projection || options->sans_lagrange || options->schema_csts) return MH_ERR_UNSUPPORTED;
+  if (!options->with_projection) return MH_ERR_UNSUPPORTED;
   if (mesh->nx < 2 || mesh->ny < 2 || (mesh->dim == 3 && mesh->nz < 2)) return MH_ERR_UNSUPPORTED;
-  const size_t need = case_footprint(*mesh, nb_env, 1);
+  const size_t need = case_footprint(*mesh, nb_env, *options);
   if ((double)(E->used + need) > E->budget) return MH_ERR_NOMEM;  // admission by HBM footprint
   int sl = -1;
   for (int i = 0; i < E->max_slots && sl < 0; ++i) {

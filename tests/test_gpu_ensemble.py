@@ -63,6 +63,53 @@ def test_alone_equals_mixed(graph, stiff, dim3, o3):
     E.close()
 
 
+def _two_d(c, n):
+    c.dim, c.n, c.length = 2, (n, n, 1), (1., 1., 0.)
+    return c
+
+
+def _variant_mix():
+    """CSTS runs (2D and 3D) and 2D advection runs (RIDER bubble, ADVECTION square, with and
+    without time reversal) beside ordinary ones: four batch groups."""
+    rng = np.random.default_rng(7)
+    out = []
+    for i in range(3):
+        c = cases.riemann2d(int(rng.integers(16, 40)), int(rng.integers(16, 40)), rng, stiffened=True, name=f"c{i}")
+        c.schema_csts = i < 2
+        out.append(c)
+    c = cases.riemann3d(10, rng, stiffened=True, t_end=1e30, name="c3d")
+    c.schema_csts = True
+    out.append(c)
+    out.append(_two_d(cases.rider("r2", 34, .2), 40))
+    out.append(_two_d(cases.rider("r2rev", 38, .2, 1.), 32))
+    out.append(_two_d(cases.advection("a2", 21, .2, n=36), 36))
+    return out
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_variant_groups_alone_equal_mixed(graph):
+    """CSTS and advection-mode cases form their own batch groups (forward velocity step,
+    CSTS cell kernel; prescribed node velocity, no dual sweeps) and give the same bits as
+    alone."""
+    from mahyco_amd.lib import Ensemble
+    print(f"[ensemble] variants graph={graph}", flush=True)
+    mix = _variant_mix()
+    ncyc = 30
+    E = Ensemble(16, 4e9)
+    slots = [E.add(c, max_iterations=ncyc) for c in mix]
+    assert all(s is not None for s in slots)
+    E.step(ncyc + 6, graph=graph)
+    n, st, tm, it = E.poll()
+    for s, c in zip(slots, mix):
+        assert st[s] == T.MH_STOPPED and it[s] == ncyc, (c.name, st[s], it[s])
+        ref = _alone(c, ncyc)
+        assert tm[s] == ref["t"], c.name
+        for f in (T.MH_F_DENSITY, T.MH_F_PRESSURE, T.MH_F_INTERNAL_ENERGY, T.MH_F_FRACVOL):
+            assert np.array_equal(E.get(s, f), ref[f]), (c.name, f)
+        assert np.array_equal(E.get(s, T.MH_N_VELOCITY, -1, 0), ref["u"]), c.name
+    E.close()
+
+
 def test_join_and_leave():
     """A case admitted while others run, after one left, still equals its solo run."""
     from mahyco_amd.lib import Ensemble

@@ -29,6 +29,11 @@ def _riemann3d(seed, n=14):
     return cases.riemann3d(n, rng, stiffened=True, t_end=1e30, name=f"riemann3d_{seed}")
 
 
+def _two_d(c, n):
+    c.dim, c.n, c.length = 2, (n, n, 1), (1., 1., 0.)
+    return c
+
+
 def _sedov2d_mono():
     c = cases.sedov_2d()
     c.envs = c.envs[:1]
@@ -100,6 +105,9 @@ CASES = {
     "advection_rotation_ordre3": (lambda: cases.advection("a_rot_o3", 21, .1, n=48, ordre=3), 100),
     "advection_ty_pb_superbeeG": (lambda: _variant(cases.advection("a_ty_pbg", 19, .2, n=40, dt=2e-3),
                                                    pente_borne=True, debar_fix=1, limiter=T.MH_SUPERBEE_G), 100),
+    # 2D advection meshes (the datasets are 3D with one cell in z)
+    "rider2d_rotation_reverse": (lambda: _two_d(cases.rider("r2rev", 34, .3, 2.), 48), 300),
+    "advection2d_t45_ordre3": (lambda: _variant(_two_d(cases.advection("a2", 20, .2, n=40), 40), ordre_projection=3), 200),
     # CSTS (single environment): backward / forward velocity steps, corner-work energy, mean
     # face velocities; 2D and 3D, order 2 and 3, with and without the tiled kernels' paths
     "sedov3d_csts_20": (lambda: _variant(cases.config3_sedov3d(20), schema_csts=True), 100),

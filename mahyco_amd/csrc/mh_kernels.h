@@ -2595,12 +2595,28 @@ __device__ __forceinline__ void k_dt_body(const MhDev& S) {
   // without a NaN candidate anywhere the fold is a plain minimum, in any order
   if (threadIdx.x == 0) any_nan = 0;
   __syncthreads();
-  double pm = __longlong_as_double(0x7ff8000000000000LL);
+  // four independent running minima keep four loads in flight per thread (the minimum of
+  // non-NaN values does not depend on the order)
+  const double qnan = __longlong_as_double(0x7ff8000000000000LL);
+  double pm = qnan, pm1 = qnan, pm2 = qnan, pm3 = qnan;
   int pn = 0;
-  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+  const int bd = (int)blockDim.x;
+  int b = threadIdx.x;
+  for (; b + 3 * bd < nblocks; b += 4 * bd) {
+    const double a0 = S.partial[2 * b], a1 = S.partial[2 * (b + bd)];
+    const double a2 = S.partial[2 * (b + 2 * bd)], a3 = S.partial[2 * (b + 3 * bd)];
+    pn |= (S.partial[2 * b + 1] != 0.) | (S.partial[2 * (b + bd) + 1] != 0.) |
+          (S.partial[2 * (b + 2 * bd) + 1] != 0.) | (S.partial[2 * (b + 3 * bd) + 1] != 0.);
+    pm = min_skip_nan(pm, a0);
+    pm1 = min_skip_nan(pm1, a1);
+    pm2 = min_skip_nan(pm2, a2);
+    pm3 = min_skip_nan(pm3, a3);
+  }
+  for (; b < nblocks; b += bd) {
     pm = min_skip_nan(pm, S.partial[2 * b]);
     pn |= S.partial[2 * b + 1] != 0.;
   }
+  pm = min_skip_nan(min_skip_nan(pm, pm1), min_skip_nan(pm2, pm3));
   for (int off = 32; off > 0; off >>= 1) pm = min_skip_nan(pm, __shfl_xor(pm, off, 64));
   if (__ballot(pn) != 0 && lane == 0) any_nan = 1;
   if (lane == 0) rm[wid] = pm;

@@ -54,3 +54,26 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(L, "_lib", None)
     with pytest.raises(RuntimeError):
         L.load()
+
+
+def test_footprint_counts_variant_arrays():
+    """Ensemble admission by HBM footprint (host-only call): every option that allocates more
+    device arrays raises the estimate, by the arrays it adds."""
+    from mahyco_amd import cases
+    from mahyco_amd import lib as L
+    if not os.path.exists(L.LIB_PATH):
+        pytest.skip("HIP library not built")
+    base = cases.config2_bisod3d(32, 16, 16)
+    nc, nn, m = 32 * 16 * 16, 33 * 17 * 17, 2
+    f0 = L.footprint(base)
+
+    def with_(**kw):
+        c = cases.config2_bisod3d(32, 16, 16)
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return L.footprint(c)
+
+    assert with_(schema_csts=True) - f0 == 8 * (3 * nn + 3 * nc) + nc
+    assert with_(ordre_projection=3) - f0 == 8 * 3 * nc + nc  # cell widths, mixed flags
+    assert with_(pente_borne=True) - f0 == 8 * (3 * nc + (3 * m + 1) * nc)  # widths, first-sweep phi
+    assert L.footprint(cases.config2_bisod3d(64, 16, 16)) > f0

@@ -737,7 +737,7 @@ int validate(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const Mh
     return MH_ERR_UNSUPPORTED;
   if (o->cas_model == MH_CASE_ADVECTION && nb_env < 2) return MH_ERR_INVALID;  // the square takes env 1's density
   if (o->cas_model == MH_CASE_RIDER && (o->cas_test < 23 || o->cas_test > 38)) return MH_ERR_UNSUPPORTED;
-  if (o->cas_model == MH_CASE_RIDER && o->cas_test >= 31 && nb_env < 2) return MH_ERR_INVALID;
+  if (o->cas_model == MH_CASE_RIDER && nb_env < 2) return MH_ERR_INVALID;  // the bubble takes env 1's density
   if (o->sans_lagrange && (!o->with_projection || !(o->deltat_init > 0.) ||
                            !(o->final_time / o->deltat_init <= 1e7)))
     return MH_ERR_UNSUPPORTED;  // the reverse factors are tabulated per iteration up to the final time

@@ -2058,6 +2058,8 @@ OracleCase* oracle_create(const MhMesh* mesh, const MhEnvironment* envs, int nb_
   // CSTS with Newton reads p before setting it (MahycoModule.cc:1595-1601); with explicit
   // pressure the EOS runs before the forward velocity step; advection mode has no Lagrange step
   if (opt->schema_csts && (opt->with_newton || opt->pression_explicite || opt->sans_lagrange)) return nullptr;
+  // RIDER / ADVECTION give the bubble environment 1's density, Mono tests included
+  if ((opt->cas_model == MH_CASE_RIDER || opt->cas_model == MH_CASE_ADVECTION) && nb_env < 2) return nullptr;
   OracleCase* h = new OracleCase();
   h->m = nb_env;
   for (int k = 0; k < nb_env; ++k) h->env[k] = envs[k];

@@ -181,3 +181,13 @@ def test_csts_unsupported_combinations_refused():
     for c in bad:
         with pytest.raises(Exception):
             GpuCase(c)
+
+
+def test_bubble_cases_need_two_environments():
+    """RIDER and ADVECTION (Mono tests included) give the bubble environment 1's density: a
+    single environment is refused at creation."""
+    from mahyco_amd.lib import GpuCase
+    for c in (cases.advection("adv", 7, .1, n=10), cases.rider("r", 23, .1)):
+        c.envs = c.envs[:1]
+        with pytest.raises(Exception):
+            GpuCase(c)

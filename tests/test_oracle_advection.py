@@ -120,3 +120,12 @@ def test_two_environments_courant_one(test, sx, sy):
     np.testing.assert_allclose(r[ahead], 0., rtol=0, atol=1e-8)
     assert np.all(mask[left] == 2)
     np.testing.assert_allclose(r[left], 1., rtol=0, atol=1e-8)
+
+
+def test_bubble_cases_need_two_environments():
+    """The RIDER and ADVECTION services give the bubble environment 1's density (also in
+    their Mono tests), so a single environment is refused."""
+    for c in (cases.advection("adv", 7, 1.), cases.rider("r", 23, 1.)):
+        c.envs = c.envs[:1]
+        with pytest.raises(Exception):
+            _oracle(c)

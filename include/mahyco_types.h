@@ -67,6 +67,15 @@ enum MhCaseModel {
   MH_CASE_ADVECTION = 5    /* casTest/ADVECTION/ADVECTIONService.cc; cas_test 7-10 (mono), 18-21 (square in 2 envs) */
 };
 
+/* characteristic length of a cell (computeGeometricValues, MahycoModule.cc:1195-1236) */
+typedef enum {
+  MH_L_RACINE_CUBIQUE_VOLUME = 0, /* V^(1/d) */
+  MH_L_FACES_OPPOSEES = 1,        /* d1 d2 d3 / (d1 d2 + d1 d3 + d2 d3), medians between opposite faces (3D) */
+  MH_L_MONODIM_X = 2,             /* |x(node 2) - x(node 0)| */
+  MH_L_MONODIM_Y = 3,
+  MH_L_MONODIM_Z = 4
+} MhCaracteristicLength;
+
 /* Node coordinate convention of the Cartesian generator. */
 enum MhCoordRule { MH_COORD_MUL = 0 /* x_i = o + i*(l/n) */, MH_COORD_DIV = 1 /* x_i = o + (l*i)/n */ };
 
@@ -142,6 +151,8 @@ typedef struct MhOptions {
      n+1 before the remap, energy from the corner-force work at n and n+1 (MahycoModule.cc:451-453,
      574-690, 1661-1697; PrepareRemap.cc:210, 236-240) */
   int schema_csts;
+  int longueur_caracteristique;  /* MhCaracteristicLength (Mahyco.axl longueur-caracteristique) */
+  int pad_lc;
 } MhOptions;
 
 /* Field identifiers for get/set.  Cell fields take an environment index

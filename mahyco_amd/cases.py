@@ -82,6 +82,7 @@ class Case:
     with_newton: bool = False
     pression_explicite: bool = False
     schema_csts: bool = False
+    longueur_caracteristique: int = 0  # MhCaracteristicLength
     sans_lagrange: bool = False
     rider_reverse: bool = False
     rider_parametre: float = 0.0
@@ -155,6 +156,7 @@ class Case:
         o.with_newton = int(self.with_newton)
         o.pression_explicite = int(self.pression_explicite)
         o.schema_csts = int(self.schema_csts)
+        o.longueur_caracteristique = int(self.longueur_caracteristique)
         o.sans_lagrange = int(self.sans_lagrange)
         o.rider_reverse = int(self.rider_reverse)
         o.rider_parametre = self.rider_parametre

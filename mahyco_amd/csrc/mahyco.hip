@@ -664,6 +664,7 @@ void launch_node_t(MhHandle* h) {
 }
 template <int DIM, int M>
 void launch_cell_t(MhHandle* h) {
+  if (h->dev.lc != MH_L_RACINE_CUBIQUE_VOLUME) LAUNCH(h, KN_CELL, (k_cell_length<DIM>), h->nblk_c, MH_BLOCK, h->dev);
   if (h->dev.csts) LAUNCH(h, KN_CELL, (k_cell_lag_csts<DIM, M>), h->nblk_c, MH_BLOCK, h->dev);
   else LAUNCH(h, KN_CELL, (k_cell_lag<DIM, M>), h->nblk_c, MH_BLOCK, h->dev);
 }
@@ -744,6 +745,8 @@ int validate(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const Mh
   // CSTS: the closed-form energy (with Newton the reference reads p before setting it,
   // MahycoModule.cc:1595-1601; explicit pressure applies the EOS before the forward step)
   if (o->schema_csts && (o->with_newton || o->pression_explicite || o->sans_lagrange)) return MH_ERR_UNSUPPORTED;
+  if (o->longueur_caracteristique < 0 || o->longueur_caracteristique > MH_L_MONODIM_Z) return MH_ERR_INVALID;
+  if (o->longueur_caracteristique == MH_L_FACES_OPPOSEES && mesh->dim != 3) return MH_ERR_UNSUPPORTED;  // 8 nodes
   if (o->cas_model == MH_CASE_RIEMANN && nb_env < 2) return MH_ERR_INVALID;
   if (o->cas_model == MH_CASE_POINTTRIPLE && nb_env < 3) return MH_ERR_INVALID;
   for (int k = 0; k < nb_env; ++k)
@@ -806,6 +809,7 @@ struct EnsGroup {
   bool gen = false;  // pente-borne or generalised limiters
   bool csts = false; // schema-csts
   bool adv = false;  // advection mode (sans-lagrange): prescribed node velocity, no dual sweeps
+  bool lc = false;   // characteristic length other than V^(1/d): k_cell_length first
   std::vector<int> slots;
   int2* d_cmap = nullptr;
   int2* d_nmap = nullptr;
@@ -880,7 +884,7 @@ int ens_rebuild(MhEnsembleImpl* E) {
     bool found = false;
     for (auto& g : E->groups)
       if (g.dim == h->g.dim && g.m == h->m && g.o3 == (h->dev.order2 == 2) && g.gen == ens_gen(h) &&
-          g.csts == (h->dev.csts != 0) && g.adv == (h->dev.sans_lagrange != 0)) {
+          g.csts == (h->dev.csts != 0) && g.adv == (h->dev.sans_lagrange != 0) && g.lc == (h->dev.lc != 0)) {
         g.slots.push_back(sl);
         found = true;
       }
@@ -892,6 +896,7 @@ int ens_rebuild(MhEnsembleImpl* E) {
       g.gen = ens_gen(h);
       g.csts = h->dev.csts != 0;
       g.adv = h->dev.sans_lagrange != 0;
+      g.lc = h->dev.lc != 0;
       g.slots.push_back(sl);
       E->groups.push_back(g);
     }
@@ -951,6 +956,7 @@ void ens_group_cycle(MhEnsembleImpl* E, const EnsGroup& g) {
   hipStream_t st = E->stream;
   if (g.adv) hipLaunchKernelGGL((k_node_adv_batch<DIM>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap);
   else hipLaunchKernelGGL((k_node_lag_batch<DIM>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap);
+  if (g.lc) hipLaunchKernelGGL((k_cell_length_batch<DIM>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap);
   if (g.csts) hipLaunchKernelGGL((k_cell_lag_csts_batch<DIM, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap);
   else hipLaunchKernelGGL((k_cell_lag_batch<DIM, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap);
   if (g.csts) hipLaunchKernelGGL((k_node_fwd_batch<DIM>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap);
@@ -1136,6 +1142,7 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
   S.sans_lagrange = options->sans_lagrange ? 1 : 0;
   S.rider_reverse = options->rider_reverse ? 1 : 0;
   S.csts = options->schema_csts ? 1 : 0;
+  S.lc = options->longueur_caracteristique;
   S.dt_init = options->deltat_init;
   S.with_proj = options->with_projection ? 1 : 0;
   S.vitson_max = options->calcul_vitson_max ? 1 : 0;
@@ -1860,7 +1867,8 @@ static bool same_options(const MhOptions& a, const MhOptions& b) {
             a.interface_axis == b.interface_axis && a.sedov_deposit == b.sedov_deposit &&
             a.sans_lagrange == b.sans_lagrange && a.rider_reverse == b.rider_reverse &&
             a.rider_parametre == b.rider_parametre && a.rider_rayon_bulle == b.rider_rayon_bulle &&
-            a.rider_densite_lineaire == b.rider_densite_lineaire && a.schema_csts == b.schema_csts;
+            a.rider_densite_lineaire == b.rider_densite_lineaire && a.schema_csts == b.schema_csts &&
+            a.longueur_caracteristique == b.longueur_caracteristique;
   for (int i = 0; eq && i < a.nb_bc && i < MH_MAX_BC; ++i)
     eq = a.bc[i].surface == b.bc[i].surface && a.bc[i].type == b.bc[i].type && a.bc[i].value == b.bc[i].value;
   return eq;

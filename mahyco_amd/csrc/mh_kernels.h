@@ -619,6 +619,27 @@ __device__ __forceinline__ double csts_qnm1(const MhDev& S, long c, int k, bool 
   return S.qm1[c];
 }
 
+// The characteristic lengths other than V^(1/d) (computeGeometricValues, MahycoModule.cc:
+// 1195-1236): the median lengths between opposite faces (3D), or one side of the cell.
+template <int DIM>
+__device__ __forceinline__ double char_length_alt(int lc, const D3* x) {
+  if (DIM == 3 && lc == MH_L_FACES_OPPOSEES) {
+    auto quarter_sum = [](D3 a, D3 b, D3 c, D3 d) {
+      const D3 s = a + b + c + d;
+      return d3(0.25 * s.x, 0.25 * s.y, 0.25 * s.z);
+    };
+    auto norm = [](D3 v) { return sqrt(v.x * v.x + v.y * v.y + v.z * v.z); };
+    const D3 f0 = quarter_sum(x[0], x[3], x[2], x[1]), f1 = quarter_sum(x[0], x[4], x[7], x[3]);
+    const D3 f2 = quarter_sum(x[0], x[1], x[5], x[4]), f3 = quarter_sum(x[4], x[5], x[6], x[7]);
+    const D3 f4 = quarter_sum(x[1], x[2], x[6], x[5]), f5 = quarter_sum(x[2], x[3], x[7], x[6]);
+    const double d1 = norm(f0 - f3), d2 = norm(f2 - f5), d3v = norm(f1 - f4);
+    return (d1 * d2 * d3v) / (d1 * d2 + d1 * d3v + d2 * d3v);
+  }
+  if (lc == MH_L_MONODIM_X) return fabs(x[2].x - x[0].x);
+  if (lc == MH_L_MONODIM_Y) return fabs(x[2].y - x[0].y);
+  return fabs(x[2].z - x[0].z);
+}
+
 // A cell that no environment occupies gets no remap values from computeVariablesForRemap
 // (PrepareRemap.cc:100-166 loop over environment cells): its U is 0 and its Phi is the one
 // the last sweep of the previous remap left, U / sum V_k (RemapADIService.cc:493-538).
@@ -655,7 +676,8 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
       V = cell_volume<4>(x, qv, 2);
     }
     if (V < 0.) S.err[0] = MH_ERR_NEG_VOLUME;
-    const double L = DIM == 3 ? mh_pow_third(V) : mh_pow_half(V);
+    // the other characteristic lengths come from k_cell_length, launched just before
+    const double L = S.lc == MH_L_RACINE_CUBIQUE_VOLUME ? (DIM == 3 ? mh_pow_third(V) : mh_pow_half(V)) : S.L[c];
     if (S.sans_lagrange) {
       // advection mode: no density, pseudo or energy update (MahycoModule.cc:1271, 389,
       // 1433); the remap vector of PrepareRemap.cc:100-155 from the stored state
@@ -856,6 +878,23 @@ __device__ __forceinline__ void k_cell_lag_body(const MhDev& S, int lb) {
   }
   if (!S.with_proj) block_min_store(dxs, active, S.partial, lb);
 }
+// The characteristic length of the other options, written to L before the cell kernel
+// (which then reads it instead of forming V^(1/d)): one thread per cell.
+template <int DIM>
+__device__ __forceinline__ void k_cell_length_body(const MhDev& S, int lb) {
+  const long c = lb * (long)blockDim.x + threadIdx.x;
+  if (c >= S.g.nc || mh_halted(S.sc)) return;
+  const int nx = S.g.nx, ny = S.g.ny;
+  const int i = (int)(c % nx);
+  const long r = c / nx;
+  const int j = (int)(r % ny);
+  const int k = (int)(r / ny);
+  const long b = i + (long)S.g.nnx * (j + (long)S.g.nny * k);
+  D3 x[8];
+  cell_nodes<DIM>(S, b, S.xlc, x);
+  S.L[c] = char_length_alt<DIM>(S.lc, x);
+}
+
 // CSTS (with remap): its own instance, so that the other cell kernels do not carry the
 // corner-work registers
 template <int DIM, int M>
@@ -2793,6 +2832,7 @@ __global__ __launch_bounds__(MH_BLOCK) void k_hist_final(MhDev S) { k_hist_final
 MH_WRAP(k_node_lag, 1, template <int DIM>, <DIM>, , )
 MH_WRAP(k_cell_lag, MH_CELL_MIN_WAVES, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, , )
 MH_WRAP(k_cell_lag_csts, MH_CELL_MIN_WAVES, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, , )
+MH_WRAP(k_cell_length, 1, template <int DIM>, <DIM>, , )
 MH_WRAP(k_sweep_cell, 1, template <int DIM MH_COMMA int D MH_COMMA int M MH_COMMA int LIM>, <DIM MH_COMMA D MH_COMMA M MH_COMMA LIM>,
         MH_COMMA int usel MH_COMMA int first, MH_COMMA usel MH_COMMA first)
 MH_WRAP(k_sweep_node, 1, template <int DIM MH_COMMA int D MH_COMMA int LIM>, <DIM MH_COMMA D MH_COMMA LIM>,

@@ -106,6 +106,7 @@ struct MhDev {
   // pseudo-viscosity of the previous cycle (qm1), the remap's U_q / V of the last and the
   // previous remap (qe, qem1: the partial pseudo-viscosity of every environment of a mixed
   // cell) and the environment mask before the last remap (mkm1)
+  int lc;  // MhCaracteristicLength
   int csts;
   double* un[3];
   double *qm1, *qe, *qem1;

@@ -98,8 +98,11 @@ void MahycoModule::check(int st, const char* where) const {
 
 void MahycoModule::hydroStartInit() {
   if (opt_.pseudo_centree) throw MahycoError(MH_ERR_UNSUPPORTED, "pseudo-centree");
-  if (opt_.longueur_caracteristique != "racine-cubique-volume")
-    throw MahycoError(MH_ERR_UNSUPPORTED, "longueur-caracteristique " + opt_.longueur_caracteristique);
+  static const char* kLengths[] = {"racine-cubique-volume", "faces-opposees", "monodimX", "monodimY", "monodimZ"};
+  int lc = -1;
+  for (int i = 0; i < 5; ++i)
+    if (opt_.longueur_caracteristique == kLengths[i]) lc = i;
+  if (lc < 0) throw MahycoError(MH_ERR_UNSUPPORTED, "longueur-caracteristique " + opt_.longueur_caracteristique);
   RemapADI* adi = dynamic_cast<RemapADI*>(remap_.get());
   if (!adi) throw MahycoError(MH_ERR_UNSUPPORTED, "remap service other than RemapADI");
   if (adi->isEuler() != "Full" || adi->hasConservationEnergieTotale())
@@ -140,6 +143,7 @@ void MahycoModule::hydroStartInit() {
   o.with_newton = opt_.with_newton;
   o.pression_explicite = opt_.pression_explicite;
   o.schema_csts = opt_.schema_csts;
+  o.longueur_caracteristique = lc;
   o.sans_lagrange = opt_.sans_lagrange;
   o.rider_reverse = opt_.cas.reverse_option;
   o.rider_parametre = opt_.cas.parametre;

@@ -36,6 +36,8 @@ MH_MINMOD, MH_SUPERBEE, MH_VANLEER = 0, 1, 2
 MH_MINMOD_G, MH_SUPERBEE_G, MH_VANLEER_G, MH_ARITHMETIC_G, MH_ULTRABEE_G = 3, 4, 5, 6, 7
 
 MH_CASE_SOD, MH_CASE_SEDOV, MH_CASE_RIEMANN, MH_CASE_POINTTRIPLE, MH_CASE_RIDER, MH_CASE_ADVECTION = range(6)
+# MhCaracteristicLength
+MH_L_RACINE_CUBIQUE_VOLUME, MH_L_FACES_OPPOSEES, MH_L_MONODIM_X, MH_L_MONODIM_Y, MH_L_MONODIM_Z = range(5)
 MH_COORD_MUL, MH_COORD_DIV = 0, 1
 
 # MhField
@@ -95,4 +97,4 @@ class MhOptions(C.Structure):
                 ("projection_simple_pente", C.c_int), ("with_newton", C.c_int),
                 ("pression_explicite", C.c_int), ("sans_lagrange", C.c_int), ("rider_reverse", C.c_int),
                 ("rider_parametre", C.c_double), ("rider_rayon_bulle", C.c_double),
-                ("rider_densite_lineaire", C.c_int), ("schema_csts", C.c_int)]
+                ("rider_densite_lineaire", C.c_int), ("schema_csts", C.c_int), ("longueur_caracteristique", C.c_int), ("pad_lc", C.c_int)]

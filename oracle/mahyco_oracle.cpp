@@ -933,8 +933,27 @@ int OracleCase::computeGeometricValues() {
     volume /= dim;
     cell_volume.g[c] = volume;
     if (volume < 0.) err = 1;
-    caracteristic_length[c] = g_pow_mode == 0 ? std::pow(cell_volume.g[c], racine)
-                              : (dim == 3 ? mh_pow_third(cell_volume.g[c]) : mh_pow_half(cell_volume.g[c]));
+    const int lc = opt.longueur_caracteristique;  // MahycoModule.cc:1195-1236
+    const R3* cn = nullptr;
+    R3 cl[8];
+    for (int l = 0; l < nnpc; ++l) cl[l] = node_coord[cell_nodes[c * nnpc + l]];
+    cn = cl;
+    if (lc == MH_L_FACES_OPPOSEES) {
+      const R3 f0 = 0.25 * (cn[0] + cn[3] + cn[2] + cn[1]), f1 = 0.25 * (cn[0] + cn[4] + cn[7] + cn[3]);
+      const R3 f2 = 0.25 * (cn[0] + cn[1] + cn[5] + cn[4]), f3 = 0.25 * (cn[4] + cn[5] + cn[6] + cn[7]);
+      const R3 f4 = 0.25 * (cn[1] + cn[2] + cn[6] + cn[5]), f5 = 0.25 * (cn[2] + cn[3] + cn[7] + cn[6]);
+      const double d1 = normL2(f0 - f3), d2 = normL2(f2 - f5), d3 = normL2(f1 - f4);
+      caracteristic_length[c] = (d1 * d2 * d3) / (d1 * d2 + d1 * d3 + d2 * d3);
+    } else if (lc == MH_L_MONODIM_X) {
+      caracteristic_length[c] = std::fabs(cn[2].x - cn[0].x);
+    } else if (lc == MH_L_MONODIM_Y) {
+      caracteristic_length[c] = std::fabs(cn[2].y - cn[0].y);
+    } else if (lc == MH_L_MONODIM_Z) {
+      caracteristic_length[c] = std::fabs(cn[2].z - cn[0].z);
+    } else {
+      caracteristic_length[c] = g_pow_mode == 0 ? std::pow(cell_volume.g[c], racine)
+                                : (dim == 3 ? mh_pow_third(cell_volume.g[c]) : mh_pow_half(cell_volume.g[c]));
+    }
   }
   // mixed cells: V_k = f_k V
   for (long c = 0; c < nc; ++c) {
@@ -2058,6 +2077,8 @@ OracleCase* oracle_create(const MhMesh* mesh, const MhEnvironment* envs, int nb_
   // CSTS with Newton reads p before setting it (MahycoModule.cc:1595-1601); with explicit
   // pressure the EOS runs before the forward velocity step; advection mode has no Lagrange step
   if (opt->schema_csts && (opt->with_newton || opt->pression_explicite || opt->sans_lagrange)) return nullptr;
+  if (opt->longueur_caracteristique < 0 || opt->longueur_caracteristique > MH_L_MONODIM_Z) return nullptr;
+  if (opt->longueur_caracteristique == MH_L_FACES_OPPOSEES && mesh->dim != 3) return nullptr;  // 8 nodes, 6 faces
   // RIDER / ADVECTION give the bubble environment 1's density, Mono tests included
   if ((opt->cas_model == MH_CASE_RIDER || opt->cas_model == MH_CASE_ADVECTION) && nb_env < 2) return nullptr;
   OracleCase* h = new OracleCase();

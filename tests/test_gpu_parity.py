@@ -108,6 +108,10 @@ CASES = {
     # 2D advection meshes (the datasets are 3D with one cell in z)
     "rider2d_rotation_reverse": (lambda: _two_d(cases.rider("r2rev", 34, .3, 2.), 48), 300),
     "advection2d_t45_ordre3": (lambda: _variant(_two_d(cases.advection("a2", 20, .2, n=40), 40), ordre_projection=3), 200),
+    # characteristic lengths other than V^(1/d) (k_cell_length before the cell kernel)
+    "bisod3d_faces_opposees": (lambda: _variant(cases.config2_bisod3d(32, 16, 16), longueur_caracteristique=1), 100),
+    "riemann2d_monodimX": (lambda: _variant(_riemann2d(True, 26), longueur_caracteristique=2), 150),
+    "riemann3d_monodimY_csts": (lambda: _variant(_riemann3d(27), longueur_caracteristique=3, schema_csts=True), 60),
     # CSTS (single environment): backward / forward velocity steps, corner-work energy, mean
     # face velocities; 2D and 3D, order 2 and 3, with and without the tiled kernels' paths
     "sedov3d_csts_20": (lambda: _variant(cases.config3_sedov3d(20), schema_csts=True), 100),

@@ -305,3 +305,31 @@ def test_csts_unsupported_combinations():
         setattr(c, key, True)
         with pytest.raises(Exception):
             OracleCase(c)
+
+
+@pytest.mark.parametrize("lc", [T.MH_L_FACES_OPPOSEES, T.MH_L_MONODIM_X, T.MH_L_MONODIM_Y, T.MH_L_MONODIM_Z])
+def test_characteristic_length_closed_forms(lc):
+    """longueur-caracteristique (MahycoModule.cc:1195-1236) on a box cell dx x dy x dz: the
+    medians between opposite faces are dx, dy, dz, so faces-opposees gives
+    dx dy dz / (dx dy + dx dz + dy dz); monodimX/Y give one side.  monodimZ takes the z of
+    nodes 0 and 2 like the others, which Arcane's hexahedron order puts on the same face: it is
+    0 (a run using it stops at deltat-min).  A uniform state at rest keeps the Eulerian box, so
+    the length after a cycle is exact."""
+    from oracle.pyoracle import OracleCase
+    c = _uniform(3, 6)
+    c.n, c.length = (6, 5, 4), (0.6, 0.25, 0.8)
+    c.longueur_caracteristique = lc
+    o = OracleCase(c)
+    o.run(1)
+    dx, dy, dz = 0.1, 0.05, 0.2
+    expect = {T.MH_L_FACES_OPPOSEES: dx * dy * dz / (dx * dy + dx * dz + dy * dz),
+              T.MH_L_MONODIM_X: dx, T.MH_L_MONODIM_Y: dy, T.MH_L_MONODIM_Z: 0.}[lc]
+    np.testing.assert_allclose(o.get(T.MH_F_CARACTERISTIC_LENGTH), expect, rtol=1e-14, atol=0)
+
+
+def test_faces_opposees_needs_3d():
+    from oracle.pyoracle import OracleCase
+    c = _uniform(2, 6)
+    c.longueur_caracteristique = T.MH_L_FACES_OPPOSEES
+    with pytest.raises(Exception):
+        OracleCase(c)

@@ -23,9 +23,33 @@ def olib():
 R = np.concatenate([np.linspace(-3, 3, 601), [0.5, 1.0, 2.0, 1e-300, -1e-300, 1e300]])
 
 
-def test_minmod_closed_form(olib):
+def _slope(olib, lim, gb, gf):
+    """Cell slope of computeAndLimitGradPhi (UtilesRemap.cc:325-357) from the oracle's limiter:
+    1/2 phi(gf/gb) gb [gb != 0] + 1/2 phi(gb/gf) gf [gf != 0]."""
+    s = 0.0
+    if gb != 0.0:
+        s += 0.5 * olib.oracle_flux_limiter(lim, gf / gb) * gb
+    if gf != 0.0:
+        s += 0.5 * olib.oracle_flux_limiter(lim, gb / gf) * gf
+    return s
+
+
+G = [(a, b) for a in (-3.0, -1.0, -0.25, 0.0, 0.5, 1.0, 2.0, 7.5) for b in (-2.0, -0.5, 0.0, 0.3, 1.0, 4.0, 1e-3)]
+
+
+def test_minmod_slope_is_smallest_modulus(olib):
+    """minmod: the limited slope is the one-sided gradient of smaller modulus when both have
+    the same sign, 0 otherwise (the textbook minmod of the two face gradients)."""
+    for gb, gf in G:
+        expect = 0.0 if gb * gf <= 0 else (gb if abs(gb) < abs(gf) else gf)
+        assert _slope(olib, 0, gb, gf) == pytest.approx(expect, rel=1e-15, abs=0), (gb, gf)
+
+
+def test_minmod_piecewise(olib):
+    # 0 for r <= 0, the identity on [0, 1], 1 beyond: three pieces checked on a grid
     for r in R:
-        assert olib.oracle_flux_limiter(0, r) == max(0.0, min(1.0, r))
+        expect = 0.0 if r <= 0 else (r if r <= 1 else 1.0)
+        assert olib.oracle_flux_limiter(0, r) == expect, r
 
 
 def test_superbee_closed_form(olib):
@@ -35,12 +59,25 @@ def test_superbee_closed_form(olib):
         assert olib.oracle_flux_limiter(1, r) == expect, r
 
 
-def test_vanleer_closed_form(olib):
-    for r in R:
-        expect = 0.0 if r <= 0 else 2 * r / (1 + r)
-        assert olib.oracle_flux_limiter(2, r) == expect
-    # symmetry phi(r)/r = phi(1/r) of the classical limiters
+def test_vanleer_slope_is_harmonic_mean(olib):
+    """vanLeer: the limited slope is the harmonic mean 2 gb gf / (gb + gf) of the face gradients
+    when they have the same sign, 0 otherwise (van Leer 1974)."""
+    for gb, gf in G:
+        expect = 0.0 if gb * gf <= 0 else 2 * gb * gf / (gb + gf)
+        assert _slope(olib, 2, gb, gf) == pytest.approx(expect, rel=1e-14, abs=1e-300), (gb, gf)
+
+
+def test_classical_limiters_properties(olib):
+    """Second-order TVD limiters (Sweby): phi(1) = 1, phi(r) = 0 for r <= 0, inside the region
+    min(2r, 2) >= phi >= min(r, 1) for r > 0, and symmetric: phi(r)/r = phi(1/r)."""
     for lim in (0, 1, 2):
+        assert olib.oracle_flux_limiter(lim, 1.0) == 1.0
+        for r in R:
+            v = olib.oracle_flux_limiter(lim, r)
+            if r <= 0:
+                assert v == 0.0
+            else:
+                assert min(r, 1.0) * (1 - 1e-15) <= v <= min(2 * r, 2.0) * (1 + 1e-15), (lim, r)
         for r in (0.25, 0.5, 0.8, 1.5, 3.0):
             a = olib.oracle_flux_limiter(lim, r) / r
             b = olib.oracle_flux_limiter(lim, 1.0 / r)
@@ -223,6 +260,66 @@ def test_newton_energy_stiffened_gas_root():
     root = (e_n - 0.5 * (p_n + 2 * qw - gamma * pi) * dv) / (1 + 0.5 * (gamma - 1) * rho * dv)
     assert pure.sum() > 100
     np.testing.assert_allclose(e[pure], root[pure], rtol=1e-9, atol=1e-9)
+
+
+def test_stiffened_gas_energy_update_uses_gamma_only():
+    """Default energy update (updateEnergyAndPressureforGP, MahycoModule.cc:1633-1659) on a
+    stiffened gas: the reference prices the implicit pressure work with the perfect-gas part
+    (gamma - 1) rho e alone, for StiffenedGas too (:1636 reads only getAdiabaticCst).  So each
+    pure cell's energy after a Lagrange step satisfies the VNR relation
+        e - e_n + (p_n / 2 + (gamma - 1) rho e / 2 + q_w) (1/rho - 1/rho_n) = 0,
+    q_w = q when it works against the compression (q dv < 0), and NOT the same relation with
+    the stiffened-gas pressure (gamma - 1) rho e - gamma pi, whose residual is gamma pi dv / 2
+    (the root a pi-aware update would give).  Mixed cells: e = sum_k chi_k e_k (:1453-1464),
+    and every environment's pressure is its stiffened-gas EOS of (rho_k, e_k).  Checked over
+    the first twelve steps of a stiffened-gas tube whose interface cuts a column of cells."""
+    from oracle.pyoracle import OracleCase
+    rng = np.random.default_rng(11)
+    c = cases.riemann2d(40, 4, rng, stiffened=True)
+    c.envs[0].limit_tension, c.envs[1].limit_tension = 0.35, 0.2
+    c.envs[0].adiabatic_cst, c.envs[1].adiabatic_cst = 3.1, 1.9
+    c.interface_position = 0.5 + 0.3 / 40  # the interface cuts the column of cells 20
+    c.with_projection = False
+    o = OracleCase(c)
+    gam = np.array([e.adiabatic_cst for e in c.envs])
+    pis = np.array([e.limit_tension for e in c.envs])
+    checked_pure = checked_mixed = 0
+    far = 0
+    for step in range(12):
+        mask = o.get(T.MH_F_ENV_MASK).astype(int)
+        rho_n, p_n, e_n = o.get(T.MH_F_DENSITY), o.get(T.MH_F_PRESSURE), o.get(T.MH_F_INTERNAL_ENERGY)
+        assert o.run(1) == T.MH_OK
+        assert np.array_equal(o.get(T.MH_F_ENV_MASK).astype(int), mask)  # no remap: membership fixed
+        rho, q, e = o.get(T.MH_F_DENSITY), o.get(T.MH_F_PSEUDO_VISCOSITY), o.get(T.MH_F_INTERNAL_ENERGY)
+        pure = (mask == 1) | (mask == 2)
+        k = np.where(mask == 2, 1, 0)
+        g, pi = gam[k], pis[k]
+        dv = 1.0 / rho - 1.0 / rho_n
+        qw = np.where(q * dv < 0.0, q, 0.0)
+        scale = np.abs(e) + np.abs(e_n) + (np.abs(p_n) + np.abs((g - 1) * rho * e) + np.abs(qw)) * np.abs(dv)
+        res_gamma = e - e_n + (0.5 * p_n + 0.5 * (g - 1) * rho * e + qw) * dv
+        res_pi = e - e_n + (0.5 * p_n + 0.5 * ((g - 1) * rho * e - g * pi) + qw) * dv
+        moving = pure & (np.abs(dv) > 1e-9)
+        assert np.all(np.abs(res_gamma[pure]) <= 1e-14 * scale[pure]), step
+        # the pi-aware relation misses by gamma pi dv / 2 wherever the cell moved
+        np.testing.assert_allclose(res_pi[moving], -0.5 * g[moving] * pi[moving] * dv[moving], rtol=1e-6)
+        far += int(np.sum(np.abs(res_pi[moving]) > 1e6 * 1e-14 * scale[moving]))
+        checked_pure += int(moving.sum())
+        # mixed cells: mass-fraction average of the environments' energies, EOS per environment
+        mixed = mask == 3
+        if mixed.any():
+            ek = [o.get(T.MH_F_INTERNAL_ENERGY, kk) for kk in range(2)]
+            chi = [o.get(T.MH_F_MASS_FRACTION, kk) for kk in range(2)]
+            rk = [o.get(T.MH_F_DENSITY, kk) for kk in range(2)]
+            pk = [o.get(T.MH_F_PRESSURE, kk) for kk in range(2)]
+            emix = chi[0] * ek[0] + chi[1] * ek[1]
+            np.testing.assert_allclose(e[mixed], emix[mixed], rtol=1e-15, atol=0)
+            for kk in range(2):
+                np.testing.assert_allclose(pk[kk][mixed], (gam[kk] - 1) * rk[kk][mixed] * ek[kk][mixed] - gam[kk] * pis[kk],
+                                           rtol=1e-12, atol=1e-15)
+            checked_mixed += int(mixed.sum())
+    o.close()
+    assert checked_pure > 50 and checked_mixed >= 4 and far > 50, (checked_pure, checked_mixed, far)
 
 
 def test_explicit_pressure_is_consistent_first_order():

@@ -27,6 +27,11 @@ typedef struct MhHandle MhHandle;
 int mahyco_create(const MhMesh* mesh, const MhEnvironment* envs, int nb_env,
                   const MhOptions* options, MhHandle** out);
 void mahyco_destroy(MhHandle* h);
+/* Same, with every call of the handle ordered on the caller's HIP stream (a hipStream_t of the
+ * current device, default priority, passed as void*; the caller keeps it alive until
+ * mahyco_destroy).  Replaces running the module inside the caller's own execution context. */
+int mahyco_create_on_stream(const MhMesh* mesh, const MhEnvironment* envs, int nb_env,
+                            const MhOptions* options, void* stream, MhHandle** out);
 
 /* Runs ncycles iterations of the time loop (Mahyco.config:23-36 MahycoLoop, or
  * :57-69 MahycoLagrangeLoop when with_projection = 0), asynchronously on the handle's
@@ -49,11 +54,32 @@ double mahyco_deltat(MhHandle* h);
 int mahyco_iteration(MhHandle* h);
 long mahyco_nb_cells(MhHandle* h);
 long mahyco_nb_nodes(MhHandle* h);
+/* Number of hipGraphs mahyco_cycle_graph has captured and instantiated on this handle
+   (no counterpart in the reference: lets a timing loop show that no capture fell inside it). */
+long mahyco_graph_captures(MhHandle* h);
 
 /* Copies one field (MhField) to host memory `out` (nb_cells or nb_nodes doubles).
  * Cell fields: env = -1 (cell value) or environment index (0 outside the env).
  * Node fields: comp = 0,1,2.  Replaces reading the Arcane variables by name. */
 int mahyco_get_field(MhHandle* h, int field, int env, int comp, double* out);
+
+/* Writes one field of the state from host memory (nb_cells or nb_nodes doubles); replaces a
+ * module or case service writing the named Arcane variables that the module and its services
+ * share (casTest/IInitialisations.h:26-56 initMat/initVar, MahycoModule.cc:110-140).
+ * Cell fields: env = -1 writes the cell value; env = k writes environment k's value (its
+ * partial value in a multi-environment cell, the cell value in a cell of k alone, nothing
+ * outside k), so MH_F_ENV_MASK goes first.  Settable: density, pressure, internal energy,
+ * cell mass, volume fraction, mass fraction (cell and environment values), sound speed and
+ * pseudo-viscosity (cell values: a mixed cell's environment values are re-formed by the
+ * kernels), the environment mask, node velocity and MH_N_COORD_LAGRANGE (MH_N_COORD without
+ * remap).  Volume, characteristic length, Materiau and node mass are derived: MH_ERR_INVALID.
+ * Between iterations only (not inside an entry-point sequence). */
+int mahyco_set_field(MhHandle* h, int field, int env, int comp, const double* in);
+/* The scalars that go with the state: m_global_time, m_global_deltat, m_global_old_deltat,
+ * m_global_iteration and m_sens_projection (Mahyco.axl:11, RemapADIService.cc:23-25, 59-60).
+ * Clears the stop flag and the status word; the next iteration is number iteration + 1. */
+int mahyco_set_scalars(MhHandle* h, double time, double deltat, double old_deltat, int iteration,
+                       int sens_projection);
 
 /* ---- entry points of MahycoModule (Mahyco.axl:244-261), one call each ----------
  * They run the same kernels as mahyco_cycle, grouped as the kernels fuse them:

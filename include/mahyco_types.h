@@ -173,7 +173,11 @@ enum MhField {
   /* node fields; component selects x/y/z */
   MH_N_VELOCITY = 100,
   MH_N_NODE_MASS = 101,
-  MH_N_COORD = 102
+  MH_N_COORD = 102,
+  /* node coordinates at the end of the last Lagrange phase: the positions the corner vectors
+     m_cell_cqs of the next velocity update are formed at (MahycoModule.cc:438-562, 1097-1162);
+     equal to MH_N_COORD without remap */
+  MH_N_COORD_LAGRANGE = 103
 };
 
 #ifdef __cplusplus

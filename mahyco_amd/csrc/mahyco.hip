@@ -71,6 +71,7 @@ struct MhHandle {
   int prof_cnt[KN_COUNT] = {0};
   std::map<long, hipGraphExec_t> graphs;  // key: (cycles << 8) | (parity << 4) | sens
   std::vector<hipGraph_t> graph_defs;
+  long graph_captures = 0;  // graphs captured and instantiated so far
   bool owns_stream = true;
   bool node_tiled = true;   // 3D: z-marching node kernel (MAHYCO_NODE_KERNEL=flat selects the flat one)
   int sweep_mode = MH_SWEEP_TILE;  // MAHYCO_SWEEP=flat selects one thread per cell / node
@@ -1059,6 +1060,12 @@ int mahyco_create(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, con
   return create_case(mesh, envs, nb_env, options, nullptr, nullptr, out);
 }
 
+int mahyco_create_on_stream(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const MhOptions* options,
+                            void* stream, MhHandle** out) {
+  if (!stream) return MH_ERR_INVALID;
+  return create_case(mesh, envs, nb_env, options, (hipStream_t)stream, nullptr, out);
+}
+
 static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const MhOptions* options,
                        hipStream_t stream, MhScalars* ext_sc, MhHandle** out) {
   if (!out) return MH_ERR_INVALID;
@@ -1356,18 +1363,27 @@ int mahyco_cycle_graph(MhHandle* h, int ncycles, int cycles_per_graph) {
     const long key = ((long)P << 8) | ((h->launched & 1) << 4) | h->sens;
     auto it = h->graphs.find(key);
     if (it == h->graphs.end()) {
-      hipGraph_t gdef;
+      hipGraph_t gdef = nullptr;
       HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
       const long l0 = h->launched;
       const int s0 = h->sens;
       for (int i = 0; i < P; ++i) enqueue_cycle(h);
-      HIPCHK(hipStreamEndCapture(h->stream, &gdef));
-      h->launched = l0;  // capture enqueues nothing: roll the host counters back
-      h->sens = s0;
+      const hipError_t ec = hipStreamEndCapture(h->stream, &gdef);
+      h->launched = l0;  // capture enqueues nothing: roll the host counters back,
+      h->sens = s0;      // also when the capture failed
+      if (ec != hipSuccess) {
+        if (gdef) (void)hipGraphDestroy(gdef);
+        return hip_check(ec, "hipStreamEndCapture");
+      }
       hipGraphExec_t ex;
-      HIPCHK(hipGraphInstantiate(&ex, gdef, nullptr, nullptr, 0));
+      const hipError_t ei = hipGraphInstantiate(&ex, gdef, nullptr, nullptr, 0);
+      if (ei != hipSuccess) {
+        (void)hipGraphDestroy(gdef);
+        return hip_check(ei, "hipGraphInstantiate");
+      }
       h->graph_defs.push_back(gdef);
       it = h->graphs.emplace(key, ex).first;
+      h->graph_captures++;
     }
     HIPCHK(hipGraphLaunch(it->second, h->stream));
     h->launched += P;
@@ -1408,6 +1424,7 @@ int mahyco_iteration(MhHandle* h) {
   return sc.iteration;
 }
 long mahyco_nb_cells(MhHandle* h) { return h ? h->g.nc : -1; }
+long mahyco_graph_captures(MhHandle* h) { return h ? h->graph_captures : -1; }
 long mahyco_nb_nodes(MhHandle* h) { return h ? h->g.nn : -1; }
 
 int mahyco_get_field(MhHandle* h, int field, int env, int comp, double* out) {
@@ -1461,6 +1478,7 @@ int mahyco_get_field(MhHandle* h, int field, int env, int comp, double* out) {
       }
       return d2h(h->xl[sc.iteration & 1][comp], out, nn);
     }
+    if (field == MH_N_COORD_LAGRANGE) return d2h(h->xl[sc.iteration & 1][comp], out, nn);
     return MH_ERR_INVALID;
   }
   if (env >= h->m || env < -1) return MH_ERR_INVALID;
@@ -1539,6 +1557,108 @@ int mahyco_get_field(MhHandle* h, int field, int env, int comp, double* out) {
   return MH_OK;
 }
 
+static void drop_graphs(MhHandle* h);
+
+// Writes one field of the state from host memory: the reverse of mahyco_get_field, for a
+// caller that owns the state as the reference's module does (the case services and the
+// module write the named Arcane variables, src/casTest/IInitialisations.h:26-56,
+// MahycoModule.cc:110-140).  Cell values with env = -1 go to the cell (global) arrays; with
+// env = k to environment k's value: its partial value in a cell of two or more environments,
+// the cell value in a cell of k alone (Arcane's single-environment aliasing), nothing in a
+// cell outside k.  Set MH_F_ENV_MASK before the environment values it selects.
+int mahyco_set_field(MhHandle* h, int field, int env, int comp, const double* in) {
+  if (!h || !in || h->phase != 0 || h->ext_sc) return MH_ERR_INVALID;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const long nc = h->g.nc, nn = h->g.nn;
+  MhDev& S = h->dev;
+  auto h2d = [&](double* d, const double* src, long n) -> int {
+    HIPCHK(hipMemcpy(d, src, n * sizeof(double), hipMemcpyHostToDevice));
+    return MH_OK;
+  };
+  if (field >= 100) {
+    if (comp < 0 || comp >= h->g.dim) return MH_ERR_INVALID;
+    for (long n = 0; n < nn; ++n)
+      if (!std::isfinite(in[n]) && field != MH_N_VELOCITY) return MH_ERR_INVALID;
+    if (field == MH_N_VELOCITY) return h2d(S.u[comp], in, nn);
+    // the positions the next iteration starts from (set_parity: xlp = xl[launched & 1])
+    if (field == MH_N_COORD_LAGRANGE || (field == MH_N_COORD && !h->opt.with_projection))
+      return h2d(h->xl[h->launched & 1][comp], in, nn);
+    return MH_ERR_INVALID;  // node mass follows the cell masses; Euler coordinates are the mesh's
+  }
+  if (env < -1 || env >= h->m) return MH_ERR_INVALID;
+  std::vector<uint8_t> mask(nc);
+  if (field == MH_F_ENV_MASK) {
+    if (env != -1) return MH_ERR_INVALID;
+    for (long c = 0; c < nc; ++c) {
+      const double v = in[c];
+      if (!(v >= 0. && v < (double)(1 << h->m)) || v != (double)(int)v) return MH_ERR_INVALID;
+      mask[c] = (uint8_t)(int)v;
+    }
+    HIPCHK(hipMemcpy(S.mask, mask.data(), nc, hipMemcpyHostToDevice));
+    return MH_OK;
+  }
+  double* gdst = nullptr;
+  double* edst = nullptr;
+  bool env_derived = false;  // recomputed per environment by the kernels (not stored)
+  switch (field) {
+    case MH_F_DENSITY: gdst = S.rho; edst = S.rhok; break;
+    case MH_F_PRESSURE: gdst = S.p; edst = S.pk; break;
+    case MH_F_INTERNAL_ENERGY: gdst = S.e; edst = S.ek; break;
+    case MH_F_SOUND_SPEED: gdst = S.c; env_derived = true; break;
+    case MH_F_PSEUDO_VISCOSITY: gdst = S.q; env_derived = true; break;
+    case MH_F_CELL_MASS: gdst = S.M; edst = S.Mk; break;
+    case MH_F_FRACVOL: gdst = S.fg; edst = S.fk; break;
+    case MH_F_MASS_FRACTION: gdst = S.chig; edst = S.chik; break;
+    default: return MH_ERR_INVALID;  // volume, length and Materiau are derived from the state
+  }
+  if (env < 0) return h2d(gdst, in, nc);
+  HIPCHK(hipMemcpy(mask.data(), S.mask, nc, hipMemcpyDeviceToHost));
+  std::vector<double> g(nc), e;
+  int st = MH_OK;
+  HIPCHK(hipMemcpy(g.data(), gdst, nc * sizeof(double), hipMemcpyDeviceToHost));
+  if (edst) {
+    e.resize(nc);
+    HIPCHK(hipMemcpy(e.data(), edst + (size_t)env * nc, nc * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  for (long c = 0; c < nc; ++c) {
+    const uint8_t b = mask[c];
+    if (!((b >> env) & 1)) continue;
+    if (__builtin_popcount(b) == 1) g[c] = in[c];
+    else if (edst) e[c] = in[c];
+  }
+  (void)env_derived;  // a mixed cell's environment sound speed and pseudo-viscosity are re-formed
+  if ((st = h2d(gdst, g.data(), nc)) != MH_OK) return st;
+  if (edst) st = h2d(edst + (size_t)env * nc, e.data(), nc);
+  return st;
+}
+
+// Time-loop and module scalars of the state (m_global_time, m_global_deltat,
+// m_global_old_deltat, m_global_iteration, m_sens_projection; Mahyco.axl:11): the next
+// iteration runs as iteration + 1 of a loop that reached these values.
+int mahyco_set_scalars(MhHandle* h, double time, double deltat, double old_deltat, int iteration,
+                       int sens_projection) {
+  if (!h || h->phase != 0 || h->ext_sc || iteration < 0 || sens_projection < 0 ||
+      sens_projection >= h->g.dim || !std::isfinite(time) || !(deltat > 0.) || !(old_deltat >= 0.))
+    return MH_ERR_INVALID;
+  MhScalars sc;
+  int st = read_scalars(h, &sc);
+  if (st != MH_OK) return st;
+  if ((h->launched ^ (long)iteration) & 1)  // the Lagrangian positions follow the parity
+    for (int d = 0; d < 3; ++d) std::swap(h->xl[0][d], h->xl[1][d]);
+  sc.t = time;
+  sc.dt = deltat;
+  sc.dt_old = old_deltat;
+  sc.iteration = iteration;
+  sc.stopped = 0;
+  sc.status = 0;
+  HIPCHK(hipMemcpy(h->dev.sc, &sc, sizeof sc, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(h->dev.err, 0, sizeof(int)));
+  h->launched = iteration;
+  h->sens = sens_projection;
+  drop_graphs(h);  // they hold the coordinate buffers of the old parity
+  return MH_OK;
+}
+
 // ---- entry points -------------------------------------------------------------------
 static int step_phase(MhHandle* h, int from, int to) {
   if (!h) return MH_ERR_INVALID;
@@ -1590,6 +1710,7 @@ int mahyco_apply_eos(MhHandle* h, int env) {
 
 int mahyco_compute_pression_moyenne(MhHandle* h) {
   if (!h) return MH_ERR_INVALID;
+  if (h->opt.sans_lagrange) return MH_OK;  // MahycoModule.cc:1711
   switch (h->m) {
     case 1: launch_pm_t<1>(h); break;
     case 2: launch_pm_t<2>(h); break;

@@ -2574,7 +2574,7 @@ __global__ __launch_bounds__(MH_BLOCK) void k_apply_eos(MhDev S, int env) {
   if (c >= S.g.nc) return;
   const uint8_t mk = S.mask[c];
   if (!((mk >> env) & 1)) return;
-  const double tnow = S.sc->t + S.sc->dt;
+  const double tnow = S.sc->t;  // between iterations: m_global_time of the last one
   if (__popc(mk) == 1) {
     double p, cs;
     if (S.env[env].fictif) {
@@ -2599,11 +2599,13 @@ __global__ __launch_bounds__(MH_BLOCK) void k_apply_eos(MhDev S, int env) {
 }
 
 // computePressionMoyenne (MahycoModule.cc:1708-1735) on multi-environment cells; the
-// environment sound speeds are those of applyEOS on the stored density and energy.
+// environment sound speeds are those applyEOS stores for the stored density and energy
+// (1 for a Fictif environment, FictifEOSService.cc:62).  Advection mode returns at once
+// (:1711): the host launches nothing then.
 template <int M>
 __global__ __launch_bounds__(MH_BLOCK) void k_pression_moyenne(MhDev S) {
   const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (c >= S.g.nc) return;
+  if (c >= S.g.nc || S.sans_lagrange) return;
   const uint8_t mk = S.mask[c];
   if (__popc(mk) == 1) return;
   const double opt = S.vitson_max ? 1. : 0.;
@@ -2611,8 +2613,8 @@ __global__ __launch_bounds__(MH_BLOCK) void k_pression_moyenne(MhDev S) {
   for (int kk = 0; kk < M; ++kk) {
     if (!((mk >> kk) & 1)) continue;
     const long o = kk * S.g.nc + c;
-    double p, cs;
-    eos(S.env[kk], S.rhok[o], S.ek[o], p, cs);
+    double p, cs = 1.;
+    if (!S.env[kk].fictif) eos(S.env[kk], S.rhok[o], S.ek[o], p, cs);
     ps += S.fk[o] * S.pk[o];
     cm = opt * ((cs < cm) ? cm : cs);
     cm += (1. - opt) * S.fk[o] * cs;

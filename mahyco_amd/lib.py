@@ -33,6 +33,13 @@ def load():
     lib.mahyco_create.argtypes = [C.POINTER(T.MhMesh), C.c_void_p, C.c_int, C.POINTER(T.MhOptions),
                                   C.POINTER(C.c_void_p)]
     lib.mahyco_create.restype = C.c_int
+    lib.mahyco_create_on_stream.argtypes = [C.POINTER(T.MhMesh), C.c_void_p, C.c_int, C.POINTER(T.MhOptions),
+                                            C.c_void_p, C.POINTER(C.c_void_p)]
+    lib.mahyco_create_on_stream.restype = C.c_int
+    lib.mahyco_set_field.argtypes = [H, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    lib.mahyco_set_field.restype = C.c_int
+    lib.mahyco_set_scalars.argtypes = [H, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]
+    lib.mahyco_set_scalars.restype = C.c_int
     lib.mahyco_destroy.argtypes = [H]
     lib.mahyco_destroy.restype = None
     for name in ("mahyco_cycle",):
@@ -50,7 +57,7 @@ def load():
     for name in ("mahyco_time", "mahyco_deltat"):
         getattr(lib, name).argtypes = [H]
         getattr(lib, name).restype = C.c_double
-    for name in ("mahyco_nb_cells", "mahyco_nb_nodes"):
+    for name in ("mahyco_nb_cells", "mahyco_nb_nodes", "mahyco_graph_captures"):
         getattr(lib, name).argtypes = [H]
         getattr(lib, name).restype = C.c_long
     lib.mahyco_get_field.argtypes = [H, C.c_int, C.c_int, C.c_int, C.c_void_p]
@@ -104,13 +111,18 @@ def load():
 class GpuCase:
     """One case resident on the GPU (the MahycoModule state of one mesh)."""
 
-    def __init__(self, case):
+    def __init__(self, case, stream=None):
+        """stream: a hipStream_t (as an int or c_void_p) the case's work is ordered on."""
         self.lib = load()
         self.case = case
         self._mesh, self._envs, self._opt = case.mesh_c(), case.envs_c(), case.options_c()
         h = C.c_void_p()
-        st = self.lib.mahyco_create(C.byref(self._mesh), self._envs, len(case.envs), C.byref(self._opt),
-                                    C.byref(h))
+        if stream is None:
+            st = self.lib.mahyco_create(C.byref(self._mesh), self._envs, len(case.envs), C.byref(self._opt),
+                                        C.byref(h))
+        else:
+            st = self.lib.mahyco_create_on_stream(C.byref(self._mesh), self._envs, len(case.envs),
+                                                  C.byref(self._opt), C.c_void_p(int(stream)), C.byref(h))
         if st != T.MH_OK:
             raise MahycoError(st, "mahyco_create")
         self.h = h
@@ -157,6 +169,10 @@ class GpuCase:
     def iteration(self):
         return self.lib.mahyco_iteration(self.h)
 
+    @property
+    def graph_captures(self):
+        return self.lib.mahyco_graph_captures(self.h)
+
     def get(self, field, env=-1, comp=0):
         n = self.nn if field >= 100 else self.nc
         out = np.empty(n, dtype=np.float64)
@@ -164,6 +180,21 @@ class GpuCase:
         if st != T.MH_OK:
             raise MahycoError(st, "mahyco_get_field")
         return out
+
+    def set(self, field, values, env=-1, comp=0):
+        """Writes a field of the state (mahyco_set_field)."""
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        n = self.nn if field >= 100 else self.nc
+        if a.shape != (n,):
+            raise ValueError(f"field {field}: expected {n} values, got {a.shape}")
+        st = self.lib.mahyco_set_field(self.h, field, env, comp, a.ctypes.data)
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_set_field")
+
+    def set_scalars(self, time, deltat, old_deltat, iteration, sens_projection):
+        st = self.lib.mahyco_set_scalars(self.h, time, deltat, old_deltat, iteration, sens_projection)
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_set_scalars")
 
     def history_enable(self, period=10, max_rows=4096):
         st = self.lib.mahyco_history_enable(self.h, period, max_rows)

@@ -89,11 +89,16 @@ class Served:
     iterations: int = 0
     slot: int = -1
     status: int = 0
+    live_at_admission: int = 0  # runs already on the GPU when this one was admitted
 
 
-def serve(reqs, budget=64e9, max_slots=64, step_cycles=6, graph=True):
-    """Continuous-batching request loop; times are wall-clock seconds from the first arrival."""
-    from .lib import Ensemble
+def serve(reqs, budget=64e9, max_slots=64, step_cycles=6, graph=True, on_done=None):
+    """Continuous-batching request loop; times are wall-clock seconds from the first arrival.
+
+    on_done(i, ensemble, slot), when given, is called for request i once it has finished and
+    before its slot is released (to read its fields).  A request whose footprint exceeds the
+    whole budget is refused with MH_ERR_NOMEM (iterations -1) instead of waiting forever."""
+    from .lib import Ensemble, footprint
     E = Ensemble(max_slots, budget)
     recs: List[Served] = [Served(a) for a, _ in reqs]
     queue = list(range(len(reqs)))  # in arrival order
@@ -107,12 +112,20 @@ def serve(reqs, budget=64e9, max_slots=64, step_cycles=6, graph=True):
             i = queue[0]
             slot = E.add(reqs[i][1])
             if slot is None:
+                if not live and footprint(reqs[i][1]) > budget:
+                    recs[i].admitted = recs[i].done = now()
+                    recs[i].status, recs[i].iterations = T.MH_ERR_NOMEM, -1
+                    queue.pop(0)
+                    continue
                 break
             recs[i].admitted = now()
             recs[i].slot = slot
+            recs[i].live_at_admission = len(live)
             live[slot] = i
             queue.pop(0)
         if not live:
+            if not queue:
+                break
             # idle until the next arrival
             wait = reqs[queue[0]][0] - now()
             if wait > 0:
@@ -124,6 +137,8 @@ def serve(reqs, budget=64e9, max_slots=64, step_cycles=6, graph=True):
             if st[slot] == T.MH_STOPPED or st[slot] < 0:
                 i = live.pop(slot)
                 recs[i].done = now()
+                if on_done is not None:
+                    on_done(i, E, slot)
                 recs[i].iterations = int(it[slot])
                 recs[i].status = int(st[slot])
                 if st[slot] < 0:

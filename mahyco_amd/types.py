@@ -56,6 +56,7 @@ MH_F_MATERIAU = 11
 MH_N_VELOCITY = 100
 MH_N_NODE_MASS = 101
 MH_N_COORD = 102
+MH_N_COORD_LAGRANGE = 103
 
 CELL_FIELDS = {
     "Density": MH_F_DENSITY, "Pressure": MH_F_PRESSURE, "InternalEnergy": MH_F_INTERNAL_ENERGY,

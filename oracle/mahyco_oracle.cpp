@@ -101,6 +101,7 @@ struct OracleCase {
 
   // ---------------------------------------------------------------- state
   std::vector<R3> node_coord, node_coord_0, velocity, velocity_n, force;
+  std::vector<R3> node_coord_lag;  // with remap: the Lagrangian positions before x <- x0
   std::vector<double> node_mass;
   std::vector<R3> cell_coord, face_coord, face_normal;
   std::vector<R3> cqs, cqs_n;            // [nc][nnpc]
@@ -1145,6 +1146,7 @@ int OracleCase::remap() {
     if (!opt.sans_lagrange) computeDualUremap(idir);  // withDualProjection (PrepareRemap.cc:211)
   }
   sens_projection = (sens_projection + 1) % dim;
+  node_coord_lag.swap(node_coord);  // kept for the state accessors (cqs holds what the loop needs)
   node_coord = node_coord_0;  // isEuler() == "Full"
   int st = remapVariables();
   if (st != MH_OK) return st;
@@ -2163,6 +2165,8 @@ int oracle_get_field(const OracleCase* hc, int field, int env, int comp_, double
       if (field == MH_N_VELOCITY) out[n] = comp(h->velocity[n], comp_);
       else if (field == MH_N_NODE_MASS) out[n] = h->node_mass[n];
       else if (field == MH_N_COORD) out[n] = comp(h->node_coord[n], comp_);
+      else if (field == MH_N_COORD_LAGRANGE)
+        out[n] = comp(h->opt.with_projection && !h->node_coord_lag.empty() ? h->node_coord_lag[n] : h->node_coord[n], comp_);
       else return MH_ERR_INVALID;
     }
     return MH_OK;
@@ -2190,6 +2194,107 @@ int oracle_get_field(const OracleCase* hc, int field, int env, int comp_, double
     else out[c] = h->member(env, c) ? h->ev(*v, env, c) : 0.;
   }
   return MH_OK;
+}
+
+// Writes one field of the state, with mahyco_set_field's meaning: env = -1 the cell value,
+// env = k environment k's value (partial value of a mixed cell, cell value of a cell of k
+// alone, nothing outside k).  MH_N_COORD_LAGRANGE re-forms the corner vectors the next
+// velocity update reads (MahycoModule.cc:1097-1162).
+int oracle_set_field(OracleCase* h, int field, int env, int comp_, const double* in) {
+  if (field >= 100) {
+    if (comp_ < 0 || comp_ >= h->dim) return MH_ERR_INVALID;
+    if (field == MH_N_VELOCITY) {
+      for (long n = 0; n < h->nn; ++n) compr(h->velocity[n], comp_) = in[n];
+      return MH_OK;
+    }
+    if (field != MH_N_COORD_LAGRANGE && !(field == MH_N_COORD && !h->opt.with_projection)) return MH_ERR_INVALID;
+    std::vector<R3>& x = h->opt.with_projection ? h->node_coord_lag : h->node_coord;
+    if (x.empty()) x = h->node_coord;
+    for (long n = 0; n < h->nn; ++n) compr(x[n], comp_) = in[n];
+    for (long c = 0; c < h->nc; ++c) {  // the corner vectors of computeGeometricValues at x
+      R3 coord[8];
+      for (int l = 0; l < h->nnpc; ++l) coord[l] = x[h->cell_nodes[c * h->nnpc + l]];
+      R3* q = &h->cqs[c * h->nnpc];
+      if (h->dim == 3) {
+        R3 fc[6];
+        fc[0] = 0.25 * (coord[0] + coord[3] + coord[2] + coord[1]);
+        fc[1] = 0.25 * (coord[0] + coord[4] + coord[7] + coord[3]);
+        fc[2] = 0.25 * (coord[0] + coord[1] + coord[5] + coord[4]);
+        fc[3] = 0.25 * (coord[4] + coord[5] + coord[6] + coord[7]);
+        fc[4] = 0.25 * (coord[1] + coord[2] + coord[6] + coord[5]);
+        fc[5] = 0.25 * (coord[2] + coord[3] + coord[7] + coord[6]);
+        h->computeCQs(coord, fc, q);
+      } else {
+        R3 npc[5];
+        coord[4] = coord[0];
+        for (int l = 0; l < 4; ++l) {
+          npc[l + 1].x = 0.5 * (coord[l + 1].y - coord[l].y);
+          npc[l + 1].y = 0.5 * (coord[l].x - coord[l + 1].x);
+          npc[l + 1].z = 0.;
+        }
+        npc[0] = npc[4];
+        for (int l = 0; l < 4; ++l) q[l] = npc[l + 1] + npc[l];
+      }
+    }
+    return MH_OK;
+  }
+  if (env < -1 || env >= h->m) return MH_ERR_INVALID;
+  if (field == MH_F_ENV_MASK) {
+    if (env != -1) return MH_ERR_INVALID;
+    for (long c = 0; c < h->nc; ++c) {
+      h->mask[c] = (uint8_t)(int)in[c];
+      h->nbenv[c] = (int8_t)__builtin_popcount(h->mask[c]);
+    }
+    return MH_OK;
+  }
+  MatVar* v = nullptr;
+  switch (field) {
+    case MH_F_DENSITY: v = &h->density; break;
+    case MH_F_PRESSURE: v = &h->pressure; break;
+    case MH_F_INTERNAL_ENERGY: v = &h->internal_energy; break;
+    case MH_F_SOUND_SPEED: v = &h->sound_speed; break;
+    case MH_F_PSEUDO_VISCOSITY: v = &h->pseudo; break;
+    case MH_F_CELL_MASS: v = &h->cell_mass; break;
+    case MH_F_FRACVOL: v = &h->fracvol; break;
+    case MH_F_MASS_FRACTION: v = &h->mass_fraction; break;
+    default: return MH_ERR_INVALID;
+  }
+  for (long c = 0; c < h->nc; ++c) {
+    if (env < 0) v->g[c] = in[c];
+    else if (h->member(env, c)) h->ev(*v, env, c) = in[c];
+  }
+  // the node mass the next velocity update divides by is sum M / 2^d (computeNodeMass,
+  // MahycoModule.cc:247-262, and remapVariables, RemapADIFinal.cc:299-306)
+  if (field == MH_F_CELL_MASS) h->computeNodeMass();
+  return MH_OK;
+}
+
+// m_global_time, m_global_deltat, m_global_old_deltat, m_global_iteration, m_sens_projection.
+void oracle_get_scalars(const OracleCase* h, double* t, double* dt, double* dt_old, int* it, int* sens) {
+  *t = h->global_time;
+  *dt = h->global_deltat;
+  *dt_old = h->old_deltat;
+  *it = h->global_iteration;
+  *sens = h->sens_projection;
+}
+void oracle_set_scalars(OracleCase* h, double t, double dt, double dt_old, int it, int sens) {
+  h->global_time = t;
+  h->global_deltat = dt;
+  h->global_old_deltat = dt_old;
+  h->old_deltat = dt_old;
+  h->global_iteration = it;
+  h->sens_projection = sens;
+  h->stopped = false;
+}
+
+// IEquationOfState::applyEOS(env) and MahycoModule::computePressionMoyenne as standalone calls
+// (between iterations: m_global_time is the last completed iteration's time).
+void oracle_apply_eos(OracleCase* h, int k) {
+  if (h->env[k].eos == MH_EOS_FICTIF) h->applyFictif(k);
+  else h->applyEOS(k);
+}
+void oracle_compute_pression_moyenne(OracleCase* h) {
+  if (!h->opt.sans_lagrange) h->computePressionMoyenne();  // :1711
 }
 
 int oracle_history(const OracleCase* h, double* out, int max_rows) {

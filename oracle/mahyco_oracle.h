@@ -39,6 +39,16 @@ long oracle_nb_nodes(const OracleCase* h);
  * Node fields: comp = 0,1,2 for vectors. */
 int oracle_get_field(const OracleCase* h, int field, int env, int comp, double* out);
 
+/* Writes one field of the state with mahyco_set_field's meaning (env = -1 cell value, env = k
+ * environment k's value); MH_N_COORD_LAGRANGE re-forms the corner vectors of the next step. */
+int oracle_set_field(OracleCase* h, int field, int env, int comp, const double* in);
+/* m_global_time, m_global_deltat, m_global_old_deltat, m_global_iteration, m_sens_projection. */
+void oracle_get_scalars(const OracleCase* h, double* t, double* dt, double* dt_old, int* it, int* sens);
+void oracle_set_scalars(OracleCase* h, double t, double dt, double dt_old, int it, int sens);
+/* IEquationOfState::applyEOS(env) / MahycoModule::computePressionMoyenne between iterations. */
+void oracle_apply_eos(OracleCase* h, int env);
+void oracle_compute_pression_moyenne(OracleCase* h);
+
 /* Time history rows (EnvSummation, every 10 iterations): per row
  * [time, then for env 0..m-1: CellMass avg, Pressure avg, Density vol-avg,
  * InternalEnergy mass-avg, nb_envcells]. Returns number of rows; fills up to max_rows. */

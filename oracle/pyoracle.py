@@ -47,6 +47,16 @@ def load():
     lib.oracle_get_field.restype = C.c_int
     lib.oracle_history.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
     lib.oracle_history.restype = C.c_int
+    lib.oracle_set_field.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    lib.oracle_set_field.restype = C.c_int
+    lib.oracle_get_scalars.argtypes = [C.c_void_p] + [C.c_void_p] * 5
+    lib.oracle_get_scalars.restype = None
+    lib.oracle_set_scalars.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]
+    lib.oracle_set_scalars.restype = None
+    lib.oracle_apply_eos.argtypes = [C.c_void_p, C.c_int]
+    lib.oracle_apply_eos.restype = None
+    lib.oracle_compute_pression_moyenne.argtypes = [C.c_void_p]
+    lib.oracle_compute_pression_moyenne.restype = None
     lib.oracle_set_threads.argtypes = [C.c_int]
     lib.oracle_set_pow_mode.argtypes = [C.c_int]
     _lib = lib
@@ -106,6 +116,28 @@ class OracleCase:
         if st != T.MH_OK:
             raise RuntimeError(f"oracle_get_field({field}) -> {st}")
         return out
+
+    def set(self, field, values, env=-1, comp=0):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        st = self.lib.oracle_set_field(self.h, field, env, comp, a.ctypes.data)
+        if st != T.MH_OK:
+            raise RuntimeError(f"oracle_set_field({field}) -> {st}")
+
+    def scalars(self):
+        """(time, deltat, old deltat, iteration, sens_projection)."""
+        t, dt, dto = C.c_double(), C.c_double(), C.c_double()
+        it, sens = C.c_int(), C.c_int()
+        self.lib.oracle_get_scalars(self.h, C.byref(t), C.byref(dt), C.byref(dto), C.byref(it), C.byref(sens))
+        return t.value, dt.value, dto.value, it.value, sens.value
+
+    def set_scalars(self, time, deltat, old_deltat, iteration, sens_projection):
+        self.lib.oracle_set_scalars(self.h, time, deltat, old_deltat, iteration, sens_projection)
+
+    def apply_eos(self, env):
+        self.lib.oracle_apply_eos(self.h, env)
+
+    def compute_pression_moyenne(self):
+        self.lib.oracle_compute_pression_moyenne(self.h)
 
     def history(self):
         rows = self.lib.oracle_history(self.h, None, 0)

@@ -10,6 +10,7 @@ time, time step and iteration as exact hex floats.  A GPU run that reproduces ev
 is bitwise identical to the oracle over the whole mesh.
 
     python tests/golden/make_oracle_full_size.py [bisod3d] [sedov3d] [served114] [served162] [bisod3d_csts] [rider512]
+                                                 [batch218] [batch144]
     (minutes on 8 threads)
 """
 import json
@@ -40,7 +41,16 @@ WORKLOADS = {
     # deformation, about 2e4 mixed cells) on a 512 x 512 mesh
     "bisod3d_csts": (lambda: _csts(cases.config2_bisod3d(256, 128, 128)), 100),
     "rider512": (lambda: _rider512(), 300),
+    # the smallest (64 x 64) and the largest (512 x 512) member of the 256-run batched ensemble
+    # (seed 20261202), 300 iterations each
+    "batch218": (lambda: _batch(218), 300),
+    "batch144": (lambda: _batch(144), 300),
 }
+
+
+def _batch(i):
+    from mahyco_amd import serve
+    return serve.batch_cases(256, seed=20261202)[i]
 
 
 def _csts(c):

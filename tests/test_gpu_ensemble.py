@@ -137,3 +137,95 @@ def test_admission_by_footprint():
     assert E.add(c) is None  # a third would exceed the budget
     assert E.used_bytes <= 2.5 * fp
     E.close()
+
+
+class _SlotView:
+    """One ensemble slot seen through get(field, env, comp), as GpuCase / OracleCase."""
+
+    def __init__(self, E, slot):
+        self.E, self.slot = E, slot
+
+    def get(self, field, env=-1, comp=0):
+        return self.E.get(self.slot, field, env, comp)
+
+
+def test_batched_workload_full_size():
+    """The batched workload at its size: 256 two-dimensional bi-material tubes (sides 64-512
+    drawn log-uniform, seed 20261202, 17.4 M cells) in one ensemble, 300 iterations each with
+    its own time step.  Every run reaches its horizon with finite positive density and
+    pressure; eight members (the smallest and the largest among them) are bitwise their solo
+    runs; the smallest and the largest match the oracle's digests over the whole mesh."""
+    import json
+    import os
+    from mahyco_amd import serve
+    from mahyco_amd.lib import Ensemble
+    from tests.helpers import summarize
+    batch = serve.batch_cases(256, seed=20261202)
+    sides = [c.n[0] for c in batch]
+    i_min, i_max = int(np.argmin(sides)), int(np.argmax(sides))
+    print(f"[ensemble] batched 256 runs, sides {min(sides)}..{max(sides)}", flush=True)
+    E = Ensemble(256, 200e9)
+    slots = [E.add(c, max_iterations=300) for c in batch]
+    assert all(s is not None for s in slots)
+    E.step(306, graph=True)
+    n, st, tm, it = E.poll()
+    assert all(st[s] == T.MH_STOPPED and it[s] == 300 for s in slots)
+    for s, c in zip(slots, batch):
+        for f in (T.MH_F_DENSITY, T.MH_F_PRESSURE):
+            v = E.get(s, f)
+            assert np.all(np.isfinite(v)) and np.all(v > 0), (c.name, f)
+    rng = np.random.default_rng(1)
+    picks = sorted({i_min, i_max, *rng.choice(len(batch), 6, replace=False).tolist()})
+    for i in picks:
+        c = batch[i]
+        print(f"[ensemble] solo {c.name}", flush=True)
+        ref = _alone(c, 300)
+        assert tm[slots[i]] == ref["t"], c.name
+        for f in (T.MH_F_DENSITY, T.MH_F_PRESSURE, T.MH_F_INTERNAL_ENERGY, T.MH_F_FRACVOL):
+            assert np.array_equal(E.get(slots[i], f), ref[f]), (c.name, f)
+        assert np.array_equal(E.get(slots[i], T.MH_N_VELOCITY, -1, 0), ref["u"]), c.name
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for i in (i_min, i_max):
+        with open(os.path.join(root, "tests", "golden", f"oracle_batch{i}_full.json")) as fh:
+            ref = json.load(fh)
+        assert float(tm[slots[i]]).hex() == ref["time"] and int(it[slots[i]]) == ref["iteration"]
+        got = summarize(_SlotView(E, slots[i]), batch[i])
+        bad = [k for k, r in ref["fields"].items() if got[k]["sha256"] != r["sha256"]]
+        assert not bad, (batch[i].name, bad)
+    E.close()
+
+
+def _small_stream(n=24, seed=99, rate=400.0):
+    """A short Poisson stream of small stiffened-gas runs (2D 24-64 cells a side, 3D 8-16)
+    arriving faster than they finish, so that admissions overlap running work."""
+    from mahyco_amd import serve
+    reqs = serve.serve_requests(n, seed=seed, rate=rate, t_end=0.02, lo2=24, hi2=64, lo3=8, hi3=16)
+    return reqs
+
+
+def test_served_stream_alone_equals_mixed():
+    """The continuous-batching serve loop: requests admitted while others run, every result
+    bitwise the same run solved alone (time, iteration and fields)."""
+    from mahyco_amd import serve
+    from mahyco_amd.lib import GpuCase
+    reqs = _small_stream()
+    got = {}
+
+    def keep(i, E, slot):
+        got[i] = {f: E.get(slot, f) for f in (T.MH_F_DENSITY, T.MH_F_PRESSURE, T.MH_F_INTERNAL_ENERGY,
+                                              T.MH_F_FRACVOL)}
+        got[i]["u"] = E.get(slot, T.MH_N_VELOCITY, -1, 0)
+
+    print(f"[serve] {len(reqs)} requests", flush=True)
+    recs = serve.serve(reqs, budget=8e9, max_slots=8, on_done=keep)
+    assert all(r.status == T.MH_STOPPED for r in recs), [r.status for r in recs]
+    assert sum(r.live_at_admission > 0 for r in recs) >= len(recs) // 4  # admissions overlapped running work
+    for i, (_, c) in enumerate(reqs):
+        g = GpuCase(c)
+        while g.run(60, graph=6) == T.MH_OK:
+            pass
+        assert g.iteration == recs[i].iterations, c.name
+        for f in (T.MH_F_DENSITY, T.MH_F_PRESSURE, T.MH_F_INTERNAL_ENERGY, T.MH_F_FRACVOL):
+            assert np.array_equal(g.get(f), got[i][f], equal_nan=True), (c.name, f)
+        assert np.array_equal(g.get(T.MH_N_VELOCITY, -1, 0), got[i]["u"]), c.name
+        g.close()

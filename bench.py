@@ -57,7 +57,7 @@ def kernel_bytes(kind, nc, nn, m, d):
 def measured_traffic(kernel, args):
     """HBM bytes per launch of `kernel` at this mesh from the FETCH_SIZE + WRITE_SIZE profile
     summary kept in profiles/ (scaled by a plain copy's counters), or None."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_bisod3d.json")
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{args.workload}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -99,22 +99,28 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--nx", type=int, default=256)
-    ap.add_argument("--ny", type=int, default=128)
-    ap.add_argument("--nz", type=int, default=128)
+    ap.add_argument("--workload", choices=["bisod3d", "sedov3d"], default="bisod3d",
+                    help="bisod3d: 3D bi-material tube 256x128x128, superbee (default); sedov3d: 3D Sedov "
+                         "blast 192^3 on [0,1.2]^3, one material, minmod")
+    ap.add_argument("--nx", type=int, default=None)
+    ap.add_argument("--ny", type=int, default=None)
+    ap.add_argument("--nz", type=int, default=None)
     ap.add_argument("--graph", type=int, default=6, help="cycles per hipGraph (0: eager launches)")
     ap.add_argument("--profile-steps", type=int, default=12)
     ap.add_argument("--cpu-cycles", type=int, default=20)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cpu-single-cycles", type=int, default=4, help="iterations timed on one host thread")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default kernel variants (the autotuner otherwise runs in the warmup)")
+    ap.add_argument("--tune-cycles", type=int, default=12, help="timed iterations per autotuner candidate")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (scale reference for HBM counters)")
     ap.add_argument("--mode", choices=["plain", "batched", "serve"], default="plain",
                     help="plain: single-mesh cycle (default); batched: 256 uneven 2D meshes, 300 "
                          "iterations each; serve: Poisson stream with continuous batching")
     ap.add_argument("--runs", type=int, default=256)
-    ap.add_argument("--rate", type=float, default=8.0, help="serve: arrivals per second")
+    ap.add_argument("--rate", type=float, default=16.0, help="serve: arrivals per second")
     ap.add_argument("--seed", type=int, default=20261202)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -131,7 +137,13 @@ def main():
     if args.mode == "serve":
         return bench_serve(args)
 
-    make = lambda: cases.config2_bisod3d(args.nx, args.ny, args.nz)
+    if args.workload == "sedov3d":
+        n = args.nx or 192
+        args.nx = args.ny = args.nz = n
+        make = lambda: cases.config3_sedov3d(n)
+    else:
+        args.nx, args.ny, args.nz = args.nx or 256, args.ny or 128, args.nz or 128
+        make = lambda: cases.config2_bisod3d(args.nx, args.ny, args.nz)
     case = make()
     nc, nn, m, d = case.nb_cells, case.nb_nodes, len(case.envs), case.dim
     print(f"[bench] {case.name}: cells={nc} nodes={nn} envs={m}", file=sys.stderr, flush=True)
@@ -144,6 +156,10 @@ def main():
         period = 6 if d == 3 else 2
         gp = -(-args.graph // period) * period
         warm = max(gp, -(-args.warmup // gp) * gp)
+    variant, tuned_ms = None, None
+    if not args.no_autotune:  # part of the warmup: the candidates run the case's own iterations
+        variant, tuned_ms = g.autotune(args.tune_cycles)
+        print(f"[bench] autotuned variant {variant} ({tuned_ms:.3f} ms/iteration)", file=sys.stderr, flush=True)
     g.queue(warm, graph=args.graph)
     st = g.synchronize()
     assert st == T.MH_OK, st
@@ -187,6 +203,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "warmup_run": warm,
+        "autotuned_variant": variant,
         "graph_captures_in_timed_region": captures_timed,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
@@ -194,8 +211,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded initial condition of the configuration)",
-        "config": {"workload": "bisod3d_lagrange_adi_remap", "mesh": [args.nx, args.ny, args.nz],
-                   "cells": nc, "environments": m, "limiter": "superbee", "order": 2, "cfl": 0.3,
+        "config": {"workload": f"{args.workload}_lagrange_adi_remap", "mesh": [args.nx, args.ny, args.nz],
+                   "cells": nc, "environments": m,
+                   "limiter": "minmod" if args.workload == "sedov3d" else "superbee", "order": 2, "cfl": 0.3,
                    "graph_cycles": args.graph},
         "roofline": roof,
         "roofline_cycle": {"alg_bytes_per_cycle": alg_cycle, "achieved_GBs": cyc_gbs,
@@ -206,8 +224,7 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
     if not args.no_cpu:
-        small = lambda: cases.config2_bisod3d(args.nx, args.ny, args.nz)
-        out["cpu_baseline"] = cpu_baseline(small, args.cpu_cycles, args.cpu_threads, args.cpu_single_cycles)
+        out["cpu_baseline"] = cpu_baseline(make, args.cpu_cycles, args.cpu_threads, args.cpu_single_cycles)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
@@ -233,46 +250,61 @@ def copy_calibration(mib):
     print(f"[bench] copy calibration: {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
 
 
+def _pct(v, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(v, dtype=float), q))
+
+
 def bench_batched(args):
     """256 independent 2D bi-material tubes, sides log-uniform in [64, 512], 300 iterations each:
-    one ensemble (batched launches, hipGraph) against back-to-back single-mesh solves."""
+    one ensemble (persistent batched launches, hipGraph, autotuned grid) against the same 256
+    runs solved back to back on the single-mesh path, both timed end to end (case creation,
+    graph capture, iterations, release)."""
     from mahyco_amd import serve
     batch = serve.batch_cases(args.runs, seed=args.seed)
     cells = sum(c.nb_cells for c in batch)
     print(f"[bench] batched: {len(batch)} runs, {cells} cells", file=sys.stderr, flush=True)
-    t_batch, iters, st = serve.run_batch(batch, graph=True)
+    t_batch, iters, stats, done, bpc = serve.run_batch(batch, graph=True)
     assert all(i == batch[0].max_cycles for i in iters), "a run did not reach its horizon"
-    sub = batch[: max(8, len(batch) // 8)]
-    t_b2b_sub, lat = serve.run_back_to_back(sub)
-    t_b2b = t_b2b_sub * len(batch) / len(sub)
+    print(f"[bench] batched {t_batch:.2f} s; back to back ...", file=sys.stderr, flush=True)
+    t_b2b, done_b2b = serve.run_back_to_back(batch)
     out = {"metric": "ensemble runs/s (batched) vs back-to-back single-mesh solves",
            "value": len(batch) / t_batch, "unit": "runs/s", "n_gpus": 1, "higher_is_better": True,
            "dtype": "f64", "data": f"synthetic, seed {args.seed}",
            "config": {"workload": "batched_2d_bimaterial", "runs": len(batch), "cells": cells,
                       "iterations": batch[0].max_cycles},
+           "basis": "end to end for both: case creation, graph capture, every iteration, release",
            "batched_s": t_batch, "cell_cycles_per_s": cells * batch[0].max_cycles / t_batch,
-           "back_to_back_runs_per_s": len(sub) / t_b2b_sub,
-           "back_to_back_sample": f"{len(sub)} of {len(batch)} runs, {t_b2b_sub:.2f} s",
+           "tuned_blocks_per_cu": bpc,
+           "back_to_back_s": t_b2b, "back_to_back_runs_per_s": len(batch) / t_b2b,
            "speedup_vs_back_to_back": t_b2b / t_batch,
-           "p50_s": t_batch, "p99_s": t_batch}
+           "p50_s": _pct(done, 50), "p99_s": _pct(done, 99),
+           "back_to_back_p50_s": _pct(done_b2b, 50), "back_to_back_p99_s": _pct(done_b2b, 99)}
     print(json.dumps(out))
 
 
 def bench_serve(args):
-    """Poisson stream of 2D (128^2..1024^2) and 3D (32^3..160^3) stiffened-gas runs to t=0.1,
-    admitted by HBM footprint into a continuously batched ensemble."""
+    """Poisson stream of 2D (128^2..1024^2) and 3D (32^3..160^3) stiffened-gas runs to t=0.1 at
+    an offered rate above what one GPU sustains, served by continuous batching (admission by HBM
+    footprint into a device-resident ensemble) and, on the same stream, one run at a time on
+    the single-mesh path."""
     from mahyco_amd import serve
     from mahyco_amd import types as T
     reqs = serve.serve_requests(args.runs, seed=args.seed, rate=args.rate)
     print(f"[bench] serve: {len(reqs)} requests at {args.rate}/s", file=sys.stderr, flush=True)
     recs = serve.serve(reqs, budget=96e9, max_slots=64)
     stats = serve.latency_stats(recs)
+    print(f"[bench] served {stats['runs_per_s']:.2f} runs/s; back to back ...", file=sys.stderr, flush=True)
+    b2b = serve.latency_stats(serve.serve_back_to_back(reqs))
     out = {"metric": "served runs/s, p50/p99 time-to-solution", "value": stats["runs_per_s"],
            "unit": "runs/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
            "data": f"synthetic Poisson stream, seed {args.seed}, rate {args.rate}/s",
            "config": {"workload": "served_stream_2d3d_stiffened", "runs": len(reqs), "t_end": 0.1,
-                      "hbm_budget_GB": 96},
+                      "hbm_budget_GB": 96, "offered_rate_per_s": args.rate},
            "p50_s": stats["p50_s"], "p99_s": stats["p99_s"], "span_s": stats["span_s"],
+           "overlapped_admissions": sum(r.live_at_admission > 0 for r in recs),
+           "back_to_back_runs_per_s": b2b["runs_per_s"], "back_to_back_p50_s": b2b["p50_s"],
+           "back_to_back_p99_s": b2b["p99_s"], "speedup_vs_back_to_back": stats["runs_per_s"] / b2b["runs_per_s"],
            "failed": [{"run": reqs[i][1].name, "status": T.STATUS_NAMES.get(r.status, r.status)}
                       for i, r in enumerate(recs) if r.iterations < 0]}
     print(json.dumps(out))

@@ -114,6 +114,10 @@ int mahyco_compute_delta_t(MhHandle* h);
  * StiffenedGasEOSService.cc:43-63): pressure and sound speed of environment env from its
  * density and internal energy.  Also run inside mahyco_remap / the Lagrange-only cycle. */
 int mahyco_apply_eos(MhHandle* h, int env);
+/* IEquationOfState::applyOneCellEOS(env, EnvCell) (IEquationOfState.h:50-52,
+ * PerfectGasEOSService.cc:71-90, StiffenedGasEOSService.cc:67-83, FictifEOSService.cc:80): the same for one cell of the
+ * environment (cell index i + nx (j + ny k)); nothing for a cell outside it. */
+int mahyco_apply_one_cell_eos(MhHandle* h, int env, long cell);
 /* MahycoModule::computePressionMoyenne (MahycoModule.cc:1708-1735): cell pressure and sound
  * speed of multi-environment cells.  Also run inside mahyco_remap / the Lagrange-only cycle. */
 int mahyco_compute_pression_moyenne(MhHandle* h);
@@ -174,8 +178,32 @@ int mahyco_ensemble_remove(MhEnsemble* e, int slot);
 int mahyco_ensemble_step(MhEnsemble* e, int ncycles, int use_graph);
 int mahyco_ensemble_synchronize(MhEnsemble* e);
 int mahyco_ensemble_poll(MhEnsemble* e, int* status, double* time, int* iteration);
+/* The same in two halves, so that the host can queue more iterations in between: _poll_async
+ * enqueues a copy of every slot's scalars behind the queued work (pinned host memory, no wait);
+ * _poll_result waits for that copy only and fills the arrays as mahyco_ensemble_poll does. */
+int mahyco_ensemble_poll_async(MhEnsemble* e);
+int mahyco_ensemble_poll_result(MhEnsemble* e, int* status, double* time, int* iteration);
 int mahyco_ensemble_get_field(MhEnsemble* e, int slot, int field, int env, int comp, double* out);
 double mahyco_ensemble_used_bytes(MhEnsemble* e);
+/* Graphs the ensemble has captured (a case joining or leaving an existing batch group changes
+ * only device-side work lists, so it captures nothing). */
+long mahyco_ensemble_graph_captures(MhEnsemble* e);
+/* Autotuner of the ensemble's launch shape (no reference counterpart): times `cycles`
+ * iterations of the ensemble's own cases (hipGraph replays) with each candidate persistent grid
+ * (2, 4, 8, 16 blocks per compute unit) and keeps the fastest for every batch group, and
+ * for later groups of the same kind in this process.  The iterations count as the cases' own
+ * (results do not depend on the launch shape).  Returns the choice and its time per iteration. */
+int mahyco_ensemble_autotune(MhEnsemble* e, int cycles, int* blocks_per_cu_out, double* ms_per_cycle_out);
+
+/* Autotuner of a single case's kernel variants and tile shapes (no reference counterpart):
+ * times `cycles` iterations of the case itself per candidate, one coordinate at a time: the
+ * sweeps (row segments and LDS tiles, or one thread per item), then in 3D the node kernel
+ * (flat, or z-marching with 4, 8 or 16 node planes per block) and the cell kernel (flat, or
+ * LDS-staged with 4, 8 or 16 cell planes per block), and keeps the fastest, also for later
+ * cases of the same dimensions and options in this process.  variant = node + 4 cell + 16
+ * sweep with those codes.  The iterations count as the case's own (every variant gives the same
+ * bits); tuning stops early at the final time or an error. */
+int mahyco_autotune(MhHandle* h, int cycles, int* variant_out, double* ms_per_cycle_out);
 
 /* Kernel timing: when enabled, every launch is bracketed by HIP events on the
  * handle's stream.  mahyco_profile_read fills up to `max` entries of kernel

@@ -25,14 +25,39 @@ def hipcc():
     raise RuntimeError("hipcc not found (ROCm is required to build the HIP library)")
 
 
-def build(verbose: bool = False) -> str:
+# translation units: (source, extra flags, object); the sweep unit is compiled once per
+# environment count so that the units build in parallel
+UNITS = [("mahyco.hip", [], "mahyco.o"), ("mh_lag.hip", [], "mh_lag.o")] + [
+    ("mh_sweep.hip", [f"-DMH_TU_M={m}"], f"mh_sweep_m{m}.o") for m in (1, 2, 3, 4)]
+
+
+def build(verbose: bool = False, jobs: int = 0) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
-    src = os.path.join(CSRC, "mahyco.hip")
+    cc = hipcc()
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    cmds = []
+    for src, extra, obj in UNITS:
+        cmds.append([cc] + compile_flags + extra + ["-c", "-o", os.path.join(OUT_DIR, obj),
+                                                    os.path.join(CSRC, src)])
+    jobs = jobs or min(len(cmds), os.cpu_count() or 1)
+    procs, failed = [], []
+    pending = list(cmds)
+    while pending or procs:
+        while pending and len(procs) < jobs:
+            cmd = pending.pop(0)
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            procs.append((cmd, subprocess.Popen(cmd)))
+        cmd, p = procs.pop(0)
+        if p.wait() != 0:
+            failed.append(cmd)
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
     tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp, src]
+    link = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [os.path.join(OUT_DIR, o) for _, _, o in UNITS]
     if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
+        print(" ".join(link), flush=True)
+    subprocess.run(link, check=True)
     os.replace(tmp, LIB)
     build_host(verbose)
     return LIB

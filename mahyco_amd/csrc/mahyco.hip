@@ -9,10 +9,12 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/mahyco.h"
+#include "mh_host.h"
 #include "mh_kernels.h"
 #include "mh_output.h"
 
@@ -23,84 +25,21 @@ const char* kKernelNames[] = {"k_node_lag",     "k_cell_lag",     "k_sweep_cell_
                               "k_sweep_cell_z", "k_sweep_node_x", "k_sweep_node_y",  "k_sweep_node_z",
                               "k_node_velocity", "k_finalize",    "k_dt",            "k_hcell",
                               "k_node_fwd"};
-enum {
-  KN_NODE = 0,
-  KN_CELL,
-  KN_SWEEP_CELL,  // + direction
-  KN_SWEEP_NODE = KN_SWEEP_CELL + 3,
-  KN_NODE_VEL = KN_SWEEP_NODE + 3,
-  KN_FINAL,
-  KN_DT,
-  KN_HCELL,  // cell widths of the order-3 remap
-  KN_NODE_FWD,  // CSTS forward velocity step before the remap
-  KN_COUNT
-};
-
-// sweep variants: one thread per cell / node; row segments along x and LDS tiles along y / z
-enum { MH_SWEEP_FLAT = 0, MH_SWEEP_TILE = 1 };
-
-struct ProfRec {
-  int kind;
-  hipEvent_t a, b;
-};
 
 }  // namespace
 
-struct MhEnsemble;
-struct MhHandle {
-  MhMesh mesh;
-  MhOptions opt;
-  MhEnvironment env[MH_MAX_ENV];
-  int m = 1;
-  MhGrid g;
-  MhDev dev;
-  hipStream_t stream = nullptr;
-  std::vector<void*> allocs;
-  std::vector<size_t> alloc_bytes;  // size of each allocation (checkpoints)
-  double* xl[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
-  double *Ua = nullptr, *Ub = nullptr, *dflux = nullptr, *Ud = nullptr, *Pa = nullptr, *Pb = nullptr;
-  double* u0[3] = {nullptr, nullptr, nullptr};  // advection mode: prescribed velocity
-  double* rev_cos = nullptr;                     // advection mode: reverse factor per iteration
-  long launched = 0;  // iterations queued by the host
-  int sens = 0;       // m_sens_projection of the next iteration
-  int nblk_c = 0, nblk_n = 0;
-  int phase = 0;  // entry-point order check
-  bool prof = false;
-  std::vector<ProfRec> prof_pending;
-  double prof_ms[KN_COUNT] = {0};
-  int prof_cnt[KN_COUNT] = {0};
-  std::map<long, hipGraphExec_t> graphs;  // key: (cycles << 8) | (parity << 4) | sens
-  std::vector<hipGraph_t> graph_defs;
-  long graph_captures = 0;  // graphs captured and instantiated so far
-  bool owns_stream = true;
-  bool node_tiled = true;   // 3D: z-marching node kernel (MAHYCO_NODE_KERNEL=flat selects the flat one)
-  int sweep_mode = MH_SWEEP_TILE;  // MAHYCO_SWEEP=flat selects one thread per cell / node
-  std::vector<const mh_output::VarSpec*> ens_vars;  // EnSight variables of this case's outputs
-  std::vector<double> ens_times;                    // times of the EnSight steps written
-  MhScalars* ext_sc = nullptr;  // scalars living in an ensemble's array
-  size_t bytes = 0;             // device bytes of the case
-};
+
 
 namespace {
 
-int hip_check(hipError_t e, const char* what) {
-  if (e != hipSuccess) {
-    fprintf(stderr, "mahyco: %s failed: %s\n", what, hipGetErrorString(e));
-    return MH_ERR_DEVICE;
-  }
-  return MH_OK;
-}
-#define HIPCHK(call)                                     \
-  do {                                                   \
-    int _st = hip_check((call), #call);                  \
-    if (_st != MH_OK) return _st;                        \
-  } while (0)
 
 template <class T>
 int dalloc(MhHandle* h, T** p, size_t n) {
   void* q = nullptr;
   if (n == 0) n = 1;
-  hipError_t e = hipMalloc(&q, n * sizeof(T));
+  // a case of an ensemble allocates in the order of the ensemble's stream, so that admitting
+  // or retiring a case does not wait for the work in flight
+  hipError_t e = h->async_alloc ? hipMallocAsync(&q, n * sizeof(T), h->stream) : hipMalloc(&q, n * sizeof(T));
   if (e != hipSuccess) {
     fprintf(stderr, "mahyco: hipMalloc(%zu bytes) failed: %s\n", n * sizeof(T), hipGetErrorString(e));
     return MH_ERR_NOMEM;
@@ -500,149 +439,6 @@ int host_init(MhHandle* h, HostInit& I) {
   return MH_OK;
 }
 
-void prof_begin(MhHandle* h, ProfRec& r, int kind) {
-  r.kind = kind;
-  (void)hipEventCreate(&r.a);
-  (void)hipEventCreate(&r.b);
-  (void)hipEventRecord(r.a, h->stream);
-}
-void prof_end(MhHandle* h, ProfRec& r) {
-  (void)hipEventRecord(r.b, h->stream);
-  h->prof_pending.push_back(r);
-}
-
-// Runs the statement with LIM, a compile-time limiter id: 0 minmod, 1 superbee, 2 vanLeer,
-// 3 none (first order, or an id without a classical limiter: the slopes are 0).
-#define MH_LIMSW(h, ...)                                                  \
-  do {                                                                    \
-    const int _lim = (h)->dev.order2 ? (h)->dev.limiter : 3;              \
-    if (_lim == 0) { constexpr int LIM = 0; __VA_ARGS__; }                \
-    else if (_lim == 1) { constexpr int LIM = 1; __VA_ARGS__; }           \
-    else if (_lim == 2) { constexpr int LIM = 2; __VA_ARGS__; }           \
-    else { constexpr int LIM = 3; __VA_ARGS__; }                          \
-  } while (0)
-
-#define LAUNCH(h, kind, kern, grid, block, ...)                          \
-  do {                                                                   \
-    ProfRec _r;                                                          \
-    if ((h)->prof) prof_begin((h), _r, (kind));                          \
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, (h)->stream, __VA_ARGS__); \
-    if ((h)->prof) prof_end((h), _r);                                    \
-  } while (0)
-
-// a dual (node) sweep kernel: none in advection mode (withDualProjection, PrepareRemap.cc:211)
-#define LAUNCH_DUAL(h, ...)                           \
-  do {                                                \
-    if (!(h)->dev.sans_lagrange) LAUNCH(h, __VA_ARGS__); \
-  } while (0)
-
-// blocks of the row-segment x sweep (one wavefront per MH_ROW_OUT cells of a row), a multiple of 8
-int row_blocks(MhHandle* h) {
-  const long waves = (long)h->g.ny * h->g.nz * ((h->g.nx + MH_ROW_OUT - 1) / MH_ROW_OUT);
-  const long wpb = MH_BLOCK / 64;
-  return (int)(((waves + wpb - 1) / wpb + 7) / 8 * 8);
-}
-
-// blocks of the row-segment dual x sweep (one wavefront per MH_ROW_OUT nodes of a node row), a multiple of 8
-int node_row_blocks(MhHandle* h) {
-  const long waves = (long)h->g.nny * h->g.nnz * ((h->g.nnx + MH_ROW_OUT - 1) / MH_ROW_OUT);
-  const long wpb = MH_BLOCK / 64;
-  return (int)(((waves + wpb - 1) / wpb + 7) / 8 * 8);
-}
-
-// blocks of the LDS-tiled y / z sweep (64 cells along x by MH_TILE_L along D), a multiple of 8
-// k_sweep_node_z: one block per (row segment of an xy node plane, z plane)
-int node_z_blocks(MhHandle* h) {
-  const long segs = ((long)h->g.nnx * h->g.nny + MH_BLOCK - 1) / MH_BLOCK;
-  return (int)((segs * h->g.nnz + 7) / 8 * 8);
-}
-int tile_blocks(MhHandle* h, int D) {
-  const long na = D == 1 ? h->g.ny : h->g.nz;
-  const long nt = D == 1 ? (h->g.dim == 3 ? h->g.nz : 1) : h->g.ny;
-  const long tiles = (long)((h->g.nx + 63) / 64) * nt * ((na + MH_TILE_L - 1) / MH_TILE_L);
-  return (int)((tiles + 7) / 8 * 8);
-}
-
-// cell and dual parts of a y or z sweep in the selected variant (the dual part is one
-// thread per node in both)
-template <int DIM, int D, int M>
-void launch_sweep_yz(MhHandle* h, int sel, int first, int last) {
-  const MhDev& S = h->dev;
-  if (h->sweep_mode == MH_SWEEP_FLAT)
-    MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell<DIM, D, M, LIM>), h->nblk_c, MH_BLOCK, S, sel, first));
-  else
-    MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell_tile<DIM, D, M, LIM>), tile_blocks(h, D), MH_BLOCK, S, sel, first));
-  if constexpr (DIM == 3 && D == 2) {
-    if (h->sweep_mode != MH_SWEEP_FLAT) {  // z dual sweep: blocks walk z fastest
-      MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE + 2, (k_sweep_node_z<LIM>), node_z_blocks(h), MH_BLOCK, S, sel, first, last));
-      return;
-    }
-  }
-  MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE + D, (k_sweep_node<DIM, D, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
-}
-
-template <int DIM, int M>
-void launch_sweeps_t(MhHandle* h) {
-  const MhDev& S = h->dev;
-  const int dim = DIM;
-  int dirs[3], nd = 0;
-  for (int i = 0; i < dim; ++i) {
-    const int idir = (i + h->sens) % dim;
-    const int ncell = idir == 0 ? h->g.nx : idir == 1 ? h->g.ny : h->g.nz;
-    if (ncell == 1) continue;  // RemapADIService.cc:27
-    dirs[nd++] = idir;
-  }
-  const bool o3 = S.order2 == 2;
-  // pente-borne or generalised limiters: one thread per cell and node, limiter read at run time
-  const bool gen = !o3 && (S.pb || (S.order2 && S.limiter >= MH_MINMOD_G));
-  if (S.csts) LAUNCH(h, KN_NODE_FWD, (k_node_fwd<DIM>), h->nblk_n, MH_BLOCK, S);  // PrepareRemap.cc:210
-  if (S.hc && nd > 0) LAUNCH(h, KN_HCELL, (k_hcell<DIM>), h->nblk_c, MH_BLOCK, S);
-  for (int s = 0; s < nd; ++s) {
-    const int first = s == 0, last = (s == nd - 1) && !first, sel = s & 1;
-    const int D = dirs[s];
-    const int mode = h->sweep_mode;
-    if (gen) {
-      if (D == 0) {
-        LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell_gen<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-        LAUNCH_DUAL(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0, MH_LIM_ANY>), h->nblk_n, MH_BLOCK, S, sel, first, last);
-      } else if (D == 1) {
-        LAUNCH(h, KN_SWEEP_CELL + 1, (k_sweep_cell_gen<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-        LAUNCH_DUAL(h, KN_SWEEP_NODE + 1, (k_sweep_node<DIM, 1, MH_LIM_ANY>), h->nblk_n, MH_BLOCK, S, sel, first, last);
-      } else if constexpr (DIM == 3) {
-        LAUNCH(h, KN_SWEEP_CELL + 2, (k_sweep_cell_gen<3, 2, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-        LAUNCH_DUAL(h, KN_SWEEP_NODE + 2, (k_sweep_node<3, 2, MH_LIM_ANY>), h->nblk_n, MH_BLOCK, S, sel, first, last);
-      }
-    } else if (o3) {  // order 3: one thread per cell for the cell part, node part as selected
-      if (D == 0) LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell3<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-      else if (D == 1) LAUNCH(h, KN_SWEEP_CELL + 1, (k_sweep_cell3<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-      else if constexpr (DIM == 3) LAUNCH(h, KN_SWEEP_CELL + 2, (k_sweep_cell3<3, 2, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-      if (D == 0 && mode != MH_SWEEP_FLAT)
-        MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE, (k_sweep_node_row<DIM, LIM>), node_row_blocks(h), MH_BLOCK, S, sel, first, last));
-      else if (D == 0)
-        MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
-      else if (D == 1)
-        MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE + 1, (k_sweep_node<DIM, 1, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
-      else if constexpr (DIM == 3)
-        MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE + 2, (k_sweep_node<3, 2, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
-    } else if (D == 0) {
-      if (mode == MH_SWEEP_FLAT) {
-        MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, 0, M, LIM>), h->nblk_c, MH_BLOCK, S, sel, first));
-        MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
-      } else {
-        MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell_row<DIM, M, LIM>), row_blocks(h), MH_BLOCK, S, sel, first));
-        MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE, (k_sweep_node_row<DIM, LIM>), node_row_blocks(h), MH_BLOCK, S, sel, first, last));
-      }
-    } else if (D == 1) {
-      launch_sweep_yz<DIM, 1, M>(h, sel, first, last);
-    } else {
-      if constexpr (DIM == 3) launch_sweep_yz<3, 2, M>(h, sel, first, last);
-    }
-  }
-  if (nd <= 1 && !S.sans_lagrange) LAUNCH(h, KN_NODE_VEL, (k_node_velocity<DIM>), h->nblk_n, MH_BLOCK, S, nd == 0 ? 1 : 0);
-  LAUNCH(h, KN_FINAL, (k_finalize<DIM, M>), h->nblk_c, MH_BLOCK, S, nd & 1);
-  h->sens = (h->sens + 1) % dim;
-}
-
 void set_parity(MhHandle* h) {
   const int par = (int)(h->launched & 1);
   for (int d = 0; d < 3; ++d) {
@@ -651,24 +447,6 @@ void set_parity(MhHandle* h) {
   }
 }
 
-template <int DIM, int M>
-void launch_node_t(MhHandle* h) {
-  if (h->dev.sans_lagrange) {
-    LAUNCH(h, KN_NODE, (k_node_adv<DIM>), h->nblk_n, MH_BLOCK, h->dev);
-  } else if (DIM == 3 && h->node_tiled) {
-    const dim3 grid((h->g.nnx + MH_NTX - 1) / MH_NTX, (h->g.nny + MH_NTY - 1) / MH_NTY,
-                    (h->g.nnz + MH_NT_KC - 1) / MH_NT_KC);
-    LAUNCH(h, KN_NODE, k_node_lag3_tiled, grid, MH_BLOCK, h->dev);
-  } else {
-    LAUNCH(h, KN_NODE, (k_node_lag<DIM>), h->nblk_n, MH_BLOCK, h->dev);
-  }
-}
-template <int DIM, int M>
-void launch_cell_t(MhHandle* h) {
-  if (h->dev.lc != MH_L_RACINE_CUBIQUE_VOLUME) LAUNCH(h, KN_CELL, (k_cell_length<DIM>), h->nblk_c, MH_BLOCK, h->dev);
-  if (h->dev.csts) LAUNCH(h, KN_CELL, (k_cell_lag_csts<DIM, M>), h->nblk_c, MH_BLOCK, h->dev);
-  else LAUNCH(h, KN_CELL, (k_cell_lag<DIM, M>), h->nblk_c, MH_BLOCK, h->dev);
-}
 #define MH_DISPATCH(fn, h)                                      \
   do {                                                          \
     const int _d = (h)->g.dim, _m = (h)->m;                     \
@@ -685,7 +463,7 @@ void launch_cell_t(MhHandle* h) {
     }                                                           \
   } while (0)
 
-void launch_dt(MhHandle* h) { LAUNCH(h, KN_DT, k_dt, 1, MH_DT_BLOCK, h->dev); }
+void launch_dt(MhHandle* h) { LAUNCH(h, KN_DT, k_dt<>, 1, MH_DT_BLOCK, h->dev); }
 
 // Phase functions shared by mahyco_cycle and the per-entry-point calls.
 void phase_node(MhHandle* h) {
@@ -761,9 +539,8 @@ int validate(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const Mh
 
 }  // namespace
 
-template <int M>
-static void launch_eos_t(MhHandle* h, int env) {
-  hipLaunchKernelGGL(k_apply_eos<M>, dim3(h->nblk_c), dim3(MH_BLOCK), 0, h->stream, h->dev, env);
+static void launch_eos(MhHandle* h, int env) {
+  hipLaunchKernelGGL(k_apply_eos<>, dim3(h->nblk_c), dim3(MH_BLOCK), 0, h->stream, h->dev, env);
 }
 template <int M>
 static void launch_pm_t(MhHandle* h) {
@@ -804,58 +581,8 @@ bool ens_gen(const MhHandle* h) {
   return S.order2 != 2 && (S.pb || (S.order2 && S.limiter >= MH_MINMOD_G));
 }
 
-struct EnsGroup {
-  int dim = 0, m = 0;
-  bool o3 = false;   // order-3 remap
-  bool gen = false;  // pente-borne or generalised limiters
-  bool csts = false; // schema-csts
-  bool adv = false;  // advection mode (sans-lagrange): prescribed node velocity, no dual sweeps
-  bool lc = false;   // characteristic length other than V^(1/d): k_cell_length first
-  std::vector<int> slots;
-  int2* d_cmap = nullptr;
-  int2* d_nmap = nullptr;
-  int* d_list = nullptr;
-  int ncb = 0, nnb = 0;
-  // block maps of the row-segment x sweeps and the tiled y / z cell sweeps
-  bool rows = false;
-  int2* d_rcmap = nullptr;
-  int2* d_rnmap = nullptr;
-  int2* d_tmap[2] = {nullptr, nullptr};
-  int nrcb = 0, nrnb = 0, ntb[2] = {0, 0};
-};
-
-struct MhEnsembleImpl {
-  hipStream_t stream = nullptr;
-  int max_slots = 0;
-  double budget = 0.;
-  size_t used = 0;
-  std::vector<MhHandle*> slot;     // active cases
-  std::vector<MhHandle*> pending;  // admitted, activated at the next phase boundary
-  std::vector<int> pending_slot;
-  MhScalars* d_sc = nullptr;       // scalars of every slot
-  MhDev* d_tab[2] = {nullptr, nullptr};  // case states by coordinate parity
-  std::vector<EnsGroup> groups;
-  long launched = 0;
-  bool dirty = true;
-  long version = 0;
-  std::map<long, hipGraphExec_t> graphs;
-  std::vector<hipGraph_t> graph_defs;
-};
 
 namespace {
-
-void ens_free_groups(MhEnsembleImpl* E) {
-  for (auto& g : E->groups) {
-    (void)hipFree(g.d_cmap);
-    (void)hipFree(g.d_nmap);
-    (void)hipFree(g.d_list);
-    (void)hipFree(g.d_rcmap);
-    (void)hipFree(g.d_rnmap);
-    (void)hipFree(g.d_tmap[0]);
-    (void)hipFree(g.d_tmap[1]);
-  }
-  E->groups.clear();
-}
 
 void ens_drop_graphs(MhEnsembleImpl* E) {
   for (auto& kv : E->graphs) (void)hipGraphExecDestroy(kv.second);
@@ -864,160 +591,222 @@ void ens_drop_graphs(MhEnsembleImpl* E) {
   E->graph_defs.clear();
 }
 
-// Rebuilds the state tables and the block maps after a change of composition.
-int ens_rebuild(MhEnsembleImpl* E) {
-  HIPCHK(hipStreamSynchronize(E->stream));
-  ens_drop_graphs(E);
-  ens_free_groups(E);
-  std::vector<MhDev> tab[2];
-  for (int p = 0; p < 2; ++p) tab[p].resize(E->max_slots);
+void ens_free_list(EnsList& l) {
+  (void)hipFree(l.d);
+  (void)hipFree(l.d_n);
+  l.d = nullptr;
+  l.d_n = nullptr;
+  l.cap = 0;
+}
+
+void ens_free_groups(MhEnsembleImpl* E) {
+  for (auto& g : E->groups) {
+    for (EnsList* l : {&g.cmap, &g.nmap, &g.rcmap, &g.rnmap, &g.tmap[0], &g.tmap[1], &g.ntmap}) ens_free_list(*l);
+    (void)hipFree(g.d_list);
+    (void)hipFree(g.d_nlist);
+  }
+  E->groups.clear();
+}
+
+// tiles of the z-marching node kernel over one case
+int node_tile_blocks(const MhHandle* h) {
+  return ((h->g.nnx + MH_NTX - 1) / MH_NTX) * ((h->g.nny + MH_NTY - 1) / MH_NTY) *
+         ((h->g.nnz + h->dev.nt_kc - 1) / h->dev.nt_kc);
+}
+
+// Launch shapes chosen by the autotuners, per kind of work (process-wide, so that a later
+// case or ensemble of the same kind starts with them).
+std::mutex g_tune_mu;
+std::map<std::string, int> g_tune;  // key -> choice
+
+std::string group_key(const EnsGroup& g) {
+  char b[128];
+  snprintf(b, sizeof b, "ens:%d,%d,%d,%d,%d,%d,%d,%d,%d,%d", g.dim, g.m, g.o3, g.gen, g.csts, g.adv, g.lc, g.rows,
+           g.ntile, g.lim);
+  return b;
+}
+int tune_lookup(const std::string& k) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  auto it = g_tune.find(k);
+  return it == g_tune.end() ? -1 : it->second;
+}
+void tune_store(const std::string& k, int v) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune[k] = v;
+}
+
+// Kernel variants of a single case as one number v = node + 4 cell + 16 sweep, every one of
+// which gives the same bits: node 0 the flat node kernel, 1 / 2 / 3 the z-marching one with 4 /
+// 8 / 16 node planes per block; cell 0 the flat cell kernel, 1 / 2 / 3 the LDS-staged one with
+// 4 / 8 / 16 cell planes per block; sweep 0 row segments and tiles, 1 one thread per item.
+const int kTileDepth[4] = {0, 4, 8, 16};
+void set_variant(MhHandle* h, int v) {
+  const int node = v & 3, cell = (v >> 2) & 3;
+  h->node_tiled = node != 0;
+  if (node) h->nt_kc = kTileDepth[node];
+  h->cell_tiled = cell != 0;
+  if (cell) h->ct_k = kTileDepth[cell];
+  h->sweep_mode = (v >> 4) & 1 ? MH_SWEEP_FLAT : MH_SWEEP_TILE;
+  h->dev.nt_kc = h->nt_kc;
+  h->dev.ct_k = h->ct_k;
+}
+int tile_code(int depth) { return depth == 4 ? 1 : depth == 16 ? 3 : 2; }
+int get_variant(const MhHandle* h) {
+  return (h->node_tiled ? tile_code(h->nt_kc) : 0) | ((h->cell_tiled ? tile_code(h->ct_k) : 0) << 2) |
+         (h->sweep_mode == MH_SWEEP_FLAT ? 16 : 0);
+}
+// the key of a case before its device state exists (dimensions and the options that select kernels)
+std::string case_key_dims(const MhHandle* h) {
+  char b[160];
+  snprintf(b, sizeof b, "case:%d,%d,%d,%d,%d,%d,%d,%d,%d,%d", h->g.dim, h->g.nx, h->g.ny, h->g.nz, h->m,
+           h->opt.with_projection, h->opt.ordre_projection, h->opt.projection_limiteur_id,
+           h->opt.projection_pente_borne, h->opt.schema_csts);
+  return b;
+}
+
+// the batch group a case belongs to (created on first use: the launch sequence changes)
+int ens_group_of(MhEnsembleImpl* E, MhHandle* h) {
+  EnsGroup k;
+  k.dim = h->g.dim;
+  k.m = h->m;
+  k.o3 = h->dev.order2 == 2;
+  k.gen = ens_gen(h);
+  k.csts = h->dev.csts != 0;
+  k.adv = h->dev.sans_lagrange != 0;
+  k.lc = h->dev.lc != 0;
+  k.rows = !k.o3 && !k.gen && h->sweep_mode == MH_SWEEP_TILE;
+  k.ntile = k.dim == 3 && h->node_tiled && !k.adv;
+  k.lim = (k.o3 || k.gen) ? -1 : (h->dev.order2 ? h->dev.limiter : 3);
+  for (size_t i = 0; i < E->groups.size(); ++i) {
+    const EnsGroup& g = E->groups[i];
+    if (g.dim == k.dim && g.m == k.m && g.o3 == k.o3 && g.gen == k.gen && g.csts == k.csts && g.adv == k.adv &&
+        g.lc == k.lc && g.rows == k.rows && g.ntile == k.ntile && g.lim == k.lim)
+      return (int)i;
+  }
+  const int cached = tune_lookup(group_key(k));
+  k.bpc = cached > 0 ? cached : (E->tuned_bpc > 0 ? E->tuned_bpc : 4);
+  E->groups.push_back(k);
+  E->version++;
+  return (int)E->groups.size() - 1;
+}
+
+void ens_list_add(EnsList& l, int slot, int nblocks) {
+  for (int b = 0; b < nblocks; ++b) l.h.push_back(make_int2(slot, b));
+  l.dirty = true;
+}
+void ens_list_drop(EnsList& l, int slot) {
+  size_t w = 0;
+  for (size_t r = 0; r < l.h.size(); ++r)
+    if (l.h[r].x != slot) l.h[w++] = l.h[r];
+  if (w != l.h.size()) l.dirty = true;
+  l.h.resize(w);
+}
+
+// The blocks of case h (slot sl) in every list of its group.
+void ens_lists_add(EnsGroup& g, MhHandle* h, int sl) {
+  ens_list_add(g.cmap, sl, h->nblk_c);
+  ens_list_add(g.nmap, sl, h->nblk_n);
+  if (g.rows) {
+    ens_list_add(g.rcmap, sl, row_blocks(h));
+    ens_list_add(g.rnmap, sl, node_row_blocks(h));
+    for (int D = 1; D < g.dim; ++D) ens_list_add(g.tmap[D - 1], sl, tile_blocks(h, D));
+  }
+  if (g.ntile) ens_list_add(g.ntmap, sl, node_tile_blocks(h));
+  g.slots.push_back(sl);
+  g.list_dirty = true;
+}
+void ens_lists_drop(EnsGroup& g, int sl) {
+  for (EnsList* l : {&g.cmap, &g.nmap, &g.rcmap, &g.rnmap, &g.tmap[0], &g.tmap[1], &g.ntmap}) ens_list_drop(*l, sl);
+  g.slots.erase(std::remove(g.slots.begin(), g.slots.end(), sl), g.slots.end());
+  g.list_dirty = true;
+}
+
+// Uploads what changed (state-table entries of new cases, work lists and their lengths) with
+// copies ordered on the ensemble stream, through pinned staging that is reused once the copies
+// of the previous update have run.  A list that outgrows its buffer gets a larger one, and the
+// captured graphs, which hold the old address, are dropped.
+int ens_flush(MhEnsembleImpl* E) {
+  size_t need = 0;
+  for (int sl = 0; sl < E->max_slots; ++sl)
+    if (E->tab_dirty[sl]) need += 2 * sizeof(MhDev);
+  for (auto& g : E->groups) {
+    for (EnsList* l : {&g.cmap, &g.nmap, &g.rcmap, &g.rnmap, &g.tmap[0], &g.tmap[1], &g.ntmap})
+      if (l->dirty) need += l->h.size() * sizeof(int2) + 64;
+    if (g.list_dirty) need += g.slots.size() * sizeof(int) + 64;
+  }
+  if (need == 0) return MH_OK;
+  if (E->stage_pending) {
+    HIPCHK(hipEventSynchronize(E->stage_done));
+    E->stage_pending = false;
+  }
+  if (need > E->stage_cap) {
+    if (E->stage) (void)hipHostFree(E->stage);
+    E->stage = nullptr;
+    E->stage_cap = 0;
+    size_t cap = 1 << 20;
+    while (cap < need) cap *= 2;
+    HIPCHK(hipHostMalloc((void**)&E->stage, cap, hipHostMallocDefault));
+    E->stage_cap = cap;
+  }
+  size_t off = 0;
+  auto put = [&](void* dst, const void* src, size_t bytes) -> int {
+    off = (off + 15) & ~(size_t)15;
+    memcpy(E->stage + off, src, bytes);
+    HIPCHK(hipMemcpyAsync(dst, E->stage + off, bytes, hipMemcpyHostToDevice, E->stream));
+    off += bytes;
+    return MH_OK;
+  };
+  int st;
   for (int sl = 0; sl < E->max_slots; ++sl) {
+    if (!E->tab_dirty[sl]) continue;
     MhHandle* h = E->slot[sl];
-    if (!h) continue;
     for (int p = 0; p < 2; ++p) {
       MhDev d = h->dev;
       for (int k = 0; k < 3; ++k) {
         d.xlp[k] = h->xl[p][k];
         d.xlc[k] = h->xl[p ^ 1][k];
       }
-      tab[p][sl] = d;
+      if ((st = put(E->d_tab[p] + sl, &d, sizeof d)) != MH_OK) return st;
     }
-    bool found = false;
-    for (auto& g : E->groups)
-      if (g.dim == h->g.dim && g.m == h->m && g.o3 == (h->dev.order2 == 2) && g.gen == ens_gen(h) &&
-          g.csts == (h->dev.csts != 0) && g.adv == (h->dev.sans_lagrange != 0) && g.lc == (h->dev.lc != 0)) {
-        g.slots.push_back(sl);
-        found = true;
-      }
-    if (!found) {
-      EnsGroup g;
-      g.dim = h->g.dim;
-      g.m = h->m;
-      g.o3 = h->dev.order2 == 2;
-      g.gen = ens_gen(h);
-      g.csts = h->dev.csts != 0;
-      g.adv = h->dev.sans_lagrange != 0;
-      g.lc = h->dev.lc != 0;
-      g.slots.push_back(sl);
-      E->groups.push_back(g);
-    }
+    E->tab_dirty[sl] = 0;
   }
-  for (int p = 0; p < 2; ++p)
-    HIPCHK(hipMemcpy(E->d_tab[p], tab[p].data(), sizeof(MhDev) * E->max_slots, hipMemcpyHostToDevice));
   for (auto& g : E->groups) {
-    std::vector<int2> cm, nm;
-    for (int sl : g.slots) {
-      MhHandle* h = E->slot[sl];
-      for (int b = 0; b < h->nblk_c; ++b) cm.push_back(make_int2(sl, b));
-      for (int b = 0; b < h->nblk_n; ++b) nm.push_back(make_int2(sl, b));
-    }
-    g.ncb = (int)cm.size();
-    g.nnb = (int)nm.size();
-    HIPCHK(hipMalloc(&g.d_cmap, sizeof(int2) * cm.size()));
-    HIPCHK(hipMalloc(&g.d_nmap, sizeof(int2) * nm.size()));
-    HIPCHK(hipMalloc(&g.d_list, sizeof(int) * g.slots.size()));
-    HIPCHK(hipMemcpy(g.d_cmap, cm.data(), sizeof(int2) * cm.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(g.d_nmap, nm.data(), sizeof(int2) * nm.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(g.d_list, g.slots.data(), sizeof(int) * g.slots.size(), hipMemcpyHostToDevice));
-    // rows and tiles when every case of the group runs them
-    g.rows = !g.o3 && !g.gen;
-    for (int sl : g.slots) g.rows = g.rows && E->slot[sl]->sweep_mode == MH_SWEEP_TILE;
-    if (g.rows) {
-      std::vector<int2> rc, rn, t[2];
-      for (int sl : g.slots) {
-        MhHandle* h = E->slot[sl];
-        for (int b = 0, n = row_blocks(h); b < n; ++b) rc.push_back(make_int2(sl, b));
-        for (int b = 0, n = node_row_blocks(h); b < n; ++b) rn.push_back(make_int2(sl, b));
-        for (int D = 1; D < g.dim; ++D)
-          for (int b = 0, n = tile_blocks(h, D); b < n; ++b) t[D - 1].push_back(make_int2(sl, b));
+    for (EnsList* l : {&g.cmap, &g.nmap, &g.rcmap, &g.rnmap, &g.tmap[0], &g.tmap[1], &g.ntmap}) {
+      if (!l->dirty) continue;
+      const int n = (int)l->h.size();
+      if (n > l->cap || !l->d_n) {
+        int cap = 1024;
+        while (cap < n) cap *= 2;
+        ens_free_list(*l);
+        HIPCHK(hipMalloc(&l->d, sizeof(int2) * cap));
+        HIPCHK(hipMalloc(&l->d_n, sizeof(int)));
+        l->cap = cap;
+        E->version++;  // graphs hold the old buffer
       }
-      auto put = [](const std::vector<int2>& v, int2** d, int* n) -> int {
-        *n = (int)v.size();
-        if (v.empty()) return MH_OK;
-        HIPCHK(hipMalloc(d, sizeof(int2) * v.size()));
-        HIPCHK(hipMemcpy(*d, v.data(), sizeof(int2) * v.size(), hipMemcpyHostToDevice));
-        return MH_OK;
-      };
-      int st;
-      if ((st = put(rc, &g.d_rcmap, &g.nrcb)) != MH_OK) return st;
-      if ((st = put(rn, &g.d_rnmap, &g.nrnb)) != MH_OK) return st;
-      for (int q = 0; q < 2; ++q)
-        if ((st = put(t[q], &g.d_tmap[q], &g.ntb[q])) != MH_OK) return st;
+      if (n > 0 && (st = put(l->d, l->h.data(), sizeof(int2) * n)) != MH_OK) return st;
+      if ((st = put(l->d_n, &n, sizeof(int))) != MH_OK) return st;
+      l->dirty = false;
+    }
+    if (g.list_dirty) {
+      const int n = (int)g.slots.size();
+      if (n > g.list_cap || !g.d_nlist) {
+        int cap = 64;
+        while (cap < n) cap *= 2;
+        (void)hipFree(g.d_list);
+        (void)hipFree(g.d_nlist);
+        HIPCHK(hipMalloc(&g.d_list, sizeof(int) * cap));
+        HIPCHK(hipMalloc(&g.d_nlist, sizeof(int)));
+        g.list_cap = cap;
+        E->version++;
+      }
+      if (n > 0 && (st = put(g.d_list, g.slots.data(), sizeof(int) * n)) != MH_OK) return st;
+      if ((st = put(g.d_nlist, &n, sizeof(int))) != MH_OK) return st;
+      g.list_dirty = false;
     }
   }
-  E->dirty = false;
-  E->version++;
+  HIPCHK(hipEventRecord(E->stage_done, E->stream));
+  E->stage_pending = true;
   return MH_OK;
-}
-
-template <int DIM, int M>
-void ens_group_cycle(MhEnsembleImpl* E, const EnsGroup& g) {
-  const int par = (int)(E->launched & 1);
-  const MhDev* tab = E->d_tab[par];
-  hipStream_t st = E->stream;
-  if (g.adv) hipLaunchKernelGGL((k_node_adv_batch<DIM>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap);
-  else hipLaunchKernelGGL((k_node_lag_batch<DIM>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap);
-  if (g.lc) hipLaunchKernelGGL((k_cell_length_batch<DIM>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap);
-  if (g.csts) hipLaunchKernelGGL((k_cell_lag_csts_batch<DIM, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap);
-  else hipLaunchKernelGGL((k_cell_lag_batch<DIM, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap);
-  if (g.csts) hipLaunchKernelGGL((k_node_fwd_batch<DIM>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap);
-  const int sens = (int)(E->launched % DIM);
-  if (g.o3 || g.gen) hipLaunchKernelGGL((k_hcell_batch<DIM>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap);
-  for (int s = 0; s < DIM; ++s) {
-    const int D = (s + sens) % DIM, first = s == 0, last = s == DIM - 1, sel = s & 1;
-    if (g.gen) {
-      if (D == 0) {
-        hipLaunchKernelGGL((k_sweep_cell_gen_batch<DIM, 0, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
-        if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, 0, MH_LIM_ANY>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel,
-                           first, last);
-      } else if (D == 1) {
-        hipLaunchKernelGGL((k_sweep_cell_gen_batch<DIM, 1, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
-        if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, 1, MH_LIM_ANY>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel,
-                           first, last);
-      } else if constexpr (DIM == 3) {
-        hipLaunchKernelGGL((k_sweep_cell_gen_batch<3, 2, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
-        if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<3, 2, MH_LIM_ANY>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel,
-                           first, last);
-      }
-      continue;
-    }
-    if (g.o3) {
-      if (D == 0)
-        hipLaunchKernelGGL((k_sweep_cell3_batch<DIM, 0, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
-      else if (D == 1)
-        hipLaunchKernelGGL((k_sweep_cell3_batch<DIM, 1, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
-      else if constexpr (DIM == 3)
-        hipLaunchKernelGGL((k_sweep_cell3_batch<3, 2, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
-    }
-    if (g.o3 && D == 0) {
-      if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, 0, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
-    } else if (g.o3 && D == 1) {
-      if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, 1, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
-    } else if (g.o3) {
-      if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, (DIM == 3 ? 2 : 0), -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
-    } else if (g.rows && D == 0) {
-      hipLaunchKernelGGL((k_sweep_cell_row_batch<DIM, M, -1>), dim3(g.nrcb), dim3(MH_BLOCK), 0, st, tab, g.d_rcmap, sel, first);
-      if (!g.adv) hipLaunchKernelGGL((k_sweep_node_row_batch<DIM, -1>), dim3(g.nrnb), dim3(MH_BLOCK), 0, st, tab, g.d_rnmap, sel, first, last);
-    } else if (g.rows && D == 1) {
-      hipLaunchKernelGGL((k_sweep_cell_tile_batch<DIM, 1, M, -1>), dim3(g.ntb[0]), dim3(MH_BLOCK), 0, st, tab, g.d_tmap[0], sel, first);
-      if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, 1, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
-    } else if (g.rows) {
-      if constexpr (DIM == 3)
-        hipLaunchKernelGGL((k_sweep_cell_tile_batch<3, 2, M, -1>), dim3(g.ntb[1]), dim3(MH_BLOCK), 0, st, tab, g.d_tmap[1], sel, first);
-      if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, (DIM == 3 ? 2 : 0), -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
-    } else if (D == 0) {
-      hipLaunchKernelGGL((k_sweep_cell_batch<DIM, 0, M, -1>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
-      if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, 0, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
-    } else if (D == 1) {
-      hipLaunchKernelGGL((k_sweep_cell_batch<DIM, 1, M, -1>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
-      if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, 1, -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
-    } else {
-      hipLaunchKernelGGL((k_sweep_cell_batch<DIM, (DIM == 3 ? 2 : 0), M, -1>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, sel, first);
-      if (!g.adv) hipLaunchKernelGGL((k_sweep_node_batch<DIM, (DIM == 3 ? 2 : 0), -1>), dim3(g.nnb), dim3(MH_BLOCK), 0, st, tab, g.d_nmap, sel, first, last);
-    }
-  }
-  hipLaunchKernelGGL((k_finalize_batch<DIM, M>), dim3(g.ncb), dim3(MH_BLOCK), 0, st, tab, g.d_cmap, DIM & 1);
-  hipLaunchKernelGGL(k_dt_batch, dim3((unsigned)g.slots.size()), dim3(MH_DT_BLOCK), 0, st, tab, g.d_list);
 }
 
 void ens_cycle(MhEnsembleImpl* E) {
@@ -1037,14 +826,64 @@ void ens_cycle(MhEnsembleImpl* E) {
   E->launched += 1;
 }
 
-// Activates the pending cases (only at a phase boundary).
+// Activates the pending cases (only at a phase boundary: every active case shares the
+// coordinate parity and the sweep rotation of the ensemble's iteration count).
 int ens_activate(MhEnsembleImpl* E) {
   if (E->pending.empty() || (E->launched % 6) != 0) return MH_OK;
-  for (size_t i = 0; i < E->pending.size(); ++i) E->slot[E->pending_slot[i]] = E->pending[i];
+  for (size_t i = 0; i < E->pending.size(); ++i) {
+    const int sl = E->pending_slot[i];
+    MhHandle* h = E->pending[i];
+    E->slot[sl] = h;
+    E->tab_dirty[sl] = 1;
+    ens_lists_add(E->groups[ens_group_of(E, h)], h, sl);
+  }
   E->pending.clear();
   E->pending_slot.clear();
-  E->dirty = true;
   return MH_OK;
+}
+
+// Runs ncycles iterations of every active case, through 6-iteration graphs when asked (a
+// graph is captured once per launch sequence and reused while cases join and leave).
+int ens_step(MhEnsembleImpl* E, int ncycles, int use_graph) {
+  int done = 0;
+  while (done < ncycles) {
+    int st = ens_activate(E);
+    if (st != MH_OK) return st;
+    if ((st = ens_flush(E)) != MH_OK) return st;
+    if (use_graph && (E->launched % 6) == 0 && ncycles - done >= 6 && E->pending.empty()) {
+      const long key = E->version;
+      auto it = E->graphs.find(key);
+      if (it == E->graphs.end()) {
+        ens_drop_graphs(E);  // older launch sequences are not replayed again
+        hipGraph_t gdef = nullptr;
+        HIPCHK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+        const long l0 = E->launched;
+        for (int i = 0; i < 6; ++i) ens_cycle(E);
+        const hipError_t ec = hipStreamEndCapture(E->stream, &gdef);
+        E->launched = l0;
+        if (ec != hipSuccess) {
+          if (gdef) (void)hipGraphDestroy(gdef);
+          return hip_check(ec, "hipStreamEndCapture");
+        }
+        hipGraphExec_t ex;
+        const hipError_t ei = hipGraphInstantiate(&ex, gdef, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+          (void)hipGraphDestroy(gdef);
+          return hip_check(ei, "hipGraphInstantiate");
+        }
+        E->graph_defs.push_back(gdef);
+        it = E->graphs.emplace(key, ex).first;
+        E->graph_captures++;
+      }
+      HIPCHK(hipGraphLaunch(it->second, E->stream));
+      E->launched += 6;
+      done += 6;
+      continue;
+    }
+    ens_cycle(E);
+    done += 1;
+  }
+  return hip_check(hipGetLastError(), "ensemble launch");
 }
 
 }  // namespace
@@ -1101,11 +940,18 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
     }
   }
   h->ext_sc = ext_sc;
+  h->async_alloc = ext_sc != nullptr;  // a case of an ensemble
   {
     const char* nk = getenv("MAHYCO_NODE_KERNEL");
     h->node_tiled = !(nk && strcmp(nk, "flat") == 0);
+    const char* ck = getenv("MAHYCO_CELL_KERNEL");
+    h->cell_tiled = ck && strcmp(ck, "tiled") == 0;
     const char* sw = getenv("MAHYCO_SWEEP");
     h->sweep_mode = (sw && strcmp(sw, "flat") == 0) ? MH_SWEEP_FLAT : MH_SWEEP_TILE;
+  }
+  {
+    const int tuned = tune_lookup(case_key_dims(h));
+    if (tuned >= 0) set_variant(h, tuned);
   }
   HostInit I;
   st = host_init(h, I);
@@ -1240,6 +1086,8 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
   S.Pa = h->Pa;
   S.Pb = h->Pb;
   S.nblk_c = h->nblk_c;
+  S.nt_kc = h->nt_kc;
+  S.ct_k = h->ct_k;
   S.nblk_n = h->nblk_n;
   S.max_iter = 0;
   // initial Lagrangian coordinates = Eulerian grid
@@ -1331,14 +1179,15 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
 
 void mahyco_destroy(MhHandle* h) {
   if (!h) return;
-  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  // an ensemble's case frees its memory in stream order, behind the launches that read it
+  if (h->stream && !h->async_alloc) (void)hipStreamSynchronize(h->stream);
   for (auto& kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   for (auto gdef : h->graph_defs) (void)hipGraphDestroy(gdef);
   for (auto& r : h->prof_pending) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
   }
-  for (void* p : h->allocs) (void)hipFree(p);
+  for (void* p : h->allocs) (void)(h->async_alloc ? hipFreeAsync(p, h->stream) : hipFree(p));
   if (h->stream && h->owns_stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -1699,13 +1548,14 @@ int mahyco_compute_delta_t(MhHandle* h) {
 
 int mahyco_apply_eos(MhHandle* h, int env) {
   if (!h || env < 0 || env >= h->m) return MH_ERR_INVALID;
-  switch (h->m) {
-    case 1: launch_eos_t<1>(h, env); break;
-    case 2: launch_eos_t<2>(h, env); break;
-    case 3: launch_eos_t<3>(h, env); break;
-    default: launch_eos_t<4>(h, env); break;
-  }
+  launch_eos(h, env);
   return hip_check(hipGetLastError(), "k_apply_eos");
+}
+
+int mahyco_apply_one_cell_eos(MhHandle* h, int env, long cell) {
+  if (!h || env < 0 || env >= h->m || cell < 0 || cell >= h->g.nc) return MH_ERR_INVALID;
+  hipLaunchKernelGGL(k_apply_eos_cell<>, dim3(1), dim3(64), 0, h->stream, h->dev, env, cell);
+  return hip_check(hipGetLastError(), "k_apply_eos_cell");
 }
 
 int mahyco_compute_pression_moyenne(MhHandle* h) {
@@ -1732,7 +1582,14 @@ int mahyco_ensemble_create(int max_slots, double hbm_budget_bytes, MhEnsemble** 
   E->max_slots = max_slots;
   E->budget = hbm_budget_bytes;
   E->slot.assign(max_slots, nullptr);
+  E->tab_dirty.assign(max_slots, 0);
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0) E->ncu = ncu;
+  }
   int st = hip_check(hipStreamCreate(&E->stream), "hipStreamCreate");
+  if (st == MH_OK) st = hip_check(hipEventCreateWithFlags(&E->stage_done, hipEventDisableTiming), "hipEventCreate");
   if (st == MH_OK) st = hip_check(hipMalloc(&E->d_sc, sizeof(MhScalars) * max_slots), "hipMalloc");
   if (st == MH_OK) st = hip_check(hipMalloc(&E->d_tab[0], sizeof(MhDev) * max_slots), "hipMalloc");
   if (st == MH_OK) st = hip_check(hipMalloc(&E->d_tab[1], sizeof(MhDev) * max_slots), "hipMalloc");
@@ -1752,9 +1609,14 @@ void mahyco_ensemble_destroy(MhEnsemble* e) {
   ens_free_groups(E);
   for (MhHandle* h : E->slot) mahyco_destroy(h);
   for (MhHandle* h : E->pending) mahyco_destroy(h);
+  if (E->stream) (void)hipStreamSynchronize(E->stream);  // the cases' stream-ordered frees
   (void)hipFree(E->d_sc);
   (void)hipFree(E->d_tab[0]);
   (void)hipFree(E->d_tab[1]);
+  if (E->stage) (void)hipHostFree(E->stage);
+  if (E->stage_done) (void)hipEventDestroy(E->stage_done);
+  if (E->snap) (void)hipHostFree(E->snap);
+  if (E->snap_done) (void)hipEventDestroy(E->snap_done);
   if (E->stream) (void)hipStreamDestroy(E->stream);
   delete E;
 }
@@ -1800,50 +1662,21 @@ int mahyco_ensemble_remove(MhEnsemble* e, int slot) {
       return MH_OK;
     }
   if (!E->slot[slot]) return MH_ERR_INVALID;
-  HIPCHK(hipStreamSynchronize(E->stream));
+  // out of its group's work lists for the launches enqueued from now on; its memory is
+  // released in stream order behind the launches already enqueued
+  for (auto& g : E->groups)
+    if (std::find(g.slots.begin(), g.slots.end(), slot) != g.slots.end()) ens_lists_drop(g, slot);
+  int st = ens_flush(E);
   E->used -= E->slot[slot]->bytes;
   mahyco_destroy(E->slot[slot]);
   E->slot[slot] = nullptr;
-  E->dirty = true;
-  return MH_OK;
+  return st;
 }
 
 int mahyco_ensemble_step(MhEnsemble* e, int ncycles, int use_graph) {
   MhEnsembleImpl* E = (MhEnsembleImpl*)e;
   if (!E || ncycles < 0) return MH_ERR_INVALID;
-  int done = 0;
-  while (done < ncycles) {
-    int st = ens_activate(E);
-    if (st != MH_OK) return st;
-    if (E->dirty) {
-      st = ens_rebuild(E);
-      if (st != MH_OK) return st;
-    }
-    // a 6-iteration graph when no admission can happen inside it
-    if (use_graph && (E->launched % 6) == 0 && ncycles - done >= 6 && E->pending.empty()) {
-      const long key = E->version;
-      auto it = E->graphs.find(key);
-      if (it == E->graphs.end()) {
-        hipGraph_t gdef;
-        HIPCHK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
-        const long l0 = E->launched;
-        for (int i = 0; i < 6; ++i) ens_cycle(E);
-        HIPCHK(hipStreamEndCapture(E->stream, &gdef));
-        E->launched = l0;
-        hipGraphExec_t ex;
-        HIPCHK(hipGraphInstantiate(&ex, gdef, nullptr, nullptr, 0));
-        E->graph_defs.push_back(gdef);
-        it = E->graphs.emplace(key, ex).first;
-      }
-      HIPCHK(hipGraphLaunch(it->second, E->stream));
-      E->launched += 6;
-      done += 6;
-      continue;
-    }
-    ens_cycle(E);
-    done += 1;
-  }
-  return hip_check(hipGetLastError(), "ensemble launch");
+  return ens_step(E, ncycles, use_graph);
 }
 
 int mahyco_ensemble_synchronize(MhEnsemble* e) {
@@ -1885,6 +1718,162 @@ int mahyco_ensemble_get_field(MhEnsemble* e, int slot, int field, int env, int c
 double mahyco_ensemble_used_bytes(MhEnsemble* e) {
   MhEnsembleImpl* E = (MhEnsembleImpl*)e;
   return E ? (double)E->used : 0.;
+}
+// Enqueues a copy of every slot's scalars behind the work already queued, into pinned host
+// memory, without waiting: the host can queue the next iterations before reading the result
+// with mahyco_ensemble_poll_result.
+int mahyco_ensemble_poll_async(MhEnsemble* e) {
+  MhEnsembleImpl* E = (MhEnsembleImpl*)e;
+  if (!E) return MH_ERR_INVALID;
+  if (!E->snap) {
+    HIPCHK(hipHostMalloc((void**)&E->snap, sizeof(MhScalars) * E->max_slots, hipHostMallocDefault));
+    HIPCHK(hipEventCreateWithFlags(&E->snap_done, hipEventDisableTiming));
+  }
+  if (E->snap_pending) HIPCHK(hipEventSynchronize(E->snap_done));  // the buffer is reused
+  E->snap_state.assign(E->max_slots, -100);
+  for (int i = 0; i < E->max_slots; ++i)
+    if (E->slot[i]) E->snap_state[i] = 1;
+  for (int q : E->pending_slot) E->snap_state[q] = 2;
+  HIPCHK(hipMemcpyAsync(E->snap, E->d_sc, sizeof(MhScalars) * E->max_slots, hipMemcpyDeviceToHost, E->stream));
+  HIPCHK(hipEventRecord(E->snap_done, E->stream));
+  E->snap_pending = true;
+  return MH_OK;
+}
+
+// Waits for the snapshot of the last mahyco_ensemble_poll_async (not for later work) and fills
+// status, time and iteration as mahyco_ensemble_poll does; returns the number of active slots.
+int mahyco_ensemble_poll_result(MhEnsemble* e, int* status, double* time, int* iteration) {
+  MhEnsembleImpl* E = (MhEnsembleImpl*)e;
+  if (!E || !E->snap_pending) return MH_ERR_INVALID;
+  HIPCHK(hipEventSynchronize(E->snap_done));
+  int active = 0;
+  for (int i = 0; i < E->max_slots; ++i) {
+    const MhScalars& sc = E->snap[i];
+    const int state = E->snap_state[i];
+    int stv = -100;
+    if (state == 2) stv = 2;
+    else if (state == 1) stv = sc.status != 0 ? sc.status : (sc.stopped ? MH_STOPPED : MH_OK);
+    if (state == 1) active++;
+    if (status) status[i] = stv;
+    if (time) time[i] = state == 1 ? sc.t : 0.;
+    if (iteration) iteration[i] = state == 1 ? sc.iteration : 0;
+  }
+  return active;
+}
+
+long mahyco_ensemble_graph_captures(MhEnsemble* e) {
+  MhEnsembleImpl* E = (MhEnsembleImpl*)e;
+  return E ? E->graph_captures : -1;
+}
+
+// Times `cycles` iterations (hipGraph replays, after one untimed graph) of the ensemble's own
+// cases with each candidate persistent grid (2, 4, 8, 16 blocks per CU) and keeps the
+// fastest for every group, also for later groups of the same kind.  The iterations are the
+// cases' own: the results do not depend on the launch shape.
+int mahyco_ensemble_autotune(MhEnsemble* e, int cycles, int* bpc_out, double* ms_per_cycle_out) {
+  MhEnsembleImpl* E = (MhEnsembleImpl*)e;
+  if (!E || cycles < 1) return MH_ERR_INVALID;
+  cycles = (cycles + 5) / 6 * 6;
+  const int cand[] = {2, 4, 8, 16};
+  int best = 0;
+  float best_ms = 0.f;
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  int st = MH_OK;
+  for (int c : cand) {
+    for (auto& g : E->groups) g.bpc = c;
+    E->version++;
+    if ((st = ens_step(E, 6 + (int)((6 - E->launched % 6) % 6), 1)) != MH_OK) break;  // capture
+    if ((st = hip_check(hipEventRecord(a, E->stream), "hipEventRecord")) != MH_OK) break;
+    if ((st = ens_step(E, cycles, 1)) != MH_OK) break;
+    if ((st = hip_check(hipEventRecord(b, E->stream), "hipEventRecord")) != MH_OK) break;
+    if ((st = hip_check(hipEventSynchronize(b), "hipEventSynchronize")) != MH_OK) break;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    if (best == 0 || ms < best_ms) {
+      best = c;
+      best_ms = ms;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (st != MH_OK) return st;
+  for (auto& g : E->groups) {
+    g.bpc = best;
+    tune_store(group_key(g), best);
+  }
+  E->tuned_bpc = best;
+  E->version++;
+  if (bpc_out) *bpc_out = best;
+  if (ms_per_cycle_out) *ms_per_cycle_out = best_ms / cycles;
+  return MH_OK;
+}
+
+// Times `cycles` iterations (hipGraph replays of 6, after one untimed graph) of the case
+// itself with each kernel variant (tiled or flat node kernel, tiled or flat cell kernel,
+// row/tile or one-thread-per-item sweeps; only the sweep variants in 2D) and keeps the
+// fastest, also for later cases of the same dimensions and options.  The iterations are the
+// case's own: every variant gives the same bits.  Stops at the case's final time or an error.
+int mahyco_autotune(MhHandle* h, int cycles, int* variant_out, double* ms_per_cycle_out) {
+  if (!h || cycles < 1 || h->phase != 0 || h->ext_sc) return MH_ERR_INVALID;
+  cycles = (cycles + 5) / 6 * 6;
+  const int keep = get_variant(h);
+  int best = -1;
+  float best_ms = 0.f;
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  int st = MH_OK;
+  bool stopped = false;
+  auto timed = [&](int v) -> int {  // the variant's time per `cycles` iterations, best kept
+    set_variant(h, v);
+    drop_graphs(h);
+    int s = mahyco_cycle_graph(h, 6, 6);  // capture and first replay, untimed
+    if (s == MH_OK) s = hip_check(hipEventRecord(a, h->stream), "hipEventRecord");
+    if (s == MH_OK) s = mahyco_cycle_graph(h, cycles, 6);
+    if (s == MH_OK) s = hip_check(hipEventRecord(b, h->stream), "hipEventRecord");
+    if (s == MH_OK) s = hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+    if (s != MH_OK) return s;
+    if (mahyco_status(h) != MH_OK) {  // stopped or failed: the timings would not be comparable
+      stopped = true;
+      return MH_OK;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    if (best < 0 || ms < best_ms) {
+      best = v;
+      best_ms = ms;
+    }
+    return MH_OK;
+  };
+  // one coordinate at a time: the sweep variant, then (3D) the node kernel and its tile depth,
+  // then the cell kernel and its tile depth, each with the best of the others so far
+  const int dflt = h->g.dim == 3 ? 2 : 0;  // tiled node kernel with 8 planes, flat cell kernel, tiled sweeps
+  for (int sw = 0; sw < 2 && st == MH_OK && !stopped; ++sw) st = timed((dflt & 15) | (sw << 4));
+  if (h->g.dim == 3 && h->opt.with_projection) {
+    for (int node = 0; node < 4 && st == MH_OK && !stopped; ++node) {
+      const int v = (best & ~3) | node;
+      if (v != best) st = timed(v);
+    }
+    for (int cell = 0; cell < 4 && st == MH_OK && !stopped; ++cell) {
+      const int v = (best & ~12) | (cell << 2);
+      if (v != best) st = timed(v);
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (st != MH_OK || best < 0) {
+    set_variant(h, keep);
+    drop_graphs(h);
+    return st != MH_OK ? st : mahyco_status(h);
+  }
+  set_variant(h, best);
+  drop_graphs(h);
+  tune_store(case_key_dims(h), best);
+  if (variant_out) *variant_out = best;
+  if (ms_per_cycle_out) *ms_per_cycle_out = best_ms / cycles;
+  return MH_OK;
 }
 
 int mahyco_history_enable(MhHandle* h, int period, int max_rows) {

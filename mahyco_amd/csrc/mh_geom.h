@@ -148,6 +148,65 @@ MH_HD D3 cqs_hex_corner(const D3* x) {
   return t * (5. * (n2a08 + n2a12 + n4a12 + n4a11 + n6a11 + n6a08) + (n2a04 + n2a05 + n4a09 + n4a10 + n6a07 + n6a03));
 }
 
+// Component C (0 = x, 1 = y, 2 = z) of the eight corner vectors of a hexahedron, with the
+// operations of cqs_hex on that component only: the face centre components, the differences
+// to the centre, the component of each half cross product 0.5 (a_U b_W - a_W b_U) (U = C+1,
+// W = C+2 mod 3) and the corner sums t (5 (..) + (..)) in the same order.  Every rounded
+// operation of cqs_hex acts on one component, so q[l] is bit for bit cqs_hex's q[l].C, and
+// only one component's terms are live at a time.
+template <int C>
+MH_HD void cqs_hex_comp(const D3* x, double* q) {
+  constexpr int U = (C + 1) % 3, W = (C + 2) % 3;
+  auto cmp = [](const D3& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); };
+  // one face with nodes p0..p3 in the order of its centre sum: centre, then the cross
+  // products of the edges p0p1, p1p2, p2p3, p3p0 (names of cqs_hex)
+  auto face = [&](int p0, int p1, int p2, int p3, double& e0, double& e1, double& e2, double& e3) {
+    const double cu = 0.25 * (((cmp(x[p0], U) + cmp(x[p1], U)) + cmp(x[p2], U)) + cmp(x[p3], U));
+    const double cw = 0.25 * (((cmp(x[p0], W) + cmp(x[p1], W)) + cmp(x[p2], W)) + cmp(x[p3], W));
+    const double au0 = cmp(x[p0], U) - cu, aw0 = cmp(x[p0], W) - cw;
+    const double au1 = cmp(x[p1], U) - cu, aw1 = cmp(x[p1], W) - cw;
+    const double au2 = cmp(x[p2], U) - cu, aw2 = cmp(x[p2], W) - cw;
+    const double au3 = cmp(x[p3], U) - cu, aw3 = cmp(x[p3], W) - cw;
+    e0 = 0.5 * (au0 * aw1 - aw0 * au1);
+    e1 = 0.5 * (au1 * aw2 - aw1 * au2);
+    e2 = 0.5 * (au2 * aw3 - aw2 * au3);
+    e3 = 0.5 * (au3 * aw0 - aw3 * au0);
+  };
+  double n1a04, n1a03, n1a02, n1a01, n2a05, n2a12, n2a08, n2a04, n3a01, n3a06, n3a09, n3a05;
+  double n4a09, n4a10, n4a11, n4a12, n5a02, n5a07, n5a10, n5a06, n6a03, n6a08, n6a11, n6a07;
+  face(0, 3, 2, 1, n1a04, n1a03, n1a02, n1a01);
+  face(0, 4, 7, 3, n2a05, n2a12, n2a08, n2a04);
+  face(0, 1, 5, 4, n3a01, n3a06, n3a09, n3a05);
+  face(4, 5, 6, 7, n4a09, n4a10, n4a11, n4a12);
+  face(1, 2, 6, 5, n5a02, n5a07, n5a10, n5a06);
+  face(2, 3, 7, 6, n6a03, n6a08, n6a11, n6a07);
+  const double t = 1. / 12.;
+  q[0] = t * (5. * (n1a01 + n1a04 + n2a04 + n2a05 + n3a05 + n3a01) + (n1a02 + n1a03 + n2a08 + n2a12 + n3a06 + n3a09));
+  q[1] = t * (5. * (n1a01 + n1a02 + n3a01 + n3a06 + n5a06 + n5a02) + (n1a04 + n1a03 + n3a09 + n3a05 + n5a10 + n5a07));
+  q[2] = t * (5. * (n1a02 + n1a03 + n5a07 + n5a02 + n6a07 + n6a03) + (n1a01 + n1a04 + n5a06 + n5a10 + n6a11 + n6a08));
+  q[3] = t * (5. * (n1a03 + n1a04 + n2a08 + n2a04 + n6a08 + n6a03) + (n1a01 + n1a02 + n2a05 + n2a12 + n6a07 + n6a11));
+  q[4] = t * (5. * (n2a05 + n2a12 + n3a05 + n3a09 + n4a09 + n4a12) + (n2a08 + n2a04 + n3a01 + n3a06 + n4a10 + n4a11));
+  q[5] = t * (5. * (n3a06 + n3a09 + n4a09 + n4a10 + n5a10 + n5a06) + (n3a01 + n3a05 + n4a12 + n4a11 + n5a07 + n5a02));
+  q[6] = t * (5. * (n4a11 + n4a10 + n5a10 + n5a07 + n6a07 + n6a11) + (n4a12 + n4a09 + n5a06 + n5a02 + n6a03 + n6a08));
+  q[7] = t * (5. * (n2a08 + n2a12 + n4a12 + n4a11 + n6a11 + n6a08) + (n2a04 + n2a05 + n4a09 + n4a10 + n6a07 + n6a03));
+}
+
+// Volume of a hexahedron, sum_l x_l . CQS_l / 3 (MahycoModule.cc:1149-1162), one corner-vector
+// component at a time: dot3 rounds (x.x q.x + x.y q.y) + x.z q.z, so the products of the x and
+// y passes are summed before the z pass, and the corner terms are added in corner order.  Same
+// bits as cell_volume<8>(x, q) after cqs_hex(x, q).
+MH_HD double hex_volume_lowreg(const D3* x) {
+  double q[8], d[8];
+  cqs_hex_comp<0>(x, q);
+  for (int l = 0; l < 8; ++l) d[l] = x[l].x * q[l];
+  cqs_hex_comp<1>(x, q);
+  for (int l = 0; l < 8; ++l) d[l] = d[l] + x[l].y * q[l];
+  cqs_hex_comp<2>(x, q);
+  double v = 0.;
+  for (int l = 0; l < 8; ++l) v += d[l] + x[l].z * q[l];
+  return v / 3;
+}
+
 // Quadrangle corner vectors: half edge normals of the two edges at each corner
 // (x[4] = x[0], npc[0] = npc[4]; cqs[ii] = npc[ii+1] + npc[ii]).
 MH_HD D3 quad_npc(const D3* x, int k) {  // npc[k+1], k = 0..3

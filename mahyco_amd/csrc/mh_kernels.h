@@ -39,6 +39,28 @@ __device__ __forceinline__ double amax(double a, double b) { return (a < b) ? b 
 
 __device__ __forceinline__ D3 load3(double* const* a, long n) { return d3(a[0][n], a[1][n], a[2][n]); }
 
+
+// The node coordinates of the cell whose node 0 is b, in Arcane's local node order.
+template <int DIM>
+__device__ __forceinline__ void cell_nodes(const MhDev& S, long b, double* const* xs, D3* x) {
+  const long nnx = S.g.nnx, nnxy = (long)S.g.nnx * S.g.nny;
+  if (DIM == 3) {
+    x[0] = load3(xs, b);
+    x[1] = load3(xs, b + 1);
+    x[2] = load3(xs, b + 1 + nnx);
+    x[3] = load3(xs, b + nnx);
+    x[4] = load3(xs, b + nnxy);
+    x[5] = load3(xs, b + 1 + nnxy);
+    x[6] = load3(xs, b + 1 + nnx + nnxy);
+    x[7] = load3(xs, b + nnx + nnxy);
+  } else {
+    x[0] = d3(xs[0][b], xs[1][b], 0.);
+    x[1] = d3(xs[0][b + 1], xs[1][b + 1], 0.);
+    x[2] = d3(xs[0][b + 1 + nnx], xs[1][b + 1 + nnx], 0.);
+    x[3] = d3(xs[0][b + nnx], xs[1][b + nnx], 0.);
+  }
+}
+
 // Block-wide minimum (Arcane math::min semantics on non-NaN input), written by thread 0.
 // CFL minimum with the reference's sequential semantics.  computeDeltaT folds
 // min = math::min(min, dx) over the cells in order (MahycoModule.cc:1777), and math::min(a,
@@ -333,9 +355,7 @@ __device__ __forceinline__ void k_node_adv_body(const MhDev& S, int lb) {
 #define MH_NTX 31  // node columns of a tile along x
 #endif
 #define MH_NTY (MH_BLOCK / (MH_NTX + 1) - 1)  // along y: (MH_NTX + 1)(MH_NTY + 1) cells = MH_BLOCK
-#ifndef MH_NT_KC
-#define MH_NT_KC 8  // node planes per tile
-#endif
+#define MH_NT_KC 8  // node planes per tile by default (MhDev::nt_kc)
 #define MH_NT_CELLS ((MH_NTX + 1) * (MH_NTY + 1))  // = MH_BLOCK
 // What one cell of a column needs beyond its lower face: the 4 nodes of its upper face
 // (previous Lagrangian positions) and p, q, M.  The lower face is the upper face of the
@@ -403,9 +423,9 @@ __device__ __forceinline__ void k_node_lag3_tiled_body(const MhDev& S, int bx, i
   __shared__ double up[13 * MH_NT_CELLS];  // bottom corners (0..3) of the cell plane above
   if (mh_halted(S.sc)) return;
   const int tid = threadIdx.x;
-  const int i0 = bx * MH_NTX, j0 = by * MH_NTY, k0 = bz * MH_NT_KC;
+  const int i0 = bx * MH_NTX, j0 = by * MH_NTY, k0 = bz * S.nt_kc;
   const int nx = S.g.nx, ny = S.g.ny, nz = S.g.nz;
-  const int kend = min(k0 + MH_NT_KC, S.g.nnz);
+  const int kend = min(k0 + S.nt_kc, S.g.nnz);
   const double dt = S.sc->dt;
   const double dth = 0.5 * (S.sc->dt_old + dt);
   // this thread's cell column of the tile
@@ -468,24 +488,31 @@ __device__ __forceinline__ void k_node_lag3_tiled_body(const MhDev& S, int bx, i
 }
 
 // ----------------------------------------------------------------------------- cells
-template <int DIM>
-__device__ __forceinline__ void cell_nodes(const MhDev& S, long b, double* const* xs, D3* x) {
-  const long nnx = S.g.nnx, nnxy = (long)S.g.nnx * S.g.nny;
-  if (DIM == 3) {
-    x[0] = load3(xs, b);
-    x[1] = load3(xs, b + 1);
-    x[2] = load3(xs, b + 1 + nnx);
-    x[3] = load3(xs, b + nnx);
-    x[4] = load3(xs, b + nnxy);
-    x[5] = load3(xs, b + 1 + nnxy);
-    x[6] = load3(xs, b + 1 + nnx + nnxy);
-    x[7] = load3(xs, b + nnx + nnxy);
-  } else {
-    x[0] = d3(xs[0][b], xs[1][b], 0.);
-    x[1] = d3(xs[0][b + 1], xs[1][b + 1], 0.);
-    x[2] = d3(xs[0][b + 1 + nnx], xs[1][b + 1 + nnx], 0.);
-    x[3] = d3(xs[0][b + nnx], xs[1][b + nnx], 0.);
-  }
+
+// Volume of the hexahedron whose node 0 is b (hex_volume_lowreg's three component passes):
+// the node coordinates are read again for each pass, and a compiler fence between the passes
+// keeps their values from being shared, so one component's terms are live at a time (about
+// 100 registers instead of 200 for the whole corner-vector set).  Same bits as cqs_hex +
+// cell_volume<8>.
+__device__ __forceinline__ double hex_volume_passes(const MhDev& S, long b) {
+  D3 x[8];
+  double q[8], d[8];
+  cell_nodes<3>(S, b, S.xlc, x);
+  cqs_hex_comp<0>(x, q);
+#pragma unroll
+  for (int l = 0; l < 8; ++l) d[l] = x[l].x * q[l];
+  asm volatile("" ::: "memory");
+  cell_nodes<3>(S, b, S.xlc, x);
+  cqs_hex_comp<1>(x, q);
+#pragma unroll
+  for (int l = 0; l < 8; ++l) d[l] = d[l] + x[l].y * q[l];
+  asm volatile("" ::: "memory");
+  cell_nodes<3>(S, b, S.xlc, x);
+  cqs_hex_comp<2>(x, q);
+  double v = 0.;
+#pragma unroll
+  for (int l = 0; l < 8; ++l) v += d[l] + x[l].z * q[l];
+  return v / 3;
 }
 
 __device__ __forceinline__ double node_speed(const MhDev& S, long n, int dim) {
@@ -660,15 +687,46 @@ __device__ __forceinline__ void empty_cell_phi(const MhDev& S, long c, double* u
 // Without: applies the EOS and the mean pressure and writes the CFL candidate.
 // The cell's work once its node coordinates x (at this cycle's Lagrangian positions) are
 // in registers; returns the CFL candidate of the Lagrange-only loop.
-template <int DIM, int M, int CSTS = 0>
-__device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, const D3* x) {
+// The cell's own state read by the Lagrange step, loaded ahead of the geometry so that its
+// latency is hidden behind the corner-vector arithmetic.
+struct CellPre {
+  uint8_t mk;
+  double cs, rho_n, Mc, e_c, p_c;
+};
+__device__ __forceinline__ void cell_pre_load(const MhDev& S, long c, CellPre& pr) {
+  pr.mk = S.mask[c];
+  pr.cs = S.c[c];
+  pr.rho_n = S.rho[c];
+  pr.Mc = S.M[c];
+  pr.e_c = S.e[c];
+  pr.p_c = S.p[c];
+}
+
+// VGIVEN: the volume Vgiven was formed by the caller (the LDS-tiled kernel); x is unused.
+template <int DIM, int M, int CSTS = 0, int VGIVEN = 0, int REMAP = 0>
+__device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, const D3* x, const CellPre& pre,
+                                                double Vgiven = 0.) {
+  // REMAP: known at compile time to be the Lagrange step of the remap loop (with_proj set,
+  // sans_lagrange not), so the other branches are not compiled
+  const bool with_proj = REMAP ? true : (S.with_proj != 0);
+  const bool sans_lagrange = REMAP ? false : (S.sans_lagrange != 0);
   double* __restrict__ U = S.Ua;
   constexpr int NV = 3 * M + 1;
   double dxs = 1.79769313486231570815e+308;
   {
+    const uint8_t mk = pre.mk;
+    const double cs = pre.cs;
+    const double rho_n = pre.rho_n;
+    const double Mc = pre.Mc;
+    const double e_c = pre.e_c;
+    const double p_c = pre.p_c;
     D3 qv[8];
     double V;
-    if (DIM == 3) {
+    if constexpr (VGIVEN) {
+      V = Vgiven;
+    } else if constexpr (DIM == 3 && !CSTS) {
+      V = hex_volume_passes(S, b);  // the corner vectors themselves are not needed (x unused)
+    } else if (DIM == 3) {
       cqs_hex(x, qv);
       V = cell_volume<8>(x, qv, 3);
     } else {
@@ -678,16 +736,15 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
     if (V < 0.) S.err[0] = MH_ERR_NEG_VOLUME;
     // the other characteristic lengths come from k_cell_length, launched just before
     const double L = S.lc == MH_L_RACINE_CUBIQUE_VOLUME ? (DIM == 3 ? mh_pow_third(V) : mh_pow_half(V)) : S.L[c];
-    if (S.sans_lagrange) {
+    if (sans_lagrange) {
       // advection mode: no density, pseudo or energy update (MahycoModule.cc:1271, 389,
       // 1433); the remap vector of PrepareRemap.cc:100-155 from the stored state
-      const uint8_t mk = S.mask[c];
       const long nc = S.g.nc;
       double u_out[NV];
       for (int v = 0; v < NV; ++v) u_out[v] = 0.;
       if (__popc(mk) == 1) {
         const int ke = __ffs(mk) - 1;
-        const double rho = S.rho[c], e = S.e[c];
+        const double rho = rho_n, e = e_c;
         u_out[ke] = V;
         u_out[M + ke] = V * rho;
         u_out[2 * M + ke] = V * rho * e;
@@ -720,10 +777,6 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
       return dxs;
     }
     const double dt = S.sc->dt;
-    const uint8_t mk = S.mask[c];
-    const double cs = S.c[c];
-    const double rho_n = S.rho[c];
-    const double Mc = S.M[c];
     const double rho = Mc / V;
     const double tau = 0.5 * (1.0 / rho_n + 1.0 / rho);
     const double divu = 1.0 / dt * (1.0 / rho - 1.0 / rho_n) / tau;
@@ -739,11 +792,11 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
       if constexpr (CSTS) {
         double cn1, cn, cdn, cvon;
         csts_work<DIM>(S, b, qv, cn1, cn, cdn, cvon);
-        e = csts_e(ep, S.e[c], S.M[c], rho, S.p[c], S.q[c], csts_qnm1(S, c, ke, false), q, cn1, cn, cdn, cvon);
+        e = csts_e(ep, e_c, Mc, rho, p_c, S.q[c], csts_qnm1(S, c, ke, false), q, cn1, cn, cdn, cvon);
       } else {
-        e = lag_energy(S, ep, S.e[c], S.p[c], rho, rho_n, q);
+        e = lag_energy(S, ep, e_c, p_c, rho, rho_n, q);
       }
-      if (S.with_proj) {
+      if (with_proj) {
         for (int kk = 0; kk < M; ++kk)
           if (kk == ke) {
             u_out[kk] = V;
@@ -791,10 +844,10 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
         if (divu < 0.0) qk[kk] = 1. / tk * (-ep.a1 * L * cs * divu + ep.a2 * L * L * divu * divu);
         if constexpr (CSTS) {
           // with remap: the history of the remap (qe, csts_qnm1); without: the environment's own
-          const double qn = S.with_proj ? S.qe[c] : S.qkn[o];
-          const double qnm1 = S.with_proj ? csts_qnm1(S, c, kk, true) : S.qknm1[o];
+          const double qn = with_proj ? S.qe[c] : S.qkn[o];
+          const double qnm1 = with_proj ? csts_qnm1(S, c, kk, true) : S.qknm1[o];
           ek[kk] = csts_e(ep, S.ek[o], S.Mk[o], rk[kk], S.pk[o], qn, qnm1, qk[kk], cn1, cn, cdn, cvon);
-          if (!S.with_proj) {
+          if (!with_proj) {
             S.qknm1[o] = qn;
             S.qkn[o] = qk[kk];
           }
@@ -805,7 +858,7 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
         if ((mk >> kk) & 1) qsum += qk[kk] * fk[kk];
       for (int kk = 0; kk < M; ++kk)
         if ((mk >> kk) & 1) esum += S.chik[kk * nc + c] * ek[kk];
-      if (S.with_proj) {
+      if (with_proj) {
         for (int kk = 0; kk < M; ++kk) {
           if (!((mk >> kk) & 1)) continue;
           u_out[kk] = Vk[kk];
@@ -851,8 +904,8 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
         dxs = L / (cmix + cell_vmax<DIM>(S, b));
       }
     }
-    if (S.with_proj && !mk && !S.pb && S.nsweep > 0) empty_cell_phi<M>(S, c, u_out);
-    if (S.with_proj)
+    if (with_proj && !mk && !S.pb && S.nsweep > 0) empty_cell_phi<M>(S, c, u_out);
+    if (with_proj)
       for (int v = 0; v < NV; ++v) U[v * nc + c] = u_out[v];
     S.Vl[c] = V;
     S.L[c] = L;
@@ -872,12 +925,129 @@ __device__ __forceinline__ void k_cell_lag_body(const MhDev& S, int lb) {
     const int j = (int)(r % ny);
     const int k = (int)(r / ny);
     const long b = i + (long)S.g.nnx * (j + (long)S.g.nny * k);
-    D3 x[8];
-    cell_nodes<DIM>(S, b, S.xlc, x);
-    dxs = cell_lag_core<DIM, M, CSTS>(S, c, b, x);
+    CellPre pre;
+    cell_pre_load(S, c, pre);
+    if constexpr (DIM == 3 && !CSTS) {
+      dxs = cell_lag_core<DIM, M, CSTS>(S, c, b, nullptr, pre);  // the volume reads its nodes itself
+    } else {
+      D3 x[8];
+      cell_nodes<DIM>(S, b, S.xlc, x);
+      dxs = cell_lag_core<DIM, M, CSTS>(S, c, b, x, pre);
+    }
   }
   if (!S.with_proj) block_min_store(dxs, active, S.partial, lb);
 }
+// 3D cell kernel of the remap loop with the node neighbourhood staged in LDS: a block owns
+// MH_CT_X x MH_CT_Y cells of MH_CT_K consecutive z planes and marches up in z.  The node
+// coordinates of a tile plane ((MH_CT_X+1) x (MH_CT_Y+1) nodes, every node read from HBM once
+// per tile instead of by each of its cells) go to a rolling set of three LDS planes: the two a
+// cell plane needs, and the next one, whose global loads are issued before the cell plane's
+// arithmetic and written to LDS after it.  The cell's own state is also loaded one plane ahead.
+// The volume takes its three component passes from LDS; everything else is cell_lag_core.
+#define MH_CT_X 64
+#define MH_CT_Y (MH_BLOCK / MH_CT_X)
+#define MH_CT_K 8  // cell planes per tile by default (MhDev::ct_k)
+#define MH_CT_NX (MH_CT_X + 1)
+#define MH_CT_NP (MH_CT_NX * (MH_CT_Y + 1))  // nodes of a tile plane
+#define MH_CT_PER ((MH_CT_NP + MH_BLOCK - 1) / MH_BLOCK)  // node entries loaded per thread
+#ifndef MH_CT_MIN_WAVES
+#define MH_CT_MIN_WAVES 3
+#endif
+
+// Volume of the cell whose node 0 is entry s of the bottom plane (hex_volume_passes with the
+// nodes read from LDS; same bits).
+__device__ __forceinline__ double hex_volume_lds(const double* bot, const double* top, int s) {
+  const int o[4] = {s, s + 1, s + 1 + MH_CT_NX, s + MH_CT_NX};
+  auto load = [&](D3* x) {
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      x[l] = d3(bot[o[l]], bot[MH_CT_NP + o[l]], bot[2 * MH_CT_NP + o[l]]);
+      x[l + 4] = d3(top[o[l]], top[MH_CT_NP + o[l]], top[2 * MH_CT_NP + o[l]]);
+    }
+  };
+  D3 x[8];
+  double q[8], d[8];
+  load(x);
+  cqs_hex_comp<0>(x, q);
+#pragma unroll
+  for (int l = 0; l < 8; ++l) d[l] = x[l].x * q[l];
+  asm volatile("" ::: "memory");
+  load(x);
+  cqs_hex_comp<1>(x, q);
+#pragma unroll
+  for (int l = 0; l < 8; ++l) d[l] = d[l] + x[l].y * q[l];
+  asm volatile("" ::: "memory");
+  load(x);
+  cqs_hex_comp<2>(x, q);
+  double v = 0.;
+#pragma unroll
+  for (int l = 0; l < 8; ++l) v += d[l] + x[l].z * q[l];
+  return v / 3;
+}
+
+template <int M>
+__device__ __forceinline__ void k_cell_lag3_tiled_body(const MhDev& S, int bx, int by, int bz) {
+  __shared__ double pl[3][3 * MH_CT_NP];
+  if (mh_halted(S.sc)) return;
+  const int tid = threadIdx.x, tx = tid % MH_CT_X, ty = tid / MH_CT_X;
+  const int nx = S.g.nx, ny = S.g.ny, nz = S.g.nz;
+  const int i0 = bx * MH_CT_X, j0 = by * MH_CT_Y, k0 = bz * S.ct_k;
+  const int kend = min(k0 + S.ct_k, nz);
+  const int i = i0 + tx, j = j0 + ty;
+  const bool ok = i < nx && j < ny;
+  const long nnx = S.g.nnx, nnxy = (long)S.g.nnx * S.g.nny;
+  // the node entries of a tile plane this thread loads
+  double nv[MH_CT_PER][3];
+  auto fetch = [&](int kn) {
+#pragma unroll
+    for (int r = 0; r < MH_CT_PER; ++r) {
+      const int e = tid + r * MH_BLOCK;
+      const int gi = i0 + e % MH_CT_NX, gj = j0 + e / MH_CT_NX;
+      if (e < MH_CT_NP && gi <= nx && gj <= ny) {
+        const long n = gi + nnx * gj + nnxy * kn;
+        nv[r][0] = S.xlc[0][n];
+        nv[r][1] = S.xlc[1][n];
+        nv[r][2] = S.xlc[2][n];
+      }
+    }
+  };
+  auto put = [&](double* P) {
+#pragma unroll
+    for (int r = 0; r < MH_CT_PER; ++r) {
+      const int e = tid + r * MH_BLOCK;
+      if (e < MH_CT_NP) {
+        P[e] = nv[r][0];
+        P[MH_CT_NP + e] = nv[r][1];
+        P[2 * MH_CT_NP + e] = nv[r][2];
+      }
+    }
+  };
+  fetch(k0);
+  put(pl[0]);
+  fetch(k0 + 1);
+  put(pl[1]);
+  const long cplane = (long)nx * ny;
+  const long c0 = i + (long)nx * j;
+  CellPre cur, nxt;
+  if (ok && k0 < kend) cell_pre_load(S, c0 + cplane * k0, nxt);
+  __syncthreads();
+  for (int k = k0; k < kend; ++k) {
+    const int sl = k - k0;
+    const bool more = k + 1 < kend;
+    if (more) fetch(k + 2);  // node plane of the next step, into registers now
+    cur = nxt;
+    if (ok && more) cell_pre_load(S, c0 + cplane * (k + 1), nxt);
+    if (ok) {
+      const long c = c0 + cplane * k;
+      const long b = i + nnx * j + nnxy * k;
+      const double V = hex_volume_lds(pl[sl % 3], pl[(sl + 1) % 3], tx + MH_CT_NX * ty);
+      (void)cell_lag_core<3, M, 0, 1, 1>(S, c, b, nullptr, cur, V);
+    }
+    if (more) put(pl[(sl + 2) % 3]);
+    __syncthreads();
+  }
+}
+
 // The characteristic length of the other options, written to L before the cell kernel
 // (which then reads it instead of forming V^(1/d)): one thread per cell.
 template <int DIM>
@@ -2297,8 +2467,7 @@ __device__ __forceinline__ void k_sweep_node_body(const MhDev& S, int lb, int ps
 // time on the same XCD (xcd_block), instead of blocks a plane (~nnx nny / 256 blocks) apart.
 // Grid: ceil(nnx nny / MH_BLOCK) * nnz blocks, rounded up to a multiple of 8.
 template <int LIM>
-__global__ __launch_bounds__(MH_BLOCK) void k_sweep_node_z(MhDev S, int psel, int first, int last) {
-  const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+__device__ __forceinline__ void k_sweep_node_z_body(const MhDev& S, int lb, int psel, int first, int last) {
   const long nnxy = (long)S.g.nnx * S.g.nny;
   const int nnz = S.g.nnz;
   const long seg = lb / nnz;
@@ -2306,6 +2475,10 @@ __global__ __launch_bounds__(MH_BLOCK) void k_sweep_node_z(MhDev S, int psel, in
   const long off = seg * MH_BLOCK + threadIdx.x;
   if (off >= nnxy) return;  // also the rounding blocks: seg beyond the plane's segments
   k_sweep_node_at<3, 2, LIM>(S, z * nnxy + off, psel, first, last);
+}
+template <int LIM>
+__global__ __launch_bounds__(MH_BLOCK) void k_sweep_node_z(MhDev S, int psel, int first, int last) {
+  k_sweep_node_z_body<LIM>(S, xcd_block((int)blockIdx.x, (int)gridDim.x), psel, first, last);
 }
 
 // u = mu / m for every node with a non-zero projected mass (RemapADIFinal.cc:286-297), for
@@ -2568,10 +2741,7 @@ __device__ __forceinline__ void k_finalize_body(const MhDev& S, int lb, int usel
 
 // applyEOS of one environment on its member cells (PerfectGasEOSService.cc:45-67,
 // StiffenedGasEOSService.cc:43-63), as a standalone call.
-template <int M>
-__global__ __launch_bounds__(MH_BLOCK) void k_apply_eos(MhDev S, int env) {
-  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (c >= S.g.nc) return;
+__device__ __forceinline__ void apply_eos_at(const MhDev& S, int env, long c) {
   const uint8_t mk = S.mask[c];
   if (!((mk >> env) & 1)) return;
   const double tnow = S.sc->t;  // between iterations: m_global_time of the last one
@@ -2596,6 +2766,17 @@ __global__ __launch_bounds__(MH_BLOCK) void k_apply_eos(MhDev S, int env) {
     }
     S.pk[o] = p;
   }
+}
+template <int U = 0>
+__global__ __launch_bounds__(MH_BLOCK) void k_apply_eos(MhDev S, int env) {
+  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (c >= S.g.nc) return;
+  apply_eos_at(S, env, c);
+}
+// applyOneCellEOS (IEquationOfState.h:50-52): the same for one cell
+template <int U = 0>
+__global__ __launch_bounds__(64) void k_apply_eos_cell(MhDev S, int env, long c) {
+  if (threadIdx.x == 0) apply_eos_at(S, env, c);
 }
 
 // computePressionMoyenne (MahycoModule.cc:1708-1735) on multi-environment cells; the
@@ -2623,14 +2804,12 @@ __global__ __launch_bounds__(MH_BLOCK) void k_pression_moyenne(MhDev S) {
   S.c[c] = cm;
 }
 
-// computeDeltaT (MahycoModule.cc:1739-1864) on the block minima, plus the time increment
-// of the iteration and the final-time stop rule.  One block.
-__device__ __forceinline__ void k_dt_body(const MhDev& S) {
-  const int nblocks = S.nblk_c;
+// The CFL minimum of computeDeltaT (MahycoModule.cc:1767-1792) from the per-block minima
+// partial[2b] (and NaN flags partial[2b+1]) of nblocks blocks, in cell order.  Called by every
+// thread of a one-block launch; returns true on thread 0 only, with the minimum in *mn.
+__device__ __forceinline__ bool cfl_fold(const double* __restrict__ partial, int nblocks, double* mn) {
   __shared__ double rm[MH_DT_BLOCK / 64];
   __shared__ int rn[MH_DT_BLOCK / 64];
-  MhScalars* sc = S.sc;
-  if (mh_halted(sc)) return;
   const int nw = (int)(blockDim.x >> 6), lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   __shared__ int any_nan;
   // without a NaN candidate anywhere the fold is a plain minimum, in any order
@@ -2644,18 +2823,18 @@ __device__ __forceinline__ void k_dt_body(const MhDev& S) {
   const int bd = (int)blockDim.x;
   int b = threadIdx.x;
   for (; b + 3 * bd < nblocks; b += 4 * bd) {
-    const double a0 = S.partial[2 * b], a1 = S.partial[2 * (b + bd)];
-    const double a2 = S.partial[2 * (b + 2 * bd)], a3 = S.partial[2 * (b + 3 * bd)];
-    pn |= (S.partial[2 * b + 1] != 0.) | (S.partial[2 * (b + bd) + 1] != 0.) |
-          (S.partial[2 * (b + 2 * bd) + 1] != 0.) | (S.partial[2 * (b + 3 * bd) + 1] != 0.);
+    const double a0 = partial[2 * b], a1 = partial[2 * (b + bd)];
+    const double a2 = partial[2 * (b + 2 * bd)], a3 = partial[2 * (b + 3 * bd)];
+    pn |= (partial[2 * b + 1] != 0.) | (partial[2 * (b + bd) + 1] != 0.) |
+          (partial[2 * (b + 2 * bd) + 1] != 0.) | (partial[2 * (b + 3 * bd) + 1] != 0.);
     pm = min_skip_nan(pm, a0);
     pm1 = min_skip_nan(pm1, a1);
     pm2 = min_skip_nan(pm2, a2);
     pm3 = min_skip_nan(pm3, a3);
   }
   for (; b < nblocks; b += bd) {
-    pm = min_skip_nan(pm, S.partial[2 * b]);
-    pn |= S.partial[2 * b + 1] != 0.;
+    pm = min_skip_nan(pm, partial[2 * b]);
+    pn |= partial[2 * b + 1] != 0.;
   }
   pm = min_skip_nan(min_skip_nan(pm, pm1), min_skip_nan(pm2, pm3));
   for (int off = 32; off > 0; off >>= 1) pm = min_skip_nan(pm, __shfl_xor(pm, off, 64));
@@ -2664,7 +2843,7 @@ __device__ __forceinline__ void k_dt_body(const MhDev& S) {
   __syncthreads();
   MhCflMin tot;
   if (!any_nan) {
-    if (threadIdx.x != 0) return;
+    if (threadIdx.x != 0) return false;
     tot.nan = 0;
     tot.m = rm[0];
     for (int w = 1; w < nw; ++w) tot.m = min_skip_nan(tot.m, rm[w]);
@@ -2679,8 +2858,8 @@ __device__ __forceinline__ void k_dt_body(const MhDev& S) {
       const int b = b0 + lane;
       MhCflMin o = cfl_leaf(0., false);
       if (b < bend) {
-        o.m = S.partial[2 * b];
-        o.nan = S.partial[2 * b + 1] != 0.;
+        o.m = partial[2 * b];
+        o.nan = partial[2 * b + 1] != 0.;
       }
       o = cfl_wave(o);
       acc = cfl_combine(acc, o);  // lane 0 holds the run's value; other lanes are ignored
@@ -2690,7 +2869,7 @@ __device__ __forceinline__ void k_dt_body(const MhDev& S) {
       rn[wid] = acc.nan;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
+    if (threadIdx.x != 0) return false;
     tot.m = rm[0];
     tot.nan = rn[0];
     for (int w = 1; w < nw; ++w) {
@@ -2700,7 +2879,17 @@ __device__ __forceinline__ void k_dt_body(const MhDev& S) {
       tot = cfl_combine(tot, o);
     }
   }
-  const double mn = tot.nan ? tot.m : amin(1.79769313486231570815e+308, tot.m);
+  *mn = tot.nan ? tot.m : amin(1.79769313486231570815e+308, tot.m);
+  return true;
+}
+
+// computeDeltaT (MahycoModule.cc:1739-1864) on the block minima, plus the time increment
+// of the iteration and the final-time stop rule.  One block.
+__device__ __forceinline__ void k_dt_body(const MhDev& S) {
+  MhScalars* sc = S.sc;
+  if (mh_halted(sc)) return;
+  double mn;
+  if (!cfl_fold(S.partial, S.nblk_c, &mn)) return;
   const double dt = sc->dt;
   const double t = sc->t + dt;
   double new_dt = S.cfl * mn;
@@ -2816,24 +3005,31 @@ __global__ __launch_bounds__(MH_BLOCK) void k_hist_final(MhDev S) { k_hist_final
 // Single-mesh grids use the XCD-contiguous block order of xcd_block.
 
 #ifndef MH_CELL_MIN_WAVES
-#define MH_CELL_MIN_WAVES 1  // register budget of k_cell_lag, in waves per SIMD
+#define MH_CELL_MIN_WAVES 3  // register budget of k_cell_lag, in waves per SIMD
 #endif
 #ifndef MH_NODE_MIN_WAVES
 #define MH_NODE_MIN_WAVES 1  // register budget of k_node_lag3_tiled, in waves per SIMD
 #endif
+// Batched variants are persistent: a fixed grid walks the work list bmap[0 .. *nwork) (one
+// entry per (case, block) of every case of a batch group) with a grid stride, so the list and
+// its length can change between launches of a captured graph without re-capture.
 #define MH_WRAP(NAME, WAVES, TPL, TARGS, PARAMS, ARGS)                                      \
   TPL __global__ __launch_bounds__(MH_BLOCK, WAVES) void NAME(MhDev S PARAMS) {              \
     NAME##_body TARGS(S, xcd_block((int)blockIdx.x, (int)gridDim.x) ARGS);                   \
   }                                                                                          \
-  TPL __global__ __launch_bounds__(MH_BLOCK) void NAME##_batch(const MhDev* __restrict__ tab, \
-                                                               const int2* __restrict__ bmap PARAMS) { \
-    const int2 bm = bmap[blockIdx.x];                                                        \
-    NAME##_body TARGS(tab[bm.x], bm.y ARGS);                                                 \
+  TPL __global__ __launch_bounds__(MH_BLOCK, WAVES) void NAME##_batch(                        \
+      const MhDev* __restrict__ tab, const int2* __restrict__ bmap, const int* __restrict__ nwork PARAMS) { \
+    const int n = *nwork;                                                                    \
+    for (int w = (int)blockIdx.x; w < n; w += (int)gridDim.x) {                              \
+      const int2 bm = bmap[w];                                                               \
+      NAME##_body TARGS(tab[bm.x], bm.y ARGS);                                               \
+      __syncthreads();                                                                       \
+    }                                                                                        \
   }
 #define MH_COMMA ,
 MH_WRAP(k_node_lag, 1, template <int DIM>, <DIM>, , )
 MH_WRAP(k_cell_lag, MH_CELL_MIN_WAVES, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, , )
-MH_WRAP(k_cell_lag_csts, MH_CELL_MIN_WAVES, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, , )
+MH_WRAP(k_cell_lag_csts, 1, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, , )
 MH_WRAP(k_cell_length, 1, template <int DIM>, <DIM>, , )
 MH_WRAP(k_sweep_cell, 1, template <int DIM MH_COMMA int D MH_COMMA int M MH_COMMA int LIM>, <DIM MH_COMMA D MH_COMMA M MH_COMMA LIM>,
         MH_COMMA int usel MH_COMMA int first, MH_COMMA usel MH_COMMA first)
@@ -2856,14 +3052,42 @@ MH_WRAP(k_sweep_cell_gen, 1, template <int DIM MH_COMMA int D MH_COMMA int M>, <
 MH_WRAP(k_sweep_cell3, 1, template <int DIM MH_COMMA int D MH_COMMA int M>, <DIM MH_COMMA D MH_COMMA M>,
         MH_COMMA int usel MH_COMMA int first, MH_COMMA usel MH_COMMA first)
 
+// (the plain kernels are templates with an unused parameter, so that every translation unit
+// that launches one may instantiate it)
+template <int U = 0>
 __global__ __launch_bounds__(MH_BLOCK, MH_NODE_MIN_WAVES) void k_node_lag3_tiled(MhDev S) {
   k_node_lag3_tiled_body(S, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
+template <int U = 0>
 __global__ __launch_bounds__(MH_DT_BLOCK) void k_dt(MhDev S) { k_dt_body(S); }
+template <int M>
+__global__ __launch_bounds__(MH_BLOCK, MH_CT_MIN_WAVES) void k_cell_lag3_tiled(MhDev S) {
+  k_cell_lag3_tiled_body<M>(S, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+template <int U = 0>
 __global__ __launch_bounds__(MH_DT_BLOCK) void k_dt_batch(const MhDev* __restrict__ tab,
-                                                          const int* __restrict__ list) {
-  k_dt_body(tab[list[blockIdx.x]]);
+                                                          const int* __restrict__ list,
+                                                          const int* __restrict__ nlist) {
+  const int n = *nlist;
+  for (int w = (int)blockIdx.x; w < n; w += (int)gridDim.x) {
+    k_dt_body(tab[list[w]]);
+    __syncthreads();
+  }
+}
+// The z-marching node kernel over a batch group: work item (case, tile), tile index
+// bx + ntx (by + nty bz) with ntx, nty the case's tile counts along x and y.
+template <int U = 0>
+__global__ __launch_bounds__(MH_BLOCK, MH_NODE_MIN_WAVES) void k_node_lag3_tiled_batch(
+    const MhDev* __restrict__ tab, const int2* __restrict__ bmap, const int* __restrict__ nwork) {
+  const int n = *nwork;
+  for (int w = (int)blockIdx.x; w < n; w += (int)gridDim.x) {
+    const int2 bm = bmap[w];
+    const MhDev& S = tab[bm.x];
+    const int ntx = (S.g.nnx + MH_NTX - 1) / MH_NTX, nty = (S.g.nny + MH_NTY - 1) / MH_NTY;
+    k_node_lag3_tiled_body(S, bm.y % ntx, (bm.y / ntx) % nty, bm.y / (ntx * nty));
+    __syncthreads();
+  }
 }
 
 #endif

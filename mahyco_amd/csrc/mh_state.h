@@ -123,6 +123,8 @@ struct MhDev {
                        // mean p_k, volume-weighted rho_k, mass-weighted e_k, member count)
   double* hist_part;   // per-block partial sums, 6m doubles per block
   int hist_max;        // capacity in rows
+  int nt_kc;           // node planes per block of the z-marching node kernel (autotuned)
+  int ct_k;            // cell planes per block of the LDS-staged cell kernel (autotuned)
   int pad3;
 };
 

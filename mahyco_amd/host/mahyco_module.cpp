@@ -53,6 +53,9 @@ MhEnvironment FictifEOS::description() const {
   return e;
 }
 void FictifEOS::applyEOS(int env) { module_->check(mahyco_apply_eos(module_->h_, env), "FictifEOS::applyEOS"); }
+void IEquationOfState::applyOneCellEOS(int env, long cell) {
+  module_->check(mahyco_apply_one_cell_eos(module_->h_, env, cell), "IEquationOfState::applyOneCellEOS");
+}
 // initEOS runs inside mahyco_create (HydroStartInit, MahycoModule.cc:143-152).
 void PerfectGasEOS::initEOS(int) {}
 void StiffenedGasEOS::initEOS(int) {}

@@ -40,6 +40,8 @@ class IEquationOfState {
   virtual void initEOS(int env) = 0;   // initial energy and sound speed (done by HydroStartInit)
   virtual void ReinitEOS(int env) = 0; // no state to rebuild on restart
   virtual void applyEOS(int env) = 0;  // pressure and sound speed of the env's cells
+  // the same for one cell of the environment (IEquationOfState.h:52; Newton's iterates)
+  virtual void applyOneCellEOS(int env, long cell);
   virtual void Endommagement(int env) = 0;  // inert at the default thresholds
   virtual double getAdiabaticCst(int env) = 0;
   virtual double getdensityDamageThresold(int env) = 0;
@@ -243,6 +245,7 @@ class MahycoModule {
   friend class PerfectGasEOS;
   friend class StiffenedGasEOS;
   friend class FictifEOS;
+  friend class IEquationOfState;
   friend class RemapADI;
 };
 

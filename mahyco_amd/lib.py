@@ -36,6 +36,12 @@ def load():
     lib.mahyco_create_on_stream.argtypes = [C.POINTER(T.MhMesh), C.c_void_p, C.c_int, C.POINTER(T.MhOptions),
                                             C.c_void_p, C.POINTER(C.c_void_p)]
     lib.mahyco_create_on_stream.restype = C.c_int
+    lib.mahyco_autotune.argtypes = [H, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    lib.mahyco_autotune.restype = C.c_int
+    lib.mahyco_ensemble_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    lib.mahyco_ensemble_autotune.restype = C.c_int
+    lib.mahyco_ensemble_graph_captures.argtypes = [C.c_void_p]
+    lib.mahyco_ensemble_graph_captures.restype = C.c_long
     lib.mahyco_set_field.argtypes = [H, C.c_int, C.c_int, C.c_int, C.c_void_p]
     lib.mahyco_set_field.restype = C.c_int
     lib.mahyco_set_scalars.argtypes = [H, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]
@@ -191,6 +197,15 @@ class GpuCase:
         if st != T.MH_OK:
             raise MahycoError(st, "mahyco_set_field")
 
+    def autotune(self, cycles=12):
+        """Times the kernel variants on the case's own iterations and keeps the fastest
+        (mahyco_autotune); returns (variant, ms per iteration)."""
+        v, ms = C.c_int(-1), C.c_double(0.)
+        st = self.lib.mahyco_autotune(self.h, cycles, C.byref(v), C.byref(ms))
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_autotune")
+        return v.value, ms.value
+
     def set_scalars(self, time, deltat, old_deltat, iteration, sens_projection):
         st = self.lib.mahyco_set_scalars(self.h, time, deltat, old_deltat, iteration, sens_projection)
         if st != T.MH_OK:
@@ -312,6 +327,19 @@ class Ensemble:
 
     def synchronize(self):
         return self.lib.mahyco_ensemble_synchronize(self.e)
+
+    @property
+    def graph_captures(self):
+        return self.lib.mahyco_ensemble_graph_captures(self.e)
+
+    def autotune(self, cycles=12):
+        """Times the persistent grid sizes on the ensemble's own iterations and keeps the
+        fastest (mahyco_ensemble_autotune); returns (blocks per CU, ms per iteration)."""
+        b, ms = C.c_int(0), C.c_double(0.)
+        st = self.lib.mahyco_ensemble_autotune(self.e, cycles, C.byref(b), C.byref(ms))
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_ensemble_autotune")
+        return b.value, ms.value
 
     def poll(self):
         st = np.zeros(self.max_slots, dtype=np.int32)

@@ -45,40 +45,45 @@ def serve_requests(n=64, seed=20261202, rate=4.0, t_end=0.1, lo2=128, hi2=1024, 
     return reqs
 
 
-def run_batch(batch, graph=True, max_slots=None, budget=200e9, poll_every=30):
-    """All cases in one ensemble; returns (seconds, per-case iterations)."""
+def run_batch(batch, graph=True, max_slots=None, budget=200e9, poll_every=30, tune_cycles=12):
+    """All cases in one ensemble, timed end to end: the creation of every case (device memory and
+    initial state), the autotuner's timed iterations (the runs' own iterations), every other
+    iteration, the completion polls and the release.  Returns (seconds, per-case iterations,
+    per-case statuses, per-case completion times from the start, persistent blocks per CU)."""
     from .lib import Ensemble
     ncyc = batch[0].max_cycles
+    t0 = time.perf_counter()
     E = Ensemble(max_slots or len(batch), budget)
     slots = [E.add(c, max_iterations=ncyc) for c in batch]
     assert all(s is not None for s in slots), "batch exceeds the HBM budget"
-    E.step(6, graph=graph)  # activation + graph capture outside the timed region? no: part of it
-    E.synchronize()
-    t0 = time.perf_counter()
-    done = 6
-    while done < ncyc:
-        k = min(poll_every, ncyc - done)
-        E.step(k, graph=graph)
-        done += k
-    E.synchronize()
-    dt = time.perf_counter() - t0
-    n, st, tm, it = E.poll()
+    bpc = E.autotune(tune_cycles)[0] if tune_cycles else None
+    done = {}
+    while len(done) < len(slots):
+        E.step(poll_every, graph=graph)
+        n, st, tm, it = E.poll()
+        now = time.perf_counter() - t0
+        for i, s in enumerate(slots):
+            if i not in done and (st[s] == T.MH_STOPPED or st[s] < 0):
+                done[i] = now
+    iters = [int(it[s]) for s in slots]
+    stats = [int(st[s]) for s in slots]
     E.close()
-    return dt, it[:len(batch)], st[:len(batch)]
+    return time.perf_counter() - t0, iters, stats, [done[i] for i in range(len(slots))], bpc
 
 
 def run_back_to_back(batch, graph=6):
-    """Each case alone on the GPU, one after the other (the single-mesh path)."""
+    """Each case alone on the GPU, one after the other (the single-mesh path), timed on the
+    same basis as run_batch: creation, graph capture, iterations and release of every case.
+    Returns (seconds, per-case completion times from the start)."""
     from .lib import GpuCase
     t0 = time.perf_counter()
-    lat = []
+    done = []
     for c in batch:
-        t1 = time.perf_counter()
         g = GpuCase(c)
         g.run(c.max_cycles, graph=graph)
         g.close()
-        lat.append(time.perf_counter() - t1)
-    return time.perf_counter() - t0, lat
+        done.append(time.perf_counter() - t0)
+    return time.perf_counter() - t0, done
 
 
 @dataclass

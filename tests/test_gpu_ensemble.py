@@ -229,3 +229,68 @@ def test_served_stream_alone_equals_mixed():
             assert np.array_equal(g.get(f), got[i][f], equal_nan=True), (c.name, f)
         assert np.array_equal(g.get(T.MH_N_VELOCITY, -1, 0), got[i]["u"]), c.name
         g.close()
+
+
+def test_admission_keeps_the_captured_graph():
+    """A case joining a batch group that already runs changes only the device-side work lists:
+    no new graph is captured, and the late case still equals its solo run."""
+    from mahyco_amd.lib import Ensemble
+    mix = _mix(9, n=6)
+    E = Ensemble(8, 4e9)
+    for c in mix[:3]:
+        assert E.add(c, max_iterations=48) is not None
+    E.step(12)
+    captures = E.graph_captures
+    assert captures >= 1
+    late = E.add(mix[3], max_iterations=24)
+    E.step(12)
+    E.remove(0)
+    E.step(18)
+    assert E.graph_captures == captures
+    n, st, tm, it = E.poll()
+    assert st[late] == T.MH_STOPPED and it[late] == 24
+    ref = _alone(mix[3], 24)
+    assert tm[late] == ref["t"]
+    for f in (T.MH_F_DENSITY, T.MH_F_PRESSURE, T.MH_F_INTERNAL_ENERGY, T.MH_F_FRACVOL):
+        assert np.array_equal(E.get(late, f), ref[f]), f
+    E.close()
+
+
+def test_ensemble_autotune_same_bits():
+    """The ensemble autotuner times its candidate grids on the cases' own iterations; the runs
+    end with the bits of their solo runs."""
+    from mahyco_amd.lib import Ensemble
+    mix = _mix(13, n=5, dim3=True, stiff=True)
+    ncyc = 96
+    E = Ensemble(8, 4e9)
+    slots = [E.add(c, max_iterations=ncyc) for c in mix]
+    bpc, ms = E.autotune(6)
+    assert bpc in (2, 4, 8, 16) and ms > 0
+    E.step(ncyc)
+    n, st, tm, it = E.poll()
+    for s, c in zip(slots, mix):
+        assert st[s] == T.MH_STOPPED and it[s] == ncyc, c.name
+        ref = _alone(c, ncyc)
+        assert tm[s] == ref["t"], c.name
+        for f in (T.MH_F_DENSITY, T.MH_F_PRESSURE, T.MH_F_INTERNAL_ENERGY):
+            assert np.array_equal(E.get(s, f), ref[f]), (c.name, f)
+    E.close()
+
+
+def test_case_autotune_same_bits():
+    """The single-case autotuner (kernel variants and tile depths, timed on the case's own
+    iterations) leaves the results bit for bit those of the default kernels."""
+    from mahyco_amd.lib import GpuCase
+    from tests.helpers import bitwise_equal, fields
+    case = cases.config2_bisod3d(40, 20, 18, cycles=100000)
+    a, b = GpuCase(case), GpuCase(case)
+    v, ms = b.autotune(6)
+    n = b.iteration + 12
+    assert a.run(n, graph=6) == T.MH_OK
+    assert b.run(12, graph=6) == T.MH_OK
+    assert a.iteration == b.iteration == n
+    fa, fb = fields(a, 2, 3), fields(b, 2, 3)
+    for k in fa:
+        assert bitwise_equal(fa[k], fb[k]), (v, k)
+    a.close()
+    b.close()

@@ -219,6 +219,23 @@ def test_node_kernel_tiled_equals_flat(monkeypatch, shape):
         assert bitwise_equal(res["flat"][k], res["tiled"][k]), k
 
 
+@pytest.mark.parametrize("shape", [(70, 9, 19), (64, 4, 8), (5, 33, 17)])
+def test_cell_kernel_tiled_equals_flat(monkeypatch, shape):
+    """The LDS-staged z-marching cell kernel (an autotuner candidate) and the one-thread-per-
+    cell kernel give identical bits (sizes leave partial tiles in x, y and z)."""
+    from mahyco_amd.lib import GpuCase
+    res = {}
+    for mode in ("flat", "tiled"):
+        monkeypatch.setenv("MAHYCO_CELL_KERNEL", mode)
+        c = cases.config2_bisod3d(*shape)
+        g = GpuCase(c)
+        g.run(20)
+        res[mode] = fields(g, 2, 3)
+        g.close()
+    for k in res["flat"]:
+        assert bitwise_equal(res["flat"][k], res["tiled"][k]), k
+
+
 @pytest.mark.parametrize("make", [lambda: cases.config2_bisod3d(13, 37, 21), lambda: cases.config2_bisod3d(9, 2, 40),
                                   lambda: cases.config2_bisod3d(7, 17, 3), lambda: cases.config2_bisod3d(130, 5, 4),
                                   lambda: cases.config2_bisod3d(70, 35, 18), lambda: cases.bitriplepoint(),

@@ -279,7 +279,11 @@ def bench_batched(args):
            "back_to_back_s": t_b2b, "back_to_back_runs_per_s": len(batch) / t_b2b,
            "speedup_vs_back_to_back": t_b2b / t_batch,
            "p50_s": _pct(done, 50), "p99_s": _pct(done, 99),
-           "back_to_back_p50_s": _pct(done_b2b, 50), "back_to_back_p99_s": _pct(done_b2b, 99)}
+           "back_to_back_p50_s": _pct(done_b2b, 50), "back_to_back_p99_s": _pct(done_b2b, 99),
+           "batched_phases_s": serve.run_batch.phases,
+           "back_to_back_iterate_s": serve.run_back_to_back.iterate_s,
+           "speedup_iterations_only": serve.run_back_to_back.iterate_s / (serve.run_batch.phases["autotune_s"] +
+                                                                         serve.run_batch.phases["iterate_s"])}
     print(json.dumps(out))
 
 

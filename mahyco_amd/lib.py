@@ -36,6 +36,12 @@ def load():
     lib.mahyco_create_on_stream.argtypes = [C.POINTER(T.MhMesh), C.c_void_p, C.c_int, C.POINTER(T.MhOptions),
                                             C.c_void_p, C.POINTER(C.c_void_p)]
     lib.mahyco_create_on_stream.restype = C.c_int
+    lib.mahyco_apply_one_cell_eos.argtypes = [H, C.c_int, C.c_long]
+    lib.mahyco_apply_one_cell_eos.restype = C.c_int
+    lib.mahyco_ensemble_poll_async.argtypes = [C.c_void_p]
+    lib.mahyco_ensemble_poll_async.restype = C.c_int
+    lib.mahyco_ensemble_poll_result.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.mahyco_ensemble_poll_result.restype = C.c_int
     lib.mahyco_autotune.argtypes = [H, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     lib.mahyco_autotune.restype = C.c_int
     lib.mahyco_ensemble_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
@@ -346,6 +352,22 @@ class Ensemble:
         tm = np.zeros(self.max_slots)
         it = np.zeros(self.max_slots, dtype=np.int32)
         n = self.lib.mahyco_ensemble_poll(self.e, st.ctypes.data, tm.ctypes.data, it.ctypes.data)
+        return n, st, tm, it
+
+    def poll_async(self):
+        """Queues a snapshot of every slot's scalars behind the queued work (no wait)."""
+        st = self.lib.mahyco_ensemble_poll_async(self.e)
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_ensemble_poll_async")
+
+    def poll_result(self):
+        """Waits for the last poll_async snapshot only; same result as poll()."""
+        st = np.zeros(self.max_slots, dtype=np.int32)
+        tm = np.zeros(self.max_slots)
+        it = np.zeros(self.max_slots, dtype=np.int32)
+        n = self.lib.mahyco_ensemble_poll_result(self.e, st.ctypes.data, tm.ctypes.data, it.ctypes.data)
+        if n < 0:
+            raise MahycoError(n, "mahyco_ensemble_poll_result")
         return n, st, tm, it
 
     def get(self, slot, field, env=-1, comp=0):

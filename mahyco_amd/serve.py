@@ -56,7 +56,10 @@ def run_batch(batch, graph=True, max_slots=None, budget=200e9, poll_every=30, tu
     E = Ensemble(max_slots or len(batch), budget)
     slots = [E.add(c, max_iterations=ncyc) for c in batch]
     assert all(s is not None for s in slots), "batch exceeds the HBM budget"
+    E.synchronize()
+    phases = {"create_s": time.perf_counter() - t0}
     bpc = E.autotune(tune_cycles)[0] if tune_cycles else None
+    phases["autotune_s"] = time.perf_counter() - t0 - phases["create_s"]
     done = {}
     while len(done) < len(slots):
         E.step(poll_every, graph=graph)
@@ -67,7 +70,11 @@ def run_batch(batch, graph=True, max_slots=None, budget=200e9, poll_every=30, tu
                 done[i] = now
     iters = [int(it[s]) for s in slots]
     stats = [int(st[s]) for s in slots]
+    t_run = time.perf_counter()
     E.close()
+    phases["iterate_s"] = t_run - t0 - phases["create_s"] - phases["autotune_s"]
+    phases["release_s"] = time.perf_counter() - t_run
+    run_batch.phases = phases
     return time.perf_counter() - t0, iters, stats, [done[i] for i in range(len(slots))], bpc
 
 
@@ -78,11 +85,16 @@ def run_back_to_back(batch, graph=6):
     from .lib import GpuCase
     t0 = time.perf_counter()
     done = []
+    iterate = 0.
     for c in batch:
         g = GpuCase(c)
+        g.synchronize()
+        t1 = time.perf_counter()
         g.run(c.max_cycles, graph=graph)
+        iterate += time.perf_counter() - t1
         g.close()
         done.append(time.perf_counter() - t0)
+    run_back_to_back.iterate_s = iterate  # the iterations alone (graph captures included)
     return time.perf_counter() - t0, done
 
 
@@ -110,6 +122,10 @@ def serve(reqs, budget=64e9, max_slots=64, step_cycles=6, graph=True, on_done=No
     live = {}
     t0 = time.perf_counter() - reqs[0][0]  # the clock starts at the first arrival
     now = lambda: time.perf_counter() - t0
+    # Slot states are read one step late: snapshot j is queued after step j and read after
+    # step j+1, so a slot admitted after snapshot j was queued is not in it (admitted_at).
+    nsnap = 0  # snapshots queued so far
+    admitted_at = {}
     while queue or live:
         t = now()
         # admit every arrived request that fits (FIFO, by HBM footprint and slots)
@@ -127,6 +143,7 @@ def serve(reqs, budget=64e9, max_slots=64, step_cycles=6, graph=True, on_done=No
             recs[i].slot = slot
             recs[i].live_at_admission = len(live)
             live[slot] = i
+            admitted_at[slot] = nsnap
             queue.pop(0)
         if not live:
             if not queue:
@@ -136,9 +153,19 @@ def serve(reqs, budget=64e9, max_slots=64, step_cycles=6, graph=True, on_done=No
             if wait > 0:
                 time.sleep(min(wait, 0.05))
             continue
+        # queue the next iterations, then read the state as it was before them: the GPU works
+        # while the host retires and admits
         E.step(step_cycles, graph=graph)
-        n, st, tm, it = E.poll()
+        snap = nsnap - 1  # the snapshot in flight, if any
+        if snap >= 0:
+            n, st, tm, it = E.poll_result()
+        E.poll_async()
+        nsnap += 1
+        if snap < 0:
+            continue
         for slot in list(live):
+            if admitted_at[slot] > snap:
+                continue  # admitted after that snapshot was queued
             if st[slot] == T.MH_STOPPED or st[slot] < 0:
                 i = live.pop(slot)
                 recs[i].done = now()

@@ -110,6 +110,19 @@ int mahyco_update_energy_and_pressure(MhHandle* h);
 int mahyco_remap(MhHandle* h);
 int mahyco_compute_delta_t(MhHandle* h);
 
+/* ComputeDeltaT on caller-owned device arrays, allocating nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops).  L and c are the characteristic length and sound speed per cell, u0/u1/u2 the
+ * node velocity (u2 unused in 2D); the step, with the rules of MahycoModule.cc:1739-1833 (cfl,
+ * deltat_min and deltat_max of options, growth capped at 1.05 old_deltat), and the status
+ * (MH_OK or MH_ERR_DT_MIN) go to the device words dt_out and status_out, ordered on `stream`
+ * (a hipStream_t as void*, null for the null stream).  The per-block minima live in a static
+ * device array of 65536 blocks: meshes over 16,777,216 cells are refused with MH_ERR_INVALID.
+ * Calls that overlap on one GPU share that array and race. */
+int mahyco_compute_delta_t_noalloc(const MhMesh* mesh, const MhOptions* options, double old_deltat,
+                                   const double* L, const double* c, const double* u0, const double* u1,
+                                   const double* u2, double* dt_out, int* status_out, void* stream);
+
 /* IEquationOfState::applyEOS(env) (IEquationOfState.h:45-49; PerfectGasEOSService.cc:45-67,
  * StiffenedGasEOSService.cc:43-63): pressure and sound speed of environment env from its
  * density and internal energy.  Also run inside mahyco_remap / the Lagrange-only cycle. */

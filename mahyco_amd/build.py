@@ -27,7 +27,7 @@ def hipcc():
 
 # translation units: (source, extra flags, object); the sweep unit is compiled once per
 # environment count so that the units build in parallel
-UNITS = [("mahyco.hip", [], "mahyco.o"), ("mh_lag.hip", [], "mh_lag.o")] + [
+UNITS = [("mahyco.hip", [], "mahyco.o"), ("mh_lag.hip", [], "mh_lag.o"), ("mh_noalloc.hip", [], "mh_noalloc.o")] + [
     ("mh_sweep.hip", [f"-DMH_TU_M={m}"], f"mh_sweep_m{m}.o") for m in (1, 2, 3, 4)]
 
 

@@ -152,7 +152,9 @@ typedef struct MhOptions {
      574-690, 1661-1697; PrepareRemap.cc:210, 236-240) */
   int schema_csts;
   int longueur_caracteristique;  /* MhCaracteristicLength (Mahyco.axl longueur-caracteristique) */
-  int pad_lc;
+  /* conservation-energie-totale (RemapADI.axl:178): the kinetic energy the dual remap loses is put
+     back into the cells' internal energy (RemapADIFinal.cc:318-345); not with sans_lagrange */
+  int conservation_energie_totale;
 } MhOptions;
 
 /* Field identifiers for get/set.  Cell fields take an environment index

@@ -83,6 +83,7 @@ class Case:
     pression_explicite: bool = False
     schema_csts: bool = False
     longueur_caracteristique: int = 0  # MhCaracteristicLength
+    conservation_energie_totale: bool = False
     sans_lagrange: bool = False
     rider_reverse: bool = False
     rider_parametre: float = 0.0
@@ -157,6 +158,7 @@ class Case:
         o.pression_explicite = int(self.pression_explicite)
         o.schema_csts = int(self.schema_csts)
         o.longueur_caracteristique = int(self.longueur_caracteristique)
+        o.conservation_energie_totale = int(self.conservation_energie_totale)
         o.sans_lagrange = int(self.sans_lagrange)
         o.rider_reverse = int(self.rider_reverse)
         o.rider_parametre = self.rider_parametre

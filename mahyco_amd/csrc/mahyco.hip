@@ -24,7 +24,7 @@ namespace {
 const char* kKernelNames[] = {"k_node_lag",     "k_cell_lag",     "k_sweep_cell_x",  "k_sweep_cell_y",
                               "k_sweep_cell_z", "k_sweep_node_x", "k_sweep_node_y",  "k_sweep_node_z",
                               "k_node_velocity", "k_finalize",    "k_dt",            "k_hcell",
-                              "k_node_fwd"};
+                              "k_node_fwd",     "k_sweep_ke",     "k_cons_ec"};
 
 }  // namespace
 
@@ -524,6 +524,8 @@ int validate(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const Mh
   // CSTS: the closed-form energy (with Newton the reference reads p before setting it,
   // MahycoModule.cc:1595-1601; explicit pressure applies the EOS before the forward step)
   if (o->schema_csts && (o->with_newton || o->pression_explicite || o->sans_lagrange)) return MH_ERR_UNSUPPORTED;
+  // total-energy conservation reads the dual remap's kinetic energy, which advection mode does not run
+  if (o->conservation_energie_totale && o->sans_lagrange) return MH_ERR_UNSUPPORTED;
   if (o->longueur_caracteristique < 0 || o->longueur_caracteristique > MH_L_MONODIM_Z) return MH_ERR_INVALID;
   if (o->longueur_caracteristique == MH_L_FACES_OPPOSEES && mesh->dim != 3) return MH_ERR_UNSUPPORTED;  // 8 nodes
   if (o->cas_model == MH_CASE_RIEMANN && nb_env < 2) return MH_ERR_INVALID;
@@ -566,6 +568,7 @@ size_t case_footprint(const MhMesh& mesh, int m, const MhOptions& o) {
     if (o.ordre_projection == 3 || o.projection_pente_borne || gen_lim) words += (long)mesh.dim * nc;  // hc
     if (o.ordre_projection == 3 || o.pente_borne_mixte) bytes += nc;                                  // est
     if (o.projection_pente_borne) words += nv * nc;                                                   // phi0
+    if (o.conservation_energie_totale) words += 3 * nn + nc;                                          // ke_*, dec
     if (o.sans_lagrange && o.deltat_init > 0.) words += 3 * nn + (long)(o.final_time / o.deltat_init) + 8;
   }
   if (o.schema_csts) {  // saved velocity, pseudo-viscosity history
@@ -621,8 +624,8 @@ std::map<std::string, int> g_tune;  // key -> choice
 
 std::string group_key(const EnsGroup& g) {
   char b[128];
-  snprintf(b, sizeof b, "ens:%d,%d,%d,%d,%d,%d,%d,%d,%d,%d", g.dim, g.m, g.o3, g.gen, g.csts, g.adv, g.lc, g.rows,
-           g.ntile, g.lim);
+  snprintf(b, sizeof b, "ens:%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d", g.dim, g.m, g.o3, g.gen, g.csts, g.adv, g.lc, g.rows,
+           g.ntile, g.lim, g.cons);
   return b;
 }
 int tune_lookup(const std::string& k) {
@@ -677,10 +680,11 @@ int ens_group_of(MhEnsembleImpl* E, MhHandle* h) {
   k.rows = !k.o3 && !k.gen && h->sweep_mode == MH_SWEEP_TILE;
   k.ntile = k.dim == 3 && h->node_tiled && !k.adv;
   k.lim = (k.o3 || k.gen) ? -1 : (h->dev.order2 ? h->dev.limiter : 3);
+  k.cons = h->dev.dec != nullptr;
   for (size_t i = 0; i < E->groups.size(); ++i) {
     const EnsGroup& g = E->groups[i];
     if (g.dim == k.dim && g.m == k.m && g.o3 == k.o3 && g.gen == k.gen && g.csts == k.csts && g.adv == k.adv &&
-        g.lc == k.lc && g.rows == k.rows && g.ntile == k.ntile && g.lim == k.lim)
+        g.lc == k.lc && g.rows == k.rows && g.ntile == k.ntile && g.lim == k.lim && g.cons == k.cons)
       return (int)i;
   }
   const int cached = tune_lookup(group_key(k));
@@ -1053,6 +1057,12 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
       ok |= dalloc(h, &S.hc, (size_t)g.dim * nc);
     if (options->ordre_projection == 3 || options->pente_borne_mixte) ok |= dalloc(h, &S.est, (size_t)nc);
     if (options->projection_pente_borne) ok |= dalloc(h, &S.phi0, (size_t)nv * nc);
+    if (options->conservation_energie_totale) {  // dual kinetic energy, its phi, the energy returned
+      ok |= dalloc(h, &S.ke_U, nn);
+      ok |= dalloc(h, &S.ke_Pa, nn);
+      ok |= dalloc(h, &S.ke_Pb, nn);
+      ok |= dalloc(h, &S.dec, nc);
+    }
     if (options->sans_lagrange) {  // prescribed velocity and the reverse factors
       for (int d = 0; d < 3; ++d) ok |= dalloc(h, &h->u0[d], nn);
       ok |= dalloc(h, &h->rev_cos, I.rev_cos.size());

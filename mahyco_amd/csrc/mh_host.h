@@ -27,6 +27,8 @@ enum {
   KN_DT,
   KN_HCELL,  // cell widths of the order-3 remap
   KN_NODE_FWD,  // CSTS forward velocity step before the remap
+  KN_SWEEP_KE,  // dual kinetic energy of a sweep (conservation-energie-totale)
+  KN_CONS_EC,   // the kinetic energy each cell receives back (conservation-energie-totale)
   KN_COUNT
 };
 
@@ -179,6 +181,7 @@ struct EnsGroup {
   bool lc = false;   // characteristic length other than V^(1/d): k_cell_length first
   bool rows = false; // row-segment x sweeps and LDS-tiled y / z cell sweeps (else one thread per item)
   bool ntile = false;  // 3D: z-marching node kernel
+  bool cons = false;   // conservation-energie-totale: dual kinetic energy and its return to the cells
   int lim = -1;
   int bpc = 4;         // persistent blocks per CU of the group's launches (autotuned)
   std::vector<int> slots;

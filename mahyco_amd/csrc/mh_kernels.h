@@ -2319,12 +2319,48 @@ __device__ __forceinline__ void k_sweep_cell_tile_body(const MhDev& S, int lb_, 
 // (DualPhaseRemap.cc:75-108), upwind momentum update (:110-239), filter and phi (:241-257).
 // Nodes with no neighbour along D keep their values.  On the last sweep the node velocity
 // u = mu / m is formed for every node (RemapADIFinal.cc:286-297).
+// The mass fluxes through an inner node's back and front dual faces along D
+// (DualPhaseRemap.cc:75-108): 1/2 (2D) or 1/4 (3D) of the dual mass fluxes of the cells
+// behind / ahead of the node, over those cells in face order and the environments in order.
+template <int DIM, int D>
+__device__ __forceinline__ void dual_mass_fluxes(const MhDev& S, int i, int j, int k, int a, double& back,
+                                                 double& front) {
+  const double* __restrict__ dflux = S.dflux;
+  const long nc = S.g.nc;
+  const double w = DIM == 3 ? 0.25 : 0.5;
+  const int M = S.m;
+  back = 0.;
+  front = 0.;
+  const long sD = D == 0 ? 1 : D == 1 ? (long)S.g.nx : (long)S.g.nx * S.g.ny;
+  // transverse axes (t1 fastest, t2 slowest) and their cell counts
+  const int t1 = D == 0 ? 1 : 0;
+  const int t2 = D == 2 ? 1 : 2;
+  const int it1 = t1 == 0 ? i : j;
+  const int it2 = t2 == 1 ? j : k;
+  const int n1 = t1 == 0 ? S.g.nx : S.g.ny;
+  const int n2 = t2 == 1 ? S.g.ny : S.g.nz;
+  const long st1 = t1 == 0 ? 1 : (long)S.g.nx;
+  const long st2 = t2 == 1 ? (long)S.g.nx : (long)S.g.nx * S.g.ny;
+  const int lo2 = DIM == 3 ? it2 - 1 : 0, hi2 = DIM == 3 ? it2 : 0;
+  // cell at position a along D and (p1, p2) along the transverse axes
+  const long cbase = D == 0 ? (long)a : D == 1 ? (long)a * S.g.nx : (long)a * S.g.nx * S.g.ny;
+  for (int p2 = lo2; p2 <= hi2; ++p2) {
+    if (DIM == 3 && (p2 < 0 || p2 >= n2)) continue;
+    for (int p1 = it1 - 1; p1 <= it1; ++p1) {
+      if (p1 < 0 || p1 >= n1) continue;
+      const long cf = cbase + p1 * st1 + (DIM == 3 ? p2 * st2 : 0);
+      const long cb = cf - sD;
+      for (int kk = 0; kk < M; ++kk) back += w * dflux[kk * nc + cb];
+      for (int kk = 0; kk < M; ++kk) front += w * dflux[kk * nc + cf];
+    }
+  }
+}
+
 template <int DIM, int D, int LIM>
 __device__ __forceinline__ void k_sweep_node_at(const MhDev& S, long n, int psel, int first, int last) {
   double* __restrict__ Ud = S.Ud;
   const double* __restrict__ Pin = psel ? S.Pa : S.Pb;
   double* __restrict__ Pout = psel ? S.Pb : S.Pa;
-  const double* __restrict__ dflux = S.dflux;
   constexpr int NC = DIM;  // velocity components carried
   if (mh_halted(S.sc)) return;
   const int nnx = S.g.nnx, nny = S.g.nny;
@@ -2335,7 +2371,7 @@ __device__ __forceinline__ void k_sweep_node_at(const MhDev& S, long n, int psel
   const int a = D == 0 ? i : D == 1 ? j : k;
   const int na = D == 0 ? S.g.nx : D == 1 ? S.g.ny : S.g.nz;
   const long nD = D == 0 ? 1 : D == 1 ? (long)nnx : (long)nnx * nny;
-  const long nn = S.g.nn, nc = S.g.nc;
+  const long nn = S.g.nn;
   double U[NC + 1];
   if (first) {
     const double mass = S.mn[n];
@@ -2405,33 +2441,8 @@ __device__ __forceinline__ void k_sweep_node_at(const MhDev& S, long n, int psel
       G[o + 1][cc] = g;
     }
   }
-  // back / front mass fluxes over the faces of direction D through this node
-  const double w = DIM == 3 ? 0.25 : 0.5;
-  const int M = S.m;
-  double back = 0., front = 0.;
-  const long sD = D == 0 ? 1 : D == 1 ? (long)S.g.nx : (long)S.g.nx * S.g.ny;
-  // transverse axes (t1 fastest, t2 slowest) and their cell counts
-  const int t1 = D == 0 ? 1 : 0;
-  const int t2 = D == 2 ? 1 : 2;
-  const int it1 = t1 == 0 ? i : j;
-  const int it2 = t2 == 1 ? j : k;
-  const int n1 = t1 == 0 ? S.g.nx : S.g.ny;
-  const int n2 = t2 == 1 ? S.g.ny : S.g.nz;
-  const long st1 = t1 == 0 ? 1 : (long)S.g.nx;
-  const long st2 = t2 == 1 ? (long)S.g.nx : (long)S.g.nx * S.g.ny;
-  const int lo2 = DIM == 3 ? it2 - 1 : 0, hi2 = DIM == 3 ? it2 : 0;
-  // cell at position a along D and (p1, p2) along the transverse axes
-  const long cbase = D == 0 ? (long)a : D == 1 ? (long)a * S.g.nx : (long)a * S.g.nx * S.g.ny;
-  for (int p2 = lo2; p2 <= hi2; ++p2) {
-    if (DIM == 3 && (p2 < 0 || p2 >= n2)) continue;
-    for (int p1 = it1 - 1; p1 <= it1; ++p1) {
-      if (p1 < 0 || p1 >= n1) continue;
-      const long cf = cbase + p1 * st1 + (DIM == 3 ? p2 * st2 : 0);
-      const long cb = cf - sD;
-      for (int kk = 0; kk < M; ++kk) back += w * dflux[kk * nc + cb];
-      for (int kk = 0; kk < M; ++kk) front += w * dflux[kk * nc + cf];
-    }
-  }
+  double back, front;
+  dual_mass_fluxes<DIM, D>(S, i, j, k, a, back, front);
   U[NC] += back - front;
   const double o2 = (double)S.order2;
   const double ufront = 0.5 * (P[2][D] + P[3][D]);
@@ -2500,6 +2511,114 @@ __device__ __forceinline__ void k_node_velocity_body(const MhDev& S, int lb, int
   }
   if (U[DIM] != 0.)
     for (int cc = 0; cc < DIM; ++cc) S.u[cc][n] = U[cc] / U[DIM];
+}
+
+// ------------------------------------------------- conservation-energie-totale (RemapADI.axl:178)
+// The dual kinetic energy of one sweep along D, run after the node sweep of that direction:
+// dual U = 1/2 m |u|^2 and phi = 1/2 |u|^2 at the first sweep (PrepareRemap.cc:179-190), then
+// for inner nodes the upwind phi through the back and front dual faces with the node sweep's
+// mass fluxes and upwind choices (DualPhaseRemap.cc:132-238).  Its limited gradient is never
+// formed (UtilesRemap.cc:369-400 covers the velocity only), so the slope term multiplies 0.
+// phi = U / the node mass the node sweep has just written (:251-257).
+template <int DIM, int D>
+__device__ __forceinline__ void k_sweep_ke_body(const MhDev& S, int lb, int psel, int first) {
+  const long n = lb * (long)blockDim.x + threadIdx.x;
+  if (n >= S.g.nn) return;
+  if (mh_halted(S.sc)) return;
+  const double* __restrict__ Pin = psel ? S.Pa : S.Pb;
+  const double* __restrict__ Kin = psel ? S.ke_Pa : S.ke_Pb;
+  double* __restrict__ Kout = psel ? S.ke_Pb : S.ke_Pa;
+  const long nn = S.g.nn;
+  const int nnx = S.g.nnx, nny = S.g.nny;
+  const int i = (int)(n % nnx);
+  const long r = n / nnx;
+  const int j = (int)(r % nny);
+  const int k = (int)(r / nny);
+  const int a = D == 0 ? i : D == 1 ? j : k;
+  const int na = D == 0 ? S.g.nx : D == 1 ? S.g.ny : S.g.nz;
+  const long nD = D == 0 ? 1 : D == 1 ? (long)nnx : (long)nnx * nny;
+  auto sq = [&](long m2) {
+    const double ux = S.u[0][m2], uy = S.u[1][m2];
+    double v = ux * ux + uy * uy;
+    if (DIM == 3) v = v + S.u[2][m2] * S.u[2][m2];
+    return v;
+  };
+  auto phi = [&](long m2) { return first ? 0.5 * sq(m2) : Kin[m2]; };
+  double U = first ? 0.5 * S.mn[n] * sq(n) : S.ke_U[n];
+  if (!(a >= 1 && a <= na - 1)) {
+    S.ke_U[n] = U;
+    Kout[n] = phi(n);
+    return;
+  }
+  double back, front;
+  dual_mass_fluxes<DIM, D>(S, i, j, k, a, back, front);
+  const double dt = S.sc->dt;
+  const double o2 = (double)S.order2;
+  const double zero_slope = 0.;
+  auto vel = [&](long m2) { return first ? S.u[D][m2] : Pin[D * nn + m2]; };
+  const double xb = S.xlc[D][n - nD], xn = S.xlc[D][n], xf = S.xlc[D][n + nD];
+  const double ufront = 0.5 * (vel(n) + vel(n + nD));
+  const int sf = ufront > 0 ? 1 : -1;
+  const double FE = phi(front < 0. ? n + nD : n) + o2 * (0.5 * zero_slope * (sf * (xf - xn) - dt * ufront));
+  const double uback = 0.5 * (vel(n - nD) + vel(n));
+  const int sb = uback > 0 ? 1 : -1;
+  const double BE = phi(back > 0. ? n - nD : n) + o2 * (0.5 * zero_slope * (sb * (xn - xb) - dt * uback));
+  U += back * BE - front * FE;
+  S.ke_U[n] = U;
+  Kout[n] = U / S.Ud[DIM * nn + n];
+}
+
+// The dual state of a remap with no sweep (every direction one cell wide): U = (m u, m) and
+// the kinetic energy 1/2 m |u|^2, untouched by any sweep (PrepareRemap.cc:169-190).
+template <int DIM>
+__device__ __forceinline__ void k_dual_init_body(const MhDev& S, int lb) {
+  const long n = lb * (long)blockDim.x + threadIdx.x;
+  if (n >= S.g.nn) return;
+  if (mh_halted(S.sc)) return;
+  const long nn = S.g.nn;
+  const double mass = S.mn[n];
+  double v = 0.;
+  for (int cc = 0; cc < DIM; ++cc) {
+    const double uc = S.u[cc][n];
+    S.Ud[cc * nn + n] = mass * uc;
+    v = cc == 0 ? uc * uc : v + uc * uc;
+  }
+  S.Ud[DIM * nn + n] = mass;
+  S.ke_U[n] = 0.5 * mass * v;
+}
+
+// The kinetic energy each cell gets back (RemapADIFinal.cc:318-339), once the node velocity
+// is final: over the cell's nodes in cell order, 0.25 (also in 3D) times the projected
+// minus the reconstructed kinetic energy per unit mass, for nodes with a non-zero projected
+// mass; kept if positive (std::max(0., .)).  k_finalize adds it to the internal energies.
+template <int DIM>
+__device__ __forceinline__ void k_cons_ec_body(const MhDev& S, int lb) {
+  const long c = lb * (long)blockDim.x + threadIdx.x;
+  if (c >= S.g.nc) return;
+  if (mh_halted(S.sc)) return;
+  const long nn = S.g.nn;
+  const int i = (int)(c % S.g.nx);
+  const long r = c / S.g.nx;
+  const int j = (int)(r % S.g.ny);
+  const int k = (int)(r / S.g.ny);
+  const long nnx = S.g.nnx, nnxy = (long)S.g.nnx * S.g.nny;
+  const long b = i + nnx * (j + (long)S.g.nny * k);
+  const long off[8] = {0, 1, 1 + nnx, nnx, nnxy, 1 + nnxy, 1 + nnx + nnxy, nnx + nnxy};
+  double d = 0.;
+#pragma unroll
+  for (int l = 0; l < (DIM == 3 ? 8 : 4); ++l) {
+    const long n = b + off[l];
+    const double mp = S.Ud[DIM * nn + n];
+    if (mp != 0.) {
+      const double ux = S.u[0][n], uy = S.u[1][n];
+      double v = ux * ux + uy * uy;
+      if (DIM == 3) v = v + S.u[2][n] * S.u[2][n];
+      const double ec_proj = S.ke_U[n] / mp;
+      const double ec_reconst = 0.5 * v;
+      d += 0.25 * (ec_proj - ec_reconst);
+    }
+  }
+  S.dec[c] = (0. < d) ? d : 0.;
 }
 
 // ----------------------------------------------------------------------------- finalize
@@ -2650,6 +2769,21 @@ __device__ __forceinline__ void k_finalize_body(const MhDev& S, int lb, int usel
         }
       }
       if (dk[kk] != dk[kk] || ek[kk] != ek[kk]) S.err[0] = MH_ERR_NAN_REMAP;
+    }
+    if (S.dec) {  // conservation-energie-totale (RemapADIFinal.cc:340-344): the energy k_cons_ec
+                  // found, to each environment by its mass fraction (for a single-environment
+                  // cell that is the cell value) and to the value of any other cell
+      const double dec = S.dec[c];
+      for (int kk = 0; kk < M; ++kk) {
+        if (!((nm >> kk) & 1)) continue;
+        if (pure) {
+          e_g += chi[kk] * dec;
+          ek[kk] = e_g;
+        } else {
+          ek[kk] += chi[kk] * dec;
+        }
+      }
+      if (!pure) e_g += dec;
     }
     // applyEOS per environment, computePressionMoyenne (MahycoModule.cc:1708-1735)
     double p_g = 0., c_g = 0.;
@@ -3051,6 +3185,10 @@ MH_WRAP(k_sweep_cell_gen, 1, template <int DIM MH_COMMA int D MH_COMMA int M>, <
         MH_COMMA int usel MH_COMMA int first, MH_COMMA usel MH_COMMA first)
 MH_WRAP(k_sweep_cell3, 1, template <int DIM MH_COMMA int D MH_COMMA int M>, <DIM MH_COMMA D MH_COMMA M>,
         MH_COMMA int usel MH_COMMA int first, MH_COMMA usel MH_COMMA first)
+MH_WRAP(k_sweep_ke, 1, template <int DIM MH_COMMA int D>, <DIM MH_COMMA D>, MH_COMMA int psel MH_COMMA int first,
+        MH_COMMA psel MH_COMMA first)
+MH_WRAP(k_dual_init, 1, template <int DIM>, <DIM>, , )
+MH_WRAP(k_cons_ec, 1, template <int DIM>, <DIM>, , )
 
 // (the plain kernels are templates with an unused parameter, so that every translation unit
 // that launches one may instantiate it)

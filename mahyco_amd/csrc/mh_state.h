@@ -78,6 +78,9 @@ struct MhDev {
   // remap buffers: U ping-pong [nv][nc], mass fluxes [m][nc], dual state [4][nn] (in
   // place), dual phi ping-pong [3][nn]
   double *Ua, *Ub, *dflux, *Ud, *Pa, *Pb;
+  // conservation-energie-totale: the dual kinetic energy [nn] (in place), its phi ping-pong
+  // [nn] each, and the energy each cell receives back [nc]; null otherwise
+  double *ke_U, *ke_Pa, *ke_Pb, *dec;
   // cell widths per direction [dim][nc] (order 3, pente-borne, generalised limiters), and
   // the mixed-cell flag remapVariables leaves per cell (order 3, pente-borne-mixte):
   // MH_EST_MIXED, else the index of the one environment with a positive volume fraction,

@@ -80,8 +80,16 @@ void launch_sweeps_t(MhHandle* h) {
     } else {
       if constexpr (DIM == 3) launch_sweep_yz<3, 2, M>(h, sel, first, last);
     }
+    if (S.dec) {  // conservation-energie-totale: the dual kinetic energy of this sweep
+      if (D == 0) LAUNCH(h, KN_SWEEP_KE, (k_sweep_ke<DIM, 0>), h->nblk_n, MH_BLOCK, S, sel, first);
+      else if (D == 1) LAUNCH(h, KN_SWEEP_KE, (k_sweep_ke<DIM, 1>), h->nblk_n, MH_BLOCK, S, sel, first);
+      else if constexpr (DIM == 3) LAUNCH(h, KN_SWEEP_KE, (k_sweep_ke<3, 2>), h->nblk_n, MH_BLOCK, S, sel, first);
+    }
   }
-  if (nd <= 1 && !S.sans_lagrange) LAUNCH(h, KN_NODE_VEL, (k_node_velocity<DIM>), h->nblk_n, MH_BLOCK, S, nd == 0 ? 1 : 0);
+  if (S.dec && nd == 0) LAUNCH(h, KN_SWEEP_KE, (k_dual_init<DIM>), h->nblk_n, MH_BLOCK, S);
+  if (nd <= 1 && !S.sans_lagrange)
+    LAUNCH(h, KN_NODE_VEL, (k_node_velocity<DIM>), h->nblk_n, MH_BLOCK, S, nd == 0 && !S.dec ? 1 : 0);
+  if (S.dec) LAUNCH(h, KN_CONS_EC, (k_cons_ec<DIM>), h->nblk_c, MH_BLOCK, S);
   LAUNCH(h, KN_FINAL, (k_finalize<DIM, M>), h->nblk_c, MH_BLOCK, S, nd & 1);
   h->sens = (h->sens + 1) % dim;
 }
@@ -141,7 +149,14 @@ void ens_group_cycle_l(MhEnsembleImpl* E, const EnsGroup& g) {
     if (D == 0) ens_sweep<DIM, 0, M, LIM>(g, tab, st, G, sel, first, last);
     else if (D == 1) ens_sweep<DIM, 1, M, LIM>(g, tab, st, G, sel, first, last);
     else if constexpr (DIM == 3) ens_sweep<3, 2, M, LIM>(g, tab, st, G, sel, first, last);
+    if (g.cons) {  // the dual kinetic energy of this sweep
+      if (D == 0) hipLaunchKernelGGL((k_sweep_ke_batch<DIM, 0>), G, B, 0, st, tab, MH_LIST(g.nmap), sel, first);
+      else if (D == 1) hipLaunchKernelGGL((k_sweep_ke_batch<DIM, 1>), G, B, 0, st, tab, MH_LIST(g.nmap), sel, first);
+      else if constexpr (DIM == 3)
+        hipLaunchKernelGGL((k_sweep_ke_batch<3, 2>), G, B, 0, st, tab, MH_LIST(g.nmap), sel, first);
+    }
   }
+  if (g.cons) hipLaunchKernelGGL((k_cons_ec_batch<DIM>), G, B, 0, st, tab, MH_LIST(g.cmap));
   hipLaunchKernelGGL((k_finalize_batch<DIM, M>), G, B, 0, st, tab, MH_LIST(g.cmap), DIM & 1);
   hipLaunchKernelGGL((k_dt_batch<>), dim3((unsigned)E->ncu), dim3(MH_DT_BLOCK), 0, st, tab, g.d_list, g.d_nlist);
 }

@@ -108,8 +108,7 @@ void MahycoModule::hydroStartInit() {
   if (lc < 0) throw MahycoError(MH_ERR_UNSUPPORTED, "longueur-caracteristique " + opt_.longueur_caracteristique);
   RemapADI* adi = dynamic_cast<RemapADI*>(remap_.get());
   if (!adi) throw MahycoError(MH_ERR_UNSUPPORTED, "remap service other than RemapADI");
-  if (adi->isEuler() != "Full" || adi->hasConservationEnergieTotale())
-    throw MahycoError(MH_ERR_UNSUPPORTED, "RemapADI: is-euler-scheme other than Full, or total-energy conservation");
+  if (adi->isEuler() != "Full") throw MahycoError(MH_ERR_UNSUPPORTED, "RemapADI: is-euler-scheme other than Full");
   std::vector<MhEnvironment> ev;
   for (auto& e : envs_) {
     MhEnvironment d = e.eos->description();
@@ -142,6 +141,7 @@ void MahycoModule::hydroStartInit() {
   o.pente_borne_debar_fix = ro.projection_pente_borne_debar_fix;
   o.pente_borne_mixte = ro.projection_pente_borne_mixte;
   o.projection_simple_pente = ro.projection_simple_pente;
+  o.conservation_energie_totale = adi->hasConservationEnergieTotale();
   o.remap_threshold = ro.threshold;
   o.with_newton = opt_.with_newton;
   o.pression_explicite = opt_.pression_explicite;

@@ -98,4 +98,4 @@ class MhOptions(C.Structure):
                 ("projection_simple_pente", C.c_int), ("with_newton", C.c_int),
                 ("pression_explicite", C.c_int), ("sans_lagrange", C.c_int), ("rider_reverse", C.c_int),
                 ("rider_parametre", C.c_double), ("rider_rayon_bulle", C.c_double),
-                ("rider_densite_lineaire", C.c_int), ("schema_csts", C.c_int), ("longueur_caracteristique", C.c_int), ("pad_lc", C.c_int)]
+                ("rider_densite_lineaire", C.c_int), ("schema_csts", C.c_int), ("longueur_caracteristique", C.c_int), ("conservation_energie_totale", C.c_int)]

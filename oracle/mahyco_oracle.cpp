@@ -1977,6 +1977,28 @@ int OracleCase::remapVariables() {
     for (int k = 0; k < m; ++k)
       if (member(k, c)) ev(cell_volume, k, c) = ev(fracvol, k, c) * cell_volume.g[c];
   }
+  if (opt.conservation_energie_totale) {  // :318-345
+    // per cell, a quarter of (projected minus reconstructed) kinetic energy per unit mass
+    // over its nodes in cell order (0.25 in 3D as well), kept if positive, given to each
+    // environment by its mass fraction; a cell with other than one environment also adds
+    // it to its global value (a single-environment cell's value is the global one)
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < nc; ++c) {
+      double delta_ec = 0.;
+      for (int l = 0; l < nnpc; ++l) {
+        const long n = cell_nodes[c * nnpc + l];
+        if (u_dual[n * 5 + 3] != 0.) {
+          const double ec_proj = u_dual[n * 5 + 4] / u_dual[n * 5 + 3];
+          const double ec_reconst = 0.5 * squareNormL2(velocity[n]);
+          delta_ec += 0.25 * (ec_proj - ec_reconst);
+        }
+      }
+      delta_ec = std::max(0., delta_ec);
+      for (int k = 0; k < m; ++k)
+        if (member(k, c)) ev(internal_energy, k, c) += ev(mass_fraction, k, c) * delta_ec;
+      if (nbenv[c] != 1) internal_energy.g[c] += delta_ec;
+    }
+  }
   return MH_OK;
 }
 
@@ -2167,6 +2189,9 @@ int oracle_get_field(const OracleCase* hc, int field, int env, int comp_, double
       else if (field == MH_N_COORD) out[n] = comp(h->node_coord[n], comp_);
       else if (field == MH_N_COORD_LAGRANGE)
         out[n] = comp(h->opt.with_projection && !h->node_coord_lag.empty() ? h->node_coord_lag[n] : h->node_coord[n], comp_);
+      // the last remap's dual state (oracle only, for tests): comp 0-2 momentum, 3 mass, 4 kinetic energy
+      else if (field == ORACLE_N_DUAL_U && comp_ >= 0 && comp_ < 5)
+        out[n] = h->u_dual.empty() ? 0. : h->u_dual[n * 5 + comp_];
       else return MH_ERR_INVALID;
     }
     return MH_OK;

@@ -36,7 +36,9 @@ long oracle_nb_cells(const OracleCase* h);
 long oracle_nb_nodes(const OracleCase* h);
 
 /* Copies a field (MhField) to out.  Cell fields: env = -1 (cell value) or env index.
- * Node fields: comp = 0,1,2 for vectors. */
+ * Node fields: comp = 0,1,2 for vectors.  ORACLE_N_DUAL_U (the oracle's own id) reads the
+ * dual state the last remap left, comp 0-2 momentum, 3 mass, 4 kinetic energy. */
+#define ORACLE_N_DUAL_U 200
 int oracle_get_field(const OracleCase* h, int field, int env, int comp, double* out);
 
 /* Writes one field of the state with mahyco_set_field's meaning (env = -1 cell value, env = k

@@ -69,8 +69,9 @@ def _two_d(c, n):
 
 
 def _variant_mix():
-    """CSTS runs (2D and 3D) and 2D advection runs (RIDER bubble, ADVECTION square, with and
-    without time reversal) beside ordinary ones: four batch groups."""
+    """CSTS runs (2D and 3D), 2D advection runs (RIDER bubble, ADVECTION square, with and
+    without time reversal) and runs with total-energy conservation (2D, and 3D with CSTS)
+    beside ordinary ones: six batch groups."""
     rng = np.random.default_rng(7)
     out = []
     for i in range(3):
@@ -83,14 +84,20 @@ def _variant_mix():
     out.append(_two_d(cases.rider("r2", 34, .2), 40))
     out.append(_two_d(cases.rider("r2rev", 38, .2, 1.), 32))
     out.append(_two_d(cases.advection("a2", 21, .2, n=36), 36))
+    c = cases.riemann2d(28, 20, rng, stiffened=True, name="cons2d")
+    c.conservation_energie_totale = True
+    out.append(c)
+    c = cases.riemann3d(9, np.random.default_rng(3), stiffened=True, t_end=1e30, name="cons3d_csts")
+    c.conservation_energie_totale = c.schema_csts = True
+    out.append(c)
     return out
 
 
 @pytest.mark.parametrize("graph", [False, True])
 def test_variant_groups_alone_equal_mixed(graph):
-    """CSTS and advection-mode cases form their own batch groups (forward velocity step,
-    CSTS cell kernel; prescribed node velocity, no dual sweeps) and give the same bits as
-    alone."""
+    """CSTS, advection-mode and total-energy-conservation cases form their own batch groups
+    (forward velocity step, CSTS cell kernel; prescribed node velocity, no dual sweeps; dual
+    kinetic energy and its return) and give the same bits as alone."""
     from mahyco_amd.lib import Ensemble
     print(f"[ensemble] variants graph={graph}", flush=True)
     mix = _variant_mix()

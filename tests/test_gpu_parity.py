@@ -125,6 +125,20 @@ CASES = {
     "sod2d_lagrange_csts": (lambda: _variant(cases.config1_sod2d_lagrange(), schema_csts=True), 800),
     "riemann2d_lagrange_csts": (lambda: _variant(_riemann2d(True, 24), with_projection=False, schema_csts=True), 150),
     "riemann3d_lagrange_csts": (lambda: _variant(_riemann3d(25), with_projection=False, schema_csts=True), 60),
+    # conservation-energie-totale: the dual kinetic energy through every sweep and its return
+    # to the cells (RemapADIFinal.cc:318-345); one and two sweeps, orders 1-3, pente-borne,
+    # generalised limiters, CSTS, mixed cells
+    "bisod_x_conservation_fixture": (lambda: _variant(cases.bisod_x(), conservation_energie_totale=True), 400),
+    "bisod_x_one_sweep_conservation": (lambda: _variant(cases.bisod_x(40, 1, 1), conservation_energie_totale=True), 120),
+    "riemann2d_stiffened_conservation": (lambda: _variant(_riemann2d(True, 28), conservation_energie_totale=True), 150),
+    "riemann2d_conservation_ordre1": (lambda: _variant(_riemann2d(False, 31), conservation_energie_totale=True,
+                                                       ordre_projection=1), 150),
+    "bisod3d_conservation_ordre3": (lambda: _variant(cases.ordre3(lambda: cases.config2_bisod3d(32, 16, 16), "b3d_o3"),
+                                                     conservation_energie_totale=True), 100),
+    "riemann2d_conservation_pb_vanleerG": (lambda: _variant(_riemann2d(False, 30), conservation_energie_totale=True,
+                                                            pente_borne=True, debar_fix=1, limiter=T.MH_VANLEER_G), 150),
+    "riemann3d_conservation_csts": (lambda: _variant(_riemann3d(29), conservation_energie_totale=True,
+                                                     schema_csts=True), 60),
 }
 
 
@@ -185,6 +199,15 @@ def test_csts_unsupported_combinations_refused():
     for c in bad:
         with pytest.raises(Exception):
             GpuCase(c)
+
+
+def test_conservation_with_advection_refused():
+    """Total-energy conservation reads the dual remap's kinetic energy, which advection mode
+    does not run: the combination is refused at creation."""
+    from mahyco_amd.lib import GpuCase
+    c = _variant(cases.rider("r", 34, .2), conservation_energie_totale=True)
+    with pytest.raises(Exception):
+        GpuCase(c)
 
 
 def test_bubble_cases_need_two_environments():

@@ -320,7 +320,8 @@ RIDER_TESTS = {"MonoRiderTx": 23, "MonoRiderTy": 24, "MonoRiderT45": 25, "MonoRi
 
 
 def _fictif(rho):
-    return Env(densite_initiale=rho, pression_initiale=0., eos=T.MH_EOS_FICTIF, fictif_tdebut=0., fictif_tfin=10.)
+    return Env(densite_initiale=rho, pression_initiale=0., eos=T.MH_EOS_FICTIF, specific_heat=2.4, fictif_tdebut=0.,
+               fictif_tfin=10.)
 
 
 def rider(name, cas_test, final_time, reverse=None, bcs=None) -> Case:

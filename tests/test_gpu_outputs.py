@@ -54,6 +54,25 @@ def test_outputs_match_reference_files(name, tmp_path):
     g.ensight_write(out / "depouillement", variables)
     g.history_write_gnuplot(out / "courbes" / "gnuplot", env_names)
     g.close()
+    _compare(out, ref)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_dataset_run_writes_the_reference_files(name, tmp_path):
+    """The dataset itself (tests/golden/datasets, the reference's Donnees.arc) through the
+    reader and runner (python -m mahyco_amd.arc): the same files as the reference's run."""
+    from mahyco_amd import arc
+    from mahyco_amd import types as T
+    print(f"[outputs] dataset {name}", flush=True)
+    ref = tmp_path / "ref"
+    with tarfile.open(os.path.join(OUTPUTS, name + ".tar.gz")) as t:
+        t.extractall(ref)
+    st, it, tm = arc.run_arc(os.path.join(GOLDEN, "datasets", name + ".arc"), str(tmp_path / "out"))
+    assert st == T.MH_STOPPED
+    _compare(tmp_path / "out", ref)
+
+
+def _compare(out, ref):
     mine, theirs = _files(out), _files(ref)
     assert sorted(mine) == sorted(theirs)
     for rel in sorted(theirs):

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/mahyco.h"
+#include "mh_init.h"
 #include "mh_host.h"
 #include "mh_kernels.h"
 #include "mh_output.h"
@@ -35,27 +36,137 @@ namespace {
 
 template <class T>
 int dalloc(MhHandle* h, T** p, size_t n) {
-  void* q = nullptr;
   if (n == 0) n = 1;
+  const size_t bytes = n * sizeof(T);
+  if (h->planning) {  // a piece of the case's block, handed out by commit_arena
+    h->plan.emplace_back((void**)p, bytes);
+    *p = nullptr;
+    return MH_OK;
+  }
+  void* q = nullptr;
   // a case of an ensemble allocates in the order of the ensemble's stream, so that admitting
   // or retiring a case does not wait for the work in flight
-  hipError_t e = h->async_alloc ? hipMallocAsync(&q, n * sizeof(T), h->stream) : hipMalloc(&q, n * sizeof(T));
+  hipError_t e = h->async_alloc ? hipMallocAsync(&q, bytes, h->stream) : hipMalloc(&q, bytes);
   if (e != hipSuccess) {
-    fprintf(stderr, "mahyco: hipMalloc(%zu bytes) failed: %s\n", n * sizeof(T), hipGetErrorString(e));
+    fprintf(stderr, "mahyco: hipMalloc(%zu bytes) failed: %s\n", bytes, hipGetErrorString(e));
     return MH_ERR_NOMEM;
   }
+  h->owned.push_back(q);
   h->allocs.push_back(q);
-  h->alloc_bytes.push_back(n * sizeof(T));
-  h->bytes += n * sizeof(T);
+  h->alloc_bytes.push_back(bytes);
+  h->bytes += bytes;
   *p = (T*)q;
   return MH_OK;
 }
 
-template <class T>
-int upload(MhHandle* h, T* d, const std::vector<T>& v) {
-  HIPCHK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, h->stream));
+// One device block for every array dalloc recorded since `planning` was set: one allocation
+// (and one release) per case instead of one per array.
+int commit_arena(MhHandle* h) {
+  h->planning = false;
+  auto round = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  size_t total = 0;
+  for (auto& s : h->plan) total += round(s.second);
+  void* base = nullptr;
+  hipError_t e = h->async_alloc ? hipMallocAsync(&base, total, h->astream) : hipMalloc(&base, total);
+  if (e != hipSuccess) {
+    fprintf(stderr, "mahyco: hipMalloc(%zu bytes) failed: %s\n", total, hipGetErrorString(e));
+    h->plan.clear();
+    return MH_ERR_NOMEM;
+  }
+  h->owned.push_back(base);
+  h->bytes += total;
+  size_t off = 0;
+  for (auto& s : h->plan) {
+    void* q = (char*)base + off;
+    *s.first = q;
+    h->allocs.push_back(q);
+    h->alloc_bytes.push_back(s.second);
+    off += round(s.second);
+  }
+  h->plan.clear();
   return MH_OK;
 }
+
+template <class T>
+__global__ void k_fill_value(T* __restrict__ d, long n, T v) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) d[i] = v;
+}
+template <class T>
+__global__ void k_scatter_values(T* __restrict__ d, const long* __restrict__ idx, const T* __restrict__ val, long k) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < k; i += (long)gridDim.x * blockDim.x) d[idx[i]] = val[i];
+}
+// both copies of the initial node coordinates from the generator's axis coordinates
+__global__ void k_init_coords(MhGrid g, const double* __restrict__ ax, const double* __restrict__ ay,
+                              const double* __restrict__ az, double* x0, double* y0, double* z0, double* x1,
+                              double* y1, double* z1) {
+  for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < g.nn; n += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(n % g.nnx);
+    const long r = n / g.nnx;
+    const int j = (int)(r % g.nny);
+    const int k = (int)(r / g.nny);
+    x0[n] = x1[n] = ax[i];
+    y0[n] = y1[n] = ay[j];
+    z0[n] = z1[n] = g.dim == 3 ? az[k] : 0.;
+  }
+}
+__global__ void k_set_scalars(MhScalars* d, MhScalars v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *d = v;
+}
+inline unsigned fill_blocks(long n) { return (unsigned)std::min<long>((n + MH_BLOCK - 1) / MH_BLOCK, 4096); }
+
+// The initial uploads of a case, on h->astream.  An array that holds one value everywhere but
+// in a few places (zeros and ones outside mixed cells, a uniform velocity) is filled on the
+// device and its exceptions scattered, instead of crossing PCIe whole.  Host buffers handed
+// to the copies are kept until the stream is synchronised.
+struct Uploader {
+  MhHandle* h;
+  std::vector<std::vector<char>> keep;
+  std::vector<void*> temp;  // device buffers of scattered exceptions, released after use
+  template <class T>
+  int operator()(T* d, const std::vector<T>& v) {
+    const size_t n = v.size();
+    if (n == 0) return MH_OK;
+    if (n >= 4096) {
+      const T v0 = v[0];
+      std::vector<long> idx;
+      std::vector<T> val;
+      bool dense = false;
+      for (size_t i = 1; i < n; ++i)
+        if (memcmp(&v[i], &v0, sizeof(T)) != 0) {
+          if (idx.size() >= n / 8) {
+            dense = true;
+            break;
+          }
+          idx.push_back((long)i);
+          val.push_back(v[i]);
+        }
+      if (!dense) {
+        hipLaunchKernelGGL(k_fill_value<T>, dim3(fill_blocks((long)n)), dim3(MH_BLOCK), 0, h->astream, d, (long)n, v0);
+        const long k = (long)idx.size();
+        if (k > 0) {
+          std::vector<char> buf(k * sizeof(long) + k * sizeof(T));
+          memcpy(buf.data(), idx.data(), k * sizeof(long));
+          memcpy(buf.data() + k * sizeof(long), val.data(), k * sizeof(T));
+          void* q = nullptr;
+          HIPCHK(hipMallocAsync(&q, buf.size(), h->astream));
+          temp.push_back(q);
+          HIPCHK(hipMemcpyAsync(q, buf.data(), buf.size(), hipMemcpyHostToDevice, h->astream));
+          hipLaunchKernelGGL(k_scatter_values<T>, dim3(fill_blocks(k)), dim3(MH_BLOCK), 0, h->astream, d,
+                             (const long*)q, (const T*)((char*)q + k * sizeof(long)), k);
+          keep.push_back(std::move(buf));
+        }
+        return hip_check(hipGetLastError(), "fill");
+      }
+    }
+    HIPCHK(hipMemcpyAsync(d, v.data(), n * sizeof(T), hipMemcpyHostToDevice, h->astream));
+    return MH_OK;
+  }
+  int release() {
+    for (void* q : temp) HIPCHK(hipFreeAsync(q, h->astream));
+    temp.clear();
+    return MH_OK;
+  }
+};
 
 // Node coordinate along one axis of the Cartesian generator.
 double axis_coord(const MhMesh& mesh, int d, int i, int n) {
@@ -64,38 +175,25 @@ double axis_coord(const MhMesh& mesh, int d, int i, int n) {
   return o + (double)i * (l / (double)n);
 }
 
-// Host-side case initialisation: restates the init entry points HydroStartInit
-// (MahycoModule.cc:21-210, MahycoAnnexe.cc:11-142), the case services' initMat/initVar
-// (SODService.cc, SEDOVService.cc, POINTTRIPLEService.cc), IEquationOfState::initEOS and
-// ComputeCellMass (:221-237).  The node mass is formed by the first node kernel.
+// The host's part of a case's initial state: the generator's axis coordinates, the node
+// velocities (the RIDER and ADVECTION fields call glibc's sincos and atan2, as the reference
+// does) and the time-reversal table.  The cells are initialised on the device (mh_init.h); the
+// node mass is formed by the first node kernel.
 struct HostInit {
-  std::vector<double> ax, ay, az, VE, rho, p, c, q, M, e, fg, fk, chik, rhok, ek, pk, Mk;
-  std::vector<double> chig;  // global mass fraction: 1 everywhere, as Arcane starts it
-  std::vector<uint8_t> mask;
-  std::vector<double> u[3];
+  std::vector<double> ax, ay, az;  // node coordinates of the generator along each axis
+  std::vector<double> u[3];     // node velocities of the RIDER / ADVECTION fields (else empty)
+  double u_uniform[3] = {0., 0., 0.};  // the velocity of every node otherwise
   std::vector<double> rev_cos;  // advection mode: cos(pi t_n parametre) per iteration
 };
 
-// RIDERService (casTest/RIDER/RIDERService.cc) constants: the bubble centre of the
-// translations (0.2, 0.2) or of the rotations and later tests (0.5, 0.75), and the
-// reference's Pi (TypesMahyco.h:42).
+// The reference's Pi of the RIDER velocity fields (TypesMahyco.h:42).
 const double kRiderPi = 3.14159265359;
-void rider_radii(const D3* x, int nnode, bool low_centre, double& rmin, double& rmax) {
-  const double xb = low_centre ? 0.20 : 0.50, yb = low_centre ? 0.20 : 0.75;
-  rmin = 10.;
-  rmax = 0.;
-  for (int l = 0; l < nnode; ++l) {
-    const double rnode = std::sqrt((x[l].x - xb) * (x[l].x - xb) + (x[l].y - yb) * (x[l].y - yb));
-    rmin = std::min(rmin, rnode);
-    rmax = std::max(rmax, rnode);
-  }
-}
 
 int host_init(MhHandle* h, HostInit& I) {
   const MhGrid& g = h->g;
   const MhOptions& opt = h->opt;
-  const int m = h->m, dim = g.dim;
-  const long nc = g.nc, nn = g.nn;
+  const int dim = g.dim;
+  const long nn = g.nn;
   I.ax.resize(g.nnx);
   I.ay.resize(g.nny);
   I.az.resize(dim == 3 ? g.nnz : 1, 0.);
@@ -104,243 +202,9 @@ int host_init(MhHandle* h, HostInit& I) {
   if (dim == 3)
     for (int k = 0; k < g.nnz; ++k) I.az[k] = axis_coord(h->mesh, 2, k, g.nz);
   auto node = [&](int i, int j, int k) { return d3(I.ax[i], I.ay[j], dim == 3 ? I.az[k] : 0.); };
-  const size_t ne = (size_t)m * nc;
-  I.VE.assign(nc, 0.);
-  I.rho.assign(nc, 0.);
-  I.p.assign(nc, 0.);
-  I.c.assign(nc, 0.);
-  I.q.assign(nc, 0.);
-  I.M.assign(nc, 0.);
-  I.e.assign(nc, 0.);
-  I.fg.assign(nc, 1.);
-  I.fk.assign(ne, 1.);
-  I.chik.assign(ne, 1.);
-  I.chig.assign(nc, 1.);
-  I.rhok.assign(ne, 0.);
-  I.ek.assign(ne, 0.);
-  I.pk.assign(ne, 0.);
-  I.Mk.assign(ne, 0.);
-  I.mask.assign(nc, 1);
-  for (int d = 0; d < 3; ++d) I.u[d].assign(nn, 0.);
-  std::vector<double> xcx(nc), xcy(nc), xcz(nc), f1(nc, 0.);
-  std::vector<double> rider_frac(nc, -1.);  // RIDER cells cut by the bubble: its fraction
-  const double w = dim == 3 ? .125 : .25;
-  for (int k = 0; k < g.nz; ++k)
-    for (int j = 0; j < g.ny; ++j)
-      for (int i = 0; i < g.nx; ++i) {
-        const long c = i + (long)g.nx * (j + (long)g.ny * k);
-        D3 x[8], qv[8];
-        x[0] = node(i, j, k);
-        x[1] = node(i + 1, j, k);
-        x[2] = node(i + 1, j + 1, k);
-        x[3] = node(i, j + 1, k);
-        if (dim == 3) {
-          x[4] = node(i, j, k + 1);
-          x[5] = node(i + 1, j, k + 1);
-          x[6] = node(i + 1, j + 1, k + 1);
-          x[7] = node(i, j + 1, k + 1);
-        }
-        const int nnode = dim == 3 ? 8 : 4;
-        D3 s = d3(0., 0., 0.);
-        for (int l = 0; l < nnode; ++l) s = s + x[l];
-        const D3 xc = w * s;
-        xcx[c] = xc.x;
-        xcy[c] = xc.y;
-        xcz[c] = xc.z;
-        double V;
-        if (dim == 3) {
-          cqs_hex(x, qv);
-          V = cell_volume<8>(x, qv, 3);
-        } else {
-          cqs_quad(x, qv);
-          V = cell_volume<4>(x, qv, 2);
-        }
-        I.VE[c] = V;
-        // Materiau (initMat) and membership (MahycoAnnexe.cc:97-139)
-        double mat = 0.;
-        if (opt.cas_model == MH_CASE_SOD) {
-          const int t = opt.cas_test;
-          if (t == 13 || t == 14 || t == 15) {
-            const double rr = t == 13 ? xc.x : t == 14 ? xc.y : xc.z;
-            mat = (rr < 0.5) ? 0. : 1.;
-          }
-        } else if (opt.cas_model == MH_CASE_POINTTRIPLE) {
-          if (opt.cas_test != 2) {
-            if (xc.x <= 0.01) mat = 0.;
-            else mat = (xc.y <= 0.015) ? 1. : 2.;
-          }
-        } else if (opt.cas_model == MH_CASE_RIDER) {  // RIDERService::initMat :19-53
-          if (opt.cas_test >= 31) {
-            double rmin, rmax;
-            rider_radii(x, nnode, opt.cas_test < 34, rmin, rmax);
-            const double rb = opt.rider_rayon_bulle;
-            if (rmax < rb) mat = 1.;
-            else if ((rmax >= rb) && (rmin < rb)) mat = (rb - rmin) / (rmax - rmin);
-          }
-          // initVar (initVarMono :58-158, initVar :163-290), per cell
-          const bool mono = opt.cas_test <= 30;
-          const bool lin = opt.rider_densite_lineaire != 0;
-          const double rb = opt.rider_rayon_bulle;
-          double rmin, rmax;
-          rider_radii(x, nnode, opt.cas_test < (mono ? 26 : 34), rmin, rmax);
-          const double xb = opt.cas_test < (mono ? 26 : 34) ? 0.20 : 0.50;
-          const double yb = opt.cas_test < (mono ? 26 : 34) ? 0.20 : 0.75;
-          const double croissance = lin ? 1. / rb : 1.;
-          I.rho[c] = h->env[0].densite_initiale;
-          I.p[c] = h->env[0].pression_initiale;
-          I.e[c] = h->env[0].energie_initiale;
-          const double r = std::sqrt((xc.x - xb) * (xc.x - xb) + (xc.y - yb) * (xc.y - yb));
-          const double r_cell = lin ? r : rmax;
-          if (r_cell < rb) {
-            I.rho[c] = h->env[1].densite_initiale;
-            if (lin) I.rho[c] = h->env[1].densite_initiale * std::min(croissance * 2 * (rb - r), 1.);
-            I.p[c] = h->env[1].pression_initiale;
-          } else if ((rmax >= rb) && (rmin < rb) && !lin) {
-            rider_frac[c] = (rb - rmin) / (rmax - rmin);
-          }
-        } else if (opt.cas_model == MH_CASE_ADVECTION) {  // ADVECTIONService::initMat :11-65
-          if (opt.cas_test >= 18) {  // the Mono* tests (7-10) put every cell in environment 0
-            const double yb = opt.cas_test < 21 ? 0.50 : 0.75;
-            const double rb = 0.25;  // a constant here, the rayon-bulle option in initVar
-            if (std::fabs(xc.x - 0.50) < rb && std::fabs(xc.y - yb) < rb) mat = 1.;
-          }
-        } else if (opt.cas_model == MH_CASE_RIEMANN) {
-          const int a = opt.interface_axis;
-          const D3 lo3 = x[0], hi3 = a == 0 ? x[1] : a == 1 ? x[3] : x[4];
-          const double lo = a == 0 ? lo3.x : a == 1 ? lo3.y : lo3.z;
-          const double hi = a == 0 ? hi3.x : a == 1 ? hi3.y : hi3.z;
-          double fr = (hi - opt.interface_position) / (hi - lo);
-          fr = fr < 0. ? 0. : (fr > 1. ? 1. : fr);
-          f1[c] = fr;
-          mat = fr;
-        }
-        uint8_t bits;
-        if (mat == 0.) bits = 1;
-        else if (mat > 0. && mat < 1.) bits = 3;
-        else if (mat == 1.) bits = 2;
-        else if (mat > 1. && mat < 2.) bits = 6;
-        else if (mat == 2.) bits = 4;
-        else if (mat > 2. && mat < 3.) bits = 12;
-        else if (mat == 3.) bits = 8;
-        else bits = 15;
-        bits &= (uint8_t)((1u << m) - 1u);
-        if (bits == 0) return MH_ERR_INVALID;
-        I.mask[c] = bits;
-        // Sedov deposit uses the cell volume of the initial geometry
-        if (opt.cas_model == MH_CASE_SEDOV) {
-          const double rhoInit = h->env[0].densite_initiale, pInit = h->env[0].pression_initiale;
-          bool center = false;
-          for (int l = 0; l < nnode; ++l) {
-            double rn;
-            if (dim == 3)
-              rn = std::sqrt((x[l].x - 0.) * (x[l].x - 0.) + (x[l].y - 0.) * (x[l].y - 0.) + (x[l].z - 0.) * (x[l].z - 0.));
-            else
-              rn = std::sqrt((x[l].x - 0.) * (x[l].x - 0.) + (x[l].y - 0.) * (x[l].y - 0.));
-            if (rn < 1.e-10) center = true;
-          }
-          I.e[c] = 0.244816e-5;
-          I.rho[c] = rhoInit;
-          I.p[c] = pInit;
-          if (center) {
-            I.e[c] += opt.sedov_deposit / V;
-            I.p[c] = 0.4 * rhoInit * I.e[c];
-          }
-        }
-      }
-  // initVar of the other case models (global values; partial values of multi-env cells)
-  for (long c = 0; c < nc; ++c) {
-    const uint8_t bits = I.mask[c];
-    const bool pure = __builtin_popcount(bits) == 1;
-    if (opt.cas_model == MH_CASE_SOD) {
-      const int t = opt.cas_test;
-      if (t == 3 || t == 4 || t == 5) {  // SODService::initVarMono :11-40
-        const double rr = t == 3 ? xcx[c] : t == 4 ? xcy[c] : xcz[c];
-        if (rr < 0.5) {
-          I.p[c] = h->env[0].pression_initiale;
-          I.rho[c] = h->env[0].densite_initiale;
-        } else {
-          I.p[c] = h->env[0].pression_initiale / 10.;
-          I.rho[c] = h->env[0].densite_initiale / 8.;
-        }
-      } else {  // SODService::initVar :71-121
-        const int s = bits == 1 ? 0 : 1;
-        I.rho[c] = h->env[s].densite_initiale;
-        I.p[c] = h->env[s].pression_initiale;
-        if (!pure)
-          for (int k = 0; k < m; ++k)
-            if ((bits >> k) & 1) {
-              I.rhok[k * nc + c] = h->env[s].densite_initiale;
-              I.pk[k * nc + c] = h->env[s].pression_initiale;
-            }
-      }
-    } else if (opt.cas_model == MH_CASE_POINTTRIPLE) {  // POINTTRIPLEService::initVar :64-128
-      const int s = xcx[c] <= 0.01 ? 0 : (xcy[c] <= 0.015 ? 1 : 2);
-      I.rho[c] = h->env[s].densite_initiale;
-      I.p[c] = h->env[s].pression_initiale;
-      if (!pure && ((bits >> s) & 1)) {
-        I.rhok[s * nc + c] = h->env[s].densite_initiale;
-        I.pk[s * nc + c] = h->env[s].pression_initiale;
-      }
-    } else if (opt.cas_model == MH_CASE_RIDER) {
-      const double frac_b = rider_frac[c];
-      if (frac_b >= 0.) {  // environment 0 takes the bubble density and fraction (RIDERService.cc:252-266)
-        if (opt.cas_test >= 31)
-          for (int k = 0; k < m && k < 2; ++k) {
-            if (!((bits >> k) & 1)) continue;
-            const double rk = h->env[k == 0 ? 1 : 0].densite_initiale;
-            const double fr = k == 0 ? frac_b : 1. - frac_b;
-            if (pure) {  // the environment value is the cell value
-              I.rho[c] = rk;
-              I.fg[c] = fr;
-              I.chig[c] = fr;
-              I.p[c] = 0.;
-            } else {
-              I.rhok[k * nc + c] = rk;
-              I.fk[k * nc + c] = fr;
-              I.chik[k * nc + c] = fr;
-              I.pk[k * nc + c] = 0.;
-            }
-          }
-        I.rho[c] = frac_b * h->env[1].densite_initiale;
-        I.p[c] = h->env[1].pression_initiale;
-      }
-    } else if (opt.cas_model == MH_CASE_ADVECTION) {  // ADVECTIONService::initVarMono :67-145, initVar :146-246
-      // a square of half side rayon-bulle, no mixed cells, internal energy left at 0
-      const bool lin = opt.rider_densite_lineaire != 0;
-      const double rb = opt.rider_rayon_bulle;
-      const double xb = 0.50, yb = opt.cas_test < (opt.cas_test <= 10 ? 10 : 21) ? 0.50 : 0.75;
-      const double croissance = lin ? 1. / rb : 1.;
-      I.rho[c] = h->env[0].densite_initiale;
-      I.p[c] = h->env[0].pression_initiale;
-      const double r = std::sqrt((xcx[c] - xb) * (xcx[c] - xb) + (xcy[c] - yb) * (xcy[c] - yb));
-      if (std::fabs(xcx[c] - xb) < rb && std::fabs(xcy[c] - yb) < rb) {
-        I.rho[c] = h->env[1].densite_initiale;
-        if (lin) I.rho[c] = h->env[1].densite_initiale * croissance * r;
-        I.p[c] = h->env[1].pression_initiale;
-      }
-    } else if (opt.cas_model == MH_CASE_RIEMANN) {
-      if (pure) {
-        const int k = bits == 1 ? 0 : 1;
-        I.rho[c] = h->env[k].densite_initiale;
-        I.p[c] = h->env[k].pression_initiale;
-      } else {
-        I.fk[0 * nc + c] = 1.0 - f1[c];
-        I.fk[1 * nc + c] = f1[c];
-        double ft = 0.;
-        for (int k = 0; k < 2; ++k) ft += I.fk[k * nc + c] * h->env[k].densite_initiale;
-        for (int k = 0; k < 2; ++k) {
-          I.rhok[k * nc + c] = h->env[k].densite_initiale;
-          I.pk[k * nc + c] = h->env[k].pression_initiale;
-          I.chik[k * nc + c] = I.fk[k * nc + c] * h->env[k].densite_initiale / ft;
-        }
-        I.rho[c] = 0.;
-        I.p[c] = 0.;
-        I.c[c] = 0.;
-      }
-    }
-  }
   const double* v0 = h->env[0].vitesse_initiale;
   if (opt.cas_model == MH_CASE_RIDER || opt.cas_model == MH_CASE_ADVECTION) {
+    for (int d = 0; d < 3; ++d) I.u[d].assign(nn, 0.);
     // RIDERService.cc:134-157, 267-289 and ADVECTIONService.cc:131-144, 233-246, numbered as
     // RIDER's fields: 23 Tx, 24 Ty, 25 T45, 26 rotation, 27/29 vortex, 28/30 deformation
     const int t = opt.cas_model == MH_CASE_ADVECTION ? (opt.cas_test <= 10 ? opt.cas_test : opt.cas_test - 11) + 16
@@ -385,8 +249,9 @@ int host_init(MhHandle* h, HostInit& I) {
       I.u[1][n] = uy;
       I.u[2][n] = 0.;
     }
-  } else if (opt.cas_model != MH_CASE_SEDOV)
-    for (int d = 0; d < 3; ++d) std::fill(I.u[d].begin(), I.u[d].end(), dim == 2 && d == 2 ? 0. : v0[d]);
+  } else if (opt.cas_model != MH_CASE_SEDOV) {
+    for (int d = 0; d < 3; ++d) I.u_uniform[d] = dim == 2 && d == 2 ? 0. : v0[d];
+  }
   if (opt.sans_lagrange) {  // cos(pi t parametre) at t_n = t_(n-1) + deltat-init, as the device accumulates it
     const long nmax = (long)std::ceil(opt.final_time / opt.deltat_init) + 8;
     I.rev_cos.resize(nmax);
@@ -395,46 +260,6 @@ int host_init(MhHandle* h, HostInit& I) {
       t = t + opt.deltat_init;
       I.rev_cos[it] = std::cos(kRiderPi * t * opt.rider_parametre);
     }
-  }
-  // initEOS per environment (PerfectGasEOSService.cc:12-37, StiffenedGasEOSService.cc:14-33),
-  // not in advection mode (MahycoModule.cc:143)
-  for (long c = 0; c < nc && !opt.sans_lagrange; ++c) {
-    const uint8_t bits = I.mask[c];
-    const bool pure = __builtin_popcount(bits) == 1;
-    for (int k = 0; k < m; ++k) {
-      if (!((bits >> k) & 1)) continue;
-      const double gamma = h->env[k].adiabatic_cst;
-      double& pp = pure ? I.p[c] : I.pk[k * nc + c];
-      double& rr = pure ? I.rho[c] : I.rhok[k * nc + c];
-      double& ee = pure ? I.e[c] : I.ek[k * nc + c];
-      double cs;
-      if (h->env[k].eos == MH_EOS_STIFFENED_GAS) {
-        const double pi = h->env[k].limit_tension;
-        ee = (pp + (gamma * pi)) / ((gamma - 1.) * rr);
-        cs = std::sqrt((gamma / rr) * (pp + pi));
-      } else {
-        ee = pp / ((gamma - 1.) * rr);
-        cs = std::sqrt(gamma * pp / rr);
-      }
-      if (pure) I.c[c] = cs;
-      else {
-        // mixed-cell averages, MahycoModule.cc:154-170 (density and pressure accumulate
-        // onto the values left by initVar)
-        if (k == __builtin_ctz(bits)) I.e[c] = 0.;
-        I.e[c] += ee * I.chik[k * nc + c];
-        I.c[c] = (cs < I.c[c]) ? I.c[c] : cs;
-        I.rho[c] += rr * I.fk[k * nc + c];
-        I.p[c] += pp * I.fk[k * nc + c];
-      }
-    }
-  }
-  // ComputeCellMass (:221-237); m_euler_volume = V (:174-178)
-  for (long c = 0; c < nc; ++c) {
-    I.M[c] = I.rho[c] * I.VE[c];
-    const uint8_t bits = I.mask[c];
-    if (__builtin_popcount(bits) != 1)
-      for (int k = 0; k < m; ++k)
-        if ((bits >> k) & 1) I.Mk[k * nc + c] = I.chik[k * nc + c] * I.M[c];
   }
   return MH_OK;
 }
@@ -896,7 +721,8 @@ int ens_step(MhEnsembleImpl* E, int ncycles, int use_graph) {
 extern "C" {
 
 static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const MhOptions* options,
-                       hipStream_t stream, MhScalars* ext_sc, MhHandle** out);
+                       hipStream_t stream, MhScalars* ext_sc, MhHandle** out, hipStream_t copy_stream = nullptr,
+                       hipEvent_t copy_done = nullptr);
 
 int mahyco_create(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const MhOptions* options,
                   MhHandle** out) {
@@ -910,7 +736,8 @@ int mahyco_create_on_stream(const MhMesh* mesh, const MhEnvironment* envs, int n
 }
 
 static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env, const MhOptions* options,
-                       hipStream_t stream, MhScalars* ext_sc, MhHandle** out) {
+                       hipStream_t stream, MhScalars* ext_sc, MhHandle** out, hipStream_t copy_stream,
+                       hipEvent_t copy_done) {
   if (!out) return MH_ERR_INVALID;
   *out = nullptr;
   int st = validate(mesh, envs, nb_env, options);
@@ -945,6 +772,7 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
   }
   h->ext_sc = ext_sc;
   h->async_alloc = ext_sc != nullptr;  // a case of an ensemble
+  h->astream = copy_stream ? copy_stream : h->stream;
   {
     const char* nk = getenv("MAHYCO_NODE_KERNEL");
     h->node_tiled = !(nk && strcmp(nk, "flat") == 0);
@@ -1012,7 +840,10 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
   const long nc = g.nc, nn = g.nn;
   const size_t ne = (size_t)nb_env * nc;
   double *ax, *ay, *az, *VE;
+  int* d_bad;  // set by the initialisation kernel for a cell outside the case's environments
   int ok = MH_OK;
+  h->planning = true;  // one device block for the arrays below (commit_arena)
+  ok |= dalloc(h, &d_bad, 1);
   ok |= dalloc(h, &ax, g.nnx);
   ok |= dalloc(h, &ay, g.nny);
   ok |= dalloc(h, &az, I.az.size());
@@ -1081,6 +912,9 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
       ok |= dalloc(h, &S.qknm1, ne);
     }
   }
+  if (ok == MH_OK) ok = commit_arena(h);
+  h->planning = false;
+  h->plan.clear();
   if (ok != MH_OK) {
     mahyco_destroy(h);
     return MH_ERR_NOMEM;
@@ -1100,17 +934,6 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
   S.ct_k = h->ct_k;
   S.nblk_n = h->nblk_n;
   S.max_iter = 0;
-  // initial Lagrangian coordinates = Eulerian grid
-  std::vector<double> xs[3];
-  for (int d = 0; d < 3; ++d) xs[d].assign(nn, 0.);
-  for (int k = 0; k < g.nnz; ++k)
-    for (int j = 0; j < g.nny; ++j)
-      for (int i = 0; i < g.nnx; ++i) {
-        const long n = i + (long)g.nnx * (j + (long)g.nny * k);
-        xs[0][n] = I.ax[i];
-        xs[1][n] = I.ay[j];
-        xs[2][n] = g.dim == 3 ? I.az[k] : 0.;
-      }
   MhScalars sc;
   memset(&sc, 0, sizeof(sc));
   sc.t = 0.;
@@ -1118,70 +941,92 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
   sc.dt_old = 0.;
   std::vector<int> err(4, 0);
   ok = MH_OK;
-  ok |= upload(h, ax, I.ax);
-  ok |= upload(h, ay, I.ay);
-  ok |= upload(h, az, I.az);
-  ok |= upload(h, VE, I.VE);
-  for (int d = 0; d < 3; ++d) {
-    ok |= upload(h, S.u[d], I.u[d]);
-    ok |= upload(h, h->xl[0][d], xs[d]);
-    ok |= upload(h, h->xl[1][d], xs[d]);
+  Uploader upload{h};
+  ok |= upload(ax, I.ax);
+  ok |= upload(ay, I.ay);
+  ok |= upload(az, I.az);
+  const std::vector<int> zero1(1, 0);
+  ok |= upload(d_bad, zero1);
+  {  // every cell's initial state, on the device
+    MhInitArgs A;
+    A.g = g;
+    A.m = nb_env;
+    A.opt = *options;
+    for (int k = 0; k < MH_MAX_ENV; ++k) A.env[k] = h->env[k];
+    A.ax = ax;
+    A.ay = ay;
+    A.az = az;
+    A.VE = VE;
+    A.rho = S.rho;
+    A.p = S.p;
+    A.c = S.c;
+    A.q = S.q;
+    A.M = S.M;
+    A.e = S.e;
+    A.fg = S.fg;
+    A.chig = S.chig;
+    A.fk = S.fk;
+    A.chik = S.chik;
+    A.rhok = S.rhok;
+    A.ek = S.ek;
+    A.pk = S.pk;
+    A.Mk = S.Mk;
+    A.mask = S.mask;
+    A.bad = d_bad;
+    if (g.dim == 3) hipLaunchKernelGGL(k_init_cells<3>, dim3(fill_blocks(nc)), dim3(256), 0, h->astream, A);
+    else hipLaunchKernelGGL(k_init_cells<2>, dim3(fill_blocks(nc)), dim3(256), 0, h->astream, A);
+    ok |= hip_check(hipGetLastError(), "k_init_cells");
   }
-  std::vector<double> zc, zn, ze;  // kept until the stream is synchronised below (asynchronous uploads)
+  for (int d = 0; d < 3; ++d) {
+    if (!I.u[d].empty()) ok |= upload(S.u[d], I.u[d]);
+    else hipLaunchKernelGGL(k_fill_value<double>, dim3(fill_blocks(nn)), dim3(MH_BLOCK), 0, h->astream, S.u[d], nn,
+                            I.u_uniform[d]);
+  }
+  // initial Lagrangian coordinates = Eulerian grid, both copies, formed on the device
+  hipLaunchKernelGGL(k_init_coords, dim3(fill_blocks(nn)), dim3(MH_BLOCK), 0, h->astream, g, ax, ay, az,
+                     h->xl[0][0], h->xl[0][1], h->xl[0][2], h->xl[1][0], h->xl[1][1], h->xl[1][2]);
+  auto fill = [&](auto* d, long n, auto v) {
+    hipLaunchKernelGGL(k_fill_value<decltype(v)>, dim3(fill_blocks(n)), dim3(MH_BLOCK), 0, h->astream, d, n, v);
+  };
   if (S.csts) {  // the pseudo-viscosities at n and n-1 start at 0; the saved velocity is written before use
-    zc.assign(nc, 0.);
-    zn.assign(nn, 0.);
-    ok |= upload(h, S.qm1, zc);
-    ok |= upload(h, S.qe, zc);
-    ok |= upload(h, S.qem1, zc);
-    ok |= upload(h, S.mkm1, I.mask);
-    for (int d = 0; d < 3; ++d) ok |= upload(h, S.un[d], zn);
+    fill(S.qm1, nc, 0.);
+    fill(S.qe, nc, 0.);
+    fill(S.qem1, nc, 0.);
+    ok |= hip_check(hipMemcpyAsync(S.mkm1, S.mask, (size_t)nc, hipMemcpyDeviceToDevice, h->astream), "copy mask");
+    for (int d = 0; d < 3; ++d) fill(S.un[d], nn, 0.);
     if (!options->with_projection) {
-      ze.assign(ne, 0.);
-      ok |= upload(h, S.qkn, ze);
-      ok |= upload(h, S.qknm1, ze);
+      fill(S.qkn, (long)ne, 0.);
+      fill(S.qknm1, (long)ne, 0.);
     }
   }
-  ok |= upload(h, S.rho, I.rho);
-  ok |= upload(h, S.p, I.p);
-  ok |= upload(h, S.c, I.c);
-  ok |= upload(h, S.q, I.q);
-  ok |= upload(h, S.M, I.M);
-  ok |= upload(h, S.e, I.e);
-  ok |= upload(h, S.fg, I.fg);
-  ok |= upload(h, S.mask, I.mask);
-  ok |= upload(h, S.fk, I.fk);
-  ok |= upload(h, S.chik, I.chik);
-  ok |= upload(h, S.chig, I.chig);
-  ok |= upload(h, S.rhok, I.rhok);
-  ok |= upload(h, S.ek, I.ek);
-  ok |= upload(h, S.pk, I.pk);
-  ok |= upload(h, S.Mk, I.Mk);
-  ok |= upload(h, S.err, err);
-  std::vector<int8_t> est0;  // mixed-cell flags: Arcane starts est_mixte and est_pure at 0
-  if (S.est) {
-    est0.assign(nc, 0);
-    ok |= upload(h, S.est, est0);
-  }
+  ok |= upload(S.err, err);
+  if (S.est) fill(S.est, nc, (int8_t)0);  // mixed-cell flags: Arcane starts est_mixte and est_pure at 0
   if (h->rev_cos) {
     for (int d = 0; d < 3; ++d) {
-      ok |= upload(h, h->u0[d], I.u[d]);
+      ok |= hip_check(hipMemcpyAsync(h->u0[d], S.u[d], nn * sizeof(double), hipMemcpyDeviceToDevice, h->astream),
+                      "copy velocity");
       S.u0[d] = h->u0[d];
     }
-    ok |= upload(h, h->rev_cos, I.rev_cos);
+    ok |= upload(h->rev_cos, I.rev_cos);
     S.rev_cos = h->rev_cos;
     S.rev_n = (long)I.rev_cos.size();
   }
-  std::vector<double> phi00;  // pente-borne phi: Arcane starts m_phi_lagrange at 0
-  if (S.phi0) {
-    phi00.assign((size_t)S.nv * nc, 0.);
-    ok |= upload(h, S.phi0, phi00);
+  if (S.phi0) fill(S.phi0, (long)S.nv * nc, 0.);  // pente-borne phi: Arcane starts m_phi_lagrange at 0
+  // the scalars may sit in an ensemble's array, which the iterations' stream writes: set them in
+  // that stream's order (a kernel argument, no host buffer)
+  hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(64), 0, h->stream, S.sc, sc);
+  ok |= hip_check(hipGetLastError(), "upload scalars");
+  ok |= upload.release();
+  if (h->astream != h->stream) {  // the case's first launches wait for its uploads, not the host
+    ok |= hip_check(hipEventRecord(copy_done, h->astream), "hipEventRecord");
+    ok |= hip_check(hipStreamWaitEvent(h->stream, copy_done, 0), "hipStreamWaitEvent");
   }
-  ok |= hip_check(hipMemcpyAsync(S.sc, &sc, sizeof(sc), hipMemcpyHostToDevice, h->stream), "upload scalars");
-  ok |= hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
-  if (ok != MH_OK) {
+  int bad = 0;
+  ok |= hip_check(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, h->astream), "read init flag");
+  ok |= hip_check(hipStreamSynchronize(h->astream), "hipStreamSynchronize");
+  if (ok != MH_OK || bad) {
     mahyco_destroy(h);
-    return MH_ERR_DEVICE;
+    return ok != MH_OK ? MH_ERR_DEVICE : MH_ERR_INVALID;  // a cell in none of the case's environments
   }
   *out = h;
   return MH_OK;
@@ -1197,7 +1042,7 @@ void mahyco_destroy(MhHandle* h) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
   }
-  for (void* p : h->allocs) (void)(h->async_alloc ? hipFreeAsync(p, h->stream) : hipFree(p));
+  for (void* p : h->owned) (void)(h->async_alloc ? hipFreeAsync(p, h->stream) : hipFree(p));
   if (h->stream && h->owns_stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -1599,7 +1444,19 @@ int mahyco_ensemble_create(int max_slots, double hbm_budget_bytes, MhEnsemble** 
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0) E->ncu = ncu;
   }
   int st = hip_check(hipStreamCreate(&E->stream), "hipStreamCreate");
+  if (st == MH_OK) st = hip_check(hipStreamCreate(&E->copy_stream), "hipStreamCreate");
+  if (st == MH_OK) st = hip_check(hipEventCreateWithFlags(&E->copy_done, hipEventDisableTiming), "hipEventCreate");
   if (st == MH_OK) st = hip_check(hipEventCreateWithFlags(&E->stage_done, hipEventDisableTiming), "hipEventCreate");
+  if (st == MH_OK) {
+    // released cases' memory stays in the device pool up to the budget, for the next admissions
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+      uint64_t thr = 0;
+      (void)hipMemPoolGetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+      uint64_t want = (uint64_t)std::min(hbm_budget_bytes, 1.8e19);
+      if (want > thr) (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &want);
+    }
+  }
   if (st == MH_OK) st = hip_check(hipMalloc(&E->d_sc, sizeof(MhScalars) * max_slots), "hipMalloc");
   if (st == MH_OK) st = hip_check(hipMalloc(&E->d_tab[0], sizeof(MhDev) * max_slots), "hipMalloc");
   if (st == MH_OK) st = hip_check(hipMalloc(&E->d_tab[1], sizeof(MhDev) * max_slots), "hipMalloc");
@@ -1627,6 +1484,11 @@ void mahyco_ensemble_destroy(MhEnsemble* e) {
   if (E->stage_done) (void)hipEventDestroy(E->stage_done);
   if (E->snap) (void)hipHostFree(E->snap);
   if (E->snap_done) (void)hipEventDestroy(E->snap_done);
+  if (E->copy_stream) {
+    (void)hipStreamSynchronize(E->copy_stream);
+    (void)hipStreamDestroy(E->copy_stream);
+  }
+  if (E->copy_done) (void)hipEventDestroy(E->copy_done);
   if (E->stream) (void)hipStreamDestroy(E->stream);
   delete E;
 }
@@ -1649,7 +1511,7 @@ int mahyco_ensemble_add(MhEnsemble* e, const MhMesh* mesh, const MhEnvironment* 
   }
   if (sl < 0) return MH_ERR_NOMEM;
   MhHandle* h = nullptr;
-  int st = create_case(mesh, envs, nb_env, options, E->stream, E->d_sc + sl, &h);
+  int st = create_case(mesh, envs, nb_env, options, E->stream, E->d_sc + sl, &h, E->copy_stream, E->copy_done);
   if (st != MH_OK) return st;
   h->dev.max_iter = max_iterations > 0 ? max_iterations : 0;
   h->bytes = need;
@@ -1883,6 +1745,17 @@ int mahyco_autotune(MhHandle* h, int cycles, int* variant_out, double* ms_per_cy
   tune_store(case_key_dims(h), best);
   if (variant_out) *variant_out = best;
   if (ms_per_cycle_out) *ms_per_cycle_out = best_ms / cycles;
+  return MH_OK;
+}
+
+// The kernel-variant code of a case, as mahyco_autotune returns it, and its application: a
+// caller can keep a tuned choice and apply it to later cases of the same shape.  Every variant
+// gives the same bits; setting one drops the captured graphs.
+int mahyco_get_variant(const MhHandle* h) { return h ? get_variant(h) : MH_ERR_INVALID; }
+int mahyco_set_variant(MhHandle* h, int variant) {
+  if (!h || variant < 0 || variant > 31 || h->phase != 0) return MH_ERR_INVALID;
+  set_variant(h, variant);
+  drop_graphs(h);
   return MH_OK;
 }
 

@@ -77,6 +77,13 @@ struct MhHandle {
   MhScalars* ext_sc = nullptr;  // scalars living in an ensemble's array
   bool async_alloc = false;     // ensemble case: stream-ordered device allocation and release
   size_t bytes = 0;             // device bytes of the case
+  // The state arrays of a case are one device block: while `planning`, dalloc records each
+  // array, and commit_arena allocates the block and hands out 256-byte aligned pieces
+  // (allocs / alloc_bytes list the pieces).  `owned` holds the blocks to free.
+  bool planning = false;
+  std::vector<std::pair<void**, size_t>> plan;
+  std::vector<void*> owned;
+  hipStream_t astream = nullptr;  // stream of the case's allocation and initial uploads
 };
 
 inline int hip_check(hipError_t e, const char* what) {
@@ -120,12 +127,6 @@ inline void prof_end(MhHandle* h, ProfRec& r) {
     if ((h)->prof) prof_begin((h), _r, (kind));                          \
     hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, (h)->stream, __VA_ARGS__); \
     if ((h)->prof) prof_end((h), _r);                                    \
-  } while (0)
-
-// a dual (node) sweep kernel: none in advection mode (withDualProjection, PrepareRemap.cc:211)
-#define LAUNCH_DUAL(h, ...)                           \
-  do {                                                \
-    if (!(h)->dev.sans_lagrange) LAUNCH(h, __VA_ARGS__); \
   } while (0)
 
 // blocks of the row-segment x sweep (one wavefront per MH_ROW_OUT cells of a row), a multiple of 8
@@ -194,6 +195,10 @@ struct EnsGroup {
 
 struct MhEnsembleImpl {
   hipStream_t stream = nullptr;
+  // admissions allocate and upload on their own stream, which the iterations' stream then
+  // waits for: admitting a case does not wait for the iterations in flight
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t copy_done = nullptr;
   int max_slots = 0;
   double budget = 0.;
   size_t used = 0;

@@ -8,22 +8,62 @@
 #error "MH_TU_M (environment count of this unit) must be defined"
 #endif
 
-template <int DIM, int D, int M>
-void launch_sweep_yz(MhHandle* h, int sel, int first, int last) {
+// Cell part of one sweep direction (computeGradPhiFace ... computeUremap).
+template <int DIM, int M>
+void launch_cell_part(MhHandle* h, int D, int sel, int first, bool gen, bool o3) {
   const MhDev& S = h->dev;
-  if (h->sweep_mode == MH_SWEEP_FLAT)
-    MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell<DIM, D, M, LIM>), h->nblk_c, MH_BLOCK, S, sel, first));
-  else
-    MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL + D, (k_sweep_cell_tile<DIM, D, M, LIM>), tile_blocks(h, D), MH_BLOCK, S, sel, first));
-  if constexpr (DIM == 3 && D == 2) {
-    if (h->sweep_mode != MH_SWEEP_FLAT) {  // z dual sweep: blocks walk z fastest
-      MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE + 2, (k_sweep_node_z<LIM>), node_z_blocks(h), MH_BLOCK, S, sel, first, last));
-      return;
-    }
+  const bool flat = h->sweep_mode == MH_SWEEP_FLAT;
+  if (gen) {
+    if (D == 0) LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell_gen<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+    else if (D == 1) LAUNCH(h, KN_SWEEP_CELL + 1, (k_sweep_cell_gen<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+    else if constexpr (DIM == 3) LAUNCH(h, KN_SWEEP_CELL + 2, (k_sweep_cell_gen<3, 2, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+  } else if (o3) {
+    if (D == 0) LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell3<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+    else if (D == 1) LAUNCH(h, KN_SWEEP_CELL + 1, (k_sweep_cell3<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+    else if constexpr (DIM == 3) LAUNCH(h, KN_SWEEP_CELL + 2, (k_sweep_cell3<3, 2, M>), h->nblk_c, MH_BLOCK, S, sel, first);
+  } else if (D == 0) {
+    if (flat) MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, 0, M, LIM>), h->nblk_c, MH_BLOCK, S, sel, first));
+    else MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell_row<DIM, M, LIM>), row_blocks(h), MH_BLOCK, S, sel, first));
+  } else if (D == 1) {
+    if (flat) MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL + 1, (k_sweep_cell<DIM, 1, M, LIM>), h->nblk_c, MH_BLOCK, S, sel, first));
+    else MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL + 1, (k_sweep_cell_tile<DIM, 1, M, LIM>), tile_blocks(h, 1), MH_BLOCK, S, sel, first));
+  } else if constexpr (DIM == 3) {
+    if (flat) MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL + 2, (k_sweep_cell<3, 2, M, LIM>), h->nblk_c, MH_BLOCK, S, sel, first));
+    else MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL + 2, (k_sweep_cell_tile<3, 2, M, LIM>), tile_blocks(h, 2), MH_BLOCK, S, sel, first));
   }
-  MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE + D, (k_sweep_node<DIM, D, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
 }
 
+// Dual part of one sweep direction (computeDualUremap), then the dual kinetic energy when
+// the total energy is conserved.  None in advection mode (withDualProjection, PrepareRemap.cc:211).
+template <int DIM>
+void launch_dual_part(MhHandle* h, int D, int sel, int first, int last, bool gen, bool o3) {
+  const MhDev& S = h->dev;
+  if (S.sans_lagrange) return;
+  const bool flat = h->sweep_mode == MH_SWEEP_FLAT;
+  if (gen) {
+    if (D == 0) LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0, MH_LIM_ANY>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+    else if (D == 1) LAUNCH(h, KN_SWEEP_NODE + 1, (k_sweep_node<DIM, 1, MH_LIM_ANY>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+    else if constexpr (DIM == 3) LAUNCH(h, KN_SWEEP_NODE + 2, (k_sweep_node<3, 2, MH_LIM_ANY>), h->nblk_n, MH_BLOCK, S, sel, first, last);
+  } else if (D == 0) {
+    if (flat) MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
+    else MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node_row<DIM, LIM>), node_row_blocks(h), MH_BLOCK, S, sel, first, last));
+  } else if (D == 1) {
+    MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE + 1, (k_sweep_node<DIM, 1, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
+  } else if constexpr (DIM == 3) {
+    if (!flat && !o3)  // z dual sweep: blocks walk z fastest
+      MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE + 2, (k_sweep_node_z<LIM>), node_z_blocks(h), MH_BLOCK, S, sel, first, last));
+    else
+      MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE + 2, (k_sweep_node<3, 2, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
+  }
+  if (S.dec) {  // conservation-energie-totale: the dual kinetic energy of this sweep
+    if (D == 0) LAUNCH(h, KN_SWEEP_KE, (k_sweep_ke<DIM, 0>), h->nblk_n, MH_BLOCK, S, sel, first);
+    else if (D == 1) LAUNCH(h, KN_SWEEP_KE, (k_sweep_ke<DIM, 1>), h->nblk_n, MH_BLOCK, S, sel, first);
+    else if constexpr (DIM == 3) LAUNCH(h, KN_SWEEP_KE, (k_sweep_ke<3, 2>), h->nblk_n, MH_BLOCK, S, sel, first);
+  }
+}
+
+// The remap of one iteration (RemapADIService::appliRemap, RemapADIService.cc:15-90, and
+// remapVariables).
 template <int DIM, int M>
 void launch_sweeps_t(MhHandle* h) {
   const MhDev& S = h->dev;
@@ -43,48 +83,8 @@ void launch_sweeps_t(MhHandle* h) {
   for (int s = 0; s < nd; ++s) {
     const int first = s == 0, last = (s == nd - 1) && !first, sel = s & 1;
     const int D = dirs[s];
-    const int mode = h->sweep_mode;
-    if (gen) {
-      if (D == 0) {
-        LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell_gen<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-        LAUNCH_DUAL(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0, MH_LIM_ANY>), h->nblk_n, MH_BLOCK, S, sel, first, last);
-      } else if (D == 1) {
-        LAUNCH(h, KN_SWEEP_CELL + 1, (k_sweep_cell_gen<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-        LAUNCH_DUAL(h, KN_SWEEP_NODE + 1, (k_sweep_node<DIM, 1, MH_LIM_ANY>), h->nblk_n, MH_BLOCK, S, sel, first, last);
-      } else if constexpr (DIM == 3) {
-        LAUNCH(h, KN_SWEEP_CELL + 2, (k_sweep_cell_gen<3, 2, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-        LAUNCH_DUAL(h, KN_SWEEP_NODE + 2, (k_sweep_node<3, 2, MH_LIM_ANY>), h->nblk_n, MH_BLOCK, S, sel, first, last);
-      }
-    } else if (o3) {  // order 3: one thread per cell for the cell part, node part as selected
-      if (D == 0) LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell3<DIM, 0, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-      else if (D == 1) LAUNCH(h, KN_SWEEP_CELL + 1, (k_sweep_cell3<DIM, 1, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-      else if constexpr (DIM == 3) LAUNCH(h, KN_SWEEP_CELL + 2, (k_sweep_cell3<3, 2, M>), h->nblk_c, MH_BLOCK, S, sel, first);
-      if (D == 0 && mode != MH_SWEEP_FLAT)
-        MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE, (k_sweep_node_row<DIM, LIM>), node_row_blocks(h), MH_BLOCK, S, sel, first, last));
-      else if (D == 0)
-        MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
-      else if (D == 1)
-        MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE + 1, (k_sweep_node<DIM, 1, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
-      else if constexpr (DIM == 3)
-        MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE + 2, (k_sweep_node<3, 2, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
-    } else if (D == 0) {
-      if (mode == MH_SWEEP_FLAT) {
-        MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell<DIM, 0, M, LIM>), h->nblk_c, MH_BLOCK, S, sel, first));
-        MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
-      } else {
-        MH_LIMSW(h, LAUNCH(h, KN_SWEEP_CELL, (k_sweep_cell_row<DIM, M, LIM>), row_blocks(h), MH_BLOCK, S, sel, first));
-        MH_LIMSW(h, LAUNCH_DUAL(h, KN_SWEEP_NODE, (k_sweep_node_row<DIM, LIM>), node_row_blocks(h), MH_BLOCK, S, sel, first, last));
-      }
-    } else if (D == 1) {
-      launch_sweep_yz<DIM, 1, M>(h, sel, first, last);
-    } else {
-      if constexpr (DIM == 3) launch_sweep_yz<3, 2, M>(h, sel, first, last);
-    }
-    if (S.dec) {  // conservation-energie-totale: the dual kinetic energy of this sweep
-      if (D == 0) LAUNCH(h, KN_SWEEP_KE, (k_sweep_ke<DIM, 0>), h->nblk_n, MH_BLOCK, S, sel, first);
-      else if (D == 1) LAUNCH(h, KN_SWEEP_KE, (k_sweep_ke<DIM, 1>), h->nblk_n, MH_BLOCK, S, sel, first);
-      else if constexpr (DIM == 3) LAUNCH(h, KN_SWEEP_KE, (k_sweep_ke<3, 2>), h->nblk_n, MH_BLOCK, S, sel, first);
-    }
+    launch_cell_part<DIM, M>(h, D, sel, first, gen, o3);
+    launch_dual_part<DIM>(h, D, sel, first, last, gen, o3);
   }
   if (S.dec && nd == 0) LAUNCH(h, KN_SWEEP_KE, (k_dual_init<DIM>), h->nblk_n, MH_BLOCK, S);
   if (nd <= 1 && !S.sans_lagrange)

@@ -114,6 +114,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default kernel variants (the autotuner otherwise runs in the warmup)")
     ap.add_argument("--tune-cycles", type=int, default=12, help="timed iterations per autotuner candidate")
+    ap.add_argument("--variant", type=int, default=None,
+                    help="apply this kernel-variant code (as the autotuner reports it) instead of tuning")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (scale reference for HBM counters)")
     ap.add_argument("--mode", choices=["plain", "batched", "serve"], default="plain",
@@ -157,7 +159,9 @@ def main():
         gp = -(-args.graph // period) * period
         warm = max(gp, -(-args.warmup // gp) * gp)
     variant, tuned_ms = None, None
-    if not args.no_autotune:  # part of the warmup: the candidates run the case's own iterations
+    if args.variant is not None:  # a kernel-variant code chosen by an earlier autotuner run
+        g.set_variant(args.variant)
+    elif not args.no_autotune:  # part of the warmup: the candidates run the case's own iterations
         variant, tuned_ms = g.autotune(args.tune_cycles)
         print(f"[bench] autotuned variant {variant} ({tuned_ms:.3f} ms/iteration)", file=sys.stderr, flush=True)
     g.queue(warm, graph=args.graph)
@@ -204,6 +208,7 @@ def main():
         "warmup": args.warmup,
         "warmup_run": warm,
         "autotuned_variant": variant,
+        "variant": g.variant,
         "graph_captures_in_timed_region": captures_timed,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,

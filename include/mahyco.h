@@ -218,6 +218,14 @@ int mahyco_ensemble_autotune(MhEnsemble* e, int cycles, int* blocks_per_cu_out, 
  * bits); tuning stops early at the final time or an error. */
 int mahyco_autotune(MhHandle* h, int cycles, int* variant_out, double* ms_per_cycle_out);
 
+/* The kernel-variant code of a case (the code mahyco_autotune returns: node 0 flat, 1-3 the
+ * z-marching node kernel with 4/8/16 planes; + 4 x cell 0 flat, 1-3 the LDS-staged cell kernel
+ * with 4/8/16 planes; + 16 for one thread per item in every sweep) and its application, so that a
+ * tuned choice can be kept and applied to later cases.  Every variant gives the same bits.  No
+ * counterpart in the reference, which has one kernel per loop. */
+int mahyco_get_variant(const MhHandle* h);
+int mahyco_set_variant(MhHandle* h, int variant);
+
 /* Kernel timing: when enabled, every launch is bracketed by HIP events on the
  * handle's stream.  mahyco_profile_read fills up to `max` entries of kernel
  * names (NUL-separated into `names`, `names_len` bytes), total milliseconds and

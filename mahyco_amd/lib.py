@@ -44,6 +44,10 @@ def load():
     lib.mahyco_ensemble_poll_result.restype = C.c_int
     lib.mahyco_autotune.argtypes = [H, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     lib.mahyco_autotune.restype = C.c_int
+    lib.mahyco_get_variant.argtypes = [H]
+    lib.mahyco_get_variant.restype = C.c_int
+    lib.mahyco_set_variant.argtypes = [H, C.c_int]
+    lib.mahyco_set_variant.restype = C.c_int
     lib.mahyco_ensemble_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     lib.mahyco_ensemble_autotune.restype = C.c_int
     lib.mahyco_ensemble_graph_captures.argtypes = [C.c_void_p]
@@ -211,6 +215,16 @@ class GpuCase:
         if st != T.MH_OK:
             raise MahycoError(st, "mahyco_autotune")
         return v.value, ms.value
+
+    @property
+    def variant(self):
+        """Kernel-variant code of the case (mahyco_get_variant)."""
+        return self.lib.mahyco_get_variant(self.h)
+
+    def set_variant(self, variant):
+        st = self.lib.mahyco_set_variant(self.h, int(variant))
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_set_variant")
 
     def set_scalars(self, time, deltat, old_deltat, iteration, sens_projection):
         st = self.lib.mahyco_set_scalars(self.h, time, deltat, old_deltat, iteration, sens_projection)

@@ -259,3 +259,24 @@ def test_sweep_rows_and_tiles_equal_flat(monkeypatch, make):
     for key, r in res.items():
         for k in ref:
             assert bitwise_equal(ref[k], r[k]), (key, k)
+
+
+def test_set_variant_same_bits_and_reported():
+    """mahyco_set_variant applies a kernel-variant code (as the autotuner reports it) and
+    mahyco_get_variant returns it; every variant gives the same bits."""
+    from mahyco_amd.lib import GpuCase, MahycoError
+    res = {}
+    for v in (0, 2, 15, 16 | 5):
+        g = GpuCase(cases.config2_bisod3d(33, 17, 19))
+        g.set_variant(v)
+        assert g.variant == v
+        g.run(18, graph=6)
+        res[v] = fields(g, 2, 3)
+        g.close()
+    for v in res:
+        for k in res[0]:
+            assert bitwise_equal(res[0][k], res[v][k]), (v, k)
+    g = GpuCase(cases.config2_bisod3d(8, 8, 8))
+    with pytest.raises(MahycoError):
+        g.set_variant(32)
+    g.close()

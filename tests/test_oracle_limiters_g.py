@@ -10,7 +10,11 @@ four ids are pinned here:
 * mirror symmetry: reflecting the stencil (swapping the + and - sides, negating the slopes)
   negates the slope and swaps the two thresholds, which catches a transposed width;
 * simple-pente keeps the slope under both one-sided bounds and its thresholds are the
-  neighbour values."""
+  neighbour values;
+* on unequal widths: arithmeticG is the slope of the quadratic through three cell averages at
+  the middle cell's centre, vanLeerG the harmonic mean with the same weights, and the superbee
+  thresholds the middle values at which the superbee reconstruction reaches a neighbour value.
+The vanLeer and arithmetic thresholds (computeY0) on unequal widths are fixed by none of these."""
 import ctypes as C
 import itertools
 
@@ -129,3 +133,51 @@ def test_simple_pente_bounds(olib):
             assert abs(lim) <= bound * (1 + 1e-15) and abs(lim) <= abs(free) * (1 + 1e-15)
             if gp * gm > 0:
                 assert np.sign(lim) == np.sign(yp - ym) or lim == 0.
+
+
+def test_arithmetic_g_is_the_quadratic_slope_on_unequal_widths(olib):
+    """Cells of widths h-, h0, h+ holding the averages of a quadratic: arithmeticG returns the
+    quadratic's derivative at the middle cell's centre, on any widths (the weights with h+ and h-
+    exchanged do not: 2/3 instead of 0 for x^2 on widths 1, 1, 2)."""
+    rng = np.random.default_rng(6)
+    n = 0
+    for _ in range(400):
+        hm, h0, hp = rng.uniform(0.1, 3.0, 3)
+        c0, c1, c2 = rng.uniform(-2, 2, 3)
+        prim = lambda x: c0 * x + c1 * x * x / 2 + c2 * x ** 3 / 3
+        avg = lambda a, b: (prim(b) - prim(a)) / (b - a)
+        a0, b0 = -h0 / 2, h0 / 2
+        ym, y0, yp = avg(a0 - hm, a0), avg(a0, b0), avg(b0, b0 + hp)
+        gp, gm = (yp - y0) / (0.5 * (h0 + hp)), (y0 - ym) / (0.5 * (h0 + hm))
+        if gp * gm < 0:
+            continue  # opposite slopes give no slope
+        n += 1
+        s = olib.oracle_flux_limiter_g(T.MH_ARITHMETIC_G, gp, gm, y0, yp, ym, h0, hp, hm, 0)
+        np.testing.assert_allclose(s, c1, rtol=1e-9, atol=1e-10)
+    assert n > 100
+
+
+def test_vanleer_g_is_the_harmonic_mean_with_the_same_weights(olib):
+    """vanLeerG is the weighted harmonic mean of the two slopes with arithmeticG's weights:
+    1 / vanLeerG(g+, g-) = arithmeticG(1/g+, 1/g-), on any widths."""
+    rng = np.random.default_rng(7)
+    for _ in range(300):
+        hm, h0, hp = rng.uniform(0.1, 3.0, 3)
+        gp, gm = rng.uniform(0.05, 3.0, 2) * rng.choice([-1., 1.])
+        vl = olib.oracle_flux_limiter_g(T.MH_VANLEER_G, gp, gm, 0., 1., -1., h0, hp, hm, 0)
+        ar = olib.oracle_flux_limiter_g(T.MH_ARITHMETIC_G, 1. / gp, 1. / gm, 0., 1., -1., h0, hp, hm, 0)
+        np.testing.assert_allclose(vl * ar, 1., rtol=1e-13)
+
+
+def test_superbee_thresholds_on_unequal_widths(olib):
+    """superbeeG's thresholds are the middle values at which its reconstruction, with slope
+    max(|g+|, |g-|), reaches the neighbour value at the face: y0+ + h0/2 s = y+ and
+    y0- - h0/2 s = y-, on any widths."""
+    rng = np.random.default_rng(8)
+    for _ in range(300):
+        hm, h0, hp = rng.uniform(0.1, 3.0, 3)
+        ym, yp = sorted(rng.uniform(-2, 2, 2))
+        for t, sign, target in ((0, 1., yp), (1, -1., ym)):
+            y0 = olib.oracle_compute_y0(T.MH_SUPERBEE_G, 0., yp, ym, h0, hp, hm, t, 0)
+            gp, gm = (yp - y0) / (0.5 * (h0 + hp)), (y0 - ym) / (0.5 * (h0 + hm))
+            np.testing.assert_allclose(y0 + sign * 0.5 * h0 * max(abs(gp), abs(gm)), target, rtol=1e-12, atol=1e-12)

@@ -73,7 +73,9 @@ int mahyco_get_field(MhHandle* h, int field, int env, int comp, double* out);
  * pseudo-viscosity (cell values: a mixed cell's environment values are re-formed by the
  * kernels), the environment mask, node velocity and MH_N_COORD_LAGRANGE (MH_N_COORD without
  * remap).  Volume, characteristic length, Materiau and node mass are derived: MH_ERR_INVALID.
- * Between iterations only (not inside an entry-point sequence). */
+ * Between iterations, and for cell fields also between mahyco_update_energy_and_pressure and
+ * mahyco_remap in staged mode (mahyco_set_staged); MH_ERR_INVALID inside an entry-point
+ * sequence otherwise. */
 int mahyco_set_field(MhHandle* h, int field, int env, int comp, const double* in);
 /* The scalars that go with the state: m_global_time, m_global_deltat, m_global_old_deltat,
  * m_global_iteration and m_sens_projection (Mahyco.axl:11, RemapADIService.cc:23-25, 59-60).
@@ -225,6 +227,23 @@ int mahyco_autotune(MhHandle* h, int cycles, int* variant_out, double* ms_per_cy
  * counterpart in the reference, which has one kernel per loop. */
 int mahyco_get_variant(const MhHandle* h);
 int mahyco_set_variant(MhHandle* h, int variant);
+
+/* Staged Lagrange step (on = 1; 0 returns to the fused kernels).  The reference's
+ * UpdateEnergyAndPressure stores the Lagrangian state (MahycoModule.cc:1430-1480, with applyEOS
+ * and computePressionMoyenne after it) and its Remap forms the remap vector from that state
+ * (computeVariablesForRemap, PrepareRemap.cc:100-155), so a module entry point placed between
+ * the two in the time loop can read and change the state.  By default the cell kernel forms
+ * the remap vector itself and stores no Lagrangian state.  In staged mode the cell kernel
+ * stores density, internal energy, pressure, sound speed and pseudo-viscosity (per environment
+ * and per cell) and mahyco_remap forms the remap vector from them: between
+ * mahyco_update_energy_and_pressure and mahyco_remap, mahyco_get_field reads that state and
+ * mahyco_set_field may change cell values (node values only between iterations).  The remap
+ * reads density, internal energy, volume fraction, pseudo-viscosity and the environment mask;
+ * pressure and sound speed are formed again after it.  Same bits as the fused kernels when the
+ * state is not changed.  Between iterations only; MH_ERR_UNSUPPORTED without remap, with
+ * schema-csts, in advection mode or with a Fictif environment, MH_ERR_INVALID for a case of an
+ * ensemble. */
+int mahyco_set_staged(MhHandle* h, int on);
 
 /* Kernel timing: when enabled, every launch is bracketed by HIP events on the
  * handle's stream.  mahyco_profile_read fills up to `max` entries of kernel

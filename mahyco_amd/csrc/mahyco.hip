@@ -25,7 +25,7 @@ namespace {
 const char* kKernelNames[] = {"k_node_lag",     "k_cell_lag",     "k_sweep_cell_x",  "k_sweep_cell_y",
                               "k_sweep_cell_z", "k_sweep_node_x", "k_sweep_node_y",  "k_sweep_node_z",
                               "k_node_velocity", "k_finalize",    "k_dt",            "k_hcell",
-                              "k_node_fwd",     "k_sweep_ke",     "k_cons_ec"};
+                              "k_node_fwd",     "k_sweep_ke",     "k_cons_ec",      "k_prepare_remap"};
 
 }  // namespace
 
@@ -297,7 +297,9 @@ void phase_node(MhHandle* h) {
 }
 void phase_cell(MhHandle* h) { MH_DISPATCH(launch_cell_t, h); }
 void phase_remap(MhHandle* h) {
-  if (h->opt.with_projection) MH_DISPATCH(launch_sweeps_t, h);
+  if (!h->opt.with_projection) return;
+  if (h->staged) MH_DISPATCH(launch_prepare_t, h);
+  MH_DISPATCH(launch_sweeps_t, h);
 }
 template <int DIM, int M>
 void launch_hist_t(MhHandle* h) {
@@ -1271,7 +1273,9 @@ static void drop_graphs(MhHandle* h);
 // the cell value in a cell of k alone (Arcane's single-environment aliasing), nothing in a
 // cell outside k.  Set MH_F_ENV_MASK before the environment values it selects.
 int mahyco_set_field(MhHandle* h, int field, int env, int comp, const double* in) {
-  if (!h || !in || h->phase != 0 || h->ext_sc) return MH_ERR_INVALID;
+  if (!h || !in || h->ext_sc) return MH_ERR_INVALID;
+  // between iterations; in staged mode also cell values between the energy update and the remap
+  if (h->phase != 0 && !(h->staged && h->phase == 9 && field < 100)) return MH_ERR_INVALID;
   HIPCHK(hipStreamSynchronize(h->stream));
   const long nc = h->g.nc, nn = h->g.nn;
   MhDev& S = h->dev;
@@ -1752,6 +1756,25 @@ int mahyco_autotune(MhHandle* h, int cycles, int* variant_out, double* ms_per_cy
 // caller can keep a tuned choice and apply it to later cases of the same shape.  Every variant
 // gives the same bits; setting one drops the captured graphs.
 int mahyco_get_variant(const MhHandle* h) { return h ? get_variant(h) : MH_ERR_INVALID; }
+
+// Staged Lagrange step.  The reference stores the state of UpdateEnergyAndPressure (density,
+// energy, pressure, sound speed, pseudo-viscosity; MahycoModule.cc:1430-1480, applyEOS and
+// computePressionMoyenne after it) and forms the remap vector from that state at the remap
+// (computeVariablesForRemap, PrepareRemap.cc:100-155), so a module can change the state in
+// between.  The fused cell kernel writes the remap vector directly; staged mode writes the
+// state instead and k_prepare_remap forms U at mahyco_remap.  Same bits when nothing changes.
+int mahyco_set_staged(MhHandle* h, int on) {
+  if (!h || h->phase != 0 || h->ext_sc) return MH_ERR_INVALID;
+  if (on) {
+    if (!h->opt.with_projection || h->opt.schema_csts || h->opt.sans_lagrange) return MH_ERR_UNSUPPORTED;
+    for (int k = 0; k < h->m; ++k)
+      if (h->dev.env[k].fictif) return MH_ERR_UNSUPPORTED;  // its EOS follows the remap's time only
+  }
+  h->staged = on != 0;
+  h->dev.staged = h->staged ? 1 : 0;
+  drop_graphs(h);  // they hold the other cell kernel
+  return MH_OK;
+}
 int mahyco_set_variant(MhHandle* h, int variant) {
   if (!h || variant < 0 || variant > 31 || h->phase != 0) return MH_ERR_INVALID;
   set_variant(h, variant);

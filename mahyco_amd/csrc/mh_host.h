@@ -29,6 +29,7 @@ enum {
   KN_NODE_FWD,  // CSTS forward velocity step before the remap
   KN_SWEEP_KE,  // dual kinetic energy of a sweep (conservation-energie-totale)
   KN_CONS_EC,   // the kinetic energy each cell receives back (conservation-energie-totale)
+  KN_PREPARE,   // remap vector from the stored Lagrangian state (staged mode)
   KN_COUNT
 };
 
@@ -69,6 +70,7 @@ struct MhHandle {
   bool owns_stream = true;
   bool node_tiled = true;   // 3D: z-marching node kernel (MAHYCO_NODE_KERNEL=flat selects the flat one)
   bool cell_tiled = false;  // 3D remap loop: LDS-staged cell kernel (MAHYCO_CELL_KERNEL=tiled; an autotuner candidate)
+  bool staged = false;      // Lagrangian state written before the remap, U formed from it (mahyco_set_staged)
   int nt_kc = MH_NT_KC;     // node planes per block of the tiled node kernel
   int ct_k = MH_CT_K;       // cell planes per block of the tiled cell kernel
   int sweep_mode = MH_SWEEP_TILE;  // MAHYCO_SWEEP=flat selects one thread per cell / node
@@ -234,6 +236,7 @@ struct MhEnsembleImpl {
 template <int DIM, int M> void launch_node_t(MhHandle* h);
 template <int DIM, int M> void launch_cell_t(MhHandle* h);
 template <int DIM, int M> void launch_sweeps_t(MhHandle* h);
+template <int DIM, int M> void launch_prepare_t(MhHandle* h);
 template <int DIM, int M> void ens_group_cycle(MhEnsembleImpl* E, const EnsGroup& g);
 
 #define MH_EXTERN_DM(fn, ARG)                                                                 \
@@ -244,6 +247,7 @@ template <int DIM, int M> void ens_group_cycle(MhEnsembleImpl* E, const EnsGroup
 MH_EXTERN_DM(launch_node_t, MhHandle*)
 MH_EXTERN_DM(launch_cell_t, MhHandle*)
 MH_EXTERN_DM(launch_sweeps_t, MhHandle*)
+MH_EXTERN_DM(launch_prepare_t, MhHandle*)
 #define MH_ENS_ARGS MhEnsembleImpl*, const EnsGroup&
 MH_EXTERN_DM(ens_group_cycle, MH_ENS_ARGS)
 

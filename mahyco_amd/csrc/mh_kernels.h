@@ -707,8 +707,9 @@ template <int DIM, int M, int CSTS = 0, int VGIVEN = 0, int REMAP = 0>
 __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, const D3* x, const CellPre& pre,
                                                 double Vgiven = 0.) {
   // REMAP: known at compile time to be the Lagrange step of the remap loop (with_proj set,
-  // sans_lagrange not), so the other branches are not compiled
-  const bool with_proj = REMAP ? true : (S.with_proj != 0);
+  // sans_lagrange not), so the other branches are not compiled.  Staged mode takes the
+  // Lagrange-only branch (state and EOS written back) and k_prepare_remap forms U later.
+  const bool with_proj = REMAP ? true : (S.with_proj != 0 && !S.staged);
   const bool sans_lagrange = REMAP ? false : (S.sans_lagrange != 0);
   double* __restrict__ U = S.Ua;
   constexpr int NV = 3 * M + 1;
@@ -824,7 +825,7 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
         S.p[c] = p;
         S.c[c] = cc;
         S.q[c] = q;
-        dxs = L / (cc + cell_vmax<DIM>(S, b));
+        if (!S.staged) dxs = L / (cc + cell_vmax<DIM>(S, b));
       }
     } else {
       double fk[M], rk[M], qk[M], ek[M], Vk[M];
@@ -901,7 +902,7 @@ __device__ __forceinline__ double cell_lag_core(const MhDev& S, long c, long b, 
         S.p[c] = ps;
         S.c[c] = cmix;
         S.q[c] = qsum;
-        dxs = L / (cmix + cell_vmax<DIM>(S, b));
+        if (!S.staged) dxs = L / (cmix + cell_vmax<DIM>(S, b));
       }
     }
     if (with_proj && !mk && !S.pb && S.nsweep > 0) empty_cell_phi<M>(S, c, u_out);
@@ -937,6 +938,59 @@ __device__ __forceinline__ void k_cell_lag_body(const MhDev& S, int lb) {
   }
   if (!S.with_proj) block_min_store(dxs, active, S.partial, lb);
 }
+// computeVariablesForRemap (PrepareRemap.cc:100-155) from the stored Lagrangian state, for
+// staged mode: the cell kernel wrote the state of UpdateEnergyAndPressure and the caller may
+// have changed it.  U = (V_k, V_k rho_k, V_k rho_k e_k, V q) with V_k = f_k V for a cell of
+// several environments and V for a single-environment cell, the pente-borne phi of the first
+// sweep, and an empty cell's phi as the fused cell kernel forms them (same operations, so an
+// unchanged state gives the fused kernel's bits).
+template <int DIM, int M>
+__device__ __forceinline__ void k_prepare_remap_body(const MhDev& S, int lb) {
+  const long c = lb * (long)blockDim.x + threadIdx.x;
+  if (c >= S.g.nc || mh_halted(S.sc)) return;
+  constexpr int NV = 3 * M + 1;
+  const long nc = S.g.nc;
+  const uint8_t mk = S.mask[c];
+  const double V = S.Vl[c];
+  const double q = S.q[c];
+  double u_out[NV];
+  for (int v = 0; v < NV; ++v) u_out[v] = 0.;
+  if (__popc(mk) == 1) {
+    const int ke = __ffs(mk) - 1;
+    const double rho = S.rho[c], e = S.e[c];
+    for (int kk = 0; kk < M; ++kk)
+      if (kk == ke) {
+        u_out[kk] = V;
+        u_out[M + kk] = V * rho;
+        u_out[2 * M + kk] = V * rho * e;
+      }
+    if (S.pb) {
+      S.phi0[ke * nc + c] = S.fg[c];
+      S.phi0[(M + ke) * nc + c] = rho;
+      S.phi0[(2 * M + ke) * nc + c] = e;
+      S.phi0[3 * M * nc + c] = q;
+    }
+  } else {
+    for (int kk = 0; kk < M; ++kk) {
+      if (!((mk >> kk) & 1)) continue;
+      const long o = kk * nc + c;
+      const double Vk = S.fk[o] * V;
+      u_out[kk] = Vk;
+      u_out[M + kk] = Vk * S.rhok[o];
+      u_out[2 * M + kk] = Vk * S.rhok[o] * S.ek[o];
+      if (S.pb) {
+        S.phi0[o] = S.fk[o];
+        S.phi0[(M + kk) * nc + c] = S.rhok[o];
+        S.phi0[(2 * M + kk) * nc + c] = S.ek[o];
+      }
+    }
+    if (S.pb && mk) S.phi0[3 * M * nc + c] = q;
+  }
+  u_out[3 * M] = V * q;
+  if (!mk && !S.pb && S.nsweep > 0) empty_cell_phi<M>(S, c, u_out);
+  for (int v = 0; v < NV; ++v) S.Ua[v * nc + c] = u_out[v];
+}
+
 // 3D cell kernel of the remap loop with the node neighbourhood staged in LDS: a block owns
 // MH_CT_X x MH_CT_Y cells of MH_CT_K consecutive z planes and marches up in z.  The node
 // coordinates of a tile plane ((MH_CT_X+1) x (MH_CT_Y+1) nodes, every node read from HBM once
@@ -3189,6 +3243,7 @@ MH_WRAP(k_sweep_ke, 1, template <int DIM MH_COMMA int D>, <DIM MH_COMMA D>, MH_C
         MH_COMMA psel MH_COMMA first)
 MH_WRAP(k_dual_init, 1, template <int DIM>, <DIM>, , )
 MH_WRAP(k_cons_ec, 1, template <int DIM>, <DIM>, , )
+MH_WRAP(k_prepare_remap, 1, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, , )
 
 // (the plain kernels are templates with an unused parameter, so that every translation unit
 // that launches one may instantiate it)

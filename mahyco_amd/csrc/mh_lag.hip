@@ -19,7 +19,7 @@ void launch_cell_t(MhHandle* h) {
   if (h->dev.lc != MH_L_RACINE_CUBIQUE_VOLUME) LAUNCH(h, KN_CELL, (k_cell_length<DIM>), h->nblk_c, MH_BLOCK, h->dev);
   if (h->dev.csts) {
     LAUNCH(h, KN_CELL, (k_cell_lag_csts<DIM, M>), h->nblk_c, MH_BLOCK, h->dev);
-  } else if (DIM == 3 && h->dev.with_proj && !h->dev.sans_lagrange && h->cell_tiled) {
+  } else if (DIM == 3 && h->dev.with_proj && !h->dev.sans_lagrange && !h->staged && h->cell_tiled) {
     // the Lagrange-only loop keeps k_cell_lag: its CFL candidates are folded per block in cell order
     const dim3 grid((h->g.nx + MH_CT_X - 1) / MH_CT_X, (h->g.ny + MH_CT_Y - 1) / MH_CT_Y,
                     (h->g.nz + h->dev.ct_k - 1) / h->dev.ct_k);
@@ -27,6 +27,12 @@ void launch_cell_t(MhHandle* h) {
   } else {
     LAUNCH(h, KN_CELL, (k_cell_lag<DIM, M>), h->nblk_c, MH_BLOCK, h->dev);
   }
+}
+
+// staged mode: the remap vector from the state the cell kernel wrote (and the caller may have changed)
+template <int DIM, int M>
+void launch_prepare_t(MhHandle* h) {
+  LAUNCH(h, KN_PREPARE, (k_prepare_remap<DIM, M>), h->nblk_c, MH_BLOCK, h->dev);
 }
 
 template void launch_node_t<2, 1>(MhHandle*);
@@ -45,3 +51,11 @@ template void launch_cell_t<3, 1>(MhHandle*);
 template void launch_cell_t<3, 2>(MhHandle*);
 template void launch_cell_t<3, 3>(MhHandle*);
 template void launch_cell_t<3, 4>(MhHandle*);
+template void launch_prepare_t<2, 1>(MhHandle*);
+template void launch_prepare_t<2, 2>(MhHandle*);
+template void launch_prepare_t<2, 3>(MhHandle*);
+template void launch_prepare_t<2, 4>(MhHandle*);
+template void launch_prepare_t<3, 1>(MhHandle*);
+template void launch_prepare_t<3, 2>(MhHandle*);
+template void launch_prepare_t<3, 3>(MhHandle*);
+template void launch_prepare_t<3, 4>(MhHandle*);

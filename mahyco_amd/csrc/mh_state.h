@@ -128,7 +128,7 @@ struct MhDev {
   int hist_max;        // capacity in rows
   int nt_kc;           // node planes per block of the z-marching node kernel (autotuned)
   int ct_k;            // cell planes per block of the LDS-staged cell kernel (autotuned)
-  int pad3;
+  int staged;          // remap loop with the Lagrangian state written before the remap (mahyco_set_staged)
 };
 
 #endif

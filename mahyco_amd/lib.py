@@ -48,6 +48,8 @@ def load():
     lib.mahyco_get_variant.restype = C.c_int
     lib.mahyco_set_variant.argtypes = [H, C.c_int]
     lib.mahyco_set_variant.restype = C.c_int
+    lib.mahyco_set_staged.argtypes = [H, C.c_int]
+    lib.mahyco_set_staged.restype = C.c_int
     lib.mahyco_ensemble_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     lib.mahyco_ensemble_autotune.restype = C.c_int
     lib.mahyco_ensemble_graph_captures.argtypes = [C.c_void_p]
@@ -225,6 +227,37 @@ class GpuCase:
         st = self.lib.mahyco_set_variant(self.h, int(variant))
         if st != T.MH_OK:
             raise MahycoError(st, "mahyco_set_variant")
+
+    def set_staged(self, on=True):
+        """Staged Lagrange step (mahyco_set_staged): the state of UpdateEnergyAndPressure is
+        stored and the remap vector is formed from it at the remap, so that host code run
+        between the two (step(hook=...)) can read and change it."""
+        st = self.lib.mahyco_set_staged(self.h, 1 if on else 0)
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_set_staged")
+
+    ENTRY_POINTS = ("mahyco_save_values_at_n", "mahyco_update_velocity", "mahyco_apply_boundary_condition",
+                    "mahyco_update_position", "mahyco_compute_geometric_values", "mahyco_update_density",
+                    "mahyco_compute_artificial_viscosity", "mahyco_update_elasticity_and_plasticity",
+                    "mahyco_update_energy_and_pressure")
+
+    def step(self, hook=None):
+        """One iteration through the module's entry points in the time loop's order; hook(self)
+        runs between UpdateEnergyAndPressure and Remap (in staged mode it may read and change
+        the cell state with get / set)."""
+        for fn in self.ENTRY_POINTS:
+            st = getattr(self.lib, fn)(self.h)
+            if st != T.MH_OK:
+                raise MahycoError(st, fn)
+        if hook is not None:
+            hook(self)
+        if self.case.with_projection:
+            st = self.lib.mahyco_remap(self.h)
+            if st != T.MH_OK:
+                raise MahycoError(st, "mahyco_remap")
+        st = self.lib.mahyco_compute_delta_t(self.h)
+        if st != T.MH_OK:
+            raise MahycoError(st, "mahyco_compute_delta_t")
 
     def set_scalars(self, time, deltat, old_deltat, iteration, sens_projection):
         st = self.lib.mahyco_set_scalars(self.h, time, deltat, old_deltat, iteration, sens_projection)

@@ -1968,6 +1968,20 @@ __device__ __forceinline__ void k_sweep_cell_gen_body(const MhDev& S, int lb, in
 // back-face value of cell i+1).  Lanes 2..61 write their cell; the two lanes at each end
 // only feed the stencil.  Same formulas in the same order as k_sweep_cell_body.
 #define MH_ROW_OUT 60
+// Order of the cell rows (j, k) of the x sweep: slabs of MH_ROW_KT cell planes, k fastest
+// inside a slab, so the four cell rows around a node row (j-1..j, k-1..k) run within
+// 2 MH_ROW_KT rows of each other and find that row's coordinates and velocity in L2.
+#ifndef MH_ROW_KT
+#define MH_ROW_KT 4
+#endif
+__device__ __forceinline__ void row_jk(const MhGrid& g, long r, int& j, int& k) {
+  const long slab = (long)g.ny * MH_ROW_KT;
+  const int kt = (int)(r / slab);
+  const int h = min(MH_ROW_KT, g.nz - kt * MH_ROW_KT);  // planes of this slab (1 in 2D)
+  const long rem = r - kt * slab;
+  j = (int)(rem / h);
+  k = kt * MH_ROW_KT + (int)(rem % h);
+}
 template <int DIM, int M, int LIM>
 __device__ __forceinline__ void k_sweep_cell_row_body(const MhDev& S, int lb, int usel, int first) {
   constexpr int NV = 3 * M + 1;
@@ -1981,7 +1995,9 @@ __device__ __forceinline__ void k_sweep_cell_row_body(const MhDev& S, int lb, in
   const double* __restrict__ Uin = usel ? S.Ub : S.Ua;
   double* __restrict__ Uout = usel ? S.Ua : S.Ub;
   double* __restrict__ dflux = S.dflux;
-  const long row = gw / wpr;
+  int j, k;
+  row_jk(S.g, gw / wpr, j, k);
+  const long row = j + (long)S.g.ny * k;
   const int i = (int)(gw % wpr) * MH_ROW_OUT - 2 + lane;
   const bool valid = i >= 0 && i < nx;
   const bool validp = i - 1 >= 0 && i - 1 < nx;
@@ -1989,8 +2005,6 @@ __device__ __forceinline__ void k_sweep_cell_row_body(const MhDev& S, int lb, in
   const long nc = S.g.nc;
   const long c = row * nx + i;
   const bool own = !(first && valid && S.mask[c] == 0);  // an empty cell's U starts from 0
-  const int j = (int)(row % S.g.ny);
-  const int k = (int)(row / S.g.ny);
   const long b = i + (long)S.g.nnx * (j + (long)S.g.nny * k);
   const double dt = S.sc->dt;
   double den = 1., xc = 0.;
@@ -2215,7 +2229,9 @@ __device__ __forceinline__ void k_sweep_node_row_body(const MhDev& S, int lb, in
 // the thread owning that row, and shared through LDS (four barriers per remap slot).
 // Wavefront w holds rows w, w+4, ... of the tile, 64 consecutive cells in x, so every
 // global access is coalesced.  Same formulas in the same order as k_sweep_cell_body.
-#define MH_TILE_L 16
+#ifndef MH_TILE_L
+#define MH_TILE_L 16  // cells along the sweep direction written per tile (a multiple of 4)
+#endif
 #define MH_TILE_R (MH_TILE_L + 4)  // rows held, halos included
 #define MH_TILE_Q (MH_TILE_R / 4)  // rows per thread
 #ifndef MH_TILE_MIN_WAVES

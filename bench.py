@@ -94,6 +94,37 @@ def cpu_baseline(case_fn, cycles, threads, single_cycles=1):
     return out
 
 
+def cpu_baseline_runs(runs, threads, budget_s, sample_iters=10):
+    """The oracle on the host cores solving the runs of an ensemble workload one after the
+    other, on a bounded sample: runs = [(case, iterations to its end)] in the workload's order;
+    for each, `sample_iters` iterations are timed after one untimed and scaled by its iteration
+    count (the oracle and the GPU give the same bits, so the same count), until budget_s seconds
+    have been timed.  Per-run solve times exclude case creation."""
+    from oracle.pyoracle import OracleCase
+    est, spent = [], 0.
+    for case, total in runs:
+        if total < 2:
+            continue
+        o = OracleCase(case, pow_mode=0, threads=threads)
+        o.run(1)
+        n = min(sample_iters, total - 1)
+        t0 = time.perf_counter()
+        o.run(n)
+        dt = time.perf_counter() - t0
+        o.close()
+        est.append(dt / n * total)
+        spent += dt
+        if spent >= budget_s:
+            break
+    if not est:
+        return None
+    return {"value": len(est) / sum(est), "unit": "runs/s", "cores": threads, "kind": "port",
+            "p50_s": _pct(est, 50), "p99_s": _pct(est, 99),
+            "sample": f"the first {len(est)} runs back to back: {sample_iters} iterations of each timed after 1 "
+                      f"untimed, scaled by its iteration count ({spent:.1f} s timed, OMP threads={threads}); "
+                      "iterations only, no case creation"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +142,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cpu-single-cycles", type=int, default=4, help="iterations timed on one host thread")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0,
+                    help="batched / serve: seconds of oracle iterations timed for the CPU baseline")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default kernel variants (the autotuner otherwise runs in the warmup)")
     ap.add_argument("--tune-cycles", type=int, default=12, help="timed iterations per autotuner candidate")
@@ -289,6 +322,8 @@ def bench_batched(args):
            "back_to_back_iterate_s": serve.run_back_to_back.iterate_s,
            "speedup_iterations_only": serve.run_back_to_back.iterate_s / (serve.run_batch.phases["autotune_s"] +
                                                                          serve.run_batch.phases["iterate_s"])}
+    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline_runs(
+        [(c, c.max_cycles) for c in batch], args.cpu_threads, args.cpu_budget)
     print(json.dumps(out))
 
 
@@ -316,6 +351,9 @@ def bench_serve(args):
            "back_to_back_p99_s": b2b["p99_s"], "speedup_vs_back_to_back": stats["runs_per_s"] / b2b["runs_per_s"],
            "failed": [{"run": reqs[i][1].name, "status": T.STATUS_NAMES.get(r.status, r.status)}
                       for i, r in enumerate(recs) if r.iterations < 0]}
+    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline_runs(
+        [(reqs[i][1], r.iterations) for i, r in enumerate(recs) if r.iterations > 0], args.cpu_threads,
+        args.cpu_budget)
     print(json.dumps(out))
 
 

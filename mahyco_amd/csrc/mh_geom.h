@@ -191,6 +191,71 @@ MH_HD void cqs_hex_comp(const D3* x, double* q) {
   q[7] = t * (5. * (n2a08 + n2a12 + n4a12 + n4a11 + n6a11 + n6a08) + (n2a04 + n2a05 + n4a09 + n4a10 + n6a07 + n6a03));
 }
 
+// cqs_hex_comp<C> with the corner sums accumulated face by face.  Every corner sum of cqs_hex
+// adds two terms of each of its three faces, and its faces come in increasing order (corner 0:
+// faces 1, 2, 3; 1: 1, 3, 5; 2: 1, 5, 6; 3: 1, 2, 6; 4: 2, 3, 4; 5: 3, 4, 5; 6: 4, 5, 6; 7: 2,
+// 4, 6), so running the faces 1..6 and adding each face's terms to the sums of its four corners
+// performs every sum's additions in cqs_hex's order: same bits.  Only the 16 running sums and one
+// face's four terms are live, instead of the 24 terms of all faces.
+template <int C>
+MH_HD void cqs_hex_comp_fm(const D3* x, double* q) {
+  constexpr int U = (C + 1) % 3, W = (C + 2) % 3;
+  auto cmp = [](const D3& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); };
+  auto face = [&](int p0, int p1, int p2, int p3, double& e0, double& e1, double& e2, double& e3) {
+    const double cu = 0.25 * (((cmp(x[p0], U) + cmp(x[p1], U)) + cmp(x[p2], U)) + cmp(x[p3], U));
+    const double cw = 0.25 * (((cmp(x[p0], W) + cmp(x[p1], W)) + cmp(x[p2], W)) + cmp(x[p3], W));
+    const double au0 = cmp(x[p0], U) - cu, aw0 = cmp(x[p0], W) - cw;
+    const double au1 = cmp(x[p1], U) - cu, aw1 = cmp(x[p1], W) - cw;
+    const double au2 = cmp(x[p2], U) - cu, aw2 = cmp(x[p2], W) - cw;
+    const double au3 = cmp(x[p3], U) - cu, aw3 = cmp(x[p3], W) - cw;
+    e0 = 0.5 * (au0 * aw1 - aw0 * au1);
+    e1 = 0.5 * (au1 * aw2 - aw1 * au2);
+    e2 = 0.5 * (au2 * aw3 - aw2 * au3);
+    e3 = 0.5 * (au3 * aw0 - aw3 * au0);
+  };
+  const double t = 1. / 12.;
+  double A[8], B[8];  // the 5 x part and the 1 x part of each corner sum
+  double a, b, c, d;
+  face(0, 3, 2, 1, a, b, c, d);  // n1a04, n1a03, n1a02, n1a01
+  A[0] = d + a; B[0] = c + b;
+  A[1] = d + c; B[1] = a + b;
+  A[2] = c + b; B[2] = d + a;
+  A[3] = b + a; B[3] = d + c;
+  face(0, 4, 7, 3, a, b, c, d);  // n2a05, n2a12, n2a08, n2a04
+  A[0] = (A[0] + d) + a; B[0] = (B[0] + c) + b;
+  A[3] = (A[3] + c) + d; B[3] = (B[3] + a) + b;
+  A[4] = a + b; B[4] = c + d;
+  A[7] = c + b; B[7] = d + a;
+  face(0, 1, 5, 4, a, b, c, d);  // n3a01, n3a06, n3a09, n3a05
+  A[0] = (A[0] + d) + a; B[0] = (B[0] + b) + c;
+  A[1] = (A[1] + a) + b; B[1] = (B[1] + c) + d;
+  A[4] = (A[4] + d) + c; B[4] = (B[4] + a) + b;
+  A[5] = b + c; B[5] = a + d;
+  q[0] = t * (5. * A[0] + B[0]);
+  face(4, 5, 6, 7, a, b, c, d);  // n4a09, n4a10, n4a11, n4a12
+  A[4] = (A[4] + a) + d; B[4] = (B[4] + b) + c;
+  A[5] = (A[5] + a) + b; B[5] = (B[5] + d) + c;
+  A[6] = c + b; B[6] = d + a;
+  A[7] = (A[7] + d) + c; B[7] = (B[7] + a) + b;
+  q[4] = t * (5. * A[4] + B[4]);
+  face(1, 2, 6, 5, a, b, c, d);  // n5a02, n5a07, n5a10, n5a06
+  A[1] = (A[1] + d) + a; B[1] = (B[1] + c) + b;
+  A[2] = (A[2] + b) + a; B[2] = (B[2] + d) + c;
+  A[5] = (A[5] + c) + d; B[5] = (B[5] + b) + a;
+  A[6] = (A[6] + c) + b; B[6] = (B[6] + d) + a;
+  q[1] = t * (5. * A[1] + B[1]);
+  q[5] = t * (5. * A[5] + B[5]);
+  face(2, 3, 7, 6, a, b, c, d);  // n6a03, n6a08, n6a11, n6a07
+  A[2] = (A[2] + d) + a; B[2] = (B[2] + c) + b;
+  A[3] = (A[3] + b) + a; B[3] = (B[3] + d) + c;
+  A[6] = (A[6] + d) + c; B[6] = (B[6] + a) + b;
+  A[7] = (A[7] + c) + b; B[7] = (B[7] + d) + a;
+  q[2] = t * (5. * A[2] + B[2]);
+  q[3] = t * (5. * A[3] + B[3]);
+  q[6] = t * (5. * A[6] + B[6]);
+  q[7] = t * (5. * A[7] + B[7]);
+}
+
 // Volume of a hexahedron, sum_l x_l . CQS_l / 3 (MahycoModule.cc:1149-1162), one corner-vector
 // component at a time: dot3 rounds (x.x q.x + x.y q.y) + x.z q.z, so the products of the x and
 // y passes are summed before the z pass, and the corner terms are added in corner order.  Same

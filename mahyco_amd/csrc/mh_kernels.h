@@ -491,24 +491,25 @@ __device__ __forceinline__ void k_node_lag3_tiled_body(const MhDev& S, int bx, i
 
 // Volume of the hexahedron whose node 0 is b (hex_volume_lowreg's three component passes):
 // the node coordinates are read again for each pass, and a compiler fence between the passes
-// keeps their values from being shared, so one component's terms are live at a time (about
-// 100 registers instead of 200 for the whole corner-vector set).  Same bits as cqs_hex +
-// cell_volume<8>.
+// keeps their values from being shared, so one component's terms are live at a time; each pass
+// adds its corner sums face by face (cqs_hex_comp_fm), so only the running sums and one face's
+// terms are live and the cell kernel fits 128 registers (4 waves per SIMD).  Same bits as
+// cqs_hex + cell_volume<8>.
 __device__ __forceinline__ double hex_volume_passes(const MhDev& S, long b) {
   D3 x[8];
   double q[8], d[8];
   cell_nodes<3>(S, b, S.xlc, x);
-  cqs_hex_comp<0>(x, q);
+  cqs_hex_comp_fm<0>(x, q);
 #pragma unroll
   for (int l = 0; l < 8; ++l) d[l] = x[l].x * q[l];
   asm volatile("" ::: "memory");
   cell_nodes<3>(S, b, S.xlc, x);
-  cqs_hex_comp<1>(x, q);
+  cqs_hex_comp_fm<1>(x, q);
 #pragma unroll
   for (int l = 0; l < 8; ++l) d[l] = d[l] + x[l].y * q[l];
   asm volatile("" ::: "memory");
   cell_nodes<3>(S, b, S.xlc, x);
-  cqs_hex_comp<2>(x, q);
+  cqs_hex_comp_fm<2>(x, q);
   double v = 0.;
 #pragma unroll
   for (int l = 0; l < 8; ++l) v += d[l] + x[l].z * q[l];
@@ -3209,7 +3210,7 @@ __global__ __launch_bounds__(MH_BLOCK) void k_hist_final(MhDev S) { k_hist_final
 // Single-mesh grids use the XCD-contiguous block order of xcd_block.
 
 #ifndef MH_CELL_MIN_WAVES
-#define MH_CELL_MIN_WAVES 3  // register budget of k_cell_lag, in waves per SIMD
+#define MH_CELL_MIN_WAVES 4  // register budget of k_cell_lag, in waves per SIMD
 #endif
 #ifndef MH_NODE_MIN_WAVES
 #define MH_NODE_MIN_WAVES 1  // register budget of k_node_lag3_tiled, in waves per SIMD

@@ -146,10 +146,12 @@ inline int node_row_blocks(MhHandle* h) {
 }
 
 // blocks of the LDS-tiled y / z sweep (64 cells along x by MH_TILE_L along D), a multiple of 8
-// k_sweep_node_z: one block per (row segment of an xy node plane, z plane)
-inline int node_z_blocks(MhHandle* h) {
-  const long segs = ((long)h->g.nnx * h->g.nny + MH_BLOCK - 1) / MH_BLOCK;
-  return (int)((segs * h->g.nnz + 7) / 8 * 8);
+// k_sweep_node_march: one thread per (x node, other transverse node, segment of MH_NM_L nodes along D)
+inline int node_march_blocks(MhHandle* h, int D) {
+  const long na = D == 1 ? h->g.ny : h->g.nz;
+  const long nt2 = D == 1 ? h->g.nnz : h->g.nny;
+  const long threads = (long)h->g.nnx * nt2 * ((na + MH_NM_L) / MH_NM_L);
+  return (int)(((threads + MH_BLOCK - 1) / MH_BLOCK + 7) / 8 * 8);
 }
 inline int tile_blocks(MhHandle* h, int D) {
   const long na = D == 1 ? h->g.ny : h->g.nz;

@@ -48,10 +48,13 @@ void launch_dual_part(MhHandle* h, int D, int sel, int first, int last, bool gen
     if (flat) MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node<DIM, 0, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
     else MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE, (k_sweep_node_row<DIM, LIM>), node_row_blocks(h), MH_BLOCK, S, sel, first, last));
   } else if (D == 1) {
-    MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE + 1, (k_sweep_node<DIM, 1, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
+    if (!flat && !o3)  // threads march along y over segments of nodes
+      MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE + 1, (k_sweep_node_march<DIM, 1, LIM>), node_march_blocks(h, 1), MH_BLOCK, S, sel, first, last));
+    else
+      MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE + 1, (k_sweep_node<DIM, 1, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
   } else if constexpr (DIM == 3) {
-    if (!flat && !o3)  // z dual sweep: blocks walk z fastest
-      MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE + 2, (k_sweep_node_z<LIM>), node_z_blocks(h), MH_BLOCK, S, sel, first, last));
+    if (!flat && !o3)  // threads march along z over segments of nodes
+      MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE + 2, (k_sweep_node_march<3, 2, LIM>), node_march_blocks(h, 2), MH_BLOCK, S, sel, first, last));
     else
       MH_LIMSW(h, LAUNCH(h, KN_SWEEP_NODE + 2, (k_sweep_node<3, 2, LIM>), h->nblk_n, MH_BLOCK, S, sel, first, last));
   }

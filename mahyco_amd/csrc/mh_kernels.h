@@ -2234,6 +2234,9 @@ __device__ __forceinline__ void k_sweep_node_row_body(const MhDev& S, int lb, in
 #define MH_TILE_L 16  // cells along the sweep direction written per tile (a multiple of 4)
 #endif
 #define MH_TILE_R (MH_TILE_L + 4)  // rows held, halos included
+#ifndef MH_TILE_3SYNC
+#define MH_TILE_3SYNC 1  // one environment: three barriers per remap slot instead of four
+#endif
 #define MH_TILE_Q (MH_TILE_R / 4)  // rows per thread
 #ifndef MH_TILE_MIN_WAVES
 #define MH_TILE_MIN_WAVES 4  // waves per SIMD the register allocation must allow
@@ -2274,6 +2277,11 @@ __device__ __forceinline__ void k_sweep_cell_tile_body(const MhDev& S, int lb_, 
   // (between rows r-1 and r) and the upwind data of that face
   bool val[MH_TILE_Q], inner[MH_TILE_Q], up[MH_TILE_Q];
   double den[MH_TILE_Q], dx[MH_TILE_Q], w[MH_TILE_Q], hh[MH_TILE_Q], xcq[MH_TILE_Q];
+  // one environment: the face gradients are formed where the slopes are (three barriers per
+  // slot); with more, the registers this takes cost more than the barrier saves
+  constexpr bool THREE = MH_TILE_3SYNC != 0 && M == 1;
+  bool innern[THREE ? MH_TILE_Q : 1];  // face r+1 (rows r and r+1) and its Delta x
+  double dxn[THREE ? MH_TILE_Q : 1];
   unsigned ownm = 0;  // bit q: the cell's own U enters its update
 #pragma unroll
   for (int q = 0; q < MH_TILE_Q; ++q) {
@@ -2305,6 +2313,11 @@ __device__ __forceinline__ void k_sweep_cell_tile_body(const MhDev& S, int lb_, 
     const double xcp = r >= 1 ? ssl[r - 1][lx] : 0.;
     dx[q] = (val[q] && valp) ? xcq[q] - xcp : 0.;
     inner[q] = val[q] && valp;  // face between cells a-1 and a
+    if constexpr (THREE) {  // face r+1 as the thread of row r+1 forms it
+      const bool valn = r + 1 < MH_TILE_R && i < nx && a + 1 >= 0 && a + 1 < na;
+      innern[q] = valn && val[q];
+      dxn[q] = innern[q] ? ssl[r + 1][lx] - xcq[q] : 0.;
+    }
     up[q] = false;
     w[q] = 0.;
     hh[q] = 0.;
@@ -2331,18 +2344,28 @@ __device__ __forceinline__ void k_sweep_cell_tile_body(const MhDev& S, int lb_, 
       sphi[ly + 4 * q][lx] = val[q] ? uq[q] / den[q] : 0.;
     }
     __syncthreads();
+    if constexpr (!THREE) {
 #pragma unroll
-    for (int q = 0; q < MH_TILE_Q; ++q) {
-      const int r = ly + 4 * q;
-      sgf[r][lx] = (r >= 1 && inner[q]) ? (sphi[r][lx] - sphi[r - 1][lx]) / dx[q] : 0.;
+      for (int q = 0; q < MH_TILE_Q; ++q) {
+        const int r = ly + 4 * q;
+        sgf[r][lx] = (r >= 1 && inner[q]) ? (sphi[r][lx] - sphi[r - 1][lx]) / dx[q] : 0.;
+      }
+      __syncthreads();
     }
-    __syncthreads();
 #pragma unroll
     for (int q = 0; q < MH_TILE_Q; ++q) {
       const int r = ly + 4 * q;
       double g = 0.;
       if (S.order2 && val[q] && r >= 1 && r <= MH_TILE_R - 2) {
-        const double gb = sgf[r][lx], gf = sgf[r + 1][lx];
+        double gb, gf;
+        if constexpr (THREE) {  // both face gradients of the row from Phi (the front one is also
+                                // formed by the next row's thread, with the same operands)
+          gb = inner[q] ? (sphi[r][lx] - sphi[r - 1][lx]) / dx[q] : 0.;
+          gf = innern[q] ? (sphi[r + 1][lx] - sphi[r][lx]) / dxn[q] : 0.;
+        } else {
+          gb = sgf[r][lx];
+          gf = sgf[r + 1][lx];
+        }
         if (gb != 0.) g += 0.5 * (mh_limiter<LIM>(S.limiter, gf / gb) * gb);
         if (gf != 0.) g += 0.5 * (mh_limiter<LIM>(S.limiter, gb / gf) * gf);
       }

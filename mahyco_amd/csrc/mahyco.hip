@@ -466,10 +466,10 @@ void tune_store(const std::string& k, int v) {
 }
 
 // Kernel variants of a single case as one number v = node + 4 cell + 16 sweep, every one of
-// which gives the same bits: node 0 the flat node kernel, 1 / 2 / 3 the z-marching one with 4 /
+// which gives the same bits: node 0 the flat node kernel, 1 / 2 / 3 the z-marching one with 12 /
 // 8 / 16 node planes per block; cell 0 the flat cell kernel, 1 / 2 / 3 the LDS-staged one with
-// 4 / 8 / 16 cell planes per block; sweep 0 row segments and tiles, 1 one thread per item.
-const int kTileDepth[4] = {0, 4, 8, 16};
+// 12 / 8 / 16 cell planes per block; sweep 0 row segments and tiles, 1 one thread per item.
+const int kTileDepth[4] = {0, 12, 8, 16};
 void set_variant(MhHandle* h, int v) {
   const int node = v & 3, cell = (v >> 2) & 3;
   h->node_tiled = node != 0;
@@ -480,7 +480,7 @@ void set_variant(MhHandle* h, int v) {
   h->dev.nt_kc = h->nt_kc;
   h->dev.ct_k = h->ct_k;
 }
-int tile_code(int depth) { return depth == 4 ? 1 : depth == 16 ? 3 : 2; }
+int tile_code(int depth) { return depth == 12 ? 1 : depth == 16 ? 3 : 2; }
 int get_variant(const MhHandle* h) {
   return (h->node_tiled ? tile_code(h->nt_kc) : 0) | ((h->cell_tiled ? tile_code(h->ct_k) : 0) << 2) |
          (h->sweep_mode == MH_SWEEP_FLAT ? 16 : 0);

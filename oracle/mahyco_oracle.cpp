@@ -31,11 +31,9 @@
 // elasto-plastic force term of a NoModel run (adds +-0), gravity (0), and
 // Endommagement (inert at its default thresholds -1e12 / 0).
 //
-// Parity unpinned: the vanLeer and arithmetic thresholds of computeY0 on unequal
-// cell widths (UtilesRemap.cc:226-246).  No fixture uses them and no closed form or
-// invariant found fixes them; their equal-width forms, range and mirror symmetry are
-// checked.  Every other function is pinned by the reference's fixtures or by the
-// mathematical checks of tests/test_oracle_*.py and tests/test_conservation_energie.py.
+// Every function is pinned by the reference's fixtures or by the mathematical checks of
+// tests/test_oracle_*.py, tests/test_conservation_energie.py and tests/test_geom_host.py
+// (the thresholds of computeY0 on unequal widths by the face values they are defined by).
 
 #include "mahyco_oracle.h"
 #include "../mahyco_amd/csrc/mh_math.h"
@@ -1429,8 +1427,10 @@ static inline double inty(double X, double x0, double y0, double x1, double y1, 
   return flux;
 }
 
-// computeY0, UtilesRemap.cc:194-258: monotonicity thresholds y0plus (type 0), y0moins (type 1).
-// Parity unpinned for the vanLeer and arithmetic thresholds on unequal widths.
+// computeY0, UtilesRemap.cc:194-258: monotonicity thresholds y0plus (type 0), y0moins (type 1):
+// the middle values at which the reconstruction reaches a neighbour value at the face, with
+// superbee's slope, the centred slope between the neighbour centres (arithmetic) or the
+// harmonic mean 1/s = l+/g+ + l-/g- (vanLeer, clamped to the neighbour values).
 static double computeY0(int id, double y0, double yplus, double ymoins, double h0, double hplus, double hmoins,
                         int type, bool simple_pente) {
   (void)y0;

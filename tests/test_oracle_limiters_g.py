@@ -12,9 +12,10 @@ four ids are pinned here:
 * simple-pente keeps the slope under both one-sided bounds and its thresholds are the
   neighbour values;
 * on unequal widths: arithmeticG is the slope of the quadratic through three cell averages at
-  the middle cell's centre, vanLeerG the harmonic mean with the same weights, and the superbee
-  thresholds the middle values at which the superbee reconstruction reaches a neighbour value.
-The vanLeer and arithmetic thresholds (computeY0) on unequal widths are fixed by none of these."""
+  the middle cell's centre, vanLeerG the harmonic mean with the same weights; the thresholds of
+  computeY0 are the middle values at which a reconstruction reaches a neighbour value at the face:
+  with superbee's slope, with the centred slope between the neighbour centres (arithmetic), and
+  with the harmonic mean weighted the other way round from vanLeerG (vanLeer)."""
 import ctypes as C
 import itertools
 
@@ -181,3 +182,44 @@ def test_superbee_thresholds_on_unequal_widths(olib):
             y0 = olib.oracle_compute_y0(T.MH_SUPERBEE_G, 0., yp, ym, h0, hp, hm, t, 0)
             gp, gm = (yp - y0) / (0.5 * (h0 + hp)), (y0 - ym) / (0.5 * (h0 + hm))
             np.testing.assert_allclose(y0 + sign * 0.5 * h0 * max(abs(gp), abs(gm)), target, rtol=1e-12, atol=1e-12)
+
+
+def test_arithmetic_thresholds_on_unequal_widths(olib):
+    """arithmeticG's thresholds (computeY0, UtilesRemap.cc:194-258) are the middle values at
+    which the reconstruction with the centred slope s = (y+ - y-) / (x+ - x-), the neighbour
+    centres being h0 + (h+ + h-)/2 apart, reaches the neighbour value at the face:
+    y0+ + h0/2 s = y+ and y0- - h0/2 s = y-, on any widths."""
+    rng = np.random.default_rng(10)
+    for _ in range(400):
+        hm, h0, hp = rng.uniform(0.1, 3.0, 3)
+        ym, yp = sorted(rng.uniform(-2, 2, 2))
+        s = (yp - ym) / (h0 + 0.5 * (hp + hm))
+        for t, sign, target in ((0, 1., yp), (1, -1., ym)):
+            y0 = olib.oracle_compute_y0(T.MH_ARITHMETIC_G, 0., yp, ym, h0, hp, hm, t, 0)
+            np.testing.assert_allclose(y0 + sign * 0.5 * h0 * s, target, rtol=1e-12, atol=1e-12)
+
+
+def test_vanleer_thresholds_on_unequal_widths(olib):
+    """vanLeerG's thresholds, where they lie strictly between the neighbour values (outside they
+    are clamped to them), are the middle values at which the reconstruction with the weighted
+    harmonic mean s of the one-sided slopes, 1/s = l+/g+ + l-/g- (l+- = (h0/2 + h+-) / (h0 + h+ +
+    h-)), reaches the neighbour value at the face.  The same mean with the weights the other way
+    round, vanLeerG's own slope, does not (the check below fails it by several percent)."""
+    rng = np.random.default_rng(11)
+    n, off = 0, 0.
+    for _ in range(600):
+        hm, h0, hp = rng.uniform(0.1, 3.0, 3)
+        ym, yp = sorted(rng.uniform(-2, 2, 2))
+        S = h0 + hp + hm
+        lp, lm = (h0 / 2 + hp) / S, (h0 / 2 + hm) / S
+        for t, sign, target in ((0, 1., yp), (1, -1., ym)):
+            y0 = olib.oracle_compute_y0(T.MH_VANLEER_G, 0., yp, ym, h0, hp, hm, t, 0)
+            if not ym < y0 < yp:
+                assert y0 in (ym, yp)
+                continue
+            n += 1
+            gp, gm = (yp - y0) / (0.5 * (h0 + hp)), (y0 - ym) / (0.5 * (h0 + hm))
+            np.testing.assert_allclose(y0 + sign * 0.5 * h0 / (lp / gp + lm / gm), target, rtol=1e-11,
+                                       atol=1e-11 * (yp - ym))
+            off = max(off, abs(y0 + sign * 0.5 * h0 / (lm / gp + lp / gm) - target) / (yp - ym))
+    assert n > 300 and off > 1e-2

@@ -155,6 +155,8 @@ def main():
                     help="plain: single-mesh cycle (default); batched: 256 uneven 2D meshes, 300 "
                          "iterations each; serve: Poisson stream with continuous batching")
     ap.add_argument("--runs", type=int, default=256)
+    ap.add_argument("--batched-only", action="store_true",
+                    help="batched: skip the back-to-back comparison (for profiling the batched kernels alone)")
     ap.add_argument("--rate", type=float, default=16.0, help="serve: arrivals per second")
     ap.add_argument("--seed", type=int, default=20261202)
     args = ap.parse_args()
@@ -304,6 +306,13 @@ def bench_batched(args):
     print(f"[bench] batched: {len(batch)} runs, {cells} cells", file=sys.stderr, flush=True)
     t_batch, iters, stats, done, bpc = serve.run_batch(batch, graph=True)
     assert all(i == batch[0].max_cycles for i in iters), "a run did not reach its horizon"
+    if args.batched_only:
+        print(json.dumps({"metric": "ensemble runs/s (batched)", "value": len(batch) / t_batch, "unit": "runs/s",
+                          "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+                          "config": {"workload": "batched_2d_bimaterial", "runs": len(batch), "cells": cells,
+                                     "iterations": batch[0].max_cycles},
+                          "batched_s": t_batch, "batched_phases_s": serve.run_batch.phases}))
+        return
     print(f"[bench] batched {t_batch:.2f} s; back to back ...", file=sys.stderr, flush=True)
     t_b2b, done_b2b = serve.run_back_to_back(batch)
     out = {"metric": "ensemble runs/s (batched) vs back-to-back single-mesh solves",

@@ -431,7 +431,7 @@ void ens_free_list(EnsList& l) {
 
 void ens_free_groups(MhEnsembleImpl* E) {
   for (auto& g : E->groups) {
-    for (EnsList* l : {&g.cmap, &g.nmap, &g.rcmap, &g.rnmap, &g.tmap[0], &g.tmap[1], &g.ntmap}) ens_free_list(*l);
+    for (EnsList* l : {&g.cmap, &g.nmap, &g.rcmap, &g.rnmap, &g.tmap[0], &g.tmap[1], &g.ntmap, &g.mmap[0], &g.mmap[1]}) ens_free_list(*l);
     (void)hipFree(g.d_list);
     (void)hipFree(g.d_nlist);
   }
@@ -541,13 +541,14 @@ void ens_lists_add(EnsGroup& g, MhHandle* h, int sl) {
     ens_list_add(g.rcmap, sl, row_blocks(h));
     ens_list_add(g.rnmap, sl, node_row_blocks(h));
     for (int D = 1; D < g.dim; ++D) ens_list_add(g.tmap[D - 1], sl, tile_blocks(h, D));
+    for (int D = 1; D < g.dim; ++D) ens_list_add(g.mmap[D - 1], sl, node_march_blocks(h, D));
   }
   if (g.ntile) ens_list_add(g.ntmap, sl, node_tile_blocks(h));
   g.slots.push_back(sl);
   g.list_dirty = true;
 }
 void ens_lists_drop(EnsGroup& g, int sl) {
-  for (EnsList* l : {&g.cmap, &g.nmap, &g.rcmap, &g.rnmap, &g.tmap[0], &g.tmap[1], &g.ntmap}) ens_list_drop(*l, sl);
+  for (EnsList* l : {&g.cmap, &g.nmap, &g.rcmap, &g.rnmap, &g.tmap[0], &g.tmap[1], &g.ntmap, &g.mmap[0], &g.mmap[1]}) ens_list_drop(*l, sl);
   g.slots.erase(std::remove(g.slots.begin(), g.slots.end(), sl), g.slots.end());
   g.list_dirty = true;
 }
@@ -561,7 +562,7 @@ int ens_flush(MhEnsembleImpl* E) {
   for (int sl = 0; sl < E->max_slots; ++sl)
     if (E->tab_dirty[sl]) need += 2 * sizeof(MhDev);
   for (auto& g : E->groups) {
-    for (EnsList* l : {&g.cmap, &g.nmap, &g.rcmap, &g.rnmap, &g.tmap[0], &g.tmap[1], &g.ntmap})
+    for (EnsList* l : {&g.cmap, &g.nmap, &g.rcmap, &g.rnmap, &g.tmap[0], &g.tmap[1], &g.ntmap, &g.mmap[0], &g.mmap[1]})
       if (l->dirty) need += l->h.size() * sizeof(int2) + 64;
     if (g.list_dirty) need += g.slots.size() * sizeof(int) + 64;
   }
@@ -602,7 +603,7 @@ int ens_flush(MhEnsembleImpl* E) {
     E->tab_dirty[sl] = 0;
   }
   for (auto& g : E->groups) {
-    for (EnsList* l : {&g.cmap, &g.nmap, &g.rcmap, &g.rnmap, &g.tmap[0], &g.tmap[1], &g.ntmap}) {
+    for (EnsList* l : {&g.cmap, &g.nmap, &g.rcmap, &g.rnmap, &g.tmap[0], &g.tmap[1], &g.ntmap, &g.mmap[0], &g.mmap[1]}) {
       if (!l->dirty) continue;
       const int n = (int)l->h.size();
       if (n > l->cap || !l->d_n) {

@@ -190,7 +190,8 @@ struct EnsGroup {
   int lim = -1;
   int bpc = 4;         // persistent blocks per CU of the group's launches (autotuned)
   std::vector<int> slots;
-  EnsList cmap, nmap, rcmap, rnmap, tmap[2], ntmap;  // cells, nodes, x rows, y / z tiles, node tiles
+  // cells, nodes, x rows, y / z tiles, node tiles, y / z marching node segments
+  EnsList cmap, nmap, rcmap, rnmap, tmap[2], ntmap, mmap[2];
   int* d_list = nullptr;   // slots of the group, for the per-case time step
   int* d_nlist = nullptr;
   int list_cap = 0;

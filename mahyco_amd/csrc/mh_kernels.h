@@ -2717,10 +2717,6 @@ __device__ __forceinline__ void k_sweep_node_march_body(const MhDev& S, int lb, 
   }
 }
 
-template <int DIM, int D, int LIM>
-__global__ __launch_bounds__(MH_BLOCK) void k_sweep_node_march(MhDev S, int psel, int first, int last) {
-  k_sweep_node_march_body<DIM, D, LIM>(S, xcd_block((int)blockIdx.x, (int)gridDim.x), psel, first, last);
-}
 
 // u = mu / m for every node with a non-zero projected mass (RemapADIFinal.cc:286-297), for
 // cycles whose remap has a single sweep (the sweep kernel still reads u) or none.
@@ -3420,6 +3416,8 @@ MH_WRAP(k_sweep_ke, 1, template <int DIM MH_COMMA int D>, <DIM MH_COMMA D>, MH_C
 MH_WRAP(k_dual_init, 1, template <int DIM>, <DIM>, , )
 MH_WRAP(k_cons_ec, 1, template <int DIM>, <DIM>, , )
 MH_WRAP(k_prepare_remap, 1, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, , )
+MH_WRAP(k_sweep_node_march, 1, template <int DIM MH_COMMA int D MH_COMMA int LIM>, <DIM MH_COMMA D MH_COMMA LIM>,
+        MH_COMMA int psel MH_COMMA int first MH_COMMA int last, MH_COMMA psel MH_COMMA first MH_COMMA last)
 
 // (the plain kernels are templates with an unused parameter, so that every translation unit
 // that launches one may instantiate it)

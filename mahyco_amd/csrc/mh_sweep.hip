@@ -121,7 +121,10 @@ void ens_sweep(const EnsGroup& g, const MhDev* tab, hipStream_t st, dim3 G, int 
   } else if (g.rows) {
     hipLaunchKernelGGL((k_sweep_cell_tile_batch<DIM, D, M, LIM>), G, B, 0, st, tab, MH_LIST(g.tmap[D == 0 ? 0 : D - 1]),
                        sel, first);
-    if (!g.adv)
+    if (!g.adv && !g.o3)  // threads march along D over segments of nodes
+      hipLaunchKernelGGL((k_sweep_node_march_batch<DIM, (D == 0 ? 1 : D), LIM>), G, B, 0, st, tab,
+                         MH_LIST(g.mmap[D == 0 ? 0 : D - 1]), sel, first, last);
+    else if (!g.adv)
       hipLaunchKernelGGL((k_sweep_node_batch<DIM, D, LIM>), G, B, 0, st, tab, MH_LIST(g.nmap), sel, first, last);
   } else {
     hipLaunchKernelGGL((k_sweep_cell_batch<DIM, D, M, LIM>), G, B, 0, st, tab, MH_LIST(g.cmap), sel, first);

@@ -2578,6 +2578,9 @@ __device__ __forceinline__ void k_sweep_node_body(const MhDev& S, int lb, int ps
 #ifndef MH_NM_L
 #define MH_NM_L 4
 #endif
+#ifndef MH_NM_MIN_WAVES
+#define MH_NM_MIN_WAVES 1
+#endif
 template <int DIM, int D, int LIM>
 __device__ __forceinline__ void k_sweep_node_march_body(const MhDev& S, int lb, int psel, int first, int last) {
   constexpr int NC = DIM;
@@ -3416,7 +3419,7 @@ MH_WRAP(k_sweep_ke, 1, template <int DIM MH_COMMA int D>, <DIM MH_COMMA D>, MH_C
 MH_WRAP(k_dual_init, 1, template <int DIM>, <DIM>, , )
 MH_WRAP(k_cons_ec, 1, template <int DIM>, <DIM>, , )
 MH_WRAP(k_prepare_remap, 1, template <int DIM MH_COMMA int M>, <DIM MH_COMMA M>, , )
-MH_WRAP(k_sweep_node_march, 1, template <int DIM MH_COMMA int D MH_COMMA int LIM>, <DIM MH_COMMA D MH_COMMA LIM>,
+MH_WRAP(k_sweep_node_march, MH_NM_MIN_WAVES, template <int DIM MH_COMMA int D MH_COMMA int LIM>, <DIM MH_COMMA D MH_COMMA LIM>,
         MH_COMMA int psel MH_COMMA int first MH_COMMA int last, MH_COMMA psel MH_COMMA first MH_COMMA last)
 
 // (the plain kernels are templates with an unused parameter, so that every translation unit

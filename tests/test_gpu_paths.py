@@ -240,11 +240,13 @@ def test_cell_kernel_tiled_equals_flat(monkeypatch, shape):
                                   lambda: cases.config2_bisod3d(7, 17, 3), lambda: cases.config2_bisod3d(130, 5, 4),
                                   lambda: cases.config2_bisod3d(70, 35, 18), lambda: cases.bitriplepoint(),
                                   lambda: cases.sedov_2d(),
-                                  lambda: cases.ordre3(lambda: cases.config2_bisod3d(70, 35, 18), "b3d_o3")])
+                                  lambda: cases.ordre3(lambda: cases.config2_bisod3d(70, 35, 18), "b3d_o3"),
+                                  lambda: cases.config2_bisod3d(256, 4, 128), lambda: cases.config2_bisod3d(256, 128, 3)])
 def test_sweep_rows_and_tiles_equal_flat(monkeypatch, make):
-    """The row-segment x sweeps (cell and dual parts) and the LDS-tiled y/z cell sweeps
-    give the same bits as the one-thread-per-item sweeps (sizes cover partial tiles and row
-    segments, several segments per row and tiles per line, and sweeps 2 or 3 cells long)."""
+    """The row-segment x sweeps (cell and dual parts), the LDS-tiled y/z cell sweeps and the
+    marching y/z dual sweeps give the same bits as the one-thread-per-item sweeps (sizes cover
+    partial tiles and row segments, several segments per row and tiles per line, sweeps 2 or 3
+    cells long, and marching rows of 257 x 129 nodes, more than 128 blocks each)."""
     from mahyco_amd.lib import GpuCase
     res = {}
     for mode in ("flat", "tile"):

@@ -161,8 +161,8 @@ def main():
     ap.add_argument("--seed", type=int, default=20261202)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              "(the path shards by slabs in design only)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU slab decomposition is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     from mahyco_amd import cases

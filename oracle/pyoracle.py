@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (test infrastructure only).
 
-Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker
-and the CPU timing; never by the product path.
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, for parity
+checks and the CPU timing; never by the product path.
 """
 import ctypes as C
 import os

@@ -214,15 +214,16 @@ int mahyco_ensemble_autotune(MhEnsemble* e, int cycles, int* blocks_per_cu_out, 
  * times `cycles` iterations of the case itself per candidate, one coordinate at a time: the
  * sweeps (row segments and LDS tiles, or one thread per item), then in 3D the node kernel
  * (flat, or z-marching with 8, 12 or 16 node planes per block) and the cell kernel (flat, or
- * LDS-staged with 8, 12 or 16 cell planes per block), and keeps the fastest, also for later
- * cases of the same dimensions and options in this process.  variant = node + 4 cell + 16
- * sweep with those codes.  The iterations count as the case's own (every variant gives the same
+ * LDS-staged with 8, 12 or 16 cell planes per block), then the segment length of the marching
+ * y / z dual sweeps, and keeps the fastest, also for later cases of the same dimensions and
+ * options in this process.  variant = node + 4 cell + 16 sweep + 32 march with those codes.  The iterations count as the case's own (every variant gives the same
  * bits); tuning stops early at the final time or an error. */
 int mahyco_autotune(MhHandle* h, int cycles, int* variant_out, double* ms_per_cycle_out);
 
 /* The kernel-variant code of a case (the code mahyco_autotune returns: node 0 flat, 1-3 the
  * z-marching node kernel with 12/8/16 planes; + 4 x cell 0 flat, 1-3 the LDS-staged cell kernel
- * with 12/8/16 planes; + 16 for one thread per item in every sweep) and its application, so that a
+ * with 12/8/16 planes; + 16 for one thread per item in every sweep; + 32 x march 0-4 for
+ * segments of 10/6/8/4/12 nodes in the marching y / z dual sweeps) and its application, so that a
  * tuned choice can be kept and applied to later cases.  Every variant gives the same bits.  No
  * counterpart in the reference, which has one kernel per loop. */
 int mahyco_get_variant(const MhHandle* h);

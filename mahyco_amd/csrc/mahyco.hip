@@ -470,8 +470,12 @@ void tune_store(const std::string& k, int v) {
 // 8 / 16 node planes per block; cell 0 the flat cell kernel, 1 / 2 / 3 the LDS-staged one with
 // 12 / 8 / 16 cell planes per block; sweep 0 row segments and tiles, 1 one thread per item.
 const int kTileDepth[4] = {0, 12, 8, 16};
+// target segment lengths of the marching y / z dual sweeps, by variant code bits 5-7
+constexpr int kMarchLen[5] = {10, 6, 8, 4, 12};
 void set_variant(MhHandle* h, int v) {
-  const int node = v & 3, cell = (v >> 2) & 3;
+  const int node = v & 3, cell = (v >> 2) & 3, mc = (v >> 5) & 7;
+  h->nm_len = kMarchLen[mc < 5 ? mc : 0];
+  h->dev.nm_len = h->nm_len;
   h->node_tiled = node != 0;
   if (node) h->nt_kc = kTileDepth[node];
   h->cell_tiled = cell != 0;
@@ -481,9 +485,14 @@ void set_variant(MhHandle* h, int v) {
   h->dev.ct_k = h->ct_k;
 }
 int tile_code(int depth) { return depth == 12 ? 1 : depth == 16 ? 3 : 2; }
+int march_code(int len) {
+  for (int c = 0; c < 5; ++c)
+    if (kMarchLen[c] == len) return c;
+  return 0;
+}
 int get_variant(const MhHandle* h) {
   return (h->node_tiled ? tile_code(h->nt_kc) : 0) | ((h->cell_tiled ? tile_code(h->ct_k) : 0) << 2) |
-         (h->sweep_mode == MH_SWEEP_FLAT ? 16 : 0);
+         (h->sweep_mode == MH_SWEEP_FLAT ? 16 : 0) | (march_code(h->nm_len) << 5);
 }
 // the key of a case before its device state exists (dimensions and the options that select kernels)
 std::string case_key_dims(const MhHandle* h) {
@@ -783,6 +792,8 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
     h->cell_tiled = ck && strcmp(ck, "tiled") == 0;
     const char* sw = getenv("MAHYCO_SWEEP");
     h->sweep_mode = (sw && strcmp(sw, "flat") == 0) ? MH_SWEEP_FLAT : MH_SWEEP_TILE;
+    const char* ml = getenv("MAHYCO_MARCH_LEN");  // segment length of the marching dual sweeps
+    h->nm_len = ml && atoi(ml) >= 1 ? atoi(ml) : MH_NM_L;
   }
   {
     const int tuned = tune_lookup(case_key_dims(h));
@@ -935,6 +946,7 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
   S.nblk_c = h->nblk_c;
   S.nt_kc = h->nt_kc;
   S.ct_k = h->ct_k;
+  S.nm_len = h->nm_len;
   S.nblk_n = h->nblk_n;
   S.max_iter = 0;
   MhScalars sc;
@@ -1738,6 +1750,14 @@ int mahyco_autotune(MhHandle* h, int cycles, int* variant_out, double* ms_per_cy
       if (v != best) st = timed(v);
     }
   }
+  // the segment length of the marching y / z dual sweeps, where they run (tiled sweeps, dual
+  // phase, order <= 2)
+  if (h->opt.with_projection && !h->opt.sans_lagrange && h->dev.order2 != 2 && !(best & 16)) {
+    for (int mc = 0; mc < 5 && st == MH_OK && !stopped; ++mc) {
+      const int v = (best & ~(7 << 5)) | (mc << 5);
+      if (v != best) st = timed(v);
+    }
+  }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   if (st != MH_OK || best < 0) {
@@ -1777,7 +1797,7 @@ int mahyco_set_staged(MhHandle* h, int on) {
   return MH_OK;
 }
 int mahyco_set_variant(MhHandle* h, int variant) {
-  if (!h || variant < 0 || variant > 31 || h->phase != 0) return MH_ERR_INVALID;
+  if (!h || variant < 0 || (variant >> 5) > 4 || h->phase != 0) return MH_ERR_INVALID;
   set_variant(h, variant);
   drop_graphs(h);
   return MH_OK;

@@ -71,6 +71,7 @@ struct MhHandle {
   bool node_tiled = true;   // 3D: z-marching node kernel (MAHYCO_NODE_KERNEL=flat selects the flat one)
   bool cell_tiled = false;  // 3D remap loop: LDS-staged cell kernel (MAHYCO_CELL_KERNEL=tiled; an autotuner candidate)
   bool staged = false;      // Lagrangian state written before the remap, U formed from it (mahyco_set_staged)
+  int nm_len = MH_NM_L;     // target segment length of the marching dual sweeps (MAHYCO_MARCH_LEN)
   int nt_kc = MH_NT_KC;     // node planes per block of the tiled node kernel
   int ct_k = MH_CT_K;       // cell planes per block of the tiled cell kernel
   int sweep_mode = MH_SWEEP_TILE;  // MAHYCO_SWEEP=flat selects one thread per cell / node
@@ -146,11 +147,11 @@ inline int node_row_blocks(MhHandle* h) {
 }
 
 // blocks of the LDS-tiled y / z sweep (64 cells along x by MH_TILE_L along D), a multiple of 8
-// k_sweep_node_march: one thread per (x node, other transverse node, segment of MH_NM_L nodes along D)
+// k_sweep_node_march: one thread per (x node, other transverse node, segment of about nm_len nodes along D)
 inline int node_march_blocks(MhHandle* h, int D) {
   const long na = D == 1 ? h->g.ny : h->g.nz;
   const long nt2 = D == 1 ? h->g.nnz : h->g.nny;
-  const long threads = (long)h->g.nnx * nt2 * ((na + MH_NM_L) / MH_NM_L);
+  const long threads = (long)h->g.nnx * nt2 * march_nseg((int)na + 1, h->nm_len);
   return (int)(((threads + MH_BLOCK - 1) / MH_BLOCK + 7) / 8 * 8);
 }
 inline int tile_blocks(MhHandle* h, int D) {

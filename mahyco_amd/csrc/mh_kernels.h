@@ -2573,11 +2573,15 @@ __device__ __forceinline__ void k_sweep_node_body(const MhDev& S, int lb, int ps
 // and the mass flux through the node's front dual face are carried to the next node (they are
 // its back-side values), so each is formed once per node instead of by each of the three nodes
 // whose update reads it.  Threads of a wave are consecutive along x.  Same formulas and orders
-// as k_sweep_node_at (classical limiters; the generalised ones keep k_sweep_node).  Segments of
-// 4 nodes measured fastest on the 256x128x128 tube (8 and 16: more serial steps per thread).
+// as k_sweep_node_at (classical limiters; the generalised ones keep k_sweep_node).  The segment
+// length is per case (the autotuner tries 4-12 nodes); 10 is fastest on the 256x128x128 tube and
+// the 192^3 Sedov blast.
 #ifndef MH_NM_L
-#define MH_NM_L 4
+#define MH_NM_L 10  // default target segment length (MhDev::nm_len; variant march code 0, kMarchLen[0])
 #endif
+// Segments of one node line: the n nodes split into ceil(n / len) segments whose lengths differ
+// by at most one, so that no line ends in a short segment that costs a whole pass of threads.
+MH_HD int march_nseg(int n, int len) { return (n + len - 1) / len; }
 #ifndef MH_NM_MIN_WAVES
 #define MH_NM_MIN_WAVES 1
 #endif
@@ -2589,13 +2593,13 @@ __device__ __forceinline__ void k_sweep_node_march_body(const MhDev& S, int lb, 
   const int nt2 = D == 1 ? S.g.nnz : nny;  // the other transverse node count
   const long cols = (long)nnx * nt2;
   const long g = lb * (long)blockDim.x + threadIdx.x;
-  const int nseg = (na + MH_NM_L) / MH_NM_L;  // na + 1 nodes along D
+  const int nseg = march_nseg(na + 1, S.nm_len);  // na + 1 nodes along D
   if (g >= cols * nseg) return;
   if (mh_halted(S.sc)) return;
   const int seg = (int)(g / cols);
   const long rem = g - seg * cols;
   const int i = (int)(rem % nnx), t2 = (int)(rem / nnx);
-  const int a0 = seg * MH_NM_L, a1 = min(a0 + MH_NM_L, na + 1);  // nodes a0 .. a1-1
+  const int a0 = (int)((long)seg * (na + 1) / nseg), a1 = (int)((long)(seg + 1) * (na + 1) / nseg);  // nodes a0 .. a1-1
   const long nn = S.g.nn;
   const long nD = D == 1 ? (long)nnx : (long)nnx * nny;
   const long base = D == 1 ? i + (long)nnx * nny * t2 : i + (long)nnx * t2;  // node at a = 0

@@ -129,6 +129,7 @@ struct MhDev {
   int nt_kc;           // node planes per block of the z-marching node kernel (autotuned)
   int ct_k;            // cell planes per block of the LDS-staged cell kernel (autotuned)
   int staged;          // remap loop with the Lagrangian state written before the remap (mahyco_set_staged)
+  int nm_len;          // target nodes per segment of the marching y / z dual sweeps
 };
 
 #endif

@@ -263,12 +263,33 @@ def test_sweep_rows_and_tiles_equal_flat(monkeypatch, make):
             assert bitwise_equal(ref[k], r[k]), (key, k)
 
 
+@pytest.mark.parametrize("length", [1, 3, 7, 10, 200])
+def test_march_segment_lengths_equal_flat(monkeypatch, length):
+    """The marching y/z dual sweeps split each node line into segments of at most the target
+    length, balanced to within one node; every target (one node, lengths that leave uneven
+    remainders, and one longer than the line) gives the bits of the one-thread-per-node sweep."""
+    from mahyco_amd.lib import GpuCase
+    res = {}
+    for mode, ml in (("flat", None), ("tile", str(length))):
+        monkeypatch.setenv("MAHYCO_SWEEP", mode)
+        if ml:
+            monkeypatch.setenv("MAHYCO_MARCH_LEN", ml)
+        c = cases.config2_bisod3d(33, 29, 22)
+        print(f"[paths] march length {length} {mode}", flush=True)
+        g = GpuCase(c)
+        g.run(18)
+        res[mode] = fields(g, len(c.envs), c.dim)
+        g.close()
+    for k in res["flat"]:
+        assert bitwise_equal(res["flat"][k], res["tile"][k]), k
+
+
 def test_set_variant_same_bits_and_reported():
     """mahyco_set_variant applies a kernel-variant code (as the autotuner reports it) and
     mahyco_get_variant returns it; every variant gives the same bits."""
     from mahyco_amd.lib import GpuCase, MahycoError
     res = {}
-    for v in (0, 2, 15, 16 | 5):
+    for v in (0, 2, 15, 16 | 5, 2 | 32 * 3, 1 | 32 * 4):
         g = GpuCase(cases.config2_bisod3d(33, 17, 19))
         g.set_variant(v)
         assert g.variant == v
@@ -280,5 +301,5 @@ def test_set_variant_same_bits_and_reported():
             assert bitwise_equal(res[0][k], res[v][k]), (v, k)
     g = GpuCase(cases.config2_bisod3d(8, 8, 8))
     with pytest.raises(MahycoError):
-        g.set_variant(32)
+        g.set_variant(32 * 5)
     g.close()

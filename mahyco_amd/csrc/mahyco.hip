@@ -793,7 +793,10 @@ static int create_case(const MhMesh* mesh, const MhEnvironment* envs, int nb_env
     const char* sw = getenv("MAHYCO_SWEEP");
     h->sweep_mode = (sw && strcmp(sw, "flat") == 0) ? MH_SWEEP_FLAT : MH_SWEEP_TILE;
     const char* ml = getenv("MAHYCO_MARCH_LEN");  // segment length of the marching dual sweeps
-    h->nm_len = ml && atoi(ml) >= 1 ? atoi(ml) : MH_NM_L;
+    // 10-node segments where the mesh still fills the GPU with them (about 2.6 M nodes, one round of
+    // waves), 4 on smaller meshes, whose sweeps need the threads
+    const long nn = (long)h->g.nnx * h->g.nny * (h->g.dim == 3 ? h->g.nnz : 1);
+    h->nm_len = ml && atoi(ml) >= 1 ? atoi(ml) : nn >= 10L * 4096 * 64 ? MH_NM_L : 4;
   }
   {
     const int tuned = tune_lookup(case_key_dims(h));

@@ -2574,8 +2574,8 @@ __device__ __forceinline__ void k_sweep_node_body(const MhDev& S, int lb, int ps
 // its back-side values), so each is formed once per node instead of by each of the three nodes
 // whose update reads it.  Threads of a wave are consecutive along x.  Same formulas and orders
 // as k_sweep_node_at (classical limiters; the generalised ones keep k_sweep_node).  The segment
-// length is per case (the autotuner tries 4-12 nodes); 10 is fastest on the 256x128x128 tube and
-// the 192^3 Sedov blast.
+// length is per case (the autotuner tries 4-12 nodes): 10 by default from 2.6 M nodes on (fastest
+// on the 256x128x128 tube and the 192^3 Sedov blast), 4 on smaller meshes, which need the threads.
 #ifndef MH_NM_L
 #define MH_NM_L 10  // default target segment length (MhDev::nm_len; variant march code 0, kMarchLen[0])
 #endif
